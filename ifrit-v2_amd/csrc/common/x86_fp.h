@@ -1,0 +1,108 @@
+// x86 SSE/AVX floating-point semantics needed to reproduce the tile
+// rasterizer's CPU arithmetic bit for bit on gfx950.
+//
+// The CPU path (projects/softgraphics/src/engine/tileraster/TileRasterWorker.cpp)
+// is written with SSE/AVX2 intrinsics whose results differ from plain IEEE
+// C math in four places, each emulated here:
+//   * RCPPS / RCPSS, the 12-bit reciprocal approximation
+//     (include/ifrit/core/math/fastutil/FastUtilSimd.h:27-33, 87-97);
+//   * MINPS / MAXPS, which return the second operand when the comparison is
+//     false (NaN or equal operands), unlike fminf/fmaxf;
+//   * CVTTSS2SI, which returns INT_MIN for NaN and out-of-range inputs
+//     where the GPU conversion saturates (TileRasterWorker.cpp:319-322);
+//   * DPPS, which sums ((p0 + p1) + (p2 + p3)) with masked products as +0
+//     (include/ifrit/core/math/simd/SimdVectors.h:497-510).
+// Everything else is IEEE single precision with round-to-nearest-even; the
+// kernels are compiled with -ffp-contract=off and call fmaf where the CPU
+// code calls an FMA intrinsic.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define IFX_HD __host__ __device__ __forceinline__
+#else
+#include <math.h>
+#include <string.h>
+#define IFX_HD static inline
+#endif
+
+namespace ifx {
+
+#if defined(__HIPCC__)
+__device__ __constant__ static const uint32_t kRcpTable[2048] = {
+#include "x86_rcp_table.inc"
+};
+#endif
+// Host copy of the same data for host-side setup code.
+static const uint32_t kRcpTableHost[2048] = {
+#include "x86_rcp_table.inc"
+};
+
+IFX_HD uint32_t f2u(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __float_as_uint(f);
+#else
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return u;
+#endif
+}
+IFX_HD float u2f(uint32_t u) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __uint_as_float(u);
+#else
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+#endif
+}
+
+// RCPSS/RCPPS: table lookup on the top 11 mantissa bits, exact exponent
+// scaling, sub-normal results flushed to signed zero, zero and sub-normal
+// inputs give signed infinity, infinities give signed zero.
+IFX_HD float rcp_ps(float x) {
+    uint32_t b = f2u(x);
+    uint32_t sign = b & 0x80000000u;
+    uint32_t e = (b >> 23) & 0xffu;
+    if (e == 0u) return u2f(sign | 0x7f800000u);
+    if (e == 255u) return (b & 0x7fffffu) ? u2f(b | 0x400000u) : u2f(sign);
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t t = kRcpTable[(b >> 12) & 0x7ffu];
+#else
+    uint32_t t = kRcpTableHost[(b >> 12) & 0x7ffu];
+#endif
+    int re = (int)((t >> 23) & 0xffu) + 127 - (int)e;
+    if (re <= 0) return u2f(sign);
+    return u2f(sign | (t & 0x7fffffu) | ((uint32_t)re << 23));
+}
+
+// ApproxReciprocalNR256: r = rcp(x); (r + r) - x * (r * r).
+IFX_HD float rcp_nr_ps(float x) {
+    float r = rcp_ps(x);
+    float rr = r * r;
+    float muls = x * rr;
+    float twice = r + r;
+    return twice - muls;
+}
+
+// ApproxReciprocal (scalar): r = rcp(x); r * (2 - r * x).
+IFX_HD float rcp_nr_ss(float x) {
+    float r = rcp_ps(x);
+    float rx = r * x;
+    float t = 2.0f - rx;
+    return r * t;
+}
+
+IFX_HD float min_ps(float a, float b) { return a < b ? a : b; }
+IFX_HD float max_ps(float a, float b) { return a > b ? a : b; }
+
+// CVTTSS2SI: truncation toward zero; 0x80000000 when NaN or out of range.
+IFX_HD int cvtt_i32(float x) {
+    if (!(x > -2147483904.0f && x < 2147483648.0f)) return (int)0x80000000u;
+    return (int)x;
+}
+
+IFX_HD float neg(float x) { return u2f(f2u(x) ^ 0x80000000u); }
+
+}  // namespace ifx

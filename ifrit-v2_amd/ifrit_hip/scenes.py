@@ -1,0 +1,347 @@
+"""Seeded synthetic scenes for the tile rasterizer workloads.
+
+Every scene is a plain set of arrays -- an interleaved FP32 vertex buffer,
+an int32 index buffer, a 4x4 view-projection matrix and the shading
+parameters -- so the same bytes feed the MI355X pipeline and the CPU
+checkers.  Camera matrices follow the reference's left-handed conventions
+(include/ifrit/core/math/LinalgOps.h:83-149: LookAt rows s, u, f and a
+perspective with z in [0, 1] and w = view-space z); they are computed in
+float64 and rounded once to float32.
+
+Vertex layout: position xyz, normal xyz, texture coordinate uv (32 bytes),
+or position only (12 bytes) for the depth/flat-colour stress scene.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+STRIDE_FULL = 8
+STRIDE_POS = 3
+
+PROGRAM_FLAT = 0
+PROGRAM_BLINN_PHONG = 1
+PROGRAM_DEPTH_ONLY = 2
+SAMPLER_BILINEAR = 0
+SAMPLER_TRILINEAR = 1
+SAMPLER_ANISO = 2
+
+
+def perspective(fovy: float, aspect: float, znear: float, zfar: float) -> np.ndarray:
+    """LinalgOps.h:124-149 (left-handed, z in [0, 1], w = z_view)."""
+    top = znear * math.tan(fovy / 2.0)
+    right = top * aspect
+    m = np.zeros((4, 4), dtype=np.float64)
+    m[0, 0] = 2 * znear / (2 * right)
+    m[1, 1] = 2 * znear / (2 * top)
+    m[2, 2] = zfar / (zfar - znear)
+    m[2, 3] = -(zfar * znear) / (zfar - znear)
+    m[3, 2] = 1.0
+    return m
+
+
+def look_at(eye, center, up) -> np.ndarray:
+    """LinalgOps.h:83-122: rows s, u, f then translation by -eye."""
+    eye = np.asarray(eye, np.float64)
+    f = np.asarray(center, np.float64) - eye
+    f /= np.linalg.norm(f)
+    s = np.cross(f, np.asarray(up, np.float64))
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    r = np.eye(4)
+    r[0, :3], r[1, :3], r[2, :3] = s, u, f
+    t = np.eye(4)
+    t[:3, 3] = -eye
+    return r @ t
+
+
+@dataclasses.dataclass
+class Texture:
+    width: int
+    height: int
+    levels: int
+    texels: np.ndarray | None = None      # uint8, all levels packed, or None when generated on the device
+    fbm_seed: int | None = None           # device-generated fBm texture seed
+
+
+@dataclasses.dataclass
+class Scene:
+    name: str
+    width: int
+    height: int
+    samples: int
+    vertices: np.ndarray          # float32 [V, stride]
+    indices: np.ndarray           # int32 [3T]
+    mvp: np.ndarray               # float32 [4, 4] row-major, clip = mvp @ (p, 1)
+    program: int = PROGRAM_FLAT
+    sampler: int = SAMPLER_TRILINEAR
+    max_aniso: int = 1
+    tris_per_material: int = 0
+    camera: tuple = (0.0, 0.0, 0.0)
+    ambient: float = 0.1
+    kd: float = 0.8
+    ks: float = 0.3
+    shininess: float = 32.0
+    lights: list = dataclasses.field(default_factory=list)   # [(pos4, color4)]
+    textures: list = dataclasses.field(default_factory=list)
+
+    @property
+    def stride(self) -> int:
+        return int(self.vertices.shape[1])
+
+    @property
+    def num_triangles(self) -> int:
+        return int(self.indices.size // 3)
+
+
+# ---------------------------------------------------------------------------
+# meshes
+# ---------------------------------------------------------------------------
+
+def uv_sphere(seg_u: int = 64, seg_v: int = 32, radius: float = 1.0):
+    """(seg_u+1) x (seg_v+1) vertices with seam duplicates; 2 triangles per
+    cell including the degenerate halves at the poles."""
+    u = np.arange(seg_u + 1) / seg_u
+    v = np.arange(seg_v + 1) / seg_v
+    uu, vv = np.meshgrid(u, v)
+    theta, phi = uu * 2 * np.pi, vv * np.pi
+    n = np.stack([np.sin(phi) * np.cos(theta), np.cos(phi), np.sin(phi) * np.sin(theta)], -1).reshape(-1, 3)
+    verts = np.concatenate([n * radius, n, np.stack([uu, vv], -1).reshape(-1, 2)], 1)
+    idx = []
+    w = seg_u + 1
+    for i in range(seg_v):
+        for j in range(seg_u):
+            a, b, c, d = i * w + j, (i + 1) * w + j, (i + 1) * w + j + 1, i * w + j + 1
+            idx += [a, b, c, a, c, d]
+    return verts.astype(np.float32), np.asarray(idx, np.int32)
+
+
+def icosphere(level: int):
+    t = (1 + 5 ** 0.5) / 2
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    verts = [np.asarray(p, np.float64) / np.linalg.norm(p) for p in v]
+    faces = list(f)
+    for _ in range(level):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    p = np.asarray(verts)
+    uv = np.stack([np.arctan2(p[:, 2], p[:, 0]) / (2 * np.pi) + 0.5, np.arccos(np.clip(p[:, 1], -1, 1)) / np.pi], -1)
+    return p, uv, np.asarray(faces, np.int32)
+
+
+def random_rotation(rng: np.random.Generator) -> np.ndarray:
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+# ---------------------------------------------------------------------------
+# textures (integer-exact so host and device generators agree bit for bit)
+# ---------------------------------------------------------------------------
+
+def _hash2(ix: np.ndarray, iy: np.ndarray, seed: int) -> np.ndarray:
+    h = (ix.astype(np.uint32) * np.uint32(0x8DA6B343)) ^ (iy.astype(np.uint32) * np.uint32(0xD8163841)) \
+        ^ np.uint32(seed * 0x9E3779B1 & 0xFFFFFFFF)
+    h ^= h >> np.uint32(16)
+    h *= np.uint32(0x7FEB352D)
+    h ^= h >> np.uint32(15)
+    h *= np.uint32(0x846CA68B)
+    h ^= h >> np.uint32(16)
+    return h
+
+
+def fbm_texture(size: int, seed: int, octaves: int = 6) -> np.ndarray:
+    """Tileable value-noise fBm in 16-bit fixed point; RGBA8 level 0.
+
+    Octave o has cells of size >> (o + 2) texels; corner values are 16-bit
+    hashes; bilinear weights are exact 16-bit fractions; octave o contributes
+    value >> (o + 1).  Channel c uses seed * 4 + c; alpha is 255."""
+    y, x = np.meshgrid(np.arange(size, dtype=np.int64), np.arange(size, dtype=np.int64), indexing="ij")
+    out = np.empty((size, size, 4), np.uint8)
+    for ch in range(3):
+        acc = np.zeros((size, size), np.int64)
+        for o in range(octaves):
+            cell = max(1, size >> (o + 2))
+            shift = cell.bit_length() - 1
+            ncell = size // cell
+            cx, cy = x >> shift, y >> shift
+            wx = (x & (cell - 1)) << (16 - shift)
+            wy = (y & (cell - 1)) << (16 - shift)
+            s = (seed * 4 + ch) * 16 + o
+            v00 = (_hash2(cx % ncell, cy % ncell, s) & 0xFFFF).astype(np.int64)
+            v10 = (_hash2((cx + 1) % ncell, cy % ncell, s) & 0xFFFF).astype(np.int64)
+            v01 = (_hash2(cx % ncell, (cy + 1) % ncell, s) & 0xFFFF).astype(np.int64)
+            v11 = (_hash2((cx + 1) % ncell, (cy + 1) % ncell, s) & 0xFFFF).astype(np.int64)
+            top = v00 * (65536 - wx) + v10 * wx
+            bot = v01 * (65536 - wx) + v11 * wx
+            n = (top * (65536 - wy) + bot * wy) >> 32
+            acc += n >> (o + 1)
+        out[..., ch] = np.minimum(acc >> 8, 255).astype(np.uint8)
+    out[..., 3] = 255
+    return out
+
+
+def checker_texture(size: int = 512, checks: int = 8) -> np.ndarray:
+    y, x = np.meshgrid(np.arange(size), np.arange(size), indexing="ij")
+    c = ((x // (size // checks)) + (y // (size // checks))) & 1
+    out = np.empty((size, size, 4), np.uint8)
+    out[..., 0] = np.where(c, 230, 40)
+    out[..., 1] = np.where(c, 200, 60)
+    out[..., 2] = np.where(c, 170, 90)
+    out[..., 3] = 255
+    return out
+
+
+def num_levels(w: int, h: int) -> int:
+    return int(math.floor(math.log2(max(w, h)))) + 1
+
+
+def build_mips(level0: np.ndarray) -> np.ndarray:
+    """Box filter, (a + b + c + d + 2) >> 2 per channel; levels packed."""
+    h, w = level0.shape[:2]
+    out = [level0.reshape(-1)]
+    cur = level0.astype(np.int32)
+    for _ in range(1, num_levels(w, h)):
+        ph, pw = cur.shape[:2]
+        nh, nw = max(1, ph >> 1), max(1, pw >> 1)
+        ys0 = np.minimum(np.arange(nh) * 2, ph - 1)
+        ys1 = np.minimum(np.arange(nh) * 2 + 1, ph - 1)
+        xs0 = np.minimum(np.arange(nw) * 2, pw - 1)
+        xs1 = np.minimum(np.arange(nw) * 2 + 1, pw - 1)
+        s = cur[ys0][:, xs0] + cur[ys0][:, xs1] + cur[ys1][:, xs0] + cur[ys1][:, xs1]
+        cur = (s + 2) >> 2
+        out.append(cur.astype(np.uint8).reshape(-1))
+    return np.concatenate(out)
+
+
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+
+def _mvp(eye, center, fov_deg, w, h, near=0.1, far=100.0):
+    return (perspective(math.radians(fov_deg), w / h, near, far) @ look_at(eye, center, (0, 1, 0))).astype(np.float32)
+
+
+def config1(width: int = 640, height: int = 480) -> Scene:
+    """UV sphere 64x32 (4096 tris) + 2-tri ground quad; 512^2 checker with
+    mips, trilinear; Blinn-Phong with one directional light; 1 spp."""
+    sv, si = uv_sphere(64, 32, 1.0)
+    g = np.array([[-4, -1, -4, 0, 1, 0, 0, 0], [4, -1, -4, 0, 1, 0, 4, 0], [4, -1, 4, 0, 1, 0, 4, 4],
+                  [-4, -1, 4, 0, 1, 0, 0, 4]], np.float32)
+    gi = np.array([0, 2, 1, 0, 3, 2], np.int32) + sv.shape[0]
+    verts = np.concatenate([sv, g])
+    idx = np.concatenate([si, gi])
+    eye = (0.6, 1.2, -3.2)
+    tex = checker_texture(512, 8)
+    return Scene("config1", width, height, 1, verts, idx, _mvp(eye, (0, 0, 0), 60.0, width, height),
+                 program=PROGRAM_BLINN_PHONG, sampler=SAMPLER_TRILINEAR, max_aniso=1, camera=eye,
+                 lights=[((0.4, 0.8, -0.45, 0.0), (1.0, 1.0, 1.0, 0.0))],
+                 textures=[Texture(512, 512, num_levels(512, 512), build_mips(tex))])
+
+
+def config2(width: int = 1920, height: int = 1080, samples: int = 8, spheres: int = 256, level: int = 4,
+            tex_size: int = 2048, seed: int = 20241202, host_textures: bool = False) -> Scene:
+    """Seeded icospheres (level 4: 2562 verts / 5120 tris each) at U[-20,20]^3,
+    log-uniform scale [0.2, 3], uniform rotation; per-sphere fBm texture with
+    mips, 16x anisotropic filtering, Blinn-Phong with 4 point lights."""
+    rng = np.random.default_rng(seed)
+    p, uv, f = icosphere(level)
+    nv, nt = p.shape[0], f.shape[0]
+    verts = np.empty((spheres * nv, 8), np.float32)
+    idx = np.empty((spheres * nt * 3,), np.int32)
+    for s in range(spheres):
+        pos = rng.uniform(-20, 20, 3)
+        scale = math.exp(rng.uniform(math.log(0.2), math.log(3.0)))
+        rot = random_rotation(rng)
+        wp = (p * scale) @ rot.T + pos
+        wn = p @ rot.T
+        verts[s * nv:(s + 1) * nv, 0:3] = wp
+        verts[s * nv:(s + 1) * nv, 3:6] = wn
+        verts[s * nv:(s + 1) * nv, 6:8] = uv
+        idx[s * nt * 3:(s + 1) * nt * 3] = (f + s * nv).reshape(-1)
+    lights = []
+    for _ in range(4):
+        lp = rng.uniform(-25, 25, 3)
+        lc = rng.uniform(0.4, 1.0, 3)
+        lights.append(((float(lp[0]), float(lp[1]), float(lp[2]), 1.0), (float(lc[0]), float(lc[1]), float(lc[2]), 0.002)))
+    eye = (0.0, 0.0, -38.0)
+    levels = num_levels(tex_size, tex_size)
+    textures = []
+    for s in range(spheres):
+        tseed = int(rng.integers(0, 2 ** 31))
+        t = Texture(tex_size, tex_size, levels, None, tseed)
+        if host_textures:
+            t.texels = build_mips(fbm_texture(tex_size, tseed))
+        textures.append(t)
+    return Scene("config2", width, height, samples, verts, idx, _mvp(eye, (0, 0, 0), 60.0, width, height),
+                 program=PROGRAM_BLINN_PHONG, sampler=SAMPLER_ANISO, max_aniso=16, tris_per_material=nt,
+                 camera=eye, lights=lights, textures=textures, ambient=0.08, kd=0.9, ks=0.25)
+
+
+def config3(width: int = 1920, height: int = 1080, samples: int = 4, gx: int = 2048, gy: int = 1024,
+            big: int = 16, seed: int = 7) -> Scene:
+    """2048x1024-cell heightfield displaced by 6-octave fBm (amplitude 0.05 of
+    the extent) plus 16 large triangles covering 10-60 % of the screen;
+    position-only vertices, flat colour."""
+    rng = np.random.default_rng(seed)
+    extent = 20.0
+    xs = np.linspace(-extent, extent, gx + 1)
+    zs = np.linspace(-extent / 2, extent / 2, gy + 1)
+    X, Z = np.meshgrid(xs, zs)
+    noise = fbm_texture(1024, seed)[..., 0].astype(np.float64) / 255.0
+    iy = (np.arange(gy + 1) * 1023 // gy)
+    ix = (np.arange(gx + 1) * 1023 // gx)
+    Y = (noise[iy][:, ix] - 0.5) * 0.05 * extent * 2
+    verts = np.stack([X, Y, Z], -1).reshape(-1, 3)
+    w = gx + 1
+    jj, ii = np.meshgrid(np.arange(gx), np.arange(gy))
+    a = (ii * w + jj).reshape(-1)
+    quads = np.stack([a, a + w, a + w + 1, a, a + w + 1, a + 1], 1).reshape(-1)
+    eye = np.array([0.0, 9.0, -17.0])
+    mvp64 = perspective(math.radians(60.0), width / height, 0.1, 100.0) @ look_at(eye, (0, -1.0, 0.5), (0, 1, 0))
+    inv = np.linalg.inv(mvp64)
+    bverts, bidx = [], []
+    base = verts.shape[0]
+    for k in range(big):
+        frac = rng.uniform(0.10, 0.60)
+        # a random triangle in NDC with area frac * 4 (the NDC square has area 4)
+        while True:
+            c = rng.uniform(-0.6, 0.6, 2)
+            ang = rng.uniform(0, 2 * np.pi)
+            r = math.sqrt(frac * 4 / (1.299038105676658 * rng.uniform(0.8, 1.2)))
+            pts = np.array([[c[0] + r * math.cos(ang + i * 2.0944), c[1] + r * math.sin(ang + i * 2.0944)] for i in range(3)])
+            area = 0.5 * abs((pts[1, 0] - pts[0, 0]) * (pts[2, 1] - pts[0, 1]) - (pts[2, 0] - pts[0, 0]) * (pts[1, 1] - pts[0, 1]))
+            if 0.10 * 4 <= area <= 0.60 * 4:
+                break
+        zn = rng.uniform(0.90, 0.995)
+        for q in pts:
+            ndc = np.array([q[0], q[1], zn, 1.0])
+            wp = inv @ ndc
+            bverts.append(wp[:3] / wp[3])
+        bidx += [base + 3 * k, base + 3 * k + 1, base + 3 * k + 2]
+    verts = np.concatenate([verts, np.asarray(bverts)]).astype(np.float32)
+    idx = np.concatenate([quads, np.asarray(bidx)]).astype(np.int32)
+    return Scene("config3", width, height, samples, verts, idx, mvp64.astype(np.float32), program=PROGRAM_FLAT,
+                 camera=tuple(eye))

@@ -1,0 +1,217 @@
+"""ctypes bindings of the oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this module; the product never does.  It wraps
+  * oracle/_build/libifrit_oracle.so  -- the CPU restatement (oracle_core.cpp,
+    oracle_shade.cpp), built by oracle/Makefile;
+  * oracle/_ref/libifrit_ref.so       -- the reference CPU rasterizer compiled
+    from its own sources by oracle/ref/Makefile (optional: absent where the
+    reference sources are not available).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libifrit_oracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libifrit_ref.so")
+REFERENCE_ROOT = "/root/reference"
+
+
+class RasterConfig(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("samples", C.c_int), ("ccw", C.c_int),
+                ("depth_func", C.c_int), ("clear", C.c_int), ("num_threads", C.c_int), ("live_rcp", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("input_triangles", C.c_int64), ("assembled_triangles", C.c_int64), ("clipped_inputs", C.c_int64),
+                ("bin_entries", C.c_int64), ("fragments", C.c_int64), ("covered_samples", C.c_int64)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+class Tri(C.Structure):
+    _fields_ = [("f1", C.c_float * 4), ("f2", C.c_float * 4), ("f3", C.c_float * 4), ("bx", C.c_float * 3),
+                ("by", C.c_float * 3), ("e1", C.c_float * 3), ("e2", C.c_float * 3), ("e3", C.c_float * 3),
+                ("bbox", C.c_float * 4), ("prim", C.c_int32), ("fan", C.c_int32)]
+
+
+TRI_DTYPE = np.dtype([("f1", "<f4", 4), ("f2", "<f4", 4), ("f3", "<f4", 4), ("bx", "<f4", 3), ("by", "<f4", 3),
+                      ("e1", "<f4", 3), ("e2", "<f4", 3), ("e3", "<f4", 3), ("bbox", "<f4", 4), ("prim", "<i4"),
+                      ("fan", "<i4")])
+
+
+class ShadeParams(C.Structure):
+    _fields_ = [("program", C.c_int), ("stride_floats", C.c_int), ("off_world", C.c_int), ("off_normal", C.c_int),
+                ("off_uv", C.c_int), ("tris_per_material", C.c_int), ("sampler", C.c_int), ("max_aniso", C.c_int),
+                ("camera", C.c_float * 3), ("ambient", C.c_float), ("kd", C.c_float), ("ks", C.c_float),
+                ("shininess", C.c_float), ("num_lights", C.c_int), ("light_pos", (C.c_float * 4) * 4),
+                ("light_color", (C.c_float * 4) * 4)]
+
+
+class TextureDesc(C.Structure):
+    _fields_ = [("texels", C.c_void_p), ("width", C.c_int), ("height", C.c_int), ("levels", C.c_int)]
+
+
+def build(force: bool = False) -> None:
+    """Compile the oracle (and the reference build when its sources exist)."""
+    subprocess.run(["make", "-s", "-C", HERE] + (["-B"] if force else []), check=True)
+    if os.path.isdir(REFERENCE_ROOT):
+        subprocess.run(["make", "-s", "-j8", "-C", os.path.join(HERE, "ref")] + (["-B"] if force else []), check=True)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        _lib.oracle_frame_create.restype = C.c_void_p
+        _lib.oracle_frame_destroy.argtypes = [C.c_void_p]
+        _lib.oracle_num_tris.argtypes = [C.c_void_p]
+        _lib.oracle_get_tris.argtypes = [C.c_void_p, C.c_void_p]
+        _lib.oracle_render.argtypes = [C.c_void_p, C.POINTER(RasterConfig), C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Stats)]
+        _lib.oracle_vertex_transform.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        _lib.oracle_shade.argtypes = [C.c_void_p, C.POINTER(ShadeParams), C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]
+        _lib.oracle_flat_hash.argtypes = [C.c_uint32]
+        _lib.oracle_flat_hash.restype = C.c_uint32
+        _lib.oracle_build_mips.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        _lib.oracle_sample_positions.argtypes = [C.c_int, C.c_void_p]
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        _ref = C.CDLL(REF_PATH)
+        _ref.ref_render_visibility.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                               C.c_int, C.c_void_p, C.c_void_p]
+        _ref.ref_render_timed.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int,
+                                          C.c_void_p, C.c_int, C.c_void_p]
+        _ref.ref_render_timed.restype = C.c_double
+    return _ref
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def vertex_transform(vertices: np.ndarray, mvp: np.ndarray) -> np.ndarray:
+    v = np.ascontiguousarray(vertices, np.float32)
+    m = np.ascontiguousarray(mvp, np.float32)
+    out = np.empty((v.shape[0], 4), np.float32)
+    lib().oracle_vertex_transform(_p(v), v.shape[1], v.shape[0], _p(m), _p(out))
+    return out
+
+
+def sample_positions(samples: int) -> np.ndarray:
+    out = np.empty((samples, 2), np.float32)
+    lib().oracle_sample_positions(samples, _p(out))
+    return out
+
+
+class Frame:
+    """One oracle frame: visibility outputs plus the assembled triangles."""
+
+    def __init__(self):
+        self._h = lib().oracle_frame_create()
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().oracle_frame_destroy(self._h)
+            self._h = None
+
+    def render(self, width, height, samples, clip, indices, depth_func=1, ccw=0, threads=None, live_rcp=False,
+               clear=True, depth_in=None):
+        clip = np.ascontiguousarray(clip, np.float32)
+        indices = np.ascontiguousarray(indices, np.int32)
+        n = width * height
+        ids = np.empty((n * samples,), np.uint32)
+        depth = np.empty((n * samples,), np.float32)
+        tri = np.empty((n * samples,), np.uint32)
+        cls = np.empty((n,), np.uint8)
+        st = Stats()
+        cfg = RasterConfig(width, height, samples, ccw, depth_func, 1 if clear else 0,
+                           threads or os.cpu_count() or 1, 1 if live_rcp else 0)
+        din = None if depth_in is None else np.ascontiguousarray(depth_in, np.float32)
+        rc = lib().oracle_render(self._h, C.byref(cfg), _p(clip), clip.shape[0], _p(indices), indices.size,
+                                 None if din is None else _p(din), _p(ids), _p(depth), _p(tri), _p(cls), C.byref(st))
+        if rc != 0:
+            raise RuntimeError(f"oracle_render failed ({rc})")
+        self.width, self.height, self.samples = width, height, samples
+        self.ids, self.depth, self.tri, self.cls, self.stats = ids, depth, tri, cls, st.as_dict()
+        return self
+
+    def tris(self) -> np.ndarray:
+        n = lib().oracle_num_tris(self._h)
+        out = np.zeros((n,), TRI_DTYPE)
+        if n:
+            lib().oracle_get_tris(self._h, _p(out))
+        return out
+
+    def shade(self, scene, threads=None):
+        """RGBA8 colour and boundary flags from the FP64 shading oracle."""
+        p = ShadeParams()
+        p.program = scene.program
+        p.stride_floats = scene.stride
+        p.off_world, p.off_normal, p.off_uv = 0, 3, 6
+        p.tris_per_material = scene.tris_per_material
+        p.sampler, p.max_aniso = scene.sampler, scene.max_aniso
+        p.camera[:] = list(scene.camera)
+        p.ambient, p.kd, p.ks, p.shininess = scene.ambient, scene.kd, scene.ks, scene.shininess
+        p.num_lights = len(scene.lights)
+        for i, (lp, lc) in enumerate(scene.lights):
+            p.light_pos[i][:] = list(lp)
+            p.light_color[i][:] = list(lc)
+        descs = (TextureDesc * max(1, len(scene.textures)))()
+        keep = []
+        for i, t in enumerate(scene.textures):
+            if t.texels is None:
+                raise ValueError("texture texels must be materialised for the shading oracle")
+            arr = np.ascontiguousarray(t.texels, np.uint8)
+            keep.append(arr)
+            descs[i] = TextureDesc(arr.ctypes.data, t.width, t.height, t.levels)
+        n = self.width * self.height
+        rgba = np.empty((n * 4,), np.uint8)
+        flags = np.empty((n,), np.uint8)
+        verts = np.ascontiguousarray(scene.vertices, np.float32)
+        idx = np.ascontiguousarray(scene.indices, np.int32)
+        lib().oracle_shade(self._h, C.byref(p), self.width, self.height, self.samples, _p(self.tri), _p(idx),
+                           _p(verts), descs, len(scene.textures), _p(rgba), _p(flags), threads or os.cpu_count() or 1)
+        return rgba.reshape(self.height, self.width, 4), flags.reshape(self.height, self.width)
+
+
+def render_scene(scene, threads=None, live_rcp=False):
+    clip = vertex_transform(scene.vertices, scene.mvp)
+    f = Frame().render(scene.width, scene.height, scene.samples, clip, scene.indices, threads=threads,
+                       live_rcp=live_rcp)
+    return f, clip
+
+
+def ref_visibility(width, height, clip, indices, depth_func=1):
+    clip = np.ascontiguousarray(clip, np.float32)
+    indices = np.ascontiguousarray(indices, np.int32)
+    ids = np.empty((width * height,), np.uint32)
+    depth = np.empty((width * height,), np.float32)
+    bad = ref().ref_render_visibility(width, height, 0, depth_func, _p(clip), clip.shape[0], _p(indices),
+                                      indices.size, _p(ids), _p(depth))
+    return ids, depth, bad
+
+
+def flat_hash(prim: int) -> int:
+    return lib().oracle_flat_hash(prim)

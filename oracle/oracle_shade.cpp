@@ -1,0 +1,322 @@
+// ORACLE — TEST INFRASTRUCTURE ONLY (see oracle_core.h).
+//
+// FP64 evaluation of the built-in fragment programs; see oracle_shade.h for
+// the contract.  The sampler follows the CUDA spec in
+// include/ifrit/softgraphics/engine/math/ShaderBuiltinCuda.cuh with three
+// documented corrections: texel centres at (i + 0.5) / size instead of
+// uv * (size - 1) (SBC:112-113), the footprint of the x and y derivatives
+// measured as |d(uv)/dx| and |d(uv)/dy| (SBC:236-241, 248-250 mix the axes),
+// and anisotropic taps placed symmetrically at (i + 0.5) / N - 0.5 instead of
+// i / (N + 1) - 0.5 (SBC:259-260), which shifts even a single tap by half a
+// footprint.
+#include "oracle_shade.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace {
+
+struct D3 {
+    double x, y, z;
+};
+inline D3 operator+(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline D3 operator-(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline D3 operator*(D3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline D3 normalize(D3 a) {
+    const double l = std::sqrt(dot(a, a));
+    return l > 0 ? a * (1.0 / l) : a;
+}
+
+struct Tex {
+    const uint8_t* base;
+    int w, h, levels;
+    std::vector<size_t> off;
+};
+
+void texel(const Tex& t, int l, long x, long y, double* c) {
+    const long w = std::max(1, t.w >> l), h = std::max(1, t.h >> l);
+    x = ((x % w) + w) % w;
+    y = ((y % h) + h) % h;
+    const uint8_t* p = t.base + t.off[l] + 4 * ((size_t)y * w + x);
+    for (int k = 0; k < 4; k++) c[k] = p[k] / 255.0;
+}
+
+void bilinear(const Tex& t, int l, double u, double v, double* c) {
+    const double w = std::max(1, t.w >> l), h = std::max(1, t.h >> l);
+    const double s = u * w - 0.5, r = v * h - 0.5;
+    const double fx0 = std::floor(s), fy0 = std::floor(r);
+    const double fx = s - fx0, fy = r - fy0;
+    const long x0 = (long)fx0, y0 = (long)fy0;
+    double a[4], b[4], cc[4], d[4];
+    texel(t, l, x0, y0, a);
+    texel(t, l, x0 + 1, y0, b);
+    texel(t, l, x0, y0 + 1, cc);
+    texel(t, l, x0 + 1, y0 + 1, d);
+    for (int k = 0; k < 4; k++) {
+        const double top = a[k] + (b[k] - a[k]) * fx;
+        const double bot = cc[k] + (d[k] - cc[k]) * fx;
+        c[k] = top + (bot - top) * fy;
+    }
+}
+
+void trilinear(const Tex& t, double u, double v, double lod, double* c) {
+    lod = std::min(std::max(lod, 0.0), (double)(t.levels - 1));
+    const int l0 = (int)std::floor(lod);
+    const int l1 = std::min(l0 + 1, t.levels - 1);
+    const double f = lod - l0;
+    double a[4], b[4];
+    bilinear(t, l0, u, v, a);
+    bilinear(t, l1, u, v, b);
+    for (int k = 0; k < 4; k++) c[k] = a[k] + (b[k] - a[k]) * f;
+}
+
+bool near_integer(double r) { return std::fabs(r - std::nearbyint(r)) < 1e-4 * std::max(1.0, std::fabs(r)); }
+
+// Returns true when a discontinuous decision is too close to call.
+bool sample_tex(const Tex& t, int sampler, int maxAniso, double u, double v, double dudx, double dvdx, double dudy,
+                double dvdy, double* c) {
+    if (sampler == ORS_BILINEAR) {
+        bilinear(t, 0, u, v, c);
+        return false;
+    }
+    const double px = std::sqrt((dudx * t.w) * (dudx * t.w) + (dvdx * t.h) * (dvdx * t.h));
+    const double py = std::sqrt((dudy * t.w) * (dudy * t.w) + (dvdy * t.h) * (dvdy * t.h));
+    const double pmax = std::max(px, py), pmin = std::min(px, py);
+    if (sampler == ORS_TRILINEAR || pmax <= 0) {
+        const double lod = pmax > 0 ? std::log2(pmax) : 0.0;
+        trilinear(t, u, v, lod, c);
+        return false;
+    }
+    bool flag = std::fabs(px - py) <= 1e-4 * pmax;
+    double ratio = pmin > 0 ? pmax / pmin : (double)maxAniso;
+    int n;
+    if (ratio >= maxAniso) {
+        n = maxAniso;
+        if (std::fabs(ratio - maxAniso) < 1e-4 * maxAniso) flag = true;
+    } else {
+        n = (int)std::ceil(ratio);
+        if (near_integer(ratio)) flag = true;
+    }
+    n = std::max(1, n);
+    const double lod = std::log2(pmax / n);
+    const double au = px >= py ? dudx : dudy;
+    const double av = px >= py ? dvdx : dvdy;
+    double acc[4] = {0, 0, 0, 0};
+    for (int i = 0; i < n; i++) {
+        const double o = (i + 0.5) / n - 0.5;
+        double s[4];
+        trilinear(t, u + au * o, v + av * o, lod, s);
+        for (int k = 0; k < 4; k++) acc[k] += s[k];
+    }
+    for (int k = 0; k < 4; k++) c[k] = acc[k] / n;
+    return flag;
+}
+
+struct Shader {
+    const OracleShadeParams* p;
+    const std::vector<OracleTri>* tris;
+    const int32_t* idx;
+    const float* vtx;
+    const std::vector<Tex>* tex;
+
+    // Perspective-correct vertex weights of triangle t at (x, y).
+    void weights(const OracleTri& t, double x, double y, double* w) const {
+        const double b1 = (double)t.f1[0] * x + (double)t.f1[1] * y + (double)t.f1[2];
+        const double b2 = (double)t.f2[0] * x + (double)t.f2[1] * y + (double)t.f2[2];
+        const double b3 = (double)t.f3[0] * x + (double)t.f3[1] * y + (double)t.f3[2];
+        const double s = b1 + b2 + b3;
+        const double B[3] = {b1 / s, b2 / s, b3 / s};
+        w[0] = B[0] * t.bx[0] + B[1] * t.bx[1] + B[2] * t.bx[2];
+        w[1] = B[0] * t.by[0] + B[1] * t.by[1] + B[2] * t.by[2];
+        w[2] = 1.0 - w[0] - w[1];
+    }
+    const float* vert(int prim, int k) const {
+        return vtx + (size_t)idx[3 * prim + k] * p->stride_floats;
+    }
+    void uv_at(const OracleTri& t, double x, double y, double* uv) const {
+        double w[3];
+        weights(t, x, y, w);
+        uv[0] = uv[1] = 0;
+        for (int k = 0; k < 3; k++) {
+            const float* v = vert(t.prim, k);
+            uv[0] += w[k] * v[p->off_uv];
+            uv[1] += w[k] * v[p->off_uv + 1];
+        }
+    }
+
+    // Colour of triangle ti shaded at pixel (px, py); o = 0 (1 spp, pixel
+    // corner) or 0.5 (MSAA, pixel centre).
+    bool shade(uint32_t ti, int px, int py, double o, double* rgba) const {
+        const OracleTri& t = (*tris)[ti];
+        if (p->program == ORP_FLAT) {
+            const uint32_t h = oracle_flat_hash((uint32_t)t.prim);
+            rgba[0] = (h & 0xff) / 255.0;
+            rgba[1] = ((h >> 8) & 0xff) / 255.0;
+            rgba[2] = ((h >> 16) & 0xff) / 255.0;
+            rgba[3] = 1.0;
+            return false;
+        }
+        if (p->program != ORP_BLINN_PHONG) {
+            rgba[0] = rgba[1] = rgba[2] = rgba[3] = 0;
+            return false;
+        }
+        const double x = px + o, y = py + o;
+        double w[3];
+        weights(t, x, y, w);
+        D3 P{0, 0, 0}, N{0, 0, 0};
+        double uv[2] = {0, 0};
+        for (int k = 0; k < 3; k++) {
+            const float* v = vert(t.prim, k);
+            P = P + D3{v[p->off_world], v[p->off_world + 1], v[p->off_world + 2]} * w[k];
+            N = N + D3{v[p->off_normal], v[p->off_normal + 1], v[p->off_normal + 2]} * w[k];
+            uv[0] += w[k] * v[p->off_uv];
+            uv[1] += w[k] * v[p->off_uv + 1];
+        }
+        // Fine quad derivatives: the same triangle evaluated at the quad's
+        // neighbouring shading positions (SBC:36-76).
+        const int qx = px & ~1, qy = py & ~1;
+        double a[2], b[2], c[2], d[2];
+        uv_at(t, qx + o, y, a);
+        uv_at(t, qx + 1 + o, y, b);
+        uv_at(t, x, qy + o, c);
+        uv_at(t, x, qy + 1 + o, d);
+        const int mat = p->tris_per_material > 0 ? t.prim / p->tris_per_material : 0;
+        double alb[4] = {1, 1, 1, 1};
+        bool flag = false;
+        if (!tex->empty()) {
+            const Tex& tx = (*tex)[std::min<int>(mat, (int)tex->size() - 1)];
+            flag = sample_tex(tx, p->sampler, p->max_aniso, uv[0], uv[1], b[0] - a[0], b[1] - a[1], d[0] - c[0],
+                              d[1] - c[1], alb);
+        }
+        N = normalize(N);
+        const D3 V = normalize(D3{p->camera[0], p->camera[1], p->camera[2]} - P);
+        D3 col = D3{alb[0], alb[1], alb[2]} * p->ambient;
+        for (int l = 0; l < p->num_lights; l++) {
+            const float* lp = p->light_pos[l];
+            const float* lc = p->light_color[l];
+            D3 L;
+            double att = 1.0;
+            if (lp[3] == 0.0f) {
+                L = normalize(D3{lp[0], lp[1], lp[2]});
+            } else {
+                const D3 Lv = D3{lp[0], lp[1], lp[2]} - P;
+                const double d2 = dot(Lv, Lv);
+                L = Lv * (1.0 / std::sqrt(d2));
+                att = 1.0 / (1.0 + lc[3] * d2);
+            }
+            const double ndl = dot(N, L);
+            if (std::fabs(ndl) < 1e-4) flag = true;
+            if (ndl > 0) {
+                const D3 Hh = normalize(L + V);
+                const double ndh = std::max(dot(N, Hh), 0.0);
+                const double spec = p->ks * std::pow(ndh, (double)p->shininess);
+                col.x += att * lc[0] * (p->kd * ndl * alb[0] + spec);
+                col.y += att * lc[1] * (p->kd * ndl * alb[1] + spec);
+                col.z += att * lc[2] * (p->kd * ndl * alb[2] + spec);
+            }
+        }
+        rgba[0] = col.x;
+        rgba[1] = col.y;
+        rgba[2] = col.z;
+        rgba[3] = alb[3];
+        return flag;
+    }
+};
+
+inline uint8_t to_u8(double c) {
+    c = std::min(std::max(c, 0.0), 1.0);
+    return (uint8_t)std::floor(c * 255.0 + 0.5);
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t oracle_flat_hash(uint32_t prim) {
+    uint32_t h = prim * 0x9E3779B1u;
+    h ^= h >> 15;
+    h *= 0x85EBCA77u;
+    h ^= h >> 13;
+    return h;
+}
+
+void oracle_build_mips(uint8_t* texels, int width, int height, int levels) {
+    size_t off = 0;
+    for (int l = 1; l < levels; l++) {
+        const int pw = std::max(1, width >> (l - 1)), ph = std::max(1, height >> (l - 1));
+        const int w = std::max(1, width >> l), h = std::max(1, height >> l);
+        const uint8_t* src = texels + off;
+        uint8_t* dst = texels + off + (size_t)pw * ph * 4;
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++)
+                for (int k = 0; k < 4; k++) {
+                    const int x0 = std::min(2 * x, pw - 1), x1 = std::min(2 * x + 1, pw - 1);
+                    const int y0 = std::min(2 * y, ph - 1), y1 = std::min(2 * y + 1, ph - 1);
+                    const int s = src[4 * ((size_t)y0 * pw + x0) + k] + src[4 * ((size_t)y0 * pw + x1) + k] +
+                                  src[4 * ((size_t)y1 * pw + x0) + k] + src[4 * ((size_t)y1 * pw + x1) + k];
+                    dst[4 * ((size_t)y * w + x) + k] = (uint8_t)((s + 2) >> 2);
+                }
+        off += (size_t)pw * ph * 4;
+    }
+}
+
+int oracle_shade(const OracleFrame* f, const OracleShadeParams* p, int W, int H, int S, const uint32_t* tri_ids,
+                 const int32_t* indices, const float* vertices, const OracleTexture* textures, int ntex,
+                 uint8_t* rgba8_out, uint8_t* flag_out, int num_threads) {
+    std::vector<OracleTri> tris((size_t)oracle_num_tris(f));
+    if (!tris.empty()) oracle_get_tris(f, tris.data());
+    std::vector<Tex> tex((size_t)std::max(0, ntex));
+    for (int i = 0; i < ntex; i++) {
+        tex[i].base = textures[i].texels;
+        tex[i].w = textures[i].width;
+        tex[i].h = textures[i].height;
+        tex[i].levels = textures[i].levels;
+        size_t off = 0;
+        for (int l = 0; l < textures[i].levels; l++) {
+            tex[i].off.push_back(off);
+            off += (size_t)std::max(1, textures[i].width >> l) * std::max(1, textures[i].height >> l) * 4;
+        }
+    }
+    Shader sh{p, &tris, indices, vertices, &tex};
+    const double o = S == 1 ? 0.0 : 0.5;
+    std::atomic<int> next{0}, flagged{0};
+    auto worker = [&]() {
+        for (;;) {
+            const int row = next.fetch_add(1);
+            if (row >= H) break;
+            for (int x = 0; x < W; x++) {
+                const size_t pix = (size_t)row * W + x;
+                const uint32_t* ids = tri_ids + pix * S;
+                double sum[4] = {0, 0, 0, 0};
+                bool flag = false;
+                for (int s = 0; s < S; s++) {
+                    if (ids[s] == 0xffffffffu) continue;
+                    int first = s;
+                    for (int q = 0; q < s; q++)
+                        if (ids[q] == ids[s]) { first = q; break; }
+                    if (first != s) continue;  // shade each distinct triangle once
+                    int count = 0;
+                    for (int q = s; q < S; q++) count += ids[q] == ids[s];
+                    double c[4];
+                    flag |= sh.shade(ids[s], x, row, o, c);
+                    for (int k = 0; k < 4; k++) sum[k] += std::min(std::max(c[k], 0.0), 1.0) * count;
+                }
+                for (int k = 0; k < 4; k++) rgba8_out[4 * pix + k] = to_u8(sum[k] / S);
+                if (flag_out) flag_out[pix] = flag;
+                if (flag) flagged++;
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int i = 1; i < std::max(1, num_threads); i++) th.emplace_back(worker);
+    worker();
+    for (auto& t : th) t.join();
+    return flagged.load();
+}
+
+}  // extern "C"

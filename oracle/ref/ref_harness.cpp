@@ -1,0 +1,204 @@
+// ORACLE — TEST INFRASTRUCTURE ONLY.
+//
+// Drives the reference CPU tile rasterizer, compiled from its own sources
+// under /root/reference by oracle/ref/Makefile, so that the restatement in
+// oracle/oracle_core.cpp can be checked against the real thing.  Nothing of
+// the reference is copied here: this file only calls its public C++ API
+// (TileRasterRenderer.h:86-104, FrameBuffer.h, VertexBuffer.h,
+// BufferManager.h, Shaders.h).
+//
+// Exposed entry points (ctypes):
+//   ref_render_visibility  per-pixel primitive ID and depth of the winning
+//                          fragment for given clip-space vertices;
+//   ref_render_timed       a full frame (vertex transform + Blinn-Phong-like
+//                          shading) timed with the reference's own 17 threads.
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "ifrit/softgraphics/engine/base/FrameBuffer.h"
+#include "ifrit/softgraphics/engine/base/Shaders.h"
+#include "ifrit/softgraphics/engine/base/TypeDescriptor.h"
+#include "ifrit/softgraphics/engine/base/VaryingDescriptor.h"
+#include "ifrit/softgraphics/engine/base/VertexBuffer.h"
+#include "ifrit/softgraphics/engine/bufferman/BufferManager.h"
+#include "ifrit/softgraphics/engine/tileraster/TileRasterRenderer.h"
+
+using namespace Ifrit;
+using namespace Ifrit::Graphics::SoftGraphics;
+using namespace Ifrit::Graphics::SoftGraphics::TileRaster;
+using Ifrit::Graphics::SoftGraphics::Core::Data::ImageF32;
+
+namespace {
+
+// Outputs the clip position stored in attribute 0 and passes attribute 1
+// through as varying 0.
+class PassVS : public VertexShader {
+public:
+    void execute(const void* const* in, Vector4f* outPos, Vector4f* const* outVary) override {
+        *outPos = *reinterpret_cast<const Vector4f*>(in[0]);
+        *outVary[0] = *reinterpret_cast<const Vector4f*>(in[1]);
+    }
+};
+
+// Writes (interpolated varying 0.x, fragment depth, 1, 1): with a constant
+// per-triangle varying this records the winning primitive and its depth.
+class IdDepthFS : public FragmentShader {
+public:
+    void execute(const void* vary, void* color, float* depth) override {
+        const Vector4f v = reinterpret_cast<const Vector4f*>(vary)[0];
+        Vector4f* c = reinterpret_cast<Vector4f*>(color);
+        c->x = v.x;
+        c->y = *depth;
+        c->z = 1.0f;
+        c->w = 1.0f;
+    }
+};
+
+// clip = M * (p, 1) in the MatMul order of LinalgOps.h:40-48; varying 0 is
+// the normal.
+class TransformVS : public VertexShader {
+public:
+    float m[16];
+    void execute(const void* const* in, Vector4f* outPos, Vector4f* const* outVary) override {
+        const float* p = reinterpret_cast<const float*>(in[0]);
+        const float* n = reinterpret_cast<const float*>(in[1]);
+        outPos->x = m[0] * p[0] + m[1] * p[1] + m[2] * p[2] + m[3] * 1.0f;
+        outPos->y = m[4] * p[0] + m[5] * p[1] + m[6] * p[2] + m[7] * 1.0f;
+        outPos->z = m[8] * p[0] + m[9] * p[1] + m[10] * p[2] + m[11] * 1.0f;
+        outPos->w = m[12] * p[0] + m[13] * p[1] + m[14] * p[2] + m[15] * 1.0f;
+        outVary[0]->x = n[0];
+        outVary[0]->y = n[1];
+        outVary[0]->z = n[2];
+        outVary[0]->w = 0.0f;
+    }
+};
+
+// Lambert + Blinn-style specular with one directional light.
+class LambertFS : public FragmentShader {
+public:
+    void execute(const void* vary, void* color, float* depth) override {
+        const Vector4f n = reinterpret_cast<const Vector4f*>(vary)[0];
+        const float l = 1.0f / std::sqrt(n.x * n.x + n.y * n.y + n.z * n.z + 1e-20f);
+        float d = (n.x * 0.3f + n.y * 0.8f - n.z * 0.52f) * l;
+        d = d > 0.0f ? d : 0.0f;
+        const float s = std::pow(d, 32.0f);
+        Vector4f* c = reinterpret_cast<Vector4f*>(color);
+        c->x = 0.1f + 0.8f * d + s;
+        c->y = 0.1f + 0.6f * d + s;
+        c->z = 0.1f + 0.4f * d + s;
+        c->w = 1.0f;
+    }
+};
+
+struct Target {
+    std::unique_ptr<ImageF32> color, depth;
+    FrameBuffer fb;
+    Target(int w, int h) {
+        color = std::make_unique<ImageF32>(w, h, 4);
+        depth = std::make_unique<ImageF32>(w, h, 1);
+        fb.SetColorAttachmentsCompatible(reinterpret_cast<ImageF32* const*>(&color), 1);
+        fb.SetDepthAttachmentCompatible(depth.get());
+    }
+};
+
+}  // namespace
+
+extern "C" int ref_render_visibility(int W, int H, int ccw, int depthFunc, const float* clip, int nverts,
+                                     const int32_t* idx, int nidx, uint32_t* id_out, float* depth_out) {
+    const int ntri = nidx / 3;
+    if (ntri <= 0) return -1;
+    (void)nverts;
+    Target tgt(W, H);
+    VertexBuffer vb;
+    vb.setLayout({TypeDescriptors.FLOAT4, TypeDescriptors.FLOAT4});
+    vb.setVertexCount(3 * ntri);
+    vb.allocateBuffer(3 * ntri);
+    for (int p = 0; p < ntri; p++) {
+        for (int k = 0; k < 3; k++) {
+            const float* c = clip + 4 * idx[3 * p + k];
+            vb.setValue<Vector4f>(3 * p + k, 0, Vector4f{c[0], c[1], c[2], c[3]});
+            vb.setValue<Vector4f>(3 * p + k, 1, Vector4f{(float)p, 0.0f, 0.0f, 0.0f});
+        }
+    }
+    std::vector<int> seq(3 * ntri);
+    for (int i = 0; i < 3 * ntri; i++) seq[i] = i;
+    auto mgr = std::make_shared<BufferManager::TrivialBufferManager>();
+    mgr->Init();
+    auto ib = mgr->CreateBuffer({seq.size() * sizeof(int)});
+    mgr->bufferData(ib, seq.data(), 0, seq.size() * sizeof(int));
+
+    PassVS vs;
+    IdDepthFS fs;
+    VaryingDescriptor vd;
+    vd.setVaryingDescriptors({TypeDescriptors.FLOAT4});
+    auto r = std::make_shared<TileRasterRenderer>();
+    r->Init();
+    r->bindFrameBuffer(tgt.fb);
+    r->bindVertexBuffer(vb);
+    r->bindIndexBuffer(ib);
+    r->bindVertexShaderLegacy(vs, vd);
+    r->bindFragmentShader(fs);
+    r->SetDepthFunc((IfritCompareOp)depthFunc);
+    (void)ccw;
+    r->drawElements(3 * ntri, true);
+    const float* c = tgt.color->getData();
+    int bad = 0;
+    for (size_t i = 0; i < (size_t)W * H; i++) {
+        if (c[4 * i + 3] == 1.0f) {
+            const float v = c[4 * i];
+            const float rv = std::nearbyint(v);
+            if (std::fabs(v - rv) > 0.01f) bad++;
+            id_out[i] = (uint32_t)rv;
+            depth_out[i] = c[4 * i + 1];
+        } else {
+            id_out[i] = 0xffffffffu;
+            depth_out[i] = 0.0f;
+        }
+    }
+    return bad;
+}
+
+extern "C" double ref_render_timed(int W, int H, const float* verts, int stride, int nverts, const int32_t* idx,
+                                   int nidx, const float* mvp, int frames, float* color_out) {
+    Target tgt(W, H);
+    VertexBuffer vb;
+    vb.setLayout({TypeDescriptors.FLOAT3, TypeDescriptors.FLOAT3});
+    vb.setVertexCount(nverts);
+    vb.allocateBuffer(nverts);
+    for (int i = 0; i < nverts; i++) {
+        const float* v = verts + (size_t)i * stride;
+        vb.setValue<Vector3f>(i, 0, Vector3f{v[0], v[1], v[2]});
+        vb.setValue<Vector3f>(i, 1, Vector3f{v[3], v[4], v[5]});
+    }
+    auto mgr = std::make_shared<BufferManager::TrivialBufferManager>();
+    mgr->Init();
+    auto ib = mgr->CreateBuffer({(size_t)nidx * sizeof(int)});
+    mgr->bufferData(ib, idx, 0, (size_t)nidx * sizeof(int));
+    TransformVS vs;
+    std::memcpy(vs.m, mvp, sizeof(vs.m));
+    LambertFS fs;
+    VaryingDescriptor vd;
+    vd.setVaryingDescriptors({TypeDescriptors.FLOAT4});
+    auto r = std::make_shared<TileRasterRenderer>();
+    r->Init();
+    r->bindFrameBuffer(tgt.fb);
+    r->bindVertexBuffer(vb);
+    r->bindIndexBuffer(ib);
+    r->bindVertexShaderLegacy(vs, vd);
+    r->bindFragmentShader(fs);
+    r->drawElements(nidx, true);  // warm-up
+    std::vector<double> ms;
+    for (int f = 0; f < frames; f++) {
+        auto t0 = std::chrono::steady_clock::now();
+        r->drawElements(nidx, true);
+        auto t1 = std::chrono::steady_clock::now();
+        ms.push_back(std::chrono::duration<double, std::milli>(t1 - t0).count());
+    }
+    if (color_out) std::memcpy(color_out, tgt.color->getData(), (size_t)W * H * 4 * sizeof(float));
+    std::sort(ms.begin(), ms.end());
+    return ms[ms.size() / 2];
+}
