@@ -1,0 +1,176 @@
+// Host side of the MI355X tile rasterizer: the objects the reference's API
+// passes around (images, frame buffers, vertex buffers, varying descriptors,
+// buffers) and TileRasterRenderer, which keeps the method names of
+// include/ifrit/softgraphics/engine/tileraster/TileRasterRenderer.h:86-104 and
+// the CUDA renderer's additions (TileRasterRendererCuda.h:81-115) and runs
+// every draw as a sequence of HIP kernels on one stream.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../../include/ifrit_hip.h"
+#include "../kernels/pipeline.h"
+
+namespace ifrit_hip {
+
+// Core::Data::ImageF32 (include/ifrit/softgraphics/core/data/Image.h:101-115):
+// row-major, channels interleaved.
+struct ImageF32 {
+    size_t width = 0, height = 0, channel = 0;
+    std::vector<float> data;
+};
+
+// FrameBuffer (include/ifrit/softgraphics/engine/base/FrameBuffer.h:33-56).
+struct FrameBuffer {
+    std::vector<ImageF32*> color;
+    ImageF32* depth = nullptr;
+    int width = 0, height = 0;
+};
+
+// VertexBuffer (include/ifrit/softgraphics/engine/base/VertexBuffer.h:33-90):
+// interleaved attributes with byte offsets.
+struct VertexBuffer {
+    std::vector<int> layout;   // TypeDescriptorEnum per attribute
+    std::vector<int> offsets;  // byte offset per attribute
+    int elementSize = 0;
+    int vertexCount = 0;
+    std::vector<uint8_t> buffer;
+};
+
+struct VaryingDescriptor {
+    std::vector<int> types;
+};
+
+// TrivialBufferManager / IfritBuffer (include/ifrit/softgraphics/engine/bufferman/BufferManager.h).
+struct BufferManager;
+struct Buffer {
+    BufferManager* manager = nullptr;
+    std::vector<uint8_t> data;
+};
+struct BufferManager {
+    std::vector<Buffer*> buffers;
+};
+
+constexpr uint32_t kVertexProgramTag = 0x49465653u;    // "IFVS"
+constexpr uint32_t kFragmentProgramTag = 0x49464653u;  // "IFFS"
+
+struct GpuVertexProgram {
+    uint32_t tag = kVertexProgramTag;
+    IfritGpuVertexProgramDesc desc;
+};
+struct GpuFragmentProgram {
+    uint32_t tag = kFragmentProgramTag;
+    IfritGpuFragmentProgramDesc desc;
+};
+
+struct TextureSlot {
+    int width = 0, height = 0, levels = 0;
+    uint8_t* dev = nullptr;
+    std::vector<size_t> levelOffset;  // texel offsets
+    size_t bytes = 0;
+};
+
+// Grow-only device allocation.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    bool ensure(size_t n);
+    void release();
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(ptr);
+    }
+};
+
+class TileRasterRenderer {
+public:
+    TileRasterRenderer();
+    ~TileRasterRenderer();
+
+    // reference API (TileRasterRenderer.h:86-104)
+    void Init();
+    void bindFrameBuffer(FrameBuffer& fb);
+    void bindVertexBuffer(const VertexBuffer& vb);
+    void bindIndexBuffer(const Buffer& ib);
+    void bindVertexShaderFunc(IfritVertexShaderFunctionalPtr f, VaryingDescriptor* vd);
+    void bindFragmentShaderFunc(IfritFragmentShaderFunctionalPtr f);
+    void bindVertexShader(const GpuVertexProgram& p);
+    void bindFragmentShader(const GpuFragmentProgram& p);
+    void setBlendFunc(const IfritColorAttachmentBlendState& s);
+    void SetDepthFunc(int func);
+    void optsetForceDeterministic(bool opt) { deterministic_ = opt; }
+    void optSetDepthTestEnable(bool opt);
+    void drawElements(int indexCount, bool clear);
+    void clear();
+
+    // CUDA renderer additions and backend extensions
+    void setMsaaSamples(int s);
+    void bindVertexDataFloat(const float* data, int strideFloats, int numVertices);
+    void bindIndexData(const int32_t* data, int numIndices);
+    void setRenderTarget(int width, int height);
+    int createTexture(int w, int h, int levels);
+    void copyHostBufferToImage(int tex, const void* rgba8);
+    void uploadTextureLevels(int tex, const void* rgba8);
+    void generateFbmTexture(int tex, uint32_t seed);
+    void generateMipmap(int tex);
+    void downloadTexture(int tex, void* rgba8);
+    void getVisibility(uint32_t* ids, float* depth);
+    void getCoverageClass(uint8_t* cls);
+    void getColorRGBA8(uint8_t* rgba);
+    void getFrameStats(IfritFrameStats* out) const { *out = stats_; }
+    void setStageTiming(bool on) { stageTiming_ = on; }
+    void drawAsync(int indexCount, bool clear);
+    void synchronize();
+    const char* lastError();
+    void setError(const std::string& e);
+    bool ok(hipError_t e, const char* what);
+
+private:
+    bool prepare(int indexCount);
+    void enqueue(bool clear);
+    bool finish(bool clear, int indexCount);
+    void copyOutToHost();
+    void runHostVertexShader();
+
+    hipStream_t stream_ = nullptr;
+    bool initialized_ = false;
+    bool deterministic_ = true;
+    int depthFunc_ = IF_COMPARE_OP_LESS;
+    int depthFuncSaved_ = IF_COMPARE_OP_LESS;
+    bool depthTest_ = true;
+    IfritColorAttachmentBlendState blend_{};
+    int samples_ = 1;
+    int width_ = 0, height_ = 0;
+    FrameBuffer* fb_ = nullptr;
+
+    // inputs
+    const VertexBuffer* hostVb_ = nullptr;
+    const Buffer* hostIb_ = nullptr;
+    bool vtxDirty_ = true, idxDirty_ = true;
+    int nverts_ = 0, stride_ = 0, nidx_ = 0;
+    IfritVertexShaderFunctionalPtr hostVs_ = nullptr;
+    VaryingDescriptor* hostVsVary_ = nullptr;
+    bool hostFs_ = false;
+    GpuVertexProgram vs_{};
+    GpuFragmentProgram fs_{};
+    bool vsBound_ = false, fsBound_ = false;
+
+    // device buffers
+    DevBuf vtx_, idx_, clip_, recs_, bary_, tileBox_, binCount_, binStart_, binCursor_, binList_, bigList_, ctr_;
+    DevBuf visId_, visDepth_, visSlot_, visClass_, rgba8_, rgba32f_, texDesc_;
+    unsigned clipCap_ = 0, binCap_ = 0, bigCap_ = 0;
+    int curTris_ = 0;
+    std::vector<TextureSlot> textures_;
+    bool texDescDirty_ = true;
+
+    IfritFrameStats stats_{};
+    bool stageTiming_ = false;
+    hipEvent_t ev_[8] = {};
+    std::string error_;
+    std::string errorOut_;
+};
+
+}  // namespace ifrit_hip

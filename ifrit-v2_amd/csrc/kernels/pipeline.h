@@ -1,0 +1,115 @@
+// Device-side data layout and launch interface of the MI355X tile rasterizer.
+//
+// HBM layout per draw (V vertices, T input triangles, W x H pixels, S samples):
+//   clip        float4[V]                 vertex stage output
+//   recs        TriRec[T + clipCap]       assembled triangles, slot = primitive for
+//                                         unclipped triangles, T + k for clip fans
+//   bary        float4[2 * (T+clipCap)]   vertex-weight bases bx, by (shading only)
+//   tileBox     int4[T + clipCap]         tile rectangle of each slot (x0 > x1: empty)
+//   binCount/binStart/binCursor  uint[tiles]
+//   binList     uint[binCap]              slot | 0x80000000 when the tile is fully covered
+//   visId/visDepth/visSlot  [W*H*S]       per-sample winner, pixels row-major, samples innermost
+//   rgba8       uchar4[W*H]               resolved colour
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ifrit_hip {
+
+// One assembled triangle for coverage and depth (96 bytes, 16-byte aligned):
+//   q0 = e1.xyz, e2.x   q1 = e2.yz, e3.xy   q2 = e3.z, key, unused, unused
+//   f1, f2, f3 = interpolation bases (x, y, z) and vz = (z/w) * w.
+// key = prim * 4 + fan orders fragments the way the reference's per-tile
+// sort by originalPrimitive does (TileRasterWorker.cpp:1244-1258).
+struct __align__(16) TriRec {
+    float4 q0, q1, q2;
+    float4 f1, f2, f3;
+};
+
+struct SetupParams {
+    int width, height;
+    int tilesX, tilesY;
+    int ntri, nverts;
+    int ccw;
+    int msaa;           // 1 disables the small-triangle cull
+    unsigned clipCap;   // capacity for clip fan triangles
+};
+
+struct BinParams {
+    int tilesX, tilesY;
+    int ntri;
+    unsigned clipCap;
+    unsigned binCap;
+    unsigned bigCap;
+    int bigThreshold;   // triangles spanning more tiles go to the per-workgroup binner
+};
+
+struct RasterParams {
+    int width, height;
+    int tilesX, tilesY;
+    int depthFunc;
+    int clear;
+    unsigned binCap;
+    float sx[8], sy[8];  // sample offsets from the pixel corner
+};
+
+struct TexDesc {
+    unsigned long long base;  // byte offset of level 0 in the texture pool
+    int width, height, levels, pad;
+    unsigned levelOffset[16];  // texel offsets of each level relative to base
+};
+
+struct ShadeParams {
+    int width, height, samples;
+    int program;
+    int stride, offWorld, offNormal, offUv;
+    int trisPerMaterial;
+    int sampler, maxAniso;
+    int numTextures;
+    float camera[3];
+    float ambient, kd, ks, shininess;
+    int numLights;
+    float lightPos[4][4];
+    float lightColor[4][4];
+    int writeFloat;          // also write RGBA32F
+};
+
+// Status word bits raised by the kernels; checked by the host after a frame.
+enum StatusBits : unsigned {
+    kStatusClipOverflow = 1u,
+    kStatusBinOverflow = 2u,
+    kStatusBigOverflow = 4u,
+    kStatusBadIndex = 8u,
+};
+
+// Counters written by the kernels (device memory, zeroed each draw).
+struct DeviceCounters {
+    unsigned status;
+    unsigned clipCount;
+    unsigned bigCount;
+    unsigned binTotal;
+    unsigned long long fragments;
+    unsigned long long coveredSamples;
+    unsigned long long assembled;
+    unsigned long long pad;
+};
+
+// Launch wrappers (pipeline.hip).
+void launch_vertex(const float* vtx, int stride, int nverts, const float* mvp16, float4* clip, hipStream_t s);
+void launch_reset(unsigned* binCount, unsigned* binCursor, int tiles, DeviceCounters* ctr, hipStream_t s);
+void launch_setup(const SetupParams& p, const float4* clip, const int* idx, TriRec* recs, float4* bary, int4* tileBox,
+                  DeviceCounters* ctr, hipStream_t s);
+void launch_bin(const BinParams& p, const TriRec* recs, const int4* tileBox, unsigned* binCount, unsigned* binStart,
+                unsigned* binCursor, unsigned* binList, unsigned* bigList, DeviceCounters* ctr, hipStream_t s);
+void launch_raster(const RasterParams& p, int samples, const TriRec* recs, const unsigned* binStart,
+                   const unsigned* binCount, const unsigned* binList, uint32_t* visId, float* visDepth,
+                   uint32_t* visSlot, uint8_t* visClass, DeviceCounters* ctr, hipStream_t s);
+void launch_shade(const ShadeParams& p, const TriRec* recs, const float4* bary, const uint32_t* visSlot,
+                  const float* vtx, const int* idx, const TexDesc* tex, const uint8_t* texPool, uchar4* rgba8,
+                  float4* rgba32f, hipStream_t s);
+
+// Texture helpers (texture.hip).
+void launch_fbm_texture(uint8_t* level0, int size, unsigned seed, hipStream_t s);
+void launch_mip_level(const uint8_t* src, int sw, int sh, uint8_t* dst, int dw, int dh, hipStream_t s);
+
+}  // namespace ifrit_hip

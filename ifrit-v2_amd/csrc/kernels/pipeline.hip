@@ -1,0 +1,915 @@
+// MI355X (gfx950) kernels of the tile rasterizer: vertex transform, triangle
+// setup with homogeneous clipping, tile binning, fine coverage with the
+// reference's hierarchical classes (or per-sample MSAA) and early Z, and
+// deferred shading with an MSAA resolve.
+//
+// Arithmetic: compiled with -ffp-contract=off; every fmaf below stands where
+// the reference CPU path (projects/softgraphics/src/engine/tileraster/
+// TileRasterWorker.cpp, "TRW") calls an FMA intrinsic, and the x86-specific
+// operations come from common/x86_fp.h.  This makes the triangle records,
+// coverage, depth and winning primitive bit-identical to the CPU path.
+#include <hip/hip_runtime.h>
+
+#include "../common/x86_fp.h"
+#include "pipeline.h"
+
+namespace ifrit_hip {
+
+namespace {
+
+constexpr float kClearDepth = 3.3961514e38f;  // bit pattern 0x7f7f7f7f (TRW:40)
+constexpr float kEps = 1e-8f;                  // TileRasterWorker.h:85
+
+struct Mat4 {
+    float m[16];
+};
+
+// ---------------------------------------------------------------------------
+// Vertex stage: clip = M * (p, 1), MatMul summation order (LinalgOps.h:40-48).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_vertex(const float* __restrict__ vtx, int stride, int n, Mat4 M,
+                                                float4* __restrict__ clip) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* p = vtx + (size_t)i * stride;
+    const float x = p[0], y = p[1], z = p[2];
+    float4 r;
+    r.x = M.m[0] * x + M.m[1] * y + M.m[2] * z + M.m[3] * 1.0f;
+    r.y = M.m[4] * x + M.m[5] * y + M.m[6] * z + M.m[7] * 1.0f;
+    r.z = M.m[8] * x + M.m[9] * y + M.m[10] * z + M.m[11] * 1.0f;
+    r.w = M.m[12] * x + M.m[13] * y + M.m[14] * z + M.m[15] * 1.0f;
+    clip[i] = r;
+}
+
+__global__ void k_reset(unsigned* __restrict__ a, unsigned* __restrict__ b, int n, DeviceCounters* ctr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        a[i] = 0u;
+        b[i] = 0u;
+    }
+    if (i == 0) {
+        ctr->status = 0u;
+        ctr->clipCount = 0u;
+        ctr->bigCount = 0u;
+        ctr->binTotal = 0u;
+        ctr->fragments = 0ull;
+        ctr->coveredSamples = 0ull;
+        ctr->assembled = 0ull;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Triangle setup
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void store_rec(TriRec* recs, unsigned slot, const float e[3][3], float key_bits,
+                                          const float4& f1, const float4& f2, const float4& f3) {
+    TriRec r;
+    r.q0 = make_float4(e[0][0], e[0][1], e[0][2], e[1][0]);
+    r.q1 = make_float4(e[1][1], e[1][2], e[2][0], e[2][1]);
+    r.q2 = make_float4(e[2][2], key_bits, 0.0f, 0.0f);
+    r.f1 = f1;
+    r.f2 = f2;
+    r.f3 = f3;
+    recs[slot] = r;
+}
+
+__device__ __forceinline__ int4 tile_box(const SetupParams& P, float b0, float b1, float b2, float b3) {
+    int4 t;
+    t.x = max(0, ifx::cvtt_i32(b0));
+    t.y = max(0, ifx::cvtt_i32(b1));
+    t.z = min(ifx::cvtt_i32(b2), P.tilesX - 1);
+    t.w = min(ifx::cvtt_i32(b3), P.tilesY - 1);
+    return t;
+}
+
+// Unclipped triangle, vertices rotated (v2, v3, v1) as at TRW:760-763;
+// arithmetic of the AVX2 batch setup TRW:503-727, one lane per thread.
+__device__ bool direct_setup(const SetupParams& P, float4 A, float4 B, float4 C, float e[3][3], float4& f1, float4& f2,
+                             float4& f3, float bb[4]) {
+    const float invW = 1.0f / (float)P.width, invH = 1.0f / (float)P.height;
+    const float aCsInvXR = 2.0f * invW, aCsInvYR = 2.0f * invH;
+    const float aCsFw = 2.0f * (float)P.width, aCsFh = 2.0f * (float)P.height;
+    const float aCsFhFw = (float)(P.height * P.width) * 1.0f;
+    const float aCsFw1 = (float)P.width * 1.0f, aCsFh1 = (float)P.height * 1.0f;
+    const float aInvTileWidth = ifx::rcp_nr_ss(16.0f);
+    const float aFbwDivTw = aCsFw1 * aInvTileWidth, aFbwDivTh = aCsFh1 * aInvTileWidth;
+
+    const float w1i = ifx::rcp_nr_ps(A.w), w2i = ifx::rcp_nr_ps(B.w), w3i = ifx::rcp_nr_ps(C.w);
+    const float x1 = A.x * w1i, y1 = A.y * w1i, z1 = A.z * w1i;
+    const float x2 = B.x * w2i, y2 = B.y * w2i, z2 = B.z * w2i;
+    const float x3 = C.x * w3i, y3 = C.y * w3i, z3 = C.z * w3i;
+
+    const float aV3xSubV1xS = x3 - x1, aV2ySubV1y = y2 - y1, aV3ySubV1y = y3 - y1, aV2xSubV1xS = x2 - x1;
+    const float aEdgeCoef = fmaf(ifx::neg(aV3ySubV1y), aV2xSubV1xS, aV3xSubV1xS * aV2ySubV1y);
+    const float aAr = ifx::rcp_nr_ps(aEdgeCoef);
+    const float aV3ySubV2y = y3 - y2, aV3xSubV2x = x2 - x3, aV1xSubV3x = x3 - x1, aV1ySubV3y = y1 - y3,
+                aV2xSubV1x = x1 - x2;
+    const float aS3 = aV2ySubV1y + aV2xSubV1x, aS1 = aV3ySubV2y + aV3xSubV2x, aS2 = aV1ySubV3y + aV1xSubV3x;
+    const float aCsInvX = aCsInvXR * aAr, aCsInvY = aCsInvYR * aAr;
+
+    // back-face cull (TRW:586-595)
+    const float cullD = fmaf(x3, y1, fmaf(x2, y3, x1 * y2)) - fmaf(x2, y1, fmaf(x1, y3, x3 * y2));
+    if (!(cullD > 0.0f)) return false;
+    // frustum (TRW:597-620)
+    const float minX = ifx::min_ps(x1, ifx::min_ps(x2, x3)), minY = ifx::min_ps(y1, ifx::min_ps(y2, y3));
+    const float minZ = ifx::min_ps(z1, ifx::min_ps(z2, z3));
+    const float maxX = ifx::max_ps(x1, ifx::max_ps(x2, x3)), maxY = ifx::max_ps(y1, ifx::max_ps(y2, y3));
+    const float maxZ = ifx::max_ps(z1, ifx::max_ps(z2, z3));
+    if (maxZ < 0.0f || minZ > 1.0f || minX > 1.0f || maxX < -1.0f || minY > 1.0f || maxY < -1.0f) return false;
+    // small-triangle cull (TRW:623-636), off with MSAA
+    if (!P.msaa) {
+        if (floorf(minX * aCsFw1) == floorf(maxX * aCsFw1) || floorf(minY * aCsFh1) == floorf(maxY * aCsFh1))
+            return false;
+    }
+    bb[0] = fmaf(minX, 0.5f, 0.5f) * aFbwDivTw;
+    bb[1] = fmaf(minY, 0.5f, 0.5f) * aFbwDivTh;
+    bb[2] = fmaf(maxX, 0.5f, 0.5f) * aFbwDivTw;
+    bb[3] = fmaf(maxY, 0.5f, 0.5f) * aFbwDivTh;
+
+    const float rF3Z = ifx::neg(fmaf(x1, aV2ySubV1y, fmaf(y1, aV2xSubV1x, aS3)) * aAr);
+    const float rF1Z = ifx::neg(fmaf(x2, aV3ySubV2y, fmaf(y2, aV3xSubV2x, aS1)) * aAr);
+    const float rF2Z = ifx::neg(fmaf(x3, aV1ySubV3y, fmaf(y3, aV1xSubV3x, aS2)) * aAr);
+    f1 = make_float4((aV3ySubV2y * aCsInvX) * w1i, (aV3xSubV2x * aCsInvY) * w1i, rF1Z * w1i, z1 * A.w);
+    f2 = make_float4((aV1ySubV3y * aCsInvX) * w2i, (aV1xSubV3x * aCsInvY) * w2i, rF2Z * w2i, z2 * B.w);
+    f3 = make_float4((aV2ySubV1y * aCsInvX) * w3i, (aV2xSubV1x * aCsInvY) * w3i, rF3Z * w3i, z3 * C.w);
+
+    e[0][0] = aCsFh * aV2ySubV1y;
+    e[0][1] = aCsFw * aV2xSubV1x;
+    e[0][2] = ((fmaf(x2, y1, ifx::neg(aS3)) - x1 * y2) - kEps) * aCsFhFw;
+    e[1][0] = aCsFh * aV3ySubV2y;
+    e[1][1] = aCsFw * aV3xSubV2x;
+    e[1][2] = ((fmaf(x3, y2, ifx::neg(aS1)) - x2 * y3) - kEps) * aCsFhFw;
+    e[2][0] = aCsFh * aV1ySubV3y;
+    e[2][1] = aCsFw * aV1xSubV3x;
+    e[2][2] = ((fmaf(x1, y3, ifx::neg(aS2)) - x3 * y1) - kEps) * aCsFhFw;
+    return true;
+}
+
+struct ClipV {
+    float b[4];
+    float p[4];
+};
+
+// Homogeneous clipping and fan setup (TRW:136-277) for triangles whose w
+// changes sign.  Emits up to 4 fan triangles into the clip slots.
+__device__ void clip_setup(const SetupParams& P, float4 a, float4 b, float4 c, int prim, TriRec* recs, float4* bary,
+                           int4* tileBox, DeviceCounters* ctr) {
+    ClipV src[3] = {{{1, 0, 0, 0}, {a.x, a.y, a.z, a.w}},
+                    {{0, 1, 0, 0}, {b.x, b.y, b.z, b.w}},
+                    {{0, 0, 1, 0}, {c.x, c.y, c.z, c.w}}};
+    ClipV ret[7];
+    int cnt = 0;
+    ClipV pc = src[0];
+    float npc = ifx::neg(pc.p[3]);
+    for (int j = 0; j < 3; j++) {
+        const ClipV pn = src[(j + 1) % 3];
+        const float npn = ifx::neg(pn.p[3]);
+        if (npc * npn < 0.0f) {
+            float dir[4];
+            for (int k = 0; k < 4; k++) dir[k] = pn.p[k] - pc.p[k];
+            const float t = (-1.0f + pc.p[3]) / ifx::neg(dir[3]);
+            ClipV nv;
+            for (int k = 0; k < 4; k++) nv.p[k] = fmaf(dir[k], t, pc.p[k]);
+            for (int k = 0; k < 4; k++) nv.b[k] = (pn.b[k] - pc.b[k]) * t + pc.b[k];
+            ret[cnt++] = nv;
+        }
+        if (npn < kEps) ret[cnt++] = pn;
+        pc = pn;
+        npc = npn;
+    }
+    if (cnt < 3) return;
+    for (int i = 0; i < cnt; i++) {
+        const float pd = ret[i].p[3];
+        for (int k = 0; k < 4; k++) ret[i].p[k] = ret[i].p[k] / pd;
+        ret[i].p[3] = pd;
+    }
+    const float invW = 1.0f / (float)P.width, invH = 1.0f / (float)P.height;
+    const float csInvXR = invW * 2.0f, csInvYR = invH * 2.0f;
+    const float csFw = (float)P.width * 2.0f, csFh = (float)P.height * 2.0f;
+    const float csFhFw = (float)(P.height * P.width);
+    for (int i = 0; i < cnt - 2; i++) {
+        const float* t1 = ret[0].p;
+        const float* t2 = ret[i + 1].p;
+        const float* t3 = ret[i + 2].p;
+        const float ar = 1.0f / ((t3[0] - t1[0]) * (t2[1] - t1[1]) - (t3[1] - t1[1]) * (t2[0] - t1[0]));
+        const float sV2V1y = t2[1] - t1[1], sV2V1x = t1[0] - t2[0];
+        const float sV3V2y = t3[1] - t2[1], sV3V2x = t2[0] - t3[0];
+        const float sV1V3y = t1[1] - t3[1], sV1V3x = t3[0] - t1[0];
+        const float csInvX = csInvXR * ar, csInvY = csInvYR * ar;
+        const float s3 = ifx::neg(sV2V1y) - sV2V1x, s1 = ifx::neg(sV3V2y) - sV3V2x, s2 = ifx::neg(sV1V3y) - sV1V3x;
+        const float r1 = ifx::rcp_nr_ss(t1[3]), r2 = ifx::rcp_nr_ss(t2[3]), r3 = ifx::rcp_nr_ss(t3[3]);
+        const float4 f3 = make_float4((sV2V1y * csInvX) * r3, (sV2V1x * csInvY) * r3,
+                                      (((ifx::neg(t1[0]) * sV2V1y - t1[1] * sV2V1x) + s3) * ar) * r3, t3[2] * t3[3]);
+        const float4 f1 = make_float4((sV3V2y * csInvX) * r1, (sV3V2x * csInvY) * r1,
+                                      (((ifx::neg(t2[0]) * sV3V2y - t2[1] * sV3V2x) + s1) * ar) * r1, t1[2] * t1[3]);
+        const float4 f2 = make_float4((sV1V3y * csInvX) * r2, (sV1V3x * csInvY) * r2,
+                                      (((ifx::neg(t3[0]) * sV1V3y - t3[1] * sV1V3x) + s2) * ar) * r2, t2[2] * t2[3]);
+        float e[3][3];
+        e[0][0] = csFh * sV2V1y;
+        e[0][1] = csFw * sV2V1x;
+        e[0][2] = csFhFw * (((t2[0] * t1[1] - t1[0] * t2[1]) + s3) - kEps);
+        e[1][0] = csFh * sV3V2y;
+        e[1][1] = csFw * sV3V2x;
+        e[1][2] = csFhFw * (((t3[0] * t2[1] - t2[0] * t3[1]) + s1) - kEps);
+        e[2][0] = csFh * sV1V3y;
+        e[2][1] = csFw * sV1V3x;
+        e[2][2] = csFhFw * (((t1[0] * t3[1] - t3[0] * t1[1]) + s2) - kEps);
+        // triangleCulling (TRW:301-313)
+        const float d = ((((t1[0] * t2[1] + t2[0] * t3[1]) + t3[0] * t1[1]) - t3[0] * t2[1]) - t1[0] * t3[1]) -
+                        t2[0] * t1[1];
+        if (d < 0.0f) continue;
+        // triangleFrustumClip (TRW:278-300)
+        float bmin[3], bmax[3];
+        for (int k = 0; k < 3; k++) {
+            bmin[k] = ifx::min_ps(t1[k], ifx::min_ps(t2[k], t3[k]));
+            bmax[k] = ifx::max_ps(t1[k], ifx::max_ps(t2[k], t3[k]));
+        }
+        if (bmax[2] < 0.0f || bmin[2] > 1.0f || bmax[0] < -1.0f || bmin[0] > 1.0f || bmax[1] < -1.0f ||
+            bmin[1] > 1.0f)
+            continue;
+        if (!P.msaa) {
+            const float fw = (float)P.width * 1.0f, fh = (float)P.height * 1.0f;
+            const float r0 = rintf(fmaf(bmin[0], fw, -0.5f)), r1b = rintf(fmaf(bmin[1], fh, -0.5f));
+            const float r2b = rintf(fmaf(bmax[0], fw, -0.5f)), r3b = rintf(fmaf(bmax[1], fh, -0.5f));
+            if (r0 == r2b || r3b == r1b) continue;
+        }
+        const float fbwDivTw = (1.0f * (float)P.width) / 16.0f, fbwDivTh = (1.0f * (float)P.height) / 16.0f;
+        const float bb0 = fmaf(bmin[0], 0.5f, 0.5f) * fbwDivTw, bb1 = fmaf(bmin[1], 0.5f, 0.5f) * fbwDivTh;
+        const float bb2 = fmaf(bmax[0], 0.5f, 0.5f) * fbwDivTw, bb3 = fmaf(bmax[1], 0.5f, 0.5f) * fbwDivTh;
+        const unsigned k = atomicAdd(&ctr->clipCount, 1u);
+        if (k >= P.clipCap) {
+            atomicOr(&ctr->status, kStatusClipOverflow);
+            return;
+        }
+        const unsigned slot = (unsigned)P.ntri + k;
+        store_rec(recs, slot, e, __uint_as_float((unsigned)prim * 4u + (unsigned)i), f1, f2, f3);
+        bary[2 * slot] = make_float4(ret[0].b[0], ret[i + 1].b[0], ret[i + 2].b[0], ret[0].b[1]);
+        bary[2 * slot + 1] = make_float4(ret[i + 1].b[1], ret[i + 2].b[1], 0.0f, 0.0f);
+        tileBox[slot] = tile_box(P, bb0, bb1, bb2, bb3);
+        atomicAdd(&ctr->assembled, 1ull);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_setup(SetupParams P, const float4* __restrict__ clip,
+                                               const int* __restrict__ idx, TriRec* __restrict__ recs,
+                                               float4* __restrict__ bary, int4* __restrict__ tileBox,
+                                               DeviceCounters* ctr) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.ntri) return;
+    tileBox[p] = make_int4(1, 1, 0, 0);
+    int i0 = idx[3 * p], i1 = idx[3 * p + 1], i2 = idx[3 * p + 2];
+    if (P.ccw) {
+        const int t = i0;
+        i0 = i2;
+        i2 = t;
+    }
+    if ((unsigned)i0 >= (unsigned)P.nverts || (unsigned)i1 >= (unsigned)P.nverts || (unsigned)i2 >= (unsigned)P.nverts) {
+        atomicOr(&ctr->status, kStatusBadIndex);
+        return;
+    }
+    const float4 v1 = clip[i0], v2 = clip[i1], v3 = clip[i2];
+    const float maxw = ifx::max_ps(v1.w, ifx::max_ps(v2.w, v3.w));
+    const float minw = ifx::min_ps(v1.w, ifx::min_ps(v2.w, v3.w));
+    if (maxw * minw < 0.0f) {
+        clip_setup(P, v1, v2, v3, p, recs, bary, tileBox, ctr);
+    } else if (minw > 0.0f) {
+        float e[3][3], bb[4];
+        float4 f1, f2, f3;
+        if (direct_setup(P, v2, v3, v1, e, f1, f2, f3, bb)) {
+            store_rec(recs, (unsigned)p, e, __uint_as_float((unsigned)p * 4u), f1, f2, f3);
+            bary[2 * p] = make_float4(0.0f, 0.0f, 1.0f, 1.0f);
+            bary[2 * p + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            tileBox[p] = tile_box(P, bb[0], bb[1], bb[2], bb[3]);
+            atomicAdd(&ctr->assembled, 1ull);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Binning (TRW:314-397): trivial-reject corner must be inside all three
+// edges; the trivial-accept corner marks the tile fully covered.
+// ---------------------------------------------------------------------------
+struct Edges {
+    float x[3], y[3], z[3];
+};
+
+__device__ __forceinline__ Edges load_edges(const TriRec& r) {
+    Edges E;
+    E.x[0] = r.q0.x; E.y[0] = r.q0.y; E.z[0] = r.q0.z;
+    E.x[1] = r.q0.w; E.y[1] = r.q1.x; E.z[1] = r.q1.y;
+    E.x[2] = r.q1.z; E.y[2] = r.q1.w; E.z[2] = r.q2.x;
+    return E;
+}
+
+// Returns 0: rejected, 1: partially covered, 2: fully covered.
+__device__ __forceinline__ int tile_class(const Edges& E, int tx, int ty) {
+    const float x0 = (float)tx * 16.0f, y0 = (float)ty * 16.0f;
+    const float x1 = x0 + 16.0f, y1 = y0 + 16.0f;
+    int ntr = 0, nta = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const bool right = E.x[k] < 0.0f, down = E.y[k] < 0.0f;
+        const float xr = right ? x1 : x0, yr = down ? y1 : y0;
+        const float xa = right ? x0 : x1, ya = down ? y0 : y1;
+        ntr += fmaf(E.x[k], xr, fmaf(E.y[k], yr, E.z[k])) < 0.0f;
+        nta += fmaf(E.x[k], xa, fmaf(E.y[k], ya, E.z[k])) < 0.0f;
+    }
+    return ntr != 3 ? 0 : (nta == 3 ? 2 : 1);
+}
+
+__device__ __forceinline__ unsigned num_slots(const BinParams& P, const DeviceCounters* ctr) {
+    return (unsigned)P.ntri + min(ctr->clipCount, P.clipCap);
+}
+
+__global__ void __launch_bounds__(256) k_bin_count(BinParams P, const TriRec* __restrict__ recs,
+                                                   const int4* __restrict__ tileBox, unsigned* __restrict__ binCount,
+                                                   unsigned* __restrict__ bigList, DeviceCounters* ctr) {
+    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= num_slots(P, ctr)) return;
+    const int4 b = tileBox[s];
+    if (b.x > b.z || b.y > b.w) return;
+    const int area = (b.z - b.x + 1) * (b.w - b.y + 1);
+    if (area > P.bigThreshold) {
+        const unsigned k = atomicAdd(&ctr->bigCount, 1u);
+        if (k < P.bigCap)
+            bigList[k] = s;
+        else
+            atomicOr(&ctr->status, kStatusBigOverflow);
+        return;
+    }
+    const Edges E = load_edges(recs[s]);
+    for (int ty = b.y; ty <= b.w; ty++)
+        for (int tx = b.x; tx <= b.z; tx++)
+            if (tile_class(E, tx, ty)) atomicAdd(&binCount[ty * P.tilesX + tx], 1u);
+}
+
+__global__ void __launch_bounds__(256) k_bin_big_count(BinParams P, const TriRec* __restrict__ recs,
+                                                       const int4* __restrict__ tileBox,
+                                                       unsigned* __restrict__ binCount,
+                                                       const unsigned* __restrict__ bigList, DeviceCounters* ctr) {
+    const unsigned nbig = min(ctr->bigCount, P.bigCap);
+    for (unsigned i = blockIdx.x; i < nbig; i += gridDim.x) {
+        const unsigned s = bigList[i];
+        const int4 b = tileBox[s];
+        const Edges E = load_edges(recs[s]);
+        const int w = b.z - b.x + 1;
+        const int area = w * (b.w - b.y + 1);
+        for (int t = threadIdx.x; t < area; t += blockDim.x) {
+            const int tx = b.x + t % w, ty = b.y + t / w;
+            if (tile_class(E, tx, ty)) atomicAdd(&binCount[ty * P.tilesX + tx], 1u);
+        }
+    }
+}
+
+// Exclusive scan of the per-tile counts by one 1024-thread workgroup.
+__global__ void __launch_bounds__(1024) k_bin_scan(const unsigned* __restrict__ cnt, unsigned* __restrict__ start,
+                                                   int n, unsigned cap, DeviceCounters* ctr) {
+    __shared__ unsigned part[1024];
+    const int per = (n + 1023) / 1024;
+    const int b0 = threadIdx.x * per;
+    unsigned sum = 0;
+    for (int i = 0; i < per; i++)
+        if (b0 + i < n) sum += cnt[b0 + i];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const unsigned v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    unsigned run = part[threadIdx.x] - sum;
+    for (int i = 0; i < per; i++) {
+        if (b0 + i < n) {
+            start[b0 + i] = run;
+            run += cnt[b0 + i];
+        }
+    }
+    if (threadIdx.x == 1023) {
+        ctr->binTotal = part[1023];
+        if (part[1023] > cap) atomicOr(&ctr->status, kStatusBinOverflow);
+    }
+}
+
+__device__ __forceinline__ void bin_emit(const BinParams& P, int tx, int ty, unsigned s, int cls,
+                                         const unsigned* __restrict__ binStart, unsigned* __restrict__ binCursor,
+                                         unsigned* __restrict__ binList) {
+    const int t = ty * P.tilesX + tx;
+    const unsigned pos = binStart[t] + atomicAdd(&binCursor[t], 1u);
+    if (pos < P.binCap) binList[pos] = s | (cls == 2 ? 0x80000000u : 0u);
+}
+
+__global__ void __launch_bounds__(256) k_bin_scatter(BinParams P, const TriRec* __restrict__ recs,
+                                                     const int4* __restrict__ tileBox,
+                                                     const unsigned* __restrict__ binStart,
+                                                     unsigned* __restrict__ binCursor, unsigned* __restrict__ binList,
+                                                     const DeviceCounters* ctr) {
+    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= num_slots(P, ctr)) return;
+    const int4 b = tileBox[s];
+    if (b.x > b.z || b.y > b.w) return;
+    if ((b.z - b.x + 1) * (b.w - b.y + 1) > P.bigThreshold) return;
+    const Edges E = load_edges(recs[s]);
+    for (int ty = b.y; ty <= b.w; ty++)
+        for (int tx = b.x; tx <= b.z; tx++) {
+            const int c = tile_class(E, tx, ty);
+            if (c) bin_emit(P, tx, ty, s, c, binStart, binCursor, binList);
+        }
+}
+
+__global__ void __launch_bounds__(256) k_bin_big_scatter(BinParams P, const TriRec* __restrict__ recs,
+                                                         const int4* __restrict__ tileBox,
+                                                         const unsigned* __restrict__ binStart,
+                                                         unsigned* __restrict__ binCursor,
+                                                         unsigned* __restrict__ binList,
+                                                         const unsigned* __restrict__ bigList,
+                                                         const DeviceCounters* ctr) {
+    const unsigned nbig = min(ctr->bigCount, P.bigCap);
+    for (unsigned i = blockIdx.x; i < nbig; i += gridDim.x) {
+        const unsigned s = bigList[i];
+        const int4 b = tileBox[s];
+        const Edges E = load_edges(recs[s]);
+        const int w = b.z - b.x + 1;
+        const int area = w * (b.w - b.y + 1);
+        for (int t = threadIdx.x; t < area; t += blockDim.x) {
+            const int tx = b.x + t % w, ty = b.y + t / w;
+            const int c = tile_class(E, tx, ty);
+            if (c) bin_emit(P, tx, ty, s, c, binStart, binCursor, binList);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fine raster + early Z: one 256-thread workgroup per 16x16 tile, one pixel
+// per thread; wave w owns rows 4w..4w+3, i.e. one row of 4x4 blocks, so
+// every 4x2 group and 2x2 quad of the reference's hierarchy (TRW:887-1104)
+// lies inside one wave and is classified with a single ballot.
+//
+// The reference processes a tile's fragments sorted by primitive and keeps
+// a fragment when !(depth >= current) (TRW:2286-2290, 2482-2486, 2828-2832).
+// For finite depths that is the lexicographic minimum of (depth, key), which
+// this kernel computes in any order.
+// ---------------------------------------------------------------------------
+constexpr int kChunk = 64;
+
+template <int S>
+__global__ void __launch_bounds__(256) k_raster(RasterParams P, const TriRec* __restrict__ recs,
+                                                const unsigned* __restrict__ binStart,
+                                                const unsigned* __restrict__ binCount,
+                                                const unsigned* __restrict__ binList, uint32_t* __restrict__ visId,
+                                                float* __restrict__ visDepth, uint32_t* __restrict__ visSlot,
+                                                uint8_t* __restrict__ visClass, DeviceCounters* ctr) {
+    __shared__ TriRec sRec[kChunk];
+    __shared__ unsigned sEnt[kChunk];
+    const int tile = blockIdx.x;
+    const int tx = tile % P.tilesX, ty = tile / P.tilesX;
+    const int t = threadIdx.x;
+    const int lx = t & 15, ly = t >> 4;
+    const int px = tx * 16 + lx, py = ty * 16 + ly;
+    const bool inside = px < P.width && py < P.height;
+    const size_t pix = (size_t)py * P.width + px;
+
+    float bestD[S];
+    unsigned bestKey[S], bestSlot[S];
+    unsigned cls = 0, won = 0;
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        bestD[s] = kClearDepth;
+        bestKey[s] = 0u;           // the depth already stored wins ties
+        bestSlot[s] = 0xffffffffu;
+        if (!P.clear && inside) bestD[s] = visDepth[pix * S + s];
+    }
+    const float fpx = (float)px, fpy = (float)py;
+    // this pixel's 4x4 block corners (TRW:888-919)
+    const float bMinX = fmaf((float)(lx >> 2), 4.0f, (float)(tx * 16) * 1.0f);
+    const float bMinY = fmaf((float)(ly >> 2), 4.0f, (float)(ty * 16) * 1.0f);
+    const float bMaxX = bMinX + 4.0f, bMaxY = bMinY + 4.0f;
+    // ballot bit positions of this pixel's 4x2 group and 2x2 quad
+    const int r = ly & 3;
+    const int gShift0 = (r & ~1) * 16 + (lx & ~3), gShift1 = gShift0 + 16;
+    const int qShift0 = (r & ~1) * 16 + (lx & ~1), qShift1 = qShift0 + 16;
+
+    unsigned long long frags = 0, covs = 0;
+    const unsigned start = binStart[tile];
+    const unsigned n = start >= P.binCap ? 0u : min(binCount[tile], P.binCap - start);
+    for (unsigned base = 0; base < n; base += kChunk) {
+        const unsigned cnt = min((unsigned)kChunk, n - base);
+        __syncthreads();
+        if (t < (int)cnt) sEnt[t] = binList[start + base + t];
+        __syncthreads();
+        for (int q = t; q < (int)cnt * 6; q += 256) {
+            const int e = q / 6, f = q % 6;
+            const float4* src = reinterpret_cast<const float4*>(&recs[sEnt[e] & 0x7fffffffu]);
+            reinterpret_cast<float4*>(&sRec[e])[f] = src[f];
+        }
+        __syncthreads();
+        for (unsigned j = 0; j < cnt; j++) {
+            const TriRec& R = sRec[j];
+            const unsigned ent = sEnt[j];
+            const bool full = (ent >> 31) != 0;
+            const Edges E = load_edges(R);
+            int trn = 3, tan = 3;
+            if (!full) {
+                trn = 0;
+                tan = 0;
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const bool right = E.x[k] < 0.0f, down = E.y[k] < 0.0f;
+                    const float xr = right ? bMaxX : bMinX, yr = down ? bMaxY : bMinY;
+                    const float xa = right ? bMinX : bMaxX, ya = down ? bMinY : bMaxY;
+                    const float nz = ifx::neg(E.z[k]);
+                    trn += fmaf(E.y[k], yr, E.x[k] * xr) < nz;
+                    tan += fmaf(E.y[k], ya, E.x[k] * xa) < nz;
+                }
+            }
+            const float4 F1 = R.f1, F2 = R.f2, F3 = R.f3;
+            const unsigned key = __float_as_uint(R.q2.y);
+            if constexpr (S == 1) {
+                // pixel-level test at the integer pixel position (TRW:1042-1066)
+                int pn = 0;
+#pragma unroll
+                for (int k = 0; k < 3; k++) pn += fmaf(E.y[k], fpy, E.x[k] * fpx) < ifx::neg(E.z[k]);
+                const unsigned long long M = __ballot(pn == 3);
+                unsigned c = 0;
+                if (full || (trn == 3 && tan == 3)) {
+                    c = 1;
+                } else if (trn == 3) {
+                    const bool g = ((M >> gShift0) & 0xFull) == 0xFull && ((M >> gShift1) & 0xFull) == 0xFull;
+                    const bool qd = ((M >> qShift0) & 0x3ull) == 0x3ull && ((M >> qShift1) & 0x3ull) == 0x3ull;
+                    c = g ? 1u : (qd ? 2u : (pn == 3 ? 3u : 0u));
+                }
+                if (c && inside) {
+                    float d;
+                    if (c == 1) {  // TRW:2758-2769
+                        const float b0 = fmaf(F1.x, fpx, fmaf(F1.y, fpy, F1.z));
+                        const float b1 = fmaf(F2.x, fpx, fmaf(F2.y, fpy, F2.z));
+                        const float b2 = fmaf(F3.x, fpx, fmaf(F3.y, fpy, F3.z));
+                        d = fmaf(b0, F1.w, fmaf(b1, F2.w, b2 * F3.w));
+                    } else if (c == 2) {  // TRW:2423-2434
+                        const float b0 = fmaf(F1.x, fpx, fmaf(F1.y, fpy, F1.z));
+                        const float b1 = fmaf(F2.x, fpx, fmaf(F2.y, fpy, F2.z));
+                        const float b2 = fmaf(F3.x, fpx, fmaf(F3.y, fpy, F3.z));
+                        d = fmaf(b2, F3.w, fmaf(b0, F1.w, b1 * F2.w));
+                    } else {  // DPPS dot products, TRW:2261-2270
+                        const float b0 = (fpx * F1.x + fpy * F1.y) + (1.0f * F1.z + 0.0f * F1.w);
+                        const float b1 = (fpx * F2.x + fpy * F2.y) + (1.0f * F2.z + 0.0f * F2.w);
+                        const float b2 = (fpx * F3.x + fpy * F3.y) + (1.0f * F3.z + 0.0f * F3.w);
+                        d = (F1.w * b0 + F2.w * b1) + (F3.w * b2 + 0.0f);
+                    }
+                    frags++;
+                    covs++;
+                    if (d < bestD[0] || (d == bestD[0] && key < bestKey[0])) {
+                        bestD[0] = d;
+                        bestKey[0] = key;
+                        bestSlot[0] = ent & 0x7fffffffu;
+                        cls = c;
+                        won = 1u;
+                    }
+                }
+            } else {
+                if (trn == 3 && inside) {
+                    const bool all = tan == 3;
+                    bool any = false;
+#pragma unroll
+                    for (int s = 0; s < S; s++) {
+                        const float sx = fpx + P.sx[s], sy = fpy + P.sy[s];
+                        bool cov = all;
+                        if (!all) {
+                            int pn = 0;
+#pragma unroll
+                            for (int k = 0; k < 3; k++) pn += fmaf(E.y[k], sy, E.x[k] * sx) < ifx::neg(E.z[k]);
+                            cov = pn == 3;
+                        }
+                        if (!cov) continue;
+                        any = true;
+                        covs++;
+                        const float b0 = fmaf(F1.x, sx, fmaf(F1.y, sy, F1.z));
+                        const float b1 = fmaf(F2.x, sx, fmaf(F2.y, sy, F2.z));
+                        const float b2 = fmaf(F3.x, sx, fmaf(F3.y, sy, F3.z));
+                        const float d = fmaf(b0, F1.w, fmaf(b1, F2.w, b2 * F3.w));
+                        if (d < bestD[s] || (d == bestD[s] && key < bestKey[s])) {
+                            bestD[s] = d;
+                            bestKey[s] = key;
+                            bestSlot[s] = ent & 0x7fffffffu;
+                            won |= 1u << s;
+                        }
+                    }
+                    frags += any;
+                }
+            }
+        }
+    }
+    if (inside) {
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            if ((won >> s) & 1u) {
+                visDepth[pix * S + s] = bestD[s];
+                visSlot[pix * S + s] = bestSlot[s];
+                visId[pix * S + s] = bestKey[s] >> 2;
+            } else if (P.clear) {
+                visDepth[pix * S + s] = kClearDepth;
+                visSlot[pix * S + s] = 0xffffffffu;
+                visId[pix * S + s] = 0xffffffffu;
+            }
+        }
+        if (S == 1 && visClass && (won || P.clear)) visClass[pix] = (uint8_t)cls;
+    }
+    // wave-level reduction of the statistics, one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        frags += __shfl_down(frags, off, 64);
+        covs += __shfl_down(covs, off, 64);
+    }
+    if ((t & 63) == 0) {
+        if (frags) atomicAdd(&ctr->fragments, frags);
+        if (covs) atomicAdd(&ctr->coveredSamples, covs);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Deferred shading and resolve
+// ---------------------------------------------------------------------------
+struct ShadeTri {
+    float4 f1, f2, f3;
+    float bx[3], by[3];
+    int prim;
+};
+
+__device__ __forceinline__ void load_shade_tri(const TriRec* recs, const float4* bary, unsigned slot, ShadeTri& T) {
+    const TriRec& r = recs[slot];
+    T.f1 = r.f1;
+    T.f2 = r.f2;
+    T.f3 = r.f3;
+    T.prim = (int)(__float_as_uint(r.q2.y) >> 2);
+    const float4 a = bary[2 * slot], b = bary[2 * slot + 1];
+    T.bx[0] = a.x; T.bx[1] = a.y; T.bx[2] = a.z;
+    T.by[0] = a.w; T.by[1] = b.x; T.by[2] = b.y;
+}
+
+// Perspective-correct weights of the three original vertices at (x, y).
+__device__ __forceinline__ void vertex_weights(const ShadeTri& T, float x, float y, float w[3]) {
+    const float b1 = fmaf(T.f1.x, x, fmaf(T.f1.y, y, T.f1.z));
+    const float b2 = fmaf(T.f2.x, x, fmaf(T.f2.y, y, T.f2.z));
+    const float b3 = fmaf(T.f3.x, x, fmaf(T.f3.y, y, T.f3.z));
+    const float inv = 1.0f / (b1 + b2 + b3);
+    const float B1 = b1 * inv, B2 = b2 * inv, B3 = b3 * inv;
+    w[0] = fmaf(B1, T.bx[0], fmaf(B2, T.bx[1], B3 * T.bx[2]));
+    w[1] = fmaf(B1, T.by[0], fmaf(B2, T.by[1], B3 * T.by[2]));
+    w[2] = 1.0f - w[0] - w[1];
+}
+
+__device__ __forceinline__ float4 fetch_texel(const uint8_t* pool, const TexDesc& D, int level, int x, int y) {
+    const int w = max(1, D.width >> level), h = max(1, D.height >> level);
+    x &= (w - 1);  // power-of-two sizes, repeat addressing
+    y &= (h - 1);
+    const uchar4 c = reinterpret_cast<const uchar4*>(pool + D.base)[D.levelOffset[level] + (unsigned)y * w + x];
+    return make_float4(c.x * (1.0f / 255.0f), c.y * (1.0f / 255.0f), c.z * (1.0f / 255.0f), c.w * (1.0f / 255.0f));
+}
+
+__device__ __forceinline__ float4 lerp4(float4 a, float4 b, float t) {
+    return make_float4(fmaf(b.x - a.x, t, a.x), fmaf(b.y - a.y, t, a.y), fmaf(b.z - a.z, t, a.z), fmaf(b.w - a.w, t, a.w));
+}
+
+__device__ float4 bilinear(const uint8_t* pool, const TexDesc& D, int level, float u, float v) {
+    const float w = (float)max(1, D.width >> level), h = (float)max(1, D.height >> level);
+    const float s = fmaf(u, w, -0.5f), r = fmaf(v, h, -0.5f);
+    const float fs = floorf(s), fr = floorf(r);
+    const float ax = s - fs, ay = r - fr;
+    const int x0 = (int)fs, y0 = (int)fr;
+    const float4 c00 = fetch_texel(pool, D, level, x0, y0), c10 = fetch_texel(pool, D, level, x0 + 1, y0);
+    const float4 c01 = fetch_texel(pool, D, level, x0, y0 + 1), c11 = fetch_texel(pool, D, level, x0 + 1, y0 + 1);
+    return lerp4(lerp4(c00, c10, ax), lerp4(c01, c11, ax), ay);
+}
+
+__device__ float4 trilinear(const uint8_t* pool, const TexDesc& D, float u, float v, float lod) {
+    lod = fminf(fmaxf(lod, 0.0f), (float)(D.levels - 1));
+    const int l0 = (int)floorf(lod);
+    const int l1 = min(l0 + 1, D.levels - 1);
+    const float f = lod - (float)l0;
+    const float4 a = bilinear(pool, D, l0, u, v);
+    if (f == 0.0f) return a;
+    return lerp4(a, bilinear(pool, D, l1, u, v), f);
+}
+
+__device__ float4 sample_texture(const uint8_t* pool, const TexDesc& D, int sampler, int maxAniso, float u, float v,
+                                 float dudx, float dvdx, float dudy, float dvdy) {
+    if (sampler == 0) return bilinear(pool, D, 0, u, v);
+    const float W = (float)D.width, H = (float)D.height;
+    const float px = sqrtf((dudx * W) * (dudx * W) + (dvdx * H) * (dvdx * H));
+    const float py = sqrtf((dudy * W) * (dudy * W) + (dvdy * H) * (dvdy * H));
+    const float pmax = fmaxf(px, py), pmin = fminf(px, py);
+    if (sampler == 1 || !(pmax > 0.0f)) return trilinear(pool, D, u, v, pmax > 0.0f ? log2f(pmax) : 0.0f);
+    const float ratio = pmin > 0.0f ? pmax / pmin : (float)maxAniso;
+    int n = ratio >= (float)maxAniso ? maxAniso : (int)ceilf(ratio);
+    n = max(1, n);
+    const float lod = log2f(pmax / (float)n);
+    const float au = px >= py ? dudx : dudy, av = px >= py ? dvdx : dvdy;
+    float4 acc = make_float4(0, 0, 0, 0);
+    const float invn = 1.0f / (float)n;
+    for (int i = 0; i < n; i++) {
+        const float o = ((float)i + 0.5f) * invn - 0.5f;
+        const float4 c = trilinear(pool, D, fmaf(au, o, u), fmaf(av, o, v), lod);
+        acc.x += c.x; acc.y += c.y; acc.z += c.z; acc.w += c.w;
+    }
+    return make_float4(acc.x * invn, acc.y * invn, acc.z * invn, acc.w * invn);
+}
+
+__device__ __forceinline__ uint32_t flat_hash(uint32_t prim) {
+    uint32_t h = prim * 0x9E3779B1u;
+    h ^= h >> 15;
+    h *= 0x85EBCA77u;
+    h ^= h >> 13;
+    return h;
+}
+
+__device__ __forceinline__ float3 norm3(float3 a) {
+    const float l2 = a.x * a.x + a.y * a.y + a.z * a.z;
+    const float inv = l2 > 0.0f ? 1.0f / sqrtf(l2) : 1.0f;
+    return make_float3(a.x * inv, a.y * inv, a.z * inv);
+}
+
+__device__ float4 shade_fragment(const ShadeParams& P, const ShadeTri& T, const float* vtx, const int* idx,
+                                 const TexDesc* tex, const uint8_t* pool, int px, int py, float o) {
+    if (P.program == 0) {
+        const uint32_t h = flat_hash((uint32_t)T.prim);
+        return make_float4((h & 0xff) * (1.0f / 255.0f), ((h >> 8) & 0xff) * (1.0f / 255.0f),
+                           ((h >> 16) & 0xff) * (1.0f / 255.0f), 1.0f);
+    }
+    if (P.program != 1) return make_float4(0, 0, 0, 0);
+    const float x = (float)px + o, y = (float)py + o;
+    const float* v0 = vtx + (size_t)idx[3 * T.prim] * P.stride;
+    const float* v1 = vtx + (size_t)idx[3 * T.prim + 1] * P.stride;
+    const float* v2 = vtx + (size_t)idx[3 * T.prim + 2] * P.stride;
+    float w[3];
+    vertex_weights(T, x, y, w);
+    auto interp = [&](int off) { return fmaf(w[0], v0[off], fmaf(w[1], v1[off], w[2] * v2[off])); };
+    const float3 Pw = make_float3(interp(P.offWorld), interp(P.offWorld + 1), interp(P.offWorld + 2));
+    float3 N = make_float3(interp(P.offNormal), interp(P.offNormal + 1), interp(P.offNormal + 2));
+    const float u = interp(P.offUv), v = interp(P.offUv + 1);
+    float4 alb = make_float4(1, 1, 1, 1);
+    if (P.numTextures > 0) {
+        // fine quad derivatives (SBC:36-76): same triangle at the quad's other shading positions
+        const float qx = (float)(px & ~1) + o, qy = (float)(py & ~1) + o;
+        float wa[3], wb[3], wc[3], wd[3];
+        vertex_weights(T, qx, y, wa);
+        vertex_weights(T, qx + 1.0f, y, wb);
+        vertex_weights(T, x, qy, wc);
+        vertex_weights(T, x, qy + 1.0f, wd);
+        auto iuv = [&](const float* ww, int off) { return fmaf(ww[0], v0[off], fmaf(ww[1], v1[off], ww[2] * v2[off])); };
+        const float dudx = iuv(wb, P.offUv) - iuv(wa, P.offUv), dvdx = iuv(wb, P.offUv + 1) - iuv(wa, P.offUv + 1);
+        const float dudy = iuv(wd, P.offUv) - iuv(wc, P.offUv), dvdy = iuv(wd, P.offUv + 1) - iuv(wc, P.offUv + 1);
+        int mat = P.trisPerMaterial > 0 ? T.prim / P.trisPerMaterial : 0;
+        mat = min(mat, P.numTextures - 1);
+        alb = sample_texture(pool, tex[mat], P.sampler, P.maxAniso, u, v, dudx, dvdx, dudy, dvdy);
+    }
+    N = norm3(N);
+    const float3 V = norm3(make_float3(P.camera[0] - Pw.x, P.camera[1] - Pw.y, P.camera[2] - Pw.z));
+    float3 col = make_float3(alb.x * P.ambient, alb.y * P.ambient, alb.z * P.ambient);
+    for (int l = 0; l < P.numLights; l++) {
+        float3 L;
+        float att = 1.0f;
+        if (P.lightPos[l][3] == 0.0f) {
+            L = norm3(make_float3(P.lightPos[l][0], P.lightPos[l][1], P.lightPos[l][2]));
+        } else {
+            const float3 Lv = make_float3(P.lightPos[l][0] - Pw.x, P.lightPos[l][1] - Pw.y, P.lightPos[l][2] - Pw.z);
+            const float d2 = Lv.x * Lv.x + Lv.y * Lv.y + Lv.z * Lv.z;
+            const float inv = 1.0f / sqrtf(d2);
+            L = make_float3(Lv.x * inv, Lv.y * inv, Lv.z * inv);
+            att = 1.0f / (1.0f + P.lightColor[l][3] * d2);
+        }
+        const float ndl = N.x * L.x + N.y * L.y + N.z * L.z;
+        if (ndl > 0.0f) {
+            const float3 Hh = norm3(make_float3(L.x + V.x, L.y + V.y, L.z + V.z));
+            const float ndh = fmaxf(N.x * Hh.x + N.y * Hh.y + N.z * Hh.z, 0.0f);
+            const float spec = P.ks * powf(ndh, P.shininess);
+            col.x += att * P.lightColor[l][0] * (P.kd * ndl * alb.x + spec);
+            col.y += att * P.lightColor[l][1] * (P.kd * ndl * alb.y + spec);
+            col.z += att * P.lightColor[l][2] * (P.kd * ndl * alb.z + spec);
+        }
+    }
+    return make_float4(col.x, col.y, col.z, alb.w);
+}
+
+__device__ __forceinline__ float sat(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+
+template <int S>
+__global__ void __launch_bounds__(256) k_shade(ShadeParams P, const TriRec* __restrict__ recs,
+                                               const float4* __restrict__ bary, const uint32_t* __restrict__ visSlot,
+                                               const float* __restrict__ vtx, const int* __restrict__ idx,
+                                               const TexDesc* __restrict__ tex, const uint8_t* __restrict__ pool,
+                                               uchar4* __restrict__ rgba8, float4* __restrict__ rgba32f) {
+    const int px = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int py = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (px >= P.width || py >= P.height) return;
+    const size_t pix = (size_t)py * P.width + px;
+    unsigned ids[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) ids[s] = visSlot[pix * S + s];
+    const float o = S == 1 ? 0.0f : 0.5f;
+    float4 sum = make_float4(0, 0, 0, 0);
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        if (ids[s] == 0xffffffffu) continue;
+        bool seen = false;
+#pragma unroll
+        for (int q = 0; q < s; q++) seen |= ids[q] == ids[s];
+        if (seen) continue;
+        int count = 0;
+#pragma unroll
+        for (int q = 0; q < S; q++) count += ids[q] == ids[s];
+        ShadeTri T;
+        load_shade_tri(recs, bary, ids[s], T);
+        const float4 c = shade_fragment(P, T, vtx, idx, tex, pool, px, py, o);
+        const float k = (float)count;
+        sum.x += sat(c.x) * k;
+        sum.y += sat(c.y) * k;
+        sum.z += sat(c.z) * k;
+        sum.w += sat(c.w) * k;
+    }
+    const float inv = 1.0f / (float)S;
+    const float4 r = make_float4(sum.x * inv, sum.y * inv, sum.z * inv, sum.w * inv);
+    rgba8[pix] = make_uchar4((unsigned char)floorf(r.x * 255.0f + 0.5f), (unsigned char)floorf(r.y * 255.0f + 0.5f),
+                             (unsigned char)floorf(r.z * 255.0f + 0.5f), (unsigned char)floorf(r.w * 255.0f + 0.5f));
+    if (rgba32f) rgba32f[pix] = r;
+}
+
+inline unsigned blocks(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+void launch_vertex(const float* vtx, int stride, int nverts, const float* mvp16, float4* clip, hipStream_t s) {
+    if (nverts <= 0) return;
+    Mat4 M;
+    for (int i = 0; i < 16; i++) M.m[i] = mvp16[i];
+    hipLaunchKernelGGL(k_vertex, dim3(blocks(nverts, 256)), dim3(256), 0, s, vtx, stride, nverts, M, clip);
+}
+
+void launch_reset(unsigned* binCount, unsigned* binCursor, int tiles, DeviceCounters* ctr, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset, dim3(blocks(tiles > 0 ? tiles : 1, 256)), dim3(256), 0, s, binCount, binCursor, tiles,
+                       ctr);
+}
+
+void launch_setup(const SetupParams& p, const float4* clip, const int* idx, TriRec* recs, float4* bary, int4* tileBox,
+                  DeviceCounters* ctr, hipStream_t s) {
+    if (p.ntri <= 0) return;
+    hipLaunchKernelGGL(k_setup, dim3(blocks(p.ntri, 256)), dim3(256), 0, s, p, clip, idx, recs, bary, tileBox, ctr);
+}
+
+void launch_bin(const BinParams& p, const TriRec* recs, const int4* tileBox, unsigned* binCount, unsigned* binStart,
+                unsigned* binCursor, unsigned* binList, unsigned* bigList, DeviceCounters* ctr, hipStream_t s) {
+    const size_t slots = (size_t)p.ntri + p.clipCap;
+    const int tiles = p.tilesX * p.tilesY;
+    if (slots > 0) {
+        hipLaunchKernelGGL(k_bin_count, dim3(blocks(slots, 256)), dim3(256), 0, s, p, recs, tileBox, binCount,
+                           bigList, ctr);
+        hipLaunchKernelGGL(k_bin_big_count, dim3(512), dim3(256), 0, s, p, recs, tileBox, binCount, bigList, ctr);
+    }
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, binCount, binStart, tiles, p.binCap, ctr);
+    if (slots > 0) {
+        hipLaunchKernelGGL(k_bin_scatter, dim3(blocks(slots, 256)), dim3(256), 0, s, p, recs, tileBox, binStart,
+                           binCursor, binList, ctr);
+        hipLaunchKernelGGL(k_bin_big_scatter, dim3(512), dim3(256), 0, s, p, recs, tileBox, binStart, binCursor,
+                           binList, bigList, ctr);
+    }
+}
+
+void launch_raster(const RasterParams& p, int samples, const TriRec* recs, const unsigned* binStart,
+                   const unsigned* binCount, const unsigned* binList, uint32_t* visId, float* visDepth,
+                   uint32_t* visSlot, uint8_t* visClass, DeviceCounters* ctr, hipStream_t s) {
+    const dim3 grid(p.tilesX * p.tilesY);
+    switch (samples) {
+        case 1:
+            hipLaunchKernelGGL(k_raster<1>, grid, dim3(256), 0, s, p, recs, binStart, binCount, binList, visId,
+                               visDepth, visSlot, visClass, ctr);
+            break;
+        case 4:
+            hipLaunchKernelGGL(k_raster<4>, grid, dim3(256), 0, s, p, recs, binStart, binCount, binList, visId,
+                               visDepth, visSlot, visClass, ctr);
+            break;
+        default:
+            hipLaunchKernelGGL(k_raster<8>, grid, dim3(256), 0, s, p, recs, binStart, binCount, binList, visId,
+                               visDepth, visSlot, visClass, ctr);
+            break;
+    }
+}
+
+void launch_shade(const ShadeParams& p, const TriRec* recs, const float4* bary, const uint32_t* visSlot,
+                  const float* vtx, const int* idx, const TexDesc* tex, const uint8_t* texPool, uchar4* rgba8,
+                  float4* rgba32f, hipStream_t s) {
+    const dim3 grid(blocks(p.width, 16), blocks(p.height, 16));
+    switch (p.samples) {
+        case 1:
+            hipLaunchKernelGGL(k_shade<1>, grid, dim3(256), 0, s, p, recs, bary, visSlot, vtx, idx, tex, texPool,
+                               rgba8, rgba32f);
+            break;
+        case 4:
+            hipLaunchKernelGGL(k_shade<4>, grid, dim3(256), 0, s, p, recs, bary, visSlot, vtx, idx, tex, texPool,
+                               rgba8, rgba32f);
+            break;
+        default:
+            hipLaunchKernelGGL(k_shade<8>, grid, dim3(256), 0, s, p, recs, bary, visSlot, vtx, idx, tex, texPool,
+                               rgba8, rgba32f);
+            break;
+    }
+}
+
+}  // namespace ifrit_hip

@@ -1,0 +1,256 @@
+"""Python binding of the MI355X tile rasterizer (libifrit_hip.so, C ABI in
+include/ifrit_hip.h).
+
+This is a thin ctypes layer for tests and benchmarks; the renderer itself is
+C++/HIP.  There is no fallback: if the library is missing or no GPU is
+present, loading or drawing raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libifrit_hip.so")
+
+_lib = None
+
+
+class VertexProgramDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("mvp", C.c_float * 16)]
+
+
+class FragmentProgramDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("offWorld", C.c_int32), ("offNormal", C.c_int32), ("offUv", C.c_int32),
+                ("trisPerMaterial", C.c_int32), ("sampler", C.c_int32), ("maxAniso", C.c_int32),
+                ("camera", C.c_float * 3), ("ambient", C.c_float), ("kd", C.c_float), ("ks", C.c_float),
+                ("shininess", C.c_float), ("numLights", C.c_int32), ("lightPos", (C.c_float * 4) * 4),
+                ("lightColor", (C.c_float * 4) * 4)]
+
+
+class FrameStats(C.Structure):
+    _fields_ = [("inputTriangles", C.c_int64), ("assembledTriangles", C.c_int64), ("clippedTriangles", C.c_int64),
+                ("binEntries", C.c_int64), ("fragments", C.c_int64), ("coveredSamples", C.c_int64),
+                ("status", C.c_uint32), ("retries", C.c_uint32), ("stageMs", C.c_float * 8)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "stageMs"}
+        d["stageMs"] = list(self.stageMs)
+        return d
+
+
+# Every symbol declared in include/ifrit_hip.h.
+EXPORTED_SYMBOLS = [
+    "iftrCreateInstance", "iftrDestroyInstance", "iftrBindFrameBuffer", "iftrBindVertexBuffer", "iftrBindIndexBuffer",
+    "iftrBindVertexShaderFunc", "iftrBindFragmentShaderFunc", "iftrSetBlendFunc", "iftrSetDepthFunc",
+    "iftrOptsetForceDeterministic", "iftrOptSetDepthTestEnable", "iftrDrawLegacy", "iftrClear", "iftrInit", "iftrTest",
+    "iftrBindVertexShader", "iftrBindFragmentShader", "iftrCreateFrameBuffer", "iftrDestroyFrameBuffer",
+    "iftrSetFrameBufferColorAttachmentFP32", "iftrSetFrameBufferDepthAttachmentFP32", "iftrCreateVaryingDescriptor",
+    "iftrDestroyVaryingDescriptor", "iftrWriteVaryingDescriptor", "iftrCreateVertexBuffer", "iftrDestroyVertexBuffer",
+    "iftrSetVertexBufferLayout", "iftrSetVertexBufferSize", "iftrAllocateVertexBuffer",
+    "iftrSetVertexBufferValueFloat4", "ifcrCreateImageFP32", "ifcrDestroyImageFP32", "ifcrGetImageRawDataFP32",
+    "ifbufCreateBufferManager", "ifbufDestroyBufferManager", "ifbufCreateBuffer", "ifbufDestroyBuffer",
+    "ifbufBufferData", "iftrCreateGpuVertexProgram", "iftrCreateGpuFragmentProgram", "iftrDestroyGpuProgram",
+    "iftrSetMsaaSamples", "iftrBindVertexDataFloat", "iftrBindIndexData", "iftrCreateTexture",
+    "iftrCopyHostBufferToImage", "iftrUploadTextureLevels", "iftrGenerateFbmTexture", "iftrGenerateMipmap",
+    "iftrDownloadTexture", "iftrSetRenderTarget", "iftrGetVisibility", "iftrGetCoverageClass", "iftrGetColorRGBA8",
+    "iftrGetFrameStats", "iftrSetStageTiming", "iftrDrawAsync", "iftrSynchronize", "iftrGetLastError",
+]
+
+
+def lib():
+    """Loads libifrit_hip.so (raises when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing; build it with `python ifrit-v2_amd/build.py`")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    sig = {
+        "iftrCreateInstance": ([], vp), "iftrDestroyInstance": ([vp], None), "iftrInit": ([vp], None),
+        "iftrBindFrameBuffer": ([vp, vp], None), "iftrBindVertexBuffer": ([vp, vp], None),
+        "iftrBindIndexBuffer": ([vp, vp], None), "iftrBindVertexShaderFunc": ([vp, vp, vp], None),
+        "iftrBindFragmentShaderFunc": ([vp, vp], None), "iftrSetBlendFunc": ([vp, vp], None),
+        "iftrSetDepthFunc": ([vp, C.c_int32], None), "iftrOptsetForceDeterministic": ([vp, C.c_int], None),
+        "iftrOptSetDepthTestEnable": ([vp, C.c_int], None), "iftrDrawLegacy": ([vp, C.c_int, C.c_int], None),
+        "iftrClear": ([vp], None), "iftrBindVertexShader": ([vp, vp], None),
+        "iftrBindFragmentShader": ([vp, vp], None), "iftrCreateFrameBuffer": ([], vp),
+        "iftrDestroyFrameBuffer": ([vp], None), "iftrSetFrameBufferColorAttachmentFP32": ([vp, vp, C.c_size_t], None),
+        "iftrSetFrameBufferDepthAttachmentFP32": ([vp, vp], None), "iftrCreateVaryingDescriptor": ([], vp),
+        "iftrDestroyVaryingDescriptor": ([vp], None), "iftrWriteVaryingDescriptor": ([vp, vp, C.c_size_t], None),
+        "iftrCreateVertexBuffer": ([], vp), "iftrDestroyVertexBuffer": ([vp], None),
+        "iftrSetVertexBufferLayout": ([vp, vp, C.c_size_t], None), "iftrSetVertexBufferSize": ([vp, C.c_size_t], None),
+        "iftrAllocateVertexBuffer": ([vp], None), "iftrSetVertexBufferValueFloat4": ([vp, C.c_int, C.c_int, vp], None),
+        "ifcrCreateImageFP32": ([C.c_size_t, C.c_size_t, C.c_size_t], vp), "ifcrDestroyImageFP32": ([vp], None),
+        "ifcrGetImageRawDataFP32": ([vp], C.POINTER(C.c_float)), "ifbufCreateBufferManager": ([], vp),
+        "ifbufDestroyBufferManager": ([vp], None), "ifbufCreateBuffer": ([vp, C.c_size_t], vp),
+        "ifbufDestroyBuffer": ([vp], None), "ifbufBufferData": ([vp, vp, C.c_size_t, C.c_size_t], None),
+        "iftrCreateGpuVertexProgram": ([vp], vp), "iftrCreateGpuFragmentProgram": ([vp], vp),
+        "iftrDestroyGpuProgram": ([vp], None), "iftrSetMsaaSamples": ([vp, C.c_int], None),
+        "iftrBindVertexDataFloat": ([vp, vp, C.c_int, C.c_int], None), "iftrBindIndexData": ([vp, vp, C.c_int], None),
+        "iftrCreateTexture": ([vp, C.c_int, C.c_int, C.c_int], C.c_int),
+        "iftrCopyHostBufferToImage": ([vp, C.c_int, vp], None), "iftrUploadTextureLevels": ([vp, C.c_int, vp], None),
+        "iftrGenerateFbmTexture": ([vp, C.c_int, C.c_uint32], None), "iftrGenerateMipmap": ([vp, C.c_int], None),
+        "iftrDownloadTexture": ([vp, C.c_int, vp], None), "iftrSetRenderTarget": ([vp, C.c_int, C.c_int], None),
+        "iftrGetVisibility": ([vp, vp, vp], None), "iftrGetCoverageClass": ([vp, vp], None),
+        "iftrGetColorRGBA8": ([vp, vp], None), "iftrGetFrameStats": ([vp, C.POINTER(FrameStats)], None),
+        "iftrSetStageTiming": ([vp, C.c_int], None), "iftrDrawAsync": ([vp, C.c_int, C.c_int], None),
+        "iftrSynchronize": ([vp], None), "iftrGetLastError": ([vp], C.c_char_p),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class IfritError(RuntimeError):
+    pass
+
+
+class Renderer:
+    """One renderer instance (one HIP stream) drawing a Scene."""
+
+    def __init__(self):
+        self.L = lib()
+        self.h = self.L.iftrCreateInstance()
+        self.L.iftrInit(self.h)
+        self._check()
+        self._programs = []
+        self.scene = None
+        self.samples = 1
+
+    def close(self):
+        if self.h:
+            self.L.iftrDestroyInstance(self.h)
+            self.h = None
+        for p in self._programs:
+            self.L.iftrDestroyGpuProgram(p)
+        self._programs = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self):
+        e = self.L.iftrGetLastError(self.h)
+        if e:
+            raise IfritError(e.decode())
+
+    def set_target(self, width: int, height: int, samples: int = 1):
+        self.L.iftrSetRenderTarget(self.h, width, height)
+        self.L.iftrSetMsaaSamples(self.h, samples)
+        self.width, self.height, self.samples = width, height, samples
+        self._check()
+
+    def bind_geometry(self, vertices: np.ndarray, indices: np.ndarray, mvp: np.ndarray):
+        v = np.ascontiguousarray(vertices, np.float32)
+        i = np.ascontiguousarray(indices, np.int32)
+        self.L.iftrBindVertexDataFloat(self.h, _p(v), v.shape[1], v.shape[0])
+        self.L.iftrBindIndexData(self.h, _p(i), i.size)
+        d = VertexProgramDesc()
+        d.kind = 0
+        d.mvp[:] = [float(x) for x in np.asarray(mvp, np.float32).reshape(-1)]
+        p = self.L.iftrCreateGpuVertexProgram(C.byref(d))
+        self._programs.append(p)
+        self.L.iftrBindVertexShader(self.h, p)
+        self.num_indices = int(i.size)
+        self._check()
+
+    def bind_fragment_program(self, scene):
+        d = FragmentProgramDesc()
+        d.kind = scene.program
+        d.offWorld, d.offNormal, d.offUv = 0, 3, 6
+        d.trisPerMaterial = scene.tris_per_material
+        d.sampler, d.maxAniso = scene.sampler, scene.max_aniso
+        d.camera[:] = [float(x) for x in scene.camera]
+        d.ambient, d.kd, d.ks, d.shininess = scene.ambient, scene.kd, scene.ks, scene.shininess
+        d.numLights = len(scene.lights)
+        for k, (lp, lc) in enumerate(scene.lights):
+            d.lightPos[k][:] = list(lp)
+            d.lightColor[k][:] = list(lc)
+        p = self.L.iftrCreateGpuFragmentProgram(C.byref(d))
+        self._programs.append(p)
+        self.L.iftrBindFragmentShader(self.h, p)
+        self._check()
+
+    def add_texture(self, tex) -> int:
+        tid = self.L.iftrCreateTexture(self.h, tex.width, tex.height, tex.levels)
+        self._check()
+        if tex.texels is not None:
+            arr = np.ascontiguousarray(tex.texels, np.uint8)
+            self.L.iftrUploadTextureLevels(self.h, tid, _p(arr))
+        else:
+            self.L.iftrGenerateFbmTexture(self.h, tid, tex.fbm_seed)
+            self.L.iftrGenerateMipmap(self.h, tid)
+        self._check()
+        return tid
+
+    def download_texture(self, tid: int, tex) -> np.ndarray:
+        n = sum(max(1, tex.width >> l) * max(1, tex.height >> l) for l in range(tex.levels)) * 4
+        out = np.empty((n,), np.uint8)
+        self.L.iftrDownloadTexture(self.h, tid, _p(out))
+        self._check()
+        return out
+
+    def load_scene(self, scene):
+        self.scene = scene
+        self.set_target(scene.width, scene.height, scene.samples)
+        self.bind_geometry(scene.vertices, scene.indices, scene.mvp)
+        for t in scene.textures:
+            self.add_texture(t)
+        self.bind_fragment_program(scene)
+
+    def draw(self, clear: bool = True):
+        self.L.iftrDrawLegacy(self.h, self.num_indices, 1 if clear else 0)
+        self._check()
+        st = self.stats()
+        if st["status"]:
+            raise IfritError(f"frame incomplete, status bits {st['status']:#x}")
+        return st
+
+    def draw_async(self, clear: bool = True):
+        self.L.iftrDrawAsync(self.h, self.num_indices, 1 if clear else 0)
+
+    def synchronize(self):
+        self.L.iftrSynchronize(self.h)
+        self._check()
+
+    def stage_timing(self, on: bool):
+        self.L.iftrSetStageTiming(self.h, 1 if on else 0)
+
+    def stats(self) -> dict:
+        s = FrameStats()
+        self.L.iftrGetFrameStats(self.h, C.byref(s))
+        return s.as_dict()
+
+    def visibility(self):
+        n = self.width * self.height * self.samples
+        ids = np.empty((n,), np.uint32)
+        depth = np.empty((n,), np.float32)
+        self.L.iftrGetVisibility(self.h, _p(ids), _p(depth))
+        self._check()
+        return ids, depth
+
+    def coverage_class(self):
+        out = np.empty((self.width * self.height,), np.uint8)
+        self.L.iftrGetCoverageClass(self.h, _p(out))
+        self._check()
+        return out
+
+    def color(self):
+        out = np.empty((self.height, self.width, 4), np.uint8)
+        self.L.iftrGetColorRGBA8(self.h, _p(out))
+        self._check()
+        return out

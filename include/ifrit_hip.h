@@ -1,0 +1,211 @@
+/*
+ * C ABI of the MI355X tile rasterizer.
+ *
+ * The first block keeps the symbols, signatures and argument meaning of the
+ * reference's exported C ABI, so that callers written against it (the C#
+ * P/Invoke bindings in bindings/csharp/IfritLib/Native/NativeMethods*.cs and
+ * any C caller) bind unchanged.  Handles the reference passes as C++ object
+ * pointers (TileRasterRendererWrapper*, FrameBuffer*, ImageF32*, ...) are
+ * opaque pointers here.  As in the reference, these functions return void
+ * and do not throw; errors that the reference would abort on are recorded
+ * and can be read with iftrGetLastError.
+ *
+ * The second block adds what the GPU backend needs on top: sample counts,
+ * textures, built-in GPU shader programs, visibility-buffer readback and
+ * frame statistics.
+ */
+#ifndef IFRIT_HIP_H
+#define IFRIT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct IfritRendererHandle IfritRendererHandle;       /* TileRasterRendererWrapper* */
+typedef struct IfritFrameBuffer IfritFrameBuffer;             /* FrameBuffer* */
+typedef struct IfritImageF32 IfritImageF32;                   /* Core::Data::ImageF32* */
+typedef struct IfritVertexBuffer IfritVertexBuffer;           /* VertexBuffer* */
+typedef struct IfritVaryingDescriptor IfritVaryingDescriptor; /* VaryingDescriptor* */
+
+typedef struct {
+    float x, y, z, w;
+} IfritVector4f;
+
+/* include/ifrit/softgraphics/engine/base/Shaders.h:116-118 */
+typedef void (*IfritVertexShaderFunctionalPtr)(const void* const* input, IfritVector4f* outPos,
+                                               IfritVector4f* const* outVaryings);
+typedef void (*IfritFragmentShaderFunctionalPtr)(const void* varyings, void* colorOutput, float* fragmentDepth);
+
+/* include/ifrit/softgraphics/engine/base/Structures.h:90-97 */
+typedef struct {
+    uint8_t blendEnable;
+    int32_t srcColorBlendFactor;
+    int32_t dstColorBlendFactor;
+    int32_t srcAlphaBlendFactor;
+    int32_t dstAlphaBlendFactor;
+} IfritColorAttachmentBlendState;
+
+/* IfritCompareOp, include/ifrit/softgraphics/engine/base/Constants.h:77-87 */
+enum {
+    IF_COMPARE_OP_NEVER = 0,
+    IF_COMPARE_OP_LESS = 1,
+    IF_COMPARE_OP_EQUAL = 2,
+    IF_COMPARE_OP_LESS_OR_EQUAL = 3,
+    IF_COMPARE_OP_GREATER = 4,
+    IF_COMPARE_OP_NOT_EQUAL = 5,
+    IF_COMPARE_OP_GREATER_OR_EQUAL = 6,
+    IF_COMPARE_OP_ALWAYS = 7
+};
+
+/* TypeDescriptorEnum, include/ifrit/softgraphics/engine/base/TypeDescriptor.h:25-39 */
+enum {
+    IFTP_UNDEFINED = 0,
+    IFTP_FLOAT1,
+    IFTP_FLOAT2,
+    IFTP_FLOAT3,
+    IFTP_FLOAT4,
+    IFTP_INT1,
+    IFTP_INT2,
+    IFTP_INT3,
+    IFTP_INT4,
+    IFTP_UINT1,
+    IFTP_UINT2,
+    IFTP_UINT3,
+    IFTP_UINT4
+};
+
+/* ---- Renderer: include/ifrit/softgraphics/engine/export/TileRasterRendererExport.h:33-76 ---- */
+IfritRendererHandle* iftrCreateInstance(void);                                      /* EXH:33 */
+void iftrDestroyInstance(IfritRendererHandle* h);                                   /* EXH:35 */
+void iftrBindFrameBuffer(IfritRendererHandle* h, IfritFrameBuffer* frameBuffer);    /* EXH:36-38 */
+void iftrBindVertexBuffer(IfritRendererHandle* h, const IfritVertexBuffer* vb);     /* EXH:39-41 */
+void iftrBindIndexBuffer(IfritRendererHandle* h, void* indexBuffer);                /* EXH:42-43, an ifbufCreateBuffer handle */
+void iftrBindVertexShaderFunc(IfritRendererHandle* h, IfritVertexShaderFunctionalPtr func,
+                              IfritVaryingDescriptor* vsOutDescriptors);             /* EXH:44-47 */
+void iftrBindFragmentShaderFunc(IfritRendererHandle* h, IfritFragmentShaderFunctionalPtr func); /* EXH:48-50 */
+void iftrSetBlendFunc(IfritRendererHandle* h, IfritColorAttachmentBlendState* state); /* EXH:51-53 */
+void iftrSetDepthFunc(IfritRendererHandle* h, int32_t state);                       /* EXH:54-56 */
+void iftrOptsetForceDeterministic(IfritRendererHandle* h, int opt);                 /* EXH:57-58 */
+void iftrOptSetDepthTestEnable(IfritRendererHandle* h, int opt);                    /* EXH:59-60 */
+void iftrDrawLegacy(IfritRendererHandle* h, int numVertices, int clearFramebuffer); /* EXH:61-63 */
+void iftrClear(IfritRendererHandle* h);                                             /* EXH:64-65 */
+void iftrInit(IfritRendererHandle* h);                                              /* EXH:66-67 */
+void iftrTest(void (*p)(int));                                                      /* EXH:68-69 */
+void iftrBindVertexShader(IfritRendererHandle* h, void* pVertexShader);             /* EXH:72-73 */
+void iftrBindFragmentShader(IfritRendererHandle* h, void* pFragmentShader);         /* EXH:74-75 */
+
+/* ---- Frame buffers, varyings, vertex buffers: include/ifrit/softgraphics/engine/export/EngineCommonExport.h:33-70 ---- */
+IfritFrameBuffer* iftrCreateFrameBuffer(void);
+void iftrDestroyFrameBuffer(IfritFrameBuffer* p);
+void iftrSetFrameBufferColorAttachmentFP32(IfritFrameBuffer* p, IfritImageF32* const* pImage, size_t nums);
+void iftrSetFrameBufferDepthAttachmentFP32(IfritFrameBuffer* p, IfritImageF32* pImage);
+IfritVaryingDescriptor* iftrCreateVaryingDescriptor(void);
+void iftrDestroyVaryingDescriptor(IfritVaryingDescriptor* p);
+void iftrWriteVaryingDescriptor(IfritVaryingDescriptor* p, const int* pDesc, size_t num);
+IfritVertexBuffer* iftrCreateVertexBuffer(void);
+void iftrDestroyVertexBuffer(IfritVertexBuffer* p);
+void iftrSetVertexBufferLayout(IfritVertexBuffer* p, const int* pDesc, size_t num);
+void iftrSetVertexBufferSize(IfritVertexBuffer* p, size_t num);
+void iftrAllocateVertexBuffer(IfritVertexBuffer* p);
+void iftrSetVertexBufferValueFloat4(IfritVertexBuffer* p, int index, int attr, void* value);
+
+/* ---- Images: include/ifrit/softgraphics/core/export/CommonExport.h:26-31 ---- */
+IfritImageF32* ifcrCreateImageFP32(size_t width, size_t height, size_t channel);
+void ifcrDestroyImageFP32(IfritImageF32* p);
+float* ifcrGetImageRawDataFP32(IfritImageF32* p);
+
+/* ---- Buffers: include/ifrit/softgraphics/engine/export/BufferManagerExport.h:25-35 ---- */
+void* ifbufCreateBufferManager(void);
+void ifbufDestroyBufferManager(void* p);
+void* ifbufCreateBuffer(void* pManager, size_t bufSize);
+void ifbufDestroyBuffer(void* p);
+void ifbufBufferData(void* pBuffer, const void* pData, size_t offset, size_t size);
+
+/* ======================= MI355X backend additions ======================= */
+
+/* Built-in GPU programs (host function pointers cannot run on the GPU). */
+enum {
+    IFHIP_VS_TRANSFORM = 0 /* clip = mvp * (attribute 0, 1); varyings pass attributes through */
+};
+enum {
+    IFHIP_FS_FLAT = 0,        /* colour = bytes of a hash of the primitive id */
+    IFHIP_FS_BLINN_PHONG = 1, /* textured Blinn-Phong, directional and point lights */
+    IFHIP_FS_DEPTH_ONLY = 2   /* visibility and depth only */
+};
+enum { IFHIP_SAMPLER_BILINEAR = 0, IFHIP_SAMPLER_TRILINEAR = 1, IFHIP_SAMPLER_ANISOTROPIC = 2 };
+
+typedef struct {
+    int32_t kind;     /* IFHIP_VS_TRANSFORM */
+    float mvp[16];    /* row-major, applied as mvp * (x, y, z, 1) */
+} IfritGpuVertexProgramDesc;
+
+typedef struct {
+    int32_t kind;               /* IFHIP_FS_* */
+    int32_t offWorld, offNormal, offUv; /* float offsets in the vertex record */
+    int32_t trisPerMaterial;    /* texture index = primitive / trisPerMaterial (0: texture 0) */
+    int32_t sampler;            /* IFHIP_SAMPLER_* */
+    int32_t maxAniso;
+    float camera[3];
+    float ambient, kd, ks, shininess;
+    int32_t numLights;          /* <= 4 */
+    float lightPos[4][4];       /* w = 0: direction towards the light, w = 1: point light */
+    float lightColor[4][4];     /* rgb, attenuation k of 1 / (1 + k d^2) */
+} IfritGpuFragmentProgramDesc;
+
+void* iftrCreateGpuVertexProgram(const IfritGpuVertexProgramDesc* desc);
+void* iftrCreateGpuFragmentProgram(const IfritGpuFragmentProgramDesc* desc);
+void iftrDestroyGpuProgram(void* program);
+
+/* setMsaaSamples of the CUDA renderer (TileRasterRendererCuda.h:105): 1, 4 or 8. */
+void iftrSetMsaaSamples(IfritRendererHandle* h, int samples);
+/* Attribute-0 stride used by the built-in vertex program when the bound vertex buffer is
+   given as raw floats (iftrBindVertexDataFloat). */
+void iftrBindVertexDataFloat(IfritRendererHandle* h, const float* data, int strideFloats, int numVertices);
+void iftrBindIndexData(IfritRendererHandle* h, const int32_t* data, int numIndices);
+
+/* Textures (CreateTexture / copyHostBufferToImage / generateMipmap of the CUDA renderer,
+   TileRasterRendererCuda.h:90-96), RGBA8 with a full mip chain, power-of-two sizes. */
+int iftrCreateTexture(IfritRendererHandle* h, int width, int height, int levels);
+void iftrCopyHostBufferToImage(IfritRendererHandle* h, int texture, const void* rgba8Level0);
+void iftrUploadTextureLevels(IfritRendererHandle* h, int texture, const void* rgba8AllLevels);
+void iftrGenerateFbmTexture(IfritRendererHandle* h, int texture, uint32_t seed);
+void iftrGenerateMipmap(IfritRendererHandle* h, int texture);
+void iftrDownloadTexture(IfritRendererHandle* h, int texture, void* rgba8AllLevels);
+
+/* Render target size without host attachments (device-resident frames). */
+void iftrSetRenderTarget(IfritRendererHandle* h, int width, int height);
+
+/* Frame outputs (device to host copies). */
+void iftrGetVisibility(IfritRendererHandle* h, uint32_t* primIds, float* depth); /* W*H*S each */
+void iftrGetCoverageClass(IfritRendererHandle* h, uint8_t* classes);            /* W*H, 1 sample */
+void iftrGetColorRGBA8(IfritRendererHandle* h, uint8_t* rgba);                  /* W*H*4 */
+
+typedef struct {
+    int64_t inputTriangles;
+    int64_t assembledTriangles;
+    int64_t clippedTriangles;
+    int64_t binEntries;
+    int64_t fragments;
+    int64_t coveredSamples;
+    uint32_t status;       /* overflow bits, 0 when the frame is complete */
+    uint32_t retries;      /* frames re-run with larger buffers */
+    float stageMs[8];      /* vertex, reset, setup, bin, raster, shade, (unused) x2 */
+} IfritFrameStats;
+
+void iftrGetFrameStats(IfritRendererHandle* h, IfritFrameStats* out);
+/* Per-stage timing with device events (adds event records between stages). */
+void iftrSetStageTiming(IfritRendererHandle* h, int enable);
+/* Draw without waiting for completion, then wait; for timing loops. */
+void iftrDrawAsync(IfritRendererHandle* h, int numVertices, int clearFramebuffer);
+void iftrSynchronize(IfritRendererHandle* h);
+/* NULL when no error has been recorded since the last call. */
+const char* iftrGetLastError(IfritRendererHandle* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IFRIT_HIP_H */
