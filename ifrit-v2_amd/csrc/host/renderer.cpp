@@ -334,6 +334,10 @@ bool TileRasterRenderer::prepare(int indexCount) {
         setError("no vertex program bound");
         return false;
     }
+    if (!hostVs_ && vs_.desc.kind == IFHIP_VS_CLIP_PASSTHROUGH && stride_ < 4) {
+        setError("the clip pass-through vertex program needs 4 floats per vertex");
+        return false;
+    }
     const int ntri = indexCount / 3;
     curTris_ = ntri;
     const int tiles = ((width_ + kTile - 1) / kTile) * ((height_ + kTile - 1) / kTile);
@@ -381,7 +385,9 @@ void TileRasterRenderer::enqueue(bool clear) {
         if (stageTiming_) (void)hipEventRecord(ev_[i], s);
     };
     mark(0);
-    if (!hostVs_) launch_vertex(vtx_.as<float>(), stride_, nverts_, vs_.desc.mvp, clip_.as<float4>(), s);
+    if (!hostVs_)
+        launch_vertex(vtx_.as<float>(), stride_, nverts_, vs_.desc.mvp, vs_.desc.kind == IFHIP_VS_CLIP_PASSTHROUGH ? 1 : 0,
+                      clip_.as<float4>(), s);
     mark(1);
     launch_reset(binCount_.as<unsigned>(), binCursor_.as<unsigned>(), tiles, ctr_.as<DeviceCounters>(), s);
     mark(2);

@@ -95,7 +95,8 @@ struct DeviceCounters {
 };
 
 // Launch wrappers (pipeline.hip).
-void launch_vertex(const float* vtx, int stride, int nverts, const float* mvp16, float4* clip, hipStream_t s);
+void launch_vertex(const float* vtx, int stride, int nverts, const float* mvp16, int passthrough, float4* clip,
+                   hipStream_t s);
 void launch_reset(unsigned* binCount, unsigned* binCursor, int tiles, DeviceCounters* ctr, hipStream_t s);
 void launch_setup(const SetupParams& p, const float4* clip, const int* idx, TriRec* recs, float4* bary, int4* tileBox,
                   DeviceCounters* ctr, hipStream_t s);

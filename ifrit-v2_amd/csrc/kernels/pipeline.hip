@@ -28,10 +28,14 @@ struct Mat4 {
 // Vertex stage: clip = M * (p, 1), MatMul summation order (LinalgOps.h:40-48).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_vertex(const float* __restrict__ vtx, int stride, int n, Mat4 M,
-                                                float4* __restrict__ clip) {
+                                                int passthrough, float4* __restrict__ clip) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float* p = vtx + (size_t)i * stride;
+    if (passthrough) {  // attribute 0 already holds the clip position
+        clip[i] = make_float4(p[0], p[1], p[2], p[3]);
+        return;
+    }
     const float x = p[0], y = p[1], z = p[2];
     float4 r;
     r.x = M.m[0] * x + M.m[1] * y + M.m[2] * z + M.m[3] * 1.0f;
@@ -836,11 +840,13 @@ inline unsigned blocks(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b
 
 }  // namespace
 
-void launch_vertex(const float* vtx, int stride, int nverts, const float* mvp16, float4* clip, hipStream_t s) {
+void launch_vertex(const float* vtx, int stride, int nverts, const float* mvp16, int passthrough, float4* clip,
+                   hipStream_t s) {
     if (nverts <= 0) return;
     Mat4 M;
     for (int i = 0; i < 16; i++) M.m[i] = mvp16[i];
-    hipLaunchKernelGGL(k_vertex, dim3(blocks(nverts, 256)), dim3(256), 0, s, vtx, stride, nverts, M, clip);
+    hipLaunchKernelGGL(k_vertex, dim3(blocks(nverts, 256)), dim3(256), 0, s, vtx, stride, nverts, M, passthrough,
+                       clip);
 }
 
 void launch_reset(unsigned* binCount, unsigned* binCursor, int tiles, DeviceCounters* ctr, hipStream_t s) {

@@ -154,13 +154,13 @@ class Renderer:
         self.width, self.height, self.samples = width, height, samples
         self._check()
 
-    def bind_geometry(self, vertices: np.ndarray, indices: np.ndarray, mvp: np.ndarray):
+    def bind_geometry(self, vertices: np.ndarray, indices: np.ndarray, mvp: np.ndarray, clip_passthrough=False):
         v = np.ascontiguousarray(vertices, np.float32)
         i = np.ascontiguousarray(indices, np.int32)
         self.L.iftrBindVertexDataFloat(self.h, _p(v), v.shape[1], v.shape[0])
         self.L.iftrBindIndexData(self.h, _p(i), i.size)
         d = VertexProgramDesc()
-        d.kind = 0
+        d.kind = 1 if clip_passthrough else 0
         d.mvp[:] = [float(x) for x in np.asarray(mvp, np.float32).reshape(-1)]
         p = self.L.iftrCreateGpuVertexProgram(C.byref(d))
         self._programs.append(p)

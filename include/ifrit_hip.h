@@ -128,7 +128,8 @@ void ifbufBufferData(void* pBuffer, const void* pData, size_t offset, size_t siz
 
 /* Built-in GPU programs (host function pointers cannot run on the GPU). */
 enum {
-    IFHIP_VS_TRANSFORM = 0 /* clip = mvp * (attribute 0, 1); varyings pass attributes through */
+    IFHIP_VS_TRANSFORM = 0,       /* clip = mvp * (attribute 0 xyz, 1) */
+    IFHIP_VS_CLIP_PASSTHROUGH = 1 /* clip = attribute 0 (xyzw), for pre-transformed vertices */
 };
 enum {
     IFHIP_FS_FLAT = 0,        /* colour = bytes of a hash of the primitive id */
