@@ -20,7 +20,10 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("IFRIT_HIP_ARCH", "gfx950")
 
 SOURCES = [
-    "kernels/pipeline.hip",
+    "kernels/geometry.hip",
+    "kernels/binning.hip",
+    "kernels/raster.hip",
+    "kernels/shade.hip",
     "kernels/texture.hip",
     "host/renderer.cpp",
     "host/capi.cpp",

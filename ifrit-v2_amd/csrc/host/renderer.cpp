@@ -48,7 +48,7 @@ TileRasterRenderer::TileRasterRenderer() {}
 
 TileRasterRenderer::~TileRasterRenderer() {
     if (stream_) (void)hipStreamSynchronize(stream_);
-    for (DevBuf* b : {&vtx_, &idx_, &clip_, &recs_, &bary_, &tileBox_, &binCount_, &binStart_, &binCursor_, &binList_,
+    for (DevBuf* b : {&vtx_, &idx_, &clip_, &recs_[0], &recs_[1], &recs_[2], &recs_[3], &recs_[4], &recs_[5], &bary_, &tileBox_, &slotOf_, &binCount_, &binStart_, &binCursor_, &binList_,
                       &bigList_, &ctr_, &visId_, &visDepth_, &visSlot_, &visClass_, &rgba8_, &rgba32f_, &texDesc_})
         b->release();
     for (auto& t : textures_)
@@ -347,8 +347,11 @@ bool TileRasterRenderer::prepare(int indexCount) {
     binCap_ = std::max<unsigned>(binCap_, std::max(1u << 20, (unsigned)std::min<size_t>(0xFFFFFFF0ull, (size_t)ntri * 6)));
     bigCap_ = std::max<unsigned>(bigCap_, std::max(1024u, (unsigned)(ntri / 16)));
     const size_t slots = (size_t)ntri + clipCap_;
-    bool okAll = clip_.ensure((size_t)nverts_ * 16) && recs_.ensure(slots * sizeof(TriRec)) &&
-                 bary_.ensure(slots * 32) && tileBox_.ensure(slots * 16) && binCount_.ensure((size_t)tiles * 4) &&
+    bool okAll = clip_.ensure((size_t)nverts_ * 16) && recs_[0].ensure(slots * 16) && recs_[1].ensure(slots * 16) &&
+                 recs_[2].ensure(slots * 16) && recs_[3].ensure(slots * 16) && recs_[4].ensure(slots * 16) &&
+                 recs_[5].ensure(slots * 16) &&
+                 bary_.ensure(slots * 32) && tileBox_.ensure(slots * 16) &&
+                 slotOf_.ensure((size_t)std::max(1, ntri) * 4) && binCount_.ensure((size_t)tiles * 4) &&
                  binStart_.ensure((size_t)tiles * 4) && binCursor_.ensure((size_t)tiles * 4) &&
                  binList_.ensure((size_t)binCap_ * 4) && bigList_.ensure((size_t)bigCap_ * 4) &&
                  visId_.ensure(ps * 4) && visDepth_.ensure(ps * 4) && visSlot_.ensure(ps * 4) &&
@@ -377,6 +380,11 @@ bool TileRasterRenderer::prepare(int indexCount) {
     return true;
 }
 
+TriRecs TileRasterRenderer::recPtrs() const {
+    return TriRecs{recs_[0].as<float4>(), recs_[1].as<float4>(), recs_[2].as<float4>(),
+                   recs_[3].as<float4>(), recs_[4].as<float4>(), recs_[5].as<float4>()};
+}
+
 void TileRasterRenderer::enqueue(bool clear) {
     const int tilesX = (width_ + kTile - 1) / kTile, tilesY = (height_ + kTile - 1) / kTile;
     const int tiles = tilesX * tilesY;
@@ -392,11 +400,11 @@ void TileRasterRenderer::enqueue(bool clear) {
     launch_reset(binCount_.as<unsigned>(), binCursor_.as<unsigned>(), tiles, ctr_.as<DeviceCounters>(), s);
     mark(2);
     SetupParams sp{width_, height_, tilesX, tilesY, curTris_, nverts_, 0, samples_ > 1 ? 1 : 0, clipCap_};
-    launch_setup(sp, clip_.as<float4>(), idx_.as<int>(), recs_.as<TriRec>(), bary_.as<float4>(), tileBox_.as<int4>(),
-                 ctr_.as<DeviceCounters>(), s);
+    launch_setup(sp, clip_.as<float4>(), idx_.as<int>(), recPtrs(), bary_.as<float4>(), tileBox_.as<int4>(),
+                 slotOf_.as<unsigned>(), ctr_.as<DeviceCounters>(), s);
     mark(3);
     BinParams bp{tilesX, tilesY, curTris_, clipCap_, binCap_, bigCap_, kBigThreshold};
-    launch_bin(bp, recs_.as<TriRec>(), tileBox_.as<int4>(), binCount_.as<unsigned>(), binStart_.as<unsigned>(),
+    launch_bin(bp, recPtrs(), tileBox_.as<int4>(), binCount_.as<unsigned>(), binStart_.as<unsigned>(),
                binCursor_.as<unsigned>(), binList_.as<unsigned>(), bigList_.as<unsigned>(), ctr_.as<DeviceCounters>(),
                s);
     mark(4);
@@ -413,7 +421,8 @@ void TileRasterRenderer::enqueue(bool clear) {
         rp.sx[i] = st[2 * i];
         rp.sy[i] = st[2 * i + 1];
     }
-    launch_raster(rp, samples_, recs_.as<TriRec>(), binStart_.as<unsigned>(), binCount_.as<unsigned>(),
+    launch_raster(rp, samples_, recPtrs(), slotOf_.as<unsigned>(), binStart_.as<unsigned>(),
+                  binCount_.as<unsigned>(),
                   binList_.as<unsigned>(), visId_.as<uint32_t>(), visDepth_.as<float>(), visSlot_.as<uint32_t>(),
                   visClass_.as<uint8_t>(), ctr_.as<DeviceCounters>(), s);
     mark(5);
@@ -442,7 +451,8 @@ void TileRasterRenderer::enqueue(bool clear) {
         std::memcpy(sh.lightColor, d.lightColor, sizeof(sh.lightColor));
         const bool wantFloat = fb_ && !fb_->color.empty() && fb_->color[0];
         sh.writeFloat = wantFloat ? 1 : 0;
-        launch_shade(sh, recs_.as<TriRec>(), bary_.as<float4>(), visSlot_.as<uint32_t>(), vtx_.as<float>(),
+        sh.ntri = curTris_;
+        launch_shade(sh, recPtrs(), bary_.as<float4>(), visSlot_.as<uint32_t>(), vtx_.as<float>(),
                      idx_.as<int>(), texDesc_.as<TexDesc>(), nullptr, rgba8_.as<uchar4>(),
                      wantFloat ? rgba32f_.as<float4>() : nullptr, s);
     }

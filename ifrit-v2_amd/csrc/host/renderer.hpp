@@ -159,10 +159,11 @@ private:
     bool vsBound_ = false, fsBound_ = false;
 
     // device buffers
-    DevBuf vtx_, idx_, clip_, recs_, bary_, tileBox_, binCount_, binStart_, binCursor_, binList_, bigList_, ctr_;
+    DevBuf vtx_, idx_, clip_, recs_[6], bary_, tileBox_, slotOf_, binCount_, binStart_, binCursor_, binList_, bigList_, ctr_;
     DevBuf visId_, visDepth_, visSlot_, visClass_, rgba8_, rgba32f_, texDesc_;
     unsigned clipCap_ = 0, binCap_ = 0, bigCap_ = 0;
     int curTris_ = 0;
+    TriRecs recPtrs() const;
     std::vector<TextureSlot> textures_;
     bool texDescDirty_ = true;
 

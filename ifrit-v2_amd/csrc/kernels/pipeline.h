@@ -2,10 +2,11 @@
 //
 // HBM layout per draw (V vertices, T input triangles, W x H pixels, S samples):
 //   clip        float4[V]                 vertex stage output
-//   recs        TriRec[T + clipCap]       assembled triangles, slot = primitive for
+//   recs        TriRecs, T + clipCap slots assembled triangles, slot = primitive for
 //                                         unclipped triangles, T + k for clip fans
-//   bary        float4[2 * (T+clipCap)]   vertex-weight bases bx, by (shading only)
+//   bary        float4[2 * (T+clipCap)]   vertex-weight bases bx, by of clip fans (shading only)
 //   tileBox     int4[T + clipCap]         tile rectangle of each slot (x0 > x1: empty)
+//   slotOf      uint[T]                   first slot of each primitive (its fans follow)
 //   binCount/binStart/binCursor  uint[tiles]
 //   binList     uint[binCap]              slot | 0x80000000 when the tile is fully covered
 //   visId/visDepth/visSlot  [W*H*S]       per-sample winner, pixels row-major, samples innermost
@@ -16,14 +17,24 @@
 
 namespace ifrit_hip {
 
-// One assembled triangle for coverage and depth (96 bytes, 16-byte aligned):
-//   q0 = e1.xyz, e2.x   q1 = e2.yz, e3.xy   q2 = e3.z, key, unused, unused
+// One assembled triangle for coverage and depth (6 x 16 bytes):
+//   q0 = e1.xyz, e2.x   q1 = e2.yz, e3.xy
+//   q2 = e3.z, key, pixel box (x0 | y0 << 16), (x1 | y1 << 16 | big << 31)
 //   f1, f2, f3 = interpolation bases (x, y, z) and vz = (z/w) * w.
 // key = prim * 4 + fan orders fragments the way the reference's per-tile
-// sort by originalPrimitive does (TileRasterWorker.cpp:1244-1258).
+// sort by originalPrimitive does (TileRasterWorker.cpp:1244-1258).  The
+// pixel box bounds every pixel the coverage tests can accept (the edge
+// functions' EPS bias grows a triangle slightly); triangles flagged big are
+// rasterised one pixel per lane.
 struct __align__(16) TriRec {
     float4 q0, q1, q2;
     float4 f1, f2, f3;
+};
+
+// Records are stored as six float4 streams (structure of arrays) so that a
+// wave's stores and the tile loader's gathers are 16-byte accesses.
+struct TriRecs {
+    float4 *q0, *q1, *q2, *f1, *f2, *f3;
 };
 
 struct SetupParams {
@@ -72,6 +83,7 @@ struct ShadeParams {
     float lightPos[4][4];
     float lightColor[4][4];
     int writeFloat;          // also write RGBA32F
+    int ntri;                // slots >= ntri are clip fans with their own bx, by
 };
 
 // Status word bits raised by the kernels; checked by the host after a frame.
@@ -98,14 +110,14 @@ struct DeviceCounters {
 void launch_vertex(const float* vtx, int stride, int nverts, const float* mvp16, int passthrough, float4* clip,
                    hipStream_t s);
 void launch_reset(unsigned* binCount, unsigned* binCursor, int tiles, DeviceCounters* ctr, hipStream_t s);
-void launch_setup(const SetupParams& p, const float4* clip, const int* idx, TriRec* recs, float4* bary, int4* tileBox,
-                  DeviceCounters* ctr, hipStream_t s);
-void launch_bin(const BinParams& p, const TriRec* recs, const int4* tileBox, unsigned* binCount, unsigned* binStart,
+void launch_setup(const SetupParams& p, const float4* clip, const int* idx, TriRecs recs, float4* bary, int4* tileBox,
+                  unsigned* slotOf, DeviceCounters* ctr, hipStream_t s);
+void launch_bin(const BinParams& p, TriRecs recs, const int4* tileBox, unsigned* binCount, unsigned* binStart,
                 unsigned* binCursor, unsigned* binList, unsigned* bigList, DeviceCounters* ctr, hipStream_t s);
-void launch_raster(const RasterParams& p, int samples, const TriRec* recs, const unsigned* binStart,
-                   const unsigned* binCount, const unsigned* binList, uint32_t* visId, float* visDepth,
-                   uint32_t* visSlot, uint8_t* visClass, DeviceCounters* ctr, hipStream_t s);
-void launch_shade(const ShadeParams& p, const TriRec* recs, const float4* bary, const uint32_t* visSlot,
+void launch_raster(const RasterParams& p, int samples, TriRecs recs, const unsigned* slotOf,
+                   const unsigned* binStart, const unsigned* binCount, const unsigned* binList, uint32_t* visId,
+                   float* visDepth, uint32_t* visSlot, uint8_t* visClass, DeviceCounters* ctr, hipStream_t s);
+void launch_shade(const ShadeParams& p, TriRecs recs, const float4* bary, const uint32_t* visSlot,
                   const float* vtx, const int* idx, const TexDesc* tex, const uint8_t* texPool, uchar4* rgba8,
                   float4* rgba32f, hipStream_t s);
 

@@ -1,0 +1,253 @@
+// Deferred shading and MSAA resolve on gfx950: one thread per pixel; each
+// distinct triangle among the pixel's samples is shaded once at the pixel's
+// shading position (the integer pixel position at 1 sample, the pixel centre
+// with MSAA), weighted by its sample count, clamped and box-resolved to RGBA8.
+// Texture sampling follows the CUDA path's sampler spec
+// (include/ifrit/softgraphics/engine/math/ShaderBuiltinCuda.cuh:36-295) with
+// texel centres at (i + 0.5) / size, footprints |d(uv)/dx| and |d(uv)/dy|,
+// and symmetric anisotropic taps; derivatives are fine quad differences of
+// the same triangle evaluated at the quad's other pixels.
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+
+namespace ifrit_hip {
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Deferred shading and resolve
+// ---------------------------------------------------------------------------
+struct ShadeTri {
+    float4 f1, f2, f3;
+    float bx[3], by[3];
+    int prim;
+};
+
+__device__ __forceinline__ void load_shade_tri(const TriRecs& recs, const float4* bary, int ntri, unsigned slot,
+                                               ShadeTri& T) {
+    T.f1 = recs.f1[slot];
+    T.f2 = recs.f2[slot];
+    T.f3 = recs.f3[slot];
+    T.prim = (int)(__float_as_uint(recs.q2[slot].y) >> 2);
+    if (slot < (unsigned)ntri) {
+        // unclipped triangles keep bx = (0, 0, 1), by = (1, 0, 0) (TileRasterWorker.cpp:711-712)
+        T.bx[0] = 0.0f; T.bx[1] = 0.0f; T.bx[2] = 1.0f;
+        T.by[0] = 1.0f; T.by[1] = 0.0f; T.by[2] = 0.0f;
+    } else {
+        const float4 a = bary[2 * slot], b = bary[2 * slot + 1];
+        T.bx[0] = a.x; T.bx[1] = a.y; T.bx[2] = a.z;
+        T.by[0] = a.w; T.by[1] = b.x; T.by[2] = b.y;
+    }
+}
+
+// Perspective-correct weights of the three original vertices at (x, y).
+__device__ __forceinline__ void vertex_weights(const ShadeTri& T, float x, float y, float w[3]) {
+    const float b1 = fmaf(T.f1.x, x, fmaf(T.f1.y, y, T.f1.z));
+    const float b2 = fmaf(T.f2.x, x, fmaf(T.f2.y, y, T.f2.z));
+    const float b3 = fmaf(T.f3.x, x, fmaf(T.f3.y, y, T.f3.z));
+    const float inv = 1.0f / (b1 + b2 + b3);
+    const float B1 = b1 * inv, B2 = b2 * inv, B3 = b3 * inv;
+    w[0] = fmaf(B1, T.bx[0], fmaf(B2, T.bx[1], B3 * T.bx[2]));
+    w[1] = fmaf(B1, T.by[0], fmaf(B2, T.by[1], B3 * T.by[2]));
+    w[2] = 1.0f - w[0] - w[1];
+}
+
+__device__ __forceinline__ float4 fetch_texel(const uint8_t* pool, const TexDesc& D, int level, int x, int y) {
+    const int w = max(1, D.width >> level), h = max(1, D.height >> level);
+    x &= (w - 1);  // power-of-two sizes, repeat addressing
+    y &= (h - 1);
+    const uchar4 c = reinterpret_cast<const uchar4*>(pool + D.base)[D.levelOffset[level] + (unsigned)y * w + x];
+    return make_float4(c.x * (1.0f / 255.0f), c.y * (1.0f / 255.0f), c.z * (1.0f / 255.0f), c.w * (1.0f / 255.0f));
+}
+
+__device__ __forceinline__ float4 lerp4(float4 a, float4 b, float t) {
+    return make_float4(fmaf(b.x - a.x, t, a.x), fmaf(b.y - a.y, t, a.y), fmaf(b.z - a.z, t, a.z), fmaf(b.w - a.w, t, a.w));
+}
+
+__device__ float4 bilinear(const uint8_t* pool, const TexDesc& D, int level, float u, float v) {
+    const float w = (float)max(1, D.width >> level), h = (float)max(1, D.height >> level);
+    const float s = fmaf(u, w, -0.5f), r = fmaf(v, h, -0.5f);
+    const float fs = floorf(s), fr = floorf(r);
+    const float ax = s - fs, ay = r - fr;
+    const int x0 = (int)fs, y0 = (int)fr;
+    const float4 c00 = fetch_texel(pool, D, level, x0, y0), c10 = fetch_texel(pool, D, level, x0 + 1, y0);
+    const float4 c01 = fetch_texel(pool, D, level, x0, y0 + 1), c11 = fetch_texel(pool, D, level, x0 + 1, y0 + 1);
+    return lerp4(lerp4(c00, c10, ax), lerp4(c01, c11, ax), ay);
+}
+
+__device__ float4 trilinear(const uint8_t* pool, const TexDesc& D, float u, float v, float lod) {
+    lod = fminf(fmaxf(lod, 0.0f), (float)(D.levels - 1));
+    const int l0 = (int)floorf(lod);
+    const int l1 = min(l0 + 1, D.levels - 1);
+    const float f = lod - (float)l0;
+    const float4 a = bilinear(pool, D, l0, u, v);
+    if (f == 0.0f) return a;
+    return lerp4(a, bilinear(pool, D, l1, u, v), f);
+}
+
+__device__ float4 sample_texture(const uint8_t* pool, const TexDesc& D, int sampler, int maxAniso, float u, float v,
+                                 float dudx, float dvdx, float dudy, float dvdy) {
+    if (sampler == 0) return bilinear(pool, D, 0, u, v);
+    const float W = (float)D.width, H = (float)D.height;
+    const float px = sqrtf((dudx * W) * (dudx * W) + (dvdx * H) * (dvdx * H));
+    const float py = sqrtf((dudy * W) * (dudy * W) + (dvdy * H) * (dvdy * H));
+    const float pmax = fmaxf(px, py), pmin = fminf(px, py);
+    if (sampler == 1 || !(pmax > 0.0f)) return trilinear(pool, D, u, v, pmax > 0.0f ? log2f(pmax) : 0.0f);
+    const float ratio = pmin > 0.0f ? pmax / pmin : (float)maxAniso;
+    int n = ratio >= (float)maxAniso ? maxAniso : (int)ceilf(ratio);
+    n = max(1, n);
+    const float lod = log2f(pmax / (float)n);
+    const float au = px >= py ? dudx : dudy, av = px >= py ? dvdx : dvdy;
+    float4 acc = make_float4(0, 0, 0, 0);
+    const float invn = 1.0f / (float)n;
+    for (int i = 0; i < n; i++) {
+        const float o = ((float)i + 0.5f) * invn - 0.5f;
+        const float4 c = trilinear(pool, D, fmaf(au, o, u), fmaf(av, o, v), lod);
+        acc.x += c.x; acc.y += c.y; acc.z += c.z; acc.w += c.w;
+    }
+    return make_float4(acc.x * invn, acc.y * invn, acc.z * invn, acc.w * invn);
+}
+
+__device__ __forceinline__ uint32_t flat_hash(uint32_t prim) {
+    uint32_t h = prim * 0x9E3779B1u;
+    h ^= h >> 15;
+    h *= 0x85EBCA77u;
+    h ^= h >> 13;
+    return h;
+}
+
+__device__ __forceinline__ float3 norm3(float3 a) {
+    const float l2 = a.x * a.x + a.y * a.y + a.z * a.z;
+    const float inv = l2 > 0.0f ? 1.0f / sqrtf(l2) : 1.0f;
+    return make_float3(a.x * inv, a.y * inv, a.z * inv);
+}
+
+__device__ float4 shade_fragment(const ShadeParams& P, const ShadeTri& T, const float* vtx, const int* idx,
+                                 const TexDesc* tex, const uint8_t* pool, int px, int py, float o) {
+    if (P.program == 0) {
+        const uint32_t h = flat_hash((uint32_t)T.prim);
+        return make_float4((h & 0xff) * (1.0f / 255.0f), ((h >> 8) & 0xff) * (1.0f / 255.0f),
+                           ((h >> 16) & 0xff) * (1.0f / 255.0f), 1.0f);
+    }
+    if (P.program != 1) return make_float4(0, 0, 0, 0);
+    const float x = (float)px + o, y = (float)py + o;
+    const float* v0 = vtx + (size_t)idx[3 * T.prim] * P.stride;
+    const float* v1 = vtx + (size_t)idx[3 * T.prim + 1] * P.stride;
+    const float* v2 = vtx + (size_t)idx[3 * T.prim + 2] * P.stride;
+    float w[3];
+    vertex_weights(T, x, y, w);
+    auto interp = [&](int off) { return fmaf(w[0], v0[off], fmaf(w[1], v1[off], w[2] * v2[off])); };
+    const float3 Pw = make_float3(interp(P.offWorld), interp(P.offWorld + 1), interp(P.offWorld + 2));
+    float3 N = make_float3(interp(P.offNormal), interp(P.offNormal + 1), interp(P.offNormal + 2));
+    const float u = interp(P.offUv), v = interp(P.offUv + 1);
+    float4 alb = make_float4(1, 1, 1, 1);
+    if (P.numTextures > 0) {
+        // fine quad derivatives (SBC:36-76): same triangle at the quad's other shading positions
+        const float qx = (float)(px & ~1) + o, qy = (float)(py & ~1) + o;
+        float wa[3], wb[3], wc[3], wd[3];
+        vertex_weights(T, qx, y, wa);
+        vertex_weights(T, qx + 1.0f, y, wb);
+        vertex_weights(T, x, qy, wc);
+        vertex_weights(T, x, qy + 1.0f, wd);
+        auto iuv = [&](const float* ww, int off) { return fmaf(ww[0], v0[off], fmaf(ww[1], v1[off], ww[2] * v2[off])); };
+        const float dudx = iuv(wb, P.offUv) - iuv(wa, P.offUv), dvdx = iuv(wb, P.offUv + 1) - iuv(wa, P.offUv + 1);
+        const float dudy = iuv(wd, P.offUv) - iuv(wc, P.offUv), dvdy = iuv(wd, P.offUv + 1) - iuv(wc, P.offUv + 1);
+        int mat = P.trisPerMaterial > 0 ? T.prim / P.trisPerMaterial : 0;
+        mat = min(mat, P.numTextures - 1);
+        alb = sample_texture(pool, tex[mat], P.sampler, P.maxAniso, u, v, dudx, dvdx, dudy, dvdy);
+    }
+    N = norm3(N);
+    const float3 V = norm3(make_float3(P.camera[0] - Pw.x, P.camera[1] - Pw.y, P.camera[2] - Pw.z));
+    float3 col = make_float3(alb.x * P.ambient, alb.y * P.ambient, alb.z * P.ambient);
+    for (int l = 0; l < P.numLights; l++) {
+        float3 L;
+        float att = 1.0f;
+        if (P.lightPos[l][3] == 0.0f) {
+            L = norm3(make_float3(P.lightPos[l][0], P.lightPos[l][1], P.lightPos[l][2]));
+        } else {
+            const float3 Lv = make_float3(P.lightPos[l][0] - Pw.x, P.lightPos[l][1] - Pw.y, P.lightPos[l][2] - Pw.z);
+            const float d2 = Lv.x * Lv.x + Lv.y * Lv.y + Lv.z * Lv.z;
+            const float inv = 1.0f / sqrtf(d2);
+            L = make_float3(Lv.x * inv, Lv.y * inv, Lv.z * inv);
+            att = 1.0f / (1.0f + P.lightColor[l][3] * d2);
+        }
+        const float ndl = N.x * L.x + N.y * L.y + N.z * L.z;
+        if (ndl > 0.0f) {
+            const float3 Hh = norm3(make_float3(L.x + V.x, L.y + V.y, L.z + V.z));
+            const float ndh = fmaxf(N.x * Hh.x + N.y * Hh.y + N.z * Hh.z, 0.0f);
+            const float spec = P.ks * powf(ndh, P.shininess);
+            col.x += att * P.lightColor[l][0] * (P.kd * ndl * alb.x + spec);
+            col.y += att * P.lightColor[l][1] * (P.kd * ndl * alb.y + spec);
+            col.z += att * P.lightColor[l][2] * (P.kd * ndl * alb.z + spec);
+        }
+    }
+    return make_float4(col.x, col.y, col.z, alb.w);
+}
+
+__device__ __forceinline__ float sat(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+
+template <int S>
+__global__ void __launch_bounds__(256) k_shade(ShadeParams P, TriRecs recs,
+                                               const float4* __restrict__ bary, const uint32_t* __restrict__ visSlot,
+                                               const float* __restrict__ vtx, const int* __restrict__ idx,
+                                               const TexDesc* __restrict__ tex, const uint8_t* __restrict__ pool,
+                                               uchar4* __restrict__ rgba8, float4* __restrict__ rgba32f) {
+    const int px = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int py = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (px >= P.width || py >= P.height) return;
+    const size_t pix = (size_t)py * P.width + px;
+    unsigned ids[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) ids[s] = visSlot[pix * S + s];
+    const float o = S == 1 ? 0.0f : 0.5f;
+    float4 sum = make_float4(0, 0, 0, 0);
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        if (ids[s] == 0xffffffffu) continue;
+        bool seen = false;
+#pragma unroll
+        for (int q = 0; q < s; q++) seen |= ids[q] == ids[s];
+        if (seen) continue;
+        int count = 0;
+#pragma unroll
+        for (int q = 0; q < S; q++) count += ids[q] == ids[s];
+        ShadeTri T;
+        load_shade_tri(recs, bary, P.ntri, ids[s], T);
+        const float4 c = shade_fragment(P, T, vtx, idx, tex, pool, px, py, o);
+        const float k = (float)count;
+        sum.x += sat(c.x) * k;
+        sum.y += sat(c.y) * k;
+        sum.z += sat(c.z) * k;
+        sum.w += sat(c.w) * k;
+    }
+    const float inv = 1.0f / (float)S;
+    const float4 r = make_float4(sum.x * inv, sum.y * inv, sum.z * inv, sum.w * inv);
+    rgba8[pix] = make_uchar4((unsigned char)floorf(r.x * 255.0f + 0.5f), (unsigned char)floorf(r.y * 255.0f + 0.5f),
+                             (unsigned char)floorf(r.z * 255.0f + 0.5f), (unsigned char)floorf(r.w * 255.0f + 0.5f));
+    if (rgba32f) rgba32f[pix] = r;
+}
+
+}  // namespace
+
+void launch_shade(const ShadeParams& p, TriRecs recs, const float4* bary, const uint32_t* visSlot,
+                  const float* vtx, const int* idx, const TexDesc* tex, const uint8_t* texPool, uchar4* rgba8,
+                  float4* rgba32f, hipStream_t s) {
+    const dim3 grid(dev::blocks(p.width, 16), dev::blocks(p.height, 16));
+    switch (p.samples) {
+        case 1:
+            hipLaunchKernelGGL(k_shade<1>, grid, dim3(256), 0, s, p, recs, bary, visSlot, vtx, idx, tex, texPool,
+                               rgba8, rgba32f);
+            break;
+        case 4:
+            hipLaunchKernelGGL(k_shade<4>, grid, dim3(256), 0, s, p, recs, bary, visSlot, vtx, idx, tex, texPool,
+                               rgba8, rgba32f);
+            break;
+        default:
+            hipLaunchKernelGGL(k_shade<8>, grid, dim3(256), 0, s, p, recs, bary, visSlot, vtx, idx, tex, texPool,
+                               rgba8, rgba32f);
+            break;
+    }
+}
+
+}  // namespace ifrit_hip
