@@ -39,24 +39,8 @@ static const uint32_t kRcpTableHost[2048] = {
 #include "x86_rcp_table.inc"
 };
 
-IFX_HD uint32_t f2u(float f) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __float_as_uint(f);
-#else
-    uint32_t u;
-    memcpy(&u, &f, 4);
-    return u;
-#endif
-}
-IFX_HD float u2f(uint32_t u) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __uint_as_float(u);
-#else
-    float f;
-    memcpy(&f, &u, 4);
-    return f;
-#endif
-}
+IFX_HD uint32_t f2u(float f) { return __builtin_bit_cast(uint32_t, f); }
+IFX_HD float u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
 
 // RCPSS/RCPPS: table lookup on the top 11 mantissa bits, exact exponent
 // scaling, sub-normal results flushed to signed zero, zero and sub-normal

@@ -6,6 +6,8 @@
 // stage a kernel on the renderer's stream instead of a thread barrier.
 #include "renderer.hpp"
 
+#include "../common/x86_fp.h"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -49,7 +51,7 @@ TileRasterRenderer::TileRasterRenderer() {}
 TileRasterRenderer::~TileRasterRenderer() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (DevBuf* b : {&vtx_, &idx_, &clip_, &recs_[0], &recs_[1], &recs_[2], &recs_[3], &recs_[4], &recs_[5], &bary_, &tileBox_, &slotOf_, &binCount_, &binStart_, &binCursor_, &binList_,
-                      &bigList_, &ctr_, &visId_, &visDepth_, &visSlot_, &visClass_, &rgba8_, &rgba32f_, &texDesc_})
+                      &bigList_, &ctr_, &clipList_, &statPart_, &visId_, &visDepth_, &visSlot_, &visClass_, &rgba8_, &rgba32f_, &texDesc_})
         b->release();
     for (auto& t : textures_)
         if (t.dev) (void)hipFree(t.dev);
@@ -351,7 +353,8 @@ bool TileRasterRenderer::prepare(int indexCount) {
                  recs_[2].ensure(slots * 16) && recs_[3].ensure(slots * 16) && recs_[4].ensure(slots * 16) &&
                  recs_[5].ensure(slots * 16) &&
                  bary_.ensure(slots * 32) && tileBox_.ensure(slots * 16) &&
-                 slotOf_.ensure((size_t)std::max(1, ntri) * 4) && binCount_.ensure((size_t)tiles * 4) &&
+                 slotOf_.ensure((size_t)std::max(1, ntri) * 4) && clipList_.ensure((size_t)clipCap_ * 4) &&
+                 statPart_.ensure(((size_t)(ntri + 255) / 256 + 2 * (size_t)tiles + 2) * 8) && binCount_.ensure((size_t)tiles * 4) &&
                  binStart_.ensure((size_t)tiles * 4) && binCursor_.ensure((size_t)tiles * 4) &&
                  binList_.ensure((size_t)binCap_ * 4) && bigList_.ensure((size_t)bigCap_ * 4) &&
                  visId_.ensure(ps * 4) && visDepth_.ensure(ps * 4) && visSlot_.ensure(ps * 4) &&
@@ -385,6 +388,18 @@ TriRecs TileRasterRenderer::recPtrs() const {
                    recs_[3].as<float4>(), recs_[4].as<float4>(), recs_[5].as<float4>()};
 }
 
+StatPartials TileRasterRenderer::statPtrs() const {
+    const int tilesX = (width_ + kTile - 1) / kTile, tilesY = (height_ + kTile - 1) / kTile;
+    StatPartials st{};
+    st.setupBlocks = (curTris_ + 255) / 256;
+    st.tiles = tilesX * tilesY;
+    unsigned long long* base = statPart_.as<unsigned long long>();
+    st.setupAssembled = base;
+    st.rasterFrags = base + st.setupBlocks;
+    st.rasterCovs = base + st.setupBlocks + st.tiles;
+    return st;
+}
+
 void TileRasterRenderer::enqueue(bool clear) {
     const int tilesX = (width_ + kTile - 1) / kTile, tilesY = (height_ + kTile - 1) / kTile;
     const int tiles = tilesX * tilesY;
@@ -399,9 +414,21 @@ void TileRasterRenderer::enqueue(bool clear) {
     mark(1);
     launch_reset(binCount_.as<unsigned>(), binCursor_.as<unsigned>(), tiles, ctr_.as<DeviceCounters>(), s);
     mark(2);
-    SetupParams sp{width_, height_, tilesX, tilesY, curTris_, nverts_, 0, samples_ > 1 ? 1 : 0, clipCap_};
+    SetupParams sp{};
+    sp.width = width_;
+    sp.height = height_;
+    sp.tilesX = tilesX;
+    sp.tilesY = tilesY;
+    sp.ntri = curTris_;
+    sp.nverts = nverts_;
+    sp.ccw = 0;
+    sp.msaa = samples_ > 1 ? 1 : 0;
+    sp.clipCap = clipCap_;
+    sp.invW = 1.0f / (float)width_;  // TileRasterRenderer.cpp:59-60
+    sp.invH = 1.0f / (float)height_;
+    sp.invTileWidth = ifx::rcp_nr_ss(16.0f);
     launch_setup(sp, clip_.as<float4>(), idx_.as<int>(), recPtrs(), bary_.as<float4>(), tileBox_.as<int4>(),
-                 slotOf_.as<unsigned>(), ctr_.as<DeviceCounters>(), s);
+                 slotOf_.as<unsigned>(), clipList_.as<unsigned>(), statPtrs(), ctr_.as<DeviceCounters>(), s);
     mark(3);
     BinParams bp{tilesX, tilesY, curTris_, clipCap_, binCap_, bigCap_, kBigThreshold};
     launch_bin(bp, recPtrs(), tileBox_.as<int4>(), binCount_.as<unsigned>(), binStart_.as<unsigned>(),
@@ -424,7 +451,8 @@ void TileRasterRenderer::enqueue(bool clear) {
     launch_raster(rp, samples_, recPtrs(), slotOf_.as<unsigned>(), binStart_.as<unsigned>(),
                   binCount_.as<unsigned>(),
                   binList_.as<unsigned>(), visId_.as<uint32_t>(), visDepth_.as<float>(), visSlot_.as<uint32_t>(),
-                  visClass_.as<uint8_t>(), ctr_.as<DeviceCounters>(), s);
+                  visClass_.as<uint8_t>(), statPtrs(), ctr_.as<DeviceCounters>(), s);
+    launch_finalize(statPtrs(), ctr_.as<DeviceCounters>(), s);
     mark(5);
     if (fsBound_ && fs_.desc.kind != IFHIP_FS_DEPTH_ONLY) {
         ShadeParams sh{};

@@ -160,10 +160,11 @@ private:
 
     // device buffers
     DevBuf vtx_, idx_, clip_, recs_[6], bary_, tileBox_, slotOf_, binCount_, binStart_, binCursor_, binList_, bigList_, ctr_;
-    DevBuf visId_, visDepth_, visSlot_, visClass_, rgba8_, rgba32f_, texDesc_;
+    DevBuf clipList_, statPart_, visId_, visDepth_, visSlot_, visClass_, rgba8_, rgba32f_, texDesc_;
     unsigned clipCap_ = 0, binCap_ = 0, bigCap_ = 0;
     int curTris_ = 0;
     TriRecs recPtrs() const;
+    StatPartials statPtrs() const;
     std::vector<TextureSlot> textures_;
     bool texDescDirty_ = true;
 

@@ -180,6 +180,17 @@ __device__ __forceinline__ unsigned wave_inc(unsigned* ctr, int key, bool active
     return result;
 }
 
+// Sum over a 256-thread workgroup (all threads must call it).
+__device__ __forceinline__ unsigned long long block_sum256(unsigned long long v) {
+    __shared__ unsigned long long part[4];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const unsigned long long r = part[0] + part[1] + part[2] + part[3];
+    __syncthreads();
+    return r;
+}
+
 __device__ __forceinline__ int wave_max(int v) {
     for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
     return v;

@@ -23,9 +23,8 @@ namespace ifrit_hip {
 //   f1, f2, f3 = interpolation bases (x, y, z) and vz = (z/w) * w.
 // key = prim * 4 + fan orders fragments the way the reference's per-tile
 // sort by originalPrimitive does (TileRasterWorker.cpp:1244-1258).  The
-// pixel box bounds every pixel the coverage tests can accept (the edge
-// functions' EPS bias grows a triangle slightly); triangles flagged big are
-// rasterised one pixel per lane.
+// pixel box is the padded screen extent, a hint that selects the fine-raster
+// path (triangles flagged big are rasterised one pixel per lane).
 struct __align__(16) TriRec {
     float4 q0, q1, q2;
     float4 f1, f2, f3;
@@ -43,7 +42,9 @@ struct SetupParams {
     int ntri, nverts;
     int ccw;
     int msaa;           // 1 disables the small-triangle cull
-    unsigned clipCap;   // capacity for clip fan triangles
+    unsigned clipCap;   // capacity for clip fan triangles and for triangles to clip
+    float invW, invH;   // 1 / width, 1 / height (TileRasterRenderer.cpp:59-60)
+    float invTileWidth; // ApproxReciprocal(16.0f) (TileRasterWorker.cpp:499)
 };
 
 struct BinParams {
@@ -94,6 +95,15 @@ enum StatusBits : unsigned {
     kStatusBadIndex = 8u,
 };
 
+// Per-workgroup statistics (no same-address atomics in the hot kernels);
+// k_finalize reduces them into DeviceCounters.
+struct StatPartials {
+    unsigned long long* setupAssembled;  // one per setup workgroup
+    unsigned long long* rasterFrags;     // one per tile
+    unsigned long long* rasterCovs;      // one per tile
+    int setupBlocks, tiles;
+};
+
 // Counters written by the kernels (device memory, zeroed each draw).
 struct DeviceCounters {
     unsigned status;
@@ -103,7 +113,8 @@ struct DeviceCounters {
     unsigned long long fragments;
     unsigned long long coveredSamples;
     unsigned long long assembled;
-    unsigned long long pad;
+    unsigned clipInCount;
+    unsigned pad;
 };
 
 // Launch wrappers (pipeline.hip).
@@ -111,12 +122,14 @@ void launch_vertex(const float* vtx, int stride, int nverts, const float* mvp16,
                    hipStream_t s);
 void launch_reset(unsigned* binCount, unsigned* binCursor, int tiles, DeviceCounters* ctr, hipStream_t s);
 void launch_setup(const SetupParams& p, const float4* clip, const int* idx, TriRecs recs, float4* bary, int4* tileBox,
-                  unsigned* slotOf, DeviceCounters* ctr, hipStream_t s);
+                  unsigned* slotOf, unsigned* clipList, StatPartials st, DeviceCounters* ctr, hipStream_t s);
+void launch_finalize(StatPartials st, DeviceCounters* ctr, hipStream_t s);
 void launch_bin(const BinParams& p, TriRecs recs, const int4* tileBox, unsigned* binCount, unsigned* binStart,
                 unsigned* binCursor, unsigned* binList, unsigned* bigList, DeviceCounters* ctr, hipStream_t s);
 void launch_raster(const RasterParams& p, int samples, TriRecs recs, const unsigned* slotOf,
                    const unsigned* binStart, const unsigned* binCount, const unsigned* binList, uint32_t* visId,
-                   float* visDepth, uint32_t* visSlot, uint8_t* visClass, DeviceCounters* ctr, hipStream_t s);
+                   float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st, DeviceCounters* ctr,
+                   hipStream_t s);
 void launch_shade(const ShadeParams& p, TriRecs recs, const float4* bary, const uint32_t* visSlot,
                   const float* vtx, const int* idx, const TexDesc* tex, const uint8_t* texPool, uchar4* rgba8,
                   float4* rgba32f, hipStream_t s);

@@ -14,11 +14,11 @@
 // lives in LDS as 64-bit keys (ordered depth << 32 | prim * 4 + fan) and
 // fragments are merged with 64-bit LDS atomic min in any order.
 //
-// Work split per tile list entry: triangles whose conservative pixel box
-// (computed at setup) has at most 64 pixels are rasterised one per lane, each
-// lane visiting only its box; larger ones one pixel per lane, where each wave
-// holds one row of 4x4 blocks so the 4x2 group and quad classification of a
-// pixel comes from one ballot, and waves outside the box skip the triangle.
+// Work split per tile list entry: triangles whose screen extent (a size hint
+// from setup) is at most 64 pixels are rasterised one per lane; larger ones
+// one pixel per lane, where each wave holds one row of 4x4 blocks so the 4x2
+// group and quad classification of a pixel comes from one ballot.  Both paths
+// make the same exact decisions; the split only balances the work.
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"
@@ -35,8 +35,14 @@ __device__ __forceinline__ void merge(unsigned long long* cell, float d, unsigne
     if (d == d) atomicMin(cell, frag_key(d, tie));
 }
 
-// One lane rasterises one small triangle into the tile's LDS visibility,
-// visiting only the pixels of its conservative pixel box.
+// One lane rasterises one small triangle into the tile's LDS visibility.
+// The lane first collects the 4x4 blocks that pass the reference's trivial
+// reject test (TRW:887-934) and then, per candidate block, the pixels whose
+// square may hold a covered point; it iterates only over set bits, so the
+// lanes of a wave stay busy with their own triangles instead of stepping
+// through the union of all lanes' blocks.  Both prunings are exact: the
+// block test is the reference's own, and square_may_cover() is a monotone
+// bound of the pixel test.
 template <int S>
 __device__ void raster_small(const RasterParams& P, const TriRec& R, int tx, int ty, unsigned long long* vis,
                              unsigned long long& frags, unsigned long long& covs) {
@@ -45,33 +51,37 @@ __device__ void raster_small(const RasterParams& P, const TriRec& R, int tx, int
     const unsigned tie = __float_as_uint(R.q2.y);
     const float tileMinX = (float)(tx * 16) * 1.0f, tileMinY = (float)(ty * 16) * 1.0f;
     const int tx0 = tx * 16, ty0 = ty * 16;
-    const PixBox B = pix_box(R);
-    const int x0 = max(B.x0, tx0), x1 = min(B.x1, tx0 + 15);
-    const int y0 = max(B.y0, ty0), y1 = min(B.y1, ty0 + 15);
-    if (S == 1) {
-        // 4x2 groups of the reference (TRW:1042-1104) overlapping the box
-        for (int gy = y0 & ~1; gy <= y1; gy += 2) {
-            for (int gx = x0 & ~3; gx <= x1; gx += 4) {
-                const float bMinX = fmaf((float)((gx - tx0) >> 2), 4.0f, tileMinX);
-                const float bMinY = fmaf((float)((gy - ty0) >> 2), 4.0f, tileMinY);
-                int trn, tan;
-                block_test(E, bMinX, bMinY, trn, tan);
-                if (trn != 3) continue;
+    unsigned trMask = 0, taMask = 0;
+#pragma unroll
+    for (int bi = 0; bi < 16; bi++) {
+        int trn, tan;
+        block_test(E, fmaf((float)(bi & 3), 4.0f, tileMinX), fmaf((float)(bi >> 2), 4.0f, tileMinY), trn, tan);
+        trMask |= (unsigned)(trn == 3) << bi;
+        taMask |= (unsigned)(tan == 3) << bi;
+    }
+    while (trMask) {
+        const int bi = __ffs(trMask) - 1;
+        trMask &= trMask - 1u;
+        const bool accept = (taMask >> bi) & 1u;
+        const int px0 = tx0 + (bi & 3) * 4, py0 = ty0 + (bi >> 2) * 4;
+        if (S == 1) {
+            for (int sk = 0; sk < 2; sk++) {
+                const int gy = py0 + 2 * sk;
                 // lanes of the 4x2 group: (0,0) (1,0) (0,1) (1,1) (2,0) (3,0) (2,1) (3,1)
                 unsigned acc = 0xFFu;
-                if (tan != 3) {
+                if (!accept) {
                     acc = 0;
 #pragma unroll
                     for (int l = 0; l < 8; l++) {
                         const int ox = (l & 1) + ((l >> 2) << 1), oy = (l >> 1) & 1;
-                        acc |= (unsigned)inside(E, (float)(gx + ox), (float)(gy + oy)) << l;
+                        acc |= (unsigned)inside(E, (float)(px0 + ox), (float)(gy + oy)) << l;
                     }
                 }
-#pragma unroll
-                for (int l = 0; l < 8; l++) {
-                    if (!((acc >> l) & 1u)) continue;
-                    const int ox = (l & 1) + ((l >> 2) << 1), oy = (l >> 1) & 1;
-                    const int px = gx + ox, py = gy + oy;
+                unsigned todo = acc;
+                while (todo) {
+                    const int l = __ffs(todo) - 1;
+                    todo &= todo - 1u;
+                    const int px = px0 + (l & 1) + ((l >> 2) << 1), py = gy + ((l >> 1) & 1);
                     if (px >= P.width || py >= P.height) continue;
                     const unsigned quad = (acc >> (l & 4)) & 0xFu;
                     const float fx = (float)px, fy = (float)py;
@@ -87,23 +97,26 @@ __device__ void raster_small(const RasterParams& P, const TriRec& R, int tx, int
                     merge(&vis[(py - ty0) * 16 + (px - tx0)], d, tie);
                 }
             }
-        }
-    } else {
-        for (int py = y0; py <= y1; py++) {
-            for (int px = x0; px <= x1; px++) {
-                const float bMinX = fmaf((float)((px - tx0) >> 2), 4.0f, tileMinX);
-                const float bMinY = fmaf((float)((py - ty0) >> 2), 4.0f, tileMinY);
-                int trn, tan;
-                block_test(E, bMinX, bMinY, trn, tan);
-                if (trn != 3) continue;
+        } else {
+            unsigned pm = 0xFFFFu;
+            if (!accept) {
+                pm = 0;
+#pragma unroll
+                for (int q = 0; q < 16; q++)
+                    pm |= (unsigned)square_may_cover(E, (float)(px0 + (q & 3)), (float)(py0 + (q >> 2))) << q;
+            }
+            while (pm) {
+                const int q = __ffs(pm) - 1;
+                pm &= pm - 1u;
+                const int px = px0 + (q & 3), py = py0 + (q >> 2);
+                if (px >= P.width || py >= P.height) continue;
                 const float fx = (float)px, fy = (float)py;
-                if (tan != 3 && !square_may_cover(E, fx, fy)) continue;
                 bool any = false;
                 unsigned long long* cell = &vis[((py - ty0) * 16 + (px - tx0)) * S];
 #pragma unroll
                 for (int s = 0; s < S; s++) {
                     const float sx = fx + P.sx[s], sy = fy + P.sy[s];
-                    if (tan != 3 && !inside(E, sx, sy)) continue;
+                    if (!accept && !inside(E, sx, sy)) continue;
                     any = true;
                     covs++;
                     merge(&cell[s], depth256(F1, F2, F3, sx, sy), tie);
@@ -208,7 +221,8 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P, TriRecs recs,
                                                 const unsigned* __restrict__ binCount,
                                                 const unsigned* __restrict__ binList, uint32_t* __restrict__ visId,
                                                 float* __restrict__ visDepth, uint32_t* __restrict__ visSlot,
-                                                uint8_t* __restrict__ visClass, DeviceCounters* ctr) {
+                                                uint8_t* __restrict__ visClass, StatPartials st,
+                                                DeviceCounters* ctr) {
     __shared__ unsigned long long sVis[256 * S];
     __shared__ TriRec sRec[kChunk];
     __shared__ unsigned sEnt[kChunk];
@@ -259,12 +273,8 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P, TriRecs recs,
         }
         __syncthreads();
         const int nb = sNumBig;
-        const int wy0 = ty * 16 + (t >> 6) * 4;  // rows of this wave
         for (int b = 0; b < nb; b++) {
             const int j = sBig[b];
-            const PixBox B = pix_box(sRec[j]);
-            // wave-uniform skip when the box misses this wave's 16x4 strip
-            if (B.y1 < wy0 || B.y0 > wy0 + 3 || B.x1 < tx * 16 || B.x0 > tx * 16 + 15) continue;
             raster_big<S>(P, sRec[j], (sEnt[j] >> 31) != 0, fpx, fpy, bMinX, bMinY, gShift0, qShift0, inFb,
                           &sVis[t * S], frags, covs);
         }
@@ -300,13 +310,11 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P, TriRecs recs,
         }
         if (S == 1 && visClass && (cls || P.clear)) visClass[pix] = (uint8_t)cls;
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        frags += __shfl_down(frags, off, 64);
-        covs += __shfl_down(covs, off, 64);
-    }
-    if ((t & 63) == 0) {
-        if (frags) atomicAdd(&ctr->fragments, frags);
-        if (covs) atomicAdd(&ctr->coveredSamples, covs);
+    const unsigned long long tf = block_sum256(frags);
+    const unsigned long long tc = block_sum256(covs);
+    if (t == 0) {
+        st.rasterFrags[tile] = tf;
+        st.rasterCovs[tile] = tc;
     }
 }
 
@@ -314,20 +322,21 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P, TriRecs recs,
 
 void launch_raster(const RasterParams& p, int samples, TriRecs recs, const unsigned* slotOf,
                    const unsigned* binStart, const unsigned* binCount, const unsigned* binList, uint32_t* visId,
-                   float* visDepth, uint32_t* visSlot, uint8_t* visClass, DeviceCounters* ctr, hipStream_t s) {
+                   float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st, DeviceCounters* ctr,
+                   hipStream_t s) {
     const dim3 grid(p.tilesX * p.tilesY);
     switch (samples) {
         case 1:
             hipLaunchKernelGGL(k_raster<1>, grid, dim3(256), 0, s, p, recs, slotOf, binStart, binCount, binList, visId,
-                               visDepth, visSlot, visClass, ctr);
+                               visDepth, visSlot, visClass, st, ctr);
             break;
         case 4:
             hipLaunchKernelGGL(k_raster<4>, grid, dim3(256), 0, s, p, recs, slotOf, binStart, binCount, binList, visId,
-                               visDepth, visSlot, visClass, ctr);
+                               visDepth, visSlot, visClass, st, ctr);
             break;
         default:
             hipLaunchKernelGGL(k_raster<8>, grid, dim3(256), 0, s, p, recs, slotOf, binStart, binCount, binList, visId,
-                               visDepth, visSlot, visClass, ctr);
+                               visDepth, visSlot, visClass, st, ctr);
             break;
     }
 }
