@@ -60,9 +60,18 @@ def cpu_baseline(scene, frames: int = 5):
         v = np.ascontiguousarray(scene.vertices, np.float32)
         i = np.ascontiguousarray(scene.indices, np.int32)
         m = np.ascontiguousarray(scene.mvp, np.float32)
-        ms = O.ref().ref_render_timed(scene.width, scene.height, v.ctypes.data_as(C.c_void_p), v.shape[1], v.shape[0],
-                                      i.ctypes.data_as(C.c_void_p), i.size, m.ctypes.data_as(C.c_void_p), frames,
-                                      None)
+        # the reference library logs to stdout; keep this process's stdout to the one JSON line
+        sys.stdout.flush()
+        saved = os.dup(1)
+        os.dup2(2, 1)
+        try:
+            ms = O.ref().ref_render_timed(scene.width, scene.height, v.ctypes.data_as(C.c_void_p), v.shape[1],
+                                          v.shape[0], i.ctypes.data_as(C.c_void_p), i.size,
+                                          m.ctypes.data_as(C.c_void_p), frames, None)
+            C.CDLL(None).fflush(None)
+        finally:
+            os.dup2(saved, 1)
+            os.close(saved)
         return {"value": round(1000.0 / ms, 3), "unit": "frames/s", "ms_per_frame": round(ms, 3), "cores": 17,
                 "kind": "reference",
                 "sample": f"{frames} frames (median) of the same 1.31M-triangle geometry at {scene.width}x{scene.height} "

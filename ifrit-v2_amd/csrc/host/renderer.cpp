@@ -51,7 +51,7 @@ TileRasterRenderer::TileRasterRenderer() {}
 TileRasterRenderer::~TileRasterRenderer() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (DevBuf* b : {&vtx_, &idx_, &clip_, &recs_[0], &recs_[1], &recs_[2], &recs_[3], &recs_[4], &recs_[5], &bary_, &tileBox_, &slotOf_, &binCount_, &binStart_, &binCursor_, &binList_,
-                      &bigList_, &ctr_, &clipList_, &statPart_, &visId_, &visDepth_, &visSlot_, &visClass_, &rgba8_, &rgba32f_, &texDesc_})
+                      &bigList_, &ctr_, &clipList_, &statPart_, &items_, &tilePartial_, &multiList_, &partials_, &visId_, &visDepth_, &visSlot_, &visClass_, &rgba8_, &rgba32f_, &texDesc_})
         b->release();
     for (auto& t : textures_)
         if (t.dev) (void)hipFree(t.dev);
@@ -89,6 +89,10 @@ void TileRasterRenderer::Init() {
     for (auto& e : ev_)
         if (!ok(hipEventCreate(&e), "hipEventCreate")) return;
     if (!ctr_.ensure(sizeof(DeviceCounters))) setError("hipMalloc counters");
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+        numCUs_ = cus;
     initialized_ = true;
 }
 
@@ -348,13 +352,17 @@ bool TileRasterRenderer::prepare(int indexCount) {
     clipCap_ = std::max<unsigned>(clipCap_, std::max(4096u, (unsigned)(ntri / 8)));
     binCap_ = std::max<unsigned>(binCap_, std::max(1u << 20, (unsigned)std::min<size_t>(0xFFFFFFF0ull, (size_t)ntri * 6)));
     bigCap_ = std::max<unsigned>(bigCap_, std::max(1024u, (unsigned)(ntri / 16)));
+    itemCap_ = std::max<unsigned>(itemCap_, (unsigned)tiles + binCap_ / 256u + 1u);
+    partialCap_ = std::max<unsigned>(partialCap_, std::max(256u, binCap_ / 512u));
     const size_t slots = (size_t)ntri + clipCap_;
     bool okAll = clip_.ensure((size_t)nverts_ * 16) && recs_[0].ensure(slots * 16) && recs_[1].ensure(slots * 16) &&
                  recs_[2].ensure(slots * 16) && recs_[3].ensure(slots * 16) && recs_[4].ensure(slots * 16) &&
                  recs_[5].ensure(slots * 16) &&
                  bary_.ensure(slots * 32) && tileBox_.ensure(slots * 16) &&
                  slotOf_.ensure((size_t)std::max(1, ntri) * 4) && clipList_.ensure((size_t)clipCap_ * 4) &&
-                 statPart_.ensure(((size_t)(ntri + 255) / 256 + 2 * (size_t)tiles + 2) * 8) && binCount_.ensure((size_t)tiles * 4) &&
+                 statPart_.ensure(((size_t)(ntri + 255) / 256 + 2 * (size_t)rasterGrid() + 2) * 8) &&
+                 items_.ensure((size_t)itemCap_ * 8) && tilePartial_.ensure((size_t)tiles * 4) &&
+                 multiList_.ensure((size_t)tiles * 8) && partials_.ensure((size_t)partialCap_ * 256 * samples_ * 8) && binCount_.ensure((size_t)tiles * 4) &&
                  binStart_.ensure((size_t)tiles * 4) && binCursor_.ensure((size_t)tiles * 4) &&
                  binList_.ensure((size_t)binCap_ * 4) && bigList_.ensure((size_t)bigCap_ * 4) &&
                  visId_.ensure(ps * 4) && visDepth_.ensure(ps * 4) && visSlot_.ensure(ps * 4) &&
@@ -391,14 +399,18 @@ TriRecs TileRasterRenderer::recPtrs() const {
 StatPartials TileRasterRenderer::statPtrs() const {
     const int tilesX = (width_ + kTile - 1) / kTile, tilesY = (height_ + kTile - 1) / kTile;
     StatPartials st{};
+    (void)tilesX;
+    (void)tilesY;
     st.setupBlocks = (curTris_ + 255) / 256;
-    st.tiles = tilesX * tilesY;
+    st.rasterBlocks = rasterGrid();
     unsigned long long* base = statPart_.as<unsigned long long>();
     st.setupAssembled = base;
     st.rasterFrags = base + st.setupBlocks;
-    st.rasterCovs = base + st.setupBlocks + st.tiles;
+    st.rasterCovs = base + st.setupBlocks + st.rasterBlocks;
     return st;
 }
+
+int TileRasterRenderer::rasterGrid() const { return numCUs_ * 4; }
 
 void TileRasterRenderer::enqueue(bool clear) {
     const int tilesX = (width_ + kTile - 1) / kTile, tilesY = (height_ + kTile - 1) / kTile;
@@ -448,9 +460,10 @@ void TileRasterRenderer::enqueue(bool clear) {
         rp.sx[i] = st[2 * i];
         rp.sy[i] = st[2 * i + 1];
     }
+    RasterQueue q{items_.as<uint2>(), tilePartial_.as<int>(), multiList_.as<unsigned>(),
+                  partials_.as<unsigned long long>(), itemCap_, partialCap_, rasterGrid()};
     launch_raster(rp, samples_, recPtrs(), slotOf_.as<unsigned>(), binStart_.as<unsigned>(),
-                  binCount_.as<unsigned>(),
-                  binList_.as<unsigned>(), visId_.as<uint32_t>(), visDepth_.as<float>(), visSlot_.as<uint32_t>(),
+                  binCount_.as<unsigned>(), binList_.as<unsigned>(), q, visId_.as<uint32_t>(), visDepth_.as<float>(), visSlot_.as<uint32_t>(),
                   visClass_.as<uint8_t>(), statPtrs(), ctr_.as<DeviceCounters>(), s);
     launch_finalize(statPtrs(), ctr_.as<DeviceCounters>(), s);
     mark(5);
@@ -510,8 +523,12 @@ bool TileRasterRenderer::finish(bool clear, int indexCount) {
     if (c.status & kStatusClipOverflow) clipCap_ = std::max(clipCap_ * 2, c.clipCount + 1024u);
     if (c.status & kStatusBinOverflow) binCap_ = std::max<unsigned>(binCap_ * 2, c.binTotal + (1u << 20));
     if (c.status & kStatusBigOverflow) bigCap_ = std::max(bigCap_ * 2, c.bigCount + 1024u);
+    if (c.status & kStatusRasterOverflow) {
+        itemCap_ = std::max(itemCap_ * 2, c.numItems + 1024u);
+        partialCap_ = std::max(partialCap_ * 2, c.numPartials + 256u);
+    }
     if (c.status & kStatusBadIndex) setError("index buffer references a vertex beyond the vertex buffer");
-    return (c.status & (kStatusClipOverflow | kStatusBinOverflow | kStatusBigOverflow)) == 0;
+    return (c.status & (kStatusClipOverflow | kStatusBinOverflow | kStatusBigOverflow | kStatusRasterOverflow)) == 0;
 }
 
 void TileRasterRenderer::drawElements(int indexCount, bool clear) {

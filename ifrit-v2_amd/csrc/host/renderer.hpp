@@ -160,11 +160,13 @@ private:
 
     // device buffers
     DevBuf vtx_, idx_, clip_, recs_[6], bary_, tileBox_, slotOf_, binCount_, binStart_, binCursor_, binList_, bigList_, ctr_;
-    DevBuf clipList_, statPart_, visId_, visDepth_, visSlot_, visClass_, rgba8_, rgba32f_, texDesc_;
-    unsigned clipCap_ = 0, binCap_ = 0, bigCap_ = 0;
+    DevBuf clipList_, statPart_, items_, tilePartial_, multiList_, partials_, visId_, visDepth_, visSlot_, visClass_, rgba8_, rgba32f_, texDesc_;
+    unsigned clipCap_ = 0, binCap_ = 0, bigCap_ = 0, itemCap_ = 0, partialCap_ = 0;
+    int numCUs_ = 256;
     int curTris_ = 0;
     TriRecs recPtrs() const;
     StatPartials statPtrs() const;
+    int rasterGrid() const;
     std::vector<TextureSlot> textures_;
     bool texDescDirty_ = true;
 

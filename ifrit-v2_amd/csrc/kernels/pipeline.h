@@ -93,15 +93,16 @@ enum StatusBits : unsigned {
     kStatusBinOverflow = 2u,
     kStatusBigOverflow = 4u,
     kStatusBadIndex = 8u,
+    kStatusRasterOverflow = 16u,
 };
 
 // Per-workgroup statistics (no same-address atomics in the hot kernels);
 // k_finalize reduces them into DeviceCounters.
 struct StatPartials {
     unsigned long long* setupAssembled;  // one per setup workgroup
-    unsigned long long* rasterFrags;     // one per tile
-    unsigned long long* rasterCovs;      // one per tile
-    int setupBlocks, tiles;
+    unsigned long long* rasterFrags;     // one per raster workgroup
+    unsigned long long* rasterCovs;      // one per raster workgroup
+    int setupBlocks, rasterBlocks;
 };
 
 // Counters written by the kernels (device memory, zeroed each draw).
@@ -114,7 +115,24 @@ struct DeviceCounters {
     unsigned long long coveredSamples;
     unsigned long long assembled;
     unsigned clipInCount;
-    unsigned pad;
+    unsigned itemHead;   // next raster work item
+    unsigned numItems;
+    unsigned numMulti;     // tiles cut into several parts
+    unsigned numPartials;  // partial blocks those need
+    unsigned numEmpty;     // tiles with an empty list
+};
+
+// Raster work: (tile, part) items drained through DeviceCounters::itemHead;
+// per tile the first partial block of a tile cut into several parts (-1
+// otherwise); the list of such tiles followed, from index `tiles`, by the
+// empty tiles; partial key blocks of 256 * samples keys each.
+struct RasterQueue {
+    uint2* items;
+    int* tilePartial;
+    unsigned* multiList;
+    unsigned long long* partials;
+    unsigned itemCap, partialCap;
+    int grid;
 };
 
 // Launch wrappers (pipeline.hip).
@@ -127,9 +145,9 @@ void launch_finalize(StatPartials st, DeviceCounters* ctr, hipStream_t s);
 void launch_bin(const BinParams& p, TriRecs recs, const int4* tileBox, unsigned* binCount, unsigned* binStart,
                 unsigned* binCursor, unsigned* binList, unsigned* bigList, DeviceCounters* ctr, hipStream_t s);
 void launch_raster(const RasterParams& p, int samples, TriRecs recs, const unsigned* slotOf,
-                   const unsigned* binStart, const unsigned* binCount, const unsigned* binList, uint32_t* visId,
-                   float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st, DeviceCounters* ctr,
-                   hipStream_t s);
+                   const unsigned* binStart, const unsigned* binCount, const unsigned* binList, const RasterQueue& q,
+                   uint32_t* visId, float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st,
+                   DeviceCounters* ctr, hipStream_t s);
 void launch_shade(const ShadeParams& p, TriRecs recs, const float4* bary, const uint32_t* visSlot,
                   const float* vtx, const int* idx, const TexDesc* tex, const uint8_t* texPool, uchar4* rgba8,
                   float4* rgba32f, hipStream_t s);

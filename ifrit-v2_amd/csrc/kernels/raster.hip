@@ -1,24 +1,36 @@
-// Fine raster and early Z on gfx950: one 256-thread workgroup per 16x16 tile.
+// Fine raster and early Z on gfx950.
 //
 // Coverage follows the reference's hierarchy exactly
-// (projects/softgraphics/src/engine/tileraster/TileRasterWorker.cpp:787-1223):
-// fully covered tiles from the binner, then 4x4 blocks by trivial reject /
-// accept, then 4x2 groups, 2x2 quads and single pixels at 1 sample per pixel,
-// each class with its own depth expression (TRW:2244-2918).  With MSAA every
-// sample of a partially covered block is tested individually.
+// (projects/softgraphics/src/engine/tileraster/TileRasterWorker.cpp:787-1223,
+// "TRW"): fully covered tiles from the binner, then 4x4 blocks by trivial
+// reject / accept, then 4x2 groups, 2x2 quads and single pixels at 1 sample
+// per pixel, each class with its own depth expression (TRW:2244-2918).  With
+// MSAA every sample of a partially covered block is tested individually.
 //
 // Depth test.  The reference shades a tile's fragments in primitive order and
 // keeps a fragment when !(depth >= stored) (TRW:2286-2290, 2482-2486,
 // 2828-2832).  For finite depths the survivor is the lexicographic minimum of
-// (depth, prim * 4 + fan), which is order-independent: the tile's visibility
-// lives in LDS as 64-bit keys (ordered depth << 32 | prim * 4 + fan) and
-// fragments are merged with 64-bit LDS atomic min in any order.
+// (depth, prim * 4 + fan), which is order-independent: visibility is held as
+// 64-bit keys (ordered depth << 32 | prim * 4 + fan) merged with atomic min.
 //
-// Work split per tile list entry: triangles whose screen extent (a size hint
-// from setup) is at most 64 pixels are rasterised one per lane; larger ones
-// one pixel per lane, where each wave holds one row of 4x4 blocks so the 4x2
-// group and quad classification of a pixel comes from one ballot.  Both paths
-// make the same exact decisions; the split only balances the work.
+// Work expansion.  One lane per list entry runs the block tests and builds
+// the entry's candidate mask: pixels (MSAA) or 4x2 groups (1 sample) of the
+// blocks that pass trivial reject, pruned by a monotone corner bound of the
+// pixel test, which is exact.  A workgroup scan turns the masks into a flat
+// list of (entry, pixel or group) units in LDS, and every lane then takes
+// one unit: its sample tests, depths and LDS atomic-min merges.  Lanes stay
+// busy whatever the mix of triangle sizes, which one-triangle-per-lane loops
+// do not achieve.
+//
+// Scheduling.  A tile's list is cut into parts of at most kPart entries and
+// all parts form one work queue that a persistent grid drains, so a few
+// tiles holding thousands of sub-pixel triangles do not serialise on one
+// compute unit.  Each part rasterises into LDS; a tile with one part writes
+// its outputs from LDS, each part of a longer list stores its LDS keys to a
+// private partial block and a second kernel takes the minimum over the parts
+// and writes the outputs (it also clears tiles with an empty list).  No
+// workgroup waits on another and no device-scope fence is issued (on the
+// multi-XCD MI355X each one writes back an L2).
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"
@@ -29,155 +41,114 @@ namespace {
 
 using namespace dev;
 
-constexpr int kChunk = 256;
+constexpr int kPart = 256;    // list entries per work item
+constexpr int kUnits = 4096;  // units expanded into LDS per round
+
+// Depth at a 1-sample pixel for a coverage class (1: SIMD256, 2: SIMD128, 3: scalar).
+__device__ __forceinline__ float class_depth(unsigned c, const float4& F1, const float4& F2, const float4& F3, float x,
+                                             float y) {
+    return c == 1 ? depth256(F1, F2, F3, x, y) : (c == 2 ? depth128(F1, F2, F3, x, y) : depth_scalar(F1, F2, F3, x, y));
+}
 
 __device__ __forceinline__ void merge(unsigned long long* cell, float d, unsigned tie) {
     if (d == d) atomicMin(cell, frag_key(d, tie));
 }
 
-// One lane rasterises one small triangle into the tile's LDS visibility.
-// The lane first collects the 4x4 blocks that pass the reference's trivial
-// reject test (TRW:887-934) and then, per candidate block, the pixels whose
-// square may hold a covered point; it iterates only over set bits, so the
-// lanes of a wave stay busy with their own triangles instead of stepping
-// through the union of all lanes' blocks.  Both prunings are exact: the
-// block test is the reference's own, and square_may_cover() is a monotone
-// bound of the pixel test.
+// Rejects every integer point of [x, x + w] x [y, y + h] when an edge
+// excludes the corner that minimises it; rounding is monotone, so no such
+// point can pass inside() when this returns false.
+__device__ __forceinline__ bool rect_may_cover(const Edges& E, float x, float y, float w, float h) {
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float cx = E.x[k] < 0.0f ? x + w : x;
+        const float cy = E.y[k] < 0.0f ? y + h : y;
+        ok = ok && (fmaf(E.y[k], cy, E.x[k] * cx) < ifx::neg(E.z[k]));
+    }
+    return ok;
+}
+
+// Exclusive scan over a 256-thread workgroup; every thread must call it.
+__device__ __forceinline__ unsigned scan256(unsigned v, unsigned* sWave, unsigned& total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) sWave[w] = x;
+    __syncthreads();
+    unsigned base = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) base += i < w ? sWave[i] : 0u;
+    total = sWave[0] + sWave[1] + sWave[2] + sWave[3];
+    return base + x - v;
+}
+
+// Candidate mask of one list entry in one tile.  S > 1: 256 pixel bits
+// (bit y * 16 + x, eight words); S == 1: 32 group bits (block * 2 + row pair).
+// Also returns the trivially accepted blocks.
 template <int S>
-__device__ void raster_small(const RasterParams& P, const TriRec& R, int tx, int ty, unsigned long long* vis,
-                             unsigned long long& frags, unsigned long long& covs) {
-    const Edges E = load_edges(R);
-    const float4 F1 = R.f1, F2 = R.f2, F3 = R.f3;
-    const unsigned tie = __float_as_uint(R.q2.y);
+__device__ __forceinline__ void entry_mask(const Edges& E, bool full, int tx, int ty, const unsigned* fbMask,
+                                           unsigned* m, unsigned& ta) {
+    constexpr int NW = S == 1 ? 1 : 8;
     const float tileMinX = (float)(tx * 16) * 1.0f, tileMinY = (float)(ty * 16) * 1.0f;
-    const int tx0 = tx * 16, ty0 = ty * 16;
-    unsigned trMask = 0, taMask = 0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) m[w] = 0u;
+    if (full) {
+        ta = 0xFFFFu;
+#pragma unroll
+        for (int w = 0; w < NW; w++) m[w] = fbMask[w];
+        return;
+    }
+    unsigned tr = 0;
+    ta = 0;
 #pragma unroll
     for (int bi = 0; bi < 16; bi++) {
         int trn, tan;
         block_test(E, fmaf((float)(bi & 3), 4.0f, tileMinX), fmaf((float)(bi >> 2), 4.0f, tileMinY), trn, tan);
-        trMask |= (unsigned)(trn == 3) << bi;
-        taMask |= (unsigned)(tan == 3) << bi;
+        tr |= (unsigned)(trn == 3) << bi;
+        ta |= (unsigned)(tan == 3) << bi;
     }
-    while (trMask) {
-        const int bi = __ffs(trMask) - 1;
-        trMask &= trMask - 1u;
-        const bool accept = (taMask >> bi) & 1u;
-        const int px0 = tx0 + (bi & 3) * 4, py0 = ty0 + (bi >> 2) * 4;
-        if (S == 1) {
-            for (int sk = 0; sk < 2; sk++) {
-                const int gy = py0 + 2 * sk;
-                // lanes of the 4x2 group: (0,0) (1,0) (0,1) (1,1) (2,0) (3,0) (2,1) (3,1)
-                unsigned acc = 0xFFu;
-                if (!accept) {
-                    acc = 0;
-#pragma unroll
-                    for (int l = 0; l < 8; l++) {
-                        const int ox = (l & 1) + ((l >> 2) << 1), oy = (l >> 1) & 1;
-                        acc |= (unsigned)inside(E, (float)(px0 + ox), (float)(gy + oy)) << l;
-                    }
-                }
-                unsigned todo = acc;
-                while (todo) {
-                    const int l = __ffs(todo) - 1;
-                    todo &= todo - 1u;
-                    const int px = px0 + (l & 1) + ((l >> 2) << 1), py = gy + ((l >> 1) & 1);
-                    if (px >= P.width || py >= P.height) continue;
-                    const unsigned quad = (acc >> (l & 4)) & 0xFu;
-                    const float fx = (float)px, fy = (float)py;
-                    float d;
-                    if (acc == 0xFFu)
-                        d = depth256(F1, F2, F3, fx, fy);      // BLOCK or PIXEL_PACK4X2
-                    else if (quad == 0xFu)
-                        d = depth128(F1, F2, F3, fx, fy);      // PIXEL_PACK2X2
-                    else
-                        d = depth_scalar(F1, F2, F3, fx, fy);  // PIXEL
-                    frags++;
-                    covs++;
-                    merge(&vis[(py - ty0) * 16 + (px - tx0)], d, tie);
-                }
-            }
+    while (tr) {
+        const int bi = __ffs(tr) - 1;
+        tr &= tr - 1u;
+        const bool acc = (ta >> bi) & 1u;
+        const float x0 = (float)(tx * 16 + (bi & 3) * 4), y0 = (float)(ty * 16 + (bi >> 2) * 4);
+        if constexpr (S == 1) {
+            unsigned g = 3u;
+            if (!acc)
+                g = (unsigned)rect_may_cover(E, x0, y0, 3.0f, 1.0f) |
+                    ((unsigned)rect_may_cover(E, x0, y0 + 2.0f, 3.0f, 1.0f) << 1);
+            m[0] |= g << (bi * 2);
         } else {
-            unsigned pm = 0xFFFFu;
-            if (!accept) {
-                pm = 0;
+            // rows of the block live in words (bi >> 2) * 2 and + 1, at bit
+            // offset ((row & 1) * 16 + (bi & 3) * 4)
+            unsigned lo = 0xF000Fu, hi = 0xF000Fu;  // two rows each
+            if (!acc) {
+                lo = 0;
+                hi = 0;
 #pragma unroll
-                for (int q = 0; q < 16; q++)
-                    pm |= (unsigned)square_may_cover(E, (float)(px0 + (q & 3)), (float)(py0 + (q >> 2))) << q;
-            }
-            while (pm) {
-                const int q = __ffs(pm) - 1;
-                pm &= pm - 1u;
-                const int px = px0 + (q & 3), py = py0 + (q >> 2);
-                if (px >= P.width || py >= P.height) continue;
-                const float fx = (float)px, fy = (float)py;
-                bool any = false;
-                unsigned long long* cell = &vis[((py - ty0) * 16 + (px - tx0)) * S];
-#pragma unroll
-                for (int s = 0; s < S; s++) {
-                    const float sx = fx + P.sx[s], sy = fy + P.sy[s];
-                    if (!accept && !inside(E, sx, sy)) continue;
-                    any = true;
-                    covs++;
-                    merge(&cell[s], depth256(F1, F2, F3, sx, sy), tie);
+                for (int q = 0; q < 16; q++) {
+                    const bool c = rect_may_cover(E, x0 + (float)(q & 3), y0 + (float)(q >> 2), 1.0f, 1.0f);
+                    const unsigned bit = (unsigned)c << (((q >> 2) & 1) * 16 + (q & 3));
+                    if (q < 8) lo |= bit;
+                    else hi |= bit;
                 }
-                frags += any;
             }
-        }
-    }
-}
-
-// All 256 lanes rasterise one large triangle, one pixel each.  Each lane owns
-// its pixel's LDS cells in this phase.
-template <int S>
-__device__ __forceinline__ void raster_big(const RasterParams& P, const TriRec& R, bool full, float fpx, float fpy,
-                                           float bMinX, float bMinY, int gShift0, int qShift0, bool inFb,
-                                           unsigned long long* cell, unsigned long long& frags,
-                                           unsigned long long& covs) {
-    const Edges E = load_edges(R);
-    int trn = 3, tan = 3;
-    if (!full) block_test(E, bMinX, bMinY, trn, tan);
-    const float4 F1 = R.f1, F2 = R.f2, F3 = R.f3;
-    const unsigned tie = __float_as_uint(R.q2.y);
-    if constexpr (S == 1) {
-        const bool pin = inside(E, fpx, fpy);
-        const unsigned long long M = __ballot(pin);
-        unsigned c = 0;
-        if (trn == 3 && tan == 3) {
-            c = 1;
-        } else if (trn == 3) {
-            const bool g = ((M >> gShift0) & 0xFull) == 0xFull && ((M >> (gShift0 + 16)) & 0xFull) == 0xFull;
-            const bool qd = ((M >> qShift0) & 0x3ull) == 0x3ull && ((M >> (qShift0 + 16)) & 0x3ull) == 0x3ull;
-            c = g ? 1u : (qd ? 2u : (pin ? 3u : 0u));
-        }
-        if (c && inFb) {
-            const float d = c == 1 ? depth256(F1, F2, F3, fpx, fpy)
-                                   : (c == 2 ? depth128(F1, F2, F3, fpx, fpy) : depth_scalar(F1, F2, F3, fpx, fpy));
-            frags++;
-            covs++;
-            if (d == d) {
-                const unsigned long long k = frag_key(d, tie);
-                if (k < cell[0]) cell[0] = k;
-            }
-        }
-    } else {
-        if (trn != 3 || !inFb) return;
-        if (tan != 3 && !square_may_cover(E, fpx, fpy)) return;
-        bool any = false;
+            const int sh = (bi & 3) * 4;
 #pragma unroll
-        for (int s = 0; s < S; s++) {
-            const float sx = fpx + P.sx[s], sy = fpy + P.sy[s];
-            if (tan != 3 && !inside(E, sx, sy)) continue;
-            any = true;
-            covs++;
-            const float d = depth256(F1, F2, F3, sx, sy);
-            if (d == d) {
-                const unsigned long long k = frag_key(d, tie);
-                if (k < cell[s]) cell[s] = k;
+            for (int w = 0; w < 4; w++) {
+                if ((bi >> 2) == w) {
+                    m[2 * w] |= lo << sh;
+                    m[2 * w + 1] |= hi << sh;
+                }
             }
         }
-        frags += any;
     }
+#pragma unroll
+    for (int w = 0; w < NW; w++) m[w] &= fbMask[w];
 }
 
 // Coverage class and depth of the winning triangle at a 1-sample pixel,
@@ -209,134 +180,379 @@ __device__ __forceinline__ unsigned winner_class(const TriRec& R, int tx, int ty
             }
         }
     }
-    d = c == 1 ? depth256(R.f1, R.f2, R.f3, fpx, fpy)
-               : (c == 2 ? depth128(R.f1, R.f2, R.f3, fpx, fpy) : depth_scalar(R.f1, R.f2, R.f3, fpx, fpy));
+    d = class_depth(c, R.f1, R.f2, R.f3, fpx, fpy);
     return c;
 }
 
+struct PixelCtx {
+    int tx, ty, px, py;
+    bool inFb;
+    size_t pix;
+    float fpx, fpy, bMinX, bMinY;
+};
+
+__device__ __forceinline__ PixelCtx pixel_ctx(const RasterParams& P, int tile, int lx, int ly) {
+    PixelCtx C;
+    C.tx = tile % P.tilesX;
+    C.ty = tile / P.tilesX;
+    C.px = C.tx * 16 + lx;
+    C.py = C.ty * 16 + ly;
+    C.inFb = C.px < P.width && C.py < P.height;
+    C.pix = (size_t)C.py * P.width + C.px;
+    C.fpx = (float)C.px;
+    C.fpy = (float)C.py;
+    C.bMinX = fmaf((float)(lx >> 2), 4.0f, (float)(C.tx * 16) * 1.0f);
+    C.bMinY = fmaf((float)(ly >> 2), 4.0f, (float)(C.ty * 16) * 1.0f);
+    return C;
+}
+
+// Writes the outputs of one pixel from its S final keys.
 template <int S>
-__global__ void __launch_bounds__(256) k_raster(RasterParams P, TriRecs recs,
-                                                const unsigned* __restrict__ slotOf,
+__device__ __forceinline__ void write_pixel(const RasterParams& P, const TriRecs& recs, const unsigned* slotOf,
+                                            const PixelCtx& C, const unsigned long long* keys, uint32_t* visId,
+                                            float* visDepth, uint32_t* visSlot, uint8_t* visClass) {
+    if (!C.inFb) return;
+    unsigned cls = 0;
+    bool won = false;
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const unsigned long long k = keys[s];
+        if (k == frag_key(P.clear ? kClearDepth : visDepth[C.pix * S + s], 0u)) {
+            if (P.clear) {
+                visDepth[C.pix * S + s] = kClearDepth;
+                visSlot[C.pix * S + s] = 0xffffffffu;
+                visId[C.pix * S + s] = 0xffffffffu;
+            }
+            continue;
+        }
+        won = true;
+        const unsigned tie = (unsigned)k;
+        const unsigned prim = tie >> 2;
+        const unsigned slot = slotOf[prim] + (tie & 3u);
+        float d = order_key_inv((unsigned)(k >> 32));
+        if (S == 1) {
+            cls = winner_class(load_rec(recs, slot), C.tx, C.ty, C.fpx, C.fpy, C.bMinX, C.bMinY, d);
+        } else if ((unsigned)(k >> 32) == 0x80000000u) {
+            // +0 and -0 share a key; recompute to return the winner's own bits
+            d = depth256(recs.f1[slot], recs.f2[slot], recs.f3[slot], C.fpx + P.sx[s], C.fpy + P.sy[s]);
+        }
+        visDepth[C.pix * S + s] = d;
+        visSlot[C.pix * S + s] = slot;
+        visId[C.pix * S + s] = prim;
+    }
+    if (S == 1 && visClass && (won || P.clear)) visClass[C.pix] = (uint8_t)cls;
+}
+
+template <int S>
+__global__ void __launch_bounds__(256) k_raster(RasterParams P, TriRecs recs, const unsigned* __restrict__ slotOf,
                                                 const unsigned* __restrict__ binStart,
                                                 const unsigned* __restrict__ binCount,
-                                                const unsigned* __restrict__ binList, uint32_t* __restrict__ visId,
-                                                float* __restrict__ visDepth, uint32_t* __restrict__ visSlot,
-                                                uint8_t* __restrict__ visClass, StatPartials st,
-                                                DeviceCounters* ctr) {
-    __shared__ unsigned long long sVis[256 * S];
-    __shared__ TriRec sRec[kChunk];
-    __shared__ unsigned sEnt[kChunk];
-    __shared__ unsigned short sBig[kChunk];
-    __shared__ int sNumBig;
+                                                const unsigned* __restrict__ binList, const uint2* __restrict__ items,
+                                                const int* __restrict__ tilePartial,
+                                                unsigned long long* __restrict__ partials,
+                                                uint32_t* __restrict__ visId, float* __restrict__ visDepth,
+                                                uint32_t* __restrict__ visSlot, uint8_t* __restrict__ visClass,
+                                                StatPartials st, DeviceCounters* ctr) {
+    constexpr int NW = S == 1 ? 1 : 8;
+    __shared__ unsigned long long sVis[S * 256];  // key of sample s of tile pixel p at s * 256 + p
+    __shared__ TriRec sRec[kPart];
+    __shared__ unsigned short sTa[kPart];
+    __shared__ unsigned short sUnit[kUnits];
+    __shared__ unsigned sWave[4];
+    __shared__ unsigned sItem;
 
-    const int tile = blockIdx.x;
-    const int tx = tile % P.tilesX, ty = tile / P.tilesX;
     const int t = threadIdx.x;
-    const int lx = t & 15, ly = t >> 4;
-    const int px = tx * 16 + lx, py = ty * 16 + ly;
-    const bool inFb = px < P.width && py < P.height;
-    const size_t pix = (size_t)py * P.width + px;
-    const float fpx = (float)px, fpy = (float)py;
-    const float bMinX = fmaf((float)(lx >> 2), 4.0f, (float)(tx * 16) * 1.0f);
-    const float bMinY = fmaf((float)(ly >> 2), 4.0f, (float)(ty * 16) * 1.0f);
-    const int r = ly & 3;
-    const int gShift0 = (r & ~1) * 16 + (lx & ~3);
-    const int qShift0 = (r & ~1) * 16 + (lx & ~1);
-
-#pragma unroll
-    for (int s = 0; s < S; s++) sVis[t * S + s] = frag_key(P.clear || !inFb ? kClearDepth : visDepth[pix * S + s], 0u);
-
+    const unsigned numItems = ctr->numItems;
     unsigned long long frags = 0, covs = 0;
-    const unsigned start = binStart[tile];
-    const unsigned n = start >= P.binCap ? 0u : min(binCount[tile], P.binCap - start);
-    for (unsigned base = 0; base < n; base += kChunk) {
-        const int cnt = (int)min((unsigned)kChunk, n - base);
-        __syncthreads();
-        if (t == 0) sNumBig = 0;
-        TriRec R;
-        unsigned ent = 0;
-        if (t < cnt) {
-            ent = binList[start + base + t];
-            R = load_rec(recs, ent & 0x7fffffffu);
-            sRec[t] = R;
-            sEnt[t] = ent;
-        }
-        __syncthreads();
-        if (t < cnt) {
-            const bool full = (ent >> 31) != 0;
-            if (!full && !pix_box(R).big) {
-                raster_small<S>(P, R, tx, ty, sVis, frags, covs);
-            } else {
-                const int k = atomicAdd(&sNumBig, 1);
-                sBig[k] = (unsigned short)t;
-            }
-        }
-        __syncthreads();
-        const int nb = sNumBig;
-        for (int b = 0; b < nb; b++) {
-            const int j = sBig[b];
-            raster_big<S>(P, sRec[j], (sEnt[j] >> 31) != 0, fpx, fpy, bMinX, bMinY, gShift0, qShift0, inFb,
-                          &sVis[t * S], frags, covs);
-        }
-    }
-    __syncthreads();
 
-    if (inFb) {
-        unsigned cls = 0;
+    for (;;) {
+        if (t == 0) sItem = atomicAdd(&ctr->itemHead, 1u);
+        __syncthreads();
+        const unsigned item = sItem;
+        if (item >= numItems) break;
+        const uint2 it = items[item];
+        const int tile = (int)it.x, part = (int)it.y;
+        const PixelCtx C = pixel_ctx(P, tile, t & 15, t >> 4);
+        const int tx0 = C.tx * 16, ty0 = C.ty * 16;
+        const unsigned start = binStart[tile];
+        const unsigned total = start >= P.binCap ? 0u : min(binCount[tile], P.binCap - start);
+        const unsigned base = (unsigned)part * kPart;
+        const int cnt = (int)min((unsigned)kPart, total > base ? total - base : 0u);
+        const int pbase = tilePartial[tile];
+
 #pragma unroll
-        for (int s = 0; s < S; s++) {
-            const unsigned long long k = sVis[t * S + s];
-            if (k == frag_key(P.clear ? kClearDepth : visDepth[pix * S + s], 0u)) {
-                if (P.clear) {
-                    visDepth[pix * S + s] = kClearDepth;
-                    visSlot[pix * S + s] = 0xffffffffu;
-                    visId[pix * S + s] = 0xffffffffu;
-                }
-                continue;
-            }
-            const unsigned tie = (unsigned)k;
-            const unsigned prim = tie >> 2;
-            const unsigned slot = slotOf[prim] + (tie & 3u);
-            float d = order_key_inv((unsigned)(k >> 32));
-            if (S == 1) {
-                cls = winner_class(load_rec(recs, slot), tx, ty, fpx, fpy, bMinX, bMinY, d);
-            } else if ((unsigned)(k >> 32) == 0x80000000u) {
-                // +0 and -0 share a key; recompute to return the winner's own bits
-                d = depth256(recs.f1[slot], recs.f2[slot], recs.f3[slot], fpx + P.sx[s], fpy + P.sy[s]);
-            }
-            visDepth[pix * S + s] = d;
-            visSlot[pix * S + s] = slot;
-            visId[pix * S + s] = prim;
+        for (int s = 0; s < S; s++)
+            sVis[s * 256 + t] = frag_key(P.clear || !C.inFb ? kClearDepth : visDepth[C.pix * S + s], 0u);
+
+        // framebuffer mask of the tile in the candidate layout
+        const int cols = min(16, P.width - tx0), rows = min(16, P.height - ty0);
+        unsigned fbMask[NW];
+        if constexpr (S == 1) {
+            // groups with at least one pixel inside; pixels are checked again per unit
+            unsigned f = 0;
+#pragma unroll
+            for (int g = 0; g < 32; g++)
+                f |= (unsigned)(((g >> 1) & 3) * 4 < cols && ((g >> 3) * 4 + (g & 1) * 2) < rows) << g;
+            fbMask[0] = f;
+        } else {
+            const unsigned row = cols >= 16 ? 0xFFFFu : ((1u << cols) - 1u);
+#pragma unroll
+            for (int w = 0; w < 8; w++)
+                fbMask[w] = (2 * w < rows ? row : 0u) | (2 * w + 1 < rows ? row << 16 : 0u);
         }
-        if (S == 1 && visClass && (cls || P.clear)) visClass[pix] = (uint8_t)cls;
+
+        unsigned m[NW];
+        unsigned n = 0;
+        if (t < cnt) {
+            const unsigned ent = binList[start + base + t];
+            const TriRec R = load_rec(recs, ent & 0x7fffffffu);
+            sRec[t] = R;
+            unsigned ta;
+            entry_mask<S>(load_edges(R), (ent >> 31) != 0, C.tx, C.ty, fbMask, m, ta);
+            sTa[t] = (unsigned short)ta;
+#pragma unroll
+            for (int w = 0; w < NW; w++) n += __popc(m[w]);
+        } else {
+#pragma unroll
+            for (int w = 0; w < NW; w++) m[w] = 0u;
+        }
+        unsigned T;
+        const unsigned off = scan256(n, sWave, T);
+
+        for (unsigned w0 = 0; w0 < T; w0 += kUnits) {
+            // write this lane's units that fall in [w0, w0 + kUnits)
+            int pos = (int)off - (int)w0;
+            if (n && pos < kUnits && pos + (int)n > 0) {
+#pragma unroll
+                for (int w = 0; w < NW; w++) {
+                    unsigned bits = m[w];
+                    const int c = __popc(bits);
+                    if (pos + c <= 0 || pos >= kUnits) {
+                        pos += c;
+                        continue;
+                    }
+                    while (bits) {
+                        const int b = __ffs(bits) - 1;
+                        bits &= bits - 1u;
+                        if (pos >= 0 && pos < kUnits) sUnit[pos] = (unsigned short)((t << 8) | (w * 32 + b));
+                        pos++;
+                    }
+                }
+            }
+            __syncthreads();
+            const int nu = (int)min((unsigned)kUnits, T - w0);
+            for (int i = t; i < nu; i += 256) {
+                const unsigned u = sUnit[i];
+                const int e = (int)(u >> 8), c = (int)(u & 255u);
+                const TriRec R = sRec[e];
+                const Edges E = load_edges(R);
+                const unsigned tie = __float_as_uint(R.q2.y);
+                const unsigned ta = sTa[e];
+                if constexpr (S == 1) {
+                    const int bi = c >> 1;
+                    const int px0 = tx0 + (bi & 3) * 4, gy = ty0 + (bi >> 2) * 4 + (c & 1) * 2;
+                    // pixels of the 4x2 group: (0,0) (1,0) (0,1) (1,1) (2,0) (3,0) (2,1) (3,1)
+                    unsigned acc = 0xFFu;
+                    if (!((ta >> bi) & 1u)) {
+                        acc = 0;
+#pragma unroll
+                        for (int l = 0; l < 8; l++) {
+                            const int ox = (l & 1) + ((l >> 2) << 1), oy = (l >> 1) & 1;
+                            acc |= (unsigned)inside(E, (float)(px0 + ox), (float)(gy + oy)) << l;
+                        }
+                    }
+#pragma unroll
+                    for (int l = 0; l < 8; l++) {
+                        if (!((acc >> l) & 1u)) continue;
+                        const int px = px0 + (l & 1) + ((l >> 2) << 1), py = gy + ((l >> 1) & 1);
+                        if (px >= P.width || py >= P.height) continue;
+                        const unsigned quad = (acc >> (l & 4)) & 0xFu;
+                        const unsigned cls = acc == 0xFFu ? 1u : (quad == 0xFu ? 2u : 3u);
+                        frags++;
+                        covs++;
+                        merge(&sVis[(py - ty0) * 16 + (px - tx0)],
+                              class_depth(cls, R.f1, R.f2, R.f3, (float)px, (float)py), tie);
+                    }
+                } else {
+                    const int lx = c & 15, ly = c >> 4;
+                    const bool accept = (ta >> ((ly >> 2) * 4 + (lx >> 2))) & 1u;
+                    const float fx = (float)(tx0 + lx), fy = (float)(ty0 + ly);
+                    bool any = false;
+#pragma unroll
+                    for (int s = 0; s < S; s++) {
+                        const float sx = fx + P.sx[s], sy = fy + P.sy[s];
+                        if (!accept && !inside(E, sx, sy)) continue;
+                        any = true;
+                        covs++;
+                        merge(&sVis[s * 256 + c], depth256(R.f1, R.f2, R.f3, sx, sy), tie);
+                    }
+                    frags += any;
+                }
+            }
+            __syncthreads();
+        }
+        __syncthreads();
+
+        if (pbase < 0) {
+            unsigned long long keys[S];
+#pragma unroll
+            for (int s = 0; s < S; s++) keys[s] = sVis[s * 256 + t];
+            write_pixel<S>(P, recs, slotOf, C, keys, visId, visDepth, visSlot, visClass);
+        } else {
+            unsigned long long* g = partials + ((size_t)(pbase + part) * 256 + t) * S;
+#pragma unroll
+            for (int s = 0; s < S; s++) g[s] = sVis[s * 256 + t];
+        }
     }
     const unsigned long long tf = block_sum256(frags);
     const unsigned long long tc = block_sum256(covs);
     if (t == 0) {
-        st.rasterFrags[tile] = tf;
-        st.rasterCovs[tile] = tc;
+        st.rasterFrags[blockIdx.x] = tf;
+        st.rasterCovs[blockIdx.x] = tc;
     }
+}
+
+// Minimum over the parts of every tile whose list was cut, then the outputs;
+// also the clear values of tiles with an empty list (when clearing).
+template <int S>
+__global__ void __launch_bounds__(256) k_raster_merge(RasterParams P, TriRecs recs, const unsigned* __restrict__ slotOf,
+                                                      const unsigned* __restrict__ binStart,
+                                                      const unsigned* __restrict__ binCount,
+                                                      const unsigned* __restrict__ multiList,
+                                                      const int* __restrict__ tilePartial,
+                                                      const unsigned long long* __restrict__ partials,
+                                                      uint32_t* __restrict__ visId, float* __restrict__ visDepth,
+                                                      uint32_t* __restrict__ visSlot, uint8_t* __restrict__ visClass,
+                                                      int tiles, const DeviceCounters* ctr) {
+    const unsigned numMulti = ctr->numMulti;
+    const unsigned numWork = numMulti + (P.clear ? ctr->numEmpty : 0u);
+    const int t = threadIdx.x;
+    for (unsigned m = blockIdx.x; m < numWork; m += gridDim.x) {
+        if (m >= numMulti) {
+            const PixelCtx C = pixel_ctx(P, (int)multiList[tiles + (m - numMulti)], t & 15, t >> 4);
+            unsigned long long keys[S];
+#pragma unroll
+            for (int s = 0; s < S; s++) keys[s] = frag_key(kClearDepth, 0u);
+            write_pixel<S>(P, recs, slotOf, C, keys, visId, visDepth, visSlot, visClass);
+            continue;
+        }
+        const int tile = (int)multiList[m];
+        const PixelCtx C = pixel_ctx(P, tile, t & 15, t >> 4);
+        const unsigned start = binStart[tile];
+        const unsigned total = start >= P.binCap ? 0u : min(binCount[tile], P.binCap - start);
+        const unsigned nparts = (total + kPart - 1) / kPart;
+        const unsigned long long* g = partials + ((size_t)tilePartial[tile] * 256 + t) * S;
+        unsigned long long keys[S];
+#pragma unroll
+        for (int s = 0; s < S; s++) keys[s] = g[s];
+        for (unsigned p = 1; p < nparts; p++) {
+            g += 256 * S;
+#pragma unroll
+            for (int s = 0; s < S; s++) keys[s] = min(keys[s], g[s]);
+        }
+        write_pixel<S>(P, recs, slotOf, C, keys, visId, visDepth, visSlot, visClass);
+    }
+}
+
+// Builds the work items (parts of at most kPart entries per non-empty tile);
+// for tiles with several parts, the first partial block and the merge list
+// (multiList[0, numMulti)); the empty tiles (multiList[tiles, tiles + numEmpty)).
+__global__ void __launch_bounds__(1024) k_raster_plan(const unsigned* __restrict__ binStart,
+                                                      const unsigned* __restrict__ binCount, int tiles,
+                                                      unsigned binCap, unsigned itemCap, unsigned partialCap,
+                                                      uint2* __restrict__ items, int* __restrict__ tilePartial,
+                                                      unsigned* __restrict__ multiList, DeviceCounters* ctr) {
+    __shared__ unsigned sA[1024], sB[1024], sM[1024], sE[1024];
+    const int per = (tiles + 1023) / 1024;
+    const int b0 = threadIdx.x * per;
+    unsigned a = 0, b = 0, m = 0, e = 0;  // items, partial blocks, tiles with several parts, empty tiles
+    for (int i = 0; i < per; i++) {
+        const int t = b0 + i;
+        if (t >= tiles) break;
+        const unsigned st = binStart[t];
+        const unsigned total = st >= binCap ? 0u : min(binCount[t], binCap - st);
+        const unsigned parts = (total + kPart - 1) / kPart;
+        a += parts;
+        b += parts > 1u ? parts : 0u;
+        m += parts > 1u;
+        e += parts == 0u;
+    }
+    sA[threadIdx.x] = a;
+    sB[threadIdx.x] = b;
+    sM[threadIdx.x] = m;
+    sE[threadIdx.x] = e;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const unsigned va = threadIdx.x >= (unsigned)off ? sA[threadIdx.x - off] : 0u;
+        const unsigned vb = threadIdx.x >= (unsigned)off ? sB[threadIdx.x - off] : 0u;
+        const unsigned vm = threadIdx.x >= (unsigned)off ? sM[threadIdx.x - off] : 0u;
+        const unsigned ve = threadIdx.x >= (unsigned)off ? sE[threadIdx.x - off] : 0u;
+        __syncthreads();
+        sA[threadIdx.x] += va;
+        sB[threadIdx.x] += vb;
+        sM[threadIdx.x] += vm;
+        sE[threadIdx.x] += ve;
+        __syncthreads();
+    }
+    unsigned ra = sA[threadIdx.x] - a, rb = sB[threadIdx.x] - b, rm = sM[threadIdx.x] - m, re = sE[threadIdx.x] - e;
+    const bool fits = sA[1023] <= itemCap && sB[1023] <= partialCap;
+    for (int i = 0; i < per; i++) {
+        const int t = b0 + i;
+        if (t >= tiles) break;
+        const unsigned st = binStart[t];
+        const unsigned total = st >= binCap ? 0u : min(binCount[t], binCap - st);
+        const unsigned parts = (total + kPart - 1) / kPart;
+        if (fits) {
+            for (unsigned p = 0; p < parts; p++) items[ra + p] = make_uint2((unsigned)t, p);
+            if (parts > 1u) multiList[rm] = (unsigned)t;
+            if (parts == 0u) multiList[tiles + re] = (unsigned)t;
+        }
+        tilePartial[t] = parts > 1u ? (int)rb : -1;
+        ra += parts;
+        rb += parts > 1u ? parts : 0u;
+        rm += parts > 1u;
+        re += parts == 0u;
+    }
+    if (threadIdx.x == 1023) {
+        // on overflow nothing is rasterised; the host grows the buffers and redraws
+        ctr->numItems = fits ? sA[1023] : 0u;
+        ctr->numMulti = fits ? sM[1023] : 0u;
+        ctr->numEmpty = fits ? sE[1023] : 0u;
+        ctr->numPartials = sB[1023];
+        ctr->itemHead = 0u;
+        if (!fits) atomicOr(&ctr->status, kStatusRasterOverflow);
+    }
+}
+
+template <int S>
+void launch_s(const RasterParams& p, TriRecs recs, const unsigned* slotOf, const unsigned* binStart,
+              const unsigned* binCount, const unsigned* binList, const RasterQueue& q, uint32_t* visId,
+              float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st, DeviceCounters* ctr,
+              hipStream_t s) {
+    const int tiles = p.tilesX * p.tilesY;
+    hipLaunchKernelGGL(k_raster_plan, dim3(1), dim3(1024), 0, s, binStart, binCount, tiles, p.binCap, q.itemCap,
+                       q.partialCap, q.items, q.tilePartial, q.multiList, ctr);
+    hipLaunchKernelGGL(k_raster<S>, dim3(q.grid), dim3(256), 0, s, p, recs, slotOf, binStart, binCount, binList,
+                       q.items, q.tilePartial, q.partials, visId, visDepth, visSlot, visClass, st, ctr);
+    hipLaunchKernelGGL(k_raster_merge<S>, dim3(q.grid), dim3(256), 0, s, p, recs, slotOf, binStart, binCount,
+                       q.multiList, q.tilePartial, q.partials, visId, visDepth, visSlot, visClass, tiles, ctr);
 }
 
 }  // namespace
 
-void launch_raster(const RasterParams& p, int samples, TriRecs recs, const unsigned* slotOf,
-                   const unsigned* binStart, const unsigned* binCount, const unsigned* binList, uint32_t* visId,
+void launch_raster(const RasterParams& p, int samples, TriRecs recs, const unsigned* slotOf, const unsigned* binStart,
+                   const unsigned* binCount, const unsigned* binList, const RasterQueue& q, uint32_t* visId,
                    float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st, DeviceCounters* ctr,
                    hipStream_t s) {
-    const dim3 grid(p.tilesX * p.tilesY);
     switch (samples) {
         case 1:
-            hipLaunchKernelGGL(k_raster<1>, grid, dim3(256), 0, s, p, recs, slotOf, binStart, binCount, binList, visId,
-                               visDepth, visSlot, visClass, st, ctr);
+            launch_s<1>(p, recs, slotOf, binStart, binCount, binList, q, visId, visDepth, visSlot, visClass, st, ctr, s);
             break;
         case 4:
-            hipLaunchKernelGGL(k_raster<4>, grid, dim3(256), 0, s, p, recs, slotOf, binStart, binCount, binList, visId,
-                               visDepth, visSlot, visClass, st, ctr);
+            launch_s<4>(p, recs, slotOf, binStart, binCount, binList, q, visId, visDepth, visSlot, visClass, st, ctr, s);
             break;
         default:
-            hipLaunchKernelGGL(k_raster<8>, grid, dim3(256), 0, s, p, recs, slotOf, binStart, binCount, binList, visId,
-                               visDepth, visSlot, visClass, st, ctr);
+            launch_s<8>(p, recs, slotOf, binStart, binCount, binList, q, visId, visDepth, visSlot, visClass, st, ctr, s);
             break;
     }
 }
