@@ -480,7 +480,7 @@ void TileRasterRenderer::enqueue(bool clear) {
         sh.offUv = d.offUv;
         sh.trisPerMaterial = d.trisPerMaterial;
         sh.sampler = d.sampler;
-        sh.maxAniso = std::max(1, d.maxAniso);
+        sh.maxAniso = std::min(16, std::max(1, d.maxAniso));  // the tap list encodes up to 16 taps
         sh.numTextures = (int)textures_.size();
         for (int k = 0; k < 3; k++) sh.camera[k] = d.camera[k];
         sh.ambient = d.ambient;

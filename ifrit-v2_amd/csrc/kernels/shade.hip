@@ -86,27 +86,34 @@ __device__ float4 trilinear(const uint8_t* pool, const TexDesc& D, float u, floa
     return lerp4(a, bilinear(pool, D, l1, u, v), f);
 }
 
-__device__ float4 sample_texture(const uint8_t* pool, const TexDesc& D, int sampler, int maxAniso, float u, float v,
-                                 float dudx, float dvdx, float dudy, float dvdy) {
-    if (sampler == 0) return bilinear(pool, D, 0, u, v);
+// Footprint of a textured fragment: number of taps along the major axis,
+// the level of detail and the tap step (SBC:244-272; trilinear and bilinear
+// are the one-tap cases).
+struct TexFootprint {
+    float au, av, lod;
+    int n;
+};
+
+__device__ __forceinline__ TexFootprint footprint(const TexDesc& D, int sampler, int maxAniso, float dudx, float dvdx,
+                                                  float dudy, float dvdy) {
+    TexFootprint F{0.0f, 0.0f, 0.0f, 1};
+    if (sampler == 0) return F;  // bilinear on level 0
     const float W = (float)D.width, H = (float)D.height;
     const float px = sqrtf((dudx * W) * (dudx * W) + (dvdx * H) * (dvdx * H));
     const float py = sqrtf((dudy * W) * (dudy * W) + (dvdy * H) * (dvdy * H));
     const float pmax = fmaxf(px, py), pmin = fminf(px, py);
-    if (sampler == 1 || !(pmax > 0.0f)) return trilinear(pool, D, u, v, pmax > 0.0f ? log2f(pmax) : 0.0f);
+    if (sampler == 1 || !(pmax > 0.0f)) {
+        F.lod = pmax > 0.0f ? log2f(pmax) : 0.0f;
+        return F;
+    }
     const float ratio = pmin > 0.0f ? pmax / pmin : (float)maxAniso;
     int n = ratio >= (float)maxAniso ? maxAniso : (int)ceilf(ratio);
-    n = max(1, n);
-    const float lod = log2f(pmax / (float)n);
-    const float au = px >= py ? dudx : dudy, av = px >= py ? dvdx : dvdy;
-    float4 acc = make_float4(0, 0, 0, 0);
-    const float invn = 1.0f / (float)n;
-    for (int i = 0; i < n; i++) {
-        const float o = ((float)i + 0.5f) * invn - 0.5f;
-        const float4 c = trilinear(pool, D, fmaf(au, o, u), fmaf(av, o, v), lod);
-        acc.x += c.x; acc.y += c.y; acc.z += c.z; acc.w += c.w;
-    }
-    return make_float4(acc.x * invn, acc.y * invn, acc.z * invn, acc.w * invn);
+    n = min(max(1, n), 16);  // the host clamps maxAniso to 16; the tap list holds 4-bit tap indices
+    F.n = n;
+    F.lod = log2f(pmax / (float)n);
+    F.au = px >= py ? dudx : dudy;
+    F.av = px >= py ? dvdx : dvdy;
+    return F;
 }
 
 __device__ __forceinline__ uint32_t flat_hash(uint32_t prim) {
@@ -123,14 +130,30 @@ __device__ __forceinline__ float3 norm3(float3 a) {
     return make_float3(a.x * inv, a.y * inv, a.z * inv);
 }
 
-__device__ float4 shade_fragment(const ShadeParams& P, const ShadeTri& T, const float* vtx, const int* idx,
-                                 const TexDesc* tex, const uint8_t* pool, int px, int py, float o) {
+// Everything of a fragment's colour except its albedo: colour = albedo.rgb *
+// A + B, alpha = albedo.a (Blinn-Phong: A = ambient + sum att * light * kd *
+// n.l, B = sum att * light * ks * (n.h)^shininess).  Also the texture
+// coordinates and footprint when the fragment samples a texture.
+struct FragPrep {
+    float3 A, B;
+    float u, v;
+    TexFootprint fp;
+    int mat;  // texture index, -1 when untextured
+};
+
+__device__ FragPrep prep_fragment(const ShadeParams& P, const ShadeTri& T, const float* vtx, const int* idx,
+                                  const TexDesc* tex, int px, int py, float o) {
+    FragPrep R;
+    R.mat = -1;
+    R.u = R.v = 0.0f;
+    R.fp = TexFootprint{0.0f, 0.0f, 0.0f, 0};
     if (P.program == 0) {
         const uint32_t h = flat_hash((uint32_t)T.prim);
-        return make_float4((h & 0xff) * (1.0f / 255.0f), ((h >> 8) & 0xff) * (1.0f / 255.0f),
-                           ((h >> 16) & 0xff) * (1.0f / 255.0f), 1.0f);
+        R.A = make_float3(0, 0, 0);
+        R.B = make_float3((h & 0xff) * (1.0f / 255.0f), ((h >> 8) & 0xff) * (1.0f / 255.0f),
+                          ((h >> 16) & 0xff) * (1.0f / 255.0f));
+        return R;
     }
-    if (P.program != 1) return make_float4(0, 0, 0, 0);
     const float x = (float)px + o, y = (float)py + o;
     const float* v0 = vtx + (size_t)idx[3 * T.prim] * P.stride;
     const float* v1 = vtx + (size_t)idx[3 * T.prim + 1] * P.stride;
@@ -140,9 +163,9 @@ __device__ float4 shade_fragment(const ShadeParams& P, const ShadeTri& T, const 
     auto interp = [&](int off) { return fmaf(w[0], v0[off], fmaf(w[1], v1[off], w[2] * v2[off])); };
     const float3 Pw = make_float3(interp(P.offWorld), interp(P.offWorld + 1), interp(P.offWorld + 2));
     float3 N = make_float3(interp(P.offNormal), interp(P.offNormal + 1), interp(P.offNormal + 2));
-    const float u = interp(P.offUv), v = interp(P.offUv + 1);
-    float4 alb = make_float4(1, 1, 1, 1);
     if (P.numTextures > 0) {
+        R.u = interp(P.offUv);
+        R.v = interp(P.offUv + 1);
         // fine quad derivatives (SBC:36-76): same triangle at the quad's other shading positions
         const float qx = (float)(px & ~1) + o, qy = (float)(py & ~1) + o;
         float wa[3], wb[3], wc[3], wd[3];
@@ -154,12 +177,12 @@ __device__ float4 shade_fragment(const ShadeParams& P, const ShadeTri& T, const 
         const float dudx = iuv(wb, P.offUv) - iuv(wa, P.offUv), dvdx = iuv(wb, P.offUv + 1) - iuv(wa, P.offUv + 1);
         const float dudy = iuv(wd, P.offUv) - iuv(wc, P.offUv), dvdy = iuv(wd, P.offUv + 1) - iuv(wc, P.offUv + 1);
         int mat = P.trisPerMaterial > 0 ? T.prim / P.trisPerMaterial : 0;
-        mat = min(mat, P.numTextures - 1);
-        alb = sample_texture(pool, tex[mat], P.sampler, P.maxAniso, u, v, dudx, dvdx, dudy, dvdy);
+        R.mat = min(mat, P.numTextures - 1);
+        R.fp = footprint(tex[R.mat], P.sampler, P.maxAniso, dudx, dvdx, dudy, dvdy);
     }
     N = norm3(N);
     const float3 V = norm3(make_float3(P.camera[0] - Pw.x, P.camera[1] - Pw.y, P.camera[2] - Pw.z));
-    float3 col = make_float3(alb.x * P.ambient, alb.y * P.ambient, alb.z * P.ambient);
+    float3 A = make_float3(P.ambient, P.ambient, P.ambient), B = make_float3(0, 0, 0);
     for (int l = 0; l < P.numLights; l++) {
         float3 L;
         float att = 1.0f;
@@ -177,50 +200,181 @@ __device__ float4 shade_fragment(const ShadeParams& P, const ShadeTri& T, const 
             const float3 Hh = norm3(make_float3(L.x + V.x, L.y + V.y, L.z + V.z));
             const float ndh = fmaxf(N.x * Hh.x + N.y * Hh.y + N.z * Hh.z, 0.0f);
             const float spec = P.ks * powf(ndh, P.shininess);
-            col.x += att * P.lightColor[l][0] * (P.kd * ndl * alb.x + spec);
-            col.y += att * P.lightColor[l][1] * (P.kd * ndl * alb.y + spec);
-            col.z += att * P.lightColor[l][2] * (P.kd * ndl * alb.z + spec);
+            const float d = att * P.kd * ndl;
+            A.x = fmaf(d, P.lightColor[l][0], A.x);
+            A.y = fmaf(d, P.lightColor[l][1], A.y);
+            A.z = fmaf(d, P.lightColor[l][2], A.z);
+            B.x = fmaf(att * spec, P.lightColor[l][0], B.x);
+            B.y = fmaf(att * spec, P.lightColor[l][1], B.y);
+            B.z = fmaf(att * spec, P.lightColor[l][2], B.z);
         }
     }
-    return make_float4(col.x, col.y, col.z, alb.w);
+    R.A = A;
+    R.B = B;
+    return R;
 }
 
 __device__ __forceinline__ float sat(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
 
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(v, off, 64);
+        if (lane >= off) v += y;
+    }
+    return v;
+}
+
+constexpr int kTapWindow = 128;
+
+// One wave per 8x8 pixel tile.  Work is expanded twice so that lanes stay
+// busy whatever the mix of edge pixels and sampling footprints:
+//   1. each pixel lists its distinct triangles (first-occurrence order, with
+//      their sample counts) into one flat fragment list;
+//   2. each lane prepares one fragment (interpolation, derivatives, lighting)
+//      and its texture taps go into one flat tap list, one tap per lane;
+//   3. the fragment lane sums its taps in tap order, the pixel lane sums its
+//      fragments in sample order, so results do not depend on scheduling.
 template <int S>
-__global__ void __launch_bounds__(256) k_shade(ShadeParams P, TriRecs recs,
-                                               const float4* __restrict__ bary, const uint32_t* __restrict__ visSlot,
-                                               const float* __restrict__ vtx, const int* __restrict__ idx,
-                                               const TexDesc* __restrict__ tex, const uint8_t* __restrict__ pool,
-                                               uchar4* __restrict__ rgba8, float4* __restrict__ rgba32f) {
-    const int px = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int py = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (px >= P.width || py >= P.height) return;
+__global__ void __launch_bounds__(64) k_shade(ShadeParams P, TriRecs recs, const float4* __restrict__ bary,
+                                              const uint32_t* __restrict__ visSlot, const float* __restrict__ vtx,
+                                              const int* __restrict__ idx, const TexDesc* __restrict__ tex,
+                                              const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8,
+                                              float4* __restrict__ rgba32f, int tilesX, int tiles, int perXcd) {
+    __shared__ unsigned fSlot[64 * S];
+    __shared__ unsigned char fCnt[64 * S];
+    __shared__ unsigned char fPix[64 * S];
+    __shared__ float4 cUV[64];      // u, v, au, av of the chunk's fragments
+    __shared__ float cLod[64];
+    __shared__ int cN[64];
+    __shared__ int cMat[64];
+    __shared__ float4 cCol[64];     // clamped colour of the chunk's fragments
+    __shared__ unsigned short tList[kTapWindow];
+    __shared__ float4 tCol[kTapWindow];
+
+    // consecutive tiles on one XCD share its L2 (texture and vertex reuse)
+    const int tile = (int)(blockIdx.x & 7u) * perXcd + (int)(blockIdx.x >> 3);
+    if (tile >= tiles) return;
+    const int lane = threadIdx.x;
+    const int px = (tile % tilesX) * 8 + (lane & 7);
+    const int py = (tile / tilesX) * 8 + (lane >> 3);
+    const bool inFb = px < P.width && py < P.height;
     const size_t pix = (size_t)py * P.width + px;
+
+    // 1. distinct triangles of the pixel
     unsigned ids[S];
 #pragma unroll
-    for (int s = 0; s < S; s++) ids[s] = visSlot[pix * S + s];
-    const float o = S == 1 ? 0.0f : 0.5f;
-    float4 sum = make_float4(0, 0, 0, 0);
+    for (int s = 0; s < S; s++) ids[s] = inFb ? visSlot[pix * S + s] : 0xffffffffu;
+    int nf = 0;
 #pragma unroll
     for (int s = 0; s < S; s++) {
-        if (ids[s] == 0xffffffffu) continue;
-        bool seen = false;
+        bool first = ids[s] != 0xffffffffu;
 #pragma unroll
-        for (int q = 0; q < s; q++) seen |= ids[q] == ids[s];
-        if (seen) continue;
-        int count = 0;
-#pragma unroll
-        for (int q = 0; q < S; q++) count += ids[q] == ids[s];
-        ShadeTri T;
-        load_shade_tri(recs, bary, P.ntri, ids[s], T);
-        const float4 c = shade_fragment(P, T, vtx, idx, tex, pool, px, py, o);
-        const float k = (float)count;
-        sum.x += sat(c.x) * k;
-        sum.y += sat(c.y) * k;
-        sum.z += sat(c.z) * k;
-        sum.w += sat(c.w) * k;
+        for (int q = 0; q < s; q++) first = first && ids[q] != ids[s];
+        nf += first;
     }
+    const int incl = wave_incl_scan(nf);
+    const int poff = incl - nf;
+    const int F = __shfl(incl, 63, 64);
+    {
+        int k = poff;
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            bool first = ids[s] != 0xffffffffu;
+#pragma unroll
+            for (int q = 0; q < s; q++) first = first && ids[q] != ids[s];
+            if (first) {
+                int count = 0;
+#pragma unroll
+                for (int q = 0; q < S; q++) count += ids[q] == ids[s];
+                fSlot[k] = ids[s];
+                fCnt[k] = (unsigned char)count;
+                fPix[k] = (unsigned char)lane;
+                k++;
+            }
+        }
+    }
+    __syncthreads();
+
+    const float o = S == 1 ? 0.0f : 0.5f;
+    float4 sum = make_float4(0, 0, 0, 0);
+    for (int c0 = 0; c0 < F; c0 += 64) {
+        // 2. prepare one fragment per lane
+        const int fi = c0 + lane;
+        const bool act = fi < F;
+        FragPrep R;
+        int n = 0;
+        if (act) {
+            const unsigned slot = fSlot[fi];
+            const int fl = fPix[fi];
+            ShadeTri T;
+            load_shade_tri(recs, bary, P.ntri, slot, T);
+            const int qx = (tile % tilesX) * 8 + (fl & 7), qy = (tile / tilesX) * 8 + (fl >> 3);
+            R = prep_fragment(P, T, vtx, idx, tex, qx, qy, o);
+            if (R.mat >= 0) {
+                n = R.fp.n;
+                cUV[lane] = make_float4(R.u, R.v, R.fp.au, R.fp.av);
+                cLod[lane] = R.fp.lod;
+                cN[lane] = n;
+                cMat[lane] = R.mat;
+            }
+        }
+        const int tincl = wave_incl_scan(n);
+        const int toff = tincl - n;
+        const int TT = __shfl(tincl, 63, 64);
+        float4 acc = make_float4(0, 0, 0, 0);
+        for (int w0 = 0; w0 < TT; w0 += kTapWindow) {
+            __syncthreads();
+            for (int i = 0; i < n; i++) {
+                const int pos = toff + i - w0;
+                if (pos >= 0 && pos < kTapWindow) tList[pos] = (unsigned short)((lane << 4) | i);
+            }
+            __syncthreads();
+            const int nt = min(kTapWindow, TT - w0);
+            for (int j = lane; j < nt; j += 64) {
+                const unsigned e = tList[j];
+                const int fl = (int)(e >> 4), ti = (int)(e & 15u);
+                const float4 uv = cUV[fl];
+                const int tn = cN[fl];
+                const float invn = 1.0f / (float)tn;
+                const float to = ((float)ti + 0.5f) * invn - 0.5f;
+                tCol[j] = trilinear(pool, tex[cMat[fl]], fmaf(uv.z, to, uv.x), fmaf(uv.w, to, uv.y), cLod[fl]);
+            }
+            __syncthreads();
+            const int b0 = max(toff - w0, 0), b1 = min(toff + n - w0, kTapWindow);
+            for (int j = b0; j < b1; j++) {
+                const float4 c = tCol[j];
+                acc.x += c.x;
+                acc.y += c.y;
+                acc.z += c.z;
+                acc.w += c.w;
+            }
+        }
+        if (act) {
+            float4 alb = make_float4(1, 1, 1, 1);
+            if (R.mat >= 0) {
+                const float invn = 1.0f / (float)n;
+                alb = make_float4(acc.x * invn, acc.y * invn, acc.z * invn, acc.w * invn);
+            }
+            cCol[lane] = P.program == 0 ? make_float4(sat(R.B.x), sat(R.B.y), sat(R.B.z), 1.0f)
+                                        : make_float4(sat(fmaf(alb.x, R.A.x, R.B.x)), sat(fmaf(alb.y, R.A.y, R.B.y)),
+                                                      sat(fmaf(alb.z, R.A.z, R.B.z)), sat(alb.w));
+        }
+        __syncthreads();
+        // 3. the pixel's fragments of this chunk, in sample order
+        const int k0 = max(poff, c0), k1 = min(poff + nf, c0 + 64);
+        for (int k = k0; k < k1; k++) {
+            const float4 c = cCol[k - c0];
+            const float cnt = (float)fCnt[k];
+            sum.x += c.x * cnt;
+            sum.y += c.y * cnt;
+            sum.z += c.z * cnt;
+            sum.w += c.w * cnt;
+        }
+        __syncthreads();
+    }
+    if (!inFb) return;
     const float inv = 1.0f / (float)S;
     const float4 r = make_float4(sum.x * inv, sum.y * inv, sum.z * inv, sum.w * inv);
     rgba8[pix] = make_uchar4((unsigned char)floorf(r.x * 255.0f + 0.5f), (unsigned char)floorf(r.y * 255.0f + 0.5f),
@@ -233,19 +387,21 @@ __global__ void __launch_bounds__(256) k_shade(ShadeParams P, TriRecs recs,
 void launch_shade(const ShadeParams& p, TriRecs recs, const float4* bary, const uint32_t* visSlot,
                   const float* vtx, const int* idx, const TexDesc* tex, const uint8_t* texPool, uchar4* rgba8,
                   float4* rgba32f, hipStream_t s) {
-    const dim3 grid(dev::blocks(p.width, 16), dev::blocks(p.height, 16));
+    const int tilesX = (int)dev::blocks(p.width, 8), tiles = tilesX * (int)dev::blocks(p.height, 8);
+    const int perXcd = (tiles + 7) / 8;
+    const dim3 grid(8 * perXcd);
     switch (p.samples) {
         case 1:
-            hipLaunchKernelGGL(k_shade<1>, grid, dim3(256), 0, s, p, recs, bary, visSlot, vtx, idx, tex, texPool,
-                               rgba8, rgba32f);
+            hipLaunchKernelGGL(k_shade<1>, grid, dim3(64), 0, s, p, recs, bary, visSlot, vtx, idx, tex, texPool,
+                               rgba8, rgba32f, tilesX, tiles, perXcd);
             break;
         case 4:
-            hipLaunchKernelGGL(k_shade<4>, grid, dim3(256), 0, s, p, recs, bary, visSlot, vtx, idx, tex, texPool,
-                               rgba8, rgba32f);
+            hipLaunchKernelGGL(k_shade<4>, grid, dim3(64), 0, s, p, recs, bary, visSlot, vtx, idx, tex, texPool,
+                               rgba8, rgba32f, tilesX, tiles, perXcd);
             break;
         default:
-            hipLaunchKernelGGL(k_shade<8>, grid, dim3(256), 0, s, p, recs, bary, visSlot, vtx, idx, tex, texPool,
-                               rgba8, rgba32f);
+            hipLaunchKernelGGL(k_shade<8>, grid, dim3(64), 0, s, p, recs, bary, visSlot, vtx, idx, tex, texPool,
+                               rgba8, rgba32f, tilesX, tiles, perXcd);
             break;
     }
 }

@@ -148,7 +148,7 @@ typedef struct {
     int32_t offWorld, offNormal, offUv; /* float offsets in the vertex record */
     int32_t trisPerMaterial;    /* texture index = primitive / trisPerMaterial (0: texture 0) */
     int32_t sampler;            /* IFHIP_SAMPLER_* */
-    int32_t maxAniso;
+    int32_t maxAniso;         /* anisotropic taps, clamped to [1, 16] */
     float camera[3];
     float ambient, kd, ks, shininess;
     int32_t numLights;          /* <= 4 */
