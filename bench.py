@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ms/frame & Gfrag/s at 1080p 8xMSAA; neural vs plain shade ms at ≥40 dB PSNR"
 PEAK_HBM_GBS = 8000.0
-STAGES = ["vertex", "reset", "setup", "bin", "raster", "shade"]
+STAGES = ["vertex", "reset", "setup", "bin", "raster", "raster_merge", "shade"]
 
 
 def algorithmic_bytes(scene, stage: str) -> float:
@@ -97,7 +97,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--spheres", type=int, default=256)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a float32 copy of 256 MiB (x + 0) three times: the reference kernel for "
+                         "scaling FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.add(src, 0.0, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read": src.numel() * 4, "bytes_written": dst.numel() * 4}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the renderer runs on a single "
               "MI355X; multi-GPU sharding is a design note only)", file=sys.stderr)

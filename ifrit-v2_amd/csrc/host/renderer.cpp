@@ -442,9 +442,11 @@ void TileRasterRenderer::enqueue(bool clear) {
     launch_setup(sp, clip_.as<float4>(), idx_.as<int>(), recPtrs(), bary_.as<float4>(), tileBox_.as<int4>(),
                  slotOf_.as<unsigned>(), clipList_.as<unsigned>(), statPtrs(), ctr_.as<DeviceCounters>(), s);
     mark(3);
+    RasterQueue q{items_.as<uint2>(), tilePartial_.as<int>(), multiList_.as<unsigned>(),
+                  partials_.as<unsigned long long>(), itemCap_, partialCap_, rasterGrid()};
     BinParams bp{tilesX, tilesY, curTris_, clipCap_, binCap_, bigCap_, kBigThreshold};
     launch_bin(bp, recPtrs(), tileBox_.as<int4>(), binCount_.as<unsigned>(), binStart_.as<unsigned>(),
-               binCursor_.as<unsigned>(), binList_.as<unsigned>(), bigList_.as<unsigned>(), ctr_.as<DeviceCounters>(),
+               binCursor_.as<unsigned>(), binList_.as<unsigned>(), bigList_.as<unsigned>(), q, ctr_.as<DeviceCounters>(),
                s);
     mark(4);
     RasterParams rp{};
@@ -460,13 +462,15 @@ void TileRasterRenderer::enqueue(bool clear) {
         rp.sx[i] = st[2 * i];
         rp.sy[i] = st[2 * i + 1];
     }
-    RasterQueue q{items_.as<uint2>(), tilePartial_.as<int>(), multiList_.as<unsigned>(),
-                  partials_.as<unsigned long long>(), itemCap_, partialCap_, rasterGrid()};
     launch_raster(rp, samples_, recPtrs(), slotOf_.as<unsigned>(), binStart_.as<unsigned>(),
-                  binCount_.as<unsigned>(), binList_.as<unsigned>(), q, visId_.as<uint32_t>(), visDepth_.as<float>(), visSlot_.as<uint32_t>(),
-                  visClass_.as<uint8_t>(), statPtrs(), ctr_.as<DeviceCounters>(), s);
-    launch_finalize(statPtrs(), ctr_.as<DeviceCounters>(), s);
+                  binCount_.as<unsigned>(), binList_.as<unsigned>(), q, visId_.as<uint32_t>(), visDepth_.as<float>(),
+                  visSlot_.as<uint32_t>(), visClass_.as<uint8_t>(), statPtrs(), ctr_.as<DeviceCounters>(), s);
     mark(5);
+    launch_raster_merge(rp, samples_, recPtrs(), slotOf_.as<unsigned>(), binStart_.as<unsigned>(),
+                        binCount_.as<unsigned>(), q, visId_.as<uint32_t>(), visDepth_.as<float>(),
+                        visSlot_.as<uint32_t>(), visClass_.as<uint8_t>(), ctr_.as<DeviceCounters>(), s);
+    launch_finalize(statPtrs(), ctr_.as<DeviceCounters>(), s);
+    mark(6);
     if (fsBound_ && fs_.desc.kind != IFHIP_FS_DEPTH_ONLY) {
         ShadeParams sh{};
         const IfritGpuFragmentProgramDesc& d = fs_.desc;
@@ -497,7 +501,7 @@ void TileRasterRenderer::enqueue(bool clear) {
                      idx_.as<int>(), texDesc_.as<TexDesc>(), nullptr, rgba8_.as<uchar4>(),
                      wantFloat ? rgba32f_.as<float4>() : nullptr, s);
     }
-    mark(6);
+    mark(7);
 }
 
 bool TileRasterRenderer::finish(bool clear, int indexCount) {
@@ -514,7 +518,7 @@ bool TileRasterRenderer::finish(bool clear, int indexCount) {
     stats_.coveredSamples = (int64_t)c.coveredSamples;
     stats_.status = c.status;
     if (stageTiming_) {
-        for (int i = 0; i < 6; i++) {
+        for (int i = 0; i < 7; i++) {
             float ms = 0;
             (void)hipEventElapsedTime(&ms, ev_[i], ev_[i + 1]);
             stats_.stageMs[i] = ms;

@@ -172,7 +172,7 @@ private:
 
     IfritFrameStats stats_{};
     bool stageTiming_ = false;
-    hipEvent_t ev_[8] = {};
+    hipEvent_t ev_[8] = {};  // stage boundaries: vertex, reset, setup, bin, raster, merge, shade
     std::string error_;
     std::string errorOut_;
 };

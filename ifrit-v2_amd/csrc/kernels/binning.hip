@@ -105,40 +105,111 @@ __global__ void __launch_bounds__(256) k_bin_big(BinParams P, TriRecs recs,
     }
 }
 
-// Exclusive scan of the per-tile counts by one 1024-thread workgroup.
-__global__ void __launch_bounds__(1024) k_bin_scan(const unsigned* __restrict__ cnt, unsigned* __restrict__ start,
-                                                   int n, unsigned cap, DeviceCounters* ctr) {
-    __shared__ unsigned part[1024];
-    const int per = (n + 1023) / 1024;
-    const int b0 = threadIdx.x * per;
-    unsigned sum = 0;
-    for (int i = 0; i < per; i++)
-        if (b0 + i < n) sum += cnt[b0 + i];
-    part[threadIdx.x] = sum;
+// Workgroup-wide inclusive scan over 1024 threads (16 waves).
+__device__ __forceinline__ unsigned scan1024(unsigned v, unsigned* sWave, unsigned& total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    __syncthreads();  // sWave is free
+    if (lane == 63) sWave[w] = x;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const unsigned v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
+    unsigned base = 0;
+    total = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const unsigned t = sWave[i];
+        base += i < w ? t : 0u;
+        total += t;
     }
-    unsigned run = part[threadIdx.x] - sum;
-    for (int i = 0; i < per; i++) {
-        if (b0 + i < n) {
-            start[b0 + i] = run;
-            run += cnt[b0 + i];
+    return base + x;
+}
+
+// One 1024-thread workgroup, in chunks of 8192 tiles: the exclusive scan of
+// the per-tile counts (list starts), then the raster work of each tile: parts
+// of at most kRasterPart entries, the first partial key block of a tile with
+// several parts, the list of such tiles (multiList[0, numMulti)) and the list
+// of empty tiles (multiList[tiles, tiles + numEmpty)).  Counts are staged in
+// LDS so that global loads are coalesced.
+__global__ void __launch_bounds__(1024) k_tile_scan(const unsigned* __restrict__ cnt, unsigned* __restrict__ start,
+                                                    int n, unsigned cap, RasterQueue q, DeviceCounters* ctr) {
+    constexpr int kPer = 8, kChunk = 1024 * kPer;
+    __shared__ unsigned sCnt[kChunk];
+    __shared__ unsigned sWave[16];
+    const int t = threadIdx.x;
+    unsigned carryS = 0, carryA = 0, carryB = 0, carryM = 0, carryE = 0;
+    for (int base = 0; base < n; base += kChunk) {
+        __syncthreads();
+        for (int i = t; i < kChunk; i += 1024) sCnt[i] = base + i < n ? cnt[base + i] : 0u;
+        __syncthreads();
+        unsigned sum = 0;
+#pragma unroll
+        for (int i = 0; i < kPer; i++) sum += sCnt[t * kPer + i];
+        unsigned totS;
+        unsigned run = carryS + scan1024(sum, sWave, totS) - sum;
+        // starts out; the tile's part count replaces its count in LDS
+        unsigned a = 0, b = 0, m = 0, e = 0;
+        for (int i = 0; i < kPer; i++) {
+            const int tile = base + t * kPer + i;
+            const unsigned c = sCnt[t * kPer + i];
+            const unsigned st = run;
+            run += c;
+            if (tile < n) start[tile] = st;
+            const unsigned total = st >= cap ? 0u : min(c, cap - st);
+            const unsigned p = tile < n ? (total + kRasterPart - 1) / kRasterPart : 0u;
+            sCnt[t * kPer + i] = p;
+            a += p;
+            b += p > 1u ? p : 0u;
+            m += p > 1u;
+            e += tile < n && p == 0u;
         }
+        unsigned totA, totB, totM, totE;
+        unsigned ra = carryA + scan1024(a, sWave, totA) - a;
+        unsigned rb = carryB + scan1024(b, sWave, totB) - b;
+        unsigned rm = carryM + scan1024(m, sWave, totM) - m;
+        unsigned re = carryE + scan1024(e, sWave, totE) - e;
+        for (int i = 0; i < kPer; i++) {
+            const int tile = base + t * kPer + i;
+            if (tile >= n) break;
+            const unsigned p = sCnt[t * kPer + i];
+            for (unsigned k = 0; k < p; k++)
+                if (ra + k < q.itemCap) q.items[ra + k] = make_uint2((unsigned)tile, k);
+            if (p > 1u) q.multiList[rm] = (unsigned)tile;
+            if (p == 0u) q.multiList[n + re] = (unsigned)tile;
+            q.tilePartial[tile] = p > 1u ? (int)rb : -1;
+            ra += p;
+            rb += p > 1u ? p : 0u;
+            rm += p > 1u;
+            re += p == 0u;
+        }
+        carryS += totS;
+        carryA += totA;
+        carryB += totB;
+        carryM += totM;
+        carryE += totE;
     }
-    if (threadIdx.x == 1023) {
-        ctr->binTotal = part[1023];
-        if (part[1023] > cap) atomicOr(&ctr->status, kStatusBinOverflow);
+    if (t == 0) {
+        ctr->binTotal = carryS;
+        if (carryS > cap) atomicOr(&ctr->status, kStatusBinOverflow);
+        // on overflow nothing is rasterised; the host grows the buffers and redraws
+        const bool fits = carryA <= q.itemCap && carryB <= q.partialCap;
+        ctr->numItems = fits ? carryA : 0u;
+        ctr->numMulti = fits ? carryM : 0u;
+        ctr->numEmpty = fits ? carryE : 0u;
+        ctr->numPartials = carryB;
+        ctr->itemHead = 0u;
+        if (!fits) atomicOr(&ctr->status, kStatusRasterOverflow);
     }
 }
 
 }  // namespace
 
 void launch_bin(const BinParams& p, TriRecs recs, const int4* tileBox, unsigned* binCount, unsigned* binStart,
-                unsigned* binCursor, unsigned* binList, unsigned* bigList, DeviceCounters* ctr, hipStream_t s) {
+                unsigned* binCursor, unsigned* binList, unsigned* bigList, const RasterQueue& q, DeviceCounters* ctr,
+                hipStream_t s) {
     const size_t slots = (size_t)p.ntri + p.clipCap;
     const int tiles = p.tilesX * p.tilesY;
     const dim3 g(dev::blocks(slots > 0 ? slots : 1, 256));
@@ -146,7 +217,7 @@ void launch_bin(const BinParams& p, TriRecs recs, const int4* tileBox, unsigned*
                        binList, bigList, ctr);
     hipLaunchKernelGGL(k_bin_big<false>, dim3(256), dim3(256), 0, s, p, recs, tileBox, binCount, binStart, binCursor,
                        binList, bigList, ctr);
-    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, binCount, binStart, tiles, p.binCap, ctr);
+    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, binCount, binStart, tiles, p.binCap, q, ctr);
     hipLaunchKernelGGL(k_bin_small<true>, g, dim3(256), 0, s, p, recs, tileBox, binCount, binStart, binCursor,
                        binList, bigList, ctr);
     hipLaunchKernelGGL(k_bin_big<true>, dim3(256), dim3(256), 0, s, p, recs, tileBox, binCount, binStart, binCursor,

@@ -122,6 +122,9 @@ struct DeviceCounters {
     unsigned numEmpty;     // tiles with an empty list
 };
 
+// Raster work item size: list entries per (tile, part).
+constexpr int kRasterPart = 256;
+
 // Raster work: (tile, part) items drained through DeviceCounters::itemHead;
 // per tile the first partial block of a tile cut into several parts (-1
 // otherwise); the list of such tiles followed, from index `tiles`, by the
@@ -143,11 +146,15 @@ void launch_setup(const SetupParams& p, const float4* clip, const int* idx, TriR
                   unsigned* slotOf, unsigned* clipList, StatPartials st, DeviceCounters* ctr, hipStream_t s);
 void launch_finalize(StatPartials st, DeviceCounters* ctr, hipStream_t s);
 void launch_bin(const BinParams& p, TriRecs recs, const int4* tileBox, unsigned* binCount, unsigned* binStart,
-                unsigned* binCursor, unsigned* binList, unsigned* bigList, DeviceCounters* ctr, hipStream_t s);
+                unsigned* binCursor, unsigned* binList, unsigned* bigList, const RasterQueue& q, DeviceCounters* ctr,
+                hipStream_t s);
 void launch_raster(const RasterParams& p, int samples, TriRecs recs, const unsigned* slotOf,
                    const unsigned* binStart, const unsigned* binCount, const unsigned* binList, const RasterQueue& q,
                    uint32_t* visId, float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st,
                    DeviceCounters* ctr, hipStream_t s);
+void launch_raster_merge(const RasterParams& p, int samples, TriRecs recs, const unsigned* slotOf,
+                         const unsigned* binStart, const unsigned* binCount, const RasterQueue& q, uint32_t* visId,
+                         float* visDepth, uint32_t* visSlot, uint8_t* visClass, DeviceCounters* ctr, hipStream_t s);
 void launch_shade(const ShadeParams& p, TriRecs recs, const float4* bary, const uint32_t* visSlot,
                   const float* vtx, const int* idx, const TexDesc* tex, const uint8_t* texPool, uchar4* rgba8,
                   float4* rgba32f, hipStream_t s);

@@ -41,7 +41,7 @@ namespace {
 
 using namespace dev;
 
-constexpr int kPart = 256;    // list entries per work item
+constexpr int kPart = kRasterPart;  // list entries per work item
 constexpr int kUnits = 4096;  // units expanded into LDS per round
 
 // Depth at a 1-sample pixel for a coverage class (1: SIMD256, 2: SIMD128, 3: scalar).
@@ -243,6 +243,96 @@ __device__ __forceinline__ void write_pixel(const RasterParams& P, const TriRecs
     if (S == 1 && visClass && (won || P.clear)) visClass[C.pix] = (uint8_t)cls;
 }
 
+// One tile's outputs staged in LDS in global order (pixel-major, sample-
+// minor) so that they leave with 16-byte stores of consecutive addresses;
+// per-lane stores of one pixel's S samples would touch a partial line per
+// sample.
+template <int S>
+struct TileOut {
+    float d[256 * S];
+    unsigned slot[256 * S];
+    unsigned id[256 * S];
+};
+
+// Outputs of one pixel of a cleared draw into the staging buffer; the 1-spp
+// class byte goes straight out.
+template <int S>
+__device__ __forceinline__ void stage_pixel(const RasterParams& P, const TriRecs& recs, const unsigned* slotOf,
+                                            const PixelCtx& C, int t, const unsigned long long* keys, TileOut<S>& O,
+                                            uint8_t* visClass) {
+    if (!C.inFb) return;
+    unsigned cls = 0;
+    const unsigned long long clearKey = frag_key(kClearDepth, 0u);
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const unsigned long long k = keys[s];
+        float d = kClearDepth;
+        unsigned slot = 0xffffffffu, prim = 0xffffffffu;
+        if (k != clearKey) {
+            const unsigned tie = (unsigned)k;
+            prim = tie >> 2;
+            slot = slotOf[prim] + (tie & 3u);
+            d = order_key_inv((unsigned)(k >> 32));
+            if (S == 1) {
+                cls = winner_class(load_rec(recs, slot), C.tx, C.ty, C.fpx, C.fpy, C.bMinX, C.bMinY, d);
+            } else if ((unsigned)(k >> 32) == 0x80000000u) {
+                // +0 and -0 share a key; recompute to return the winner's own bits
+                d = depth256(recs.f1[slot], recs.f2[slot], recs.f3[slot], C.fpx + P.sx[s], C.fpy + P.sy[s]);
+            }
+        }
+        O.d[t * S + s] = d;
+        O.slot[t * S + s] = slot;
+        O.id[t * S + s] = prim;
+    }
+    if (S == 1 && visClass) visClass[C.pix] = (uint8_t)cls;
+}
+
+// Stores a staged tile (CLEAR: the clear values, no staging needed) with
+// 16-byte stores; partial columns at the right edge fall back to scalars.
+template <int S, bool CLEAR>
+__device__ __forceinline__ void store_tile(const RasterParams& P, int tx, int ty, const TileOut<S>* O,
+                                           float* visDepth, uint32_t* visSlot, uint32_t* visId, uint8_t* visClass) {
+    constexpr int perRow = 16 * S / 4;  // float4 per tile row
+    const int cols = min(16, P.width - tx * 16), rows = min(16, P.height - ty * 16);
+    const bool vec = cols == 16 && (S > 1 || (P.width & 3) == 0);
+    for (int i = threadIdx.x; i < 64 * S; i += 256) {
+        const int row = i / perRow, q = i - row * perRow;
+        if (row >= rows) continue;
+        const size_t g = ((size_t)(ty * 16 + row) * P.width + tx * 16) * S + q * 4;
+        const int l = row * 16 * S + q * 4;
+        float4 d;
+        uint4 sl, id;
+        if (CLEAR) {
+            d = make_float4(kClearDepth, kClearDepth, kClearDepth, kClearDepth);
+            sl = make_uint4(~0u, ~0u, ~0u, ~0u);
+            id = sl;
+        } else {
+            d = *reinterpret_cast<const float4*>(&O->d[l]);
+            sl = *reinterpret_cast<const uint4*>(&O->slot[l]);
+            id = *reinterpret_cast<const uint4*>(&O->id[l]);
+        }
+        if (vec) {
+            *reinterpret_cast<float4*>(&visDepth[g]) = d;
+            *reinterpret_cast<uint4*>(&visSlot[g]) = sl;
+            *reinterpret_cast<uint4*>(&visId[g]) = id;
+        } else {
+            const float dv[4] = {d.x, d.y, d.z, d.w};
+            const unsigned sv[4] = {sl.x, sl.y, sl.z, sl.w}, iv[4] = {id.x, id.y, id.z, id.w};
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                if ((q * 4 + e) / S >= cols) continue;
+                visDepth[g + e] = dv[e];
+                visSlot[g + e] = sv[e];
+                visId[g + e] = iv[e];
+            }
+        }
+    }
+    if (CLEAR && S == 1 && visClass) {
+        const int row = threadIdx.x >> 4, col = threadIdx.x & 15;
+        if (row < rows && col < cols) visClass[(size_t)(ty * 16 + row) * P.width + tx * 16 + col] = 0;
+    }
+}
+
 template <int S>
 __global__ void __launch_bounds__(256) k_raster(RasterParams P, TriRecs recs, const unsigned* __restrict__ slotOf,
                                                 const unsigned* __restrict__ binStart,
@@ -256,6 +346,7 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P, TriRecs recs, co
     constexpr int NW = S == 1 ? 1 : 8;
     __shared__ unsigned long long sVis[S * 256];  // key of sample s of tile pixel p at s * 256 + p
     __shared__ TriRec sRec[kPart];
+    static_assert(sizeof(TileOut<S>) <= sizeof(TriRec) * kPart, "staging reuses sRec");
     __shared__ unsigned short sTa[kPart];
     __shared__ unsigned short sUnit[kUnits];
     __shared__ unsigned sWave[4];
@@ -393,15 +484,24 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P, TriRecs recs, co
         }
         __syncthreads();
 
-        if (pbase < 0) {
+        if (pbase >= 0) {
+            // one part of a longer list: its keys, sample-major like sVis
+            unsigned long long* g = partials + (size_t)(pbase + part) * 256 * S;
+#pragma unroll
+            for (int s = 0; s < S; s++) g[s * 256 + t] = sVis[s * 256 + t];
+        } else {
             unsigned long long keys[S];
 #pragma unroll
             for (int s = 0; s < S; s++) keys[s] = sVis[s * 256 + t];
-            write_pixel<S>(P, recs, slotOf, C, keys, visId, visDepth, visSlot, visClass);
-        } else {
-            unsigned long long* g = partials + ((size_t)(pbase + part) * 256 + t) * S;
-#pragma unroll
-            for (int s = 0; s < S; s++) g[s] = sVis[s * 256 + t];
+            if (P.clear) {
+                // sRec is free now: stage the outputs there
+                TileOut<S>* O = reinterpret_cast<TileOut<S>*>(sRec);
+                stage_pixel<S>(P, recs, slotOf, C, t, keys, *O, visClass);
+                __syncthreads();
+                store_tile<S, false>(P, C.tx, C.ty, O, visDepth, visSlot, visId, visClass);
+            } else {
+                write_pixel<S>(P, recs, slotOf, C, keys, visId, visDepth, visSlot, visClass);
+            }
         }
     }
     const unsigned long long tf = block_sum256(frags);
@@ -424,16 +524,14 @@ __global__ void __launch_bounds__(256) k_raster_merge(RasterParams P, TriRecs re
                                                       uint32_t* __restrict__ visId, float* __restrict__ visDepth,
                                                       uint32_t* __restrict__ visSlot, uint8_t* __restrict__ visClass,
                                                       int tiles, const DeviceCounters* ctr) {
+    __shared__ TileOut<S> O;
     const unsigned numMulti = ctr->numMulti;
     const unsigned numWork = numMulti + (P.clear ? ctr->numEmpty : 0u);
     const int t = threadIdx.x;
     for (unsigned m = blockIdx.x; m < numWork; m += gridDim.x) {
         if (m >= numMulti) {
-            const PixelCtx C = pixel_ctx(P, (int)multiList[tiles + (m - numMulti)], t & 15, t >> 4);
-            unsigned long long keys[S];
-#pragma unroll
-            for (int s = 0; s < S; s++) keys[s] = frag_key(kClearDepth, 0u);
-            write_pixel<S>(P, recs, slotOf, C, keys, visId, visDepth, visSlot, visClass);
+            const int tile = (int)multiList[tiles + (m - numMulti)];
+            store_tile<S, true>(P, tile % P.tilesX, tile / P.tilesX, nullptr, visDepth, visSlot, visId, visClass);
             continue;
         }
         const int tile = (int)multiList[m];
@@ -441,120 +539,57 @@ __global__ void __launch_bounds__(256) k_raster_merge(RasterParams P, TriRecs re
         const unsigned start = binStart[tile];
         const unsigned total = start >= P.binCap ? 0u : min(binCount[tile], P.binCap - start);
         const unsigned nparts = (total + kPart - 1) / kPart;
-        const unsigned long long* g = partials + ((size_t)tilePartial[tile] * 256 + t) * S;
+        const unsigned long long* g = partials + (size_t)tilePartial[tile] * 256 * S + t;
         unsigned long long keys[S];
 #pragma unroll
-        for (int s = 0; s < S; s++) keys[s] = g[s];
+        for (int s = 0; s < S; s++) keys[s] = g[s * 256];
         for (unsigned p = 1; p < nparts; p++) {
             g += 256 * S;
 #pragma unroll
-            for (int s = 0; s < S; s++) keys[s] = min(keys[s], g[s]);
+            for (int s = 0; s < S; s++) keys[s] = min(keys[s], g[s * 256]);
         }
-        write_pixel<S>(P, recs, slotOf, C, keys, visId, visDepth, visSlot, visClass);
-    }
-}
-
-// Builds the work items (parts of at most kPart entries per non-empty tile);
-// for tiles with several parts, the first partial block and the merge list
-// (multiList[0, numMulti)); the empty tiles (multiList[tiles, tiles + numEmpty)).
-__global__ void __launch_bounds__(1024) k_raster_plan(const unsigned* __restrict__ binStart,
-                                                      const unsigned* __restrict__ binCount, int tiles,
-                                                      unsigned binCap, unsigned itemCap, unsigned partialCap,
-                                                      uint2* __restrict__ items, int* __restrict__ tilePartial,
-                                                      unsigned* __restrict__ multiList, DeviceCounters* ctr) {
-    __shared__ unsigned sA[1024], sB[1024], sM[1024], sE[1024];
-    const int per = (tiles + 1023) / 1024;
-    const int b0 = threadIdx.x * per;
-    unsigned a = 0, b = 0, m = 0, e = 0;  // items, partial blocks, tiles with several parts, empty tiles
-    for (int i = 0; i < per; i++) {
-        const int t = b0 + i;
-        if (t >= tiles) break;
-        const unsigned st = binStart[t];
-        const unsigned total = st >= binCap ? 0u : min(binCount[t], binCap - st);
-        const unsigned parts = (total + kPart - 1) / kPart;
-        a += parts;
-        b += parts > 1u ? parts : 0u;
-        m += parts > 1u;
-        e += parts == 0u;
-    }
-    sA[threadIdx.x] = a;
-    sB[threadIdx.x] = b;
-    sM[threadIdx.x] = m;
-    sE[threadIdx.x] = e;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const unsigned va = threadIdx.x >= (unsigned)off ? sA[threadIdx.x - off] : 0u;
-        const unsigned vb = threadIdx.x >= (unsigned)off ? sB[threadIdx.x - off] : 0u;
-        const unsigned vm = threadIdx.x >= (unsigned)off ? sM[threadIdx.x - off] : 0u;
-        const unsigned ve = threadIdx.x >= (unsigned)off ? sE[threadIdx.x - off] : 0u;
-        __syncthreads();
-        sA[threadIdx.x] += va;
-        sB[threadIdx.x] += vb;
-        sM[threadIdx.x] += vm;
-        sE[threadIdx.x] += ve;
-        __syncthreads();
-    }
-    unsigned ra = sA[threadIdx.x] - a, rb = sB[threadIdx.x] - b, rm = sM[threadIdx.x] - m, re = sE[threadIdx.x] - e;
-    const bool fits = sA[1023] <= itemCap && sB[1023] <= partialCap;
-    for (int i = 0; i < per; i++) {
-        const int t = b0 + i;
-        if (t >= tiles) break;
-        const unsigned st = binStart[t];
-        const unsigned total = st >= binCap ? 0u : min(binCount[t], binCap - st);
-        const unsigned parts = (total + kPart - 1) / kPart;
-        if (fits) {
-            for (unsigned p = 0; p < parts; p++) items[ra + p] = make_uint2((unsigned)t, p);
-            if (parts > 1u) multiList[rm] = (unsigned)t;
-            if (parts == 0u) multiList[tiles + re] = (unsigned)t;
+        if (P.clear) {
+            __syncthreads();  // the previous tile's staging has been stored
+            stage_pixel<S>(P, recs, slotOf, C, t, keys, O, visClass);
+            __syncthreads();
+            store_tile<S, false>(P, C.tx, C.ty, &O, visDepth, visSlot, visId, visClass);
+        } else {
+            write_pixel<S>(P, recs, slotOf, C, keys, visId, visDepth, visSlot, visClass);
         }
-        tilePartial[t] = parts > 1u ? (int)rb : -1;
-        ra += parts;
-        rb += parts > 1u ? parts : 0u;
-        rm += parts > 1u;
-        re += parts == 0u;
     }
-    if (threadIdx.x == 1023) {
-        // on overflow nothing is rasterised; the host grows the buffers and redraws
-        ctr->numItems = fits ? sA[1023] : 0u;
-        ctr->numMulti = fits ? sM[1023] : 0u;
-        ctr->numEmpty = fits ? sE[1023] : 0u;
-        ctr->numPartials = sB[1023];
-        ctr->itemHead = 0u;
-        if (!fits) atomicOr(&ctr->status, kStatusRasterOverflow);
-    }
-}
-
-template <int S>
-void launch_s(const RasterParams& p, TriRecs recs, const unsigned* slotOf, const unsigned* binStart,
-              const unsigned* binCount, const unsigned* binList, const RasterQueue& q, uint32_t* visId,
-              float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st, DeviceCounters* ctr,
-              hipStream_t s) {
-    const int tiles = p.tilesX * p.tilesY;
-    hipLaunchKernelGGL(k_raster_plan, dim3(1), dim3(1024), 0, s, binStart, binCount, tiles, p.binCap, q.itemCap,
-                       q.partialCap, q.items, q.tilePartial, q.multiList, ctr);
-    hipLaunchKernelGGL(k_raster<S>, dim3(q.grid), dim3(256), 0, s, p, recs, slotOf, binStart, binCount, binList,
-                       q.items, q.tilePartial, q.partials, visId, visDepth, visSlot, visClass, st, ctr);
-    hipLaunchKernelGGL(k_raster_merge<S>, dim3(q.grid), dim3(256), 0, s, p, recs, slotOf, binStart, binCount,
-                       q.multiList, q.tilePartial, q.partials, visId, visDepth, visSlot, visClass, tiles, ctr);
 }
 
 }  // namespace
 
+// The raster and merge launches are separate so that the host can time them.
 void launch_raster(const RasterParams& p, int samples, TriRecs recs, const unsigned* slotOf, const unsigned* binStart,
                    const unsigned* binCount, const unsigned* binList, const RasterQueue& q, uint32_t* visId,
                    float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st, DeviceCounters* ctr,
                    hipStream_t s) {
+#define IFRIT_RASTER_LAUNCH(SN)                                                                                     \
+    hipLaunchKernelGGL(k_raster<SN>, dim3(q.grid), dim3(256), 0, s, p, recs, slotOf, binStart, binCount, binList, \
+                       q.items, q.tilePartial, q.partials, visId, visDepth, visSlot, visClass, st, ctr)
     switch (samples) {
-        case 1:
-            launch_s<1>(p, recs, slotOf, binStart, binCount, binList, q, visId, visDepth, visSlot, visClass, st, ctr, s);
-            break;
-        case 4:
-            launch_s<4>(p, recs, slotOf, binStart, binCount, binList, q, visId, visDepth, visSlot, visClass, st, ctr, s);
-            break;
-        default:
-            launch_s<8>(p, recs, slotOf, binStart, binCount, binList, q, visId, visDepth, visSlot, visClass, st, ctr, s);
-            break;
+        case 1: IFRIT_RASTER_LAUNCH(1); break;
+        case 4: IFRIT_RASTER_LAUNCH(4); break;
+        default: IFRIT_RASTER_LAUNCH(8); break;
     }
+#undef IFRIT_RASTER_LAUNCH
+}
+
+void launch_raster_merge(const RasterParams& p, int samples, TriRecs recs, const unsigned* slotOf,
+                         const unsigned* binStart, const unsigned* binCount, const RasterQueue& q, uint32_t* visId,
+                         float* visDepth, uint32_t* visSlot, uint8_t* visClass, DeviceCounters* ctr, hipStream_t s) {
+    const int tiles = p.tilesX * p.tilesY;
+#define IFRIT_MERGE_LAUNCH(SN)                                                                                      \
+    hipLaunchKernelGGL(k_raster_merge<SN>, dim3(q.grid), dim3(256), 0, s, p, recs, slotOf, binStart, binCount,      \
+                       q.multiList, q.tilePartial, q.partials, visId, visDepth, visSlot, visClass, tiles, ctr)
+    switch (samples) {
+        case 1: IFRIT_MERGE_LAUNCH(1); break;
+        case 4: IFRIT_MERGE_LAUNCH(4); break;
+        default: IFRIT_MERGE_LAUNCH(8); break;
+    }
+#undef IFRIT_MERGE_LAUNCH
 }
 
 }  // namespace ifrit_hip

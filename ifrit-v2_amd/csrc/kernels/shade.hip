@@ -265,7 +265,23 @@ __global__ void __launch_bounds__(64) k_shade(ShadeParams P, TriRecs recs, const
     // 1. distinct triangles of the pixel
     unsigned ids[S];
 #pragma unroll
-    for (int s = 0; s < S; s++) ids[s] = inFb ? visSlot[pix * S + s] : 0xffffffffu;
+    for (int s = 0; s < S; s++) ids[s] = 0xffffffffu;
+    if (inFb) {
+        if constexpr (S % 4 == 0) {
+            // a pixel's samples are contiguous and 16-byte aligned
+#pragma unroll
+            for (int v = 0; v < S / 4; v++) {
+                const uint4 x = reinterpret_cast<const uint4*>(visSlot + pix * S)[v];
+                ids[4 * v] = x.x;
+                ids[4 * v + 1] = x.y;
+                ids[4 * v + 2] = x.z;
+                ids[4 * v + 3] = x.w;
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < S; s++) ids[s] = visSlot[pix * S + s];
+        }
+    }
     int nf = 0;
 #pragma unroll
     for (int s = 0; s < S; s++) {

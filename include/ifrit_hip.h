@@ -193,7 +193,7 @@ typedef struct {
     int64_t coveredSamples;
     uint32_t status;       /* overflow bits, 0 when the frame is complete */
     uint32_t retries;      /* frames re-run with larger buffers */
-    float stageMs[8];      /* vertex, reset, setup, bin, raster, shade, (unused) x2 */
+    float stageMs[8];      /* vertex, reset, setup, bin, raster, raster merge, shade, (unused) */
 } IfritFrameStats;
 
 void iftrGetFrameStats(IfritRendererHandle* h, IfritFrameStats* out);
