@@ -47,6 +47,30 @@ def algorithmic_bytes(scene, stage: str) -> float:
     return 0.0
 
 
+# kernels of each timed stage (the raster stage's outputs are written by both
+# k_raster and, for tiles whose list was cut into parts, k_raster_merge)
+STAGE_KERNELS = {"vertex": ["k_vertex"], "setup": ["k_setup", "k_clip"], "bin": ["k_bin_small", "k_bin_big", "k_tile_scan"],
+                 "raster": ["k_raster", "k_raster_merge"], "shade": ["k_shade"]}
+
+
+def measured_traffic(stage: str):
+    """HBM bytes per launch of the stage's kernels from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (scaled by a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            per = json.load(f)["bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return None
+    total = 0
+    for k in STAGE_KERNELS.get(stage, []):
+        hits = [v for name, v in per.items() if name == k]
+        if not hits:
+            return None
+        total += hits[0]
+    return total
+
+
 def cpu_baseline(scene, frames: int = 5):
     """Reference CPU tile rasterizer (oracle/_ref) when built, else the oracle port."""
     import ctypes as C
@@ -152,9 +176,12 @@ def main():
             acc[k] += s["stageMs"][k]
     r.stage_timing(False)
     stage_ms = {STAGES[k]: acc[k] / nt for k in range(len(STAGES))}
-    dom = max(("raster", "shade", "setup", "bin", "vertex"), key=lambda k: stage_ms[k])
+    timed = dict(stage_ms)
+    timed["raster"] = stage_ms["raster"] + stage_ms["raster_merge"]
+    dom = max(("raster", "shade", "setup", "bin", "vertex"), key=lambda k: timed[k])
     ab = algorithmic_bytes(scene, dom)
-    achieved = ab / (stage_ms[dom] * 1e-3) / 1e9 if stage_ms[dom] > 0 else 0.0
+    achieved = ab / (timed[dom] * 1e-3) / 1e9 if timed[dom] > 0 else 0.0
+    traffic = measured_traffic(dom)
 
     frags = st["fragments"]
     value = frags / (ms * 1e-3) / 1e9
@@ -180,9 +207,9 @@ def main():
                    "triangles_per_s": round(scene.num_triangles / (ms * 1e-3), 1),
                    "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
                    "scene_setup_s": round(setup_s, 2)},
-        "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": None,
-                     "algorithmic_bytes": ab},
+        "roofline": {"kernel": dom, "kernels": STAGE_KERNELS[dom], "bound": "hbm", "achieved": round(achieved, 2),
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5),
+                     "traffic": traffic, "algorithmic_bytes": ab, "duration_ms": round(timed[dom], 4)},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene)
