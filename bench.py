@@ -32,7 +32,8 @@ STAGES = ["vertex", "reset", "setup", "bin", "raster", "raster_merge", "shade"]
 
 
 def algorithmic_bytes(scene, stage: str) -> float:
-    """Compulsory HBM bytes of one launch of each stage."""
+    """Compulsory HBM bytes of one launch of each stage: vertices in and clip positions out, indices and
+    clip positions in, per-sample depth and ID out, RGBA8 out plus one texel per pixel with its mips."""
     W, H, S = scene.width, scene.height, scene.samples
     V, T = scene.vertices.shape[0], scene.num_triangles
     if stage == "vertex":

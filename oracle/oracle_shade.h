@@ -5,7 +5,7 @@
 // sampler (include/ifrit/softgraphics/engine/math/ShaderBuiltinCuda.cuh:
 // 36-295) is the behavioural spec.  This oracle evaluates the programs in
 // double precision from the FP32 setup records of oracle_render, so that the
-// FP32 kernels can be judged against it within one RGBA8 step.
+// FP32 kernels can be compared with it within one RGBA8 step.
 //
 // Discontinuous decisions (anisotropic tap count, major-axis choice, the
 // N.L > 0 branch) are taken in double precision; pixels where the deciding
