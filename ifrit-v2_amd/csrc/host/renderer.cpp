@@ -422,9 +422,10 @@ void TileRasterRenderer::enqueue(bool clear) {
     mark(0);
     if (!hostVs_)
         launch_vertex(vtx_.as<float>(), stride_, nverts_, vs_.desc.mvp, vs_.desc.kind == IFHIP_VS_CLIP_PASSTHROUGH ? 1 : 0,
-                      clip_.as<float4>(), s);
+                      clip_.as<float4>(), binCount_.as<unsigned>(), binCursor_.as<unsigned>(), tiles,
+                      ctr_.as<DeviceCounters>(), s);
     mark(1);
-    launch_reset(binCount_.as<unsigned>(), binCursor_.as<unsigned>(), tiles, ctr_.as<DeviceCounters>(), s);
+    if (hostVs_) launch_reset(binCount_.as<unsigned>(), binCursor_.as<unsigned>(), tiles, ctr_.as<DeviceCounters>(), s);
     mark(2);
     SetupParams sp{};
     sp.width = width_;
@@ -463,13 +464,12 @@ void TileRasterRenderer::enqueue(bool clear) {
         rp.sy[i] = st[2 * i + 1];
     }
     launch_raster(rp, samples_, recPtrs(), slotOf_.as<unsigned>(), binStart_.as<unsigned>(),
-                  binCount_.as<unsigned>(), binList_.as<unsigned>(), q, visId_.as<uint32_t>(), visDepth_.as<float>(),
-                  visSlot_.as<uint32_t>(), visClass_.as<uint8_t>(), statPtrs(), ctr_.as<DeviceCounters>(), s);
+                  binCount_.as<unsigned>(), binList_.as<unsigned>(), q, visDepth_.as<float>(), visSlot_.as<uint32_t>(),
+                  visClass_.as<uint8_t>(), statPtrs(), ctr_.as<DeviceCounters>(), s);
     mark(5);
     launch_raster_merge(rp, samples_, recPtrs(), slotOf_.as<unsigned>(), binStart_.as<unsigned>(),
-                        binCount_.as<unsigned>(), q, visId_.as<uint32_t>(), visDepth_.as<float>(),
-                        visSlot_.as<uint32_t>(), visClass_.as<uint8_t>(), ctr_.as<DeviceCounters>(), s);
-    launch_finalize(statPtrs(), ctr_.as<DeviceCounters>(), s);
+                        binCount_.as<unsigned>(), q, visDepth_.as<float>(), visSlot_.as<uint32_t>(),
+                        visClass_.as<uint8_t>(), ctr_.as<DeviceCounters>(), s);
     mark(6);
     if (fsBound_ && fs_.desc.kind != IFHIP_FS_DEPTH_ONLY) {
         ShadeParams sh{};
@@ -507,6 +507,8 @@ void TileRasterRenderer::enqueue(bool clear) {
 bool TileRasterRenderer::finish(bool clear, int indexCount) {
     (void)clear;
     (void)indexCount;
+    // frame statistics are reduced from the per-workgroup partials only when read
+    launch_finalize(statPtrs(), ctr_.as<DeviceCounters>(), stream_);
     if (!ok(hipStreamSynchronize(stream_), "frame")) return false;
     DeviceCounters c{};
     if (!ok(hipMemcpy(&c, ctr_.ptr, sizeof(c), hipMemcpyDeviceToHost), "counters")) return false;
@@ -598,7 +600,11 @@ void TileRasterRenderer::clear() {
 
 void TileRasterRenderer::getVisibility(uint32_t* ids, float* depth) {
     const size_t n = (size_t)width_ * height_ * samples_;
-    if (ids) ok(hipMemcpy(ids, visId_.ptr, n * 4, hipMemcpyDeviceToHost), "id readback");
+    if (ids) {
+        launch_slot_to_prim(visSlot_.as<uint32_t>(), recPtrs(), n, visId_.as<uint32_t>(), stream_);
+        if (!ok(hipStreamSynchronize(stream_), "id expansion")) return;
+        ok(hipMemcpy(ids, visId_.ptr, n * 4, hipMemcpyDeviceToHost), "id readback");
+    }
     if (depth) ok(hipMemcpy(depth, visDepth_.ptr, n * 4, hipMemcpyDeviceToHost), "depth readback");
 }
 

@@ -140,7 +140,7 @@ struct RasterQueue {
 
 // Launch wrappers (pipeline.hip).
 void launch_vertex(const float* vtx, int stride, int nverts, const float* mvp16, int passthrough, float4* clip,
-                   hipStream_t s);
+                   unsigned* binCount, unsigned* binCursor, int tiles, DeviceCounters* ctr, hipStream_t s);
 void launch_reset(unsigned* binCount, unsigned* binCursor, int tiles, DeviceCounters* ctr, hipStream_t s);
 void launch_setup(const SetupParams& p, const float4* clip, const int* idx, TriRecs recs, float4* bary, int4* tileBox,
                   unsigned* slotOf, unsigned* clipList, StatPartials st, DeviceCounters* ctr, hipStream_t s);
@@ -150,11 +150,12 @@ void launch_bin(const BinParams& p, TriRecs recs, const int4* tileBox, unsigned*
                 hipStream_t s);
 void launch_raster(const RasterParams& p, int samples, TriRecs recs, const unsigned* slotOf,
                    const unsigned* binStart, const unsigned* binCount, const unsigned* binList, const RasterQueue& q,
-                   uint32_t* visId, float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st,
-                   DeviceCounters* ctr, hipStream_t s);
+                   float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st, DeviceCounters* ctr,
+                   hipStream_t s);
 void launch_raster_merge(const RasterParams& p, int samples, TriRecs recs, const unsigned* slotOf,
-                         const unsigned* binStart, const unsigned* binCount, const RasterQueue& q, uint32_t* visId,
-                         float* visDepth, uint32_t* visSlot, uint8_t* visClass, DeviceCounters* ctr, hipStream_t s);
+                         const unsigned* binStart, const unsigned* binCount, const RasterQueue& q, float* visDepth,
+                         uint32_t* visSlot, uint8_t* visClass, DeviceCounters* ctr, hipStream_t s);
+void launch_slot_to_prim(const uint32_t* visSlot, TriRecs recs, size_t n, uint32_t* visId, hipStream_t s);
 void launch_shade(const ShadeParams& p, TriRecs recs, const float4* bary, const uint32_t* visSlot,
                   const float* vtx, const int* idx, const TexDesc* tex, const uint8_t* texPool, uchar4* rgba8,
                   float4* rgba32f, hipStream_t s);

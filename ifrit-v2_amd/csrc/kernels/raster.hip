@@ -206,11 +206,13 @@ __device__ __forceinline__ PixelCtx pixel_ctx(const RasterParams& P, int tile, i
     return C;
 }
 
-// Writes the outputs of one pixel from its S final keys.
+// Writes the outputs of one pixel from its S final keys: depth and record
+// slot per sample (the primitive ID is the slot's key >> 2, expanded only on
+// readback).
 template <int S>
 __device__ __forceinline__ void write_pixel(const RasterParams& P, const TriRecs& recs, const unsigned* slotOf,
-                                            const PixelCtx& C, const unsigned long long* keys, uint32_t* visId,
-                                            float* visDepth, uint32_t* visSlot, uint8_t* visClass) {
+                                            const PixelCtx& C, const unsigned long long* keys, float* visDepth,
+                                            uint32_t* visSlot, uint8_t* visClass) {
     if (!C.inFb) return;
     unsigned cls = 0;
     bool won = false;
@@ -221,7 +223,6 @@ __device__ __forceinline__ void write_pixel(const RasterParams& P, const TriRecs
             if (P.clear) {
                 visDepth[C.pix * S + s] = kClearDepth;
                 visSlot[C.pix * S + s] = 0xffffffffu;
-                visId[C.pix * S + s] = 0xffffffffu;
             }
             continue;
         }
@@ -238,7 +239,6 @@ __device__ __forceinline__ void write_pixel(const RasterParams& P, const TriRecs
         }
         visDepth[C.pix * S + s] = d;
         visSlot[C.pix * S + s] = slot;
-        visId[C.pix * S + s] = prim;
     }
     if (S == 1 && visClass && (won || P.clear)) visClass[C.pix] = (uint8_t)cls;
 }
@@ -251,7 +251,6 @@ template <int S>
 struct TileOut {
     float d[256 * S];
     unsigned slot[256 * S];
-    unsigned id[256 * S];
 };
 
 // Outputs of one pixel of a cleared draw into the staging buffer; the 1-spp
@@ -267,10 +266,10 @@ __device__ __forceinline__ void stage_pixel(const RasterParams& P, const TriRecs
     for (int s = 0; s < S; s++) {
         const unsigned long long k = keys[s];
         float d = kClearDepth;
-        unsigned slot = 0xffffffffu, prim = 0xffffffffu;
+        unsigned slot = 0xffffffffu;
         if (k != clearKey) {
             const unsigned tie = (unsigned)k;
-            prim = tie >> 2;
+            const unsigned prim = tie >> 2;
             slot = slotOf[prim] + (tie & 3u);
             d = order_key_inv((unsigned)(k >> 32));
             if (S == 1) {
@@ -282,7 +281,6 @@ __device__ __forceinline__ void stage_pixel(const RasterParams& P, const TriRecs
         }
         O.d[t * S + s] = d;
         O.slot[t * S + s] = slot;
-        O.id[t * S + s] = prim;
     }
     if (S == 1 && visClass) visClass[C.pix] = (uint8_t)cls;
 }
@@ -291,7 +289,7 @@ __device__ __forceinline__ void stage_pixel(const RasterParams& P, const TriRecs
 // 16-byte stores; partial columns at the right edge fall back to scalars.
 template <int S, bool CLEAR>
 __device__ __forceinline__ void store_tile(const RasterParams& P, int tx, int ty, const TileOut<S>* O,
-                                           float* visDepth, uint32_t* visSlot, uint32_t* visId, uint8_t* visClass) {
+                                           float* visDepth, uint32_t* visSlot, uint8_t* visClass) {
     constexpr int perRow = 16 * S / 4;  // float4 per tile row
     const int cols = min(16, P.width - tx * 16), rows = min(16, P.height - ty * 16);
     const bool vec = cols == 16 && (S > 1 || (P.width & 3) == 0);
@@ -301,29 +299,25 @@ __device__ __forceinline__ void store_tile(const RasterParams& P, int tx, int ty
         const size_t g = ((size_t)(ty * 16 + row) * P.width + tx * 16) * S + q * 4;
         const int l = row * 16 * S + q * 4;
         float4 d;
-        uint4 sl, id;
+        uint4 sl;
         if (CLEAR) {
             d = make_float4(kClearDepth, kClearDepth, kClearDepth, kClearDepth);
             sl = make_uint4(~0u, ~0u, ~0u, ~0u);
-            id = sl;
         } else {
             d = *reinterpret_cast<const float4*>(&O->d[l]);
             sl = *reinterpret_cast<const uint4*>(&O->slot[l]);
-            id = *reinterpret_cast<const uint4*>(&O->id[l]);
         }
         if (vec) {
             *reinterpret_cast<float4*>(&visDepth[g]) = d;
             *reinterpret_cast<uint4*>(&visSlot[g]) = sl;
-            *reinterpret_cast<uint4*>(&visId[g]) = id;
         } else {
             const float dv[4] = {d.x, d.y, d.z, d.w};
-            const unsigned sv[4] = {sl.x, sl.y, sl.z, sl.w}, iv[4] = {id.x, id.y, id.z, id.w};
+            const unsigned sv[4] = {sl.x, sl.y, sl.z, sl.w};
 #pragma unroll
             for (int e = 0; e < 4; e++) {
                 if ((q * 4 + e) / S >= cols) continue;
                 visDepth[g + e] = dv[e];
                 visSlot[g + e] = sv[e];
-                visId[g + e] = iv[e];
             }
         }
     }
@@ -340,9 +334,9 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P, TriRecs recs, co
                                                 const unsigned* __restrict__ binList, const uint2* __restrict__ items,
                                                 const int* __restrict__ tilePartial,
                                                 unsigned long long* __restrict__ partials,
-                                                uint32_t* __restrict__ visId, float* __restrict__ visDepth,
-                                                uint32_t* __restrict__ visSlot, uint8_t* __restrict__ visClass,
-                                                StatPartials st, DeviceCounters* ctr) {
+                                                float* __restrict__ visDepth, uint32_t* __restrict__ visSlot,
+                                                uint8_t* __restrict__ visClass, StatPartials st,
+                                                DeviceCounters* ctr) {
     constexpr int NW = S == 1 ? 1 : 8;
     __shared__ unsigned long long sVis[S * 256];  // key of sample s of tile pixel p at s * 256 + p
     __shared__ TriRec sRec[kPart];
@@ -498,9 +492,9 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P, TriRecs recs, co
                 TileOut<S>* O = reinterpret_cast<TileOut<S>*>(sRec);
                 stage_pixel<S>(P, recs, slotOf, C, t, keys, *O, visClass);
                 __syncthreads();
-                store_tile<S, false>(P, C.tx, C.ty, O, visDepth, visSlot, visId, visClass);
+                store_tile<S, false>(P, C.tx, C.ty, O, visDepth, visSlot, visClass);
             } else {
-                write_pixel<S>(P, recs, slotOf, C, keys, visId, visDepth, visSlot, visClass);
+                write_pixel<S>(P, recs, slotOf, C, keys, visDepth, visSlot, visClass);
             }
         }
     }
@@ -521,9 +515,9 @@ __global__ void __launch_bounds__(256) k_raster_merge(RasterParams P, TriRecs re
                                                       const unsigned* __restrict__ multiList,
                                                       const int* __restrict__ tilePartial,
                                                       const unsigned long long* __restrict__ partials,
-                                                      uint32_t* __restrict__ visId, float* __restrict__ visDepth,
-                                                      uint32_t* __restrict__ visSlot, uint8_t* __restrict__ visClass,
-                                                      int tiles, const DeviceCounters* ctr) {
+                                                      float* __restrict__ visDepth, uint32_t* __restrict__ visSlot,
+                                                      uint8_t* __restrict__ visClass, int tiles,
+                                                      const DeviceCounters* ctr) {
     __shared__ TileOut<S> O;
     const unsigned numMulti = ctr->numMulti;
     const unsigned numWork = numMulti + (P.clear ? ctr->numEmpty : 0u);
@@ -531,7 +525,7 @@ __global__ void __launch_bounds__(256) k_raster_merge(RasterParams P, TriRecs re
     for (unsigned m = blockIdx.x; m < numWork; m += gridDim.x) {
         if (m >= numMulti) {
             const int tile = (int)multiList[tiles + (m - numMulti)];
-            store_tile<S, true>(P, tile % P.tilesX, tile / P.tilesX, nullptr, visDepth, visSlot, visId, visClass);
+            store_tile<S, true>(P, tile % P.tilesX, tile / P.tilesX, nullptr, visDepth, visSlot, visClass);
             continue;
         }
         const int tile = (int)multiList[m];
@@ -552,23 +546,35 @@ __global__ void __launch_bounds__(256) k_raster_merge(RasterParams P, TriRecs re
             __syncthreads();  // the previous tile's staging has been stored
             stage_pixel<S>(P, recs, slotOf, C, t, keys, O, visClass);
             __syncthreads();
-            store_tile<S, false>(P, C.tx, C.ty, &O, visDepth, visSlot, visId, visClass);
+            store_tile<S, false>(P, C.tx, C.ty, &O, visDepth, visSlot, visClass);
         } else {
-            write_pixel<S>(P, recs, slotOf, C, keys, visId, visDepth, visSlot, visClass);
+            write_pixel<S>(P, recs, slotOf, C, keys, visDepth, visSlot, visClass);
         }
+    }
+}
+
+// Primitive ID per sample from the record slot (readback only).
+__global__ void __launch_bounds__(256) k_slot_to_prim(const uint32_t* __restrict__ visSlot, const float4* __restrict__ q2,
+                                                      size_t n, uint32_t* __restrict__ visId) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const unsigned slot = visSlot[i];
+        visId[i] = slot == 0xffffffffu ? 0xffffffffu : (__float_as_uint(q2[slot].y) >> 2);
     }
 }
 
 }  // namespace
 
+void launch_slot_to_prim(const uint32_t* visSlot, TriRecs recs, size_t n, uint32_t* visId, hipStream_t s) {
+    hipLaunchKernelGGL(k_slot_to_prim, dim3(2048), dim3(256), 0, s, visSlot, recs.q2, n, visId);
+}
+
 // The raster and merge launches are separate so that the host can time them.
 void launch_raster(const RasterParams& p, int samples, TriRecs recs, const unsigned* slotOf, const unsigned* binStart,
-                   const unsigned* binCount, const unsigned* binList, const RasterQueue& q, uint32_t* visId,
-                   float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st, DeviceCounters* ctr,
-                   hipStream_t s) {
+                   const unsigned* binCount, const unsigned* binList, const RasterQueue& q, float* visDepth,
+                   uint32_t* visSlot, uint8_t* visClass, StatPartials st, DeviceCounters* ctr, hipStream_t s) {
 #define IFRIT_RASTER_LAUNCH(SN)                                                                                     \
     hipLaunchKernelGGL(k_raster<SN>, dim3(q.grid), dim3(256), 0, s, p, recs, slotOf, binStart, binCount, binList, \
-                       q.items, q.tilePartial, q.partials, visId, visDepth, visSlot, visClass, st, ctr)
+                       q.items, q.tilePartial, q.partials, visDepth, visSlot, visClass, st, ctr)
     switch (samples) {
         case 1: IFRIT_RASTER_LAUNCH(1); break;
         case 4: IFRIT_RASTER_LAUNCH(4); break;
@@ -578,12 +584,12 @@ void launch_raster(const RasterParams& p, int samples, TriRecs recs, const unsig
 }
 
 void launch_raster_merge(const RasterParams& p, int samples, TriRecs recs, const unsigned* slotOf,
-                         const unsigned* binStart, const unsigned* binCount, const RasterQueue& q, uint32_t* visId,
-                         float* visDepth, uint32_t* visSlot, uint8_t* visClass, DeviceCounters* ctr, hipStream_t s) {
+                         const unsigned* binStart, const unsigned* binCount, const RasterQueue& q, float* visDepth,
+                         uint32_t* visSlot, uint8_t* visClass, DeviceCounters* ctr, hipStream_t s) {
     const int tiles = p.tilesX * p.tilesY;
 #define IFRIT_MERGE_LAUNCH(SN)                                                                                      \
     hipLaunchKernelGGL(k_raster_merge<SN>, dim3(q.grid), dim3(256), 0, s, p, recs, slotOf, binStart, binCount,      \
-                       q.multiList, q.tilePartial, q.partials, visId, visDepth, visSlot, visClass, tiles, ctr)
+                       q.multiList, q.tilePartial, q.partials, visDepth, visSlot, visClass, tiles, ctr)
     switch (samples) {
         case 1: IFRIT_MERGE_LAUNCH(1); break;
         case 4: IFRIT_MERGE_LAUNCH(4); break;
