@@ -352,8 +352,8 @@ bool TileRasterRenderer::prepare(int indexCount) {
     clipCap_ = std::max<unsigned>(clipCap_, std::max(4096u, (unsigned)(ntri / 8)));
     binCap_ = std::max<unsigned>(binCap_, std::max(1u << 20, (unsigned)std::min<size_t>(0xFFFFFFF0ull, (size_t)ntri * 6)));
     bigCap_ = std::max<unsigned>(bigCap_, std::max(1024u, (unsigned)(ntri / 16)));
-    itemCap_ = std::max<unsigned>(itemCap_, (unsigned)tiles + binCap_ / 256u + 1u);
-    partialCap_ = std::max<unsigned>(partialCap_, std::max(256u, binCap_ / 512u));
+    itemCap_ = std::max<unsigned>(itemCap_, (unsigned)tiles + binCap_ / kRasterPart + 1u);
+    partialCap_ = std::max<unsigned>(partialCap_, std::max(256u, binCap_ / (8u * kRasterPart)));
     const size_t slots = (size_t)ntri + clipCap_;
     bool okAll = clip_.ensure((size_t)nverts_ * 16) && recs_[0].ensure(slots * 16) && recs_[1].ensure(slots * 16) &&
                  recs_[2].ensure(slots * 16) && recs_[3].ensure(slots * 16) && recs_[4].ensure(slots * 16) &&

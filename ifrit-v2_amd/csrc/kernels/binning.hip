@@ -132,15 +132,17 @@ __device__ __forceinline__ unsigned scan1024(unsigned v, unsigned* sWave, unsign
 // the per-tile counts (list starts), then the raster work of each tile: parts
 // of at most kRasterPart entries, the first partial key block of a tile with
 // several parts, the list of such tiles (multiList[0, numMulti)) and the list
-// of empty tiles (multiList[tiles, tiles + numEmpty)).  Counts are staged in
-// LDS so that global loads are coalesced.
+// of empty tiles (multiList[tiles, tiles + numEmpty)).  Parts of more than
+// kRasterPass entries are listed from the front of the item array and the
+// others from its back, so that the queue hands out the long ones first.
+// Counts are staged in LDS so that global loads are coalesced.
 __global__ void __launch_bounds__(1024) k_tile_scan(const unsigned* __restrict__ cnt, unsigned* __restrict__ start,
                                                     int n, unsigned cap, RasterQueue q, DeviceCounters* ctr) {
     constexpr int kPer = 8, kChunk = 1024 * kPer;
     __shared__ unsigned sCnt[kChunk];
     __shared__ unsigned sWave[16];
     const int t = threadIdx.x;
-    unsigned carryS = 0, carryA = 0, carryB = 0, carryM = 0, carryE = 0;
+    unsigned carryS = 0, carryA = 0, carryL = 0, carryB = 0, carryM = 0, carryE = 0;
     for (int base = 0; base < n; base += kChunk) {
         __syncthreads();
         for (int i = t; i < kChunk; i += 1024) sCnt[i] = base + i < n ? cnt[base + i] : 0u;
@@ -150,8 +152,8 @@ __global__ void __launch_bounds__(1024) k_tile_scan(const unsigned* __restrict__
         for (int i = 0; i < kPer; i++) sum += sCnt[t * kPer + i];
         unsigned totS;
         unsigned run = carryS + scan1024(sum, sWave, totS) - sum;
-        // starts out; the tile's part count replaces its count in LDS
-        unsigned a = 0, b = 0, m = 0, e = 0;
+        // starts out; the tile's list length within the capacity replaces its count in LDS
+        unsigned a = 0, l = 0, b = 0, m = 0, e = 0;  // long parts, short parts, partials, multi, empty
         for (int i = 0; i < kPer; i++) {
             const int tile = base + t * kPer + i;
             const unsigned c = sCnt[t * kPer + i];
@@ -160,33 +162,43 @@ __global__ void __launch_bounds__(1024) k_tile_scan(const unsigned* __restrict__
             if (tile < n) start[tile] = st;
             const unsigned total = st >= cap ? 0u : min(c, cap - st);
             const unsigned p = tile < n ? (total + kRasterPart - 1) / kRasterPart : 0u;
-            sCnt[t * kPer + i] = p;
-            a += p;
+            sCnt[t * kPer + i] = total;
+            const unsigned last = total - (p ? p - 1u : 0u) * kRasterPart;  // entries of the last part
+            const unsigned shortParts = p && last <= (unsigned)kRasterPass ? 1u : 0u;
+            a += p - shortParts;
+            l += shortParts;
             b += p > 1u ? p : 0u;
             m += p > 1u;
             e += tile < n && p == 0u;
         }
-        unsigned totA, totB, totM, totE;
+        unsigned totA, totL, totB, totM, totE;
         unsigned ra = carryA + scan1024(a, sWave, totA) - a;
+        unsigned rl = carryL + scan1024(l, sWave, totL) - l;
         unsigned rb = carryB + scan1024(b, sWave, totB) - b;
         unsigned rm = carryM + scan1024(m, sWave, totM) - m;
         unsigned re = carryE + scan1024(e, sWave, totE) - e;
         for (int i = 0; i < kPer; i++) {
             const int tile = base + t * kPer + i;
             if (tile >= n) break;
-            const unsigned p = sCnt[t * kPer + i];
-            for (unsigned k = 0; k < p; k++)
+            const unsigned total = sCnt[t * kPer + i];
+            const unsigned p = (total + kRasterPart - 1) / kRasterPart;
+            const unsigned last = total - (p ? p - 1u : 0u) * kRasterPart;
+            const unsigned longParts = p && last <= (unsigned)kRasterPass ? p - 1u : p;
+            for (unsigned k = 0; k < longParts; k++)
                 if (ra + k < q.itemCap) q.items[ra + k] = make_uint2((unsigned)tile, k);
+            if (longParts < p && rl < q.itemCap) q.items[q.itemCap - 1u - rl] = make_uint2((unsigned)tile, p - 1u);
             if (p > 1u) q.multiList[rm] = (unsigned)tile;
             if (p == 0u) q.multiList[n + re] = (unsigned)tile;
             q.tilePartial[tile] = p > 1u ? (int)rb : -1;
-            ra += p;
+            ra += longParts;
+            rl += p - longParts;
             rb += p > 1u ? p : 0u;
             rm += p > 1u;
             re += p == 0u;
         }
         carryS += totS;
         carryA += totA;
+        carryL += totL;
         carryB += totB;
         carryM += totM;
         carryE += totE;
@@ -195,8 +207,9 @@ __global__ void __launch_bounds__(1024) k_tile_scan(const unsigned* __restrict__
         ctr->binTotal = carryS;
         if (carryS > cap) atomicOr(&ctr->status, kStatusBinOverflow);
         // on overflow nothing is rasterised; the host grows the buffers and redraws
-        const bool fits = carryA <= q.itemCap && carryB <= q.partialCap;
-        ctr->numItems = fits ? carryA : 0u;
+        const bool fits = carryA + carryL <= q.itemCap && carryB <= q.partialCap;
+        ctr->numItems = fits ? carryA + carryL : 0u;
+        ctr->numLongItems = carryA;
         ctr->numMulti = fits ? carryM : 0u;
         ctr->numEmpty = fits ? carryE : 0u;
         ctr->numPartials = carryB;

@@ -120,10 +120,13 @@ struct DeviceCounters {
     unsigned numMulti;     // tiles cut into several parts
     unsigned numPartials;  // partial blocks those need
     unsigned numEmpty;     // tiles with an empty list
+    unsigned numLongItems; // items of more than kRasterPass entries (front of the item array)
 };
 
-// Raster work item size: list entries per (tile, part).
-constexpr int kRasterPart = 256;
+// Raster work: a work item covers at most kRasterPart list entries of one
+// tile, taken in passes of kRasterPass entries (the LDS capacity).
+constexpr int kRasterPass = 128;
+constexpr int kRasterPart = 512;
 
 // Raster work: (tile, part) items drained through DeviceCounters::itemHead;
 // per tile the first partial block of a tile cut into several parts (-1

@@ -13,10 +13,10 @@
 // (depth, prim * 4 + fan), which is order-independent: visibility is held as
 // 64-bit keys (ordered depth << 32 | prim * 4 + fan) merged with atomic min.
 //
-// Work expansion.  One lane per list entry runs the block tests and builds
-// the entry's candidate mask: pixels (MSAA) or 4x2 groups (1 sample) of the
-// blocks that pass trivial reject, pruned by a monotone corner bound of the
-// pixel test, which is exact.  A workgroup scan turns the masks into a flat
+// Work expansion.  Two lanes per list entry (one per half tile) run the block
+// tests and build the entry's candidate mask: pixels (MSAA) or 4x2 groups (1
+// sample) of the blocks that pass trivial reject, pruned by a monotone corner
+// bound of the pixel test, which is exact.  A workgroup scan turns the masks into a flat
 // list of (entry, pixel or group) units in LDS, and every lane then takes
 // one unit: its sample tests, depths and LDS atomic-min merges.  Lanes stay
 // busy whatever the mix of triangle sizes, which one-triangle-per-lane loops
@@ -25,7 +25,8 @@
 // Scheduling.  A tile's list is cut into parts of at most kPart entries and
 // all parts form one work queue that a persistent grid drains, so a few
 // tiles holding thousands of sub-pixel triangles do not serialise on one
-// compute unit.  Each part rasterises into LDS; a tile with one part writes
+// compute unit.  A part goes through LDS in passes of kPass entries, all
+// merging into the tile's keys.  Each part rasterises into LDS; a tile with one part writes
 // its outputs from LDS, each part of a longer list stores its LDS keys to a
 // private partial block and a second kernel takes the minimum over the parts
 // and writes the outputs (it also clears tiles with an empty list).  No
@@ -42,6 +43,7 @@ namespace {
 using namespace dev;
 
 constexpr int kPart = kRasterPart;  // list entries per work item
+constexpr int kPass = kRasterPass;  // list entries per pass through LDS
 constexpr int kUnits = 4096;  // units expanded into LDS per round
 
 // Depth at a 1-sample pixel for a coverage class (1: SIMD256, 2: SIMD128, 3: scalar).
@@ -86,18 +88,20 @@ __device__ __forceinline__ unsigned scan256(unsigned v, unsigned* sWave, unsigne
     return base + x - v;
 }
 
-// Candidate mask of one list entry in one tile.  S > 1: 256 pixel bits
-// (bit y * 16 + x, eight words); S == 1: 32 group bits (block * 2 + row pair).
-// Also returns the trivially accepted blocks.
+// Candidate mask of one half of a tile (block rows 2h and 2h + 1) for one
+// list entry; two lanes share an entry.  S > 1: 128 pixel bits in four words
+// (word w holds tile rows 4h + 2w' ... as bit (y & 1) * 16 + x of tile words
+// 4h..4h+3); S == 1: 16 group bits ((block - 8h) * 2 + row pair).  Also
+// returns the trivially accepted blocks of the half (bit block - 8h).
 template <int S>
-__device__ __forceinline__ void entry_mask(const Edges& E, bool full, int tx, int ty, const unsigned* fbMask,
-                                           unsigned* m, unsigned& ta) {
-    constexpr int NW = S == 1 ? 1 : 8;
+__device__ __forceinline__ void entry_mask_half(const Edges& E, bool full, int tx, int ty, int h,
+                                                const unsigned* fbMask, unsigned* m, unsigned& ta) {
+    constexpr int NW = S == 1 ? 1 : 4;
     const float tileMinX = (float)(tx * 16) * 1.0f, tileMinY = (float)(ty * 16) * 1.0f;
 #pragma unroll
     for (int w = 0; w < NW; w++) m[w] = 0u;
     if (full) {
-        ta = 0xFFFFu;
+        ta = 0xFFu;
 #pragma unroll
         for (int w = 0; w < NW; w++) m[w] = fbMask[w];
         return;
@@ -105,27 +109,29 @@ __device__ __forceinline__ void entry_mask(const Edges& E, bool full, int tx, in
     unsigned tr = 0;
     ta = 0;
 #pragma unroll
-    for (int bi = 0; bi < 16; bi++) {
+    for (int j = 0; j < 8; j++) {
+        const int bi = 8 * h + j;
         int trn, tan;
         block_test(E, fmaf((float)(bi & 3), 4.0f, tileMinX), fmaf((float)(bi >> 2), 4.0f, tileMinY), trn, tan);
-        tr |= (unsigned)(trn == 3) << bi;
-        ta |= (unsigned)(tan == 3) << bi;
+        tr |= (unsigned)(trn == 3) << j;
+        ta |= (unsigned)(tan == 3) << j;
     }
     while (tr) {
-        const int bi = __ffs(tr) - 1;
+        const int j = __ffs(tr) - 1;
         tr &= tr - 1u;
-        const bool acc = (ta >> bi) & 1u;
+        const int bi = 8 * h + j;
+        const bool acc = (ta >> j) & 1u;
         const float x0 = (float)(tx * 16 + (bi & 3) * 4), y0 = (float)(ty * 16 + (bi >> 2) * 4);
         if constexpr (S == 1) {
             unsigned g = 3u;
             if (!acc)
                 g = (unsigned)rect_may_cover(E, x0, y0, 3.0f, 1.0f) |
                     ((unsigned)rect_may_cover(E, x0, y0 + 2.0f, 3.0f, 1.0f) << 1);
-            m[0] |= g << (bi * 2);
+            m[0] |= g << (j * 2);
         } else {
-            // rows of the block live in words (bi >> 2) * 2 and + 1, at bit
-            // offset ((row & 1) * 16 + (bi & 3) * 4)
-            unsigned lo = 0xF000Fu, hi = 0xF000Fu;  // two rows each
+            // block rows 0-1 go to local word (j >> 2) * 2, rows 2-3 to the next,
+            // at bit offset (row & 1) * 16 + (bi & 3) * 4
+            unsigned lo = 0xF000Fu, hi = 0xF000Fu;
             if (!acc) {
                 lo = 0;
                 hi = 0;
@@ -138,12 +144,12 @@ __device__ __forceinline__ void entry_mask(const Edges& E, bool full, int tx, in
                 }
             }
             const int sh = (bi & 3) * 4;
-#pragma unroll
-            for (int w = 0; w < 4; w++) {
-                if ((bi >> 2) == w) {
-                    m[2 * w] |= lo << sh;
-                    m[2 * w + 1] |= hi << sh;
-                }
+            if (j < 4) {
+                m[0] |= lo << sh;
+                m[1] |= hi << sh;
+            } else {
+                m[2] |= lo << sh;
+                m[3] |= hi << sh;
             }
         }
     }
@@ -332,22 +338,22 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P, TriRecs recs, co
                                                 const unsigned* __restrict__ binStart,
                                                 const unsigned* __restrict__ binCount,
                                                 const unsigned* __restrict__ binList, const uint2* __restrict__ items,
-                                                const int* __restrict__ tilePartial,
+                                                unsigned itemCap, const int* __restrict__ tilePartial,
                                                 unsigned long long* __restrict__ partials,
                                                 float* __restrict__ visDepth, uint32_t* __restrict__ visSlot,
                                                 uint8_t* __restrict__ visClass, StatPartials st,
                                                 DeviceCounters* ctr) {
-    constexpr int NW = S == 1 ? 1 : 8;
+    constexpr int NW = S == 1 ? 1 : 4;  // mask words of one half tile
     __shared__ unsigned long long sVis[S * 256];  // key of sample s of tile pixel p at s * 256 + p
-    __shared__ TriRec sRec[kPart];
-    static_assert(sizeof(TileOut<S>) <= sizeof(TriRec) * kPart, "staging reuses sRec");
-    __shared__ unsigned short sTa[kPart];
+    static_assert(sizeof(TileOut<S>) <= sizeof(sVis), "output staging reuses sVis");
+    __shared__ TriRec sRec[kPass];
+    __shared__ unsigned char sTa[2 * kPass];  // accepted blocks of each half
     __shared__ unsigned short sUnit[kUnits];
     __shared__ unsigned sWave[4];
     __shared__ unsigned sItem;
 
     const int t = threadIdx.x;
-    const unsigned numItems = ctr->numItems;
+    const unsigned numItems = ctr->numItems, numLong = ctr->numLongItems;
     unsigned long long frags = 0, covs = 0;
 
     for (;;) {
@@ -355,7 +361,8 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P, TriRecs recs, co
         __syncthreads();
         const unsigned item = sItem;
         if (item >= numItems) break;
-        const uint2 it = items[item];
+        // long items from the front of the array, then short ones from its back
+        const uint2 it = items[item < numLong ? item : itemCap - 1u - (item - numLong)];
         const int tile = (int)it.x, part = (int)it.y;
         const PixelCtx C = pixel_ctx(P, tile, t & 15, t >> 4);
         const int tx0 = C.tx * 16, ty0 = C.ty * 16;
@@ -369,114 +376,127 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P, TriRecs recs, co
         for (int s = 0; s < S; s++)
             sVis[s * 256 + t] = frag_key(P.clear || !C.inFb ? kClearDepth : visDepth[C.pix * S + s], 0u);
 
-        // framebuffer mask of the tile in the candidate layout
+        // two lanes per list entry: lane h = t & 1 takes block rows 2h, 2h + 1
+        const int e0 = t >> 1, h = t & 1;
+        // framebuffer mask of this half tile in the candidate layout
         const int cols = min(16, P.width - tx0), rows = min(16, P.height - ty0);
         unsigned fbMask[NW];
         if constexpr (S == 1) {
             // groups with at least one pixel inside; pixels are checked again per unit
             unsigned f = 0;
 #pragma unroll
-            for (int g = 0; g < 32; g++)
-                f |= (unsigned)(((g >> 1) & 3) * 4 < cols && ((g >> 3) * 4 + (g & 1) * 2) < rows) << g;
+            for (int j = 0; j < 16; j++) {
+                const int g = 16 * h + j;
+                f |= (unsigned)(((g >> 1) & 3) * 4 < cols && ((g >> 3) * 4 + (g & 1) * 2) < rows) << j;
+            }
             fbMask[0] = f;
         } else {
             const unsigned row = cols >= 16 ? 0xFFFFu : ((1u << cols) - 1u);
 #pragma unroll
-            for (int w = 0; w < 8; w++)
-                fbMask[w] = (2 * w < rows ? row : 0u) | (2 * w + 1 < rows ? row << 16 : 0u);
-        }
-
-        unsigned m[NW];
-        unsigned n = 0;
-        if (t < cnt) {
-            const unsigned ent = binList[start + base + t];
-            const TriRec R = load_rec(recs, ent & 0x7fffffffu);
-            sRec[t] = R;
-            unsigned ta;
-            entry_mask<S>(load_edges(R), (ent >> 31) != 0, C.tx, C.ty, fbMask, m, ta);
-            sTa[t] = (unsigned short)ta;
-#pragma unroll
-            for (int w = 0; w < NW; w++) n += __popc(m[w]);
-        } else {
-#pragma unroll
-            for (int w = 0; w < NW; w++) m[w] = 0u;
-        }
-        unsigned T;
-        const unsigned off = scan256(n, sWave, T);
-
-        for (unsigned w0 = 0; w0 < T; w0 += kUnits) {
-            // write this lane's units that fall in [w0, w0 + kUnits)
-            int pos = (int)off - (int)w0;
-            if (n && pos < kUnits && pos + (int)n > 0) {
-#pragma unroll
-                for (int w = 0; w < NW; w++) {
-                    unsigned bits = m[w];
-                    const int c = __popc(bits);
-                    if (pos + c <= 0 || pos >= kUnits) {
-                        pos += c;
-                        continue;
-                    }
-                    while (bits) {
-                        const int b = __ffs(bits) - 1;
-                        bits &= bits - 1u;
-                        if (pos >= 0 && pos < kUnits) sUnit[pos] = (unsigned short)((t << 8) | (w * 32 + b));
-                        pos++;
-                    }
-                }
+            for (int w = 0; w < 4; w++) {
+                const int r0 = 2 * (4 * h + w);
+                fbMask[w] = (r0 < rows ? row : 0u) | (r0 + 1 < rows ? row << 16 : 0u);
             }
-            __syncthreads();
-            const int nu = (int)min((unsigned)kUnits, T - w0);
-            for (int i = t; i < nu; i += 256) {
-                const unsigned u = sUnit[i];
-                const int e = (int)(u >> 8), c = (int)(u & 255u);
-                const TriRec R = sRec[e];
-                const Edges E = load_edges(R);
-                const unsigned tie = __float_as_uint(R.q2.y);
-                const unsigned ta = sTa[e];
-                if constexpr (S == 1) {
-                    const int bi = c >> 1;
-                    const int px0 = tx0 + (bi & 3) * 4, gy = ty0 + (bi >> 2) * 4 + (c & 1) * 2;
-                    // pixels of the 4x2 group: (0,0) (1,0) (0,1) (1,1) (2,0) (3,0) (2,1) (3,1)
-                    unsigned acc = 0xFFu;
-                    if (!((ta >> bi) & 1u)) {
-                        acc = 0;
+        }
+
+        // passes of kPass entries through LDS, all merging into sVis
+        for (int pb = 0; pb < cnt; pb += kPass) {
+            const int pc = min(kPass, cnt - pb);
+            unsigned m[NW];
+            unsigned n = 0;
+            if (e0 < pc) {
+                const unsigned ent = binList[start + base + pb + e0];
+                const TriRec R = load_rec(recs, ent & 0x7fffffffu);
+                if (h == 0) sRec[e0] = R;
+                unsigned ta;
+                entry_mask_half<S>(load_edges(R), (ent >> 31) != 0, C.tx, C.ty, h, fbMask, m, ta);
+                sTa[t] = (unsigned char)ta;
 #pragma unroll
-                        for (int l = 0; l < 8; l++) {
-                            const int ox = (l & 1) + ((l >> 2) << 1), oy = (l >> 1) & 1;
-                            acc |= (unsigned)inside(E, (float)(px0 + ox), (float)(gy + oy)) << l;
+                for (int w = 0; w < NW; w++) n += __popc(m[w]);
+            } else {
+#pragma unroll
+                for (int w = 0; w < NW; w++) m[w] = 0u;
+            }
+            unsigned T;
+            const unsigned off = scan256(n, sWave, T);
+
+            for (unsigned w0 = 0; w0 < T; w0 += kUnits) {
+                // write this lane's units that fall in [w0, w0 + kUnits)
+                int pos = (int)off - (int)w0;
+                if (n && pos < kUnits && pos + (int)n > 0) {
+#pragma unroll
+                    for (int w = 0; w < NW; w++) {
+                        unsigned bits = m[w];
+                        const int c = __popc(bits);
+                        if (pos + c <= 0 || pos >= kUnits) {
+                            pos += c;
+                            continue;
+                        }
+                        while (bits) {
+                            const int b = __ffs(bits) - 1;
+                            bits &= bits - 1u;
+                            // unit: entry << 8 | tile pixel (S > 1) or group (S == 1)
+                            const int cu = S == 1 ? 16 * h + b : (4 * h + w) * 32 + b;
+                            if (pos >= 0 && pos < kUnits) sUnit[pos] = (unsigned short)((e0 << 8) | cu);
+                            pos++;
                         }
                     }
-#pragma unroll
-                    for (int l = 0; l < 8; l++) {
-                        if (!((acc >> l) & 1u)) continue;
-                        const int px = px0 + (l & 1) + ((l >> 2) << 1), py = gy + ((l >> 1) & 1);
-                        if (px >= P.width || py >= P.height) continue;
-                        const unsigned quad = (acc >> (l & 4)) & 0xFu;
-                        const unsigned cls = acc == 0xFFu ? 1u : (quad == 0xFu ? 2u : 3u);
-                        frags++;
-                        covs++;
-                        merge(&sVis[(py - ty0) * 16 + (px - tx0)],
-                              class_depth(cls, R.f1, R.f2, R.f3, (float)px, (float)py), tie);
-                    }
-                } else {
-                    const int lx = c & 15, ly = c >> 4;
-                    const bool accept = (ta >> ((ly >> 2) * 4 + (lx >> 2))) & 1u;
-                    const float fx = (float)(tx0 + lx), fy = (float)(ty0 + ly);
-                    bool any = false;
-#pragma unroll
-                    for (int s = 0; s < S; s++) {
-                        const float sx = fx + P.sx[s], sy = fy + P.sy[s];
-                        if (!accept && !inside(E, sx, sy)) continue;
-                        any = true;
-                        covs++;
-                        merge(&sVis[s * 256 + c], depth256(R.f1, R.f2, R.f3, sx, sy), tie);
-                    }
-                    frags += any;
                 }
+                __syncthreads();
+                const int nu = (int)min((unsigned)kUnits, T - w0);
+                for (int i = t; i < nu; i += 256) {
+                    const unsigned u = sUnit[i];
+                    const int e = (int)(u >> 8), c = (int)(u & 255u);
+                    const TriRec R = sRec[e];
+                    const Edges E = load_edges(R);
+                    const unsigned tie = __float_as_uint(R.q2.y);
+                    if constexpr (S == 1) {
+                        const int bi = c >> 1;
+                        const unsigned ta = sTa[2 * e + (bi >> 3)];
+                        const int px0 = tx0 + (bi & 3) * 4, gy = ty0 + (bi >> 2) * 4 + (c & 1) * 2;
+                        // pixels of the 4x2 group: (0,0) (1,0) (0,1) (1,1) (2,0) (3,0) (2,1) (3,1)
+                        unsigned acc = 0xFFu;
+                        if (!((ta >> (bi & 7)) & 1u)) {
+                            acc = 0;
+#pragma unroll
+                            for (int l = 0; l < 8; l++) {
+                                const int ox = (l & 1) + ((l >> 2) << 1), oy = (l >> 1) & 1;
+                                acc |= (unsigned)inside(E, (float)(px0 + ox), (float)(gy + oy)) << l;
+                            }
+                        }
+#pragma unroll
+                        for (int l = 0; l < 8; l++) {
+                            if (!((acc >> l) & 1u)) continue;
+                            const int px = px0 + (l & 1) + ((l >> 2) << 1), py = gy + ((l >> 1) & 1);
+                            if (px >= P.width || py >= P.height) continue;
+                            const unsigned quad = (acc >> (l & 4)) & 0xFu;
+                            const unsigned cls = acc == 0xFFu ? 1u : (quad == 0xFu ? 2u : 3u);
+                            frags++;
+                            covs++;
+                            merge(&sVis[(py - ty0) * 16 + (px - tx0)],
+                                  class_depth(cls, R.f1, R.f2, R.f3, (float)px, (float)py), tie);
+                        }
+                    } else {
+                        const int lx = c & 15, ly = c >> 4;
+                        const int bi = (ly >> 2) * 4 + (lx >> 2);
+                        const bool accept = (sTa[2 * e + (bi >> 3)] >> (bi & 7)) & 1u;
+                        const float fx = (float)(tx0 + lx), fy = (float)(ty0 + ly);
+                        bool any = false;
+#pragma unroll
+                        for (int s = 0; s < S; s++) {
+                            const float sx = fx + P.sx[s], sy = fy + P.sy[s];
+                            if (!accept && !inside(E, sx, sy)) continue;
+                            any = true;
+                            covs++;
+                            merge(&sVis[s * 256 + c], depth256(R.f1, R.f2, R.f3, sx, sy), tie);
+                        }
+                        frags += any;
+                    }
+                }
+                __syncthreads();
             }
-            __syncthreads();
+            __syncthreads();  // sRec, sTa and sWave are reused by the next pass
         }
-        __syncthreads();
 
         if (pbase >= 0) {
             // one part of a longer list: its keys, sample-major like sVis
@@ -488,8 +508,9 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P, TriRecs recs, co
 #pragma unroll
             for (int s = 0; s < S; s++) keys[s] = sVis[s * 256 + t];
             if (P.clear) {
-                // sRec is free now: stage the outputs there
-                TileOut<S>* O = reinterpret_cast<TileOut<S>*>(sRec);
+                // every key is in registers: stage the outputs over sVis
+                TileOut<S>* O = reinterpret_cast<TileOut<S>*>(sVis);
+                __syncthreads();
                 stage_pixel<S>(P, recs, slotOf, C, t, keys, *O, visClass);
                 __syncthreads();
                 store_tile<S, false>(P, C.tx, C.ty, O, visDepth, visSlot, visClass);
@@ -574,7 +595,7 @@ void launch_raster(const RasterParams& p, int samples, TriRecs recs, const unsig
                    uint32_t* visSlot, uint8_t* visClass, StatPartials st, DeviceCounters* ctr, hipStream_t s) {
 #define IFRIT_RASTER_LAUNCH(SN)                                                                                     \
     hipLaunchKernelGGL(k_raster<SN>, dim3(q.grid), dim3(256), 0, s, p, recs, slotOf, binStart, binCount, binList, \
-                       q.items, q.tilePartial, q.partials, visDepth, visSlot, visClass, st, ctr)
+                       q.items, q.itemCap, q.tilePartial, q.partials, visDepth, visSlot, visClass, st, ctr)
     switch (samples) {
         case 1: IFRIT_RASTER_LAUNCH(1); break;
         case 4: IFRIT_RASTER_LAUNCH(4); break;

@@ -18,6 +18,15 @@ namespace {
 // ---------------------------------------------------------------------------
 // Deferred shading and resolve
 // ---------------------------------------------------------------------------
+// Hardware transcendental and reciprocal instructions (about 1 ulp): the
+// colour is compared at 1/255, so the IEEE-exact library sequences buy
+// nothing here.  Visibility never goes through this file.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float flog2(float x) { return __builtin_amdgcn_logf(x); }
+__device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+
 struct ShadeTri {
     float4 f1, f2, f3;
     float bx[3], by[3];
@@ -46,7 +55,7 @@ __device__ __forceinline__ void vertex_weights(const ShadeTri& T, float x, float
     const float b1 = fmaf(T.f1.x, x, fmaf(T.f1.y, y, T.f1.z));
     const float b2 = fmaf(T.f2.x, x, fmaf(T.f2.y, y, T.f2.z));
     const float b3 = fmaf(T.f3.x, x, fmaf(T.f3.y, y, T.f3.z));
-    const float inv = 1.0f / (b1 + b2 + b3);
+    const float inv = frcp(b1 + b2 + b3);
     const float B1 = b1 * inv, B2 = b2 * inv, B3 = b3 * inv;
     w[0] = fmaf(B1, T.bx[0], fmaf(B2, T.bx[1], B3 * T.bx[2]));
     w[1] = fmaf(B1, T.by[0], fmaf(B2, T.by[1], B3 * T.by[2]));
@@ -99,18 +108,18 @@ __device__ __forceinline__ TexFootprint footprint(const TexDesc& D, int sampler,
     TexFootprint F{0.0f, 0.0f, 0.0f, 1};
     if (sampler == 0) return F;  // bilinear on level 0
     const float W = (float)D.width, H = (float)D.height;
-    const float px = sqrtf((dudx * W) * (dudx * W) + (dvdx * H) * (dvdx * H));
-    const float py = sqrtf((dudy * W) * (dudy * W) + (dvdy * H) * (dvdy * H));
+    const float px = fsqrt((dudx * W) * (dudx * W) + (dvdx * H) * (dvdx * H));
+    const float py = fsqrt((dudy * W) * (dudy * W) + (dvdy * H) * (dvdy * H));
     const float pmax = fmaxf(px, py), pmin = fminf(px, py);
     if (sampler == 1 || !(pmax > 0.0f)) {
-        F.lod = pmax > 0.0f ? log2f(pmax) : 0.0f;
+        F.lod = pmax > 0.0f ? flog2(pmax) : 0.0f;
         return F;
     }
-    const float ratio = pmin > 0.0f ? pmax / pmin : (float)maxAniso;
+    const float ratio = pmin > 0.0f ? pmax * frcp(pmin) : (float)maxAniso;
     int n = ratio >= (float)maxAniso ? maxAniso : (int)ceilf(ratio);
     n = min(max(1, n), 16);  // the host clamps maxAniso to 16; the tap list holds 4-bit tap indices
     F.n = n;
-    F.lod = log2f(pmax / (float)n);
+    F.lod = flog2(pmax * frcp((float)n));
     F.au = px >= py ? dudx : dudy;
     F.av = px >= py ? dvdx : dvdy;
     return F;
@@ -126,7 +135,7 @@ __device__ __forceinline__ uint32_t flat_hash(uint32_t prim) {
 
 __device__ __forceinline__ float3 norm3(float3 a) {
     const float l2 = a.x * a.x + a.y * a.y + a.z * a.z;
-    const float inv = l2 > 0.0f ? 1.0f / sqrtf(l2) : 1.0f;
+    const float inv = l2 > 0.0f ? frsq(l2) : 1.0f;
     return make_float3(a.x * inv, a.y * inv, a.z * inv);
 }
 
@@ -191,15 +200,17 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, const ShadeTri& T, const
         } else {
             const float3 Lv = make_float3(P.lightPos[l][0] - Pw.x, P.lightPos[l][1] - Pw.y, P.lightPos[l][2] - Pw.z);
             const float d2 = Lv.x * Lv.x + Lv.y * Lv.y + Lv.z * Lv.z;
-            const float inv = 1.0f / sqrtf(d2);
+            const float inv = frsq(d2);
             L = make_float3(Lv.x * inv, Lv.y * inv, Lv.z * inv);
-            att = 1.0f / (1.0f + P.lightColor[l][3] * d2);
+            att = frcp(1.0f + P.lightColor[l][3] * d2);
         }
         const float ndl = N.x * L.x + N.y * L.y + N.z * L.z;
         if (ndl > 0.0f) {
             const float3 Hh = norm3(make_float3(L.x + V.x, L.y + V.y, L.z + V.z));
             const float ndh = fmaxf(N.x * Hh.x + N.y * Hh.y + N.z * Hh.z, 0.0f);
-            const float spec = P.ks * powf(ndh, P.shininess);
+            // ndh^shininess (powf's value at ndh = 0: 1 for shininess 0, else 0)
+            const float pw = ndh > 0.0f ? fexp2(P.shininess * flog2(ndh)) : (P.shininess == 0.0f ? 1.0f : 0.0f);
+            const float spec = P.ks * pw;
             const float d = att * P.kd * ndl;
             A.x = fmaf(d, P.lightColor[l][0], A.x);
             A.y = fmaf(d, P.lightColor[l][1], A.y);
@@ -353,7 +364,7 @@ __global__ void __launch_bounds__(64) k_shade(ShadeParams P, TriRecs recs, const
                 const int fl = (int)(e >> 4), ti = (int)(e & 15u);
                 const float4 uv = cUV[fl];
                 const int tn = cN[fl];
-                const float invn = 1.0f / (float)tn;
+                const float invn = frcp((float)tn);
                 const float to = ((float)ti + 0.5f) * invn - 0.5f;
                 tCol[j] = trilinear(pool, tex[cMat[fl]], fmaf(uv.z, to, uv.x), fmaf(uv.w, to, uv.y), cLod[fl]);
             }
@@ -370,7 +381,7 @@ __global__ void __launch_bounds__(64) k_shade(ShadeParams P, TriRecs recs, const
         if (act) {
             float4 alb = make_float4(1, 1, 1, 1);
             if (R.mat >= 0) {
-                const float invn = 1.0f / (float)n;
+                const float invn = frcp((float)n);
                 alb = make_float4(acc.x * invn, acc.y * invn, acc.z * invn, acc.w * invn);
             }
             cCol[lane] = P.program == 0 ? make_float4(sat(R.B.x), sat(R.B.y), sat(R.B.z), 1.0f)
