@@ -482,6 +482,7 @@ void TileRasterRenderer::enqueue(bool clear) {
         sh.offWorld = d.offWorld;
         sh.offNormal = d.offNormal;
         sh.offUv = d.offUv;
+        sh.offTangent = d.offTangent;
         sh.trisPerMaterial = d.trisPerMaterial;
         sh.sampler = d.sampler;
         sh.maxAniso = std::min(16, std::max(1, d.maxAniso));  // the tap list encodes up to 16 taps

@@ -74,7 +74,7 @@ struct TexDesc {
 struct ShadeParams {
     int width, height, samples;
     int program;
-    int stride, offWorld, offNormal, offUv;
+    int stride, offWorld, offNormal, offUv, offTangent;
     int trisPerMaterial;
     int sampler, maxAniso;
     int numTextures;

@@ -148,7 +148,31 @@ struct FragPrep {
     float u, v;
     TexFootprint fp;
     int mat;  // texture index, -1 when untextured
+    // layered GGX: shading frame, view direction and position
+    float3 N, Tg, Bg, V, Pw;
 };
+
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// Layered GGX of one light (Trowbridge-Reitz with alpha = roughness,
+// Smith-Schlick k = alpha / 2, Schlick Fresnel F0 = 0.04, Lambert diffuse
+// weighted by 1 - F), times n.l.
+__device__ __forceinline__ float3 ggx_light(float3 n, float3 V, float3 L, float4 alb, float alpha, float kd, float ks) {
+    const float ndl = dot3(n, L), ndv = dot3(n, V);
+    if (!(ndl > 0.0f && ndv > 0.0f)) return make_float3(0, 0, 0);
+    const float3 H = norm3(make_float3(L.x + V.x, L.y + V.y, L.z + V.z));
+    const float ndh = fmaxf(dot3(n, H), 0.0f), vdh = fmaxf(dot3(V, H), 0.0f);
+    const float a2 = alpha * alpha;
+    const float den = ndh * ndh * (a2 - 1.0f) + 1.0f;
+    const float D = a2 * frcp(3.14159265f * den * den);
+    const float k = alpha * 0.5f;
+    const float G = ndv * frcp(ndv * (1.0f - k) + k) * (ndl * frcp(ndl * (1.0f - k) + k));
+    const float f = 1.0f - vdh, f2 = f * f;
+    const float F = 0.04f + 0.96f * (f2 * f2 * f);
+    const float spec = ks * (D * G * F * frcp(4.0f * ndl * ndv));
+    const float diff = kd * (1.0f - F) * (1.0f / 3.14159265f);
+    return make_float3((alb.x * diff + spec) * ndl, (alb.y * diff + spec) * ndl, (alb.z * diff + spec) * ndl);
+}
 
 __device__ FragPrep prep_fragment(const ShadeParams& P, const ShadeTri& T, const float* vtx, const int* idx,
                                   const TexDesc* tex, int px, int py, float o) {
@@ -172,6 +196,19 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, const ShadeTri& T, const
     auto interp = [&](int off) { return fmaf(w[0], v0[off], fmaf(w[1], v1[off], w[2] * v2[off])); };
     const float3 Pw = make_float3(interp(P.offWorld), interp(P.offWorld + 1), interp(P.offWorld + 2));
     float3 N = make_float3(interp(P.offNormal), interp(P.offNormal + 1), interp(P.offNormal + 2));
+    if (P.program == 3) {
+        // shading frame: interpolated normal, tangent made orthogonal to it, bitangent N x T
+        N = norm3(N);
+        float3 Tg = make_float3(interp(P.offTangent), interp(P.offTangent + 1), interp(P.offTangent + 2));
+        const float nt = dot3(N, Tg);
+        Tg = norm3(make_float3(Tg.x - N.x * nt, Tg.y - N.y * nt, Tg.z - N.z * nt));
+        R.N = N;
+        R.Tg = Tg;
+        R.Bg = make_float3(N.y * Tg.z - N.z * Tg.y, N.z * Tg.x - N.x * Tg.z, N.x * Tg.y - N.y * Tg.x);
+        R.Pw = Pw;
+        R.V = norm3(make_float3(P.camera[0] - Pw.x, P.camera[1] - Pw.y, P.camera[2] - Pw.z));
+        R.A = R.B = make_float3(0, 0, 0);
+    }
     if (P.numTextures > 0) {
         R.u = interp(P.offUv);
         R.v = interp(P.offUv + 1);
@@ -186,9 +223,10 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, const ShadeTri& T, const
         const float dudx = iuv(wb, P.offUv) - iuv(wa, P.offUv), dvdx = iuv(wb, P.offUv + 1) - iuv(wa, P.offUv + 1);
         const float dudy = iuv(wd, P.offUv) - iuv(wc, P.offUv), dvdy = iuv(wd, P.offUv + 1) - iuv(wc, P.offUv + 1);
         int mat = P.trisPerMaterial > 0 ? T.prim / P.trisPerMaterial : 0;
-        R.mat = min(mat, P.numTextures - 1);
+        R.mat = P.program == 3 ? 0 : min(mat, P.numTextures - 1);  // the layers share one footprint
         R.fp = footprint(tex[R.mat], P.sampler, P.maxAniso, dudx, dvdx, dudy, dvdy);
     }
+    if (P.program == 3) return R;  // lighting needs the sampled layers
     N = norm3(N);
     const float3 V = norm3(make_float3(P.camera[0] - Pw.x, P.camera[1] - Pw.y, P.camera[2] - Pw.z));
     float3 A = make_float3(P.ambient, P.ambient, P.ambient), B = make_float3(0, 0, 0);
@@ -227,6 +265,36 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, const ShadeTri& T, const
 
 __device__ __forceinline__ float sat(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
 
+// Clamped colour of a layered-GGX fragment from its filtered layers: albedo,
+// tangent-space normal (xyz, encoded as 0.5 n + 0.5) and roughness (w).
+__device__ float4 ggx_color(const ShadeParams& P, const FragPrep& R, float4 alb, float4 nm) {
+    const float3 nts = norm3(make_float3(2.0f * nm.x - 1.0f, 2.0f * nm.y - 1.0f, 2.0f * nm.z - 1.0f));
+    const float3 n = norm3(make_float3(R.Tg.x * nts.x + R.Bg.x * nts.y + R.N.x * nts.z,
+                                       R.Tg.y * nts.x + R.Bg.y * nts.y + R.N.y * nts.z,
+                                       R.Tg.z * nts.x + R.Bg.z * nts.y + R.N.z * nts.z));
+    const float alpha = 0.05f + 0.95f * nm.w;
+    float3 col = make_float3(alb.x * P.ambient, alb.y * P.ambient, alb.z * P.ambient);
+    for (int l = 0; l < P.numLights; l++) {
+        float3 L;
+        float att = 1.0f;
+        if (P.lightPos[l][3] == 0.0f) {
+            L = norm3(make_float3(P.lightPos[l][0], P.lightPos[l][1], P.lightPos[l][2]));
+        } else {
+            const float3 Lv = make_float3(P.lightPos[l][0] - R.Pw.x, P.lightPos[l][1] - R.Pw.y,
+                                          P.lightPos[l][2] - R.Pw.z);
+            const float d2 = dot3(Lv, Lv);
+            const float inv = frsq(d2);
+            L = make_float3(Lv.x * inv, Lv.y * inv, Lv.z * inv);
+            att = frcp(1.0f + P.lightColor[l][3] * d2);
+        }
+        const float3 c = ggx_light(n, R.V, L, alb, alpha, P.kd, P.ks);
+        col.x += att * P.lightColor[l][0] * c.x;
+        col.y += att * P.lightColor[l][1] * c.y;
+        col.z += att * P.lightColor[l][2] * c.z;
+    }
+    return make_float4(sat(col.x), sat(col.y), sat(col.z), sat(alb.w));
+}
+
 __device__ __forceinline__ int wave_incl_scan(int v) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -263,6 +331,7 @@ __global__ void __launch_bounds__(64) k_shade(ShadeParams P, TriRecs recs, const
     __shared__ float4 cCol[64];     // clamped colour of the chunk's fragments
     __shared__ unsigned short tList[kTapWindow];
     __shared__ float4 tCol[kTapWindow];
+    __shared__ float4 tCol2[kTapWindow];  // layered GGX: normal xyz and roughness of each tap
 
     // consecutive tiles on one XCD share its L2 (texture and vertex reuse)
     const int tile = (int)(blockIdx.x & 7u) * perXcd + (int)(blockIdx.x >> 3);
@@ -350,7 +419,7 @@ __global__ void __launch_bounds__(64) k_shade(ShadeParams P, TriRecs recs, const
         const int tincl = wave_incl_scan(n);
         const int toff = tincl - n;
         const int TT = __shfl(tincl, 63, 64);
-        float4 acc = make_float4(0, 0, 0, 0);
+        float4 acc = make_float4(0, 0, 0, 0), acc2 = make_float4(0, 0, 0, 0);
         for (int w0 = 0; w0 < TT; w0 += kTapWindow) {
             __syncthreads();
             for (int i = 0; i < n; i++) {
@@ -366,7 +435,13 @@ __global__ void __launch_bounds__(64) k_shade(ShadeParams P, TriRecs recs, const
                 const int tn = cN[fl];
                 const float invn = frcp((float)tn);
                 const float to = ((float)ti + 0.5f) * invn - 0.5f;
-                tCol[j] = trilinear(pool, tex[cMat[fl]], fmaf(uv.z, to, uv.x), fmaf(uv.w, to, uv.y), cLod[fl]);
+                const float tu = fmaf(uv.z, to, uv.x), tv = fmaf(uv.w, to, uv.y);
+                tCol[j] = trilinear(pool, tex[cMat[fl]], tu, tv, cLod[fl]);
+                if (P.program == 3) {
+                    const float4 nm = trilinear(pool, tex[1], tu, tv, cLod[fl]);
+                    const float4 rg = trilinear(pool, tex[2], tu, tv, cLod[fl]);
+                    tCol2[j] = make_float4(nm.x, nm.y, nm.z, rg.x);
+                }
             }
             __syncthreads();
             const int b0 = max(toff - w0, 0), b1 = min(toff + n - w0, kTapWindow);
@@ -376,6 +451,13 @@ __global__ void __launch_bounds__(64) k_shade(ShadeParams P, TriRecs recs, const
                 acc.y += c.y;
                 acc.z += c.z;
                 acc.w += c.w;
+                if (P.program == 3) {
+                    const float4 c2 = tCol2[j];
+                    acc2.x += c2.x;
+                    acc2.y += c2.y;
+                    acc2.z += c2.z;
+                    acc2.w += c2.w;
+                }
             }
         }
         if (act) {
@@ -384,9 +466,19 @@ __global__ void __launch_bounds__(64) k_shade(ShadeParams P, TriRecs recs, const
                 const float invn = frcp((float)n);
                 alb = make_float4(acc.x * invn, acc.y * invn, acc.z * invn, acc.w * invn);
             }
-            cCol[lane] = P.program == 0 ? make_float4(sat(R.B.x), sat(R.B.y), sat(R.B.z), 1.0f)
-                                        : make_float4(sat(fmaf(alb.x, R.A.x, R.B.x)), sat(fmaf(alb.y, R.A.y, R.B.y)),
-                                                      sat(fmaf(alb.z, R.A.z, R.B.z)), sat(alb.w));
+            if (P.program == 3) {
+                float4 nm = make_float4(0.5f, 0.5f, 1.0f, 0.0f);
+                if (R.mat >= 0) {
+                    const float invn = frcp((float)n);
+                    nm = make_float4(acc2.x * invn, acc2.y * invn, acc2.z * invn, acc2.w * invn);
+                }
+                cCol[lane] = ggx_color(P, R, alb, nm);
+            } else {
+                cCol[lane] = P.program == 0 ? make_float4(sat(R.B.x), sat(R.B.y), sat(R.B.z), 1.0f)
+                                            : make_float4(sat(fmaf(alb.x, R.A.x, R.B.x)),
+                                                          sat(fmaf(alb.y, R.A.y, R.B.y)),
+                                                          sat(fmaf(alb.z, R.A.z, R.B.z)), sat(alb.w));
+            }
         }
         __syncthreads();
         // 3. the pixel's fragments of this chunk, in sample order

@@ -28,7 +28,7 @@ class FragmentProgramDesc(C.Structure):
                 ("trisPerMaterial", C.c_int32), ("sampler", C.c_int32), ("maxAniso", C.c_int32),
                 ("camera", C.c_float * 3), ("ambient", C.c_float), ("kd", C.c_float), ("ks", C.c_float),
                 ("shininess", C.c_float), ("numLights", C.c_int32), ("lightPos", (C.c_float * 4) * 4),
-                ("lightColor", (C.c_float * 4) * 4)]
+                ("lightColor", (C.c_float * 4) * 4), ("offTangent", C.c_int32)]
 
 
 class FrameStats(C.Structure):
@@ -172,6 +172,7 @@ class Renderer:
         d = FragmentProgramDesc()
         d.kind = scene.program
         d.offWorld, d.offNormal, d.offUv = 0, 3, 6
+        d.offTangent = 8 if scene.stride >= 11 else 0
         d.trisPerMaterial = scene.tris_per_material
         d.sampler, d.maxAniso = scene.sampler, scene.max_aniso
         d.camera[:] = [float(x) for x in scene.camera]

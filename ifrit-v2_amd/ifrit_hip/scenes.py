@@ -24,6 +24,7 @@ STRIDE_POS = 3
 PROGRAM_FLAT = 0
 PROGRAM_BLINN_PHONG = 1
 PROGRAM_DEPTH_ONLY = 2
+PROGRAM_GGX_LAYERED = 3
 SAMPLER_BILINEAR = 0
 SAMPLER_TRILINEAR = 1
 SAMPLER_ANISO = 2
@@ -172,15 +173,15 @@ def _hash2(ix: np.ndarray, iy: np.ndarray, seed: int) -> np.ndarray:
     return h
 
 
-def fbm_texture(size: int, seed: int, octaves: int = 6) -> np.ndarray:
+def fbm_texture(size: int, seed: int, octaves: int = 6, channels: int = 3) -> np.ndarray:
     """Tileable value-noise fBm in 16-bit fixed point; RGBA8 level 0.
 
     Octave o has cells of size >> (o + 2) texels; corner values are 16-bit
     hashes; bilinear weights are exact 16-bit fractions; octave o contributes
     value >> (o + 1).  Channel c uses seed * 4 + c; alpha is 255."""
     y, x = np.meshgrid(np.arange(size, dtype=np.int64), np.arange(size, dtype=np.int64), indexing="ij")
-    out = np.empty((size, size, 4), np.uint8)
-    for ch in range(3):
+    out = np.zeros((size, size, 4), np.uint8)
+    for ch in range(channels):
         acc = np.zeros((size, size), np.int64)
         for o in range(octaves):
             cell = max(1, size >> (o + 2))
@@ -199,6 +200,20 @@ def fbm_texture(size: int, seed: int, octaves: int = 6) -> np.ndarray:
             n = (top * (65536 - wy) + bot * wy) >> 32
             acc += n >> (o + 1)
         out[..., ch] = np.minimum(acc >> 8, 255).astype(np.uint8)
+    out[..., 3] = 255
+    return out
+
+
+def normal_map(height: np.ndarray, strength: float = 4.0) -> np.ndarray:
+    """Tangent-space normal map (RGBA8, n encoded as 0.5 n + 0.5) from a
+    uint8 height field with wrap-around central differences."""
+    h = height.astype(np.float64) / 255.0
+    dx = (np.roll(h, -1, axis=1) - np.roll(h, 1, axis=1)) * strength
+    dy = (np.roll(h, -1, axis=0) - np.roll(h, 1, axis=0)) * strength
+    n = np.stack([-dx, -dy, np.ones_like(h)], -1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    out = np.empty(h.shape + (4,), np.uint8)
+    out[..., :3] = np.floor((n * 0.5 + 0.5) * 255.0 + 0.5).astype(np.uint8)
     out[..., 3] = 255
     return out
 
@@ -298,6 +313,56 @@ def config2(width: int = 1920, height: int = 1080, samples: int = 8, spheres: in
     return Scene("config2", width, height, samples, verts, idx, _mvp(eye, (0, 0, 0), 60.0, width, height),
                  program=PROGRAM_BLINN_PHONG, sampler=SAMPLER_ANISO, max_aniso=16, tris_per_material=nt,
                  camera=eye, lights=lights, textures=textures, ambient=0.08, kd=0.9, ks=0.25)
+
+
+def sphere_tangents(p: np.ndarray) -> np.ndarray:
+    """Unit tangents d p / d u of the icosphere parameterisation
+    (u = atan2(z, x) / 2 pi + 0.5): direction (-z, 0, x), (1, 0, 0) at the poles."""
+    t = np.stack([-p[:, 2], np.zeros(len(p)), p[:, 0]], -1)
+    n = np.linalg.norm(t, axis=1, keepdims=True)
+    return np.where(n > 1e-9, t / np.maximum(n, 1e-300), np.array([1.0, 0.0, 0.0]))
+
+
+def material_layers(size: int, seed: int, host: bool = True):
+    """The three 2^k x 2^k layers of the layered-GGX material: fBm albedo,
+    a normal map from an fBm height field, fBm roughness (red channel)."""
+    rng = np.random.default_rng(seed)
+    sa, sh, sr = (int(x) for x in rng.integers(0, 2 ** 31, 3))
+    levels = num_levels(size, size)
+    albedo = Texture(size, size, levels, None, sa)
+    rough = Texture(size, size, levels, None, sr)
+    if host:
+        albedo.texels = build_mips(fbm_texture(size, sa))
+        rough.texels = build_mips(fbm_texture(size, sr, channels=1))
+    normal = Texture(size, size, levels, build_mips(normal_map(fbm_texture(size, sh, channels=1)[..., 0])))
+    return [albedo, normal, rough]
+
+
+def config5(width: int = 1920, height: int = 1080, samples: int = 1, spheres: int = 256, level: int = 4,
+            tex_size: int = 2048, seed: int = 20241202, host_textures: bool = False) -> Scene:
+    """Config-2 geometry with per-vertex tangents (stride 11: pos, normal, uv,
+    tangent) and one layered-GGX material shared by every sphere (albedo,
+    tangent-space normal, roughness layers at tex_size^2 with mips, 16x
+    anisotropic filtering), one directional light plus ambient; 1 sample per
+    pixel for the shade pass."""
+    base = config2(width, height, samples, spheres, level, tex_size, seed)
+    rng = np.random.default_rng(seed)
+    p, uv, f = icosphere(level)
+    tl = sphere_tangents(p)
+    nv = p.shape[0]
+    verts = np.empty((spheres * nv, 11), np.float32)
+    verts[:, :8] = base.vertices
+    for s in range(spheres):
+        # the same draws as config2, in the same order
+        rng.uniform(-20, 20, 3)
+        rng.uniform(math.log(0.2), math.log(3.0))
+        rot = random_rotation(rng)
+        verts[s * nv:(s + 1) * nv, 8:11] = tl @ rot.T
+    eye = base.camera
+    return Scene("config5", width, height, samples, verts, base.indices, base.mvp, program=PROGRAM_GGX_LAYERED,
+                 sampler=SAMPLER_ANISO, max_aniso=16, camera=eye, ambient=0.1, kd=1.0, ks=1.0,
+                 lights=[((0.4, 0.8, -0.45, 0.0), (3.6, 3.45, 3.3, 0.0))],
+                 textures=material_layers(tex_size, seed + 5, host=host_textures))
 
 
 def config3(width: int = 1920, height: int = 1080, samples: int = 4, gx: int = 2048, gy: int = 1024,

@@ -134,7 +134,9 @@ enum {
 enum {
     IFHIP_FS_FLAT = 0,        /* colour = bytes of a hash of the primitive id */
     IFHIP_FS_BLINN_PHONG = 1, /* textured Blinn-Phong, directional and point lights */
-    IFHIP_FS_DEPTH_ONLY = 2   /* visibility and depth only */
+    IFHIP_FS_DEPTH_ONLY = 2,  /* visibility and depth only */
+    IFHIP_FS_GGX_LAYERED = 3  /* GGX over three layers: texture 0 albedo, 1 tangent-space normal,
+                                 2 roughness (red); needs a tangent in the vertex record */
 };
 enum { IFHIP_SAMPLER_BILINEAR = 0, IFHIP_SAMPLER_TRILINEAR = 1, IFHIP_SAMPLER_ANISOTROPIC = 2 };
 
@@ -154,6 +156,7 @@ typedef struct {
     int32_t numLights;          /* <= 4 */
     float lightPos[4][4];       /* w = 0: direction towards the light, w = 1: point light */
     float lightColor[4][4];     /* rgb, attenuation k of 1 / (1 + k d^2) */
+    int32_t offTangent;         /* float offset of the tangent (xyz), IFHIP_FS_GGX_LAYERED */
 } IfritGpuFragmentProgramDesc;
 
 void* iftrCreateGpuVertexProgram(const IfritGpuVertexProgramDesc* desc);

@@ -51,7 +51,7 @@ class ShadeParams(C.Structure):
                 ("off_uv", C.c_int), ("tris_per_material", C.c_int), ("sampler", C.c_int), ("max_aniso", C.c_int),
                 ("camera", C.c_float * 3), ("ambient", C.c_float), ("kd", C.c_float), ("ks", C.c_float),
                 ("shininess", C.c_float), ("num_lights", C.c_int), ("light_pos", (C.c_float * 4) * 4),
-                ("light_color", (C.c_float * 4) * 4)]
+                ("light_color", (C.c_float * 4) * 4), ("off_tangent", C.c_int)]
 
 
 class TextureDesc(C.Structure):
@@ -170,6 +170,7 @@ class Frame:
         p.program = scene.program
         p.stride_floats = scene.stride
         p.off_world, p.off_normal, p.off_uv = 0, 3, 6
+        p.off_tangent = 8 if scene.stride >= 11 else 0
         p.tris_per_material = scene.tris_per_material
         p.sampler, p.max_aniso = scene.sampler, scene.max_aniso
         p.camera[:] = list(scene.camera)

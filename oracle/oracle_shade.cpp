@@ -31,6 +31,29 @@ inline D3 normalize(D3 a) {
     const double l = std::sqrt(dot(a, a));
     return l > 0 ? a * (1.0 / l) : a;
 }
+inline D3 cross(D3 a, D3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+// Layered GGX (the build's config-5 reference material): Trowbridge-Reitz
+// distribution with alpha = roughness, Smith-Schlick geometry with
+// k = alpha / 2, Schlick Fresnel with F0 = 0.04, Lambert diffuse weighted by
+// (1 - F); roughness = 0.05 + 0.95 * texture red.
+const double kPi = 3.14159265358979323846;
+D3 ggx_light(D3 n, D3 V, D3 L, D3 alb, double alpha, double kd, double ks, bool* flag) {
+    const double ndl = dot(n, L), ndv = dot(n, V);
+    if (std::fabs(ndl) < 1e-4 || std::fabs(ndv) < 1e-4) *flag = true;
+    if (!(ndl > 0 && ndv > 0)) return {0, 0, 0};
+    const D3 H = normalize(L + V);
+    const double ndh = std::max(dot(n, H), 0.0), vdh = std::max(dot(V, H), 0.0);
+    const double a2 = alpha * alpha;
+    const double den = ndh * ndh * (a2 - 1.0) + 1.0;
+    const double D = a2 / (kPi * den * den);
+    const double k = alpha * 0.5;
+    const double G = (ndv / (ndv * (1.0 - k) + k)) * (ndl / (ndl * (1.0 - k) + k));
+    const double F = 0.04 + 0.96 * std::pow(1.0 - vdh, 5.0);
+    const double spec = D * G * F / (4.0 * ndl * ndv);
+    const double diff = kd * (1.0 - F) / kPi;
+    return D3{alb.x * diff + ks * spec, alb.y * diff + ks * spec, alb.z * diff + ks * spec} * ndl;
+}
 
 struct Tex {
     const uint8_t* base;
@@ -161,7 +184,7 @@ struct Shader {
             rgba[3] = 1.0;
             return false;
         }
-        if (p->program != ORP_BLINN_PHONG) {
+        if (p->program != ORP_BLINN_PHONG && p->program != ORP_GGX_LAYERED) {
             rgba[0] = rgba[1] = rgba[2] = rgba[3] = 0;
             return false;
         }
@@ -185,6 +208,51 @@ struct Shader {
         uv_at(t, qx + 1 + o, y, b);
         uv_at(t, x, qy + o, c);
         uv_at(t, x, qy + 1 + o, d);
+        if (p->program == ORP_GGX_LAYERED) {
+            D3 T{0, 0, 0};
+            for (int k = 0; k < 3; k++) {
+                const float* v = vert(t.prim, k);
+                T = T + D3{v[p->off_tangent], v[p->off_tangent + 1], v[p->off_tangent + 2]} * w[k];
+            }
+            double alb[4] = {1, 1, 1, 1}, nm[4] = {0.5, 0.5, 1, 1}, rg[4] = {0, 0, 0, 0};
+            bool flag = false;
+            if (tex->size() >= 3) {
+                const double du = b[0] - a[0], dv = b[1] - a[1], eu = d[0] - c[0], ev = d[1] - c[1];
+                flag |= sample_tex((*tex)[0], p->sampler, p->max_aniso, uv[0], uv[1], du, dv, eu, ev, alb);
+                flag |= sample_tex((*tex)[1], p->sampler, p->max_aniso, uv[0], uv[1], du, dv, eu, ev, nm);
+                flag |= sample_tex((*tex)[2], p->sampler, p->max_aniso, uv[0], uv[1], du, dv, eu, ev, rg);
+            }
+            N = normalize(N);
+            T = normalize(T - N * dot(N, T));
+            const D3 B = cross(N, T);
+            const D3 nts = normalize(D3{2 * nm[0] - 1, 2 * nm[1] - 1, 2 * nm[2] - 1});
+            const D3 n = normalize(T * nts.x + B * nts.y + N * nts.z);
+            const double alpha = 0.05 + 0.95 * rg[0];
+            const D3 V = normalize(D3{p->camera[0], p->camera[1], p->camera[2]} - P);
+            const D3 A{alb[0], alb[1], alb[2]};
+            D3 col = A * p->ambient;
+            for (int l = 0; l < p->num_lights; l++) {
+                const float* lp = p->light_pos[l];
+                const float* lc = p->light_color[l];
+                D3 L;
+                double att = 1.0;
+                if (lp[3] == 0.0f) {
+                    L = normalize(D3{lp[0], lp[1], lp[2]});
+                } else {
+                    const D3 Lv = D3{lp[0], lp[1], lp[2]} - P;
+                    const double d2 = dot(Lv, Lv);
+                    L = Lv * (1.0 / std::sqrt(d2));
+                    att = 1.0 / (1.0 + lc[3] * d2);
+                }
+                const D3 c = ggx_light(n, V, L, A, alpha, p->kd, p->ks, &flag);
+                col = col + D3{c.x * lc[0], c.y * lc[1], c.z * lc[2]} * att;
+            }
+            rgba[0] = col.x;
+            rgba[1] = col.y;
+            rgba[2] = col.z;
+            rgba[3] = alb[3];
+            return flag;
+        }
         const int mat = p->tris_per_material > 0 ? t.prim / p->tris_per_material : 0;
         double alb[4] = {1, 1, 1, 1};
         bool flag = false;

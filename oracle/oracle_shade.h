@@ -23,7 +23,8 @@ extern "C" {
 enum OracleProgram {
     ORP_FLAT = 0,         // colour = bytes of hash(prim), alpha 255
     ORP_BLINN_PHONG = 1,  // textured Blinn-Phong, directional and point lights
-    ORP_DEPTH_ONLY = 2    // no colour output
+    ORP_DEPTH_ONLY = 2,   // no colour output
+    ORP_GGX_LAYERED = 3   // layered GGX: texture 0 albedo, 1 tangent-space normal, 2 roughness
 };
 enum OracleSampler { ORS_BILINEAR = 0, ORS_TRILINEAR = 1, ORS_ANISO = 2 };
 
@@ -41,6 +42,7 @@ typedef struct {
     int num_lights;         // <= 4
     float light_pos[4][4];  // w = 0: direction towards the light; w = 1: point light position
     float light_color[4][4];  // rgb, attenuation coefficient k in 1 / (1 + k d^2)
+    int off_tangent;        // float offset of the tangent (xyz), ORP_GGX_LAYERED
 } OracleShadeParams;
 
 typedef struct {

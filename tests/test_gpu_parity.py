@@ -122,6 +122,26 @@ def test_config2_scaled_visibility_and_color(renderer_factory):
     assert bad.sum() == 0
 
 
+@pytest.mark.parametrize("samples", [1, 4])
+def test_config5_layered_ggx_color(renderer_factory, samples):
+    """Layered GGX (albedo, tangent-space normal and roughness layers, 16x
+    anisotropic) against the FP64 oracle on the same visibility."""
+    s = scenes.config5(width=480, height=270, samples=samples, spheres=48, level=3, tex_size=256,
+                       host_textures=True)
+    f, _ = O.render_scene(s)
+    r, st, ids, depth = _render_gpu(renderer_factory, s)
+    _compare_vis(s, f, ids, depth, f"config5 scaled s={samples}")
+    rgba = r.color()
+    ref, flags = f.shade(s)
+    diff = np.abs(rgba.astype(int) - ref.astype(int)).max(-1)
+    bad = (diff > 1) & (flags == 0)
+    covered = (f.ids.reshape(s.height, s.width, samples) != 0xFFFFFFFF).any(-1)
+    print(f"[config5 scaled s={samples}] covered {covered.sum()} colour max diff {diff.max()} "
+          f">1LSB unflagged {bad.sum()} flagged {flags.sum()}", flush=True)
+    assert covered.sum() > 1000
+    assert bad.sum() == 0
+
+
 def test_fbm_and_mips_match_host_definition(renderer_factory):
     r = renderer_factory()
     r.set_target(16, 16, 1)
