@@ -25,7 +25,9 @@ SOURCES = [
     "kernels/raster.hip",
     "kernels/shade.hip",
     "kernels/texture.hip",
+    "kernels/neural.hip",
     "host/renderer.cpp",
+    "host/neural.cpp",
     "host/capi.cpp",
 ]
 

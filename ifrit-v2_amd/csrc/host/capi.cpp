@@ -285,4 +285,21 @@ void iftrSynchronize(IfritRendererHandle* h) {
 }
 const char* iftrGetLastError(IfritRendererHandle* h) { return h ? h->renderer.lastError() : "null renderer handle"; }
 
+void iftrNeuralTrain(IfritRendererHandle* h, const IfritNeuralTrainDesc* d, IfritNeuralTrainReport* report) {
+    if (!d) return;
+    guarded(h, [&](TileRasterRenderer& r) { r.trainNeural(*d, report); });
+}
+void iftrNeuralSetParams(IfritRendererHandle* h, const float* params) {
+    if (!params) return;
+    guarded(h, [&](TileRasterRenderer& r) { r.setNeuralParams(params); });
+}
+void iftrNeuralGetParams(IfritRendererHandle* h, float* params) {
+    if (!params) return;
+    guarded(h, [&](TileRasterRenderer& r) { r.getNeuralParams(params); });
+}
+void iftrNeuralEval(IfritRendererHandle* h, const float* in, float* out, int n, int precision) {
+    if (!in || !out) return;
+    guarded(h, [&](TileRasterRenderer& r) { r.evalNeural(in, out, n, precision); });
+}
+
 }  // extern "C"

@@ -51,7 +51,9 @@ TileRasterRenderer::TileRasterRenderer() {}
 TileRasterRenderer::~TileRasterRenderer() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (DevBuf* b : {&vtx_, &idx_, &clip_, &recs_[0], &recs_[1], &recs_[2], &recs_[3], &recs_[4], &recs_[5], &bary_, &tileBox_, &slotOf_, &binCount_, &binStart_, &binCursor_, &binList_,
-                      &bigList_, &ctr_, &clipList_, &statPart_, &items_, &tilePartial_, &multiList_, &partials_, &visId_, &visDepth_, &visSlot_, &visClass_, &rgba8_, &rgba32f_, &texDesc_})
+                      &bigList_, &ctr_, &clipList_, &statPart_, &items_, &tilePartial_, &multiList_, &partials_, &visId_, &visDepth_, &visSlot_, &visClass_, &rgba8_, &rgba32f_, &texDesc_, &nnParams_, &nnGrads_, &nnM_, &nnV_,
+                      &nnPacked_, &nnQinfo_, &lat32_, &latGrad_, &latM_, &latV_, &lat16_, &nnData_, &nnFeat_, &nnCount_,
+                      &nnLoss_, &nnEvalIn_, &nnEvalOut_})
         b->release();
     for (auto& t : textures_)
         if (t.dev) (void)hipFree(t.dev);
@@ -340,6 +342,14 @@ bool TileRasterRenderer::prepare(int indexCount) {
         setError("no vertex program bound");
         return false;
     }
+    if (fsBound_ && fs_.desc.kind == IFHIP_FS_NEURAL && (!nnReady_ || textures_.empty())) {
+        setError("the neural material has not been trained (iftrNeuralTrain) or its textures are not bound");
+        return false;
+    }
+    if (fsBound_ && fs_.desc.kind >= IFHIP_FS_GGX_LAYERED && (stride_ < 11 || fs_.desc.offTangent + 3 > stride_)) {
+        setError("the layered-GGX and neural programs need a tangent in the vertex record");
+        return false;
+    }
     if (!hostVs_ && vs_.desc.kind == IFHIP_VS_CLIP_PASSTHROUGH && stride_ < 4) {
         setError("the clip pass-through vertex program needs 4 floats per vertex");
         return false;
@@ -498,6 +508,7 @@ void TileRasterRenderer::enqueue(bool clear) {
         const bool wantFloat = fb_ && !fb_->color.empty() && fb_->color[0];
         sh.writeFloat = wantFloat ? 1 : 0;
         sh.ntri = curTris_;
+        sh.neural = neuralShadeParams();
         launch_shade(sh, recPtrs(), bary_.as<float4>(), visSlot_.as<uint32_t>(), vtx_.as<float>(),
                      idx_.as<int>(), texDesc_.as<TexDesc>(), nullptr, rgba8_.as<uchar4>(),
                      wantFloat ? rgba32f_.as<float4>() : nullptr, s);

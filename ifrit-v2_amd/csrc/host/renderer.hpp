@@ -128,6 +128,12 @@ public:
     void setError(const std::string& e);
     bool ok(hipError_t e, const char* what);
 
+    // neural material (neural.cpp)
+    void trainNeural(const IfritNeuralTrainDesc& d, IfritNeuralTrainReport* report);
+    void setNeuralParams(const float* params);
+    void getNeuralParams(float* params);
+    void evalNeural(const float* in, float* out, int n, int precision);
+
 private:
     bool prepare(int indexCount);
     void enqueue(bool clear);
@@ -169,6 +175,16 @@ private:
     int rasterGrid() const;
     std::vector<TextureSlot> textures_;
     bool texDescDirty_ = true;
+
+    // neural material state
+    DevBuf nnParams_, nnGrads_, nnM_, nnV_, nnPacked_, nnQinfo_, lat32_, latGrad_, latM_, latV_, lat16_;
+    DevBuf nnData_, nnFeat_, nnCount_, nnLoss_, nnEvalIn_, nnEvalOut_;
+    NeuralLatent lat16Desc_{}, lat32Desc_{};
+    int nnLodShift_ = 0;
+    bool nnReady_ = false;
+    bool neuralLatentAlloc(int size);
+    bool neuralPackAll(unsigned dataCount);
+    NeuralShadeParams neuralShadeParams() const;
 
     IfritFrameStats stats_{};
     bool stageTiming_ = false;

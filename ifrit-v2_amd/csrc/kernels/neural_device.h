@@ -1,0 +1,241 @@
+// Device code of the neural material shared by the shading and training
+// kernels: latent texture sampling and the wave-level MFMA MLP.
+#pragma once
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "neural.h"
+
+namespace ifrit_hip {
+namespace nn {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// float -> bf16 bits, round to nearest even (inputs are finite)
+__device__ __forceinline__ unsigned short bf16_bits(float x) {
+    const unsigned u = __float_as_uint(x);
+    return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+__device__ __forceinline__ bf16x8 to_bf16x8(const float* v) {
+    u16x8 b;
+#pragma unroll
+    for (int q = 0; q < 8; q++) b[q] = bf16_bits(v[q]);
+    return __builtin_bit_cast(bf16x8, b);
+}
+
+__device__ __forceinline__ bf16x8 load_bf16x8(const uint16_t* p) {
+    return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(p));
+}
+
+// 8 floats * scale -> 8 FP8 (E4M3, saturating hardware conversion)
+__device__ __forceinline__ long to_fp8x8(const float* v, float scale) {
+    long r = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q += 2) {
+        const int p = __builtin_amdgcn_cvt_pk_fp8_f32(v[q] * scale, v[q + 1] * scale, 0, false);
+        r |= (long)(p & 0xffff) << (8 * q);
+    }
+    return r;
+}
+
+__device__ __forceinline__ f32x16 mfma_bf16(bf16x8 a, bf16x8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x16 mfma_fp8(long a, long b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ f32x16 load_c(const float* p) {
+    f32x16 c;
+#pragma unroll
+    for (int r = 0; r < 16; r++) c[r] = p[r];
+    return c;
+}
+
+// The 8 B-operand values of k step s (0..3) for lane group g, taken from
+// accumulator tile s / 2 after ReLU.
+__device__ __forceinline__ void relu_slice(const f32x16& z, int s, float* v) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) v[q] = fmaxf(z[8 * (s & 1) + q], 0.0f);
+}
+
+// Forward pass for the 64 fragments of one wave (all 64 lanes must call it),
+// one 32-fragment column tile at a time to bound the live accumulators.
+//   xs: LDS [64][16] inputs (fragment-major, inputs 14 and 15 zero).
+//   hs1, hs2: optional LDS [64][64] outputs of the ReLU layers (training).
+//   ys: LDS [64][4] outputs (o = 0..2).
+template <bool FP8>
+__device__ __forceinline__ void mlp_forward(const NeuralPacked* __restrict__ W, const float* xs, float* hs1, float* hs2,
+                                            float* ys) {
+    const int l = threadIdx.x & 63, g = l >> 5, c = l & 31;
+#pragma unroll 1
+    for (int nt = 0; nt < 2; nt++) {
+        const int f = 32 * nt + c;
+        // layer 1
+        f32x16 z1[2];
+        {
+            float xv[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) xv[q] = xs[f * 16 + 8 * g + q];
+            if constexpr (FP8) {
+                const float inv = 1.0f / (W->qscale[0] * W->qscale[3]);
+                const long bx = to_fp8x8(xv, W->qscale[3]);
+#pragma unroll
+                for (int mt = 0; mt < 2; mt++) {
+                    f32x16 acc = {};
+                    acc = mfma_fp8(*reinterpret_cast<const long*>(&W->Q1[mt][l][0]), bx, acc);
+                    const f32x16 b = load_c(&W->C1[mt][g][0]);
+#pragma unroll
+                    for (int r = 0; r < 16; r++) z1[mt][r] = fmaf(acc[r], inv, b[r]);
+                }
+            } else {
+                const bf16x8 bx = to_bf16x8(xv);
+#pragma unroll
+                for (int mt = 0; mt < 2; mt++)
+                    z1[mt] = mfma_bf16(load_bf16x8(&W->A1[mt][l][0]), bx, load_c(&W->C1[mt][g][0]));
+            }
+        }
+        if (hs1) {
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) hs1[f * 64 + 32 * mt + acc_row(g, r)] = fmaxf(z1[mt][r], 0.0f);
+        }
+        // layer 2
+        f32x16 z2[2];
+        if constexpr (FP8) {
+            const float sh = W->qscale[4], inv = 1.0f / (W->qscale[1] * sh);
+            long bq[4];
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                float v[8];
+                relu_slice(z1[s >> 1], s, v);
+                bq[s] = to_fp8x8(v, sh);
+            }
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++) {
+                f32x16 acc = {};
+#pragma unroll
+                for (int s = 0; s < 4; s++)
+                    acc = mfma_fp8(*reinterpret_cast<const long*>(&W->Q2[mt][s][l][0]), bq[s], acc);
+                const f32x16 b = load_c(&W->C2[mt][g][0]);
+#pragma unroll
+                for (int r = 0; r < 16; r++) z2[mt][r] = fmaf(acc[r], inv, b[r]);
+            }
+        } else {
+            bf16x8 bb[4];
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                float v[8];
+                relu_slice(z1[s >> 1], s, v);
+                bb[s] = to_bf16x8(v);
+            }
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++) {
+                f32x16 acc = load_c(&W->C2[mt][g][0]);
+#pragma unroll
+                for (int s = 0; s < 4; s++) acc = mfma_bf16(load_bf16x8(&W->A2[mt][s][l][0]), bb[s], acc);
+                z2[mt] = acc;
+            }
+        }
+        if (hs2) {
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) hs2[f * 64 + 32 * mt + acc_row(g, r)] = fmaxf(z2[mt][r], 0.0f);
+        }
+        // layer 3
+        f32x16 y;
+        if constexpr (FP8) {
+            const float sh = W->qscale[5], inv = 1.0f / (W->qscale[2] * sh);
+            f32x16 acc = {};
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                float v[8];
+                relu_slice(z2[s >> 1], s, v);
+                acc = mfma_fp8(*reinterpret_cast<const long*>(&W->Q3[s][l][0]), to_fp8x8(v, sh), acc);
+            }
+            const f32x16 b = load_c(&W->C3[g][0]);
+#pragma unroll
+            for (int r = 0; r < 16; r++) y[r] = fmaf(acc[r], inv, b[r]);
+        } else {
+            y = load_c(&W->C3[g][0]);
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                float v[8];
+                relu_slice(z2[s >> 1], s, v);
+                y = mfma_bf16(load_bf16x8(&W->A3[s][l][0]), to_bf16x8(v), y);
+            }
+        }
+        // outputs are rows 0..2: lane group 0, registers 0..2
+        if (g == 0) {
+#pragma unroll
+            for (int o = 0; o < 3; o++) ys[f * 4 + o] = y[o];
+        }
+    }
+}
+
+// Visits the texels of the latent footprint: n anisotropic taps along
+// (au, av), each trilinear between the two levels around lod; visit(texel
+// index, weight), weights sum to 1.
+template <class F>
+__device__ __forceinline__ void latent_visit(const NeuralLatent& L, float u, float v, float au, float av, float lod,
+                                             int n, F&& visit) {
+    lod = fminf(fmaxf(lod, 0.0f), (float)(L.levels - 1));
+    const int l0 = (int)floorf(lod);
+    const int l1 = min(l0 + 1, L.levels - 1);
+    const float f = lod - (float)l0;
+    const float invn = __builtin_amdgcn_rcpf((float)n);
+    for (int i = 0; i < n; i++) {
+        const float o = ((float)i + 0.5f) * invn - 0.5f;
+        const float tu = fmaf(au, o, u), tv = fmaf(av, o, v);
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int lev = k == 0 ? l0 : l1;
+            const float wl = (k == 0 ? 1.0f - f : f) * invn;
+            if (wl == 0.0f) continue;
+            const int size = max(1, L.size >> lev);
+            const float s = fmaf(tu, (float)size, -0.5f), r = fmaf(tv, (float)size, -0.5f);
+            const float fs = floorf(s), fr = floorf(r);
+            const float ax = s - fs, ay = r - fr;
+            const int x0 = (int)fs, y0 = (int)fr, m = size - 1;  // power of two: repeat by masking
+            const unsigned base = L.levelOffset[lev];
+            const int xa = x0 & m, xb = (x0 + 1) & m, ya = y0 & m, yb = (y0 + 1) & m;
+            visit(base + (unsigned)(ya * size + xa), wl * (1.0f - ax) * (1.0f - ay));
+            visit(base + (unsigned)(ya * size + xb), wl * ax * (1.0f - ay));
+            visit(base + (unsigned)(yb * size + xa), wl * (1.0f - ax) * ay);
+            visit(base + (unsigned)(yb * size + xb), wl * ax * ay);
+        }
+    }
+}
+
+// FP16 latent texel (8 channels, one 16-byte load).
+__device__ __forceinline__ void latent_texel_h(const NeuralLatent& L, unsigned t, float* c) {
+    const uint4 raw = reinterpret_cast<const uint4*>(L.base)[t];
+    const unsigned w[4] = {raw.x, raw.y, raw.z, raw.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const __half2 h = __builtin_bit_cast(__half2, w[k]);
+        const float2 f = __half22float2(h);
+        c[2 * k] = f.x;
+        c[2 * k + 1] = f.y;
+    }
+}
+
+// Trilinear sample of one tap (level weights included) of the FP16 latent.
+__device__ __forceinline__ void latent_tap_h(const NeuralLatent& L, float tu, float tv, float lod, float* out) {
+#pragma unroll
+    for (int k = 0; k < kLatentCh; k++) out[k] = 0.0f;
+    latent_visit(L, tu, tv, 0.0f, 0.0f, lod, 1, [&](unsigned t, float w) {
+        float c[8];
+        latent_texel_h(L, t, c);
+#pragma unroll
+        for (int k = 0; k < kLatentCh; k++) out[k] = fmaf(w, c[k], out[k]);
+    });
+}
+
+}  // namespace nn
+}  // namespace ifrit_hip

@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "neural.h"
+
 namespace ifrit_hip {
 
 // One assembled triangle for coverage and depth (6 x 16 bytes):
@@ -85,6 +87,7 @@ struct ShadeParams {
     float lightColor[4][4];
     int writeFloat;          // also write RGBA32F
     int ntri;                // slots >= ntri are clip fans with their own bx, by
+    NeuralShadeParams neural;  // programs 4 (neural material) and 5 (training features)
 };
 
 // Status word bits raised by the kernels; checked by the host after a frame.

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"
+#include "neural_device.h"
 
 namespace ifrit_hip {
 
@@ -196,7 +197,7 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, const ShadeTri& T, const
     auto interp = [&](int off) { return fmaf(w[0], v0[off], fmaf(w[1], v1[off], w[2] * v2[off])); };
     const float3 Pw = make_float3(interp(P.offWorld), interp(P.offWorld + 1), interp(P.offWorld + 2));
     float3 N = make_float3(interp(P.offNormal), interp(P.offNormal + 1), interp(P.offNormal + 2));
-    if (P.program == 3) {
+    if (P.program >= 3) {
         // shading frame: interpolated normal, tangent made orthogonal to it, bitangent N x T
         N = norm3(N);
         float3 Tg = make_float3(interp(P.offTangent), interp(P.offTangent + 1), interp(P.offTangent + 2));
@@ -223,10 +224,10 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, const ShadeTri& T, const
         const float dudx = iuv(wb, P.offUv) - iuv(wa, P.offUv), dvdx = iuv(wb, P.offUv + 1) - iuv(wa, P.offUv + 1);
         const float dudy = iuv(wd, P.offUv) - iuv(wc, P.offUv), dvdy = iuv(wd, P.offUv + 1) - iuv(wc, P.offUv + 1);
         int mat = P.trisPerMaterial > 0 ? T.prim / P.trisPerMaterial : 0;
-        R.mat = P.program == 3 ? 0 : min(mat, P.numTextures - 1);  // the layers share one footprint
+        R.mat = P.program >= 3 ? 0 : min(mat, P.numTextures - 1);  // the layers share one footprint
         R.fp = footprint(tex[R.mat], P.sampler, P.maxAniso, dudx, dvdx, dudy, dvdy);
     }
-    if (P.program == 3) return R;  // lighting needs the sampled layers
+    if (P.program >= 3) return R;  // lighting needs the sampled layers (or the network)
     N = norm3(N);
     const float3 V = norm3(make_float3(P.camera[0] - Pw.x, P.camera[1] - Pw.y, P.camera[2] - Pw.z));
     float3 A = make_float3(P.ambient, P.ambient, P.ambient), B = make_float3(0, 0, 0);
@@ -295,6 +296,20 @@ __device__ float4 ggx_color(const ShadeParams& P, const FragPrep& R, float4 alb,
     return make_float4(sat(col.x), sat(col.y), sat(col.z), sat(alb.w));
 }
 
+// View and light (light 0) directions in the fragment's tangent frame: the
+// neural material's direction inputs.
+__device__ __forceinline__ void frame_dirs(const ShadeParams& P, const FragPrep& R, float3& vt, float3& lt) {
+    vt = make_float3(dot3(R.V, R.Tg), dot3(R.V, R.Bg), dot3(R.V, R.N));
+    float3 L = make_float3(0, 0, 1);
+    if (P.numLights > 0) {
+        if (P.lightPos[0][3] == 0.0f)
+            L = norm3(make_float3(P.lightPos[0][0], P.lightPos[0][1], P.lightPos[0][2]));
+        else
+            L = norm3(make_float3(P.lightPos[0][0] - R.Pw.x, P.lightPos[0][1] - R.Pw.y, P.lightPos[0][2] - R.Pw.z));
+    }
+    lt = make_float3(dot3(L, R.Tg), dot3(L, R.Bg), dot3(L, R.N));
+}
+
 __device__ __forceinline__ int wave_incl_scan(int v) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -315,12 +330,12 @@ constexpr int kTapWindow = 128;
 //      and its texture taps go into one flat tap list, one tap per lane;
 //   3. the fragment lane sums its taps in tap order, the pixel lane sums its
 //      fragments in sample order, so results do not depend on scheduling.
-template <int S>
-__global__ void __launch_bounds__(64) k_shade(ShadeParams P, TriRecs recs, const float4* __restrict__ bary,
-                                              const uint32_t* __restrict__ visSlot, const float* __restrict__ vtx,
-                                              const int* __restrict__ idx, const TexDesc* __restrict__ tex,
-                                              const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8,
-                                              float4* __restrict__ rgba32f, int tilesX, int tiles, int perXcd) {
+template <int S, bool NEURAL, bool FP8>
+__device__ __forceinline__ void shade_tile(const ShadeParams& P, const TriRecs& recs, const float4* __restrict__ bary,
+                                           const uint32_t* __restrict__ visSlot, const float* __restrict__ vtx,
+                                           const int* __restrict__ idx, const TexDesc* __restrict__ tex,
+                                           const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8,
+                                           float4* __restrict__ rgba32f, int tilesX, int tiles, int perXcd) {
     __shared__ unsigned fSlot[64 * S];
     __shared__ unsigned char fCnt[64 * S];
     __shared__ unsigned char fPix[64 * S];
@@ -329,9 +344,13 @@ __global__ void __launch_bounds__(64) k_shade(ShadeParams P, TriRecs recs, const
     __shared__ int cN[64];
     __shared__ int cMat[64];
     __shared__ float4 cCol[64];     // clamped colour of the chunk's fragments
+    __shared__ float4 xdir[64];     // network inputs 10..13 (view z, light xyz)
     __shared__ unsigned short tList[kTapWindow];
-    __shared__ float4 tCol[kTapWindow];
-    __shared__ float4 tCol2[kTapWindow];  // layered GGX: normal xyz and roughness of each tap
+    // per tap: colour or latent channels 0-3 (tCol); layered GGX normal xyz
+    // and roughness, or latent channels 4-7 (tCol2); network inputs after the taps
+    __shared__ float4 tBuf[2 * kTapWindow];
+    float4* const tCol = tBuf;
+    float4* const tCol2 = tBuf + kTapWindow;
 
     // consecutive tiles on one XCD share its L2 (texture and vertex reuse)
     const int tile = (int)(blockIdx.x & 7u) * perXcd + (int)(blockIdx.x >> 3);
@@ -393,6 +412,7 @@ __global__ void __launch_bounds__(64) k_shade(ShadeParams P, TriRecs recs, const
     }
     __syncthreads();
 
+    if (P.program == 5 && S == 1 && inFb && nf == 0) P.neural.featOut[4 * pix + 1] = make_float4(0, 0, 0, 0);
     const float o = S == 1 ? 0.0f : 0.5f;
     float4 sum = make_float4(0, 0, 0, 0);
     for (int c0 = 0; c0 < F; c0 += 64) {
@@ -436,8 +456,15 @@ __global__ void __launch_bounds__(64) k_shade(ShadeParams P, TriRecs recs, const
                 const float invn = frcp((float)tn);
                 const float to = ((float)ti + 0.5f) * invn - 0.5f;
                 const float tu = fmaf(uv.z, to, uv.x), tv = fmaf(uv.w, to, uv.y);
+                if (NEURAL) {
+                    float z[kLatentCh];
+                    nn::latent_tap_h(P.neural.latent, tu, tv, cLod[fl] - (float)P.neural.lodShift, z);
+                    tCol[j] = make_float4(z[0], z[1], z[2], z[3]);
+                    tCol2[j] = make_float4(z[4], z[5], z[6], z[7]);
+                    continue;
+                }
                 tCol[j] = trilinear(pool, tex[cMat[fl]], tu, tv, cLod[fl]);
-                if (P.program == 3) {
+                if (P.program == 3 || P.program == 5) {
                     const float4 nm = trilinear(pool, tex[1], tu, tv, cLod[fl]);
                     const float4 rg = trilinear(pool, tex[2], tu, tv, cLod[fl]);
                     tCol2[j] = make_float4(nm.x, nm.y, nm.z, rg.x);
@@ -451,7 +478,7 @@ __global__ void __launch_bounds__(64) k_shade(ShadeParams P, TriRecs recs, const
                 acc.y += c.y;
                 acc.z += c.z;
                 acc.w += c.w;
-                if (P.program == 3) {
+                if (P.program >= 3) {
                     const float4 c2 = tCol2[j];
                     acc2.x += c2.x;
                     acc2.y += c2.y;
@@ -466,19 +493,60 @@ __global__ void __launch_bounds__(64) k_shade(ShadeParams P, TriRecs recs, const
                 const float invn = frcp((float)n);
                 alb = make_float4(acc.x * invn, acc.y * invn, acc.z * invn, acc.w * invn);
             }
-            if (P.program == 3) {
+            if (P.program == 3 || P.program == 5) {
                 float4 nm = make_float4(0.5f, 0.5f, 1.0f, 0.0f);
                 if (R.mat >= 0) {
                     const float invn = frcp((float)n);
                     nm = make_float4(acc2.x * invn, acc2.y * invn, acc2.z * invn, acc2.w * invn);
                 }
                 cCol[lane] = ggx_color(P, R, alb, nm);
+                if (P.program == 5 && S == 1 && R.mat >= 0) {
+                    // training record of this pixel: latent footprint, view and light in the frame, target
+                    const int fl = fPix[fi];
+                    const size_t fpix = (size_t)((tile / tilesX) * 8 + (fl >> 3)) * P.width + (tile % tilesX) * 8 + (fl & 7);
+                    float3 vt, lt;
+                    frame_dirs(P, R, vt, lt);
+                    const float4 cc = cCol[lane];
+                    float4* rec = P.neural.featOut + 4 * fpix;
+                    rec[0] = make_float4(R.u, R.v, R.fp.au, R.fp.av);
+                    rec[1] = make_float4(R.fp.lod - (float)P.neural.lodShift, (float)n, vt.x, vt.y);
+                    rec[2] = make_float4(vt.z, lt.x, lt.y, lt.z);
+                    rec[3] = make_float4(cc.x, cc.y, cc.z, 0.0f);
+                }
+            } else if (NEURAL) {
+                // latent and directions of this fragment; the network runs for the whole wave below
+                const float invn = n > 0 ? frcp((float)n) : 0.0f;
+                float3 vt, lt;
+                frame_dirs(P, R, vt, lt);
+                cUV[lane] = make_float4(acc.x * invn, acc.y * invn, acc.z * invn, acc.w * invn);
+                cCol[lane] = make_float4(acc2.x * invn, acc2.y * invn, acc2.z * invn, acc2.w * invn);
+                cN[lane] = __float_as_int(vt.x);
+                cLod[lane] = vt.y;
+                xdir[lane] = make_float4(vt.z, lt.x, lt.y, lt.z);
             } else {
                 cCol[lane] = P.program == 0 ? make_float4(sat(R.B.x), sat(R.B.y), sat(R.B.z), 1.0f)
                                             : make_float4(sat(fmaf(alb.x, R.A.x, R.B.x)),
                                                           sat(fmaf(alb.y, R.A.y, R.B.y)),
                                                           sat(fmaf(alb.z, R.A.z, R.B.z)), sat(alb.w));
             }
+        }
+        if constexpr (NEURAL) {
+            // network inputs [64][16] over the tap buffer, outputs [64][4] over cUV
+            __syncthreads();
+            float* xs = reinterpret_cast<float*>(tBuf);
+            float* ys = reinterpret_cast<float*>(cUV);
+            {
+                const float4 z0 = cUV[lane], z1 = cCol[lane], d = xdir[lane];
+                const float v[16] = {z0.x, z0.y, z0.z, z0.w, z1.x, z1.y, z1.z, z1.w,
+                                     __int_as_float(cN[lane]), cLod[lane], d.x, d.y, d.z, d.w, 0.0f, 0.0f};
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < 16; k++) xs[lane * 16 + k] = act ? v[k] : 0.0f;
+            }
+            __syncthreads();
+            nn::mlp_forward<FP8>(P.neural.packed, xs, nullptr, nullptr, ys);
+            __syncthreads();
+            if (act) cCol[lane] = make_float4(sat(ys[lane * 4]), sat(ys[lane * 4 + 1]), sat(ys[lane * 4 + 2]), 1.0f);
         }
         __syncthreads();
         // 3. the pixel's fragments of this chunk, in sample order
@@ -501,6 +569,26 @@ __global__ void __launch_bounds__(64) k_shade(ShadeParams P, TriRecs recs, const
     if (rgba32f) rgba32f[pix] = r;
 }
 
+template <int S>
+__global__ void __launch_bounds__(64) k_shade(ShadeParams P, TriRecs recs, const float4* __restrict__ bary,
+                                              const uint32_t* __restrict__ visSlot, const float* __restrict__ vtx,
+                                              const int* __restrict__ idx, const TexDesc* __restrict__ tex,
+                                              const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8,
+                                              float4* __restrict__ rgba32f, int tilesX, int tiles, int perXcd) {
+    shade_tile<S, false, false>(P, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tilesX, tiles, perXcd);
+}
+
+// The neural material: the MFMA network needs more registers; two waves per
+// SIMD keep the latent fetches overlapped.
+template <int S, bool FP8>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8)))
+k_shade_neural(ShadeParams P, TriRecs recs, const float4* __restrict__ bary, const uint32_t* __restrict__ visSlot,
+               const float* __restrict__ vtx, const int* __restrict__ idx, const TexDesc* __restrict__ tex,
+               const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8, float4* __restrict__ rgba32f,
+               int tilesX, int tiles, int perXcd) {
+    shade_tile<S, true, FP8>(P, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tilesX, tiles, perXcd);
+}
+
 }  // namespace
 
 void launch_shade(const ShadeParams& p, TriRecs recs, const float4* bary, const uint32_t* visSlot,
@@ -509,20 +597,23 @@ void launch_shade(const ShadeParams& p, TriRecs recs, const float4* bary, const 
     const int tilesX = (int)dev::blocks(p.width, 8), tiles = tilesX * (int)dev::blocks(p.height, 8);
     const int perXcd = (tiles + 7) / 8;
     const dim3 grid(8 * perXcd);
+#define IFRIT_ARGS p, recs, bary, visSlot, vtx, idx, tex, texPool, rgba8, rgba32f, tilesX, tiles, perXcd
+#define IFRIT_SHADE(SN)                                                                                             \
+    do {                                                                                                            \
+        if (p.program != 4)                                                                                         \
+            hipLaunchKernelGGL((k_shade<SN>), grid, dim3(64), 0, s, IFRIT_ARGS);                                    \
+        else if (p.neural.precision == 1)                                                                           \
+            hipLaunchKernelGGL((k_shade_neural<SN, true>), grid, dim3(64), 0, s, IFRIT_ARGS);                       \
+        else                                                                                                        \
+            hipLaunchKernelGGL((k_shade_neural<SN, false>), grid, dim3(64), 0, s, IFRIT_ARGS);                      \
+    } while (0)
     switch (p.samples) {
-        case 1:
-            hipLaunchKernelGGL(k_shade<1>, grid, dim3(64), 0, s, p, recs, bary, visSlot, vtx, idx, tex, texPool,
-                               rgba8, rgba32f, tilesX, tiles, perXcd);
-            break;
-        case 4:
-            hipLaunchKernelGGL(k_shade<4>, grid, dim3(64), 0, s, p, recs, bary, visSlot, vtx, idx, tex, texPool,
-                               rgba8, rgba32f, tilesX, tiles, perXcd);
-            break;
-        default:
-            hipLaunchKernelGGL(k_shade<8>, grid, dim3(64), 0, s, p, recs, bary, visSlot, vtx, idx, tex, texPool,
-                               rgba8, rgba32f, tilesX, tiles, perXcd);
-            break;
+        case 1: IFRIT_SHADE(1); break;
+        case 4: IFRIT_SHADE(4); break;
+        default: IFRIT_SHADE(8); break;
     }
+#undef IFRIT_SHADE
+#undef IFRIT_ARGS
 }
 
 }  // namespace ifrit_hip

@@ -28,7 +28,25 @@ class FragmentProgramDesc(C.Structure):
                 ("trisPerMaterial", C.c_int32), ("sampler", C.c_int32), ("maxAniso", C.c_int32),
                 ("camera", C.c_float * 3), ("ambient", C.c_float), ("kd", C.c_float), ("ks", C.c_float),
                 ("shininess", C.c_float), ("numLights", C.c_int32), ("lightPos", (C.c_float * 4) * 4),
-                ("lightColor", (C.c_float * 4) * 4), ("offTangent", C.c_int32)]
+                ("lightColor", (C.c_float * 4) * 4), ("offTangent", C.c_int32), ("precision", C.c_int32)]
+
+
+class NeuralTrainDesc(C.Structure):
+    _fields_ = [("views", C.c_int32), ("viewWidth", C.c_int32), ("viewHeight", C.c_int32),
+                ("maxFragments", C.c_int32), ("steps", C.c_int32), ("batch", C.c_int32), ("lrLatent", C.c_float),
+                ("lrMlp", C.c_float), ("seed", C.c_uint32), ("latentSize", C.c_int32), ("eyeDistMin", C.c_float),
+                ("eyeDistMax", C.c_float), ("fovY", C.c_float), ("targetJitter", C.c_float)]
+
+
+class NeuralTrainReport(C.Structure):
+    _fields_ = [("fragments", C.c_int64), ("lossFirst", C.c_float), ("lossLast", C.c_float),
+                ("datasetMs", C.c_float), ("trainMs", C.c_float)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+NEURAL_PARAMS = 5508  # W1[16][64], b1[64], W2[64][64], b2[64], W3[64][4], b3[4]
 
 
 class FrameStats(C.Structure):
@@ -58,6 +76,7 @@ EXPORTED_SYMBOLS = [
     "iftrCopyHostBufferToImage", "iftrUploadTextureLevels", "iftrGenerateFbmTexture", "iftrGenerateMipmap",
     "iftrDownloadTexture", "iftrSetRenderTarget", "iftrGetVisibility", "iftrGetCoverageClass", "iftrGetColorRGBA8",
     "iftrGetFrameStats", "iftrSetStageTiming", "iftrDrawAsync", "iftrSynchronize", "iftrGetLastError",
+    "iftrNeuralTrain", "iftrNeuralSetParams", "iftrNeuralGetParams", "iftrNeuralEval",
 ]
 
 
@@ -100,6 +119,9 @@ def lib():
         "iftrGetColorRGBA8": ([vp, vp], None), "iftrGetFrameStats": ([vp, C.POINTER(FrameStats)], None),
         "iftrSetStageTiming": ([vp, C.c_int], None), "iftrDrawAsync": ([vp, C.c_int, C.c_int], None),
         "iftrSynchronize": ([vp], None), "iftrGetLastError": ([vp], C.c_char_p),
+        "iftrNeuralTrain": ([vp, C.POINTER(NeuralTrainDesc), C.POINTER(NeuralTrainReport)], None),
+        "iftrNeuralSetParams": ([vp, vp], None), "iftrNeuralGetParams": ([vp, vp], None),
+        "iftrNeuralEval": ([vp, vp, vp, C.c_int, C.c_int], None),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -168,9 +190,10 @@ class Renderer:
         self.num_indices = int(i.size)
         self._check()
 
-    def bind_fragment_program(self, scene):
+    def bind_fragment_program(self, scene, program=None, precision=0):
         d = FragmentProgramDesc()
-        d.kind = scene.program
+        d.kind = scene.program if program is None else program
+        d.precision = precision
         d.offWorld, d.offNormal, d.offUv = 0, 3, 6
         d.offTangent = 8 if scene.stride >= 11 else 0
         d.trisPerMaterial = scene.tris_per_material
@@ -185,6 +208,37 @@ class Renderer:
         self._programs.append(p)
         self.L.iftrBindFragmentShader(self.h, p)
         self._check()
+
+    def train_neural(self, views=512, view_size=(256, 256), max_fragments=16 << 20, steps=2000, batch=65536,
+                     lr_latent=1e-2, lr_mlp=2e-3, seed=1, latent_size=1024, eye_dist=(20.0, 45.0),
+                     fov_y=1.0471975511965976, target_jitter=6.0) -> dict:
+        """Trains the neural material on the bound layered-GGX program (see iftrNeuralTrain)."""
+        d = NeuralTrainDesc(views, view_size[0], view_size[1], max_fragments, steps, batch, lr_latent, lr_mlp, seed,
+                            latent_size, eye_dist[0], eye_dist[1], fov_y, target_jitter)
+        r = NeuralTrainReport()
+        self.L.iftrNeuralTrain(self.h, C.byref(d), C.byref(r))
+        self._check()
+        return r.as_dict()
+
+    def set_neural_params(self, params: np.ndarray):
+        p = np.ascontiguousarray(params, np.float32).reshape(-1)
+        assert p.size == NEURAL_PARAMS
+        self.L.iftrNeuralSetParams(self.h, _p(p))
+        self._check()
+
+    def get_neural_params(self) -> np.ndarray:
+        p = np.empty(NEURAL_PARAMS, np.float32)
+        self.L.iftrNeuralGetParams(self.h, _p(p))
+        self._check()
+        return p
+
+    def eval_neural(self, inputs: np.ndarray, precision: int = 0) -> np.ndarray:
+        x = np.zeros((inputs.shape[0], 16), np.float32)
+        x[:, :inputs.shape[1]] = inputs
+        out = np.empty((inputs.shape[0], 4), np.float32)
+        self.L.iftrNeuralEval(self.h, _p(x), _p(out), x.shape[0], precision)
+        self._check()
+        return out[:, :3]
 
     def add_texture(self, tex) -> int:
         tid = self.L.iftrCreateTexture(self.h, tex.width, tex.height, tex.levels)

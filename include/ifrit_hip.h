@@ -135,8 +135,10 @@ enum {
     IFHIP_FS_FLAT = 0,        /* colour = bytes of a hash of the primitive id */
     IFHIP_FS_BLINN_PHONG = 1, /* textured Blinn-Phong, directional and point lights */
     IFHIP_FS_DEPTH_ONLY = 2,  /* visibility and depth only */
-    IFHIP_FS_GGX_LAYERED = 3  /* GGX over three layers: texture 0 albedo, 1 tangent-space normal,
+    IFHIP_FS_GGX_LAYERED = 3, /* GGX over three layers: texture 0 albedo, 1 tangent-space normal,
                                  2 roughness (red); needs a tangent in the vertex record */
+    IFHIP_FS_NEURAL = 4       /* the neural material trained by iftrNeuralTrain (same inputs as
+                                 IFHIP_FS_GGX_LAYERED; light 0 only) */
 };
 enum { IFHIP_SAMPLER_BILINEAR = 0, IFHIP_SAMPLER_TRILINEAR = 1, IFHIP_SAMPLER_ANISOTROPIC = 2 };
 
@@ -157,6 +159,7 @@ typedef struct {
     float lightPos[4][4];       /* w = 0: direction towards the light, w = 1: point light */
     float lightColor[4][4];     /* rgb, attenuation k of 1 / (1 + k d^2) */
     int32_t offTangent;         /* float offset of the tangent (xyz), IFHIP_FS_GGX_LAYERED */
+    int32_t precision;          /* IFHIP_FS_NEURAL: 0 BF16 MFMA, 1 FP8 (E4M3) MFMA */
 } IfritGpuFragmentProgramDesc;
 
 void* iftrCreateGpuVertexProgram(const IfritGpuVertexProgramDesc* desc);
@@ -207,6 +210,34 @@ void iftrDrawAsync(IfritRendererHandle* h, int numVertices, int clearFramebuffer
 void iftrSynchronize(IfritRendererHandle* h);
 /* NULL when no error has been recorded since the last call. */
 const char* iftrGetLastError(IfritRendererHandle* h);
+
+/* Neural material: an 8-channel FP16 latent texture with mips plus an MLP
+   14 -> 64 -> 64 -> 3 (ReLU) on BF16 / FP8 MFMA, trained on fragments of the bound
+   layered-GGX material (IFHIP_FS_GGX_LAYERED must be bound, with its three textures)
+   rendered from random views of the bound geometry.  No counterpart in the reference. */
+typedef struct {
+    int32_t views;                    /* training views */
+    int32_t viewWidth, viewHeight;    /* training view size (1 sample per pixel) */
+    int32_t maxFragments;             /* data set capacity */
+    int32_t steps, batch;             /* Adam steps, fragments per step (multiple of 256) */
+    float lrLatent, lrMlp;            /* initial learning rates (cosine decay to 5 %) */
+    uint32_t seed;
+    int32_t latentSize;               /* power of two */
+    float eyeDistMin, eyeDistMax;     /* camera distance from the origin */
+    float fovY;                       /* radians */
+    float targetJitter;               /* look-at point uniform in [-j, j]^3 */
+} IfritNeuralTrainDesc;
+typedef struct {
+    int64_t fragments;                /* training fragments collected */
+    float lossFirst, lossLast;        /* mean squared error per channel over the first / last step */
+    float datasetMs, trainMs;
+} IfritNeuralTrainReport;
+void iftrNeuralTrain(IfritRendererHandle* h, const IfritNeuralTrainDesc* desc, IfritNeuralTrainReport* report);
+/* Parameters as 5508 floats: W1[16][64], b1[64], W2[64][64], b2[64], W3[64][4], b3[4]. */
+void iftrNeuralSetParams(IfritRendererHandle* h, const float* params);
+void iftrNeuralGetParams(IfritRendererHandle* h, float* params);
+/* The MLP alone on n inputs ([n][16] floats, 14 used) -> [n][4] outputs (3 used). */
+void iftrNeuralEval(IfritRendererHandle* h, const float* in, float* out, int n, int precision);
 
 #ifdef __cplusplus
 }
