@@ -115,6 +115,58 @@ def cpu_baseline(scene, frames: int = 5):
             "kind": "port", "sample": f"{frames} frames of visibility (no shading) at the full workload size"}
 
 
+def neural_leg(args):
+    """Config 5: layered-GGX material vs the neural material on the config-2
+    geometry at 1080p, 1 sample per pixel.  Shade-pass time from HIP events;
+    PSNR and max error of the neural frames against the layered-GGX frame of
+    the same visibility (that frame is within one RGBA8 step of the FP64
+    oracle, which the GPU tests check)."""
+    import numpy as np
+
+    from ifrit_hip import Renderer, scenes
+
+    scene = scenes.config5(spheres=args.spheres)
+    r = Renderer()
+    r.load_scene(scene)
+    r.stage_timing(True)
+
+    def shade_ms(n=10):
+        acc = 0.0
+        for _ in range(n):
+            r.draw()
+            acc += r.stats()["stageMs"][STAGES.index("shade")]
+        return acc / n
+
+    r.draw()
+    plain_ms = shade_ms()
+    plain = r.color()
+    ids, _ = r.visibility()
+    mask = (ids != 0xFFFFFFFF).reshape(scene.height, scene.width)
+    rep = r.train_neural(views=args.neural_views, view_size=(256, 256), max_fragments=16 << 20,
+                         steps=args.neural_steps, batch=65536, latent_size=1024, single_tap=args.neural_single_tap)
+    out = {"workload": "config5: config-2 geometry, 1920x1080, 1 spp, layered GGX (3 x 2048^2 layers, 16x aniso) "
+                       "vs 8-ch FP16 1024^2 latent + MLP 14-64-64-3",
+           "plain_shade_ms": round(plain_ms, 4), "train": {k: (round(v, 6) if isinstance(v, float) else v)
+                                                            for k, v in rep.items()},
+           "psnr_reference": "layered-GGX frame on the GPU (within 1 RGBA8 step of the FP64 oracle)"}
+    out["psnr_bound_db"] = 40.0
+    for prec, name in ((0, "bf16"), (1, "fp8")):
+        r.bind_fragment_program(scene, program=4, precision=prec)
+        r.draw()
+        ms = shade_ms()
+        img = r.color()
+        d = (img[mask][:, :3].astype(np.float64) - plain[mask][:, :3].astype(np.float64)) / 255.0
+        # the library's validation: PSNR against the material frame; below the bound the neural
+        # program at this precision falls back to the material from here on
+        psnr = r.validate_neural(prec, out["psnr_bound_db"])
+        fallback = psnr < out["psnr_bound_db"]
+        out[name] = {"shade_ms": round(ms, 4), "psnr_db": round(psnr, 2),
+                     "max_err_lsb": int(round(np.abs(d).max() * 255)), "speedup_vs_plain": round(plain_ms / ms, 3),
+                     "falls_back_to_plain": bool(fallback)}
+    r.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +174,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--spheres", type=int, default=256)
+    ap.add_argument("--no-neural", action="store_true", help="skip the neural-material leg")
+    ap.add_argument("--neural-steps", type=int, default=3000)
+    ap.add_argument("--neural-views", type=int, default=512)
+    ap.add_argument("--neural-single-tap", type=int, default=1,
+                    help="1: one trilinear latent tap per fragment, 0: the material's anisotropic taps")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a float32 copy of 256 MiB (x + 0) three times: the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -212,6 +269,8 @@ def main():
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5),
                      "traffic": traffic, "algorithmic_bytes": ab, "duration_ms": round(timed[dom], 4)},
     }
+    if not args.no_neural:
+        out["neural"] = neural_leg(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene)
     print(json.dumps(out), flush=True)

@@ -297,6 +297,11 @@ void iftrNeuralGetParams(IfritRendererHandle* h, float* params) {
     if (!params) return;
     guarded(h, [&](TileRasterRenderer& r) { r.getNeuralParams(params); });
 }
+float iftrNeuralValidate(IfritRendererHandle* h, int precision, float minPsnrDb, int numIndices) {
+    float r = 0.0f;
+    guarded(h, [&](TileRasterRenderer& rr) { r = rr.validateNeural(precision, minPsnrDb, numIndices); });
+    return r;
+}
 void iftrNeuralEval(IfritRendererHandle* h, const float* in, float* out, int n, int precision) {
     if (!in || !out) return;
     guarded(h, [&](TileRasterRenderer& r) { r.evalNeural(in, out, n, precision); });

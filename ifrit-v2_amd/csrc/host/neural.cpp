@@ -18,8 +18,8 @@ namespace ifrit_hip {
 
 void neural_pack(const float* params, float* qinfo, NeuralPacked* out, int withFp8, hipStream_t s);
 void neural_train_step(const NeuralRecord* data, unsigned numData, const NeuralLatent& latent, float* latentGrad,
-                       const NeuralPacked* packed, float* grads, float* loss, unsigned step, unsigned batch,
-                       unsigned chunksPerWave, hipStream_t s);
+                       const NeuralPacked* packed, float* gradPartial, float* grads, float* loss, unsigned step,
+                       unsigned batch, unsigned chunksPerWave, hipStream_t s);
 void neural_adam(float* p, float* g, float* m, float* v, size_t n, float lr, float b1, float b2, int t, hipStream_t s);
 void neural_init_latent(float* lat, size_t n, unsigned seed, float a, hipStream_t s);
 void neural_to_half(const float* src, void* dst, size_t n, hipStream_t s);
@@ -102,6 +102,7 @@ NeuralShadeParams TileRasterRenderer::neuralShadeParams() const {
     p.packed = nnPacked_.as<NeuralPacked>();
     p.precision = fs_.desc.precision == 1 ? 1 : 0;
     p.lodShift = nnLodShift_;
+    p.singleTap = nnSingleTap_;
     p.featOut = nnFeat_.as<float4>();
     return p;
 }
@@ -138,6 +139,7 @@ void TileRasterRenderer::trainNeural(const IfritNeuralTrainDesc& d, IfritNeuralT
     }
     if (!neuralLatentAlloc(d.latentSize)) return;
     nnLodShift_ = ilog2(textures_[0].width) - ilog2(d.latentSize);
+    nnSingleTap_ = d.singleTap ? 1 : 0;
     const size_t npix = (size_t)d.viewWidth * d.viewHeight;
     if (!(nnParams_.ensure(kNumParams * 4) && nnGrads_.ensure(kNumParams * 4) && nnM_.ensure(kNumParams * 4) &&
           nnV_.ensure(kNumParams * 4) && nnPacked_.ensure(sizeof(NeuralPacked)) && nnQinfo_.ensure(8 * 4) &&
@@ -171,8 +173,11 @@ void TileRasterRenderer::trainNeural(const IfritNeuralTrainDesc& d, IfritNeuralT
         const double eye[3] = {dist * rr * std::cos(phi), dist * z, dist * rr * std::sin(phi)};
         const double at[3] = {d.targetJitter * (2 * uni(rng) - 1), d.targetJitter * (2 * uni(rng) - 1),
                               d.targetJitter * (2 * uni(rng) - 1)};
+        // field of view log-uniform in [fovYMin, fovY]: pixel footprints from the framing view to close-ups
+        const double fmin = d.fovYMin > 0 ? std::min((double)d.fovYMin, (double)d.fovY) : d.fovY;
+        const double fov = std::exp(std::log(fmin) + (std::log((double)d.fovY) - std::log(fmin)) * uni(rng));
         double P[16], V[16];
-        perspective(d.fovY, (double)d.viewWidth / d.viewHeight, 0.1, 200.0, P);
+        perspective(fov, (double)d.viewWidth / d.viewHeight, 0.1, 200.0, P);
         look_at(eye, at, V);
         for (int r = 0; r < 4; r++)
             for (int c = 0; c < 4; c++) {
@@ -228,13 +233,18 @@ void TileRasterRenderer::trainNeural(const IfritNeuralTrainDesc& d, IfritNeuralT
         return;
 
     // 3. Adam steps, cosine learning-rate decay to 5 %
-    const unsigned chunksPerWave = 4;
+    const unsigned chunksPerWave = 1;
     const unsigned batch = (unsigned)d.batch / (64u * chunksPerWave) * (64u * chunksPerWave);
+    if (!nnGradPart_.ensure((size_t)(batch / (64u * chunksPerWave)) * kNumParams * 4)) {
+        setError("hipMalloc gradient partials");
+        return;
+    }
     for (int step = 1; step <= d.steps; step++) {
         const float decay = 0.05f + 0.95f * 0.5f * (1.0f + std::cos((float)M_PI * (step - 1) / d.steps));
         neural_pack(nnParams_.as<float>(), nnQinfo_.as<float>(), nnPacked_.as<NeuralPacked>(), 0, s);
         neural_train_step(nnData_.as<NeuralRecord>(), numData, lat32Desc_, latGrad_.as<float>(),
-                          nnPacked_.as<NeuralPacked>(), nnGrads_.as<float>(), nnLoss_.as<float>() + (step - 1),
+                          nnPacked_.as<NeuralPacked>(), nnGradPart_.as<float>(), nnGrads_.as<float>(),
+                          nnLoss_.as<float>() + (step - 1),
                           (unsigned)step, batch, chunksPerWave, s);
         neural_adam(nnParams_.as<float>(), nnGrads_.as<float>(), nnM_.as<float>(), nnV_.as<float>(), kNumParams,
                     d.lrMlp * decay, 0.9f, 0.99f, step, s);
@@ -251,6 +261,7 @@ void TileRasterRenderer::trainNeural(const IfritNeuralTrainDesc& d, IfritNeuralT
     // 4. inference operands
     if (!neuralPackAll(numData)) return;
     nnReady_ = true;
+    nnFallback_[0] = nnFallback_[1] = false;
     if (report) {
         report->fragments = numData;
         report->lossFirst = loss.front() / (3.0f * batch);
@@ -277,6 +288,7 @@ void TileRasterRenderer::setNeuralParams(const float* params) {
     if (!ok(hipMemcpy(nnQinfo_.ptr, q, sizeof(q), hipMemcpyHostToDevice), "qinfo")) return;
     neural_pack(nnParams_.as<float>(), nnQinfo_.as<float>(), nnPacked_.as<NeuralPacked>(), 1, stream_);
     if (ok(hipStreamSynchronize(stream_), "neural pack")) nnReady_ = true;
+    nnFallback_[0] = nnFallback_[1] = false;
 }
 
 void TileRasterRenderer::getNeuralParams(float* params) {
@@ -301,6 +313,60 @@ void TileRasterRenderer::evalNeural(const float* in, float* out, int n, int prec
     neural_eval(nnEvalIn_.as<float>(), nnEvalOut_.as<float>(), n, nnPacked_.as<NeuralPacked>(), precision == 1, stream_);
     if (!ok(hipStreamSynchronize(stream_), "eval")) return;
     ok(hipMemcpy(out, nnEvalOut_.ptr, (size_t)n * 16, hipMemcpyDeviceToHost), "eval out");
+}
+
+float TileRasterRenderer::validateNeural(int precision, float minPsnrDb, int indexCount) {
+    if (!nnReady_) {
+        setError("the neural material has not been trained or set");
+        return 0.0f;
+    }
+    if (!fsBound_ || (fs_.desc.kind != IFHIP_FS_NEURAL && fs_.desc.kind != IFHIP_FS_GGX_LAYERED)) {
+        setError("validation needs the layered-GGX or neural program bound");
+        return 0.0f;
+    }
+    const int p = precision == 1 ? 1 : 0;
+    const IfritGpuFragmentProgramDesc saved = fs_.desc;
+    FrameBuffer* fbSaved = fb_;
+    fb_ = nullptr;
+    const size_t px = (size_t)width_ * height_;
+    std::vector<uint8_t> plain(px * 4), neural(px * 4);
+    std::vector<uint32_t> slot0(px);
+    fs_.desc.kind = IFHIP_FS_GGX_LAYERED;
+    drawElements(indexCount, true);
+    getColorRGBA8(plain.data());
+    {
+        std::vector<uint32_t> slots(px * samples_);
+        ok(hipMemcpy(slots.data(), visSlot_.ptr, slots.size() * 4, hipMemcpyDeviceToHost), "slots");
+        for (size_t i = 0; i < px; i++) {
+            uint32_t any = 0xffffffffu;
+            for (int s = 0; s < samples_; s++) any &= slots[i * samples_ + s];
+            slot0[i] = any;  // all samples empty only when every slot is empty
+        }
+    }
+    fs_.desc.kind = IFHIP_FS_NEURAL;
+    fs_.desc.precision = p;
+    const bool fb = nnFallback_[p];
+    nnFallback_[p] = false;
+    drawElements(indexCount, true);
+    getColorRGBA8(neural.data());
+    nnFallback_[p] = fb;
+    fs_.desc = saved;
+    fb_ = fbSaved;
+    if (!error_.empty()) return 0.0f;
+    double se = 0.0;
+    size_t n = 0;
+    for (size_t i = 0; i < px; i++) {
+        if (slot0[i] == 0xffffffffu) continue;
+        for (int c = 0; c < 3; c++) {
+            const double d = ((double)neural[4 * i + c] - (double)plain[4 * i + c]) / 255.0;
+            se += d * d;
+        }
+        n += 3;
+    }
+    const double mse = n ? se / (double)n : 0.0;
+    const float psnr = (float)(mse > 0 ? 10.0 * std::log10(1.0 / mse) : 120.0);
+    nnFallback_[p] = psnr < minPsnrDb;
+    return psnr;
 }
 
 }  // namespace ifrit_hip

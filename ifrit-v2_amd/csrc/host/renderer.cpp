@@ -53,7 +53,7 @@ TileRasterRenderer::~TileRasterRenderer() {
     for (DevBuf* b : {&vtx_, &idx_, &clip_, &recs_[0], &recs_[1], &recs_[2], &recs_[3], &recs_[4], &recs_[5], &bary_, &tileBox_, &slotOf_, &binCount_, &binStart_, &binCursor_, &binList_,
                       &bigList_, &ctr_, &clipList_, &statPart_, &items_, &tilePartial_, &multiList_, &partials_, &visId_, &visDepth_, &visSlot_, &visClass_, &rgba8_, &rgba32f_, &texDesc_, &nnParams_, &nnGrads_, &nnM_, &nnV_,
                       &nnPacked_, &nnQinfo_, &lat32_, &latGrad_, &latM_, &latV_, &lat16_, &nnData_, &nnFeat_, &nnCount_,
-                      &nnLoss_, &nnEvalIn_, &nnEvalOut_})
+                      &nnLoss_, &nnEvalIn_, &nnEvalOut_, &nnGradPart_})
         b->release();
     for (auto& t : textures_)
         if (t.dev) (void)hipFree(t.dev);
@@ -487,7 +487,8 @@ void TileRasterRenderer::enqueue(bool clear) {
         sh.width = width_;
         sh.height = height_;
         sh.samples = samples_;
-        sh.program = d.kind;
+        // the neural program falls back to the layered-GGX material when validation found it below its bound
+        sh.program = (d.kind == IFHIP_FS_NEURAL && nnFallback_[d.precision == 1 ? 1 : 0]) ? IFHIP_FS_GGX_LAYERED : d.kind;
         sh.stride = stride_;
         sh.offWorld = d.offWorld;
         sh.offNormal = d.offNormal;

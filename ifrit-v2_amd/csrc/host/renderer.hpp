@@ -133,6 +133,7 @@ public:
     void setNeuralParams(const float* params);
     void getNeuralParams(float* params);
     void evalNeural(const float* in, float* out, int n, int precision);
+    float validateNeural(int precision, float minPsnrDb, int indexCount);
 
 private:
     bool prepare(int indexCount);
@@ -178,9 +179,11 @@ private:
 
     // neural material state
     DevBuf nnParams_, nnGrads_, nnM_, nnV_, nnPacked_, nnQinfo_, lat32_, latGrad_, latM_, latV_, lat16_;
-    DevBuf nnData_, nnFeat_, nnCount_, nnLoss_, nnEvalIn_, nnEvalOut_;
+    DevBuf nnData_, nnFeat_, nnCount_, nnLoss_, nnEvalIn_, nnEvalOut_, nnGradPart_;
     NeuralLatent lat16Desc_{}, lat32Desc_{};
     int nnLodShift_ = 0;
+    int nnSingleTap_ = 0;
+    bool nnFallback_[2] = {false, false};  // per precision: below the PSNR bound, shade with the material
     bool nnReady_ = false;
     bool neuralLatentAlloc(int size);
     bool neuralPackAll(unsigned dataCount);

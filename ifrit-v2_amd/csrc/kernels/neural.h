@@ -73,6 +73,7 @@ struct NeuralShadeParams {
     const NeuralPacked* packed;
     int precision;              // 0: BF16, 1: FP8
     int lodShift;               // log2(material layer size / latent size)
+    int singleTap;              // 1: one trilinear latent tap at the isotropic footprint
     float4* featOut;            // feature records (program 5), one per pixel
 };
 

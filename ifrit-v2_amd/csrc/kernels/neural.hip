@@ -127,7 +127,7 @@ struct TrainArgs {
     NeuralLatent latent;  // FP32 master (base = float*)
     float* latentGrad;
     const NeuralPacked* packed;
-    float* grads;         // kNumParams
+    float* gradPartial;   // kNumParams per wave, reduced by k_reduce_grads
     float* loss;          // one float for this step
     unsigned step, chunksPerWave;
     float invBatch;
@@ -341,15 +341,26 @@ __global__ void __launch_bounds__(64) k_train(TrainArgs A) {
             });
         }
     }
-    // this wave's weight gradients
-    for (int k = 0; k < kMlpIn; k++) atomicAdd(&A.grads[kW1 + k * 64 + l], gW1[k]);
-    atomicAdd(&A.grads[kB1 + l], gB1);
-    for (int m = 0; m < 64; m++) atomicAdd(&A.grads[kW2 + l * 64 + m], gW2[m]);
-    atomicAdd(&A.grads[kB2 + l], gB2);
-    for (int o = 0; o < 3; o++) atomicAdd(&A.grads[kW3 + l * 4 + o], gW3[o]);
-    if (l < 3) atomicAdd(&A.grads[kB3 + l], gB3);
+    // this wave's weight gradients (every parameter written by one lane)
+    float* gp = A.gradPartial + (size_t)blockIdx.x * kNumParams;
+    for (int k = 0; k < 16; k++) gp[kW1 + k * 64 + l] = k < kMlpIn ? gW1[k] : 0.0f;
+    gp[kB1 + l] = gB1;
+    for (int m = 0; m < 64; m++) gp[kW2 + l * 64 + m] = gW2[m];
+    gp[kB2 + l] = gB2;
+    for (int o = 0; o < 4; o++) gp[kW3 + l * 4 + o] = o < 3 ? gW3[o] : 0.0f;
+    if (l < 4) gp[kB3 + l] = l < 3 ? gB3 : 0.0f;
     for (int off = 32; off > 0; off >>= 1) lossAcc += __shfl_down(lossAcc, off, 64);
     if (l == 0) atomicAdd(A.loss, lossAcc);
+}
+
+// Sum of the per-wave gradients.
+__global__ void __launch_bounds__(256) k_reduce_grads(const float* __restrict__ part, int waves,
+                                                      float* __restrict__ grads) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= kNumParams) return;
+    float a = 0.0f;
+    for (int w = 0; w < waves; w++) a += part[(size_t)w * kNumParams + i];
+    grads[i] = a;
 }
 
 // Adam on n parameters; the gradient is read and cleared.
@@ -465,11 +476,13 @@ void neural_pack(const float* params, float* qinfo, NeuralPacked* out, int withF
 }
 
 void neural_train_step(const NeuralRecord* data, unsigned numData, const NeuralLatent& latent, float* latentGrad,
-                       const NeuralPacked* packed, float* grads, float* loss, unsigned step, unsigned batch,
-                       unsigned chunksPerWave, hipStream_t s) {
-    TrainArgs a{data, numData, latent, latentGrad, packed, grads, loss, step, chunksPerWave, 1.0f / (float)batch};
+                       const NeuralPacked* packed, float* gradPartial, float* grads, float* loss, unsigned step,
+                       unsigned batch, unsigned chunksPerWave, hipStream_t s) {
+    TrainArgs a{data, numData, latent, latentGrad, packed, gradPartial, loss, step, chunksPerWave,
+                1.0f / (float)batch};
     const unsigned waves = batch / (64u * chunksPerWave);
     hipLaunchKernelGGL(k_train, dim3(waves), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_reduce_grads, dim3((kNumParams + 255) / 256), dim3(256), 0, s, gradPartial, (int)waves, grads);
 }
 
 void neural_adam(float* p, float* g, float* m, float* v, size_t n, float lr, float b1, float b2, int t, hipStream_t s) {

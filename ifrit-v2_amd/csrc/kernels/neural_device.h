@@ -12,6 +12,7 @@ namespace nn {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x8 __attribute__((ext_vector_type(8)));
 
 // float -> bf16 bits, round to nearest even (inputs are finite)
 __device__ __forceinline__ unsigned short bf16_bits(float x) {
@@ -19,11 +20,12 @@ __device__ __forceinline__ unsigned short bf16_bits(float x) {
     return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 
+// 8 floats -> bf16 (round to nearest even; v_cvt_pk_bf16_f32 on gfx950)
 __device__ __forceinline__ bf16x8 to_bf16x8(const float* v) {
-    u16x8 b;
+    f32x8 f;
 #pragma unroll
-    for (int q = 0; q < 8; q++) b[q] = bf16_bits(v[q]);
-    return __builtin_bit_cast(bf16x8, b);
+    for (int q = 0; q < 8; q++) f[q] = v[q];
+    return __builtin_convertvector(f, bf16x8);
 }
 
 __device__ __forceinline__ bf16x8 load_bf16x8(const uint16_t* p) {
@@ -73,6 +75,8 @@ __device__ __forceinline__ void mlp_forward(const NeuralPacked* __restrict__ W, 
     const int l = threadIdx.x & 63, g = l >> 5, c = l & 31;
 #pragma unroll 1
     for (int nt = 0; nt < 2; nt++) {
+        // weights are re-read per tile (L1 hits) rather than held in 56 registers
+        __asm__ __volatile__("" ::: "memory");
         const int f = 32 * nt + c;
         // layer 1
         f32x16 z1[2];
@@ -225,16 +229,26 @@ __device__ __forceinline__ void latent_texel_h(const NeuralLatent& L, unsigned t
     }
 }
 
-// Trilinear sample of one tap (level weights included) of the FP16 latent.
+// Trilinear sample of one tap (level weights included) of the FP16 latent,
+// accumulated in packed FP16 (two channels per instruction).
 __device__ __forceinline__ void latent_tap_h(const NeuralLatent& L, float tu, float tv, float lod, float* out) {
+    __half2 acc[4];
 #pragma unroll
-    for (int k = 0; k < kLatentCh; k++) out[k] = 0.0f;
+    for (int k = 0; k < 4; k++) acc[k] = __float2half2_rn(0.0f);
     latent_visit(L, tu, tv, 0.0f, 0.0f, lod, 1, [&](unsigned t, float w) {
-        float c[8];
-        latent_texel_h(L, t, c);
-#pragma unroll
-        for (int k = 0; k < kLatentCh; k++) out[k] = fmaf(w, c[k], out[k]);
+        const uint4 raw = reinterpret_cast<const uint4*>(L.base)[t];
+        const __half2 w2 = __float2half2_rn(w);
+        acc[0] = __hfma2(w2, __builtin_bit_cast(__half2, raw.x), acc[0]);
+        acc[1] = __hfma2(w2, __builtin_bit_cast(__half2, raw.y), acc[1]);
+        acc[2] = __hfma2(w2, __builtin_bit_cast(__half2, raw.z), acc[2]);
+        acc[3] = __hfma2(w2, __builtin_bit_cast(__half2, raw.w), acc[3]);
     });
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float2 f = __half22float2(acc[k]);
+        out[2 * k] = f.x;
+        out[2 * k + 1] = f.y;
+    }
 }
 
 }  // namespace nn

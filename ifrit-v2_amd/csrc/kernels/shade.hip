@@ -227,6 +227,12 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, const ShadeTri& T, const
         R.mat = P.program >= 3 ? 0 : min(mat, P.numTextures - 1);  // the layers share one footprint
         R.fp = footprint(tex[R.mat], P.sampler, P.maxAniso, dudx, dvdx, dudy, dvdy);
     }
+    if (P.program == 4 && P.neural.singleTap && R.mat >= 0) {
+        // the neural material's latent: one trilinear tap at the isotropic footprint log2(pmax)
+        R.fp.lod += flog2((float)R.fp.n);
+        R.fp.n = 1;
+        R.fp.au = R.fp.av = 0.0f;
+    }
     if (P.program >= 3) return R;  // lighting needs the sampled layers (or the network)
     N = norm3(N);
     const float3 V = norm3(make_float3(P.camera[0] - Pw.x, P.camera[1] - Pw.y, P.camera[2] - Pw.z));
@@ -508,8 +514,11 @@ __device__ __forceinline__ void shade_tile(const ShadeParams& P, const TriRecs& 
                     frame_dirs(P, R, vt, lt);
                     const float4 cc = cCol[lane];
                     float4* rec = P.neural.featOut + 4 * fpix;
-                    rec[0] = make_float4(R.u, R.v, R.fp.au, R.fp.av);
-                    rec[1] = make_float4(R.fp.lod - (float)P.neural.lodShift, (float)n, vt.x, vt.y);
+                    // the latent footprint the neural material will sample (see prep_fragment)
+                    const bool one = P.neural.singleTap != 0;
+                    rec[0] = make_float4(R.u, R.v, one ? 0.0f : R.fp.au, one ? 0.0f : R.fp.av);
+                    rec[1] = make_float4(R.fp.lod + (one ? flog2((float)n) : 0.0f) - (float)P.neural.lodShift,
+                                         one ? 1.0f : (float)n, vt.x, vt.y);
                     rec[2] = make_float4(vt.z, lt.x, lt.y, lt.z);
                     rec[3] = make_float4(cc.x, cc.y, cc.z, 0.0f);
                 }
@@ -578,10 +587,10 @@ __global__ void __launch_bounds__(64) k_shade(ShadeParams P, TriRecs recs, const
     shade_tile<S, false, false>(P, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tilesX, tiles, perXcd);
 }
 
-// The neural material: the MFMA network needs more registers; two waves per
-// SIMD keep the latent fetches overlapped.
+// The neural material, held to four waves per SIMD like the other programs so
+// that the latent fetches stay overlapped.
 template <int S, bool FP8>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_shade_neural(ShadeParams P, TriRecs recs, const float4* __restrict__ bary, const uint32_t* __restrict__ visSlot,
                const float* __restrict__ vtx, const int* __restrict__ idx, const TexDesc* __restrict__ tex,
                const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8, float4* __restrict__ rgba32f,

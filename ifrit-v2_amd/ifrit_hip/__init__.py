@@ -35,7 +35,8 @@ class NeuralTrainDesc(C.Structure):
     _fields_ = [("views", C.c_int32), ("viewWidth", C.c_int32), ("viewHeight", C.c_int32),
                 ("maxFragments", C.c_int32), ("steps", C.c_int32), ("batch", C.c_int32), ("lrLatent", C.c_float),
                 ("lrMlp", C.c_float), ("seed", C.c_uint32), ("latentSize", C.c_int32), ("eyeDistMin", C.c_float),
-                ("eyeDistMax", C.c_float), ("fovY", C.c_float), ("targetJitter", C.c_float)]
+                ("eyeDistMax", C.c_float), ("fovY", C.c_float), ("targetJitter", C.c_float), ("fovYMin", C.c_float),
+                ("singleTap", C.c_int32)]
 
 
 class NeuralTrainReport(C.Structure):
@@ -76,7 +77,7 @@ EXPORTED_SYMBOLS = [
     "iftrCopyHostBufferToImage", "iftrUploadTextureLevels", "iftrGenerateFbmTexture", "iftrGenerateMipmap",
     "iftrDownloadTexture", "iftrSetRenderTarget", "iftrGetVisibility", "iftrGetCoverageClass", "iftrGetColorRGBA8",
     "iftrGetFrameStats", "iftrSetStageTiming", "iftrDrawAsync", "iftrSynchronize", "iftrGetLastError",
-    "iftrNeuralTrain", "iftrNeuralSetParams", "iftrNeuralGetParams", "iftrNeuralEval",
+    "iftrNeuralTrain", "iftrNeuralSetParams", "iftrNeuralGetParams", "iftrNeuralEval", "iftrNeuralValidate",
 ]
 
 
@@ -122,6 +123,7 @@ def lib():
         "iftrNeuralTrain": ([vp, C.POINTER(NeuralTrainDesc), C.POINTER(NeuralTrainReport)], None),
         "iftrNeuralSetParams": ([vp, vp], None), "iftrNeuralGetParams": ([vp, vp], None),
         "iftrNeuralEval": ([vp, vp, vp, C.c_int, C.c_int], None),
+        "iftrNeuralValidate": ([vp, C.c_int, C.c_float, C.c_int], C.c_float),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -211,14 +213,22 @@ class Renderer:
 
     def train_neural(self, views=512, view_size=(256, 256), max_fragments=16 << 20, steps=2000, batch=65536,
                      lr_latent=1e-2, lr_mlp=2e-3, seed=1, latent_size=1024, eye_dist=(20.0, 45.0),
-                     fov_y=1.0471975511965976, target_jitter=6.0) -> dict:
+                     fov_y=1.0471975511965976, target_jitter=6.0, fov_y_min=0.14, single_tap=False) -> dict:
         """Trains the neural material on the bound layered-GGX program (see iftrNeuralTrain)."""
         d = NeuralTrainDesc(views, view_size[0], view_size[1], max_fragments, steps, batch, lr_latent, lr_mlp, seed,
-                            latent_size, eye_dist[0], eye_dist[1], fov_y, target_jitter)
+                            latent_size, eye_dist[0], eye_dist[1], fov_y, target_jitter, fov_y_min,
+                            1 if single_tap else 0)
         r = NeuralTrainReport()
         self.L.iftrNeuralTrain(self.h, C.byref(d), C.byref(r))
         self._check()
         return r.as_dict()
+
+    def validate_neural(self, precision: int = 0, min_psnr_db: float = 40.0) -> float:
+        """PSNR of the neural frame against the layered-GGX frame of the current view; below the bound
+        the neural program at this precision falls back to the material."""
+        v = self.L.iftrNeuralValidate(self.h, precision, min_psnr_db, self.num_indices)
+        self._check()
+        return float(v)
 
     def set_neural_params(self, params: np.ndarray):
         p = np.ascontiguousarray(params, np.float32).reshape(-1)

@@ -224,8 +224,11 @@ typedef struct {
     uint32_t seed;
     int32_t latentSize;               /* power of two */
     float eyeDistMin, eyeDistMax;     /* camera distance from the origin */
-    float fovY;                       /* radians */
+    float fovY;                       /* radians; the largest field of view */
     float targetJitter;               /* look-at point uniform in [-j, j]^3 */
+    float fovYMin;                    /* > 0: field of view log-uniform in [fovYMin, fovY] per view */
+    int32_t singleTap;                /* 1: one trilinear latent tap at the isotropic footprint,
+                                         0: the material's anisotropic taps */
 } IfritNeuralTrainDesc;
 typedef struct {
     int64_t fragments;                /* training fragments collected */
@@ -236,6 +239,11 @@ void iftrNeuralTrain(IfritRendererHandle* h, const IfritNeuralTrainDesc* desc, I
 /* Parameters as 5508 floats: W1[16][64], b1[64], W2[64][64], b2[64], W3[64][4], b3[4]. */
 void iftrNeuralSetParams(IfritRendererHandle* h, const float* params);
 void iftrNeuralGetParams(IfritRendererHandle* h, float* params);
+/* Renders the current view with the layered-GGX material and with the neural material at the given
+   precision (0 BF16, 1 FP8) and returns the PSNR (dB, RGB over covered pixels) of the neural frame
+   against the material's.  Below minPsnrDb, IFHIP_FS_NEURAL at that precision falls back to the
+   layered-GGX material until the next validation or training. */
+float iftrNeuralValidate(IfritRendererHandle* h, int precision, float minPsnrDb, int numIndices);
 /* The MLP alone on n inputs ([n][16] floats, 14 used) -> [n][4] outputs (3 used). */
 void iftrNeuralEval(IfritRendererHandle* h, const float* in, float* out, int n, int precision);
 

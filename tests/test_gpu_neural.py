@@ -92,3 +92,21 @@ def test_training_reduces_loss_and_approaches_the_material(renderer_factory):
     assert ps_plain > 45
     assert out[0][0] > 20
     assert out[1][0] > 18
+
+
+def test_validation_falls_back_below_the_bound(renderer_factory):
+    """An untrained network is far from the material: validation reports a low
+    PSNR and the neural program then draws the material itself."""
+    s = scenes.config5(width=256, height=144, samples=1, spheres=16, level=3, tex_size=128, host_textures=True)
+    r = renderer_factory()
+    r.load_scene(s)
+    r.draw()
+    plain = r.color()
+    r.set_neural_params(_random_params(np.random.default_rng(9)))
+    r.bind_fragment_program(s, program=4, precision=0)
+    psnr = r.validate_neural(0, 40.0)
+    print(f"[neural fallback] untrained PSNR {psnr:.1f} dB", flush=True)
+    assert psnr < 40.0
+    r.draw()
+    assert np.array_equal(r.color(), plain), "below the bound the neural program must draw the material"
+    assert r.validate_neural(0, 0.0) == psnr  # a zero bound clears the fallback again
