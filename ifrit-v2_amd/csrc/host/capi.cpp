@@ -307,4 +307,31 @@ void iftrNeuralEval(IfritRendererHandle* h, const float* in, float* out, int n, 
     guarded(h, [&](TileRasterRenderer& r) { r.evalNeural(in, out, n, precision); });
 }
 
+void iftrSetViews(IfritRendererHandle* h, const IfritViewDesc* views, int numViews) {
+    guarded(h, [&](TileRasterRenderer& r) { r.setViews(views, numViews); });
+}
+void iftrGetViewVisibility(IfritRendererHandle* h, int view, uint32_t* ids, float* depth) {
+    guarded(h, [&](TileRasterRenderer& r) { r.getViewVisibility(view, ids, depth); });
+}
+void iftrGetViewColorRGBA8(IfritRendererHandle* h, int view, uint8_t* rgba) {
+    if (rgba) guarded(h, [&](TileRasterRenderer& r) { r.getViewColorRGBA8(view, rgba); });
+}
+void iftrSetGraphMode(IfritRendererHandle* h, int enable) {
+    guarded(h, [&](TileRasterRenderer& r) { r.setGraphMode(enable != 0); });
+}
+void iftrSetTuning(IfritRendererHandle* h, const IfritTuning* t) {
+    if (t) guarded(h, [&](TileRasterRenderer& r) { r.setTuning(*t); });
+}
+void iftrGetTuning(IfritRendererHandle* h, IfritTuning* t) {
+    if (t) guarded(h, [&](TileRasterRenderer& r) { r.getTuning(t); });
+}
+void iftrAutotune(IfritRendererHandle* h, int numIndices, int frames, IfritAutotuneReport* report) {
+    guarded(h, [&](TileRasterRenderer& r) { r.autotune(numIndices, frames, report); });
+}
+uint64_t iftrGetDigest(IfritRendererHandle* h) {
+    uint64_t d = 0;
+    guarded(h, [&](TileRasterRenderer& r) { d = r.digest(); });
+    return d;
+}
+
 }  // extern "C"

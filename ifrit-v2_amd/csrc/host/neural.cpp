@@ -153,6 +153,8 @@ void TileRasterRenderer::trainNeural(const IfritNeuralTrainDesc& d, IfritNeuralT
 
     // 1. training fragments from random views
     FrameBuffer* fbSaved = fb_;
+    std::vector<HostView> batchSaved;
+    batchSaved.swap(batch_);  // training views are single-view draws
     const int wSaved = width_, hSaved = height_, sSaved = samples_;
     const IfritGpuVertexProgramDesc vsSaved = vs_.desc;
     const IfritGpuFragmentProgramDesc fsSaved = fs_.desc;
@@ -197,6 +199,7 @@ void TileRasterRenderer::trainNeural(const IfritNeuralTrainDesc& d, IfritNeuralT
         }
     }
     fb_ = fbSaved;
+    batch_.swap(batchSaved);
     width_ = wSaved;
     height_ = hSaved;
     samples_ = sSaved;
@@ -325,6 +328,10 @@ float TileRasterRenderer::validateNeural(int precision, float minPsnrDb, int ind
         return 0.0f;
     }
     const int p = precision == 1 ? 1 : 0;
+    if (!batch_.empty()) {
+        setError("validation renders the single bound target: clear the view batch first");
+        return 0.0f;
+    }
     const IfritGpuFragmentProgramDesc saved = fs_.desc;
     FrameBuffer* fbSaved = fb_;
     fb_ = nullptr;

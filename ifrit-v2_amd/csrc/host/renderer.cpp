@@ -16,7 +16,7 @@ namespace ifrit_hip {
 
 namespace {
 constexpr int kTile = 16;
-constexpr int kBigThreshold = 64;
+constexpr int kMaxViews = 65535;  // view indices are 16-bit
 
 // Sample offsets from the pixel corner: 1x samples the corner itself, like
 // the CPU path; 4x and 8x are the standard patterns of
@@ -27,9 +27,12 @@ const float kS8[16] = {0.5625f, 0.3125f, 0.4375f, 0.6875f, 0.8125f, 0.5625f, 0.3
                        0.1875f, 0.8125f, 0.0625f, 0.4375f, 0.6875f, 0.9375f, 0.9375f, 0.0625f};
 }  // namespace
 
+unsigned long long DevBuf::generation = 0;
+
 bool DevBuf::ensure(size_t n) {
     if (n <= bytes && ptr) return true;
     release();
+    generation++;
     size_t cap = std::max<size_t>(n, 256);
     if (hipMalloc(&ptr, cap) != hipSuccess) {
         ptr = nullptr;
@@ -50,10 +53,12 @@ TileRasterRenderer::TileRasterRenderer() {}
 
 TileRasterRenderer::~TileRasterRenderer() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    dropGraph();
     for (DevBuf* b : {&vtx_, &idx_, &clip_, &recs_[0], &recs_[1], &recs_[2], &recs_[3], &recs_[4], &recs_[5], &bary_, &tileBox_, &slotOf_, &binCount_, &binStart_, &binCursor_, &binList_,
                       &bigList_, &ctr_, &clipList_, &statPart_, &items_, &tilePartial_, &multiList_, &partials_, &visId_, &visDepth_, &visSlot_, &visClass_, &rgba8_, &rgba32f_, &texDesc_, &nnParams_, &nnGrads_, &nnM_, &nnV_,
                       &nnPacked_, &nnQinfo_, &lat32_, &latGrad_, &latM_, &latV_, &lat16_, &nnData_, &nnFeat_, &nnCount_,
-                      &nnLoss_, &nnEvalIn_, &nnEvalOut_, &nnGradPart_})
+                      &nnLoss_, &nnEvalIn_, &nnEvalOut_, &nnGradPart_, &viewDev_, &tileView_, &shadeView_,
+                      &viewCtr_, &digest_})
         b->release();
     for (auto& t : textures_)
         if (t.dev) (void)hipFree(t.dev);
@@ -289,12 +294,16 @@ void TileRasterRenderer::runHostVertexShader() {
 bool TileRasterRenderer::prepare(int indexCount) {
     Init();
     if (!initialized_) return false;
-    if (width_ <= 0 || height_ <= 0) {
+    if (batch_.empty() && (width_ <= 0 || height_ <= 0)) {
         setError("no frame buffer bound");
         return false;
     }
-    if (width_ > 16384 || height_ > 16384) {
+    if (batch_.empty() && (width_ > 16384 || height_ > 16384)) {
         setError("frame buffer larger than 16384 x 16384");
+        return false;
+    }
+    if (!batch_.empty() && (hostVs_ || !vsBound_)) {
+        setError("multi-view batches need a GPU vertex program");
         return false;
     }
     if (depthFunc_ != IF_COMPARE_OP_LESS) {
@@ -356,27 +365,32 @@ bool TileRasterRenderer::prepare(int indexCount) {
     }
     const int ntri = indexCount / 3;
     curTris_ = ntri;
-    const int tiles = ((width_ + kTile - 1) / kTile) * ((height_ + kTile - 1) / kTile);
-    const size_t px = (size_t)width_ * height_;
-    const size_t ps = px * samples_;
     clipCap_ = std::max<unsigned>(clipCap_, std::max(4096u, (unsigned)(ntri / 8)));
-    binCap_ = std::max<unsigned>(binCap_, std::max(1u << 20, (unsigned)std::min<size_t>(0xFFFFFFF0ull, (size_t)ntri * 6)));
-    bigCap_ = std::max<unsigned>(bigCap_, std::max(1024u, (unsigned)(ntri / 16)));
-    itemCap_ = std::max<unsigned>(itemCap_, (unsigned)tiles + binCap_ / kRasterPart + 1u);
-    partialCap_ = std::max<unsigned>(partialCap_, std::max(256u, binCap_ / (8u * kRasterPart)));
-    const size_t slots = (size_t)ntri + clipCap_;
-    bool okAll = clip_.ensure((size_t)nverts_ * 16) && recs_[0].ensure(slots * 16) && recs_[1].ensure(slots * 16) &&
-                 recs_[2].ensure(slots * 16) && recs_[3].ensure(slots * 16) && recs_[4].ensure(slots * 16) &&
-                 recs_[5].ensure(slots * 16) &&
-                 bary_.ensure(slots * 32) && tileBox_.ensure(slots * 16) &&
-                 slotOf_.ensure((size_t)std::max(1, ntri) * 4) && clipList_.ensure((size_t)clipCap_ * 4) &&
-                 statPart_.ensure(((size_t)(ntri + 255) / 256 + 2 * (size_t)rasterGrid() + 2) * 8) &&
-                 items_.ensure((size_t)itemCap_ * 8) && tilePartial_.ensure((size_t)tiles * 4) &&
-                 multiList_.ensure((size_t)tiles * 8) && partials_.ensure((size_t)partialCap_ * 256 * samples_ * 8) && binCount_.ensure((size_t)tiles * 4) &&
-                 binStart_.ensure((size_t)tiles * 4) && binCursor_.ensure((size_t)tiles * 4) &&
+    if (!layoutViews(ntri)) return false;
+    const size_t nv = viewDesc_.size();
+    const size_t tiles = totalTiles_;
+    const size_t px = totalPixels_;
+    const size_t ps = px * samples_;
+    const unsigned part = 1u << partShift_;
+    binCap_ = std::max<unsigned>(
+        binCap_, std::max(1u << 20, (unsigned)std::min<size_t>(0xFFFFFFF0ull, (size_t)ntri * 6 * nv)));
+    bigCap_ = std::max<unsigned>(bigCap_, std::max(1024u, (unsigned)std::min<size_t>(0x7FFFFFFFull, ntri * nv / 16)));
+    itemCap_ = std::max<unsigned>(itemCap_, (unsigned)tiles + binCap_ / part + 1u);
+    partialCap_ = std::max<unsigned>(partialCap_, std::max(256u, binCap_ / (8u * part)));
+    const size_t slots = ((size_t)ntri + clipCap_) * nv;
+    const bool wantFloat = batch_.empty() && fb_ && !fb_->color.empty() && fb_->color[0];
+    bool okAll = clip_.ensure((size_t)nverts_ * 16 * nv) && recs_[0].ensure(slots * 16) &&
+                 recs_[1].ensure(slots * 16) && recs_[2].ensure(slots * 16) && recs_[3].ensure(slots * 16) &&
+                 recs_[4].ensure(slots * 16) && recs_[5].ensure(slots * 16) && bary_.ensure(slots * 32) &&
+                 tileBox_.ensure(slots * 16) && slotOf_.ensure((size_t)std::max(1, ntri) * 4 * nv) &&
+                 clipList_.ensure((size_t)clipCap_ * 4 * nv) &&
+                 statPart_.ensure(((size_t)(ntri + 255) / 256 * nv + 2 * (size_t)rasterGrid() + 2) * 8) &&
+                 items_.ensure((size_t)itemCap_ * 8) && tilePartial_.ensure(tiles * 4) &&
+                 multiList_.ensure(tiles * 8) && partials_.ensure((size_t)partialCap_ * 256 * samples_ * 8) &&
+                 binCount_.ensure(tiles * 4) && binStart_.ensure(tiles * 4) && binCursor_.ensure(tiles * 4) &&
                  binList_.ensure((size_t)binCap_ * 4) && bigList_.ensure((size_t)bigCap_ * 4) &&
                  visId_.ensure(ps * 4) && visDepth_.ensure(ps * 4) && visSlot_.ensure(ps * 4) &&
-                 visClass_.ensure(px) && rgba8_.ensure(px * 4) && rgba32f_.ensure(px * 16);
+                 visClass_.ensure(px) && rgba8_.ensure(px * 4) && (!wantFloat || rgba32f_.ensure(px * 16));
     if (!okAll) {
         setError("device allocation failed");
         return false;
@@ -401,17 +415,112 @@ bool TileRasterRenderer::prepare(int indexCount) {
     return true;
 }
 
+// Builds the view table of the next draw: the batch, or one view from the
+// bound target and programs.  Tiles, record slots and pixels of the views are
+// laid out one after another; each view's pixels start on a 64-pixel boundary.
+bool TileRasterRenderer::layoutViews(int ntri) {
+    std::vector<HostView> hv = batch_;
+    if (hv.empty()) {
+        HostView v{};
+        v.width = width_;
+        v.height = height_;
+        std::memcpy(v.mvp, vs_.desc.mvp, sizeof(v.mvp));
+        for (int k = 0; k < 3; k++) v.camera[k] = fs_.desc.camera[k];
+        v.shade = fsBound_ && fs_.desc.kind != IFHIP_FS_DEPTH_ONLY ? 1 : 0;
+        hv.push_back(v);
+    }
+    const unsigned sv = (unsigned)ntri + clipCap_;
+    if ((unsigned long long)sv * hv.size() >= 0x80000000ull) {
+        setError("too many triangles times views: record slots must stay below 2^31");
+        return false;
+    }
+    std::vector<ViewDesc> d(hv.size());
+    std::vector<int> layout;
+    layout.reserve(3 * hv.size() + 1);
+    unsigned tiles = 0, shade = 0;
+    size_t pix = 0;
+    for (size_t v = 0; v < hv.size(); v++) {
+        const HostView& h = hv[v];
+        ViewDesc& D = d[v];
+        std::memset(&D, 0, sizeof(D));
+        std::memcpy(D.mvp, h.mvp, sizeof(D.mvp));
+        for (int k = 0; k < 3; k++) D.camera[k] = h.camera[k];
+        D.shade = h.shade;
+        D.width = h.width;
+        D.height = h.height;
+        D.tilesX = (h.width + kTile - 1) / kTile;
+        D.tilesY = (h.height + kTile - 1) / kTile;
+        D.invW = 1.0f / (float)h.width;  // TileRasterRenderer.cpp:59-60
+        D.invH = 1.0f / (float)h.height;
+        D.tileBase = tiles;
+        tiles += (unsigned)(D.tilesX * D.tilesY);
+        D.shadeBase = shade;
+        if (h.shade)
+            shade += (unsigned)(((h.width + shadeTile_ - 1) / shadeTile_) * ((h.height + shadeTile_ - 1) / shadeTile_));
+        D.slotBase = (unsigned)v * sv;
+        D.primBase = (unsigned)v * (unsigned)ntri;
+        D.pixBase = pix;
+        pix += ((size_t)h.width * h.height + 63) & ~(size_t)63;
+        layout.push_back(h.width);
+        layout.push_back(h.height);
+        layout.push_back(h.shade);
+    }
+    layout.push_back(shadeTile_);
+    totalTiles_ = tiles;
+    shadeTiles_ = shade;
+    totalPixels_ = pix;
+    const bool descChanged = d.size() != viewDesc_.size() ||
+                             std::memcmp(d.data(), viewDesc_.data(), d.size() * sizeof(ViewDesc)) != 0;
+    if (descChanged) {
+        if (!viewDev_.ensure(d.size() * sizeof(ViewDesc)) || !viewCtr_.ensure(d.size() * sizeof(ViewCounters)))
+            return setError("hipMalloc views"), false;
+        // the previous draw may still read the table
+        if (!ok(hipStreamSynchronize(stream_), "view table") ||
+            !ok(hipMemcpy(viewDev_.ptr, d.data(), d.size() * sizeof(ViewDesc), hipMemcpyHostToDevice), "view table"))
+            return false;
+        viewDesc_ = d;
+    }
+    if (layout != viewLayout_) {
+        std::vector<unsigned short> tv(std::max(1u, tiles)), sh(std::max(1u, shade));
+        for (size_t v = 0; v < d.size(); v++) {
+            const unsigned n = (unsigned)(d[v].tilesX * d[v].tilesY);
+            std::fill(tv.begin() + d[v].tileBase, tv.begin() + d[v].tileBase + n, (unsigned short)v);
+            if (hv[v].shade) {
+                const unsigned m = (unsigned)(((hv[v].width + shadeTile_ - 1) / shadeTile_) *
+                                              ((hv[v].height + shadeTile_ - 1) / shadeTile_));
+                std::fill(sh.begin() + d[v].shadeBase, sh.begin() + d[v].shadeBase + m, (unsigned short)v);
+            }
+        }
+        if (!tileView_.ensure(tv.size() * 2) || !shadeView_.ensure(sh.size() * 2))
+            return setError("hipMalloc view maps"), false;
+        if (!ok(hipStreamSynchronize(stream_), "view maps") ||
+            !ok(hipMemcpy(tileView_.ptr, tv.data(), tv.size() * 2, hipMemcpyHostToDevice), "view maps") ||
+            !ok(hipMemcpy(shadeView_.ptr, sh.data(), sh.size() * 2, hipMemcpyHostToDevice), "view maps"))
+            return false;
+        viewLayout_ = layout;
+    }
+    return true;
+}
+
+Views TileRasterRenderer::deviceViews() const {
+    Views V{};
+    V.desc = viewDev_.as<ViewDesc>();
+    V.tileView = tileView_.as<unsigned short>();
+    V.shadeView = shadeView_.as<unsigned short>();
+    V.ctr = viewCtr_.as<ViewCounters>();
+    V.count = (int)viewDesc_.size();
+    V.slotsPerView = (unsigned)curTris_ + clipCap_;
+    return V;
+}
+
 TriRecs TileRasterRenderer::recPtrs() const {
     return TriRecs{recs_[0].as<float4>(), recs_[1].as<float4>(), recs_[2].as<float4>(),
                    recs_[3].as<float4>(), recs_[4].as<float4>(), recs_[5].as<float4>()};
 }
 
 StatPartials TileRasterRenderer::statPtrs() const {
-    const int tilesX = (width_ + kTile - 1) / kTile, tilesY = (height_ + kTile - 1) / kTile;
     StatPartials st{};
-    (void)tilesX;
-    (void)tilesY;
-    st.setupBlocks = (curTris_ + 255) / 256;
+    st.setupBlocks = (curTris_ + 255) / 256 * (int)std::max<size_t>(1, viewDesc_.size());
     st.rasterBlocks = rasterGrid();
     unsigned long long* base = statPart_.as<unsigned long long>();
     st.setupAssembled = base;
@@ -420,51 +529,41 @@ StatPartials TileRasterRenderer::statPtrs() const {
     return st;
 }
 
-int TileRasterRenderer::rasterGrid() const { return numCUs_ * 4; }
+int TileRasterRenderer::rasterGrid() const { return numCUs_ * rasterGroupsPerCU_; }
 
 void TileRasterRenderer::enqueue(bool clear) {
-    const int tilesX = (width_ + kTile - 1) / kTile, tilesY = (height_ + kTile - 1) / kTile;
-    const int tiles = tilesX * tilesY;
+    const int tiles = (int)totalTiles_;
+    const Views V = deviceViews();
     hipStream_t s = stream_;
     auto mark = [&](int i) {
-        if (stageTiming_) (void)hipEventRecord(ev_[i], s);
+        if (stageTiming_ && !capturing_) (void)hipEventRecord(ev_[i], s);
     };
     mark(0);
     if (!hostVs_)
-        launch_vertex(vtx_.as<float>(), stride_, nverts_, vs_.desc.mvp, vs_.desc.kind == IFHIP_VS_CLIP_PASSTHROUGH ? 1 : 0,
+        launch_vertex(vtx_.as<float>(), stride_, nverts_, V, vs_.desc.kind == IFHIP_VS_CLIP_PASSTHROUGH ? 1 : 0,
                       clip_.as<float4>(), binCount_.as<unsigned>(), binCursor_.as<unsigned>(), tiles,
                       ctr_.as<DeviceCounters>(), s);
     mark(1);
-    if (hostVs_) launch_reset(binCount_.as<unsigned>(), binCursor_.as<unsigned>(), tiles, ctr_.as<DeviceCounters>(), s);
+    if (hostVs_) launch_reset(binCount_.as<unsigned>(), binCursor_.as<unsigned>(), tiles, V, ctr_.as<DeviceCounters>(), s);
     mark(2);
     SetupParams sp{};
-    sp.width = width_;
-    sp.height = height_;
-    sp.tilesX = tilesX;
-    sp.tilesY = tilesY;
     sp.ntri = curTris_;
     sp.nverts = nverts_;
     sp.ccw = 0;
     sp.msaa = samples_ > 1 ? 1 : 0;
     sp.clipCap = clipCap_;
-    sp.invW = 1.0f / (float)width_;  // TileRasterRenderer.cpp:59-60
-    sp.invH = 1.0f / (float)height_;
     sp.invTileWidth = ifx::rcp_nr_ss(16.0f);
-    launch_setup(sp, clip_.as<float4>(), idx_.as<int>(), recPtrs(), bary_.as<float4>(), tileBox_.as<int4>(),
+    launch_setup(sp, V, clip_.as<float4>(), idx_.as<int>(), recPtrs(), bary_.as<float4>(), tileBox_.as<int4>(),
                  slotOf_.as<unsigned>(), clipList_.as<unsigned>(), statPtrs(), ctr_.as<DeviceCounters>(), s);
     mark(3);
     RasterQueue q{items_.as<uint2>(), tilePartial_.as<int>(), multiList_.as<unsigned>(),
-                  partials_.as<unsigned long long>(), itemCap_, partialCap_, rasterGrid()};
-    BinParams bp{tilesX, tilesY, curTris_, clipCap_, binCap_, bigCap_, kBigThreshold};
-    launch_bin(bp, recPtrs(), tileBox_.as<int4>(), binCount_.as<unsigned>(), binStart_.as<unsigned>(),
-               binCursor_.as<unsigned>(), binList_.as<unsigned>(), bigList_.as<unsigned>(), q, ctr_.as<DeviceCounters>(),
-               s);
+                  partials_.as<unsigned long long>(), itemCap_, partialCap_, rasterGrid(), partShift_};
+    BinParams bp{tiles, curTris_, clipCap_, binCap_, bigCap_, bigThreshold_};
+    launch_bin(bp, V, recPtrs(), tileBox_.as<int4>(), binCount_.as<unsigned>(), binStart_.as<unsigned>(),
+               binCursor_.as<unsigned>(), binList_.as<unsigned>(), bigList_.as<unsigned>(), q,
+               ctr_.as<DeviceCounters>(), s);
     mark(4);
     RasterParams rp{};
-    rp.width = width_;
-    rp.height = height_;
-    rp.tilesX = tilesX;
-    rp.tilesY = tilesY;
     rp.depthFunc = depthFunc_;
     rp.clear = clear ? 1 : 0;
     rp.binCap = binCap_;
@@ -473,19 +572,19 @@ void TileRasterRenderer::enqueue(bool clear) {
         rp.sx[i] = st[2 * i];
         rp.sy[i] = st[2 * i + 1];
     }
-    launch_raster(rp, samples_, recPtrs(), slotOf_.as<unsigned>(), binStart_.as<unsigned>(),
-                  binCount_.as<unsigned>(), binList_.as<unsigned>(), q, visDepth_.as<float>(), visSlot_.as<uint32_t>(),
-                  visClass_.as<uint8_t>(), statPtrs(), ctr_.as<DeviceCounters>(), s);
+    // the 1-sample coverage class is a single-view diagnostic
+    uint8_t* cls = batch_.empty() ? visClass_.as<uint8_t>() : nullptr;
+    launch_raster(rp, V, samples_, recPtrs(), slotOf_.as<unsigned>(), binStart_.as<unsigned>(),
+                  binCount_.as<unsigned>(), binList_.as<unsigned>(), q, visDepth_.as<float>(),
+                  visSlot_.as<uint32_t>(), cls, statPtrs(), ctr_.as<DeviceCounters>(), s);
     mark(5);
-    launch_raster_merge(rp, samples_, recPtrs(), slotOf_.as<unsigned>(), binStart_.as<unsigned>(),
-                        binCount_.as<unsigned>(), q, visDepth_.as<float>(), visSlot_.as<uint32_t>(),
-                        visClass_.as<uint8_t>(), ctr_.as<DeviceCounters>(), s);
+    launch_raster_merge(rp, V, samples_, recPtrs(), slotOf_.as<unsigned>(), binStart_.as<unsigned>(),
+                        binCount_.as<unsigned>(), q, visDepth_.as<float>(), visSlot_.as<uint32_t>(), cls, tiles,
+                        ctr_.as<DeviceCounters>(), s);
     mark(6);
-    if (fsBound_ && fs_.desc.kind != IFHIP_FS_DEPTH_ONLY) {
+    if (fsBound_ && fs_.desc.kind != IFHIP_FS_DEPTH_ONLY && shadeTiles_ > 0) {
         ShadeParams sh{};
         const IfritGpuFragmentProgramDesc& d = fs_.desc;
-        sh.width = width_;
-        sh.height = height_;
         sh.samples = samples_;
         // the neural program falls back to the layered-GGX material when validation found it below its bound
         sh.program = (d.kind == IFHIP_FS_NEURAL && nnFallback_[d.precision == 1 ? 1 : 0]) ? IFHIP_FS_GGX_LAYERED : d.kind;
@@ -498,7 +597,6 @@ void TileRasterRenderer::enqueue(bool clear) {
         sh.sampler = d.sampler;
         sh.maxAniso = std::min(16, std::max(1, d.maxAniso));  // the tap list encodes up to 16 taps
         sh.numTextures = (int)textures_.size();
-        for (int k = 0; k < 3; k++) sh.camera[k] = d.camera[k];
         sh.ambient = d.ambient;
         sh.kd = d.kd;
         sh.ks = d.ks;
@@ -506,40 +604,101 @@ void TileRasterRenderer::enqueue(bool clear) {
         sh.numLights = std::min(4, std::max(0, d.numLights));
         std::memcpy(sh.lightPos, d.lightPos, sizeof(sh.lightPos));
         std::memcpy(sh.lightColor, d.lightColor, sizeof(sh.lightColor));
-        const bool wantFloat = fb_ && !fb_->color.empty() && fb_->color[0];
+        const bool wantFloat = batch_.empty() && fb_ && !fb_->color.empty() && fb_->color[0];
         sh.writeFloat = wantFloat ? 1 : 0;
         sh.ntri = curTris_;
+        sh.shadeTile = shadeTile_;
         sh.neural = neuralShadeParams();
-        launch_shade(sh, recPtrs(), bary_.as<float4>(), visSlot_.as<uint32_t>(), vtx_.as<float>(),
-                     idx_.as<int>(), texDesc_.as<TexDesc>(), nullptr, rgba8_.as<uchar4>(),
+        launch_shade(sh, V, (int)shadeTiles_, recPtrs(), bary_.as<float4>(), visSlot_.as<uint32_t>(),
+                     vtx_.as<float>(), idx_.as<int>(), texDesc_.as<TexDesc>(), nullptr, rgba8_.as<uchar4>(),
                      wantFloat ? rgba32f_.as<float4>() : nullptr, s);
     }
     mark(7);
+}
+
+// Everything a captured graph bakes into its launches.  View tables live in
+// device memory, so cameras and transforms change without a new capture.
+std::vector<uint8_t> TileRasterRenderer::graphKey(bool clear) const {
+    std::vector<uint8_t> k;
+    auto put = [&](const void* p, size_t n) {
+        const uint8_t* b = static_cast<const uint8_t*>(p);
+        k.insert(k.end(), b, b + n);
+    };
+    const int ints[] = {clear ? 1 : 0, curTris_, nverts_, stride_, samples_, depthFunc_, (int)viewDesc_.size(),
+                        (int)totalTiles_, (int)shadeTiles_, vs_.desc.kind, fsBound_ ? 1 : 0, hostVs_ ? 1 : 0,
+                        partShift_, rasterGroupsPerCU_, shadeTile_, bigThreshold_, nnFallback_[0] ? 1 : 0,
+                        nnFallback_[1] ? 1 : 0, (int)textures_.size(), stageTiming_ ? 1 : 0,
+                        fb_ && !fb_->color.empty() && fb_->color[0] ? 1 : 0, batch_.empty() ? 1 : 0};
+    put(ints, sizeof(ints));
+    const unsigned caps[] = {clipCap_, binCap_, bigCap_, itemCap_, partialCap_};
+    put(caps, sizeof(caps));
+    put(&totalPixels_, sizeof(totalPixels_));
+    put(&DevBuf::generation, sizeof(DevBuf::generation));
+    IfritGpuFragmentProgramDesc fd = fs_.desc;
+    std::memset(fd.camera, 0, sizeof(fd.camera));  // cameras live in the view table
+    put(&fd, sizeof(fd));
+    const NeuralShadeParams np = neuralShadeParams();
+    put(&np, sizeof(np));
+    return k;
+}
+
+void TileRasterRenderer::dropGraph() {
+    if (graphExec_) (void)hipGraphExecDestroy(graphExec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graphExec_ = nullptr;
+    graph_ = nullptr;
+    graphKey_.clear();
+}
+
+// Issues one draw: directly, or as a replay of the captured graph of this
+// configuration (captured on first use).
+void TileRasterRenderer::submit(bool clear) {
+    if (!graphMode_) {
+        enqueue(clear);
+        return;
+    }
+    std::vector<uint8_t> key = graphKey(clear);
+    if (!graphExec_ || key != graphKey_) {
+        dropGraph();
+        if (!ok(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "graph capture")) return;
+        capturing_ = true;
+        enqueue(clear);
+        capturing_ = false;
+        hipGraph_t g = nullptr;
+        if (!ok(hipStreamEndCapture(stream_, &g), "graph capture")) return;
+        graph_ = g;
+        if (!ok(hipGraphInstantiate(&graphExec_, graph_, nullptr, nullptr, 0), "graph instantiate")) {
+            dropGraph();
+            return;
+        }
+        graphKey_ = std::move(key);
+    }
+    ok(hipGraphLaunch(graphExec_, stream_), "graph launch");
 }
 
 bool TileRasterRenderer::finish(bool clear, int indexCount) {
     (void)clear;
     (void)indexCount;
     // frame statistics are reduced from the per-workgroup partials only when read
-    launch_finalize(statPtrs(), ctr_.as<DeviceCounters>(), stream_);
+    launch_finalize(statPtrs(), deviceViews(), ctr_.as<DeviceCounters>(), stream_);
     if (!ok(hipStreamSynchronize(stream_), "frame")) return false;
     DeviceCounters c{};
     if (!ok(hipMemcpy(&c, ctr_.ptr, sizeof(c), hipMemcpyDeviceToHost), "counters")) return false;
-    stats_.inputTriangles = curTris_;
+    stats_.inputTriangles = (int64_t)curTris_ * (int64_t)viewDesc_.size();
     stats_.assembledTriangles = (int64_t)c.assembled;
     stats_.clippedTriangles = (int64_t)c.clipCount;
     stats_.binEntries = c.binTotal;
     stats_.fragments = (int64_t)c.fragments;
     stats_.coveredSamples = (int64_t)c.coveredSamples;
     stats_.status = c.status;
-    if (stageTiming_) {
+    if (stageTiming_ && !graphMode_) {
         for (int i = 0; i < 7; i++) {
             float ms = 0;
             (void)hipEventElapsedTime(&ms, ev_[i], ev_[i + 1]);
             stats_.stageMs[i] = ms;
         }
     }
-    if (c.status & kStatusClipOverflow) clipCap_ = std::max(clipCap_ * 2, c.clipCount + 1024u);
+    if (c.status & kStatusClipOverflow) clipCap_ = std::max(clipCap_ * 2, c.clipMax + 1024u);
     if (c.status & kStatusBinOverflow) binCap_ = std::max<unsigned>(binCap_ * 2, c.binTotal + (1u << 20));
     if (c.status & kStatusBigOverflow) bigCap_ = std::max(bigCap_ * 2, c.bigCount + 1024u);
     if (c.status & kStatusRasterOverflow) {
@@ -554,7 +713,7 @@ void TileRasterRenderer::drawElements(int indexCount, bool clear) {
     stats_.retries = 0;
     for (int attempt = 0; attempt < 4; attempt++) {
         if (!prepare(indexCount)) return;
-        enqueue(clear);
+        submit(clear);
         if (!ok(hipGetLastError(), "kernel launch")) return;
         if (finish(clear, indexCount)) break;
         stats_.retries++;
@@ -568,7 +727,7 @@ void TileRasterRenderer::drawElements(int indexCount, bool clear) {
 
 void TileRasterRenderer::drawAsync(int indexCount, bool clear) {
     if (!prepare(indexCount)) return;
-    enqueue(clear);
+    submit(clear);
     ok(hipGetLastError(), "kernel launch");
 }
 
@@ -578,7 +737,7 @@ void TileRasterRenderer::synchronize() {
 }
 
 void TileRasterRenderer::copyOutToHost() {
-    if (!fb_) return;
+    if (!fb_ || !batch_.empty()) return;
     const size_t px = (size_t)width_ * height_;
     if (!fb_->color.empty() && fb_->color[0] && fsBound_) {
         ImageF32* img = fb_->color[0];
@@ -611,22 +770,182 @@ void TileRasterRenderer::clear() {
     }
 }
 
-void TileRasterRenderer::getVisibility(uint32_t* ids, float* depth) {
-    const size_t n = (size_t)width_ * height_ * samples_;
+void TileRasterRenderer::getViewVisibility(int view, uint32_t* ids, float* depth) {
+    if (view < 0 || view >= (int)viewDesc_.size()) return setError("bad view index");
+    const ViewDesc& D = viewDesc_[view];
+    const size_t n = (size_t)D.width * D.height * samples_, off = (size_t)D.pixBase * samples_;
     if (ids) {
-        launch_slot_to_prim(visSlot_.as<uint32_t>(), recPtrs(), n, visId_.as<uint32_t>(), stream_);
+        launch_slot_to_prim(visSlot_.as<uint32_t>() + off, recPtrs(), n, visId_.as<uint32_t>() + off, stream_);
         if (!ok(hipStreamSynchronize(stream_), "id expansion")) return;
-        ok(hipMemcpy(ids, visId_.ptr, n * 4, hipMemcpyDeviceToHost), "id readback");
+        ok(hipMemcpy(ids, visId_.as<uint32_t>() + off, n * 4, hipMemcpyDeviceToHost), "id readback");
     }
-    if (depth) ok(hipMemcpy(depth, visDepth_.ptr, n * 4, hipMemcpyDeviceToHost), "depth readback");
+    if (depth) ok(hipMemcpy(depth, visDepth_.as<float>() + off, n * 4, hipMemcpyDeviceToHost), "depth readback");
 }
 
+void TileRasterRenderer::getViewColorRGBA8(int view, uint8_t* rgba) {
+    if (view < 0 || view >= (int)viewDesc_.size()) return setError("bad view index");
+    const ViewDesc& D = viewDesc_[view];
+    ok(hipMemcpy(rgba, rgba8_.as<uint8_t>() + (size_t)D.pixBase * 4, (size_t)D.width * D.height * 4,
+                 hipMemcpyDeviceToHost),
+       "colour readback");
+}
+
+void TileRasterRenderer::getVisibility(uint32_t* ids, float* depth) { getViewVisibility(0, ids, depth); }
+
 void TileRasterRenderer::getCoverageClass(uint8_t* cls) {
+    if (!batch_.empty()) return setError("coverage classes are kept for single-view draws only");
     ok(hipMemcpy(cls, visClass_.ptr, (size_t)width_ * height_, hipMemcpyDeviceToHost), "class readback");
 }
 
-void TileRasterRenderer::getColorRGBA8(uint8_t* rgba) {
-    ok(hipMemcpy(rgba, rgba8_.ptr, (size_t)width_ * height_ * 4, hipMemcpyDeviceToHost), "colour readback");
+void TileRasterRenderer::getColorRGBA8(uint8_t* rgba) { getViewColorRGBA8(0, rgba); }
+
+void TileRasterRenderer::setViews(const IfritViewDesc* views, int n) {
+    if (n < 0 || n > kMaxViews || (n > 0 && !views)) return setError("bad view count");
+    std::vector<HostView> b((size_t)n);
+    for (int i = 0; i < n; i++) {
+        const IfritViewDesc& v = views[i];
+        if (v.width <= 0 || v.height <= 0 || v.width > 16384 || v.height > 16384)
+            return setError("view sizes must be 1..16384");
+        b[i].width = v.width;
+        b[i].height = v.height;
+        std::memcpy(b[i].mvp, v.mvp, sizeof(b[i].mvp));
+        for (int k = 0; k < 3; k++) b[i].camera[k] = v.camera[k];
+        b[i].shade = v.shade ? 1 : 0;
+    }
+    batch_ = std::move(b);
+}
+
+void TileRasterRenderer::setGraphMode(bool on) {
+    if (!on) dropGraph();
+    graphMode_ = on;
+}
+
+void TileRasterRenderer::setTuning(const IfritTuning& t) {
+    const int part = t.rasterPartEntries;
+    if ((part != 256 && part != 512 && part != 1024) || t.rasterGroupsPerCU < 1 || t.rasterGroupsPerCU > 8 ||
+        (t.shadeTile != 8 && t.shadeTile != 16 && t.shadeTile != 32) || t.bigTriangleTiles < 1)
+        return setError("tuning out of range: part 256/512/1024, 1..8 groups per CU, shading tile 8/16/32");
+    partShift_ = part == 256 ? 8 : (part == 512 ? 9 : 10);
+    rasterGroupsPerCU_ = t.rasterGroupsPerCU;
+    shadeTile_ = t.shadeTile;
+    bigThreshold_ = t.bigTriangleTiles;
+    if (t.binCapacity && t.binCapacity != binCap_) {
+        // an explicit capacity may also shrink the list (it grows again on overflow)
+        binCap_ = t.binCapacity;
+        binList_.release();
+        DevBuf::generation++;
+    }
+}
+
+void TileRasterRenderer::getTuning(IfritTuning* t) const {
+    t->rasterPartEntries = 1 << partShift_;
+    t->rasterGroupsPerCU = rasterGroupsPerCU_;
+    t->shadeTile = shadeTile_;
+    t->bigTriangleTiles = bigThreshold_;
+    t->binCapacity = binCap_;
+}
+
+uint64_t TileRasterRenderer::digest() {
+    if (viewDesc_.empty() || !digest_.ensure(3 * 8)) return 0;
+    const size_t ps = totalPixels_ * samples_;
+    // primitive IDs (not record slots, whose numbering depends on capacities)
+    launch_slot_to_prim(visSlot_.as<uint32_t>(), recPtrs(), ps, visId_.as<uint32_t>(), stream_);
+    if (!ok(hipMemsetAsync(digest_.ptr, 0, 3 * 8, stream_), "digest")) return 0;
+    // samples outside any view (64-pixel padding) hold stale values: digest views one by one
+    for (const ViewDesc& D : viewDesc_) {
+        const size_t off = (size_t)D.pixBase, n = (size_t)D.width * D.height;
+        launch_digest(reinterpret_cast<const uint32_t*>(visDepth_.as<float>() + off * samples_), n * samples_,
+                      digest_.as<unsigned long long>(), stream_);
+        launch_digest(visId_.as<uint32_t>() + off * samples_, n * samples_, digest_.as<unsigned long long>() + 1,
+                      stream_);
+        if (D.shade && fsBound_ && fs_.desc.kind != IFHIP_FS_DEPTH_ONLY)
+            launch_digest(reinterpret_cast<const uint32_t*>(rgba8_.as<uchar4>() + off), n,
+                          digest_.as<unsigned long long>() + 2, stream_);
+    }
+    unsigned long long h[3] = {0, 0, 0};
+    if (!ok(hipMemcpyAsync(h, digest_.ptr, sizeof(h), hipMemcpyDeviceToHost, stream_), "digest") ||
+        !ok(hipStreamSynchronize(stream_), "digest"))
+        return 0;
+    return h[0] ^ (h[1] * 0x9E3779B97F4A7C15ull) ^ (h[2] * 0xC2B2AE3D27D4EB4Full);
+}
+
+// Coordinate search: each parameter in turn over its values with the others
+// held at the best so far, two rounds; every configuration must reproduce the
+// first one's digest.
+void TileRasterRenderer::autotune(int indexCount, int frames, IfritAutotuneReport* out) {
+    IfritAutotuneReport rep{};
+    const bool graphSaved = graphMode_, timingSaved = stageTiming_;
+    setGraphMode(true);
+    stageTiming_ = false;
+    frames = std::max(1, frames);
+    IfritTuning best{};
+    getTuning(&best);
+    unsigned long long ref = 0;
+    bool haveRef = false;
+    rep.identical = 1;
+    hipEvent_t e0 = ev_[0], e1 = ev_[1];
+    auto measure = [&](IfritTuning t) -> float {
+        t.binCapacity = 0;  // keep the current capacity (it grows on overflow)
+        setTuning(t);
+        drawElements(indexCount, true);  // captures the graph, grows buffers if needed
+        if (!error_.empty()) return -1.0f;
+        const unsigned long long d = digest();
+        if (!haveRef) {
+            ref = d;
+            haveRef = true;
+        } else if (d != ref) {
+            rep.identical = 0;
+        }
+        drawAsync(indexCount, true);  // warm replay
+        (void)hipEventRecord(e0, stream_);
+        for (int f = 0; f < frames; f++) drawAsync(indexCount, true);
+        (void)hipEventRecord(e1, stream_);
+        if (!ok(hipEventSynchronize(e1), "autotune")) return -1.0f;
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= (float)frames;
+        if (rep.candidates < 64) {
+            rep.tried[rep.candidates] = t;
+            rep.ms[rep.candidates] = ms;
+        }
+        rep.candidates++;
+        return ms;
+    };
+    float bestMs = measure(best);
+    rep.defaultMs = bestMs;
+    const int parts[] = {256, 512, 1024}, groups[] = {2, 4, 8}, shades[] = {8, 16, 32}, bigs[] = {16, 64, 256};
+    for (int round = 0; round < 2 && bestMs >= 0.0f; round++) {
+        for (int knob = 0; knob < 4; knob++) {
+            for (int i = 0; i < 3; i++) {
+                IfritTuning t = best;
+                int* field = knob == 0 ? &t.rasterPartEntries
+                                       : (knob == 1 ? &t.rasterGroupsPerCU : (knob == 2 ? &t.shadeTile : &t.bigTriangleTiles));
+                const int val = knob == 0 ? parts[i] : (knob == 1 ? groups[i] : (knob == 2 ? shades[i] : bigs[i]));
+                if (*field == val) continue;
+                *field = val;
+                const float ms = measure(t);
+                if (ms < 0.0f) break;
+                if (ms < bestMs) {
+                    bestMs = ms;
+                    best = t;
+                }
+            }
+        }
+    }
+    // bin capacity: the measured demand plus 1/8, in 64 Ki-entry steps
+    setTuning(best);
+    drawElements(indexCount, true);
+    rep.binDemand = (uint32_t)stats_.binEntries;
+    const unsigned long long want = ((unsigned long long)rep.binDemand * 9 / 8 + 0xFFFF) & ~0xFFFFull;
+    best.binCapacity = (uint32_t)std::min<unsigned long long>(std::max<unsigned long long>(want, 1u << 20), 0xFFFFFFF0ull);
+    setTuning(best);
+    drawElements(indexCount, true);
+    if (digest() != ref) rep.identical = 0;
+    getTuning(&rep.best);
+    rep.bestMs = bestMs;
+    setGraphMode(graphSaved);
+    stageTiming_ = timingSaved;
+    if (out) *out = rep;
 }
 
 }  // namespace ifrit_hip

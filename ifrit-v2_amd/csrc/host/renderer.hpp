@@ -73,8 +73,10 @@ struct TextureSlot {
     size_t bytes = 0;
 };
 
-// Grow-only device allocation.
+// Grow-only device allocation.  Every (re)allocation bumps `generation`, so
+// that captured graphs holding old addresses are recaptured.
 struct DevBuf {
+    static unsigned long long generation;
     void* ptr = nullptr;
     size_t bytes = 0;
     bool ensure(size_t n);
@@ -128,6 +130,16 @@ public:
     void setError(const std::string& e);
     bool ok(hipError_t e, const char* what);
 
+    // multi-view batches, graph replay and tuning (extensions)
+    void setViews(const IfritViewDesc* views, int n);
+    void getViewVisibility(int view, uint32_t* ids, float* depth);
+    void getViewColorRGBA8(int view, uint8_t* rgba);
+    void setGraphMode(bool on);
+    void setTuning(const IfritTuning& t);
+    void getTuning(IfritTuning* t) const;
+    void autotune(int indexCount, int frames, IfritAutotuneReport* out);
+    uint64_t digest();
+
     // neural material (neural.cpp)
     void trainNeural(const IfritNeuralTrainDesc& d, IfritNeuralTrainReport* report);
     void setNeuralParams(const float* params);
@@ -137,7 +149,12 @@ public:
 
 private:
     bool prepare(int indexCount);
+    bool layoutViews(int ntri);
     void enqueue(bool clear);
+    void submit(bool clear);
+    std::vector<uint8_t> graphKey(bool clear) const;
+    void dropGraph();
+    Views deviceViews() const;
     bool finish(bool clear, int indexCount);
     void copyOutToHost();
     void runHostVertexShader();
@@ -164,6 +181,31 @@ private:
     GpuVertexProgram vs_{};
     GpuFragmentProgram fs_{};
     bool vsBound_ = false, fsBound_ = false;
+
+    // views: batch_ empty = one view from the bound target, vertex and fragment programs
+    struct HostView {
+        int width, height;
+        float mvp[16];
+        float camera[3];
+        int shade;
+    };
+    std::vector<HostView> batch_;
+    std::vector<ViewDesc> viewDesc_;        // device table as last uploaded
+    std::vector<int> viewLayout_;           // (width, height, shade) per view + shading tile
+    unsigned totalTiles_ = 0, shadeTiles_ = 0;
+    size_t totalPixels_ = 0;
+    DevBuf viewDev_, tileView_, shadeView_, viewCtr_, digest_;
+
+    // launch parameters (IfritTuning) and graph replay
+    int partShift_ = kRasterPartShift;
+    int rasterGroupsPerCU_ = 4;
+    int shadeTile_ = 8;
+    int bigThreshold_ = 64;
+    bool graphMode_ = false;
+    bool capturing_ = false;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graphExec_ = nullptr;
+    std::vector<uint8_t> graphKey_;
 
     // device buffers
     DevBuf vtx_, idx_, clip_, recs_[6], bary_, tileBox_, slotOf_, binCount_, binStart_, binCursor_, binList_, bigList_, ctr_;

@@ -22,20 +22,20 @@ using dev::Edges;
 using dev::load_edges;
 using dev::tile_class;
 
-__device__ __forceinline__ unsigned num_slots(const BinParams& P, const DeviceCounters* ctr) {
-    return (unsigned)P.ntri + min(ctr->clipCount, P.clipCap);
-}
-
-// Shared traversal of the small triangles.  SCATTER = false counts,
-// SCATTER = true writes the lists.
+// Shared traversal of the small triangles, one thread per (slot, view) with
+// blockIdx.y = view.  SCATTER = false counts, SCATTER = true writes the lists.
 template <bool SCATTER>
-__global__ void __launch_bounds__(256) k_bin_small(BinParams P, TriRecs recs,
+__global__ void __launch_bounds__(256) k_bin_small(BinParams P, Views V, TriRecs recs,
                                                    const int4* __restrict__ tileBox, unsigned* __restrict__ binCount,
                                                    const unsigned* __restrict__ binStart,
                                                    unsigned* __restrict__ binCursor, unsigned* __restrict__ binList,
                                                    unsigned* __restrict__ bigList, DeviceCounters* ctr) {
-    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = s < num_slots(P, ctr);
+    const ViewDesc& D = V.desc[blockIdx.y];
+    const unsigned tileBase = D.tileBase;
+    const int tilesX = D.tilesX;
+    const unsigned local = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned s = D.slotBase + local;
+    bool valid = local < (unsigned)P.ntri + min(V.ctr[blockIdx.y].clipCount, P.clipCap);
     int4 b = make_int4(1, 1, 0, 0);
     if (valid) b = tileBox[s];
     valid = valid && b.x <= b.z && b.y <= b.w;
@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(256) k_bin_small(BinParams P, TriRecs recs,
         int t = 0, c = 0;
         if (active) {
             const int tx = b.x + i % w, ty = b.y + i / w;
-            t = ty * P.tilesX + tx;
+            t = (int)tileBase + ty * tilesX + tx;
             c = tile_class(E, tx, ty);
         }
         const bool hit = active && c != 0;
@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(256) k_bin_small(BinParams P, TriRecs recs,
 }
 
 template <bool SCATTER>
-__global__ void __launch_bounds__(256) k_bin_big(BinParams P, TriRecs recs,
+__global__ void __launch_bounds__(256) k_bin_big(BinParams P, Views V, TriRecs recs,
                                                  const int4* __restrict__ tileBox, unsigned* __restrict__ binCount,
                                                  const unsigned* __restrict__ binStart, unsigned* __restrict__ binCursor,
                                                  unsigned* __restrict__ binList, const unsigned* __restrict__ bigList,
@@ -86,6 +86,9 @@ __global__ void __launch_bounds__(256) k_bin_big(BinParams P, TriRecs recs,
     const unsigned nbig = min(ctr->bigCount, P.bigCap);
     for (unsigned i = blockIdx.x; i < nbig; i += gridDim.x) {
         const unsigned s = bigList[i];
+        const ViewDesc& D = V.desc[s / V.slotsPerView];
+        const unsigned tileBase = D.tileBase;
+        const int tilesX = D.tilesX;
         const int4 b = tileBox[s];
         const Edges E = load_edges(recs, s);
         const int w = b.z - b.x + 1;
@@ -94,7 +97,7 @@ __global__ void __launch_bounds__(256) k_bin_big(BinParams P, TriRecs recs,
             const int tx = b.x + q % w, ty = b.y + q / w;
             const int c = tile_class(E, tx, ty);
             if (!c) continue;
-            const int t = ty * P.tilesX + tx;
+            const int t = (int)tileBase + ty * tilesX + tx;
             if (!SCATTER) {
                 atomicAdd(&binCount[t], 1u);
             } else {
@@ -130,7 +133,7 @@ __device__ __forceinline__ unsigned scan1024(unsigned v, unsigned* sWave, unsign
 
 // One 1024-thread workgroup, in chunks of 8192 tiles: the exclusive scan of
 // the per-tile counts (list starts), then the raster work of each tile: parts
-// of at most kRasterPart entries, the first partial key block of a tile with
+// of at most 1 << q.partShift entries, the first partial key block of a tile with
 // several parts, the list of such tiles (multiList[0, numMulti)) and the list
 // of empty tiles (multiList[tiles, tiles + numEmpty)).  Parts of more than
 // kRasterPass entries are listed from the front of the item array and the
@@ -142,6 +145,8 @@ __global__ void __launch_bounds__(1024) k_tile_scan(const unsigned* __restrict__
     __shared__ unsigned sCnt[kChunk];
     __shared__ unsigned sWave[16];
     const int t = threadIdx.x;
+    const int sh = q.partShift;
+    const unsigned part = 1u << sh;
     unsigned carryS = 0, carryA = 0, carryL = 0, carryB = 0, carryM = 0, carryE = 0;
     for (int base = 0; base < n; base += kChunk) {
         __syncthreads();
@@ -161,9 +166,9 @@ __global__ void __launch_bounds__(1024) k_tile_scan(const unsigned* __restrict__
             run += c;
             if (tile < n) start[tile] = st;
             const unsigned total = st >= cap ? 0u : min(c, cap - st);
-            const unsigned p = tile < n ? (total + kRasterPart - 1) / kRasterPart : 0u;
+            const unsigned p = tile < n ? (total + part - 1u) >> sh : 0u;
             sCnt[t * kPer + i] = total;
-            const unsigned last = total - (p ? p - 1u : 0u) * kRasterPart;  // entries of the last part
+            const unsigned last = total - (p ? p - 1u : 0u) * part;  // entries of the last part
             const unsigned shortParts = p && last <= (unsigned)kRasterPass ? 1u : 0u;
             a += p - shortParts;
             l += shortParts;
@@ -181,8 +186,8 @@ __global__ void __launch_bounds__(1024) k_tile_scan(const unsigned* __restrict__
             const int tile = base + t * kPer + i;
             if (tile >= n) break;
             const unsigned total = sCnt[t * kPer + i];
-            const unsigned p = (total + kRasterPart - 1) / kRasterPart;
-            const unsigned last = total - (p ? p - 1u : 0u) * kRasterPart;
+            const unsigned p = (total + part - 1u) >> sh;
+            const unsigned last = total - (p ? p - 1u : 0u) * part;
             const unsigned longParts = p && last <= (unsigned)kRasterPass ? p - 1u : p;
             for (unsigned k = 0; k < longParts; k++)
                 if (ra + k < q.itemCap) q.items[ra + k] = make_uint2((unsigned)tile, k);
@@ -220,21 +225,20 @@ __global__ void __launch_bounds__(1024) k_tile_scan(const unsigned* __restrict__
 
 }  // namespace
 
-void launch_bin(const BinParams& p, TriRecs recs, const int4* tileBox, unsigned* binCount, unsigned* binStart,
-                unsigned* binCursor, unsigned* binList, unsigned* bigList, const RasterQueue& q, DeviceCounters* ctr,
-                hipStream_t s) {
+void launch_bin(const BinParams& p, const Views& views, TriRecs recs, const int4* tileBox, unsigned* binCount,
+                unsigned* binStart, unsigned* binCursor, unsigned* binList, unsigned* bigList, const RasterQueue& q,
+                DeviceCounters* ctr, hipStream_t s) {
     const size_t slots = (size_t)p.ntri + p.clipCap;
-    const int tiles = p.tilesX * p.tilesY;
-    const dim3 g(dev::blocks(slots > 0 ? slots : 1, 256));
-    hipLaunchKernelGGL(k_bin_small<false>, g, dim3(256), 0, s, p, recs, tileBox, binCount, binStart, binCursor,
+    const dim3 g(dev::blocks(slots > 0 ? slots : 1, 256), views.count);
+    hipLaunchKernelGGL(k_bin_small<false>, g, dim3(256), 0, s, p, views, recs, tileBox, binCount, binStart, binCursor,
                        binList, bigList, ctr);
-    hipLaunchKernelGGL(k_bin_big<false>, dim3(256), dim3(256), 0, s, p, recs, tileBox, binCount, binStart, binCursor,
+    hipLaunchKernelGGL(k_bin_big<false>, dim3(256), dim3(256), 0, s, p, views, recs, tileBox, binCount, binStart,
+                       binCursor, binList, bigList, ctr);
+    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, binCount, binStart, p.tiles, p.binCap, q, ctr);
+    hipLaunchKernelGGL(k_bin_small<true>, g, dim3(256), 0, s, p, views, recs, tileBox, binCount, binStart, binCursor,
                        binList, bigList, ctr);
-    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, binCount, binStart, tiles, p.binCap, q, ctr);
-    hipLaunchKernelGGL(k_bin_small<true>, g, dim3(256), 0, s, p, recs, tileBox, binCount, binStart, binCursor,
-                       binList, bigList, ctr);
-    hipLaunchKernelGGL(k_bin_big<true>, dim3(256), dim3(256), 0, s, p, recs, tileBox, binCount, binStart, binCursor,
-                       binList, bigList, ctr);
+    hipLaunchKernelGGL(k_bin_big<true>, dim3(256), dim3(256), 0, s, p, views, recs, tileBox, binCount, binStart,
+                       binCursor, binList, bigList, ctr);
 }
 
 }  // namespace ifrit_hip

@@ -1,16 +1,23 @@
 // Device-side data layout and launch interface of the MI355X tile rasterizer.
 //
-// HBM layout per draw (V vertices, T input triangles, W x H pixels, S samples):
-//   clip        float4[V]                 vertex stage output
-//   recs        TriRecs, T + clipCap slots assembled triangles, slot = primitive for
-//                                         unclipped triangles, T + k for clip fans
-//   bary        float4[2 * (T+clipCap)]   vertex-weight bases bx, by of clip fans (shading only)
-//   tileBox     int4[T + clipCap]         tile rectangle of each slot (x0 > x1: empty)
-//   slotOf      uint[T]                   first slot of each primitive (its fans follow)
+// A draw renders a batch of views (one or more) of the same vertex and index
+// data; every stage is one launch over all views.  HBM layout per draw
+// (V vertices, T input triangles, N views, SV = T + clipCap record slots per
+// view, S samples):
+//   views       ViewDesc[N]               per-view camera, size and bases
+//   tileView    ushort[tiles]             view of each 16x16 tile (tiles of all views)
+//   shadeView   ushort[shadeTiles]        view of each shading tile
+//   clip        float4[N][V]              vertex stage output
+//   recs        TriRecs, N * SV slots     assembled triangles; in view v, slot v*SV + prim
+//                                         for unclipped triangles, v*SV + T + k for clip fans
+//   bary        float4[2 * N * SV]        vertex-weight bases bx, by of clip fans (shading only)
+//   tileBox     int4[N * SV]              view-local tile rectangle of each slot (x0 > x1: empty)
+//   slotOf      uint[N][T]                first slot of each primitive (its fans follow)
 //   binCount/binStart/binCursor  uint[tiles]
 //   binList     uint[binCap]              slot | 0x80000000 when the tile is fully covered
-//   visId/visDepth/visSlot  [W*H*S]       per-sample winner, pixels row-major, samples innermost
-//   rgba8       uchar4[W*H]               resolved colour
+//   visId/visDepth/visSlot  [pixels * S]  per-sample winner; views one after another, pixels
+//                                         row-major within a view, samples innermost
+//   rgba8       uchar4[pixels]            resolved colour
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -38,6 +45,39 @@ struct TriRecs {
     float4 *q0, *q1, *q2, *f1, *f2, *f3;
 };
 
+// One view of a draw.  View v owns tiles [tileBase, tileBase + tilesX *
+// tilesY), record slots [slotBase, slotBase + SV), slotOf entries
+// [primBase, primBase + T) and pixels [pixBase, pixBase + width * height).
+struct ViewDesc {
+    float mvp[16];
+    float camera[3];
+    int shade;                   // 1: the fragment program runs for this view
+    int width, height, tilesX, tilesY;
+    float invW, invH;            // 1 / width, 1 / height (TileRasterRenderer.cpp:59-60)
+    unsigned tileBase;           // first 16x16 tile
+    unsigned shadeBase;          // first shading tile (shaded views only)
+    unsigned slotBase;           // first record slot (v * SV)
+    unsigned primBase;           // first slotOf entry (v * T)
+    unsigned long long pixBase;  // first pixel of the view's outputs
+};
+
+// Per-view counters of the clipping stage (zeroed by the vertex launch).
+struct ViewCounters {
+    unsigned clipCount;    // clip fan slots taken
+    unsigned clipInCount;  // triangles crossing w = 0
+};
+
+struct Views {
+    const ViewDesc* desc;
+    const unsigned short* tileView;   // view of each 16x16 tile
+    const unsigned short* shadeView;  // view of each shading tile
+    ViewCounters* ctr;
+    int count;
+    unsigned slotsPerView;            // SV = T + clipCap
+};
+
+// Per-view fields (width, height, tiles, invW, invH) are filled in by the
+// kernels from the view's ViewDesc.
 struct SetupParams {
     int width, height;
     int tilesX, tilesY;
@@ -50,7 +90,7 @@ struct SetupParams {
 };
 
 struct BinParams {
-    int tilesX, tilesY;
+    int tiles;          // tiles of all views
     int ntri;
     unsigned clipCap;
     unsigned binCap;
@@ -59,8 +99,6 @@ struct BinParams {
 };
 
 struct RasterParams {
-    int width, height;
-    int tilesX, tilesY;
     int depthFunc;
     int clear;
     unsigned binCap;
@@ -74,19 +112,20 @@ struct TexDesc {
 };
 
 struct ShadeParams {
-    int width, height, samples;
+    int samples;
     int program;
     int stride, offWorld, offNormal, offUv, offTangent;
     int trisPerMaterial;
     int sampler, maxAniso;
     int numTextures;
-    float camera[3];
     float ambient, kd, ks, shininess;
     int numLights;
     float lightPos[4][4];
     float lightColor[4][4];
     int writeFloat;          // also write RGBA32F
-    int ntri;                // slots >= ntri are clip fans with their own bx, by
+    int ntri;                // view-local slots >= ntri are clip fans with their own bx, by
+    int shadeTile;           // pixels per side of a shading tile (8, 16 or 32): one wave shades
+                             // its 8x8 sub-tiles in turn
     NeuralShadeParams neural;  // programs 4 (neural material) and 5 (training features)
 };
 
@@ -111,13 +150,13 @@ struct StatPartials {
 // Counters written by the kernels (device memory, zeroed each draw).
 struct DeviceCounters {
     unsigned status;
-    unsigned clipCount;
+    unsigned clipCount;  // clip fan slots of all views (reduced with the statistics)
     unsigned bigCount;
     unsigned binTotal;
     unsigned long long fragments;
     unsigned long long coveredSamples;
     unsigned long long assembled;
-    unsigned clipInCount;
+    unsigned clipMax;    // largest per-view clipCount or clipInCount (sizes clipCap on overflow)
     unsigned itemHead;   // next raster work item
     unsigned numItems;
     unsigned numMulti;     // tiles cut into several parts
@@ -126,10 +165,11 @@ struct DeviceCounters {
     unsigned numLongItems; // items of more than kRasterPass entries (front of the item array)
 };
 
-// Raster work: a work item covers at most kRasterPart list entries of one
-// tile, taken in passes of kRasterPass entries (the LDS capacity).
+// Raster work: a work item covers at most 1 << partShift list entries of
+// one tile (256 to 1024), taken in passes of kRasterPass entries (the LDS
+// capacity).
 constexpr int kRasterPass = 128;
-constexpr int kRasterPart = 512;
+constexpr int kRasterPartShift = 9;
 
 // Raster work: (tile, part) items drained through DeviceCounters::itemHead;
 // per tile the first partial block of a tile cut into several parts (-1
@@ -142,31 +182,35 @@ struct RasterQueue {
     unsigned long long* partials;
     unsigned itemCap, partialCap;
     int grid;
+    int partShift;
 };
 
 // Launch wrappers (pipeline.hip).
-void launch_vertex(const float* vtx, int stride, int nverts, const float* mvp16, int passthrough, float4* clip,
+void launch_vertex(const float* vtx, int stride, int nverts, const Views& views, int passthrough, float4* clip,
                    unsigned* binCount, unsigned* binCursor, int tiles, DeviceCounters* ctr, hipStream_t s);
-void launch_reset(unsigned* binCount, unsigned* binCursor, int tiles, DeviceCounters* ctr, hipStream_t s);
-void launch_setup(const SetupParams& p, const float4* clip, const int* idx, TriRecs recs, float4* bary, int4* tileBox,
+void launch_reset(unsigned* binCount, unsigned* binCursor, int tiles, const Views& views, DeviceCounters* ctr,
+                  hipStream_t s);
+void launch_setup(const SetupParams& p, const Views& views, const float4* clip, const int* idx, TriRecs recs, float4* bary, int4* tileBox,
                   unsigned* slotOf, unsigned* clipList, StatPartials st, DeviceCounters* ctr, hipStream_t s);
-void launch_finalize(StatPartials st, DeviceCounters* ctr, hipStream_t s);
-void launch_bin(const BinParams& p, TriRecs recs, const int4* tileBox, unsigned* binCount, unsigned* binStart,
+void launch_finalize(StatPartials st, const Views& views, DeviceCounters* ctr, hipStream_t s);
+void launch_bin(const BinParams& p, const Views& views, TriRecs recs, const int4* tileBox, unsigned* binCount, unsigned* binStart,
                 unsigned* binCursor, unsigned* binList, unsigned* bigList, const RasterQueue& q, DeviceCounters* ctr,
                 hipStream_t s);
-void launch_raster(const RasterParams& p, int samples, TriRecs recs, const unsigned* slotOf,
+void launch_raster(const RasterParams& p, const Views& views, int samples, TriRecs recs, const unsigned* slotOf,
                    const unsigned* binStart, const unsigned* binCount, const unsigned* binList, const RasterQueue& q,
                    float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st, DeviceCounters* ctr,
                    hipStream_t s);
-void launch_raster_merge(const RasterParams& p, int samples, TriRecs recs, const unsigned* slotOf,
+void launch_raster_merge(const RasterParams& p, const Views& views, int samples, TriRecs recs, const unsigned* slotOf,
                          const unsigned* binStart, const unsigned* binCount, const RasterQueue& q, float* visDepth,
-                         uint32_t* visSlot, uint8_t* visClass, DeviceCounters* ctr, hipStream_t s);
+                         uint32_t* visSlot, uint8_t* visClass, int tiles, DeviceCounters* ctr, hipStream_t s);
 void launch_slot_to_prim(const uint32_t* visSlot, TriRecs recs, size_t n, uint32_t* visId, hipStream_t s);
-void launch_shade(const ShadeParams& p, TriRecs recs, const float4* bary, const uint32_t* visSlot,
+void launch_shade(const ShadeParams& p, const Views& views, int shadeTiles, TriRecs recs, const float4* bary, const uint32_t* visSlot,
                   const float* vtx, const int* idx, const TexDesc* tex, const uint8_t* texPool, uchar4* rgba8,
                   float4* rgba32f, hipStream_t s);
 
 // Texture helpers (texture.hip).
+// Order-independent 64-bit digest of n words (equal buffers give equal digests).
+void launch_digest(const uint32_t* data, size_t n, unsigned long long* out, hipStream_t s);
 void launch_fbm_texture(uint8_t* level0, int size, unsigned seed, hipStream_t s);
 void launch_mip_level(const uint8_t* src, int sw, int sh, uint8_t* dst, int dw, int dh, hipStream_t s);
 

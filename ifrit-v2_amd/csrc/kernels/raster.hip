@@ -22,7 +22,7 @@
 // busy whatever the mix of triangle sizes, which one-triangle-per-lane loops
 // do not achieve.
 //
-// Scheduling.  A tile's list is cut into parts of at most kPart entries and
+// Scheduling.  A tile's list is cut into parts of at most 1 << partShift entries and
 // all parts form one work queue that a persistent grid drains, so a few
 // tiles holding thousands of sub-pixel triangles do not serialise on one
 // compute unit.  A part goes through LDS in passes of kPass entries, all
@@ -34,6 +34,8 @@
 // multi-XCD MI355X each one writes back an L2).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device_common.h"
 
 namespace ifrit_hip {
@@ -42,7 +44,6 @@ namespace {
 
 using namespace dev;
 
-constexpr int kPart = kRasterPart;  // list entries per work item
 constexpr int kPass = kRasterPass;  // list entries per pass through LDS
 constexpr int kUnits = 4096;  // units expanded into LDS per round
 
@@ -190,6 +191,27 @@ __device__ __forceinline__ unsigned winner_class(const TriRec& R, int tx, int ty
     return c;
 }
 
+// The view a tile belongs to: its size and where its outputs and slotOf
+// entries start.
+struct ViewGeom {
+    int width, height, tilesX;
+    int tile;  // view-local tile index
+    size_t pixBase;
+    unsigned primBase;
+};
+
+__device__ __forceinline__ ViewGeom view_of_tile(const Views& V, int gtile) {
+    const ViewDesc& D = V.desc[V.tileView[gtile]];
+    ViewGeom G;
+    G.width = D.width;
+    G.height = D.height;
+    G.tilesX = D.tilesX;
+    G.tile = gtile - (int)D.tileBase;
+    G.pixBase = (size_t)D.pixBase;
+    G.primBase = D.primBase;
+    return G;
+}
+
 struct PixelCtx {
     int tx, ty, px, py;
     bool inFb;
@@ -197,14 +219,14 @@ struct PixelCtx {
     float fpx, fpy, bMinX, bMinY;
 };
 
-__device__ __forceinline__ PixelCtx pixel_ctx(const RasterParams& P, int tile, int lx, int ly) {
+__device__ __forceinline__ PixelCtx pixel_ctx(const ViewGeom& G, int lx, int ly) {
     PixelCtx C;
-    C.tx = tile % P.tilesX;
-    C.ty = tile / P.tilesX;
+    C.tx = G.tile % G.tilesX;
+    C.ty = G.tile / G.tilesX;
     C.px = C.tx * 16 + lx;
     C.py = C.ty * 16 + ly;
-    C.inFb = C.px < P.width && C.py < P.height;
-    C.pix = (size_t)C.py * P.width + C.px;
+    C.inFb = C.px < G.width && C.py < G.height;
+    C.pix = (size_t)C.py * G.width + C.px;
     C.fpx = (float)C.px;
     C.fpy = (float)C.py;
     C.bMinX = fmaf((float)(lx >> 2), 4.0f, (float)(C.tx * 16) * 1.0f);
@@ -293,16 +315,18 @@ __device__ __forceinline__ void stage_pixel(const RasterParams& P, const TriRecs
 
 // Stores a staged tile (CLEAR: the clear values, no staging needed) with
 // 16-byte stores; partial columns at the right edge fall back to scalars.
+// visDepth, visSlot and visClass point at the view's outputs; a view's first
+// pixel starts a 16-byte line when S > 1 or its pixel base is a multiple of 4.
 template <int S, bool CLEAR>
-__device__ __forceinline__ void store_tile(const RasterParams& P, int tx, int ty, const TileOut<S>* O,
+__device__ __forceinline__ void store_tile(const ViewGeom& G, int tx, int ty, const TileOut<S>* O,
                                            float* visDepth, uint32_t* visSlot, uint8_t* visClass) {
     constexpr int perRow = 16 * S / 4;  // float4 per tile row
-    const int cols = min(16, P.width - tx * 16), rows = min(16, P.height - ty * 16);
-    const bool vec = cols == 16 && (S > 1 || (P.width & 3) == 0);
+    const int cols = min(16, G.width - tx * 16), rows = min(16, G.height - ty * 16);
+    const bool vec = cols == 16 && (S > 1 || ((G.width & 3) == 0 && (G.pixBase & 3) == 0));
     for (int i = threadIdx.x; i < 64 * S; i += 256) {
         const int row = i / perRow, q = i - row * perRow;
         if (row >= rows) continue;
-        const size_t g = ((size_t)(ty * 16 + row) * P.width + tx * 16) * S + q * 4;
+        const size_t g = ((size_t)(ty * 16 + row) * G.width + tx * 16) * S + q * 4;
         const int l = row * 16 * S + q * 4;
         float4 d;
         uint4 sl;
@@ -329,19 +353,20 @@ __device__ __forceinline__ void store_tile(const RasterParams& P, int tx, int ty
     }
     if (CLEAR && S == 1 && visClass) {
         const int row = threadIdx.x >> 4, col = threadIdx.x & 15;
-        if (row < rows && col < cols) visClass[(size_t)(ty * 16 + row) * P.width + tx * 16 + col] = 0;
+        if (row < rows && col < cols) visClass[(size_t)(ty * 16 + row) * G.width + tx * 16 + col] = 0;
     }
 }
 
 template <int S>
-__global__ void __launch_bounds__(256) k_raster(RasterParams P, TriRecs recs, const unsigned* __restrict__ slotOf,
+__global__ void __launch_bounds__(256) k_raster(RasterParams P, Views V, TriRecs recs,
+                                                const unsigned* __restrict__ slotOfAll,
                                                 const unsigned* __restrict__ binStart,
                                                 const unsigned* __restrict__ binCount,
                                                 const unsigned* __restrict__ binList, const uint2* __restrict__ items,
-                                                unsigned itemCap, const int* __restrict__ tilePartial,
+                                                unsigned itemCap, int partShift, const int* __restrict__ tilePartial,
                                                 unsigned long long* __restrict__ partials,
-                                                float* __restrict__ visDepth, uint32_t* __restrict__ visSlot,
-                                                uint8_t* __restrict__ visClass, StatPartials st,
+                                                float* __restrict__ visDepthAll, uint32_t* __restrict__ visSlotAll,
+                                                uint8_t* __restrict__ visClassAll, StatPartials st,
                                                 DeviceCounters* ctr) {
     constexpr int NW = S == 1 ? 1 : 4;  // mask words of one half tile
     __shared__ unsigned long long sVis[S * 256];  // key of sample s of tile pixel p at s * 256 + p
@@ -363,13 +388,18 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P, TriRecs recs, co
         if (item >= numItems) break;
         // long items from the front of the array, then short ones from its back
         const uint2 it = items[item < numLong ? item : itemCap - 1u - (item - numLong)];
-        const int tile = (int)it.x, part = (int)it.y;
-        const PixelCtx C = pixel_ctx(P, tile, t & 15, t >> 4);
+        const int tile = __builtin_amdgcn_readfirstlane((int)it.x), part = __builtin_amdgcn_readfirstlane((int)it.y);
+        const ViewGeom G = view_of_tile(V, tile);
+        float* const visDepth = visDepthAll + G.pixBase * S;
+        uint32_t* const visSlot = visSlotAll + G.pixBase * S;
+        uint8_t* const visClass = visClassAll ? visClassAll + G.pixBase : nullptr;
+        const unsigned* const slotOf = slotOfAll + G.primBase;
+        const PixelCtx C = pixel_ctx(G, t & 15, t >> 4);
         const int tx0 = C.tx * 16, ty0 = C.ty * 16;
         const unsigned start = binStart[tile];
         const unsigned total = start >= P.binCap ? 0u : min(binCount[tile], P.binCap - start);
-        const unsigned base = (unsigned)part * kPart;
-        const int cnt = (int)min((unsigned)kPart, total > base ? total - base : 0u);
+        const unsigned base = (unsigned)part << partShift;
+        const int cnt = (int)min(1u << partShift, total > base ? total - base : 0u);
         const int pbase = tilePartial[tile];
 
 #pragma unroll
@@ -379,7 +409,7 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P, TriRecs recs, co
         // two lanes per list entry: lane h = t & 1 takes block rows 2h, 2h + 1
         const int e0 = t >> 1, h = t & 1;
         // framebuffer mask of this half tile in the candidate layout
-        const int cols = min(16, P.width - tx0), rows = min(16, P.height - ty0);
+        const int cols = min(16, G.width - tx0), rows = min(16, G.height - ty0);
         unsigned fbMask[NW];
         if constexpr (S == 1) {
             // groups with at least one pixel inside; pixels are checked again per unit
@@ -468,7 +498,7 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P, TriRecs recs, co
                         for (int l = 0; l < 8; l++) {
                             if (!((acc >> l) & 1u)) continue;
                             const int px = px0 + (l & 1) + ((l >> 2) << 1), py = gy + ((l >> 1) & 1);
-                            if (px >= P.width || py >= P.height) continue;
+                            if (px >= G.width || py >= G.height) continue;
                             const unsigned quad = (acc >> (l & 4)) & 0xFu;
                             const unsigned cls = acc == 0xFFu ? 1u : (quad == 0xFu ? 2u : 3u);
                             frags++;
@@ -513,7 +543,7 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P, TriRecs recs, co
                 __syncthreads();
                 stage_pixel<S>(P, recs, slotOf, C, t, keys, *O, visClass);
                 __syncthreads();
-                store_tile<S, false>(P, C.tx, C.ty, O, visDepth, visSlot, visClass);
+                store_tile<S, false>(G, C.tx, C.ty, O, visDepth, visSlot, visClass);
             } else {
                 write_pixel<S>(P, recs, slotOf, C, keys, visDepth, visSlot, visClass);
             }
@@ -530,30 +560,36 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P, TriRecs recs, co
 // Minimum over the parts of every tile whose list was cut, then the outputs;
 // also the clear values of tiles with an empty list (when clearing).
 template <int S>
-__global__ void __launch_bounds__(256) k_raster_merge(RasterParams P, TriRecs recs, const unsigned* __restrict__ slotOf,
+__global__ void __launch_bounds__(256) k_raster_merge(RasterParams P, Views V, TriRecs recs,
+                                                      const unsigned* __restrict__ slotOfAll,
                                                       const unsigned* __restrict__ binStart,
                                                       const unsigned* __restrict__ binCount,
                                                       const unsigned* __restrict__ multiList,
                                                       const int* __restrict__ tilePartial,
                                                       const unsigned long long* __restrict__ partials,
-                                                      float* __restrict__ visDepth, uint32_t* __restrict__ visSlot,
-                                                      uint8_t* __restrict__ visClass, int tiles,
+                                                      float* __restrict__ visDepthAll, uint32_t* __restrict__ visSlotAll,
+                                                      uint8_t* __restrict__ visClassAll, int tiles, int partShift,
                                                       const DeviceCounters* ctr) {
     __shared__ TileOut<S> O;
     const unsigned numMulti = ctr->numMulti;
     const unsigned numWork = numMulti + (P.clear ? ctr->numEmpty : 0u);
     const int t = threadIdx.x;
     for (unsigned m = blockIdx.x; m < numWork; m += gridDim.x) {
-        if (m >= numMulti) {
-            const int tile = (int)multiList[tiles + (m - numMulti)];
-            store_tile<S, true>(P, tile % P.tilesX, tile / P.tilesX, nullptr, visDepth, visSlot, visClass);
+        const bool empty = m >= numMulti;
+        const int tile = (int)multiList[empty ? tiles + (m - numMulti) : m];
+        const ViewGeom G = view_of_tile(V, tile);
+        float* const visDepth = visDepthAll + G.pixBase * S;
+        uint32_t* const visSlot = visSlotAll + G.pixBase * S;
+        uint8_t* const visClass = visClassAll ? visClassAll + G.pixBase : nullptr;
+        if (empty) {
+            store_tile<S, true>(G, G.tile % G.tilesX, G.tile / G.tilesX, nullptr, visDepth, visSlot, visClass);
             continue;
         }
-        const int tile = (int)multiList[m];
-        const PixelCtx C = pixel_ctx(P, tile, t & 15, t >> 4);
+        const unsigned* const slotOf = slotOfAll + G.primBase;
+        const PixelCtx C = pixel_ctx(G, t & 15, t >> 4);
         const unsigned start = binStart[tile];
         const unsigned total = start >= P.binCap ? 0u : min(binCount[tile], P.binCap - start);
-        const unsigned nparts = (total + kPart - 1) / kPart;
+        const unsigned nparts = (total + (1u << partShift) - 1u) >> partShift;
         const unsigned long long* g = partials + (size_t)tilePartial[tile] * 256 * S + t;
         unsigned long long keys[S];
 #pragma unroll
@@ -567,7 +603,7 @@ __global__ void __launch_bounds__(256) k_raster_merge(RasterParams P, TriRecs re
             __syncthreads();  // the previous tile's staging has been stored
             stage_pixel<S>(P, recs, slotOf, C, t, keys, O, visClass);
             __syncthreads();
-            store_tile<S, false>(P, C.tx, C.ty, &O, visDepth, visSlot, visClass);
+            store_tile<S, false>(G, C.tx, C.ty, &O, visDepth, visSlot, visClass);
         } else {
             write_pixel<S>(P, recs, slotOf, C, keys, visDepth, visSlot, visClass);
         }
@@ -583,19 +619,46 @@ __global__ void __launch_bounds__(256) k_slot_to_prim(const uint32_t* __restrict
     }
 }
 
+// Order-independent digest: the sum over i of SplitMix64(i << 32 ^ data[i])
+// (the finaliser of Steele, Lea and Flood, 2014), reduced per workgroup and
+// added into *out.
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void __launch_bounds__(256) k_digest(const uint32_t* __restrict__ data, size_t n,
+                                                unsigned long long* out) {
+    unsigned long long acc = 0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        acc += splitmix64(((unsigned long long)i << 32) ^ data[i]);
+    acc = block_sum256(acc);
+    if (threadIdx.x == 0) atomicAdd(out, acc);
+}
+
 }  // namespace
+
+void launch_digest(const uint32_t* data, size_t n, unsigned long long* out, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_digest, dim3((unsigned)std::min<size_t>(2048, (n + 255) / 256)), dim3(256), 0, s, data, n,
+                       out);
+}
 
 void launch_slot_to_prim(const uint32_t* visSlot, TriRecs recs, size_t n, uint32_t* visId, hipStream_t s) {
     hipLaunchKernelGGL(k_slot_to_prim, dim3(2048), dim3(256), 0, s, visSlot, recs.q2, n, visId);
 }
 
 // The raster and merge launches are separate so that the host can time them.
-void launch_raster(const RasterParams& p, int samples, TriRecs recs, const unsigned* slotOf, const unsigned* binStart,
-                   const unsigned* binCount, const unsigned* binList, const RasterQueue& q, float* visDepth,
-                   uint32_t* visSlot, uint8_t* visClass, StatPartials st, DeviceCounters* ctr, hipStream_t s) {
-#define IFRIT_RASTER_LAUNCH(SN)                                                                                     \
-    hipLaunchKernelGGL(k_raster<SN>, dim3(q.grid), dim3(256), 0, s, p, recs, slotOf, binStart, binCount, binList, \
-                       q.items, q.itemCap, q.tilePartial, q.partials, visDepth, visSlot, visClass, st, ctr)
+void launch_raster(const RasterParams& p, const Views& views, int samples, TriRecs recs, const unsigned* slotOf,
+                   const unsigned* binStart, const unsigned* binCount, const unsigned* binList, const RasterQueue& q,
+                   float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st, DeviceCounters* ctr,
+                   hipStream_t s) {
+#define IFRIT_RASTER_LAUNCH(SN)                                                                                    \
+    hipLaunchKernelGGL(k_raster<SN>, dim3(q.grid), dim3(256), 0, s, p, views, recs, slotOf, binStart, binCount,  \
+                       binList, q.items, q.itemCap, q.partShift, q.tilePartial, q.partials, visDepth, visSlot,     \
+                       visClass, st, ctr)
     switch (samples) {
         case 1: IFRIT_RASTER_LAUNCH(1); break;
         case 4: IFRIT_RASTER_LAUNCH(4); break;
@@ -604,13 +667,13 @@ void launch_raster(const RasterParams& p, int samples, TriRecs recs, const unsig
 #undef IFRIT_RASTER_LAUNCH
 }
 
-void launch_raster_merge(const RasterParams& p, int samples, TriRecs recs, const unsigned* slotOf,
+void launch_raster_merge(const RasterParams& p, const Views& views, int samples, TriRecs recs, const unsigned* slotOf,
                          const unsigned* binStart, const unsigned* binCount, const RasterQueue& q, float* visDepth,
-                         uint32_t* visSlot, uint8_t* visClass, DeviceCounters* ctr, hipStream_t s) {
-    const int tiles = p.tilesX * p.tilesY;
-#define IFRIT_MERGE_LAUNCH(SN)                                                                                      \
-    hipLaunchKernelGGL(k_raster_merge<SN>, dim3(q.grid), dim3(256), 0, s, p, recs, slotOf, binStart, binCount,      \
-                       q.multiList, q.tilePartial, q.partials, visDepth, visSlot, visClass, tiles, ctr)
+                         uint32_t* visSlot, uint8_t* visClass, int tiles, DeviceCounters* ctr, hipStream_t s) {
+#define IFRIT_MERGE_LAUNCH(SN)                                                                                     \
+    hipLaunchKernelGGL(k_raster_merge<SN>, dim3(q.grid), dim3(256), 0, s, p, views, recs, slotOf, binStart,        \
+                       binCount, q.multiList, q.tilePartial, q.partials, visDepth, visSlot, visClass, tiles,       \
+                       q.partShift, ctr)
     switch (samples) {
         case 1: IFRIT_MERGE_LAUNCH(1); break;
         case 4: IFRIT_MERGE_LAUNCH(4); break;

@@ -34,13 +34,14 @@ struct ShadeTri {
     int prim;
 };
 
-__device__ __forceinline__ void load_shade_tri(const TriRecs& recs, const float4* bary, int ntri, unsigned slot,
-                                               ShadeTri& T) {
+// Slots below fanStart (the view's first clip fan slot) are unclipped triangles.
+__device__ __forceinline__ void load_shade_tri(const TriRecs& recs, const float4* bary, unsigned fanStart,
+                                               unsigned slot, ShadeTri& T) {
     T.f1 = recs.f1[slot];
     T.f2 = recs.f2[slot];
     T.f3 = recs.f3[slot];
     T.prim = (int)(__float_as_uint(recs.q2[slot].y) >> 2);
-    if (slot < (unsigned)ntri) {
+    if (slot < fanStart) {
         // unclipped triangles keep bx = (0, 0, 1), by = (1, 0, 0) (TileRasterWorker.cpp:711-712)
         T.bx[0] = 0.0f; T.bx[1] = 0.0f; T.bx[2] = 1.0f;
         T.by[0] = 1.0f; T.by[1] = 0.0f; T.by[2] = 0.0f;
@@ -175,8 +176,8 @@ __device__ __forceinline__ float3 ggx_light(float3 n, float3 V, float3 L, float4
     return make_float3((alb.x * diff + spec) * ndl, (alb.y * diff + spec) * ndl, (alb.z * diff + spec) * ndl);
 }
 
-__device__ FragPrep prep_fragment(const ShadeParams& P, const ShadeTri& T, const float* vtx, const int* idx,
-                                  const TexDesc* tex, int px, int py, float o) {
+__device__ FragPrep prep_fragment(const ShadeParams& P, float3 cam, const ShadeTri& T, const float* vtx,
+                                  const int* idx, const TexDesc* tex, int px, int py, float o) {
     FragPrep R;
     R.mat = -1;
     R.u = R.v = 0.0f;
@@ -207,7 +208,7 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, const ShadeTri& T, const
         R.Tg = Tg;
         R.Bg = make_float3(N.y * Tg.z - N.z * Tg.y, N.z * Tg.x - N.x * Tg.z, N.x * Tg.y - N.y * Tg.x);
         R.Pw = Pw;
-        R.V = norm3(make_float3(P.camera[0] - Pw.x, P.camera[1] - Pw.y, P.camera[2] - Pw.z));
+        R.V = norm3(make_float3(cam.x - Pw.x, cam.y - Pw.y, cam.z - Pw.z));
         R.A = R.B = make_float3(0, 0, 0);
     }
     if (P.numTextures > 0) {
@@ -235,7 +236,7 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, const ShadeTri& T, const
     }
     if (P.program >= 3) return R;  // lighting needs the sampled layers (or the network)
     N = norm3(N);
-    const float3 V = norm3(make_float3(P.camera[0] - Pw.x, P.camera[1] - Pw.y, P.camera[2] - Pw.z));
+    const float3 V = norm3(make_float3(cam.x - Pw.x, cam.y - Pw.y, cam.z - Pw.z));
     float3 A = make_float3(P.ambient, P.ambient, P.ambient), B = make_float3(0, 0, 0);
     for (int l = 0; l < P.numLights; l++) {
         float3 L;
@@ -328,7 +329,7 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 
 constexpr int kTapWindow = 128;
 
-// One wave per 8x8 pixel tile.  Work is expanded twice so that lanes stay
+// One wave per 8x8 pixel sub-tile.  Work is expanded twice so that lanes stay
 // busy whatever the mix of edge pixels and sampling footprints:
 //   1. each pixel lists its distinct triangles (first-occurrence order, with
 //      their sample counts) into one flat fragment list;
@@ -336,12 +337,15 @@ constexpr int kTapWindow = 128;
 //      and its texture taps go into one flat tap list, one tap per lane;
 //   3. the fragment lane sums its taps in tap order, the pixel lane sums its
 //      fragments in sample order, so results do not depend on scheduling.
+// (ox, oy) is the sub-tile's first pixel in its view; visSlot, rgba8 and
+// rgba32f point at the view's outputs and pixBase is the view's first pixel.
 template <int S, bool NEURAL, bool FP8>
-__device__ __forceinline__ void shade_tile(const ShadeParams& P, const TriRecs& recs, const float4* __restrict__ bary,
-                                           const uint32_t* __restrict__ visSlot, const float* __restrict__ vtx,
-                                           const int* __restrict__ idx, const TexDesc* __restrict__ tex,
-                                           const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8,
-                                           float4* __restrict__ rgba32f, int tilesX, int tiles, int perXcd) {
+__device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, int width, int height,
+                                              unsigned fanStart, int ox, int oy, size_t pixBase, const TriRecs& recs,
+                                              const float4* __restrict__ bary, const uint32_t* __restrict__ visSlot,
+                                              const float* __restrict__ vtx, const int* __restrict__ idx,
+                                              const TexDesc* __restrict__ tex, const uint8_t* __restrict__ pool,
+                                              uchar4* __restrict__ rgba8, float4* __restrict__ rgba32f) {
     __shared__ unsigned fSlot[64 * S];
     __shared__ unsigned char fCnt[64 * S];
     __shared__ unsigned char fPix[64 * S];
@@ -358,14 +362,11 @@ __device__ __forceinline__ void shade_tile(const ShadeParams& P, const TriRecs& 
     float4* const tCol = tBuf;
     float4* const tCol2 = tBuf + kTapWindow;
 
-    // consecutive tiles on one XCD share its L2 (texture and vertex reuse)
-    const int tile = (int)(blockIdx.x & 7u) * perXcd + (int)(blockIdx.x >> 3);
-    if (tile >= tiles) return;
     const int lane = threadIdx.x;
-    const int px = (tile % tilesX) * 8 + (lane & 7);
-    const int py = (tile / tilesX) * 8 + (lane >> 3);
-    const bool inFb = px < P.width && py < P.height;
-    const size_t pix = (size_t)py * P.width + px;
+    const int px = ox + (lane & 7);
+    const int py = oy + (lane >> 3);
+    const bool inFb = px < width && py < height;
+    const size_t pix = (size_t)py * width + px;
 
     // 1. distinct triangles of the pixel
     unsigned ids[S];
@@ -418,7 +419,7 @@ __device__ __forceinline__ void shade_tile(const ShadeParams& P, const TriRecs& 
     }
     __syncthreads();
 
-    if (P.program == 5 && S == 1 && inFb && nf == 0) P.neural.featOut[4 * pix + 1] = make_float4(0, 0, 0, 0);
+    if (P.program == 5 && S == 1 && inFb && nf == 0) P.neural.featOut[4 * (pixBase + pix) + 1] = make_float4(0, 0, 0, 0);
     const float o = S == 1 ? 0.0f : 0.5f;
     float4 sum = make_float4(0, 0, 0, 0);
     for (int c0 = 0; c0 < F; c0 += 64) {
@@ -431,9 +432,9 @@ __device__ __forceinline__ void shade_tile(const ShadeParams& P, const TriRecs& 
             const unsigned slot = fSlot[fi];
             const int fl = fPix[fi];
             ShadeTri T;
-            load_shade_tri(recs, bary, P.ntri, slot, T);
-            const int qx = (tile % tilesX) * 8 + (fl & 7), qy = (tile / tilesX) * 8 + (fl >> 3);
-            R = prep_fragment(P, T, vtx, idx, tex, qx, qy, o);
+            load_shade_tri(recs, bary, fanStart, slot, T);
+            const int qx = ox + (fl & 7), qy = oy + (fl >> 3);
+            R = prep_fragment(P, cam, T, vtx, idx, tex, qx, qy, o);
             if (R.mat >= 0) {
                 n = R.fp.n;
                 cUV[lane] = make_float4(R.u, R.v, R.fp.au, R.fp.av);
@@ -509,7 +510,7 @@ __device__ __forceinline__ void shade_tile(const ShadeParams& P, const TriRecs& 
                 if (P.program == 5 && S == 1 && R.mat >= 0) {
                     // training record of this pixel: latent footprint, view and light in the frame, target
                     const int fl = fPix[fi];
-                    const size_t fpix = (size_t)((tile / tilesX) * 8 + (fl >> 3)) * P.width + (tile % tilesX) * 8 + (fl & 7);
+                    const size_t fpix = pixBase + (size_t)(oy + (fl >> 3)) * width + ox + (fl & 7);
                     float3 vt, lt;
                     frame_dirs(P, R, vt, lt);
                     const float4 cc = cCol[lane];
@@ -578,35 +579,63 @@ __device__ __forceinline__ void shade_tile(const ShadeParams& P, const TriRecs& 
     if (rgba32f) rgba32f[pix] = r;
 }
 
+// One wave per shading tile of P.shadeTile pixels (8, 16 or 32), its 8x8
+// sub-tiles in row order; consecutive tiles on one XCD share its L2 (texture
+// and vertex reuse).  Tiles of all shaded views are numbered one after another.
+template <int S, bool NEURAL, bool FP8>
+__device__ __forceinline__ void shade_tiles(const ShadeParams& P, const Views& V, const TriRecs& recs,
+                                            const float4* __restrict__ bary, const uint32_t* __restrict__ visSlot,
+                                            const float* __restrict__ vtx, const int* __restrict__ idx,
+                                            const TexDesc* __restrict__ tex, const uint8_t* __restrict__ pool,
+                                            uchar4* __restrict__ rgba8, float4* __restrict__ rgba32f, int tiles,
+                                            int perXcd) {
+    const int tile = (int)(blockIdx.x & 7u) * perXcd + (int)(blockIdx.x >> 3);
+    if (tile >= tiles) return;
+    const ViewDesc& D = V.desc[V.shadeView[tile]];
+    const int T = P.shadeTile, n = T >> 3;
+    const int width = D.width, height = D.height;
+    const int local = tile - (int)D.shadeBase, perRow = (width + T - 1) / T;
+    const int ox0 = (local % perRow) * T, oy0 = (local / perRow) * T;
+    const size_t pb = (size_t)D.pixBase;
+    const float3 cam = make_float3(D.camera[0], D.camera[1], D.camera[2]);
+    const unsigned fanStart = D.slotBase + (unsigned)P.ntri;
+    for (int k = 0; k < n * n; k++) {
+        const int ox = ox0 + (k % n) * 8, oy = oy0 + (k / n) * 8;
+        if (ox >= width || oy >= height) continue;
+        shade_subtile<S, NEURAL, FP8>(P, cam, width, height, fanStart, ox, oy, pb, recs, bary, visSlot + pb * S, vtx,
+                                      idx, tex, pool, rgba8 + pb, rgba32f ? rgba32f + pb : nullptr);
+    }
+}
+
 template <int S>
-__global__ void __launch_bounds__(64) k_shade(ShadeParams P, TriRecs recs, const float4* __restrict__ bary,
+__global__ void __launch_bounds__(64) k_shade(ShadeParams P, Views V, TriRecs recs, const float4* __restrict__ bary,
                                               const uint32_t* __restrict__ visSlot, const float* __restrict__ vtx,
                                               const int* __restrict__ idx, const TexDesc* __restrict__ tex,
                                               const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8,
-                                              float4* __restrict__ rgba32f, int tilesX, int tiles, int perXcd) {
-    shade_tile<S, false, false>(P, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tilesX, tiles, perXcd);
+                                              float4* __restrict__ rgba32f, int tiles, int perXcd) {
+    shade_tiles<S, false, false>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles, perXcd);
 }
 
 // The neural material, held to four waves per SIMD like the other programs so
 // that the latent fetches stay overlapped.
 template <int S, bool FP8>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
-k_shade_neural(ShadeParams P, TriRecs recs, const float4* __restrict__ bary, const uint32_t* __restrict__ visSlot,
-               const float* __restrict__ vtx, const int* __restrict__ idx, const TexDesc* __restrict__ tex,
-               const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8, float4* __restrict__ rgba32f,
-               int tilesX, int tiles, int perXcd) {
-    shade_tile<S, true, FP8>(P, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tilesX, tiles, perXcd);
+k_shade_neural(ShadeParams P, Views V, TriRecs recs, const float4* __restrict__ bary,
+               const uint32_t* __restrict__ visSlot, const float* __restrict__ vtx, const int* __restrict__ idx,
+               const TexDesc* __restrict__ tex, const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8,
+               float4* __restrict__ rgba32f, int tiles, int perXcd) {
+    shade_tiles<S, true, FP8>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles, perXcd);
 }
 
 }  // namespace
 
-void launch_shade(const ShadeParams& p, TriRecs recs, const float4* bary, const uint32_t* visSlot,
-                  const float* vtx, const int* idx, const TexDesc* tex, const uint8_t* texPool, uchar4* rgba8,
-                  float4* rgba32f, hipStream_t s) {
-    const int tilesX = (int)dev::blocks(p.width, 8), tiles = tilesX * (int)dev::blocks(p.height, 8);
-    const int perXcd = (tiles + 7) / 8;
+void launch_shade(const ShadeParams& p, const Views& views, int shadeTiles, TriRecs recs, const float4* bary,
+                  const uint32_t* visSlot, const float* vtx, const int* idx, const TexDesc* tex,
+                  const uint8_t* texPool, uchar4* rgba8, float4* rgba32f, hipStream_t s) {
+    if (shadeTiles <= 0) return;
+    const int perXcd = (shadeTiles + 7) / 8;
     const dim3 grid(8 * perXcd);
-#define IFRIT_ARGS p, recs, bary, visSlot, vtx, idx, tex, texPool, rgba8, rgba32f, tilesX, tiles, perXcd
+#define IFRIT_ARGS p, views, recs, bary, visSlot, vtx, idx, tex, texPool, rgba8, rgba32f, shadeTiles, perXcd
 #define IFRIT_SHADE(SN)                                                                                             \
     do {                                                                                                            \
         if (p.program != 4)                                                                                         \

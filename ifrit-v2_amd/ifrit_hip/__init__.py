@@ -61,6 +61,30 @@ class FrameStats(C.Structure):
         return d
 
 
+class ViewDesc(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("mvp", C.c_float * 16), ("camera", C.c_float * 3),
+                ("shade", C.c_int32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("rasterPartEntries", C.c_int32), ("rasterGroupsPerCU", C.c_int32), ("shadeTile", C.c_int32),
+                ("bigTriangleTiles", C.c_int32), ("binCapacity", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class AutotuneReport(C.Structure):
+    _fields_ = [("candidates", C.c_int32), ("identical", C.c_int32), ("defaultMs", C.c_float), ("bestMs", C.c_float),
+                ("binDemand", C.c_uint32), ("best", Tuning), ("tried", Tuning * 64), ("ms", C.c_float * 64)]
+
+    def as_dict(self):
+        n = min(self.candidates, 64)
+        return {"candidates": self.candidates, "identical": bool(self.identical), "default_ms": self.defaultMs,
+                "best_ms": self.bestMs, "bin_demand": self.binDemand, "best": self.best.as_dict(),
+                "tried": [dict(self.tried[i].as_dict(), ms=self.ms[i]) for i in range(n)]}
+
+
 # Every symbol declared in include/ifrit_hip.h.
 EXPORTED_SYMBOLS = [
     "iftrCreateInstance", "iftrDestroyInstance", "iftrBindFrameBuffer", "iftrBindVertexBuffer", "iftrBindIndexBuffer",
@@ -78,6 +102,8 @@ EXPORTED_SYMBOLS = [
     "iftrDownloadTexture", "iftrSetRenderTarget", "iftrGetVisibility", "iftrGetCoverageClass", "iftrGetColorRGBA8",
     "iftrGetFrameStats", "iftrSetStageTiming", "iftrDrawAsync", "iftrSynchronize", "iftrGetLastError",
     "iftrNeuralTrain", "iftrNeuralSetParams", "iftrNeuralGetParams", "iftrNeuralEval", "iftrNeuralValidate",
+    "iftrSetViews", "iftrGetViewVisibility", "iftrGetViewColorRGBA8", "iftrSetGraphMode", "iftrSetTuning",
+    "iftrGetTuning", "iftrAutotune", "iftrGetDigest",
 ]
 
 
@@ -124,6 +150,11 @@ def lib():
         "iftrNeuralSetParams": ([vp, vp], None), "iftrNeuralGetParams": ([vp, vp], None),
         "iftrNeuralEval": ([vp, vp, vp, C.c_int, C.c_int], None),
         "iftrNeuralValidate": ([vp, C.c_int, C.c_float, C.c_int], C.c_float),
+        "iftrSetViews": ([vp, vp, C.c_int], None), "iftrGetViewVisibility": ([vp, C.c_int, vp, vp], None),
+        "iftrGetViewColorRGBA8": ([vp, C.c_int, vp], None), "iftrSetGraphMode": ([vp, C.c_int], None),
+        "iftrSetTuning": ([vp, C.POINTER(Tuning)], None), "iftrGetTuning": ([vp, C.POINTER(Tuning)], None),
+        "iftrAutotune": ([vp, C.c_int, C.c_int, C.POINTER(AutotuneReport)], None),
+        "iftrGetDigest": ([vp], C.c_uint64),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -319,3 +350,62 @@ class Renderer:
         self.L.iftrGetColorRGBA8(self.h, _p(out))
         self._check()
         return out
+
+    # multi-view batches
+    def set_views(self, views):
+        """views: sequence of dicts with width, height, mvp (4x4), camera (3) and shade (bool); an
+        empty sequence returns to the single target."""
+        arr = (ViewDesc * max(1, len(views)))()
+        for i, v in enumerate(views):
+            arr[i].width = int(v["width"])
+            arr[i].height = int(v["height"])
+            arr[i].mvp[:] = [float(x) for x in np.asarray(v["mvp"], np.float32).reshape(-1)]
+            arr[i].camera[:] = [float(x) for x in np.asarray(v.get("camera", (0, 0, 0)), np.float32)]
+            arr[i].shade = 1 if v.get("shade", True) else 0
+        self.L.iftrSetViews(self.h, C.cast(arr, C.c_void_p), len(views))
+        self._check()
+        self.views = [dict(v) for v in views]
+
+    def view_visibility(self, view: int):
+        v = self.views[view]
+        n = int(v["width"]) * int(v["height"]) * self.samples
+        ids = np.empty((n,), np.uint32)
+        depth = np.empty((n,), np.float32)
+        self.L.iftrGetViewVisibility(self.h, view, _p(ids), _p(depth))
+        self._check()
+        return ids, depth
+
+    def view_color(self, view: int):
+        v = self.views[view]
+        out = np.empty((int(v["height"]), int(v["width"]), 4), np.uint8)
+        self.L.iftrGetViewColorRGBA8(self.h, view, _p(out))
+        self._check()
+        return out
+
+    def set_graph_mode(self, on: bool):
+        self.L.iftrSetGraphMode(self.h, 1 if on else 0)
+        self._check()
+
+    def set_tuning(self, **kw):
+        t = Tuning()
+        self.L.iftrGetTuning(self.h, C.byref(t))
+        for k, v in kw.items():
+            setattr(t, k, v)
+        self.L.iftrSetTuning(self.h, C.byref(t))
+        self._check()
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        self.L.iftrGetTuning(self.h, C.byref(t))
+        return t.as_dict()
+
+    def autotune(self, frames: int = 3) -> dict:
+        rep = AutotuneReport()
+        self.L.iftrAutotune(self.h, self.num_indices, frames, C.byref(rep))
+        self._check()
+        return rep.as_dict()
+
+    def digest(self) -> int:
+        d = self.L.iftrGetDigest(self.h)
+        self._check()
+        return int(d)

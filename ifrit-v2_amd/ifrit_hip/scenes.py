@@ -58,6 +58,19 @@ def look_at(eye, center, up) -> np.ndarray:
     return r @ t
 
 
+def orthographic(half_w: float, half_h: float, znear: float, zfar: float) -> np.ndarray:
+    """Orthographic projection in the conventions of `perspective`: left-handed,
+    x, y in [-1, 1] over [-half_w, half_w] x [-half_h, half_h], z in [0, 1] over
+    [znear, zfar], w = 1."""
+    m = np.zeros((4, 4), dtype=np.float64)
+    m[0, 0] = 1.0 / half_w
+    m[1, 1] = 1.0 / half_h
+    m[2, 2] = 1.0 / (zfar - znear)
+    m[2, 3] = -znear / (zfar - znear)
+    m[3, 3] = 1.0
+    return m
+
+
 @dataclasses.dataclass
 class Texture:
     width: int
@@ -313,6 +326,51 @@ def config2(width: int = 1920, height: int = 1080, samples: int = 8, spheres: in
     return Scene("config2", width, height, samples, verts, idx, _mvp(eye, (0, 0, 0), 60.0, width, height),
                  program=PROGRAM_BLINN_PHONG, sampler=SAMPLER_ANISO, max_aniso=16, tris_per_material=nt,
                  camera=eye, lights=lights, textures=textures, ambient=0.08, kd=0.9, ks=0.25)
+
+
+def config4_views(seed: int = 20241204, cascades: int = 4, cascade_size: int = 4096, cube_faces: int = 6,
+                  cube_size: int = 1024, cameras: int = 54, max_size=(1920, 1080), min_size: int = 256):
+    """The views of config 4 over the config-2 scene (sphere centres in [-20, 20]^3):
+    - shadow cascades, depth only: orthographic views along a seeded light direction
+      towards the origin, extents 6, 12, 24, 48 (each twice the previous), square;
+    - cube faces, depth only: 90-degree views from the origin along +x, -x, +y, -y,
+      +z, -z (the origin may lie inside a sphere: triangles cross w = 0);
+    - seeded shaded cameras: eye on a shell of radius U[25, 45], look-at point in
+      [-6, 6]^3, vertical field of view U[40, 75] degrees, width U[min, max_w],
+      height U[min, max_h].
+    Returns dicts for Renderer.set_views."""
+    rng = np.random.default_rng(seed)
+    views = []
+    d = rng.normal(size=3)
+    d /= np.linalg.norm(d)
+    up = (0.0, 1.0, 0.0) if abs(d[1]) < 0.9 else (1.0, 0.0, 0.0)
+    eye = -60.0 * d
+    for c in range(cascades):
+        ext = 6.0 * 2.0 ** c
+        m = orthographic(ext, ext, 1.0, 120.0) @ look_at(eye, (0.0, 0.0, 0.0), up)
+        views.append(dict(width=cascade_size, height=cascade_size, mvp=m.astype(np.float32),
+                          camera=tuple(float(x) for x in eye), shade=False, kind="cascade"))
+    dirs = [((1, 0, 0), (0, 1, 0)), ((-1, 0, 0), (0, 1, 0)), ((0, 1, 0), (0, 0, -1)), ((0, -1, 0), (0, 0, 1)),
+            ((0, 0, 1), (0, 1, 0)), ((0, 0, -1), (0, 1, 0))]
+    for f in range(cube_faces):
+        fwd, upv = dirs[f % 6]
+        m = perspective(math.pi / 2, 1.0, 0.1, 100.0) @ look_at((0.0, 0.0, 0.0), fwd, upv)
+        views.append(dict(width=cube_size, height=cube_size, mvp=m.astype(np.float32), camera=(0.0, 0.0, 0.0),
+                          shade=False, kind="cube"))
+    for _ in range(cameras):
+        w = int(rng.integers(min_size, max_size[0] + 1))
+        h = int(rng.integers(min_size, max_size[1] + 1))
+        z = rng.uniform(-1, 1)
+        phi = rng.uniform(0, 2 * math.pi)
+        r = rng.uniform(25, 45)
+        rr = math.sqrt(max(0.0, 1 - z * z))
+        e = (r * rr * math.cos(phi), r * z, r * rr * math.sin(phi))
+        at = tuple(rng.uniform(-6, 6, 3))
+        fov = math.radians(rng.uniform(40, 75))
+        m = perspective(fov, w / h, 0.1, 100.0) @ look_at(e, at, (0, 1, 0))
+        views.append(dict(width=w, height=h, mvp=m.astype(np.float32), camera=tuple(float(x) for x in e),
+                          shade=True, kind="camera"))
+    return views
 
 
 def sphere_tangents(p: np.ndarray) -> np.ndarray:

@@ -247,6 +247,52 @@ float iftrNeuralValidate(IfritRendererHandle* h, int precision, float minPsnrDb,
 /* The MLP alone on n inputs ([n][16] floats, 14 used) -> [n][4] outputs (3 used). */
 void iftrNeuralEval(IfritRendererHandle* h, const float* in, float* out, int n, int precision);
 
+/* Multi-view batches: every draw renders all views of the batch from the bound vertex and index
+   data, each stage one launch over all views (the raster stage one persistent launch over the
+   tiles of every view).  The bound GPU vertex program supplies the kind of transform and the
+   bound fragment program the shading of views with shade = 1; views with shade = 0 produce
+   visibility and FP32 depth only.  The MSAA sample count applies to every view.  No
+   counterpart in the reference. */
+typedef struct {
+    int32_t width, height;            /* 1..16384 */
+    float mvp[16];                    /* row-major, clip = mvp * (x, y, z, 1) */
+    float camera[3];                  /* eye position for view-dependent shading */
+    int32_t shade;                    /* 1: run the bound fragment program, 0: depth only */
+} IfritViewDesc;
+/* numViews = 0 returns to the single target of iftrBindFrameBuffer / iftrSetRenderTarget. */
+void iftrSetViews(IfritRendererHandle* h, const IfritViewDesc* views, int numViews);
+void iftrGetViewVisibility(IfritRendererHandle* h, int view, uint32_t* primIds, float* depth); /* W*H*S each */
+void iftrGetViewColorRGBA8(IfritRendererHandle* h, int view, uint8_t* rgba);                  /* W*H*4 */
+/* 1: each draw configuration's launches are captured once in a hipGraph, later draws replay it. */
+void iftrSetGraphMode(IfritRendererHandle* h, int enable);
+
+/* Launch parameters that change speed but never results.  The 16-pixel coverage tile is fixed:
+   the reference's tile and block classification decides which depth expression a pixel uses. */
+typedef struct {
+    int32_t rasterPartEntries;        /* bin-list entries per raster work item: 256, 512 or 1024 */
+    int32_t rasterGroupsPerCU;        /* persistent raster workgroups per compute unit, 1..8 */
+    int32_t shadeTile;                /* shading tile side in pixels: 8, 16 or 32 */
+    int32_t bigTriangleTiles;         /* triangles over more tiles are binned by a whole workgroup */
+    uint32_t binCapacity;             /* bin-list entries reserved (grown on demand when exceeded) */
+} IfritTuning;
+void iftrSetTuning(IfritRendererHandle* h, const IfritTuning* tuning);
+void iftrGetTuning(IfritRendererHandle* h, IfritTuning* tuning);
+typedef struct {
+    int32_t candidates;               /* configurations timed */
+    int32_t identical;                /* 1: every configuration produced the same outputs */
+    float defaultMs, bestMs;          /* mean draw time with the starting and the chosen tuning */
+    uint32_t binDemand;               /* bin-list entries the draw needs */
+    IfritTuning best;                 /* chosen and applied */
+    IfritTuning tried[64];
+    float ms[64];
+} IfritAutotuneReport;
+/* Coordinate search over the tuning parameters, each configuration timed over `frames` graph
+   replays of the current draw (batch or single view); the bin capacity is then set to the
+   measured demand plus 1/8. */
+void iftrAutotune(IfritRendererHandle* h, int numIndices, int frames, IfritAutotuneReport* report);
+/* Order-independent 64-bit digest of the last draw's depth, primitive IDs and colour. */
+uint64_t iftrGetDigest(IfritRendererHandle* h);
+
 #ifdef __cplusplus
 }
 #endif

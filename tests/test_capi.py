@@ -145,3 +145,51 @@ def test_python_renderer_fails_loudly_without_device():
         r = Renderer()
         r.set_target(16, 16)
         r.draw()
+
+
+def test_batch_and_tuning_struct_layouts():
+    """The ctypes mirrors match the C layouts of include/ifrit_hip.h (compiled here with gcc)."""
+    import os
+    import tempfile
+
+    import ifrit_hip
+
+    src = r'''
+#include <stddef.h>
+#include <stdio.h>
+#include "ifrit_hip.h"
+int main(void) {
+    printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(IfritViewDesc), offsetof(IfritViewDesc, camera),
+           offsetof(IfritViewDesc, shade), sizeof(IfritTuning), sizeof(IfritAutotuneReport),
+           offsetof(IfritAutotuneReport, tried), offsetof(IfritAutotuneReport, ms));
+    return 0;
+}
+'''
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "layout.c")
+        exe = os.path.join(d, "layout")
+        with open(c, "w") as f:
+            f.write(src)
+        subprocess.run(["gcc", "-I", inc, c, "-o", exe], check=True)
+        got = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    want = [C.sizeof(ifrit_hip.ViewDesc), ifrit_hip.ViewDesc.camera.offset, ifrit_hip.ViewDesc.shade.offset,
+            C.sizeof(ifrit_hip.Tuning), C.sizeof(ifrit_hip.AutotuneReport), ifrit_hip.AutotuneReport.tried.offset,
+            ifrit_hip.AutotuneReport.ms.offset]
+    assert got == want
+
+
+def test_batch_calls_without_device_report_errors(lib):
+    import ifrit_hip
+
+    h = lib.iftrCreateInstance()
+    views = (ifrit_hip.ViewDesc * 2)()
+    views[0].width, views[0].height = 0, 10  # invalid size
+    lib.iftrSetViews(h, C.cast(views, C.c_void_p), 2)
+    assert lib.iftrGetLastError(h)
+    t = ifrit_hip.Tuning(300, 4, 8, 64, 0)  # 300 is not a permitted part size
+    lib.iftrSetTuning(h, C.byref(t))
+    assert lib.iftrGetLastError(h)
+    lib.iftrGetTuning(h, C.byref(t))
+    assert (t.rasterPartEntries, t.rasterGroupsPerCU, t.shadeTile) == (512, 4, 8)
+    lib.iftrDestroyInstance(h)

@@ -144,3 +144,35 @@ def test_flat_program_colours_by_primitive_hash():
         h = O.flat_hash(p)
         px = rgba[ids == p]
         assert px.size and np.all(px[:, 0] == (h & 255)) and np.all(px[:, 1] == ((h >> 8) & 255))
+
+
+def test_orthographic_maps_the_box_to_the_unit_cube():
+    m = scenes.orthographic(6.0, 3.0, 1.0, 121.0)
+    for p, want in (((6.0, -3.0, 1.0), (1.0, -1.0, 0.0)), ((-6.0, 3.0, 121.0), (-1.0, 1.0, 1.0)),
+                    ((0.0, 0.0, 61.0), (0.0, 0.0, 0.5))):
+        c = m @ np.array([*p, 1.0])
+        assert c[3] == 1.0
+        assert np.allclose(c[:3], want)
+
+
+def test_config4_views_shapes():
+    v = scenes.config4_views()
+    assert len(v) == 64
+    kinds = [x["kind"] for x in v]
+    assert kinds.count("cascade") == 4 and kinds.count("cube") == 6 and kinds.count("camera") == 54
+    assert all(x["width"] == x["height"] == 4096 and not x["shade"] for x in v[:4])
+    assert all(x["width"] == x["height"] == 1024 and not x["shade"] for x in v[4:10])
+    assert all(256 <= x["width"] <= 1920 and 256 <= x["height"] <= 1080 and x["shade"] for x in v[10:])
+    pixels = sum(x["width"] * x["height"] for x in v)
+    assert 100e6 < pixels < 130e6
+    # cube faces: the view direction maps to the image centre at w = distance
+    for x in v[4:10]:
+        m = np.asarray(x["mvp"], np.float64)
+        fwd = np.linalg.inv(m) @ np.array([0.0, 0.0, 0.5, 1.0])
+        d = fwd[:3] / fwd[3]
+        assert np.isclose(np.abs(d).max(), np.linalg.norm(d))  # along one axis
+    # the widest cascade holds the whole scene: sphere centres in [-20, 20]^3 plus radius 3
+    m = np.asarray(v[3]["mvp"], np.float64)
+    corners = np.array([[sx, sy, sz, 1.0] for sx in (-23, 23) for sy in (-23, 23) for sz in (-23, 23)])
+    c = corners @ m.T
+    assert np.all(np.abs(c[:, :2]) <= 1.0) and np.all((c[:, 2] >= 0) & (c[:, 2] <= 1))
