@@ -167,6 +167,71 @@ def neural_leg(args):
     return out
 
 
+def batch_leg(args):
+    """Config 4: 64 views of the config-2 scene in one draw at 1 sample per pixel: 4 shadow
+    cascades at 4096^2 and 6 cube faces at 1024^2 (depth only, FP32), 54 seeded cameras of
+    uneven size (shaded).  Every stage is one launch over all views and the draw is one
+    hipGraph replay.  The autotuner first picks the launch parameters (it checks that every
+    configuration produces the same digest of depth, IDs and colour)."""
+    import torch
+
+    from ifrit_hip import Renderer, scenes
+
+    scene = scenes.config2(spheres=args.spheres, samples=1)
+    views = scenes.config4_views()
+    r = Renderer()
+    r.load_scene(scene)
+    r.set_views(views)
+    r.set_graph_mode(True)
+    r.draw()
+    t0 = time.perf_counter()
+    tune = r.autotune(frames=3)
+    tune_s = time.perf_counter() - t0
+    for _ in range(2):
+        r.draw()
+    st = r.stats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        r.draw_async()
+    r.synchronize()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / args.batch_steps
+    if r.stats()["status"]:
+        raise SystemExit("batch incomplete during timing")
+    # per-stage device time (direct launches with events between the stages)
+    r.set_graph_mode(False)
+    r.stage_timing(True)
+    acc = [0.0] * len(STAGES)
+    for _ in range(3):
+        r.draw()
+        for k, v in enumerate(r.stats()["stageMs"][:len(STAGES)]):
+            acc[k] += v / 3
+    r.stage_timing(False)
+    pixels = sum(v["width"] * v["height"] for v in views)
+    shaded = sum(v["width"] * v["height"] for v in views if v["shade"])
+    V, T = scene.vertices.shape[0], scene.num_triangles
+    # compulsory bytes: vertices and indices once, depth + ID per pixel of every view, RGBA8 and one
+    # texel with its mips per shaded pixel
+    ab = V * 32 + T * 12 + pixels * 8 + shaded * 4 + shaded * 4 * 4 / 3
+    achieved = ab / (ms * 1e-3) / 1e9
+    r.close()
+    return {"workload": "config4: 64 views of the config-2 scene, 1 spp: 4 cascades 4096^2 + 6 cube faces "
+                        "1024^2 depth-only, 54 seeded cameras 256..1920 x 256..1080 shaded (Blinn-Phong, 16x aniso)",
+            "views": len(views), "pixels": pixels, "shaded_pixels": shaded, "steps": args.batch_steps,
+            "ms_per_batch": round(ms, 4), "gpixels_per_s": round(pixels / (ms * 1e-3) / 1e9, 3),
+            "triangles_per_s": round(len(views) * T / (ms * 1e-3), 1),
+            "fragments_per_s": round(st["fragments"] / (ms * 1e-3), 1), "fragments": st["fragments"],
+            "bin_entries": st["binEntries"], "clipped_triangles": st["clippedTriangles"],
+            "stage_ms": {STAGES[k]: round(acc[k], 4) for k in range(len(STAGES))},
+            "autotune": {"best": tune["best"], "default_ms": round(tune["default_ms"], 4),
+                         "best_ms": round(tune["best_ms"], 4), "candidates": tune["candidates"],
+                         "identical": tune["identical"], "bin_demand": tune["bin_demand"],
+                         "seconds": round(tune_s, 2)},
+            "roofline": {"bound": "hbm", "algorithmic_bytes": round(ab), "achieved": round(achieved, 2),
+                         "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +244,8 @@ def main():
     ap.add_argument("--neural-views", type=int, default=512)
     ap.add_argument("--neural-single-tap", type=int, default=1,
                     help="1: one trilinear latent tap per fragment, 0: the material's anisotropic taps")
+    ap.add_argument("--no-batch", action="store_true", help="skip the multi-view batch leg")
+    ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a float32 copy of 256 MiB (x + 0) three times: the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -269,6 +336,8 @@ def main():
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5),
                      "traffic": traffic, "algorithmic_bytes": ab, "duration_ms": round(timed[dom], 4)},
     }
+    if not args.no_batch:
+        out["batch"] = batch_leg(args)
     if not args.no_neural:
         out["neural"] = neural_leg(args)
     if not args.no_cpu:

@@ -372,8 +372,9 @@ bool TileRasterRenderer::prepare(int indexCount) {
     const size_t px = totalPixels_;
     const size_t ps = px * samples_;
     const unsigned part = 1u << partShift_;
-    binCap_ = std::max<unsigned>(
-        binCap_, std::max(1u << 20, (unsigned)std::min<size_t>(0xFFFFFFF0ull, (size_t)ntri * 6 * nv)));
+    if (!binCapSet_)  // default: room for six tiles per triangle and view (grown on overflow)
+        binCap_ = std::max<unsigned>(
+            binCap_, std::max(1u << 20, (unsigned)std::min<size_t>(0xFFFFFFF0ull, (size_t)ntri * 6 * nv)));
     bigCap_ = std::max<unsigned>(bigCap_, std::max(1024u, (unsigned)std::min<size_t>(0x7FFFFFFFull, ntri * nv / 16)));
     itemCap_ = std::max<unsigned>(itemCap_, (unsigned)tiles + binCap_ / part + 1u);
     partialCap_ = std::max<unsigned>(partialCap_, std::max(256u, binCap_ / (8u * part)));
@@ -832,6 +833,7 @@ void TileRasterRenderer::setTuning(const IfritTuning& t) {
     if (t.binCapacity && t.binCapacity != binCap_) {
         // an explicit capacity may also shrink the list (it grows again on overflow)
         binCap_ = t.binCapacity;
+        binCapSet_ = true;
         binList_.release();
         DevBuf::generation++;
     }

@@ -201,6 +201,7 @@ private:
     int rasterGroupsPerCU_ = 4;
     int shadeTile_ = 8;
     int bigThreshold_ = 64;
+    bool binCapSet_ = false;  // binCap_ given by setTuning: no default floor
     bool graphMode_ = false;
     bool capturing_ = false;
     hipGraph_t graph_ = nullptr;

@@ -214,5 +214,16 @@ def ref_visibility(width, height, clip, indices, depth_func=1):
     return ids, depth, bad
 
 
+def ggx_light(n, v, l, albedo, alpha, kd, ks):
+    """The layered-GGX material's one-light term (FP64), times n.l."""
+    L = lib()
+    f = L.oracle_ggx_light
+    f.restype = None
+    a = [np.ascontiguousarray(x, np.float64) for x in (n, v, l, albedo)]
+    out = np.zeros(3, np.float64)
+    f(*[_p(x) for x in a], C.c_double(alpha), C.c_double(kd), C.c_double(ks), _p(out))
+    return out
+
+
 def flat_hash(prim: int) -> int:
     return lib().oracle_flat_hash(prim)

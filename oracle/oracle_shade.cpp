@@ -305,6 +305,17 @@ inline uint8_t to_u8(double c) {
 
 extern "C" {
 
+// One light of the layered-GGX material (ggx_light above), for the closed-form tests.
+void oracle_ggx_light(const double* n, const double* v, const double* l, const double* alb, double alpha, double kd,
+                      double ks, double* out) {
+    bool flag = false;
+    const D3 c = ggx_light(D3{n[0], n[1], n[2]}, D3{v[0], v[1], v[2]}, D3{l[0], l[1], l[2]},
+                           D3{alb[0], alb[1], alb[2]}, alpha, kd, ks, &flag);
+    out[0] = c.x;
+    out[1] = c.y;
+    out[2] = c.z;
+}
+
 uint32_t oracle_flat_hash(uint32_t prim) {
     uint32_t h = prim * 0x9E3779B1u;
     h ^= h >> 15;

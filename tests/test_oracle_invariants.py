@@ -283,3 +283,61 @@ def test_equal_depth_keeps_lowest_primitive(S):
     # a later primitive cannot displace an equal-depth earlier one
     g = O.Frame().render(W, H, S, clip, np.concatenate([idx[3:], idx[:3]]).astype(np.int32), threads=1)
     assert np.all(g.ids[g.ids != EMPTY] == 0)
+
+
+# --- the layered-GGX material's BRDF (oracle_shade.cpp ggx_light) -------------------------------
+
+def _unit(v):
+    v = np.asarray(v, np.float64)
+    return v / np.linalg.norm(v)
+
+
+@pytest.mark.parametrize("alpha", [0.05, 0.3, 1.0])
+def test_ggx_normal_incidence_closed_form(alpha):
+    """n = v = l: D = 1/(pi alpha^2), G = 1, F = F0 = 0.04, so the term is
+    albedo * 0.96 kd / pi + ks * 0.04 / (4 pi alpha^2)."""
+    z = np.array([0.0, 0.0, 1.0])
+    alb = np.array([0.2, 0.5, 0.9])
+    got = O.ggx_light(z, z, z, alb, alpha, 0.7, 0.4)
+    want = alb * 0.96 * 0.7 / np.pi + 0.4 * 0.04 / (4 * np.pi * alpha * alpha)
+    assert np.allclose(got, want, rtol=1e-12)
+
+
+def test_ggx_is_zero_below_the_horizon():
+    n = np.array([0.0, 0.0, 1.0])
+    assert not O.ggx_light(n, _unit([0.3, 0, 1]), _unit([0.2, 0.1, -0.5]), [1, 1, 1], 0.4, 1, 1).any()
+    assert not O.ggx_light(n, _unit([0.3, 0, -1]), _unit([0.2, 0.1, 0.5]), [1, 1, 1], 0.4, 1, 1).any()
+
+
+def test_ggx_brdf_is_reciprocal():
+    """f(v, l) = f(l, v): the term divided by n.l must not change when view and light swap (a
+    transposed operand, e.g. n.v where n.l belongs, breaks this)."""
+    rng = np.random.default_rng(3)
+    n = np.array([0.0, 0.0, 1.0])
+    for _ in range(200):
+        v = _unit(rng.normal(size=3) + [0, 0, 1.5])
+        l = _unit(rng.normal(size=3) + [0, 0, 1.5])
+        if v[2] <= 0.05 or l[2] <= 0.05:
+            continue
+        a = rng.uniform(0.05, 1.0)
+        f1 = O.ggx_light(n, v, l, [0.3, 0.6, 0.8], a, 0.8, 0.5) / l[2]
+        f2 = O.ggx_light(n, l, v, [0.3, 0.6, 0.8], a, 0.8, 0.5) / v[2]
+        assert np.allclose(f1, f2, rtol=1e-10)
+
+
+@pytest.mark.parametrize("alpha", [0.2, 0.5, 1.0])
+def test_ggx_distribution_is_normalised(alpha):
+    """Retro-reflection (v = l, so h = v and F = F0) isolates D(n.h) * G1(n.v)^2; with the
+    Smith-Schlick G1 taken out, the recovered D must satisfy the textbook normalisation
+    integral over the hemisphere of D(h) (n.h) d(omega_h) = 1 (Walter et al. 2007)."""
+    n = np.array([0.0, 0.0, 1.0])
+    k = alpha / 2
+    c = np.linspace(1e-4, 1.0, 20001)
+    d = np.empty_like(c)
+    for i, ci in enumerate(c):
+        v = np.array([np.sqrt(1 - ci * ci), 0.0, ci])
+        spec = O.ggx_light(n, v, v, [0, 0, 0], alpha, 0.0, 1.0)[0]  # = D G F / (4 c c) * c
+        g1 = ci / (ci * (1 - k) + k)
+        d[i] = spec * 4 * ci / (0.04 * g1 * g1)
+    integral = 2 * np.pi * np.trapezoid(d * c, c)
+    assert abs(integral - 1.0) < 2e-3
