@@ -28,11 +28,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ms/frame & Gfrag/s at 1080p 8xMSAA; neural vs plain shade ms at ≥40 dB PSNR"
 PEAK_HBM_GBS = 8000.0
-STAGES = ["vertex", "reset", "setup", "bin", "raster", "raster_merge", "shade"]
+STAGES = ["vertex", "setup", "bin", "raster", "raster_merge", "shade"]
 
 
 def algorithmic_bytes(scene, stage: str) -> float:
-    """Compulsory HBM bytes of one launch of each stage: vertices in and clip positions out, indices and
+    """Compulsory HBM bytes of one launch of each stage: positions in and clip positions out, indices and
     clip positions in, per-sample depth and ID out, RGBA8 out plus one texel per pixel with its mips."""
     W, H, S = scene.width, scene.height, scene.samples
     V, T = scene.vertices.shape[0], scene.num_triangles
@@ -49,8 +49,9 @@ def algorithmic_bytes(scene, stage: str) -> float:
 
 
 # kernels of each timed stage (the raster stage's outputs are written by both
-# k_raster and, for tiles whose list was cut into parts, k_raster_merge)
-STAGE_KERNELS = {"vertex": ["k_vertex"], "setup": ["k_setup", "k_clip"], "bin": ["k_bin_small", "k_bin_big", "k_tile_scan"],
+# k_raster and, for tiles whose list was cut into items, k_raster_merge)
+STAGE_KERNELS = {"vertex": ["k_vertex"], "setup": ["k_setup", "k_clip"],
+                 "bin": ["k_bin_small", "k_bin_big", "k_plan_sums", "k_plan_scan", "k_plan_write"],
                  "raster": ["k_raster", "k_raster_merge"], "shade": ["k_shade"]}
 
 

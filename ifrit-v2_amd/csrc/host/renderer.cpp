@@ -54,11 +54,11 @@ TileRasterRenderer::TileRasterRenderer() {}
 TileRasterRenderer::~TileRasterRenderer() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     dropGraph();
-    for (DevBuf* b : {&vtx_, &idx_, &clip_, &recs_[0], &recs_[1], &recs_[2], &recs_[3], &recs_[4], &recs_[5], &bary_, &tileBox_, &slotOf_, &binCount_, &binStart_, &binCursor_, &binList_,
+    for (DevBuf* b : {&vtx_, &idx_, &clip_, &recs_[0], &recs_[1], &recs_[2], &recs_[3], &recs_[4], &recs_[5], &bary_, &binQueue_, &slotOf_, &binCount_, &binStart_, &binCursor_, &binList_,
                       &bigList_, &ctr_, &clipList_, &statPart_, &items_, &tilePartial_, &multiList_, &partials_, &visId_, &visDepth_, &visSlot_, &visClass_, &rgba8_, &rgba32f_, &texDesc_, &nnParams_, &nnGrads_, &nnM_, &nnV_,
                       &nnPacked_, &nnQinfo_, &lat32_, &latGrad_, &latM_, &latV_, &lat16_, &nnData_, &nnFeat_, &nnCount_,
                       &nnLoss_, &nnEvalIn_, &nnEvalOut_, &nnGradPart_, &viewDev_, &tileView_, &shadeView_,
-                      &viewCtr_, &digest_})
+                      &viewCtr_, &digest_, &planSums_})
         b->release();
     for (auto& t : textures_)
         if (t.dev) (void)hipFree(t.dev);
@@ -306,8 +306,8 @@ bool TileRasterRenderer::prepare(int indexCount) {
         setError("multi-view batches need a GPU vertex program");
         return false;
     }
-    if (depthFunc_ != IF_COMPARE_OP_LESS) {
-        setError("the GPU backend implements the LESS depth test only");
+    if (depthFunc_ < IF_COMPARE_OP_NEVER || depthFunc_ > IF_COMPARE_OP_ALWAYS) {
+        setError("unknown depth function");
         return false;
     }
     if (blend_.blendEnable) {
@@ -383,10 +383,11 @@ bool TileRasterRenderer::prepare(int indexCount) {
     bool okAll = clip_.ensure((size_t)nverts_ * 16 * nv) && recs_[0].ensure(slots * 16) &&
                  recs_[1].ensure(slots * 16) && recs_[2].ensure(slots * 16) && recs_[3].ensure(slots * 16) &&
                  recs_[4].ensure(slots * 16) && recs_[5].ensure(slots * 16) && bary_.ensure(slots * 32) &&
-                 tileBox_.ensure(slots * 16) && slotOf_.ensure((size_t)std::max(1, ntri) * 4 * nv) &&
+                 binQueue_.ensure(slots * 16) && slotOf_.ensure((size_t)std::max(1, ntri) * 4 * nv) &&
                  clipList_.ensure((size_t)clipCap_ * 4 * nv) &&
                  statPart_.ensure(((size_t)(ntri + 255) / 256 * nv + 2 * (size_t)rasterGrid() + 2) * 8) &&
-                 items_.ensure((size_t)itemCap_ * 8) && tilePartial_.ensure(tiles * 4) &&
+                 items_.ensure((size_t)itemCap_ * 16) && tilePartial_.ensure(tiles * 4) &&
+                 planSums_.ensure(((tiles + 2047) / 2048 + 1) * 6 * 4) &&
                  multiList_.ensure(tiles * 8) && partials_.ensure((size_t)partialCap_ * 256 * samples_ * 8) &&
                  binCount_.ensure(tiles * 4) && binStart_.ensure(tiles * 4) && binCursor_.ensure(tiles * 4) &&
                  binList_.ensure((size_t)binCap_ * 4) && bigList_.ensure((size_t)bigCap_ * 4) &&
@@ -521,7 +522,9 @@ TriRecs TileRasterRenderer::recPtrs() const {
 
 StatPartials TileRasterRenderer::statPtrs() const {
     StatPartials st{};
-    st.setupBlocks = (curTris_ + 255) / 256 * (int)std::max<size_t>(1, viewDesc_.size());
+    // setup workgroups: 1024 threads for up to four views, 256 beyond
+    const int nv = (int)std::max<size_t>(1, viewDesc_.size()), per = nv <= 4 ? 1024 : 256;
+    st.setupBlocks = (curTris_ + per - 1) / per * nv;
     st.rasterBlocks = rasterGrid();
     unsigned long long* base = statPart_.as<unsigned long long>();
     st.setupAssembled = base;
@@ -530,7 +533,11 @@ StatPartials TileRasterRenderer::statPtrs() const {
     return st;
 }
 
-int TileRasterRenderer::rasterGrid() const { return numCUs_ * rasterGroupsPerCU_; }
+// Raster workgroups: rasterGroupsPerCU_ per compute unit, four times as many
+// when a 1-sample tile takes one wave.
+int TileRasterRenderer::rasterGrid() const {
+    return numCUs_ * rasterGroupsPerCU_ * (samples_ == 1 && tileThreads_ == 64 ? 4 : 1);
+}
 
 void TileRasterRenderer::enqueue(bool clear) {
     const int tiles = (int)totalTiles_;
@@ -540,13 +547,13 @@ void TileRasterRenderer::enqueue(bool clear) {
         if (stageTiming_ && !capturing_) (void)hipEventRecord(ev_[i], s);
     };
     mark(0);
-    if (!hostVs_)
+    if (hostVs_)
+        launch_reset(binCount_.as<unsigned>(), binCursor_.as<unsigned>(), tiles, V, ctr_.as<DeviceCounters>(), s);
+    else  // the vertex launch also clears the counters
         launch_vertex(vtx_.as<float>(), stride_, nverts_, V, vs_.desc.kind == IFHIP_VS_CLIP_PASSTHROUGH ? 1 : 0,
                       clip_.as<float4>(), binCount_.as<unsigned>(), binCursor_.as<unsigned>(), tiles,
                       ctr_.as<DeviceCounters>(), s);
     mark(1);
-    if (hostVs_) launch_reset(binCount_.as<unsigned>(), binCursor_.as<unsigned>(), tiles, V, ctr_.as<DeviceCounters>(), s);
-    mark(2);
     SetupParams sp{};
     sp.ntri = curTris_;
     sp.nverts = nverts_;
@@ -554,16 +561,20 @@ void TileRasterRenderer::enqueue(bool clear) {
     sp.msaa = samples_ > 1 ? 1 : 0;
     sp.clipCap = clipCap_;
     sp.invTileWidth = ifx::rcp_nr_ss(16.0f);
-    launch_setup(sp, V, clip_.as<float4>(), idx_.as<int>(), recPtrs(), bary_.as<float4>(), tileBox_.as<int4>(),
+    launch_setup(sp, V, clip_.as<float4>(), idx_.as<int>(), recPtrs(), bary_.as<float4>(), binQueue_.as<uint4>(),
                  slotOf_.as<unsigned>(), clipList_.as<unsigned>(), statPtrs(), ctr_.as<DeviceCounters>(), s);
-    mark(3);
-    RasterQueue q{items_.as<uint2>(), tilePartial_.as<int>(), multiList_.as<unsigned>(),
-                  partials_.as<unsigned long long>(), itemCap_, partialCap_, rasterGrid(), partShift_};
-    BinParams bp{tiles, curTris_, clipCap_, binCap_, bigCap_, bigThreshold_};
-    launch_bin(bp, V, recPtrs(), tileBox_.as<int4>(), binCount_.as<unsigned>(), binStart_.as<unsigned>(),
+    mark(2);
+    RasterQueue q{items_.as<uint4>(), tilePartial_.as<int>(), multiList_.as<unsigned>(),
+                  partials_.as<unsigned long long>(), planSums_.as<unsigned>(), itemCap_, partialCap_, rasterGrid(),
+                  numCUs_ * rasterGroupsPerCU_, partShift_, samples_ == 1 ? tileThreads_ : 256};
+    // enough workgroups for every view's queue, at most 16 per compute unit
+    const size_t queueChunks = ((size_t)curTris_ + clipCap_ + 255) / 256 * viewDesc_.size();
+    BinParams bp{tiles, (int)std::max<size_t>(1, std::min<size_t>(queueChunks, (size_t)numCUs_ * 16)), curTris_,
+                 clipCap_, binCap_, bigCap_, bigThreshold_};
+    launch_bin(bp, V, recPtrs(), binQueue_.as<uint4>(), binCount_.as<unsigned>(), binStart_.as<unsigned>(),
                binCursor_.as<unsigned>(), binList_.as<unsigned>(), bigList_.as<unsigned>(), q,
                ctr_.as<DeviceCounters>(), s);
-    mark(4);
+    mark(3);
     RasterParams rp{};
     rp.depthFunc = depthFunc_;
     rp.clear = clear ? 1 : 0;
@@ -575,14 +586,23 @@ void TileRasterRenderer::enqueue(bool clear) {
     }
     // the 1-sample coverage class is a single-view diagnostic
     uint8_t* cls = batch_.empty() ? visClass_.as<uint8_t>() : nullptr;
-    launch_raster(rp, V, samples_, recPtrs(), slotOf_.as<unsigned>(), binStart_.as<unsigned>(),
-                  binCount_.as<unsigned>(), binList_.as<unsigned>(), q, visDepth_.as<float>(),
-                  visSlot_.as<uint32_t>(), cls, statPtrs(), ctr_.as<DeviceCounters>(), s);
+    if (depthFunc_ == IF_COMPARE_OP_LESS) {
+        // order-independent: the lexicographic minimum of (depth, primitive)
+        launch_raster(rp, V, samples_, recPtrs(), slotOf_.as<unsigned>(), binStart_.as<unsigned>(),
+                      binCount_.as<unsigned>(), binList_.as<unsigned>(), q, visDepth_.as<float>(),
+                      visSlot_.as<uint32_t>(), cls, statPtrs(), ctr_.as<DeviceCounters>(), s);
+        mark(4);
+        launch_raster_merge(rp, V, samples_, recPtrs(), slotOf_.as<unsigned>(), binStart_.as<unsigned>(),
+                            binCount_.as<unsigned>(), q, visDepth_.as<float>(), visSlot_.as<uint32_t>(), cls, tiles,
+                            ctr_.as<DeviceCounters>(), s);
+    } else {
+        // every other depth function folds each pixel's fragments in primitive order
+        launch_raster_ordered(rp, V, samples_, recPtrs(), binStart_.as<unsigned>(), binCount_.as<unsigned>(),
+                              binList_.as<unsigned>(), tiles, rasterGrid(), visDepth_.as<float>(),
+                              visSlot_.as<uint32_t>(), cls, statPtrs(), s);
+        mark(4);
+    }
     mark(5);
-    launch_raster_merge(rp, V, samples_, recPtrs(), slotOf_.as<unsigned>(), binStart_.as<unsigned>(),
-                        binCount_.as<unsigned>(), q, visDepth_.as<float>(), visSlot_.as<uint32_t>(), cls, tiles,
-                        ctr_.as<DeviceCounters>(), s);
-    mark(6);
     if (fsBound_ && fs_.desc.kind != IFHIP_FS_DEPTH_ONLY && shadeTiles_ > 0) {
         ShadeParams sh{};
         const IfritGpuFragmentProgramDesc& d = fs_.desc;
@@ -614,7 +634,7 @@ void TileRasterRenderer::enqueue(bool clear) {
                      vtx_.as<float>(), idx_.as<int>(), texDesc_.as<TexDesc>(), nullptr, rgba8_.as<uchar4>(),
                      wantFloat ? rgba32f_.as<float4>() : nullptr, s);
     }
-    mark(7);
+    mark(6);
 }
 
 // Everything a captured graph bakes into its launches.  View tables live in
@@ -693,7 +713,7 @@ bool TileRasterRenderer::finish(bool clear, int indexCount) {
     stats_.coveredSamples = (int64_t)c.coveredSamples;
     stats_.status = c.status;
     if (stageTiming_ && !graphMode_) {
-        for (int i = 0; i < 7; i++) {
+        for (int i = 0; i < 6; i++) {
             float ms = 0;
             (void)hipEventElapsedTime(&ms, ev_[i], ev_[i + 1]);
             stats_.stageMs[i] = ms;
@@ -824,12 +844,15 @@ void TileRasterRenderer::setGraphMode(bool on) {
 void TileRasterRenderer::setTuning(const IfritTuning& t) {
     const int part = t.rasterPartEntries;
     if ((part != 256 && part != 512 && part != 1024) || t.rasterGroupsPerCU < 1 || t.rasterGroupsPerCU > 8 ||
-        (t.shadeTile != 8 && t.shadeTile != 16 && t.shadeTile != 32) || t.bigTriangleTiles < 1)
-        return setError("tuning out of range: part 256/512/1024, 1..8 groups per CU, shading tile 8/16/32");
+        (t.shadeTile != 8 && t.shadeTile != 16 && t.shadeTile != 32) || t.bigTriangleTiles < 1 ||
+        (t.rasterTileThreads != 64 && t.rasterTileThreads != 256))
+        return setError("tuning out of range: part 256/512/1024, 1..8 groups per CU, shading tile 8/16/32, "
+                        "64 or 256 threads per tile");
     partShift_ = part == 256 ? 8 : (part == 512 ? 9 : 10);
     rasterGroupsPerCU_ = t.rasterGroupsPerCU;
     shadeTile_ = t.shadeTile;
     bigThreshold_ = t.bigTriangleTiles;
+    tileThreads_ = t.rasterTileThreads;
     if (t.binCapacity && t.binCapacity != binCap_) {
         // an explicit capacity may also shrink the list (it grows again on overflow)
         binCap_ = t.binCapacity;
@@ -845,6 +868,7 @@ void TileRasterRenderer::getTuning(IfritTuning* t) const {
     t->shadeTile = shadeTile_;
     t->bigTriangleTiles = bigThreshold_;
     t->binCapacity = binCap_;
+    t->rasterTileThreads = tileThreads_;
 }
 
 uint64_t TileRasterRenderer::digest() {
@@ -915,14 +939,17 @@ void TileRasterRenderer::autotune(int indexCount, int frames, IfritAutotuneRepor
     };
     float bestMs = measure(best);
     rep.defaultMs = bestMs;
-    const int parts[] = {256, 512, 1024}, groups[] = {2, 4, 8}, shades[] = {8, 16, 32}, bigs[] = {16, 64, 256};
+    // candidate values per parameter (threads per tile only matter at 1 sample per pixel)
+    const int values[5][3] = {{256, 512, 1024}, {2, 4, 8}, {8, 16, 32}, {16, 64, 256}, {256, 64, 64}};
+    const int knobs = samples_ == 1 ? 5 : 4;
     for (int round = 0; round < 2 && bestMs >= 0.0f; round++) {
-        for (int knob = 0; knob < 4; knob++) {
+        for (int knob = 0; knob < knobs; knob++) {
             for (int i = 0; i < 3; i++) {
                 IfritTuning t = best;
-                int* field = knob == 0 ? &t.rasterPartEntries
-                                       : (knob == 1 ? &t.rasterGroupsPerCU : (knob == 2 ? &t.shadeTile : &t.bigTriangleTiles));
-                const int val = knob == 0 ? parts[i] : (knob == 1 ? groups[i] : (knob == 2 ? shades[i] : bigs[i]));
+                int* const fields[5] = {&t.rasterPartEntries, &t.rasterGroupsPerCU, &t.shadeTile, &t.bigTriangleTiles,
+                                        &t.rasterTileThreads};
+                int* field = fields[knob];
+                const int val = values[knob][i];
                 if (*field == val) continue;
                 *field = val;
                 const float ms = measure(t);

@@ -194,13 +194,14 @@ private:
     std::vector<int> viewLayout_;           // (width, height, shade) per view + shading tile
     unsigned totalTiles_ = 0, shadeTiles_ = 0;
     size_t totalPixels_ = 0;
-    DevBuf viewDev_, tileView_, shadeView_, viewCtr_, digest_;
+    DevBuf viewDev_, tileView_, shadeView_, viewCtr_, digest_, planSums_;
 
     // launch parameters (IfritTuning) and graph replay
     int partShift_ = kRasterPartShift;
     int rasterGroupsPerCU_ = 4;
     int shadeTile_ = 8;
     int bigThreshold_ = 64;
+    int tileThreads_ = 256;   // raster threads per tile at 1 sample per pixel
     bool binCapSet_ = false;  // binCap_ given by setTuning: no default floor
     bool graphMode_ = false;
     bool capturing_ = false;
@@ -209,7 +210,7 @@ private:
     std::vector<uint8_t> graphKey_;
 
     // device buffers
-    DevBuf vtx_, idx_, clip_, recs_[6], bary_, tileBox_, slotOf_, binCount_, binStart_, binCursor_, binList_, bigList_, ctr_;
+    DevBuf vtx_, idx_, clip_, recs_[6], bary_, binQueue_, slotOf_, binCount_, binStart_, binCursor_, binList_, bigList_, ctr_;
     DevBuf clipList_, statPart_, items_, tilePartial_, multiList_, partials_, visId_, visDepth_, visSlot_, visClass_, rgba8_, rgba32f_, texDesc_;
     unsigned clipCap_ = 0, binCap_ = 0, bigCap_ = 0, itemCap_ = 0, partialCap_ = 0;
     int numCUs_ = 256;
@@ -234,7 +235,7 @@ private:
 
     IfritFrameStats stats_{};
     bool stageTiming_ = false;
-    hipEvent_t ev_[8] = {};  // stage boundaries: vertex, reset, setup, bin, raster, merge, shade
+    hipEvent_t ev_[8] = {};  // stage boundaries: vertex (with counter reset), setup, bin, raster, merge, shade
     std::string error_;
     std::string errorOut_;
 };

@@ -22,56 +22,75 @@ using dev::Edges;
 using dev::load_edges;
 using dev::tile_class;
 
-// Shared traversal of the small triangles, one thread per (slot, view) with
-// blockIdx.y = view.  SCATTER = false counts, SCATTER = true writes the lists.
+__device__ __forceinline__ int4 unpack_box(const uint4& e) {
+    return make_int4((int)(e.y & 0xFFFFu), (int)(e.y >> 16), (int)(e.z & 0xFFFFu), (int)(e.z >> 16));
+}
+
+// Shared traversal of the small triangles over the bin queues of all views.
+// The queue entries of view v are numbered in chunks of 256 that continue
+// from the previous view's, and chunk c goes to workgroup c mod gridDim.x, so
+// the work spreads evenly whatever the views' sizes.  SCATTER = false counts
+// (and defers the big triangles), SCATTER = true writes the lists.
 template <bool SCATTER>
 __global__ void __launch_bounds__(256) k_bin_small(BinParams P, Views V, TriRecs recs,
-                                                   const int4* __restrict__ tileBox, unsigned* __restrict__ binCount,
+                                                   const uint4* __restrict__ queueAll, unsigned* __restrict__ binCount,
                                                    const unsigned* __restrict__ binStart,
                                                    unsigned* __restrict__ binCursor, unsigned* __restrict__ binList,
                                                    unsigned* __restrict__ bigList, DeviceCounters* ctr) {
-    const ViewDesc& D = V.desc[blockIdx.y];
-    const unsigned tileBase = D.tileBase;
-    const int tilesX = D.tilesX;
-    const unsigned local = blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned s = D.slotBase + local;
-    bool valid = local < (unsigned)P.ntri + min(V.ctr[blockIdx.y].clipCount, P.clipCap);
-    int4 b = make_int4(1, 1, 0, 0);
-    if (valid) b = tileBox[s];
-    valid = valid && b.x <= b.z && b.y <= b.w;
-    const int w = b.z - b.x + 1;
-    const int area = valid ? w * (b.w - b.y + 1) : 0;
-    const bool big = valid && area > P.bigThreshold;
-    if (!SCATTER) {
-        // deferred triangles go to the per-workgroup pass
-        const unsigned k = dev::wave_inc(&ctr->bigCount, 0, big);
-        if (big) {
-            if (k < P.bigCap)
-                bigList[k] = s;
-            else
-                atomicOr(&ctr->status, kStatusBigOverflow);
-        }
-    }
-    const int n = (valid && !big) ? area : 0;
-    Edges E;
-    if (n) E = load_edges(recs, s);
-    const int iters = dev::wave_max(n);
-    for (int i = 0; i < iters; i++) {
-        const bool active = i < n;
-        int t = 0, c = 0;
-        if (active) {
-            const int tx = b.x + i % w, ty = b.y + i / w;
-            t = (int)tileBase + ty * tilesX + tx;
-            c = tile_class(E, tx, ty);
-        }
-        const bool hit = active && c != 0;
-        if (!SCATTER) {
-            (void)dev::wave_inc(binCount, t, hit);
-        } else {
-            const unsigned pos = dev::wave_inc(binCursor, t, hit);
-            if (hit) {
-                const unsigned at = binStart[t] + pos;
-                if (at < P.binCap) binList[at] = s | (c == 2 ? 0x80000000u : 0u);
+    const unsigned G = gridDim.x;
+    unsigned chunkBase = 0;
+    for (int v = 0; v < V.count; v++) {
+        const unsigned nq = min(V.ctr[v].queued, V.slotsPerView);
+        const unsigned nch = (nq + 255u) / 256u;
+        const unsigned c0 = (blockIdx.x + G - chunkBase % G) % G;
+        chunkBase += nch;
+        if (c0 >= nch) continue;
+        const ViewDesc& D = V.desc[v];
+        const unsigned tileBase = D.tileBase;
+        const int tilesX = D.tilesX;
+        const uint4* queue = queueAll + (size_t)v * V.slotsPerView;
+        for (unsigned c = c0; c < nch; c += G) {
+            const unsigned qi = c * 256u + threadIdx.x;
+            const bool valid = qi < nq;
+            uint4 e = make_uint4(0u, 0u, 0u, 0u);
+            if (valid) e = queue[qi];
+            const unsigned s = e.x;
+            const int4 b = unpack_box(e);
+            const int w = b.z - b.x + 1;
+            const int area = valid ? w * (b.w - b.y + 1) : 0;
+            const bool big = valid && area > P.bigThreshold;
+            if (!SCATTER) {
+                // deferred triangles go to the per-workgroup pass (as their queue position)
+                const unsigned k = dev::wave_inc(&ctr->bigCount, 0, big);
+                if (big) {
+                    if (k < P.bigCap)
+                        bigList[k] = (unsigned)v * V.slotsPerView + qi;
+                    else
+                        atomicOr(&ctr->status, kStatusBigOverflow);
+                }
+            }
+            const int n = (valid && !big) ? area : 0;
+            Edges E;
+            if (n) E = load_edges(recs, s);
+            const int iters = dev::wave_max(n);
+            for (int i = 0; i < iters; i++) {
+                const bool active = i < n;
+                int t = 0, cls = 0;
+                if (active) {
+                    const int tx = b.x + i % w, ty = b.y + i / w;
+                    t = (int)tileBase + ty * tilesX + tx;
+                    cls = tile_class(E, tx, ty);
+                }
+                const bool hit = active && cls != 0;
+                if (!SCATTER) {
+                    (void)dev::wave_inc(binCount, t, hit);
+                } else {
+                    const unsigned pos = dev::wave_inc(binCursor, t, hit);
+                    if (hit) {
+                        const unsigned at = binStart[t] + pos;
+                        if (at < P.binCap) binList[at] = s | (cls == 2 ? 0x80000000u : 0u);
+                    }
+                }
             }
         }
     }
@@ -79,17 +98,19 @@ __global__ void __launch_bounds__(256) k_bin_small(BinParams P, Views V, TriRecs
 
 template <bool SCATTER>
 __global__ void __launch_bounds__(256) k_bin_big(BinParams P, Views V, TriRecs recs,
-                                                 const int4* __restrict__ tileBox, unsigned* __restrict__ binCount,
+                                                 const uint4* __restrict__ queueAll, unsigned* __restrict__ binCount,
                                                  const unsigned* __restrict__ binStart, unsigned* __restrict__ binCursor,
                                                  unsigned* __restrict__ binList, const unsigned* __restrict__ bigList,
                                                  const DeviceCounters* ctr) {
     const unsigned nbig = min(ctr->bigCount, P.bigCap);
     for (unsigned i = blockIdx.x; i < nbig; i += gridDim.x) {
-        const unsigned s = bigList[i];
-        const ViewDesc& D = V.desc[s / V.slotsPerView];
+        const unsigned qi = bigList[i];
+        const ViewDesc& D = V.desc[qi / V.slotsPerView];
         const unsigned tileBase = D.tileBase;
         const int tilesX = D.tilesX;
-        const int4 b = tileBox[s];
+        const uint4 e = queueAll[qi];
+        const unsigned s = e.x;
+        const int4 b = unpack_box(e);
         const Edges E = load_edges(recs, s);
         const int w = b.z - b.x + 1;
         const int area = w * (b.w - b.y + 1);
@@ -108,136 +129,199 @@ __global__ void __launch_bounds__(256) k_bin_big(BinParams P, Views V, TriRecs r
     }
 }
 
-// Workgroup-wide inclusive scan over 1024 threads (16 waves).
-__device__ __forceinline__ unsigned scan1024(unsigned v, unsigned* sWave, unsigned& total) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    unsigned x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const unsigned y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
-    __syncthreads();  // sWave is free
-    if (lane == 63) sWave[w] = x;
-    __syncthreads();
-    unsigned base = 0;
-    total = 0;
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-        const unsigned t = sWave[i];
-        base += i < w ? t : 0u;
-        total += t;
-    }
-    return base + x;
+// ---------------------------------------------------------------------------
+// Raster planning over the tiles of all views, in three launches so that
+// every phase is spread over the chip: (A) per block of 2048 tiles the sums
+// of six per-tile quantities, (B) one workgroup's exclusive scan of the block
+// sums and the totals, (C) per block the list starts and the work lists.
+// A tile's list is cut into items of at most 1 << partShift entries; items
+// of more than kRasterPass entries are listed from the front of the item
+// array and the others from its back, so that the queue hands out the long
+// ones first.  Tiles with several items get a partial key block per item and
+// go to multiList[0, numMulti); empty tiles go to multiList[tiles, ...).
+// Lists need not be in tile order: every tile's range is contiguous and the
+// ranges are disjoint.
+// ---------------------------------------------------------------------------
+constexpr int kPlanPer = 8;                 // tiles per thread
+constexpr int kPlanBlock = 256 * kPlanPer;  // tiles per block
+constexpr int kPlanQ = 6;                   // entries, long items, short items, partial blocks, multi, empty
+
+struct TilePlan {
+    unsigned q[kPlanQ];
+};
+
+__device__ __forceinline__ TilePlan tile_plan(unsigned c, bool valid, int sh) {
+    TilePlan w;
+    const unsigned part = 1u << sh;
+    const unsigned p = valid ? (c + part - 1u) >> sh : 0u;
+    const unsigned last = c - (p ? p - 1u : 0u) * part;  // entries of the last item
+    const unsigned shortItems = p && last <= (unsigned)kRasterPass ? 1u : 0u;
+    w.q[0] = valid ? c : 0u;
+    w.q[1] = p - shortItems;
+    w.q[2] = shortItems;
+    w.q[3] = p > 1u ? p : 0u;
+    w.q[4] = p > 1u ? 1u : 0u;
+    w.q[5] = valid && p == 0u ? 1u : 0u;
+    return w;
 }
 
-// One 1024-thread workgroup, in chunks of 8192 tiles: the exclusive scan of
-// the per-tile counts (list starts), then the raster work of each tile: parts
-// of at most 1 << q.partShift entries, the first partial key block of a tile with
-// several parts, the list of such tiles (multiList[0, numMulti)) and the list
-// of empty tiles (multiList[tiles, tiles + numEmpty)).  Parts of more than
-// kRasterPass entries are listed from the front of the item array and the
-// others from its back, so that the queue hands out the long ones first.
-// Counts are staged in LDS so that global loads are coalesced.
-__global__ void __launch_bounds__(1024) k_tile_scan(const unsigned* __restrict__ cnt, unsigned* __restrict__ start,
-                                                    int n, unsigned cap, RasterQueue q, DeviceCounters* ctr) {
-    constexpr int kPer = 8, kChunk = 1024 * kPer;
-    __shared__ unsigned sCnt[kChunk];
-    __shared__ unsigned sWave[16];
-    const int t = threadIdx.x;
-    const int sh = q.partShift;
-    const unsigned part = 1u << sh;
-    unsigned carryS = 0, carryA = 0, carryL = 0, carryB = 0, carryM = 0, carryE = 0;
-    for (int base = 0; base < n; base += kChunk) {
-        __syncthreads();
-        for (int i = t; i < kChunk; i += 1024) sCnt[i] = base + i < n ? cnt[base + i] : 0u;
-        __syncthreads();
-        unsigned sum = 0;
+// Tile of thread t's i-th slot in block b: a thread takes kPlanPer
+// consecutive tiles, so the work lists follow the tile order (neighbouring
+// tiles share triangles, and the raster runs neighbouring items together).
+__device__ __forceinline__ int plan_tile(int b, int i, int t) { return b * kPlanBlock + t * kPlanPer + i; }
+
+// Exclusive scan of kPlanQ values over a 256-thread workgroup; every thread must call it.
+__device__ __forceinline__ void scan256x6(unsigned* v, unsigned (*sWave)[4], unsigned* total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-        for (int i = 0; i < kPer; i++) sum += sCnt[t * kPer + i];
-        unsigned totS;
-        unsigned run = carryS + scan1024(sum, sWave, totS) - sum;
-        // starts out; the tile's list length within the capacity replaces its count in LDS
-        unsigned a = 0, l = 0, b = 0, m = 0, e = 0;  // long parts, short parts, partials, multi, empty
-        for (int i = 0; i < kPer; i++) {
-            const int tile = base + t * kPer + i;
-            const unsigned c = sCnt[t * kPer + i];
-            const unsigned st = run;
-            run += c;
-            if (tile < n) start[tile] = st;
-            const unsigned total = st >= cap ? 0u : min(c, cap - st);
-            const unsigned p = tile < n ? (total + part - 1u) >> sh : 0u;
-            sCnt[t * kPer + i] = total;
-            const unsigned last = total - (p ? p - 1u : 0u) * part;  // entries of the last part
-            const unsigned shortParts = p && last <= (unsigned)kRasterPass ? 1u : 0u;
-            a += p - shortParts;
-            l += shortParts;
-            b += p > 1u ? p : 0u;
-            m += p > 1u;
-            e += tile < n && p == 0u;
+    for (int k = 0; k < kPlanQ; k++) {
+        unsigned x = v[k];
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const unsigned y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
         }
-        unsigned totA, totL, totB, totM, totE;
-        unsigned ra = carryA + scan1024(a, sWave, totA) - a;
-        unsigned rl = carryL + scan1024(l, sWave, totL) - l;
-        unsigned rb = carryB + scan1024(b, sWave, totB) - b;
-        unsigned rm = carryM + scan1024(m, sWave, totM) - m;
-        unsigned re = carryE + scan1024(e, sWave, totE) - e;
-        for (int i = 0; i < kPer; i++) {
-            const int tile = base + t * kPer + i;
-            if (tile >= n) break;
-            const unsigned total = sCnt[t * kPer + i];
-            const unsigned p = (total + part - 1u) >> sh;
-            const unsigned last = total - (p ? p - 1u : 0u) * part;
-            const unsigned longParts = p && last <= (unsigned)kRasterPass ? p - 1u : p;
-            for (unsigned k = 0; k < longParts; k++)
-                if (ra + k < q.itemCap) q.items[ra + k] = make_uint2((unsigned)tile, k);
-            if (longParts < p && rl < q.itemCap) q.items[q.itemCap - 1u - rl] = make_uint2((unsigned)tile, p - 1u);
-            if (p > 1u) q.multiList[rm] = (unsigned)tile;
-            if (p == 0u) q.multiList[n + re] = (unsigned)tile;
-            q.tilePartial[tile] = p > 1u ? (int)rb : -1;
-            ra += longParts;
-            rl += p - longParts;
-            rb += p > 1u ? p : 0u;
-            rm += p > 1u;
-            re += p == 0u;
+        if (lane == 63) sWave[k][w] = x;
+        v[k] = x - v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPlanQ; k++) {
+        unsigned base = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) base += i < w ? sWave[k][i] : 0u;
+        v[k] += base;
+        if (total) total[k] = sWave[k][0] + sWave[k][1] + sWave[k][2] + sWave[k][3];
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) k_plan_sums(const unsigned* __restrict__ cnt, int n, int sh,
+                                                   unsigned* __restrict__ blockSums) {
+    __shared__ unsigned sWave[kPlanQ][4];
+    const int t = threadIdx.x;
+    unsigned v[kPlanQ] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < kPlanPer; i++) {
+        const int tile = plan_tile(blockIdx.x, i, t);
+        const TilePlan w = tile_plan(tile < n ? cnt[tile] : 0u, tile < n, sh);
+#pragma unroll
+        for (int k = 0; k < kPlanQ; k++) v[k] += w.q[k];
+    }
+    unsigned total[kPlanQ];
+    scan256x6(v, sWave, total);
+    if (t < kPlanQ) blockSums[blockIdx.x * kPlanQ + t] = total[t];
+}
+
+// One 1024-thread workgroup: exclusive offsets of the block sums (in place)
+// and the totals.  On a bin or work-list overflow nothing is rasterised; the
+// host grows the buffers and draws again.
+__global__ void __launch_bounds__(1024) k_plan_scan(unsigned* __restrict__ blockSums, int nblk, unsigned cap,
+                                                    RasterQueue q, DeviceCounters* ctr) {
+    __shared__ unsigned sWave[kPlanQ][16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    unsigned carry[kPlanQ] = {0, 0, 0, 0, 0, 0};
+    for (int base = 0; base < nblk; base += 1024) {
+        const int b = base + t;
+#pragma unroll
+        for (int k = 0; k < kPlanQ; k++) {
+            const unsigned v = b < nblk ? blockSums[b * kPlanQ + k] : 0u;
+            unsigned x = v;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const unsigned y = __shfl_up(x, off, 64);
+                if (lane >= off) x += y;
+            }
+            __syncthreads();  // sWave[k] is free
+            if (lane == 63) sWave[k][w] = x;
+            __syncthreads();
+            unsigned pre = 0, tot = 0;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                pre += i < w ? sWave[k][i] : 0u;
+                tot += sWave[k][i];
+            }
+            if (b < nblk) blockSums[b * kPlanQ + k] = carry[k] + pre + x - v;
+            carry[k] += tot;
         }
-        carryS += totS;
-        carryA += totA;
-        carryL += totL;
-        carryB += totB;
-        carryM += totM;
-        carryE += totE;
     }
     if (t == 0) {
-        ctr->binTotal = carryS;
-        if (carryS > cap) atomicOr(&ctr->status, kStatusBinOverflow);
-        // on overflow nothing is rasterised; the host grows the buffers and redraws
-        const bool fits = carryA + carryL <= q.itemCap && carryB <= q.partialCap;
-        ctr->numItems = fits ? carryA + carryL : 0u;
-        ctr->numLongItems = carryA;
-        ctr->numMulti = fits ? carryM : 0u;
-        ctr->numEmpty = fits ? carryE : 0u;
-        ctr->numPartials = carryB;
-        ctr->itemHead = 0u;
+        ctr->binTotal = carry[0];
+        const bool binFits = carry[0] <= cap;
+        if (!binFits) atomicOr(&ctr->status, kStatusBinOverflow);
+        const bool fits = carry[1] + carry[2] <= q.itemCap && carry[3] <= q.partialCap;
         if (!fits) atomicOr(&ctr->status, kStatusRasterOverflow);
+        const bool go = binFits && fits;
+        ctr->numItems = go ? carry[1] + carry[2] : 0u;
+        ctr->numLongItems = carry[1];
+        ctr->numPartials = carry[3];
+        ctr->numMulti = go ? carry[4] : 0u;
+        ctr->numEmpty = go ? carry[5] : 0u;
+        ctr->itemHead = 0u;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_plan_write(const unsigned* __restrict__ cnt, unsigned* __restrict__ start,
+                                                    int n, int sh, const unsigned* __restrict__ blockOffs,
+                                                    RasterQueue q) {
+    __shared__ unsigned sWave[kPlanQ][4];
+    const int t = threadIdx.x;
+    unsigned c[kPlanPer];
+    unsigned v[kPlanQ] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < kPlanPer; i++) {
+        const int tile = plan_tile(blockIdx.x, i, t);
+        c[i] = tile < n ? cnt[tile] : 0u;
+        const TilePlan w = tile_plan(c[i], tile < n, sh);
+#pragma unroll
+        for (int k = 0; k < kPlanQ; k++) v[k] += w.q[k];
+    }
+    scan256x6(v, sWave, nullptr);
+#pragma unroll
+    for (int k = 0; k < kPlanQ; k++) v[k] += blockOffs[blockIdx.x * kPlanQ + k];
+    const unsigned part = 1u << sh;
+    for (int i = 0; i < kPlanPer; i++) {
+        const int tile = plan_tile(blockIdx.x, i, t);
+        if (tile >= n) break;
+        const TilePlan w = tile_plan(c[i], true, sh);
+        const unsigned st = v[0], p = w.q[1] + w.q[2];
+        start[tile] = st;
+        for (unsigned k = 0; k < p; k++) {
+            const unsigned first = st + k * part;
+            const uint4 item = make_uint4((unsigned)tile, first, min(part, c[i] - k * part),
+                                          p > 1u ? v[3] + k : 0xFFFFFFFFu);
+            if (k < w.q[1]) {
+                if (v[1] + k < q.itemCap) q.items[v[1] + k] = item;
+            } else if (v[2] < q.itemCap) {
+                q.items[q.itemCap - 1u - v[2]] = item;
+            }
+        }
+        if (p > 1u) q.multiList[v[4]] = (unsigned)tile;
+        if (p == 0u) q.multiList[n + v[5]] = (unsigned)tile;
+        q.tilePartial[tile] = p > 1u ? (int)v[3] : -1;
+#pragma unroll
+        for (int k = 0; k < kPlanQ; k++) v[k] += w.q[k];
     }
 }
 
 }  // namespace
 
-void launch_bin(const BinParams& p, const Views& views, TriRecs recs, const int4* tileBox, unsigned* binCount,
+void launch_bin(const BinParams& p, const Views& views, TriRecs recs, const uint4* binQueue, unsigned* binCount,
                 unsigned* binStart, unsigned* binCursor, unsigned* binList, unsigned* bigList, const RasterQueue& q,
                 DeviceCounters* ctr, hipStream_t s) {
-    const size_t slots = (size_t)p.ntri + p.clipCap;
-    const dim3 g(dev::blocks(slots > 0 ? slots : 1, 256), views.count);
-    hipLaunchKernelGGL(k_bin_small<false>, g, dim3(256), 0, s, p, views, recs, tileBox, binCount, binStart, binCursor,
+    const dim3 g(p.grid);
+    hipLaunchKernelGGL(k_bin_small<false>, g, dim3(256), 0, s, p, views, recs, binQueue, binCount, binStart, binCursor,
                        binList, bigList, ctr);
-    hipLaunchKernelGGL(k_bin_big<false>, dim3(256), dim3(256), 0, s, p, views, recs, tileBox, binCount, binStart,
+    hipLaunchKernelGGL(k_bin_big<false>, dim3(256), dim3(256), 0, s, p, views, recs, binQueue, binCount, binStart,
                        binCursor, binList, bigList, ctr);
-    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, binCount, binStart, p.tiles, p.binCap, q, ctr);
-    hipLaunchKernelGGL(k_bin_small<true>, g, dim3(256), 0, s, p, views, recs, tileBox, binCount, binStart, binCursor,
+    const int nblk = (p.tiles + kPlanBlock - 1) / kPlanBlock;
+    hipLaunchKernelGGL(k_plan_sums, dim3(nblk), dim3(256), 0, s, binCount, p.tiles, q.partShift, q.planSums);
+    hipLaunchKernelGGL(k_plan_scan, dim3(1), dim3(1024), 0, s, q.planSums, nblk, p.binCap, q, ctr);
+    hipLaunchKernelGGL(k_plan_write, dim3(nblk), dim3(256), 0, s, binCount, binStart, p.tiles, q.partShift,
+                       q.planSums, q);
+    hipLaunchKernelGGL(k_bin_small<true>, g, dim3(256), 0, s, p, views, recs, binQueue, binCount, binStart, binCursor,
                        binList, bigList, ctr);
-    hipLaunchKernelGGL(k_bin_big<true>, dim3(256), dim3(256), 0, s, p, views, recs, tileBox, binCount, binStart,
+    hipLaunchKernelGGL(k_bin_big<true>, dim3(256), dim3(256), 0, s, p, views, recs, binQueue, binCount, binStart,
                        binCursor, binList, bigList, ctr);
 }
 

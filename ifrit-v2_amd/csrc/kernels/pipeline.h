@@ -11,7 +11,8 @@
 //   recs        TriRecs, N * SV slots     assembled triangles; in view v, slot v*SV + prim
 //                                         for unclipped triangles, v*SV + T + k for clip fans
 //   bary        float4[2 * N * SV]        vertex-weight bases bx, by of clip fans (shading only)
-//   tileBox     int4[N * SV]              view-local tile rectangle of each slot (x0 > x1: empty)
+//   binQueue    uint4[N * SV]             per view, the assembled triangles to bin: (slot,
+//                                         x0 | y0 << 16, x1 | y1 << 16, 0), view-local tiles
 //   slotOf      uint[N][T]                first slot of each primitive (its fans follow)
 //   binCount/binStart/binCursor  uint[tiles]
 //   binList     uint[binCap]              slot | 0x80000000 when the tile is fully covered
@@ -61,10 +62,13 @@ struct ViewDesc {
     unsigned long long pixBase;  // first pixel of the view's outputs
 };
 
-// Per-view counters of the clipping stage (zeroed by the vertex launch).
+// Per-view counters (zeroed each draw), one 128-byte line each so
+// that the views' atomics do not share a line.
 struct ViewCounters {
     unsigned clipCount;    // clip fan slots taken
     unsigned clipInCount;  // triangles crossing w = 0
+    unsigned queued;       // assembled triangles queued for binning
+    unsigned pad[29];
 };
 
 struct Views {
@@ -91,6 +95,7 @@ struct SetupParams {
 
 struct BinParams {
     int tiles;          // tiles of all views
+    int grid;           // workgroups of the binning launches
     int ntri;
     unsigned clipCap;
     unsigned binCap;
@@ -147,7 +152,7 @@ struct StatPartials {
     int setupBlocks, rasterBlocks;
 };
 
-// Counters written by the kernels (device memory, zeroed each draw).
+// Counters written by the kernels (device memory, zeroed each draw by the vertex or reset kernel).
 struct DeviceCounters {
     unsigned status;
     unsigned clipCount;  // clip fan slots of all views (reduced with the statistics)
@@ -171,18 +176,22 @@ struct DeviceCounters {
 constexpr int kRasterPass = 128;
 constexpr int kRasterPartShift = 9;
 
-// Raster work: (tile, part) items drained through DeviceCounters::itemHead;
-// per tile the first partial block of a tile cut into several parts (-1
-// otherwise); the list of such tiles followed, from index `tiles`, by the
-// empty tiles; partial key blocks of 256 * samples keys each.
+// Raster work: items {tile, first list entry, entries, partial block or
+// ~0u} drained through DeviceCounters::itemHead; per tile the first partial
+// block of a tile cut into several items (-1 otherwise); the list of such
+// tiles followed, from index `tiles`, by the empty tiles; partial key blocks
+// of 256 * samples keys each; planning scratch (6 words per 2048 tiles).
 struct RasterQueue {
-    uint2* items;
+    uint4* items;
     int* tilePartial;
     unsigned* multiList;
     unsigned long long* partials;
+    unsigned* planSums;
     unsigned itemCap, partialCap;
-    int grid;
+    int grid;         // raster workgroups
+    int mergeGrid;    // half the merge workgroups
     int partShift;
+    int tileThreads;  // threads per tile at 1 sample per pixel: 64 or 256
 };
 
 // Launch wrappers (pipeline.hip).
@@ -190,10 +199,10 @@ void launch_vertex(const float* vtx, int stride, int nverts, const Views& views,
                    unsigned* binCount, unsigned* binCursor, int tiles, DeviceCounters* ctr, hipStream_t s);
 void launch_reset(unsigned* binCount, unsigned* binCursor, int tiles, const Views& views, DeviceCounters* ctr,
                   hipStream_t s);
-void launch_setup(const SetupParams& p, const Views& views, const float4* clip, const int* idx, TriRecs recs, float4* bary, int4* tileBox,
+void launch_setup(const SetupParams& p, const Views& views, const float4* clip, const int* idx, TriRecs recs, float4* bary, uint4* binQueue,
                   unsigned* slotOf, unsigned* clipList, StatPartials st, DeviceCounters* ctr, hipStream_t s);
 void launch_finalize(StatPartials st, const Views& views, DeviceCounters* ctr, hipStream_t s);
-void launch_bin(const BinParams& p, const Views& views, TriRecs recs, const int4* tileBox, unsigned* binCount, unsigned* binStart,
+void launch_bin(const BinParams& p, const Views& views, TriRecs recs, const uint4* binQueue, unsigned* binCount, unsigned* binStart,
                 unsigned* binCursor, unsigned* binList, unsigned* bigList, const RasterQueue& q, DeviceCounters* ctr,
                 hipStream_t s);
 void launch_raster(const RasterParams& p, const Views& views, int samples, TriRecs recs, const unsigned* slotOf,
@@ -203,6 +212,12 @@ void launch_raster(const RasterParams& p, const Views& views, int samples, TriRe
 void launch_raster_merge(const RasterParams& p, const Views& views, int samples, TriRecs recs, const unsigned* slotOf,
                          const unsigned* binStart, const unsigned* binCount, const RasterQueue& q, float* visDepth,
                          uint32_t* visSlot, uint8_t* visClass, int tiles, DeviceCounters* ctr, hipStream_t s);
+// Depth functions other than LESS: each tile's list in primitive order, folded
+// per pixel (replaces launch_raster and launch_raster_merge).
+void launch_raster_ordered(const RasterParams& p, const Views& views, int samples, TriRecs recs,
+                           const unsigned* binStart, const unsigned* binCount, const unsigned* binList, int tiles,
+                           int grid, float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st,
+                           hipStream_t s);
 void launch_slot_to_prim(const uint32_t* visSlot, TriRecs recs, size_t n, uint32_t* visId, hipStream_t s);
 void launch_shade(const ShadeParams& p, const Views& views, int shadeTiles, TriRecs recs, const float4* bary, const uint32_t* visSlot,
                   const float* vtx, const int* idx, const TexDesc* tex, const uint8_t* texPool, uchar4* rgba8,

@@ -44,8 +44,7 @@ namespace {
 
 using namespace dev;
 
-constexpr int kPass = kRasterPass;  // list entries per pass through LDS
-constexpr int kUnits = 4096;  // units expanded into LDS per round
+constexpr int kPass = kRasterPass;  // the most list entries per pass through LDS
 
 // Depth at a 1-sample pixel for a coverage class (1: SIMD256, 2: SIMD128, 3: scalar).
 __device__ __forceinline__ float class_depth(unsigned c, const float4& F1, const float4& F2, const float4& F3, float x,
@@ -71,8 +70,9 @@ __device__ __forceinline__ bool rect_may_cover(const Edges& E, float x, float y,
     return ok;
 }
 
-// Exclusive scan over a 256-thread workgroup; every thread must call it.
-__device__ __forceinline__ unsigned scan256(unsigned v, unsigned* sWave, unsigned& total) {
+// Exclusive scan over an NT-thread workgroup (64 or 256); every thread must call it.
+template <int NT>
+__device__ __forceinline__ unsigned scan_nt(unsigned v, unsigned* sWave, unsigned& total) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     unsigned x = v;
 #pragma unroll
@@ -80,13 +80,31 @@ __device__ __forceinline__ unsigned scan256(unsigned v, unsigned* sWave, unsigne
         const unsigned y = __shfl_up(x, off, 64);
         if (lane >= off) x += y;
     }
-    if (lane == 63) sWave[w] = x;
-    __syncthreads();
-    unsigned base = 0;
+    if constexpr (NT == 64) {
+        total = __shfl(x, 63, 64);
+        return x - v;
+    } else {
+        if (lane == 63) sWave[w] = x;
+        __syncthreads();
+        unsigned base = 0;
 #pragma unroll
-    for (int i = 0; i < 4; i++) base += i < w ? sWave[i] : 0u;
-    total = sWave[0] + sWave[1] + sWave[2] + sWave[3];
-    return base + x - v;
+        for (int i = 0; i < NT / 64; i++) base += i < w ? sWave[i] : 0u;
+        total = 0;
+#pragma unroll
+        for (int i = 0; i < NT / 64; i++) total += sWave[i];
+        return base + x - v;
+    }
+}
+
+// Sum over an NT-thread workgroup (all threads must call it).
+template <int NT>
+__device__ __forceinline__ unsigned long long sum_nt(unsigned long long v) {
+    if constexpr (NT == 64) {
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        return v;
+    } else {
+        return block_sum256(v);
+    }
 }
 
 // Candidate mask of one half of a tile (block rows 2h and 2h + 1) for one
@@ -259,9 +277,10 @@ __device__ __forceinline__ void write_pixel(const RasterParams& P, const TriRecs
         const unsigned prim = tie >> 2;
         const unsigned slot = slotOf[prim] + (tie & 3u);
         float d = order_key_inv((unsigned)(k >> 32));
-        if (S == 1) {
+        if (S == 1 && (visClass || (unsigned)(k >> 32) == 0x80000000u)) {
+            // the class output, or +0 and -0 sharing a key: recompute from the winner's record
             cls = winner_class(load_rec(recs, slot), C.tx, C.ty, C.fpx, C.fpy, C.bMinX, C.bMinY, d);
-        } else if ((unsigned)(k >> 32) == 0x80000000u) {
+        } else if (S > 1 && (unsigned)(k >> 32) == 0x80000000u) {
             // +0 and -0 share a key; recompute to return the winner's own bits
             d = depth256(recs.f1[slot], recs.f2[slot], recs.f3[slot], C.fpx + P.sx[s], C.fpy + P.sy[s]);
         }
@@ -300,9 +319,10 @@ __device__ __forceinline__ void stage_pixel(const RasterParams& P, const TriRecs
             const unsigned prim = tie >> 2;
             slot = slotOf[prim] + (tie & 3u);
             d = order_key_inv((unsigned)(k >> 32));
-            if (S == 1) {
+            if (S == 1 && (visClass || (unsigned)(k >> 32) == 0x80000000u)) {
+                // the class output, or +0 and -0 sharing a key: recompute from the winner's record
                 cls = winner_class(load_rec(recs, slot), C.tx, C.ty, C.fpx, C.fpy, C.bMinX, C.bMinY, d);
-            } else if ((unsigned)(k >> 32) == 0x80000000u) {
+            } else if (S > 1 && (unsigned)(k >> 32) == 0x80000000u) {
                 // +0 and -0 share a key; recompute to return the winner's own bits
                 d = depth256(recs.f1[slot], recs.f2[slot], recs.f3[slot], C.fpx + P.sx[s], C.fpy + P.sy[s]);
             }
@@ -313,30 +333,23 @@ __device__ __forceinline__ void stage_pixel(const RasterParams& P, const TriRecs
     if (S == 1 && visClass) visClass[C.pix] = (uint8_t)cls;
 }
 
-// Stores a staged tile (CLEAR: the clear values, no staging needed) with
-// 16-byte stores; partial columns at the right edge fall back to scalars.
-// visDepth, visSlot and visClass point at the view's outputs; a view's first
-// pixel starts a 16-byte line when S > 1 or its pixel base is a multiple of 4.
-template <int S, bool CLEAR>
+// Stores a staged tile with 16-byte stores; partial columns at the right
+// edge fall back to scalars.  visDepth, visSlot and visClass point at the
+// view's outputs; a view's first pixel starts a 16-byte line when S > 1 or its
+// pixel base is a multiple of 4.
+template <int S, int NT = 256>
 __device__ __forceinline__ void store_tile(const ViewGeom& G, int tx, int ty, const TileOut<S>* O,
-                                           float* visDepth, uint32_t* visSlot, uint8_t* visClass) {
+                                           float* visDepth, uint32_t* visSlot) {
     constexpr int perRow = 16 * S / 4;  // float4 per tile row
     const int cols = min(16, G.width - tx * 16), rows = min(16, G.height - ty * 16);
     const bool vec = cols == 16 && (S > 1 || ((G.width & 3) == 0 && (G.pixBase & 3) == 0));
-    for (int i = threadIdx.x; i < 64 * S; i += 256) {
+    for (int i = threadIdx.x; i < 64 * S; i += NT) {
         const int row = i / perRow, q = i - row * perRow;
         if (row >= rows) continue;
         const size_t g = ((size_t)(ty * 16 + row) * G.width + tx * 16) * S + q * 4;
         const int l = row * 16 * S + q * 4;
-        float4 d;
-        uint4 sl;
-        if (CLEAR) {
-            d = make_float4(kClearDepth, kClearDepth, kClearDepth, kClearDepth);
-            sl = make_uint4(~0u, ~0u, ~0u, ~0u);
-        } else {
-            d = *reinterpret_cast<const float4*>(&O->d[l]);
-            sl = *reinterpret_cast<const uint4*>(&O->slot[l]);
-        }
+        const float4 d = *reinterpret_cast<const float4*>(&O->d[l]);
+        const uint4 sl = *reinterpret_cast<const uint4*>(&O->slot[l]);
         if (vec) {
             *reinterpret_cast<float4*>(&visDepth[g]) = d;
             *reinterpret_cast<uint4*>(&visSlot[g]) = sl;
@@ -351,29 +364,29 @@ __device__ __forceinline__ void store_tile(const ViewGeom& G, int tx, int ty, co
             }
         }
     }
-    if (CLEAR && S == 1 && visClass) {
-        const int row = threadIdx.x >> 4, col = threadIdx.x & 15;
-        if (row < rows && col < cols) visClass[(size_t)(ty * 16 + row) * G.width + tx * 16 + col] = 0;
-    }
 }
 
-template <int S>
-__global__ void __launch_bounds__(256) k_raster(RasterParams P, Views V, TriRecs recs,
-                                                const unsigned* __restrict__ slotOfAll,
-                                                const unsigned* __restrict__ binStart,
-                                                const unsigned* __restrict__ binCount,
-                                                const unsigned* __restrict__ binList, const uint2* __restrict__ items,
-                                                unsigned itemCap, int partShift, const int* __restrict__ tilePartial,
-                                                unsigned long long* __restrict__ partials,
-                                                float* __restrict__ visDepthAll, uint32_t* __restrict__ visSlotAll,
-                                                uint8_t* __restrict__ visClassAll, StatPartials st,
-                                                DeviceCounters* ctr) {
-    constexpr int NW = S == 1 ? 1 : 4;  // mask words of one half tile
+// NT threads per tile: 256 (one pixel per thread) or, at 1 sample per pixel,
+// 64 (one wave per tile, four pixels per lane: more tiles in flight per
+// compute unit, for many small tiles).  Passes hold NT / 2 entries.
+template <int S, int NT>
+__global__ void __launch_bounds__(NT) k_raster(RasterParams P, Views V, TriRecs recs,
+                                               const unsigned* __restrict__ slotOfAll,
+                                               const unsigned* __restrict__ binList, const uint4* __restrict__ items,
+                                               unsigned itemCap, unsigned long long* __restrict__ partials,
+                                               float* __restrict__ visDepthAll, uint32_t* __restrict__ visSlotAll,
+                                               uint8_t* __restrict__ visClassAll, StatPartials st,
+                                               DeviceCounters* ctr) {
+    constexpr int NW = S == 1 ? 1 : 4;      // mask words of one half tile
+    constexpr int PASS = NT / 2;            // list entries per pass through LDS
+    constexpr int UNITS = NT * 16;          // units expanded into LDS per round
+    constexpr int PPT = 256 / NT;           // tile pixels per thread
+    static_assert(PASS <= kPass, "units encode the entry in 7 bits");
     __shared__ unsigned long long sVis[S * 256];  // key of sample s of tile pixel p at s * 256 + p
     static_assert(sizeof(TileOut<S>) <= sizeof(sVis), "output staging reuses sVis");
-    __shared__ TriRec sRec[kPass];
-    __shared__ unsigned char sTa[2 * kPass];  // accepted blocks of each half
-    __shared__ unsigned short sUnit[kUnits];
+    __shared__ TriRec sRec[PASS];
+    __shared__ unsigned char sTa[2 * PASS];  // accepted blocks of each half
+    __shared__ unsigned short sUnit[UNITS];
     __shared__ unsigned sWave[4];
     __shared__ unsigned sItem;
 
@@ -387,24 +400,27 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P, Views V, TriRecs
         const unsigned item = sItem;
         if (item >= numItems) break;
         // long items from the front of the array, then short ones from its back
-        const uint2 it = items[item < numLong ? item : itemCap - 1u - (item - numLong)];
-        const int tile = __builtin_amdgcn_readfirstlane((int)it.x), part = __builtin_amdgcn_readfirstlane((int)it.y);
+        const uint4 it = items[item < numLong ? item : itemCap - 1u - (item - numLong)];
+        const int tile = __builtin_amdgcn_readfirstlane((int)it.x);
+        const unsigned first = __builtin_amdgcn_readfirstlane(it.y);
+        const int cnt = __builtin_amdgcn_readfirstlane((int)it.z);
+        const unsigned pslot = __builtin_amdgcn_readfirstlane(it.w);
         const ViewGeom G = view_of_tile(V, tile);
         float* const visDepth = visDepthAll + G.pixBase * S;
         uint32_t* const visSlot = visSlotAll + G.pixBase * S;
         uint8_t* const visClass = visClassAll ? visClassAll + G.pixBase : nullptr;
         const unsigned* const slotOf = slotOfAll + G.primBase;
-        const PixelCtx C = pixel_ctx(G, t & 15, t >> 4);
-        const int tx0 = C.tx * 16, ty0 = C.ty * 16;
-        const unsigned start = binStart[tile];
-        const unsigned total = start >= P.binCap ? 0u : min(binCount[tile], P.binCap - start);
-        const unsigned base = (unsigned)part << partShift;
-        const int cnt = (int)min(1u << partShift, total > base ? total - base : 0u);
-        const int pbase = tilePartial[tile];
+        const int ttx = G.tile % G.tilesX, tty = G.tile / G.tilesX;
+        const int tx0 = ttx * 16, ty0 = tty * 16;
 
 #pragma unroll
-        for (int s = 0; s < S; s++)
-            sVis[s * 256 + t] = frag_key(P.clear || !C.inFb ? kClearDepth : visDepth[C.pix * S + s], 0u);
+        for (int k = 0; k < PPT; k++) {
+            const int p = t + k * NT;
+            const PixelCtx C = pixel_ctx(G, p & 15, p >> 4);
+#pragma unroll
+            for (int s = 0; s < S; s++)
+                sVis[s * 256 + p] = frag_key(P.clear || !C.inFb ? kClearDepth : visDepth[C.pix * S + s], 0u);
+        }
 
         // two lanes per list entry: lane h = t & 1 takes block rows 2h, 2h + 1
         const int e0 = t >> 1, h = t & 1;
@@ -429,17 +445,17 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P, Views V, TriRecs
             }
         }
 
-        // passes of kPass entries through LDS, all merging into sVis
-        for (int pb = 0; pb < cnt; pb += kPass) {
-            const int pc = min(kPass, cnt - pb);
+        // passes of PASS entries through LDS, all merging into sVis
+        for (int pb = 0; pb < cnt; pb += PASS) {
+            const int pc = min(PASS, cnt - pb);
             unsigned m[NW];
             unsigned n = 0;
             if (e0 < pc) {
-                const unsigned ent = binList[start + base + pb + e0];
+                const unsigned ent = binList[first + pb + e0];
                 const TriRec R = load_rec(recs, ent & 0x7fffffffu);
                 if (h == 0) sRec[e0] = R;
                 unsigned ta;
-                entry_mask_half<S>(load_edges(R), (ent >> 31) != 0, C.tx, C.ty, h, fbMask, m, ta);
+                entry_mask_half<S>(load_edges(R), (ent >> 31) != 0, ttx, tty, h, fbMask, m, ta);
                 sTa[t] = (unsigned char)ta;
 #pragma unroll
                 for (int w = 0; w < NW; w++) n += __popc(m[w]);
@@ -448,17 +464,17 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P, Views V, TriRecs
                 for (int w = 0; w < NW; w++) m[w] = 0u;
             }
             unsigned T;
-            const unsigned off = scan256(n, sWave, T);
+            const unsigned off = scan_nt<NT>(n, sWave, T);
 
-            for (unsigned w0 = 0; w0 < T; w0 += kUnits) {
-                // write this lane's units that fall in [w0, w0 + kUnits)
+            for (unsigned w0 = 0; w0 < T; w0 += UNITS) {
+                // write this lane's units that fall in [w0, w0 + UNITS)
                 int pos = (int)off - (int)w0;
-                if (n && pos < kUnits && pos + (int)n > 0) {
+                if (n && pos < UNITS && pos + (int)n > 0) {
 #pragma unroll
                     for (int w = 0; w < NW; w++) {
                         unsigned bits = m[w];
                         const int c = __popc(bits);
-                        if (pos + c <= 0 || pos >= kUnits) {
+                        if (pos + c <= 0 || pos >= UNITS) {
                             pos += c;
                             continue;
                         }
@@ -467,14 +483,14 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P, Views V, TriRecs
                             bits &= bits - 1u;
                             // unit: entry << 8 | tile pixel (S > 1) or group (S == 1)
                             const int cu = S == 1 ? 16 * h + b : (4 * h + w) * 32 + b;
-                            if (pos >= 0 && pos < kUnits) sUnit[pos] = (unsigned short)((e0 << 8) | cu);
+                            if (pos >= 0 && pos < UNITS) sUnit[pos] = (unsigned short)((e0 << 8) | cu);
                             pos++;
                         }
                     }
                 }
                 __syncthreads();
-                const int nu = (int)min((unsigned)kUnits, T - w0);
-                for (int i = t; i < nu; i += 256) {
+                const int nu = (int)min((unsigned)UNITS, T - w0);
+                for (int i = t; i < nu; i += NT) {
                     const unsigned u = sUnit[i];
                     const int e = (int)(u >> 8), c = (int)(u & 255u);
                     const TriRec R = sRec[e];
@@ -528,29 +544,43 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P, Views V, TriRecs
             __syncthreads();  // sRec, sTa and sWave are reused by the next pass
         }
 
-        if (pbase >= 0) {
-            // one part of a longer list: its keys, sample-major like sVis
-            unsigned long long* g = partials + (size_t)(pbase + part) * 256 * S;
+        if (pslot != 0xFFFFFFFFu) {
+            // one item of a longer list: its keys, sample-major like sVis
+            unsigned long long* g = partials + (size_t)pslot * 256 * S;
 #pragma unroll
-            for (int s = 0; s < S; s++) g[s * 256 + t] = sVis[s * 256 + t];
+            for (int k = 0; k < PPT; k++)
+#pragma unroll
+                for (int s = 0; s < S; s++) g[s * 256 + t + k * NT] = sVis[s * 256 + t + k * NT];
         } else {
-            unsigned long long keys[S];
+            unsigned long long keys[PPT][S];
 #pragma unroll
-            for (int s = 0; s < S; s++) keys[s] = sVis[s * 256 + t];
+            for (int k = 0; k < PPT; k++)
+#pragma unroll
+                for (int s = 0; s < S; s++) keys[k][s] = sVis[s * 256 + t + k * NT];
             if (P.clear) {
                 // every key is in registers: stage the outputs over sVis
                 TileOut<S>* O = reinterpret_cast<TileOut<S>*>(sVis);
                 __syncthreads();
-                stage_pixel<S>(P, recs, slotOf, C, t, keys, *O, visClass);
+#pragma unroll
+                for (int k = 0; k < PPT; k++) {
+                    const int p = t + k * NT;
+                    stage_pixel<S>(P, recs, slotOf, pixel_ctx(G, p & 15, p >> 4), p, keys[k], *O, visClass);
+                }
                 __syncthreads();
-                store_tile<S, false>(G, C.tx, C.ty, O, visDepth, visSlot, visClass);
+                store_tile<S, NT>(G, ttx, tty, O, visDepth, visSlot);
             } else {
-                write_pixel<S>(P, recs, slotOf, C, keys, visDepth, visSlot, visClass);
+#pragma unroll
+                for (int k = 0; k < PPT; k++) {
+                    const int p = t + k * NT;
+                    write_pixel<S>(P, recs, slotOf, pixel_ctx(G, p & 15, p >> 4), keys[k], visDepth, visSlot,
+                                   visClass);
+                }
             }
         }
+        __syncthreads();  // every lane has read sItem and the LDS of this item
     }
-    const unsigned long long tf = block_sum256(frags);
-    const unsigned long long tc = block_sum256(covs);
+    const unsigned long long tf = sum_nt<NT>(frags);
+    const unsigned long long tc = sum_nt<NT>(covs);
     if (t == 0) {
         st.rasterFrags[blockIdx.x] = tf;
         st.rasterCovs[blockIdx.x] = tc;
@@ -572,19 +602,58 @@ __global__ void __launch_bounds__(256) k_raster_merge(RasterParams P, Views V, T
                                                       const DeviceCounters* ctr) {
     __shared__ TileOut<S> O;
     const unsigned numMulti = ctr->numMulti;
-    const unsigned numWork = numMulti + (P.clear ? ctr->numEmpty : 0u);
+    const unsigned numEmpty = P.clear ? ctr->numEmpty : 0u;
     const int t = threadIdx.x;
-    for (unsigned m = blockIdx.x; m < numWork; m += gridDim.x) {
-        const bool empty = m >= numMulti;
-        const int tile = (int)multiList[empty ? tiles + (m - numMulti) : m];
+    // clear values of the empty tiles: a tile is 64 * S lines of 16 bytes per output, so at 1 spp
+    // four tiles are cleared per round by 64 lanes each
+    {
+        constexpr int kLines = 64 * S;
+        constexpr int kPerRound = kLines >= 256 ? 1 : 256 / kLines;
+        constexpr int kLanes = 256 / kPerRound;
+        const int sub = t / kLanes, lane = t % kLanes;
+        for (unsigned e0 = blockIdx.x * kPerRound; e0 < numEmpty; e0 += gridDim.x * kPerRound) {
+            const unsigned e = e0 + (unsigned)sub;
+            if (e >= numEmpty) continue;
+            const int tile = (int)multiList[tiles + e];
+            const ViewGeom G = view_of_tile(V, tile);
+            const int tx = G.tile % G.tilesX, ty = G.tile / G.tilesX;
+            const int cols = min(16, G.width - tx * 16), rows = min(16, G.height - ty * 16);
+            const bool vec = cols == 16 && (S > 1 || ((G.width & 3) == 0 && (G.pixBase & 3) == 0));
+            float* const visDepth = visDepthAll + G.pixBase * S;
+            uint32_t* const visSlot = visSlotAll + G.pixBase * S;
+            constexpr int perRow = 16 * S / 4;  // lines per tile row
+            for (int i = lane; i < kLines; i += kLanes) {
+                const int row = i / perRow, q = i - row * perRow;
+                if (row >= rows) continue;
+                const size_t g = ((size_t)(ty * 16 + row) * G.width + tx * 16) * S + q * 4;
+                if (vec) {
+                    *reinterpret_cast<float4*>(&visDepth[g]) = make_float4(kClearDepth, kClearDepth, kClearDepth, kClearDepth);
+                    *reinterpret_cast<uint4*>(&visSlot[g]) = make_uint4(~0u, ~0u, ~0u, ~0u);
+                } else {
+#pragma unroll
+                    for (int e2 = 0; e2 < 4; e2++) {
+                        if ((q * 4 + e2) / S >= cols) continue;
+                        visDepth[g + e2] = kClearDepth;
+                        visSlot[g + e2] = 0xffffffffu;
+                    }
+                }
+            }
+            if (S == 1 && visClassAll) {
+                uint8_t* const visClass = visClassAll + G.pixBase;
+                for (int i = lane; i < 256; i += kLanes) {
+                    const int row = i >> 4, col = i & 15;
+                    if (row < rows && col < cols) visClass[(size_t)(ty * 16 + row) * G.width + tx * 16 + col] = 0;
+                }
+            }
+        }
+    }
+    // tiles whose list was cut into several items: the minimum over their partial blocks
+    for (unsigned m = blockIdx.x; m < numMulti; m += gridDim.x) {
+        const int tile = (int)multiList[m];
         const ViewGeom G = view_of_tile(V, tile);
         float* const visDepth = visDepthAll + G.pixBase * S;
         uint32_t* const visSlot = visSlotAll + G.pixBase * S;
         uint8_t* const visClass = visClassAll ? visClassAll + G.pixBase : nullptr;
-        if (empty) {
-            store_tile<S, true>(G, G.tile % G.tilesX, G.tile / G.tilesX, nullptr, visDepth, visSlot, visClass);
-            continue;
-        }
         const unsigned* const slotOf = slotOfAll + G.primBase;
         const PixelCtx C = pixel_ctx(G, t & 15, t >> 4);
         const unsigned start = binStart[tile];
@@ -603,10 +672,224 @@ __global__ void __launch_bounds__(256) k_raster_merge(RasterParams P, Views V, T
             __syncthreads();  // the previous tile's staging has been stored
             stage_pixel<S>(P, recs, slotOf, C, t, keys, O, visClass);
             __syncthreads();
-            store_tile<S, false>(G, C.tx, C.ty, &O, visDepth, visSlot, visClass);
+            store_tile<S>(G, C.tx, C.ty, &O, visDepth, visSlot);
         } else {
             write_pixel<S>(P, recs, slotOf, C, keys, visDepth, visSlot, visClass);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ordered raster: depth functions other than LESS (and the depth test off,
+// which is ALWAYS).  Their result depends on the order of the fragments, so
+// each tile's list is sorted by (prim * 4 + fan, entry), the order of the
+// reference's per-tile sort (TileRasterWorker.cpp:1225-1259), and every
+// thread folds the sorted fragments of its own pixel in that order
+// (TileRasterWorker.cpp:2272-2311 and the SIMD forms): the scalar path's
+// GREATER tests nothing (TileRasterWorker.cpp:2278-2280).  Lists longer than
+// the LDS buffer are taken in sorted windows of kOrderCap entries, each the
+// smallest keys above the previous window.
+// ---------------------------------------------------------------------------
+constexpr int kOrderCap = 2048;
+
+__device__ __forceinline__ bool depth_pass(int func, float d, float cur, bool scalarPath) {
+    switch (func) {
+        case 0: return false;                           // NEVER
+        case 1: return !(d >= cur);                     // LESS
+        case 2: return !(d != cur);                     // EQUAL
+        case 3: return !(d > cur);                      // LESS_OR_EQUAL
+        case 4: return scalarPath ? true : !(d <= cur); // GREATER
+        case 5: return !(d == cur);                     // NOT_EQUAL
+        case 6: return !(d < cur);                      // GREATER_OR_EQUAL
+        default: return true;                           // ALWAYS
+    }
+}
+
+// Coverage of one sample position for one triangle with the decisions of
+// the raster phases: the tile accepted, its 4x4 block accepted, or the block
+// not rejected and the position inside.
+__device__ __forceinline__ bool covers(const Edges& E, bool full, int tan, int trn, float x, float y) {
+    return full || tan == 3 || (trn == 3 && inside(E, x, y));
+}
+
+// Class of a covered 1-sample pixel (1 SIMD256, 2 SIMD128, 3 scalar): its
+// 4x2 group all inside, else its 2x2 quad all inside, else a single pixel.
+__device__ __forceinline__ unsigned pixel_class(const Edges& E, bool accepted, float fpx, float fpy, float bMinX) {
+    if (accepted) return 1u;
+    const float gy = floorf(fpy * 0.5f) * 2.0f;
+    bool g = true;
+#pragma unroll
+    for (int l = 0; l < 8; l++) g = g && inside(E, bMinX + (float)(l & 3), gy + (float)(l >> 2));
+    if (g) return 1u;
+    const float qx = floorf(fpx * 0.5f) * 2.0f;
+    bool q = true;
+#pragma unroll
+    for (int l = 0; l < 4; l++) q = q && inside(E, qx + (float)(l & 1), gy + (float)(l >> 1));
+    return q ? 2u : 3u;
+}
+
+// Ascending bitonic sort of n 64-bit keys in LDS (n a power of two).
+__device__ __forceinline__ void bitonic_sort(unsigned long long* a, int n) {
+    for (int k = 2; k <= n; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < n; i += 256) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const unsigned long long x = a[i], y = a[l];
+                    if ((x > y) == ((i & k) == 0)) {
+                        a[i] = y;
+                        a[l] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+// Sorts a bitonic sequence of n keys ascending.
+__device__ __forceinline__ void bitonic_merge(unsigned long long* a, int n) {
+    for (int j = n >> 1; j > 0; j >>= 1) {
+        for (int i = threadIdx.x; i < n; i += 256) {
+            const int l = i ^ j;
+            if (l > i) {
+                const unsigned long long x = a[i], y = a[l];
+                if (x > y) {
+                    a[i] = y;
+                    a[l] = x;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Sort key of a list entry: the record's primitive key, then the entry.
+__device__ __forceinline__ unsigned long long order_entry_key(const TriRecs& recs, unsigned ent) {
+    return ((unsigned long long)__float_as_uint(recs.q2[ent & 0x7fffffffu].y) << 32) | ent;
+}
+
+// One workgroup per tile (persistent over the tiles of all views), one
+// thread per pixel holding its S samples' depth and winning slot.
+template <int S>
+__global__ void __launch_bounds__(256) k_raster_ordered(RasterParams P, Views V, TriRecs recs,
+                                                        const unsigned* __restrict__ binStart,
+                                                        const unsigned* __restrict__ binCount,
+                                                        const unsigned* __restrict__ binList, int tiles,
+                                                        float* __restrict__ visDepthAll,
+                                                        uint32_t* __restrict__ visSlotAll,
+                                                        uint8_t* __restrict__ visClassAll, StatPartials st) {
+    __shared__ unsigned long long sA[kOrderCap], sB[kOrderCap];
+    const int t = threadIdx.x;
+    unsigned long long frags = 0, covs = 0;
+    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const ViewGeom G = view_of_tile(V, tile);
+        const PixelCtx C = pixel_ctx(G, t & 15, t >> 4);
+        float* const visDepth = visDepthAll + G.pixBase * S;
+        uint32_t* const visSlot = visSlotAll + G.pixBase * S;
+        uint8_t* const visClass = visClassAll ? visClassAll + G.pixBase : nullptr;
+        float dep[S];
+        unsigned slot[S];
+        unsigned cls = 0;
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            dep[s] = P.clear || !C.inFb ? kClearDepth : visDepth[C.pix * S + s];
+            slot[s] = P.clear || !C.inFb ? 0xffffffffu : visSlot[C.pix * S + s];
+        }
+        if (!P.clear && S == 1 && visClass && C.inFb) cls = visClass[C.pix];
+        const unsigned start = binStart[tile];
+        const unsigned cnt = start >= P.binCap ? 0u : min(binCount[tile], P.binCap - start);
+        unsigned long long last = 0;
+        bool haveLast = false;
+        for (;;) {
+            if (cnt == 0u) break;
+            // the window: the (up to) kOrderCap smallest keys above `last`, sorted in sA
+            int m;
+            if (cnt <= (unsigned)kOrderCap) {
+                int n2 = 64;
+                while (n2 < (int)cnt) n2 <<= 1;
+                for (int i = t; i < n2; i += 256) sA[i] = i < (int)cnt ? order_entry_key(recs, binList[start + i]) : ~0ull;
+                __syncthreads();
+                bitonic_sort(sA, n2);
+                m = (int)cnt;
+            } else {
+                for (int i = t; i < kOrderCap; i += 256) sA[i] = ~0ull;
+                for (unsigned b0 = 0; b0 < cnt; b0 += kOrderCap) {
+                    for (int i = t; i < kOrderCap; i += 256) {
+                        unsigned long long k = ~0ull;
+                        if (b0 + i < cnt) {
+                            k = order_entry_key(recs, binList[start + b0 + i]);
+                            if (haveLast && k <= last) k = ~0ull;
+                        }
+                        sB[i] = k;
+                    }
+                    __syncthreads();
+                    bitonic_sort(sB, kOrderCap);
+                    // the lower halves of two sorted runs form a bitonic sequence
+                    for (int i = t; i < kOrderCap; i += 256) sA[i] = min(sA[i], sB[kOrderCap - 1 - i]);
+                    __syncthreads();
+                    bitonic_merge(sA, kOrderCap);
+                }
+                // entries of the window: the keys below ~0 (sorted, so a count)
+                unsigned mine = 0;
+                for (int i = t; i < kOrderCap; i += 256) mine += sA[i] != ~0ull;
+                m = (int)block_sum256((unsigned long long)mine);
+            }
+            // fold the window in order
+            for (int i = 0; i < m; i++) {
+                const unsigned ent = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)sA[i]);
+                const unsigned sl = ent & 0x7fffffffu;
+                const bool full = (ent >> 31) != 0;
+                if (!C.inFb) continue;
+                const TriRec R = load_rec(recs, sl);
+                const Edges E = load_edges(R);
+                int trn, tan;
+                block_test(E, C.bMinX, C.bMinY, trn, tan);
+                if constexpr (S == 1) {
+                    if (!covers(E, full, tan, trn, C.fpx, C.fpy)) continue;
+                    const unsigned c = pixel_class(E, full || tan == 3, C.fpx, C.fpy, C.bMinX);
+                    const float d = class_depth(c, R.f1, R.f2, R.f3, C.fpx, C.fpy);
+                    frags++;
+                    covs++;
+                    if (!depth_pass(P.depthFunc, d, dep[0], c == 3u)) continue;
+                    dep[0] = d;
+                    slot[0] = sl;
+                    cls = c;
+                } else {
+                    bool any = false;
+#pragma unroll
+                    for (int s = 0; s < S; s++) {
+                        const float sx = C.fpx + P.sx[s], sy = C.fpy + P.sy[s];
+                        if (!covers(E, full, tan, trn, sx, sy)) continue;
+                        any = true;
+                        covs++;
+                        const float d = depth256(R.f1, R.f2, R.f3, sx, sy);
+                        if (!depth_pass(P.depthFunc, d, dep[s], false)) continue;
+                        dep[s] = d;
+                        slot[s] = sl;
+                    }
+                    frags += any;
+                }
+            }
+            const bool done = cnt <= (unsigned)kOrderCap || m < kOrderCap;
+            if (!done) last = sA[kOrderCap - 1];
+            __syncthreads();  // sA is reloaded by the next window or tile
+            if (done) break;
+            haveLast = true;
+        }
+        if (C.inFb) {
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                visDepth[C.pix * S + s] = dep[s];
+                visSlot[C.pix * S + s] = slot[s];
+            }
+            if (S == 1 && visClass) visClass[C.pix] = (uint8_t)cls;
+        }
+    }
+    const unsigned long long tf = block_sum256(frags);
+    const unsigned long long tc = block_sum256(covs);
+    if (t == 0) {
+        st.rasterFrags[blockIdx.x] = tf;
+        st.rasterCovs[blockIdx.x] = tc;
     }
 }
 
@@ -655,23 +938,44 @@ void launch_raster(const RasterParams& p, const Views& views, int samples, TriRe
                    const unsigned* binStart, const unsigned* binCount, const unsigned* binList, const RasterQueue& q,
                    float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st, DeviceCounters* ctr,
                    hipStream_t s) {
-#define IFRIT_RASTER_LAUNCH(SN)                                                                                    \
-    hipLaunchKernelGGL(k_raster<SN>, dim3(q.grid), dim3(256), 0, s, p, views, recs, slotOf, binStart, binCount,  \
-                       binList, q.items, q.itemCap, q.partShift, q.tilePartial, q.partials, visDepth, visSlot,     \
-                       visClass, st, ctr)
+    (void)binStart;
+    (void)binCount;
+#define IFRIT_RASTER_LAUNCH(SN, NT)                                                                                 \
+    hipLaunchKernelGGL((k_raster<SN, NT>), dim3(q.grid), dim3(NT), 0, s, p, views, recs, slotOf, binList, q.items, \
+                       q.itemCap, q.partials, visDepth, visSlot, visClass, st, ctr)
     switch (samples) {
-        case 1: IFRIT_RASTER_LAUNCH(1); break;
-        case 4: IFRIT_RASTER_LAUNCH(4); break;
-        default: IFRIT_RASTER_LAUNCH(8); break;
+        case 1:
+            if (q.tileThreads == 64)
+                IFRIT_RASTER_LAUNCH(1, 64);
+            else
+                IFRIT_RASTER_LAUNCH(1, 256);
+            break;
+        case 4: IFRIT_RASTER_LAUNCH(4, 256); break;
+        default: IFRIT_RASTER_LAUNCH(8, 256); break;
     }
 #undef IFRIT_RASTER_LAUNCH
+}
+
+void launch_raster_ordered(const RasterParams& p, const Views& views, int samples, TriRecs recs,
+                           const unsigned* binStart, const unsigned* binCount, const unsigned* binList, int tiles,
+                           int grid, float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st,
+                           hipStream_t s) {
+#define IFRIT_ORDERED_LAUNCH(SN)                                                                                  \
+    hipLaunchKernelGGL(k_raster_ordered<SN>, dim3(grid), dim3(256), 0, s, p, views, recs, binStart, binCount,     \
+                       binList, tiles, visDepth, visSlot, visClass, st)
+    switch (samples) {
+        case 1: IFRIT_ORDERED_LAUNCH(1); break;
+        case 4: IFRIT_ORDERED_LAUNCH(4); break;
+        default: IFRIT_ORDERED_LAUNCH(8); break;
+    }
+#undef IFRIT_ORDERED_LAUNCH
 }
 
 void launch_raster_merge(const RasterParams& p, const Views& views, int samples, TriRecs recs, const unsigned* slotOf,
                          const unsigned* binStart, const unsigned* binCount, const RasterQueue& q, float* visDepth,
                          uint32_t* visSlot, uint8_t* visClass, int tiles, DeviceCounters* ctr, hipStream_t s) {
 #define IFRIT_MERGE_LAUNCH(SN)                                                                                     \
-    hipLaunchKernelGGL(k_raster_merge<SN>, dim3(q.grid), dim3(256), 0, s, p, views, recs, slotOf, binStart,        \
+    hipLaunchKernelGGL(k_raster_merge<SN>, dim3(2 * q.mergeGrid), dim3(256), 0, s, p, views, recs, slotOf, binStart, \
                        binCount, q.multiList, q.tilePartial, q.partials, visDepth, visSlot, visClass, tiles,       \
                        q.partShift, ctr)
     switch (samples) {

@@ -68,7 +68,7 @@ class ViewDesc(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("rasterPartEntries", C.c_int32), ("rasterGroupsPerCU", C.c_int32), ("shadeTile", C.c_int32),
-                ("bigTriangleTiles", C.c_int32), ("binCapacity", C.c_uint32)]
+                ("bigTriangleTiles", C.c_int32), ("binCapacity", C.c_uint32), ("rasterTileThreads", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -202,6 +202,11 @@ class Renderer:
         e = self.L.iftrGetLastError(self.h)
         if e:
             raise IfritError(e.decode())
+
+    def set_depth_func(self, func: int):
+        """IF_COMPARE_OP_* (include/ifrit_hip.h); LESS (1) is the default."""
+        self.L.iftrSetDepthFunc(self.h, func)
+        self._check()
 
     def set_target(self, width: int, height: int, samples: int = 1):
         self.L.iftrSetRenderTarget(self.h, width, height)

@@ -199,7 +199,8 @@ typedef struct {
     int64_t coveredSamples;
     uint32_t status;       /* overflow bits, 0 when the frame is complete */
     uint32_t retries;      /* frames re-run with larger buffers */
-    float stageMs[8];      /* vertex, reset, setup, bin, raster, raster merge, shade, (unused) */
+    float stageMs[8];      /* vertex (and counter reset), setup (clip, cull), bin, raster, raster merge,
+                              shade, (unused), (unused) */
 } IfritFrameStats;
 
 void iftrGetFrameStats(IfritRendererHandle* h, IfritFrameStats* out);
@@ -274,6 +275,8 @@ typedef struct {
     int32_t shadeTile;                /* shading tile side in pixels: 8, 16 or 32 */
     int32_t bigTriangleTiles;         /* triangles over more tiles are binned by a whole workgroup */
     uint32_t binCapacity;             /* bin-list entries reserved (grown on demand when exceeded) */
+    int32_t rasterTileThreads;        /* threads per 16x16 tile at 1 sample per pixel: 256, or 64 (one
+                                         wave per tile, more tiles in flight); other sample counts use 256 */
 } IfritTuning;
 void iftrSetTuning(IfritRendererHandle* h, const IfritTuning* tuning);
 void iftrGetTuning(IfritRendererHandle* h, IfritTuning* tuning);

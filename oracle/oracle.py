@@ -81,6 +81,9 @@ def lib():
         _lib.oracle_get_tris.argtypes = [C.c_void_p, C.c_void_p]
         _lib.oracle_render.argtypes = [C.c_void_p, C.POINTER(RasterConfig), C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Stats)]
+        _lib.oracle_render_blend.argtypes = [C.c_void_p, C.POINTER(RasterConfig), C.c_void_p, C.c_int, C.c_void_p,
+                                             C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.c_void_p]
         _lib.oracle_vertex_transform.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
         _lib.oracle_shade.argtypes = [C.c_void_p, C.POINTER(ShadeParams), C.c_int, C.c_int, C.c_int, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]
@@ -104,6 +107,7 @@ def ref():
         _ref.ref_render_timed.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                           C.c_void_p, C.c_int, C.c_void_p]
         _ref.ref_render_timed.restype = C.c_double
+        _ref.ref_render_blend.argtypes = [C.c_int] * 7 + [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
     return _ref
 
 
@@ -155,6 +159,28 @@ class Frame:
             raise RuntimeError(f"oracle_render failed ({rc})")
         self.width, self.height, self.samples = width, height, samples
         self.ids, self.depth, self.tri, self.cls, self.stats = ids, depth, tri, cls, st.as_dict()
+        return self
+
+    def render_blend(self, width, height, clip, indices, factors, depth_func=1, threads=None, clear=True,
+                     depth_in=None, rgba_in=None):
+        """Blending of the translucent flat colour in primitive order (1 sample per pixel);
+        factors = (src colour, dst colour, src alpha, dst alpha) as IfritBlendFactor values."""
+        clip = np.ascontiguousarray(clip, np.float32)
+        indices = np.ascontiguousarray(indices, np.int32)
+        n = width * height
+        ids = np.empty((n,), np.uint32)
+        depth = np.empty((n,), np.float32)
+        rgba = np.empty((n, 4), np.float32)
+        cfg = RasterConfig(width, height, 1, 0, depth_func, 1 if clear else 0, threads or os.cpu_count() or 1, 0)
+        bl = (C.c_int * 4)(*factors)
+        din = None if depth_in is None else np.ascontiguousarray(depth_in, np.float32)
+        cin = None if rgba_in is None else np.ascontiguousarray(rgba_in, np.float32)
+        rc = lib().oracle_render_blend(self._h, C.byref(cfg), _p(clip), clip.shape[0], _p(indices), indices.size, bl,
+                                       None if din is None else _p(din), None if cin is None else _p(cin), _p(ids),
+                                       _p(depth), _p(rgba))
+        if rc != 0:
+            raise RuntimeError(f"oracle_render_blend failed ({rc})")
+        self.ids, self.depth, self.rgba = ids, depth, rgba.reshape(height, width, 4)
         return self
 
     def tris(self) -> np.ndarray:
@@ -222,6 +248,18 @@ def ggx_light(n, v, l, albedo, alpha, kd, ks):
     a = [np.ascontiguousarray(x, np.float64) for x in (n, v, l, albedo)]
     out = np.zeros(3, np.float64)
     f(*[_p(x) for x in a], C.c_double(alpha), C.c_double(kd), C.c_double(ks), _p(out))
+    return out
+
+
+def ref_blend(width, height, clip, indices, factors, depth_func=1):
+    """FP32 colour attachment of the reference build drawing the translucent flat colour with blending."""
+    clip = np.ascontiguousarray(clip, np.float32)
+    indices = np.ascontiguousarray(indices, np.int32)
+    out = np.empty((height, width, 4), np.float32)
+    rc = ref().ref_render_blend(width, height, depth_func, *[int(f) for f in factors], _p(clip), _p(indices),
+                                indices.size, _p(out))
+    if rc != 0:
+        raise RuntimeError(f"ref_render_blend failed ({rc})")
     return out
 
 

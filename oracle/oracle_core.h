@@ -81,6 +81,16 @@ int oracle_render(OracleFrame* f, const OracleRasterConfig* cfg, const float* cl
                   const int32_t* indices, int nindices, const float* depth_in, uint32_t* id_out, float* depth_out,
                   uint32_t* tri_out, uint8_t* class_out, OracleStats* stats);
 
+// Blending (TileRasterRenderer.cpp:212-333, TileRasterWorker.cpp:2346-2373) of
+// the translucent flat colour (a hash of the primitive index) in primitive
+// order, one sample per pixel; factors are IfritBlendFactor values.
+typedef struct {
+    int src_color, dst_color, src_alpha, dst_alpha;
+} OracleBlend;
+int oracle_render_blend(OracleFrame* f, const OracleRasterConfig* cfg, const float* clip_pos, int nverts,
+                        const int32_t* indices, int nindices, const OracleBlend* blend, const float* depth_in,
+                        const float* rgba_in, uint32_t* id_out, float* depth_out, float* rgba_out);
+
 int oracle_num_tris(const OracleFrame* f);
 // Assembled triangles sorted by (prim, fan); index = position in that order.
 void oracle_get_tris(const OracleFrame* f, OracleTri* out);

@@ -11,7 +11,10 @@
 //   ref_render_visibility  per-pixel primitive ID and depth of the winning
 //                          fragment for given clip-space vertices;
 //   ref_render_timed       a full frame (vertex transform + Blinn-Phong-like
-//                          shading) timed with the reference's own 17 threads.
+//                          shading) timed with the reference's own 17 threads;
+//   ref_render_blend       the FP32 colour attachment of a draw with blending
+//                          and a given depth function, each triangle coloured
+//                          by a hash of its index (colour and alpha).
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -55,6 +58,27 @@ public:
         c->y = *depth;
         c->z = 1.0f;
         c->w = 1.0f;
+    }
+};
+
+// Colour and alpha from a hash of the primitive index carried in varying 0
+// (the same hash and byte layout as the GPU's translucent flat program):
+// rgb = bytes 0-2 / 255, alpha = (64 + (byte 3 & 127)) / 255.
+class FlatAlphaFS : public FragmentShader {
+public:
+    void execute(const void* vary, void* color, float* depth) override {
+        (void)depth;
+        const Vector4f v = reinterpret_cast<const Vector4f*>(vary)[0];
+        const uint32_t prim = (uint32_t)std::nearbyint(v.x);
+        uint32_t h = prim * 0x9E3779B1u;
+        h ^= h >> 15;
+        h *= 0x85EBCA77u;
+        h ^= h >> 13;
+        Vector4f* c = reinterpret_cast<Vector4f*>(color);
+        c->x = (float)(h & 0xffu) * (1.0f / 255.0f);
+        c->y = (float)((h >> 8) & 0xffu) * (1.0f / 255.0f);
+        c->z = (float)((h >> 16) & 0xffu) * (1.0f / 255.0f);
+        c->w = (float)(64u + ((h >> 24) & 0x7fu)) * (1.0f / 255.0f);
     }
 };
 
@@ -160,6 +184,52 @@ extern "C" int ref_render_visibility(int W, int H, int ccw, int depthFunc, const
         }
     }
     return bad;
+}
+
+extern "C" int ref_render_blend(int W, int H, int depthFunc, int srcColor, int dstColor, int srcAlpha, int dstAlpha,
+                                const float* clip, const int32_t* idx, int nidx, float* rgba_out) {
+    const int ntri = nidx / 3;
+    if (ntri <= 0) return -1;
+    Target tgt(W, H);
+    VertexBuffer vb;
+    vb.setLayout({TypeDescriptors.FLOAT4, TypeDescriptors.FLOAT4});
+    vb.setVertexCount(3 * ntri);
+    vb.allocateBuffer(3 * ntri);
+    for (int p = 0; p < ntri; p++) {
+        for (int k = 0; k < 3; k++) {
+            const float* c = clip + 4 * idx[3 * p + k];
+            vb.setValue<Vector4f>(3 * p + k, 0, Vector4f{c[0], c[1], c[2], c[3]});
+            vb.setValue<Vector4f>(3 * p + k, 1, Vector4f{(float)p, 0.0f, 0.0f, 0.0f});
+        }
+    }
+    std::vector<int> seq(3 * ntri);
+    for (int i = 0; i < 3 * ntri; i++) seq[i] = i;
+    auto mgr = std::make_shared<BufferManager::TrivialBufferManager>();
+    mgr->Init();
+    auto ib = mgr->CreateBuffer({seq.size() * sizeof(int)});
+    mgr->bufferData(ib, seq.data(), 0, seq.size() * sizeof(int));
+    PassVS vs;
+    FlatAlphaFS fs;
+    VaryingDescriptor vd;
+    vd.setVaryingDescriptors({TypeDescriptors.FLOAT4});
+    auto r = std::make_shared<TileRasterRenderer>();
+    r->Init();
+    r->bindFrameBuffer(tgt.fb);
+    r->bindVertexBuffer(vb);
+    r->bindIndexBuffer(ib);
+    r->bindVertexShaderLegacy(vs, vd);
+    r->bindFragmentShader(fs);
+    r->SetDepthFunc((IfritCompareOp)depthFunc);
+    IfritColorAttachmentBlendState bs{};
+    bs.blendEnable = true;
+    bs.srcColorBlendFactor = (IfritBlendFactor)srcColor;
+    bs.dstColorBlendFactor = (IfritBlendFactor)dstColor;
+    bs.srcAlphaBlendFactor = (IfritBlendFactor)srcAlpha;
+    bs.dstAlphaBlendFactor = (IfritBlendFactor)dstAlpha;
+    r->setBlendFunc(bs);
+    r->drawElements(3 * ntri, true);
+    std::memcpy(rgba_out, tgt.color->getData(), sizeof(float) * 4 * (size_t)W * H);
+    return 0;
 }
 
 extern "C" double ref_render_timed(int W, int H, const float* verts, int stride, int nverts, const int32_t* idx,

@@ -187,7 +187,7 @@ def test_batch_calls_without_device_report_errors(lib):
     views[0].width, views[0].height = 0, 10  # invalid size
     lib.iftrSetViews(h, C.cast(views, C.c_void_p), 2)
     assert lib.iftrGetLastError(h)
-    t = ifrit_hip.Tuning(300, 4, 8, 64, 0)  # 300 is not a permitted part size
+    t = ifrit_hip.Tuning(300, 4, 8, 64, 0, 256)  # 300 is not a permitted part size
     lib.iftrSetTuning(h, C.byref(t))
     assert lib.iftrGetLastError(h)
     lib.iftrGetTuning(h, C.byref(t))

@@ -130,7 +130,8 @@ def test_tuning_never_changes_results(renderer_factory):
     r.draw()
     ref = r.digest()
     for kw in (dict(shadeTile=16), dict(shadeTile=32), dict(rasterPartEntries=256), dict(rasterPartEntries=1024),
-               dict(rasterGroupsPerCU=2), dict(rasterGroupsPerCU=8), dict(bigTriangleTiles=16)):
+               dict(rasterGroupsPerCU=2), dict(rasterGroupsPerCU=8), dict(bigTriangleTiles=16),
+               dict(rasterTileThreads=64), dict(rasterTileThreads=64, rasterPartEntries=256)):
         r.set_tuning(**kw)
         r.draw()
         d = r.digest()

@@ -154,3 +154,65 @@ def test_vertex_transform_close_to_fp64():
     exact = v[:, :3].astype(np.float64) @ m[:, :3].T.astype(np.float64) + m[:, 3].astype(np.float64)
     scale = np.abs(v[:, :3]).astype(np.float64) @ np.abs(m[:, :3]).T + np.abs(m[:, 3])
     assert np.all(np.abs(out - exact) <= 4 * np.finfo(np.float32).eps * scale + 1e-30)
+
+
+DEPTH_FUNCS = {0: "NEVER", 1: "LESS", 2: "EQUAL", 3: "LESS_OR_EQUAL", 4: "GREATER", 5: "NOT_EQUAL",
+               6: "GREATER_OR_EQUAL", 7: "ALWAYS"}
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("func", sorted(DEPTH_FUNCS))
+def test_oracle_depth_functions_match_reference_build(func):
+    """Every depth function in primitive order, including the reference's scalar-path GREATER that
+    tests nothing (TileRasterWorker.cpp:2278-2280): the last passing fragment of each pixel."""
+    for seed in range(12):
+        W, H, clip, idx = _random_clip_scene(7000 + seed, seed % 4)
+        ids, depth, bad = O.ref_visibility(W, H, clip, idx, depth_func=func)
+        assert bad == 0
+        f = O.Frame().render(W, H, 1, clip, idx, depth_func=func, threads=4)
+        _assert_same(f"{DEPTH_FUNCS[func]} seed {seed}", f.ids, f.depth, ids, depth)
+
+
+# IfritBlendFactor (include/ifrit/softgraphics/engine/base/Constants.h:64-76)
+BF_ZERO, BF_ONE, BF_SRC_ALPHA, BF_ONE_MINUS_SRC_ALPHA, BF_DST_ALPHA, BF_ONE_MINUS_DST_ALPHA = 0, 1, 6, 7, 8, 9
+BLENDS = [(BF_SRC_ALPHA, BF_ONE_MINUS_SRC_ALPHA, BF_ONE, BF_ZERO), (BF_ONE, BF_ONE, BF_ONE, BF_ONE),
+          (BF_ONE_MINUS_SRC_ALPHA, BF_SRC_ALPHA, BF_DST_ALPHA, BF_ONE_MINUS_DST_ALPHA),
+          (BF_DST_ALPHA, BF_ONE_MINUS_DST_ALPHA, BF_ZERO, BF_ONE)]
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("blend", range(len(BLENDS)))
+@pytest.mark.parametrize("func", [1, 7])
+def test_oracle_blending_matches_reference_build(blend, func):
+    """Blending of translucent, hash-coloured triangles in primitive order: the FP32 colour
+    attachment bit for bit, including the reference's encoding of the alpha factors
+    (TileRasterRenderer.cpp:212-333)."""
+    for seed in range(8):
+        W, H, clip, idx = _random_clip_scene(9000 + seed, seed % 4)
+        want = O.ref_blend(W, H, clip, idx, BLENDS[blend], depth_func=func)
+        f = O.Frame().render_blend(W, H, clip, idx, BLENDS[blend], depth_func=func, threads=4)
+        bad = np.flatnonzero((f.rgba.view(np.uint32) != want.view(np.uint32)).any(-1))
+        assert bad.size == 0, f"seed {seed}: {bad.size} pixels differ, first {bad[:4]}"
+        assert (want != 0).any()
+
+
+def test_blend_factor_encoding_closed_forms():
+    """One translucent screen-covering triangle (primitive 1) over the cleared (0, 0, 0, 0)
+    target, closed forms of the reference's factor encoding: the factor named
+    ONE_MINUS_SRC_ALPHA gives src alpha, SRC_ALPHA gives 1 - src alpha, DST_ALPHA gives
+    1 - dst alpha = 1."""
+    clip = np.array([[5, 5, 0.5, 1], [5, 5, 0.5, 1], [5, 5, 0.5, 1],  # primitive 0: degenerate, culled
+                     [-1, -1, 0.5, 1], [3, -1, 0.5, 1], [-1, 3, 0.5, 1]], np.float32)
+    idx = np.arange(6, dtype=np.int32)
+    h = O.flat_hash(1)
+    src = np.array([h & 0xFF, (h >> 8) & 0xFF, (h >> 16) & 0xFF], np.float32) * np.float32(1.0 / 255.0)
+    a = np.float32(64 + ((h >> 24) & 0x7F)) * np.float32(1.0 / 255.0)
+    assert src.any() and 0.25 <= a <= 0.75
+    f = O.Frame().render_blend(8, 8, clip, idx, (BF_ONE_MINUS_SRC_ALPHA, BF_ONE, BF_ONE, BF_ZERO))
+    m = (f.ids == 1).reshape(8, 8)
+    assert m.sum() >= 40 and (f.rgba[~m] == 0).all()
+    assert np.allclose(f.rgba[m][:, :3], src * a, rtol=1e-6) and np.allclose(f.rgba[m][:, 3], a)
+    f = O.Frame().render_blend(8, 8, clip, idx, (BF_SRC_ALPHA, BF_ONE, BF_ZERO, BF_ONE))
+    assert np.allclose(f.rgba[m][:, :3], src * (1 - a), rtol=1e-6) and (f.rgba[..., 3] == 0).all()
+    f = O.Frame().render_blend(8, 8, clip, idx, (BF_DST_ALPHA, BF_ZERO, BF_ONE_MINUS_DST_ALPHA, BF_ZERO))
+    assert np.allclose(f.rgba[m][:, :3], src, rtol=1e-6) and (f.rgba[..., 3] == 0).all()
