@@ -307,6 +307,12 @@ void iftrNeuralEval(IfritRendererHandle* h, const float* in, float* out, int n, 
     guarded(h, [&](TileRasterRenderer& r) { r.evalNeural(in, out, n, precision); });
 }
 
+void iftrSetColorFloatOutput(IfritRendererHandle* h, int enable) {
+    guarded(h, [&](TileRasterRenderer& r) { r.setColorFloatOutput(enable != 0); });
+}
+void iftrGetColorRGBA32F(IfritRendererHandle* h, float* rgba) {
+    if (rgba) guarded(h, [&](TileRasterRenderer& r) { r.getColorRGBA32F(rgba); });
+}
 void iftrSetViews(IfritRendererHandle* h, const IfritViewDesc* views, int numViews) {
     guarded(h, [&](TileRasterRenderer& r) { r.setViews(views, numViews); });
 }

@@ -185,9 +185,38 @@ void TileRasterRenderer::bindFragmentShader(const GpuFragmentProgram& p) {
     hostFs_ = false;
 }
 
+namespace {
+// A blend factor as the reference encodes it (TileRasterRenderer.cpp:212-333):
+// (c0 + c1 (1 - src alpha) + c2 (1 - dst alpha)) (1 - c3).  The encoding makes
+// SRC_ALPHA 1 - src alpha and ONE_MINUS_SRC_ALPHA src alpha (likewise DST);
+// it is kept so that results match the reference.  False for the factors the
+// reference rejects.
+bool encode_blend_factor(int32_t f, float* c) {
+    switch (f) {
+        case 0: c[0] = 0; c[1] = 0; c[2] = 0; c[3] = 1; return true;   // ZERO
+        case 1: c[0] = 1; c[1] = 0; c[2] = 0; c[3] = 0; return true;   // ONE
+        case 6: c[0] = 0; c[1] = 1; c[2] = 0; c[3] = 0; return true;   // SRC_ALPHA
+        case 7: c[0] = 1; c[1] = -1; c[2] = 0; c[3] = 0; return true;  // ONE_MINUS_SRC_ALPHA
+        case 8: c[0] = 0; c[1] = 0; c[2] = 1; c[3] = 0; return true;   // DST_ALPHA
+        case 9: c[0] = 1; c[1] = 0; c[2] = -1; c[3] = 0; return true;  // ONE_MINUS_DST_ALPHA
+        default: return false;
+    }
+}
+}  // namespace
+
 void TileRasterRenderer::setBlendFunc(const IfritColorAttachmentBlendState& s) {
+    BlendCoefs b{};
+    if (s.blendEnable && (!encode_blend_factor(s.srcColorBlendFactor, b.sc) ||
+                          !encode_blend_factor(s.dstColorBlendFactor, b.dc) ||
+                          !encode_blend_factor(s.srcAlphaBlendFactor, b.sa) ||
+                          !encode_blend_factor(s.dstAlphaBlendFactor, b.da))) {
+        // the reference stops with "Unsupported blend factor" (TileRasterRenderer.cpp:241)
+        setError("unsupported blend factor: ZERO, ONE, SRC_ALPHA, ONE_MINUS_SRC_ALPHA, DST_ALPHA and "
+                 "ONE_MINUS_DST_ALPHA are supported");
+        return;
+    }
     blend_ = s;
-    if (s.blendEnable) setError("alpha blending is not supported by the GPU backend");
+    blendCoefs_ = b;
 }
 
 void TileRasterRenderer::SetDepthFunc(int f) {
@@ -310,8 +339,12 @@ bool TileRasterRenderer::prepare(int indexCount) {
         setError("unknown depth function");
         return false;
     }
-    if (blend_.blendEnable) {
-        setError("alpha blending is not supported by the GPU backend");
+    const int fk = fs_.desc.kind;
+    if (blend_.blendEnable && (samples_ != 1 || !batch_.empty() || !fsBound_ ||
+                               (fk != IFHIP_FS_FLAT && fk != IFHIP_FS_FLAT_ALPHA && fk != IFHIP_FS_BLINN_PHONG &&
+                                fk != IFHIP_FS_GGX_LAYERED))) {
+        setError("blending draws one target at 1 sample per pixel with the flat, translucent flat, Blinn-Phong or "
+                 "layered-GGX program");
         return false;
     }
     // inputs
@@ -355,7 +388,8 @@ bool TileRasterRenderer::prepare(int indexCount) {
         setError("the neural material has not been trained (iftrNeuralTrain) or its textures are not bound");
         return false;
     }
-    if (fsBound_ && fs_.desc.kind >= IFHIP_FS_GGX_LAYERED && (stride_ < 11 || fs_.desc.offTangent + 3 > stride_)) {
+    if (fsBound_ && (fs_.desc.kind == IFHIP_FS_GGX_LAYERED || fs_.desc.kind == IFHIP_FS_NEURAL) &&
+        (stride_ < 11 || fs_.desc.offTangent + 3 > stride_)) {
         setError("the layered-GGX and neural programs need a tangent in the vertex record");
         return false;
     }
@@ -379,7 +413,8 @@ bool TileRasterRenderer::prepare(int indexCount) {
     itemCap_ = std::max<unsigned>(itemCap_, (unsigned)tiles + binCap_ / part + 1u);
     partialCap_ = std::max<unsigned>(partialCap_, std::max(256u, binCap_ / (8u * part)));
     const size_t slots = ((size_t)ntri + clipCap_) * nv;
-    const bool wantFloat = batch_.empty() && fb_ && !fb_->color.empty() && fb_->color[0];
+    const bool wantFloat = batch_.empty() && ((fb_ && !fb_->color.empty() && fb_->color[0]) || floatOut_ ||
+                                              blend_.blendEnable);
     bool okAll = clip_.ensure((size_t)nverts_ * 16 * nv) && recs_[0].ensure(slots * 16) &&
                  recs_[1].ensure(slots * 16) && recs_[2].ensure(slots * 16) && recs_[3].ensure(slots * 16) &&
                  recs_[4].ensure(slots * 16) && recs_[5].ensure(slots * 16) && bary_.ensure(slots * 32) &&
@@ -539,6 +574,36 @@ int TileRasterRenderer::rasterGrid() const {
     return numCUs_ * rasterGroupsPerCU_ * (samples_ == 1 && tileThreads_ == 64 ? 4 : 1);
 }
 
+// Uniform block of the bound fragment program for the shading kernels.
+ShadeParams TileRasterRenderer::shadeParams(bool writeFloat) const {
+    ShadeParams sh{};
+    const IfritGpuFragmentProgramDesc& d = fs_.desc;
+    sh.samples = samples_;
+    // the neural program falls back to the layered-GGX material when validation found it below its bound
+    sh.program = (d.kind == IFHIP_FS_NEURAL && nnFallback_[d.precision == 1 ? 1 : 0]) ? IFHIP_FS_GGX_LAYERED : d.kind;
+    sh.stride = stride_;
+    sh.offWorld = d.offWorld;
+    sh.offNormal = d.offNormal;
+    sh.offUv = d.offUv;
+    sh.offTangent = d.offTangent;
+    sh.trisPerMaterial = d.trisPerMaterial;
+    sh.sampler = d.sampler;
+    sh.maxAniso = std::min(16, std::max(1, d.maxAniso));  // the tap list encodes up to 16 taps
+    sh.numTextures = (int)textures_.size();
+    sh.ambient = d.ambient;
+    sh.kd = d.kd;
+    sh.ks = d.ks;
+    sh.shininess = d.shininess;
+    sh.numLights = std::min(4, std::max(0, d.numLights));
+    std::memcpy(sh.lightPos, d.lightPos, sizeof(sh.lightPos));
+    std::memcpy(sh.lightColor, d.lightColor, sizeof(sh.lightColor));
+    sh.writeFloat = writeFloat ? 1 : 0;
+    sh.ntri = curTris_;
+    sh.shadeTile = shadeTile_;
+    sh.neural = neuralShadeParams();
+    return sh;
+}
+
 void TileRasterRenderer::enqueue(bool clear) {
     const int tiles = (int)totalTiles_;
     const Views V = deviceViews();
@@ -586,7 +651,15 @@ void TileRasterRenderer::enqueue(bool clear) {
     }
     // the 1-sample coverage class is a single-view diagnostic
     uint8_t* cls = batch_.empty() ? visClass_.as<uint8_t>() : nullptr;
-    if (depthFunc_ == IF_COMPARE_OP_LESS) {
+    if (blend_.blendEnable) {
+        // blending: fragments shaded and blended in primitive order (the shade stage is part of it)
+        launch_blend_ordered(shadeParams(true), blendCoefs_, depthFunc_, clear ? 1 : 0, binCap_, V, recPtrs(),
+                             bary_.as<float4>(), vtx_.as<float>(), idx_.as<int>(), texDesc_.as<TexDesc>(), nullptr,
+                             binStart_.as<unsigned>(), binCount_.as<unsigned>(), binList_.as<unsigned>(), tiles,
+                             rasterGrid(), visDepth_.as<float>(), visSlot_.as<uint32_t>(), cls, rgba8_.as<uchar4>(),
+                             rgba32f_.as<float4>(), statPtrs(), s);
+        mark(4);
+    } else if (depthFunc_ == IF_COMPARE_OP_LESS) {
         // order-independent: the lexicographic minimum of (depth, primitive)
         launch_raster(rp, V, samples_, recPtrs(), slotOf_.as<unsigned>(), binStart_.as<unsigned>(),
                       binCount_.as<unsigned>(), binList_.as<unsigned>(), q, visDepth_.as<float>(),
@@ -603,33 +676,9 @@ void TileRasterRenderer::enqueue(bool clear) {
         mark(4);
     }
     mark(5);
-    if (fsBound_ && fs_.desc.kind != IFHIP_FS_DEPTH_ONLY && shadeTiles_ > 0) {
-        ShadeParams sh{};
-        const IfritGpuFragmentProgramDesc& d = fs_.desc;
-        sh.samples = samples_;
-        // the neural program falls back to the layered-GGX material when validation found it below its bound
-        sh.program = (d.kind == IFHIP_FS_NEURAL && nnFallback_[d.precision == 1 ? 1 : 0]) ? IFHIP_FS_GGX_LAYERED : d.kind;
-        sh.stride = stride_;
-        sh.offWorld = d.offWorld;
-        sh.offNormal = d.offNormal;
-        sh.offUv = d.offUv;
-        sh.offTangent = d.offTangent;
-        sh.trisPerMaterial = d.trisPerMaterial;
-        sh.sampler = d.sampler;
-        sh.maxAniso = std::min(16, std::max(1, d.maxAniso));  // the tap list encodes up to 16 taps
-        sh.numTextures = (int)textures_.size();
-        sh.ambient = d.ambient;
-        sh.kd = d.kd;
-        sh.ks = d.ks;
-        sh.shininess = d.shininess;
-        sh.numLights = std::min(4, std::max(0, d.numLights));
-        std::memcpy(sh.lightPos, d.lightPos, sizeof(sh.lightPos));
-        std::memcpy(sh.lightColor, d.lightColor, sizeof(sh.lightColor));
-        const bool wantFloat = batch_.empty() && fb_ && !fb_->color.empty() && fb_->color[0];
-        sh.writeFloat = wantFloat ? 1 : 0;
-        sh.ntri = curTris_;
-        sh.shadeTile = shadeTile_;
-        sh.neural = neuralShadeParams();
+    if (fsBound_ && fs_.desc.kind != IFHIP_FS_DEPTH_ONLY && shadeTiles_ > 0 && !blend_.blendEnable) {
+        const bool wantFloat = batch_.empty() && ((fb_ && !fb_->color.empty() && fb_->color[0]) || floatOut_);
+        const ShadeParams sh = shadeParams(wantFloat);
         launch_shade(sh, V, (int)shadeTiles_, recPtrs(), bary_.as<float4>(), visSlot_.as<uint32_t>(),
                      vtx_.as<float>(), idx_.as<int>(), texDesc_.as<TexDesc>(), nullptr, rgba8_.as<uchar4>(),
                      wantFloat ? rgba32f_.as<float4>() : nullptr, s);
@@ -649,11 +698,13 @@ std::vector<uint8_t> TileRasterRenderer::graphKey(bool clear) const {
                         (int)totalTiles_, (int)shadeTiles_, vs_.desc.kind, fsBound_ ? 1 : 0, hostVs_ ? 1 : 0,
                         partShift_, rasterGroupsPerCU_, shadeTile_, bigThreshold_, nnFallback_[0] ? 1 : 0,
                         nnFallback_[1] ? 1 : 0, (int)textures_.size(), stageTiming_ ? 1 : 0,
-                        fb_ && !fb_->color.empty() && fb_->color[0] ? 1 : 0, batch_.empty() ? 1 : 0};
+                        fb_ && !fb_->color.empty() && fb_->color[0] ? 1 : 0, batch_.empty() ? 1 : 0,
+                        floatOut_ ? 1 : 0, blend_.blendEnable ? 1 : 0};
     put(ints, sizeof(ints));
     const unsigned caps[] = {clipCap_, binCap_, bigCap_, itemCap_, partialCap_};
     put(caps, sizeof(caps));
     put(&totalPixels_, sizeof(totalPixels_));
+    put(&blendCoefs_, sizeof(blendCoefs_));
     put(&DevBuf::generation, sizeof(DevBuf::generation));
     IfritGpuFragmentProgramDesc fd = fs_.desc;
     std::memset(fd.camera, 0, sizeof(fd.camera));  // cameras live in the view table
@@ -819,6 +870,12 @@ void TileRasterRenderer::getCoverageClass(uint8_t* cls) {
 }
 
 void TileRasterRenderer::getColorRGBA8(uint8_t* rgba) { getViewColorRGBA8(0, rgba); }
+
+void TileRasterRenderer::getColorRGBA32F(float* rgba) {
+    if (!batch_.empty() || (!floatOut_ && !blend_.blendEnable && !(fb_ && !fb_->color.empty() && fb_->color[0])))
+        return setError("FP32 colour is written for a single target with iftrSetColorFloatOutput or blending");
+    ok(hipMemcpy(rgba, rgba32f_.ptr, (size_t)width_ * height_ * 16, hipMemcpyDeviceToHost), "colour readback");
+}
 
 void TileRasterRenderer::setViews(const IfritViewDesc* views, int n) {
     if (n < 0 || n > kMaxViews || (n > 0 && !views)) return setError("bad view count");

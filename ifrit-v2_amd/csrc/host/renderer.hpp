@@ -122,6 +122,8 @@ public:
     void getVisibility(uint32_t* ids, float* depth);
     void getCoverageClass(uint8_t* cls);
     void getColorRGBA8(uint8_t* rgba);
+    void getColorRGBA32F(float* rgba);
+    void setColorFloatOutput(bool on) { floatOut_ = on; }
     void getFrameStats(IfritFrameStats* out) const { *out = stats_; }
     void setStageTiming(bool on) { stageTiming_ = on; }
     void drawAsync(int indexCount, bool clear);
@@ -155,6 +157,7 @@ private:
     std::vector<uint8_t> graphKey(bool clear) const;
     void dropGraph();
     Views deviceViews() const;
+    ShadeParams shadeParams(bool writeFloat) const;
     bool finish(bool clear, int indexCount);
     void copyOutToHost();
     void runHostVertexShader();
@@ -166,6 +169,8 @@ private:
     int depthFuncSaved_ = IF_COMPARE_OP_LESS;
     bool depthTest_ = true;
     IfritColorAttachmentBlendState blend_{};
+    BlendCoefs blendCoefs_{};
+    bool floatOut_ = false;  // shading also writes RGBA32F
     int samples_ = 1;
     int width_ = 0, height_ = 0;
     FrameBuffer* fb_ = nullptr;

@@ -134,6 +134,13 @@ struct ShadeParams {
     NeuralShadeParams neural;  // programs 4 (neural material) and 5 (training features)
 };
 
+// Blend factors, each encoded (c0, c1, c2, c3) as the reference does
+// (TileRasterRenderer.cpp:212-333): colour source, colour destination, alpha
+// source, alpha destination.
+struct BlendCoefs {
+    float sc[4], dc[4], sa[4], da[4];
+};
+
 // Status word bits raised by the kernels; checked by the host after a frame.
 enum StatusBits : unsigned {
     kStatusClipOverflow = 1u,
@@ -222,6 +229,16 @@ void launch_slot_to_prim(const uint32_t* visSlot, TriRecs recs, size_t n, uint32
 void launch_shade(const ShadeParams& p, const Views& views, int shadeTiles, TriRecs recs, const float4* bary, const uint32_t* visSlot,
                   const float* vtx, const int* idx, const TexDesc* tex, const uint8_t* texPool, uchar4* rgba8,
                   float4* rgba32f, hipStream_t s);
+
+// Blending, 1 sample per pixel: the ordered traversal with the depth test, each
+// passing fragment shaded and blended in primitive order (replaces the raster,
+// merge and shade launches).
+void launch_blend_ordered(const ShadeParams& p, const BlendCoefs& b, int depthFunc, int clear, unsigned binCap,
+                          const Views& views, TriRecs recs, const float4* bary, const float* vtx, const int* idx,
+                          const TexDesc* tex, const uint8_t* texPool, const unsigned* binStart,
+                          const unsigned* binCount, const unsigned* binList, int tiles, int grid, float* visDepth,
+                          uint32_t* visSlot, uint8_t* visClass, uchar4* rgba8, float4* rgba32f, StatPartials st,
+                          hipStream_t s);
 
 // Texture helpers (texture.hip).
 // Order-independent 64-bit digest of n words (equal buffers give equal digests).

@@ -9,8 +9,8 @@
 // the same triangle evaluated at the quad's other pixels.
 #include <hip/hip_runtime.h>
 
-#include "device_common.h"
 #include "neural_device.h"
+#include "raster_common.h"
 
 namespace ifrit_hip {
 
@@ -147,6 +147,7 @@ __device__ __forceinline__ float3 norm3(float3 a) {
 // coordinates and footprint when the fragment samples a texture.
 struct FragPrep {
     float3 A, B;
+    float alpha;  // translucent flat colour: alpha from the hash
     float u, v;
     TexFootprint fp;
     int mat;  // texture index, -1 when untextured
@@ -182,11 +183,15 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, float3 cam, const ShadeT
     R.mat = -1;
     R.u = R.v = 0.0f;
     R.fp = TexFootprint{0.0f, 0.0f, 0.0f, 0};
-    if (P.program == 0) {
+    R.alpha = 1.0f;
+    if (P.program == 0 || P.program == 6) {
+        // flat colour: bytes 0-2 of a hash of the primitive; the translucent flat colour also
+        // takes alpha = (64 + (byte 3 & 127)) / 255
         const uint32_t h = flat_hash((uint32_t)T.prim);
         R.A = make_float3(0, 0, 0);
         R.B = make_float3((h & 0xff) * (1.0f / 255.0f), ((h >> 8) & 0xff) * (1.0f / 255.0f),
                           ((h >> 16) & 0xff) * (1.0f / 255.0f));
+        if (P.program == 6) R.alpha = (float)(64u + ((h >> 24) & 0x7fu)) * (1.0f / 255.0f);
         return R;
     }
     const float x = (float)px + o, y = (float)py + o;
@@ -534,10 +539,11 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
                 cLod[lane] = vt.y;
                 xdir[lane] = make_float4(vt.z, lt.x, lt.y, lt.z);
             } else {
-                cCol[lane] = P.program == 0 ? make_float4(sat(R.B.x), sat(R.B.y), sat(R.B.z), 1.0f)
-                                            : make_float4(sat(fmaf(alb.x, R.A.x, R.B.x)),
-                                                          sat(fmaf(alb.y, R.A.y, R.B.y)),
-                                                          sat(fmaf(alb.z, R.A.z, R.B.z)), sat(alb.w));
+                cCol[lane] = P.program == 0 || P.program == 6
+                                 ? make_float4(sat(R.B.x), sat(R.B.y), sat(R.B.z), R.alpha)
+                                 : make_float4(sat(fmaf(alb.x, R.A.x, R.B.x)),
+                                               sat(fmaf(alb.y, R.A.y, R.B.y)), sat(fmaf(alb.z, R.A.z, R.B.z)),
+                                               sat(alb.w));
             }
         }
         if constexpr (NEURAL) {
@@ -607,6 +613,147 @@ __device__ __forceinline__ void shade_tiles(const ShadeParams& P, const Views& V
     }
 }
 
+// One fragment's colour at a 1-sample pixel, for the ordered blending path:
+// the arithmetic of the deferred shading in shade_subtile (taps summed in
+// order), so an opaque fragment gets the same colour either way.
+__device__ float4 shade_fragment(const ShadeParams& P, float3 cam, const ShadeTri& T, const float* vtx, const int* idx,
+                                 const TexDesc* tex, const uint8_t* pool, int px, int py) {
+    const FragPrep R = prep_fragment(P, cam, T, vtx, idx, tex, px, py, 0.0f);
+    if (P.program == 0 || P.program == 6) return make_float4(sat(R.B.x), sat(R.B.y), sat(R.B.z), R.alpha);
+    float4 acc = make_float4(0, 0, 0, 0), acc2 = make_float4(0, 0, 0, 0);
+    const int n = R.mat >= 0 ? R.fp.n : 0;
+    for (int i = 0; i < n; i++) {
+        const float invn = frcp((float)n);
+        const float to = ((float)i + 0.5f) * invn - 0.5f;
+        const float tu = fmaf(R.fp.au, to, R.u), tv = fmaf(R.fp.av, to, R.v);
+        const float4 c = trilinear(pool, tex[R.mat], tu, tv, R.fp.lod);
+        acc.x += c.x;
+        acc.y += c.y;
+        acc.z += c.z;
+        acc.w += c.w;
+        if (P.program == 3) {
+            const float4 nm = trilinear(pool, tex[1], tu, tv, R.fp.lod);
+            const float4 rg = trilinear(pool, tex[2], tu, tv, R.fp.lod);
+            acc2.x += nm.x;
+            acc2.y += nm.y;
+            acc2.z += nm.z;
+            acc2.w += rg.x;
+        }
+    }
+    float4 alb = make_float4(1, 1, 1, 1);
+    if (R.mat >= 0) {
+        const float invn = frcp((float)n);
+        alb = make_float4(acc.x * invn, acc.y * invn, acc.z * invn, acc.w * invn);
+    }
+    if (P.program == 3) {
+        float4 nm = make_float4(0.5f, 0.5f, 1.0f, 0.0f);
+        if (R.mat >= 0) {
+            const float invn = frcp((float)n);
+            nm = make_float4(acc2.x * invn, acc2.y * invn, acc2.z * invn, acc2.w * invn);
+        }
+        return ggx_color(P, R, alb, nm);
+    }
+    return make_float4(sat(fmaf(alb.x, R.A.x, R.B.x)), sat(fmaf(alb.y, R.A.y, R.B.y)), sat(fmaf(alb.z, R.A.z, R.B.z)),
+                       sat(alb.w));
+}
+
+// The blend of TileRasterWorker.cpp:2346-2373 in FP32, in the same operation
+// order; factors come encoded as (c0, c1, c2, c3) meaning
+// (c0 + c1 (1 - src alpha) + c2 (1 - dst alpha)) (1 - c3)
+// (TileRasterRenderer.cpp:212-333).
+__device__ __forceinline__ float blend_factor(const float* k, float srcA, float dstA) {
+    return (k[0] + k[1] * (1 - srcA) + k[2] * (1 - dstA)) * (1 - k[3]);
+}
+
+__device__ __forceinline__ float4 blend_pixel(const BlendCoefs& B, float4 src, float4 dst) {
+    const float mxSrcX = blend_factor(B.sc, src.w, dst.w), mxDstX = blend_factor(B.dc, src.w, dst.w);
+    const float mxSrcA = blend_factor(B.sa, src.w, dst.w), mxDstA = blend_factor(B.da, src.w, dst.w);
+    return make_float4(dst.x * mxDstX + src.x * mxSrcX, dst.y * mxDstX + src.y * mxSrcX,
+                       dst.z * mxDstX + src.z * mxSrcX, dst.w * mxDstA + src.w * mxSrcA);
+}
+
+using dev::Edges;
+using dev::block_test;
+using dev::block_sum256;
+using dev::kClearDepth;
+using dev::load_edges;
+using dev::load_rec;
+
+// Blending: the ordered traversal of each tile's list with the depth test of
+// the ordered raster, and every fragment that passes shaded and blended into
+// the pixel's FP32 colour in primitive order (1 sample per pixel).  Writes the
+// visibility (last writer), the FP32 colour and its RGBA8 rounding.
+__global__ void __launch_bounds__(256) k_blend_ordered(ShadeParams P, BlendCoefs B, int depthFunc, int clear,
+                                                       unsigned binCap, Views V, TriRecs recs,
+                                                       const float4* __restrict__ bary, const float* __restrict__ vtx,
+                                                       const int* __restrict__ idx, const TexDesc* __restrict__ tex,
+                                                       const uint8_t* __restrict__ pool,
+                                                       const unsigned* __restrict__ binStart,
+                                                       const unsigned* __restrict__ binCount,
+                                                       const unsigned* __restrict__ binList, int tiles,
+                                                       float* __restrict__ visDepthAll,
+                                                       uint32_t* __restrict__ visSlotAll,
+                                                       uint8_t* __restrict__ visClassAll, uchar4* __restrict__ rgba8All,
+                                                       float4* __restrict__ rgba32fAll, StatPartials st) {
+    __shared__ unsigned long long sA[rc::kOrderCap], sB[rc::kOrderCap];
+    const int t = threadIdx.x;
+    unsigned long long frags = 0;
+    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const ViewDesc& D = V.desc[V.tileView[tile]];
+        const rc::ViewGeom G = rc::view_of_tile(V, tile);
+        const rc::PixelCtx C = rc::pixel_ctx(G, t & 15, t >> 4);
+        const size_t pix = G.pixBase + C.pix;
+        const float3 cam = make_float3(D.camera[0], D.camera[1], D.camera[2]);
+        const unsigned fanStart = D.slotBase + (unsigned)P.ntri;
+        float dep = kClearDepth;
+        unsigned slot = 0xffffffffu, cls = 0;
+        float4 col = make_float4(0, 0, 0, 0);  // the colour attachment is cleared to 0 (TileRasterWorker.cpp:113-118)
+        if (!clear && C.inFb) {
+            dep = visDepthAll[pix];
+            slot = visSlotAll[pix];
+            if (visClassAll) cls = visClassAll[pix];
+            col = rgba32fAll[pix];
+        }
+        const unsigned start = binStart[tile];
+        const unsigned cnt = start >= binCap ? 0u : min(binCount[tile], binCap - start);
+        rc::for_each_ordered(recs, binList, start, cnt, sA, sB, [&](unsigned ent) {
+            const unsigned sl = ent & 0x7fffffffu;
+            const bool full = (ent >> 31) != 0;
+            if (!C.inFb) return;
+            const TriRec R = load_rec(recs, sl);
+            const Edges E = load_edges(R);
+            int trn, tan;
+            block_test(E, C.bMinX, C.bMinY, trn, tan);
+            if (!rc::covers(E, full, tan, trn, C.fpx, C.fpy)) return;
+            const unsigned c = rc::pixel_class(E, full || tan == 3, C.fpx, C.fpy, C.bMinX);
+            const float d = rc::class_depth(c, R.f1, R.f2, R.f3, C.fpx, C.fpy);
+            frags++;
+            if (!rc::depth_pass(depthFunc, d, dep, c == 3u)) return;
+            ShadeTri T;
+            load_shade_tri(recs, bary, fanStart, sl, T);
+            col = blend_pixel(B, shade_fragment(P, cam, T, vtx, idx, tex, pool, C.px, C.py), col);
+            dep = d;
+            slot = sl;
+            cls = c;
+        });
+        if (C.inFb) {
+            visDepthAll[pix] = dep;
+            visSlotAll[pix] = slot;
+            if (visClassAll) visClassAll[pix] = (uint8_t)cls;
+            rgba32fAll[pix] = col;
+            rgba8All[pix] = make_uchar4((unsigned char)floorf(sat(col.x) * 255.0f + 0.5f),
+                                        (unsigned char)floorf(sat(col.y) * 255.0f + 0.5f),
+                                        (unsigned char)floorf(sat(col.z) * 255.0f + 0.5f),
+                                        (unsigned char)floorf(sat(col.w) * 255.0f + 0.5f));
+        }
+    }
+    const unsigned long long tf = block_sum256(frags);
+    if (t == 0) {
+        st.rasterFrags[blockIdx.x] = tf;
+        st.rasterCovs[blockIdx.x] = tf;
+    }
+}
+
 template <int S>
 __global__ void __launch_bounds__(64) k_shade(ShadeParams P, Views V, TriRecs recs, const float4* __restrict__ bary,
                                               const uint32_t* __restrict__ visSlot, const float* __restrict__ vtx,
@@ -628,6 +775,17 @@ k_shade_neural(ShadeParams P, Views V, TriRecs recs, const float4* __restrict__ 
 }
 
 }  // namespace
+
+void launch_blend_ordered(const ShadeParams& p, const BlendCoefs& b, int depthFunc, int clear, unsigned binCap,
+                          const Views& views, TriRecs recs, const float4* bary, const float* vtx, const int* idx,
+                          const TexDesc* tex, const uint8_t* texPool, const unsigned* binStart,
+                          const unsigned* binCount, const unsigned* binList, int tiles, int grid, float* visDepth,
+                          uint32_t* visSlot, uint8_t* visClass, uchar4* rgba8, float4* rgba32f, StatPartials st,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_blend_ordered, dim3(grid), dim3(256), 0, s, p, b, depthFunc, clear, binCap, views, recs, bary,
+                       vtx, idx, tex, texPool, binStart, binCount, binList, tiles, visDepth, visSlot, visClass, rgba8,
+                       rgba32f, st);
+}
 
 void launch_shade(const ShadeParams& p, const Views& views, int shadeTiles, TriRecs recs, const float4* bary,
                   const uint32_t* visSlot, const float* vtx, const int* idx, const TexDesc* tex,

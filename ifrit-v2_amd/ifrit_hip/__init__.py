@@ -61,6 +61,11 @@ class FrameStats(C.Structure):
         return d
 
 
+class BlendState(C.Structure):
+    _fields_ = [("blendEnable", C.c_uint8), ("srcColorBlendFactor", C.c_int32), ("dstColorBlendFactor", C.c_int32),
+                ("srcAlphaBlendFactor", C.c_int32), ("dstAlphaBlendFactor", C.c_int32)]
+
+
 class ViewDesc(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("mvp", C.c_float * 16), ("camera", C.c_float * 3),
                 ("shade", C.c_int32)]
@@ -103,7 +108,7 @@ EXPORTED_SYMBOLS = [
     "iftrGetFrameStats", "iftrSetStageTiming", "iftrDrawAsync", "iftrSynchronize", "iftrGetLastError",
     "iftrNeuralTrain", "iftrNeuralSetParams", "iftrNeuralGetParams", "iftrNeuralEval", "iftrNeuralValidate",
     "iftrSetViews", "iftrGetViewVisibility", "iftrGetViewColorRGBA8", "iftrSetGraphMode", "iftrSetTuning",
-    "iftrGetTuning", "iftrAutotune", "iftrGetDigest",
+    "iftrGetTuning", "iftrAutotune", "iftrGetDigest", "iftrSetColorFloatOutput", "iftrGetColorRGBA32F",
 ]
 
 
@@ -155,6 +160,7 @@ def lib():
         "iftrSetTuning": ([vp, C.POINTER(Tuning)], None), "iftrGetTuning": ([vp, C.POINTER(Tuning)], None),
         "iftrAutotune": ([vp, C.c_int, C.c_int, C.POINTER(AutotuneReport)], None),
         "iftrGetDigest": ([vp], C.c_uint64),
+        "iftrSetColorFloatOutput": ([vp, C.c_int], None), "iftrGetColorRGBA32F": ([vp, vp], None),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -207,6 +213,23 @@ class Renderer:
         """IF_COMPARE_OP_* (include/ifrit_hip.h); LESS (1) is the default."""
         self.L.iftrSetDepthFunc(self.h, func)
         self._check()
+
+    def set_blend(self, factors=None):
+        """factors = (src colour, dst colour, src alpha, dst alpha) as IfritBlendFactor values, or None
+        to draw without blending."""
+        st = BlendState(0, 0, 0, 0, 0) if factors is None else BlendState(1, *[int(f) for f in factors])
+        self.L.iftrSetBlendFunc(self.h, C.byref(st))
+        self._check()
+
+    def set_float_output(self, on: bool):
+        self.L.iftrSetColorFloatOutput(self.h, 1 if on else 0)
+        self._check()
+
+    def color_f32(self):
+        out = np.empty((self.height, self.width, 4), np.float32)
+        self.L.iftrGetColorRGBA32F(self.h, _p(out))
+        self._check()
+        return out
 
     def set_target(self, width: int, height: int, samples: int = 1):
         self.L.iftrSetRenderTarget(self.h, width, height)

@@ -25,6 +25,7 @@ PROGRAM_FLAT = 0
 PROGRAM_BLINN_PHONG = 1
 PROGRAM_DEPTH_ONLY = 2
 PROGRAM_GGX_LAYERED = 3
+PROGRAM_FLAT_ALPHA = 6
 SAMPLER_BILINEAR = 0
 SAMPLER_TRILINEAR = 1
 SAMPLER_ANISO = 2

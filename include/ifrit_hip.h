@@ -137,8 +137,21 @@ enum {
     IFHIP_FS_DEPTH_ONLY = 2,  /* visibility and depth only */
     IFHIP_FS_GGX_LAYERED = 3, /* GGX over three layers: texture 0 albedo, 1 tangent-space normal,
                                  2 roughness (red); needs a tangent in the vertex record */
-    IFHIP_FS_NEURAL = 4       /* the neural material trained by iftrNeuralTrain (same inputs as
+    IFHIP_FS_NEURAL = 4,      /* the neural material trained by iftrNeuralTrain (same inputs as
                                  IFHIP_FS_GGX_LAYERED; light 0 only) */
+    IFHIP_FS_FLAT_ALPHA = 6   /* IFHIP_FS_FLAT with alpha = (64 + (hash byte 3 & 127)) / 255 */
+};
+/* IfritBlendFactor, include/ifrit/softgraphics/engine/base/Constants.h:64-76.  Blending
+   (iftrSetBlendFunc) supports ZERO, ONE, SRC_ALPHA, ONE_MINUS_SRC_ALPHA, DST_ALPHA and
+   ONE_MINUS_DST_ALPHA, with the reference's encoding (TileRasterRenderer.cpp:212-333), at one
+   sample per pixel for a single target; fragments are blended in primitive order. */
+enum {
+    IF_BLEND_FACTOR_ZERO = 0,
+    IF_BLEND_FACTOR_ONE = 1,
+    IF_BLEND_FACTOR_SRC_ALPHA = 6,
+    IF_BLEND_FACTOR_ONE_MINUS_SRC_ALPHA = 7,
+    IF_BLEND_FACTOR_DST_ALPHA = 8,
+    IF_BLEND_FACTOR_ONE_MINUS_DST_ALPHA = 9
 };
 enum { IFHIP_SAMPLER_BILINEAR = 0, IFHIP_SAMPLER_TRILINEAR = 1, IFHIP_SAMPLER_ANISOTROPIC = 2 };
 
@@ -189,6 +202,10 @@ void iftrSetRenderTarget(IfritRendererHandle* h, int width, int height);
 void iftrGetVisibility(IfritRendererHandle* h, uint32_t* primIds, float* depth); /* W*H*S each */
 void iftrGetCoverageClass(IfritRendererHandle* h, uint8_t* classes);            /* W*H, 1 sample */
 void iftrGetColorRGBA8(IfritRendererHandle* h, uint8_t* rgba);                  /* W*H*4 */
+/* Unquantised colour (W*H*4 floats): written by blending draws, or by every draw after
+   iftrSetColorFloatOutput(h, 1). */
+void iftrSetColorFloatOutput(IfritRendererHandle* h, int enable);
+void iftrGetColorRGBA32F(IfritRendererHandle* h, float* rgba);
 
 typedef struct {
     int64_t inputTriangles;
