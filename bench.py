@@ -246,6 +246,8 @@ def main():
     ap.add_argument("--neural-single-tap", type=int, default=1,
                     help="1: one trilinear latent tap per fragment, 0: the material's anisotropic taps")
     ap.add_argument("--no-batch", action="store_true", help="skip the multi-view batch leg")
+    ap.add_argument("--only", choices=["batch", "neural"],
+                    help="run one leg alone and print its JSON (for profiling; not the benchmark line)")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a float32 copy of 256 MiB (x + 0) three times: the reference kernel for "
@@ -260,6 +262,9 @@ def main():
             torch.add(src, 0.0, out=dst)
         torch.cuda.synchronize()
         print(json.dumps({"calibration": "copy", "bytes_read": src.numel() * 4, "bytes_written": dst.numel() * 4}))
+        return
+    if args.only:
+        print(json.dumps({args.only: batch_leg(args) if args.only == "batch" else neural_leg(args)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the renderer runs on a single "
