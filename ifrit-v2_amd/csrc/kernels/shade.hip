@@ -585,31 +585,38 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
     if (rgba32f) rgba32f[pix] = r;
 }
 
-// One wave per shading tile of P.shadeTile pixels (8, 16 or 32), its 8x8
-// sub-tiles in row order; consecutive tiles on one XCD share its L2 (texture
-// and vertex reuse).  Tiles of all shaded views are numbered one after another.
-template <int S, bool NEURAL, bool FP8>
+// One wave per shading tile of 8N x 8N pixels (N = 1, 2 or 4: P.shadeTile is
+// 8N), its 8x8 sub-tiles in row order; consecutive tiles on one XCD share its
+// L2 (texture and vertex reuse).  Tiles of all shaded views are numbered one
+// after another.
+template <int S, bool NEURAL, bool FP8, int N>
 __device__ __forceinline__ void shade_tiles(const ShadeParams& P, const Views& V, const TriRecs& recs,
                                             const float4* __restrict__ bary, const uint32_t* __restrict__ visSlot,
                                             const float* __restrict__ vtx, const int* __restrict__ idx,
                                             const TexDesc* __restrict__ tex, const uint8_t* __restrict__ pool,
                                             uchar4* __restrict__ rgba8, float4* __restrict__ rgba32f, int tiles,
                                             int perXcd) {
+    constexpr int T = 8 * N;
     const int tile = (int)(blockIdx.x & 7u) * perXcd + (int)(blockIdx.x >> 3);
     if (tile >= tiles) return;
     const ViewDesc& D = V.desc[V.shadeView[tile]];
-    const int T = P.shadeTile, n = T >> 3;
     const int width = D.width, height = D.height;
     const int local = tile - (int)D.shadeBase, perRow = (width + T - 1) / T;
     const int ox0 = (local % perRow) * T, oy0 = (local / perRow) * T;
     const size_t pb = (size_t)D.pixBase;
     const float3 cam = make_float3(D.camera[0], D.camera[1], D.camera[2]);
     const unsigned fanStart = D.slotBase + (unsigned)P.ntri;
-    for (int k = 0; k < n * n; k++) {
-        const int ox = ox0 + (k % n) * 8, oy = oy0 + (k / n) * 8;
-        if (ox >= width || oy >= height) continue;
-        shade_subtile<S, NEURAL, FP8>(P, cam, width, height, fanStart, ox, oy, pb, recs, bary, visSlot + pb * S, vtx,
-                                      idx, tex, pool, rgba8 + pb, rgba32f ? rgba32f + pb : nullptr);
+    if constexpr (N == 1) {
+        shade_subtile<S, NEURAL, FP8>(P, cam, width, height, fanStart, ox0, oy0, pb, recs, bary, visSlot + pb * S,
+                                      vtx, idx, tex, pool, rgba8 + pb, rgba32f ? rgba32f + pb : nullptr);
+    } else {
+#pragma unroll 1
+        for (int k = 0; k < N * N; k++) {
+            const int ox = ox0 + (k % N) * 8, oy = oy0 + (k / N) * 8;
+            if (ox >= width || oy >= height) continue;
+            shade_subtile<S, NEURAL, FP8>(P, cam, width, height, fanStart, ox, oy, pb, recs, bary, visSlot + pb * S,
+                                          vtx, idx, tex, pool, rgba8 + pb, rgba32f ? rgba32f + pb : nullptr);
+        }
     }
 }
 
@@ -754,24 +761,24 @@ __global__ void __launch_bounds__(256) k_blend_ordered(ShadeParams P, BlendCoefs
     }
 }
 
-template <int S>
+template <int S, int N>
 __global__ void __launch_bounds__(64) k_shade(ShadeParams P, Views V, TriRecs recs, const float4* __restrict__ bary,
                                               const uint32_t* __restrict__ visSlot, const float* __restrict__ vtx,
                                               const int* __restrict__ idx, const TexDesc* __restrict__ tex,
                                               const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8,
                                               float4* __restrict__ rgba32f, int tiles, int perXcd) {
-    shade_tiles<S, false, false>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles, perXcd);
+    shade_tiles<S, false, false, N>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles, perXcd);
 }
 
 // The neural material, held to four waves per SIMD like the other programs so
 // that the latent fetches stay overlapped.
-template <int S, bool FP8>
+template <int S, bool FP8, int N>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_shade_neural(ShadeParams P, Views V, TriRecs recs, const float4* __restrict__ bary,
                const uint32_t* __restrict__ visSlot, const float* __restrict__ vtx, const int* __restrict__ idx,
                const TexDesc* __restrict__ tex, const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8,
                float4* __restrict__ rgba32f, int tiles, int perXcd) {
-    shade_tiles<S, true, FP8>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles, perXcd);
+    shade_tiles<S, true, FP8, N>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles, perXcd);
 }
 
 }  // namespace
@@ -794,14 +801,23 @@ void launch_shade(const ShadeParams& p, const Views& views, int shadeTiles, TriR
     const int perXcd = (shadeTiles + 7) / 8;
     const dim3 grid(8 * perXcd);
 #define IFRIT_ARGS p, views, recs, bary, visSlot, vtx, idx, tex, texPool, rgba8, rgba32f, shadeTiles, perXcd
-#define IFRIT_SHADE(SN)                                                                                             \
+#define IFRIT_SHADE_N(SN, NN)                                                                                      \
     do {                                                                                                            \
         if (p.program != 4)                                                                                         \
-            hipLaunchKernelGGL((k_shade<SN>), grid, dim3(64), 0, s, IFRIT_ARGS);                                    \
+            hipLaunchKernelGGL((k_shade<SN, NN>), grid, dim3(64), 0, s, IFRIT_ARGS);                                \
         else if (p.neural.precision == 1)                                                                           \
-            hipLaunchKernelGGL((k_shade_neural<SN, true>), grid, dim3(64), 0, s, IFRIT_ARGS);                       \
+            hipLaunchKernelGGL((k_shade_neural<SN, true, NN>), grid, dim3(64), 0, s, IFRIT_ARGS);                   \
         else                                                                                                        \
-            hipLaunchKernelGGL((k_shade_neural<SN, false>), grid, dim3(64), 0, s, IFRIT_ARGS);                      \
+            hipLaunchKernelGGL((k_shade_neural<SN, false, NN>), grid, dim3(64), 0, s, IFRIT_ARGS);                  \
+    } while (0)
+#define IFRIT_SHADE(SN)                                                                                             \
+    do {                                                                                                            \
+        if (p.shadeTile == 32)                                                                                      \
+            IFRIT_SHADE_N(SN, 4);                                                                                   \
+        else if (p.shadeTile == 16)                                                                                 \
+            IFRIT_SHADE_N(SN, 2);                                                                                   \
+        else                                                                                                        \
+            IFRIT_SHADE_N(SN, 1);                                                                                   \
     } while (0)
     switch (p.samples) {
         case 1: IFRIT_SHADE(1); break;
@@ -809,6 +825,7 @@ void launch_shade(const ShadeParams& p, const Views& views, int shadeTiles, TriR
         default: IFRIT_SHADE(8); break;
     }
 #undef IFRIT_SHADE
+#undef IFRIT_SHADE_N
 #undef IFRIT_ARGS
 }
 
