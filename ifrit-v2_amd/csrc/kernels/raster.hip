@@ -60,9 +60,36 @@ __device__ __forceinline__ bool rect_may_cover(const Edges& E, float x, float y,
     for (int k = 0; k < 3; k++) {
         const float cx = E.x[k] < 0.0f ? x + w : x;
         const float cy = E.y[k] < 0.0f ? y + h : y;
-        ok = ok && (fmaf(E.y[k], cy, E.x[k] * cx) < ifx::neg(E.z[k]));
+        ok &= fmaf(E.y[k], cy, E.x[k] * cx) < ifx::neg(E.z[k]);
     }
     return ok;
+}
+
+// rect_may_cover of the 16 unit squares of the 4x4 block at (x0, y0), bit
+// 4 * row + column: the same products and fused sums per square (corners
+// are small integers, so x0 + column + 1 is exact either way), with each
+// edge's column products and row corners shared and no branches.
+__device__ __forceinline__ unsigned block_square_mask(const Edges& E, float x0, float y0) {
+    float ax[3][4], cy[3][4], nz[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float ox = E.x[k] < 0.0f ? 1.0f : 0.0f, oy = E.y[k] < 0.0f ? 1.0f : 0.0f;
+        nz[k] = ifx::neg(E.z[k]);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            ax[k][i] = E.x[k] * ((x0 + (float)i) + ox);
+            cy[k][i] = (y0 + (float)i) + oy;
+        }
+    }
+    unsigned m = 0;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int i = q & 3, j = q >> 2;
+        const bool c = (fmaf(E.y[0], cy[0][j], ax[0][i]) < nz[0]) & (fmaf(E.y[1], cy[1][j], ax[1][i]) < nz[1]) &
+                       (fmaf(E.y[2], cy[2][j], ax[2][i]) < nz[2]);
+        m |= (unsigned)c << q;
+    }
+    return m;
 }
 
 // Exclusive scan over an NT-thread workgroup (64 or 256); every thread must call it.
@@ -147,15 +174,10 @@ __device__ __forceinline__ void entry_mask_half(const Edges& E, bool full, int t
             // at bit offset (row & 1) * 16 + (bi & 3) * 4
             unsigned lo = 0xF000Fu, hi = 0xF000Fu;
             if (!acc) {
-                lo = 0;
-                hi = 0;
-#pragma unroll
-                for (int q = 0; q < 16; q++) {
-                    const bool c = rect_may_cover(E, x0 + (float)(q & 3), y0 + (float)(q >> 2), 1.0f, 1.0f);
-                    const unsigned bit = (unsigned)c << (((q >> 2) & 1) * 16 + (q & 3));
-                    if (q < 8) lo |= bit;
-                    else hi |= bit;
-                }
+                // rows 0-1 of the block to lo, rows 2-3 to hi, odd rows at bit 16
+                const unsigned b = block_square_mask(E, x0, y0);
+                lo = (b & 0xFu) | ((b & 0xF0u) << 12);
+                hi = ((b >> 8) & 0xFu) | ((b & 0xF000u) << 4);
             }
             const int sh = (bi & 3) * 4;
             if (j < 4) {
@@ -204,6 +226,19 @@ __device__ __forceinline__ unsigned winner_class(const TriRec& R, int tx, int ty
     return c;
 }
 
+// Record slots of a pixel's S final keys (slotOf[prim] + fan), all S lookups
+// issued before the first is used; a key that holds no fragment looks up
+// primitive 0 and its slot is not used.
+template <int S>
+__device__ __forceinline__ void key_slots(const unsigned* slotOf, const unsigned long long* keys,
+                                          unsigned long long none, unsigned* slots) {
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const unsigned tie = (unsigned)keys[s];
+        slots[s] = slotOf[keys[s] != none ? tie >> 2 : 0u] + (tie & 3u);
+    }
+}
+
 // Writes the outputs of one pixel from its S final keys: depth and record
 // slot per sample (the primitive ID is the slot's key >> 2, expanded only on
 // readback).
@@ -212,12 +247,17 @@ __device__ __forceinline__ void write_pixel(const RasterParams& P, const TriRecs
                                             const PixelCtx& C, const unsigned long long* keys, float* visDepth,
                                             uint32_t* visSlot, uint8_t* visClass) {
     if (!C.inFb) return;
+    unsigned long long prev[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) prev[s] = frag_key(P.clear ? kClearDepth : visDepth[C.pix * S + s], 0u);
+    unsigned slots[S];
+    key_slots<S>(slotOf, keys, P.clear ? frag_key(kClearDepth, 0u) : ~0ull, slots);
     unsigned cls = 0;
     bool won = false;
 #pragma unroll
     for (int s = 0; s < S; s++) {
         const unsigned long long k = keys[s];
-        if (k == frag_key(P.clear ? kClearDepth : visDepth[C.pix * S + s], 0u)) {
+        if (k == prev[s]) {
             if (P.clear) {
                 visDepth[C.pix * S + s] = kClearDepth;
                 visSlot[C.pix * S + s] = 0xffffffffu;
@@ -225,9 +265,7 @@ __device__ __forceinline__ void write_pixel(const RasterParams& P, const TriRecs
             continue;
         }
         won = true;
-        const unsigned tie = (unsigned)k;
-        const unsigned prim = tie >> 2;
-        const unsigned slot = slotOf[prim] + (tie & 3u);
+        const unsigned slot = slots[s];
         float d = order_key_inv((unsigned)(k >> 32));
         if (S == 1 && (visClass || (unsigned)(k >> 32) == 0x80000000u)) {
             // the class output, or +0 and -0 sharing a key: recompute from the winner's record
@@ -261,15 +299,15 @@ __device__ __forceinline__ void stage_pixel(const RasterParams& P, const TriRecs
     if (!C.inFb) return;
     unsigned cls = 0;
     const unsigned long long clearKey = frag_key(kClearDepth, 0u);
+    unsigned slots[S];
+    key_slots<S>(slotOf, keys, clearKey, slots);
 #pragma unroll
     for (int s = 0; s < S; s++) {
         const unsigned long long k = keys[s];
         float d = kClearDepth;
         unsigned slot = 0xffffffffu;
         if (k != clearKey) {
-            const unsigned tie = (unsigned)k;
-            const unsigned prim = tie >> 2;
-            slot = slotOf[prim] + (tie & 3u);
+            slot = slots[s];
             d = order_key_inv((unsigned)(k >> 32));
             if (S == 1 && (visClass || (unsigned)(k >> 32) == 0x80000000u)) {
                 // the class output, or +0 and -0 sharing a key: recompute from the winner's record
@@ -346,8 +384,12 @@ __global__ void __launch_bounds__(NT) k_raster(RasterParams P, Views V, TriRecs 
     const unsigned numItems = ctr->numItems, numLong = ctr->numLongItems;
     unsigned long long frags = 0, covs = 0;
 
+    // thread 0 claims the next item before the output stage of the current
+    // one, so that the claim's latency overlaps the output loads
+    unsigned claimed = 0;
+    if (t == 0) claimed = atomicAdd(&ctr->itemHead, 1u);
     for (;;) {
-        if (t == 0) sItem = atomicAdd(&ctr->itemHead, 1u);
+        if (t == 0) sItem = claimed;
         __syncthreads();
         const unsigned item = sItem;
         if (item >= numItems) break;
@@ -496,6 +538,7 @@ __global__ void __launch_bounds__(NT) k_raster(RasterParams P, Views V, TriRecs 
             __syncthreads();  // sRec, sTa and sWave are reused by the next pass
         }
 
+        if (t == 0) claimed = atomicAdd(&ctr->itemHead, 1u);
         if (pslot != 0xFFFFFFFFu) {
             // one item of a longer list: its keys, sample-major like sVis
             unsigned long long* g = partials + (size_t)pslot * 256 * S;
