@@ -358,15 +358,14 @@ __device__ __forceinline__ void store_tile(const ViewGeom& G, int tx, int ty, co
 
 // NT threads per tile: 256 (one pixel per thread) or, at 1 sample per pixel,
 // 64 (one wave per tile, four pixels per lane: more tiles in flight per
-// compute unit, for many small tiles).  Passes hold NT / 2 entries.
+// compute unit, for many small tiles).  Passes hold NT / 2 entries.  Held to
+// 128 VGPRs so that four workgroups of 256 share a compute unit.
 template <int S, int NT>
-__global__ void __launch_bounds__(NT) k_raster(RasterParams P, Views V, TriRecs recs,
-                                               const unsigned* __restrict__ slotOfAll,
-                                               const unsigned* __restrict__ binList, const uint4* __restrict__ items,
-                                               unsigned itemCap, unsigned long long* __restrict__ partials,
-                                               float* __restrict__ visDepthAll, uint32_t* __restrict__ visSlotAll,
-                                               uint8_t* __restrict__ visClassAll, StatPartials st,
-                                               DeviceCounters* ctr) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slotOfAll,
+         const unsigned* __restrict__ binList, const uint4* __restrict__ items, unsigned itemCap,
+         unsigned long long* __restrict__ partials, float* __restrict__ visDepthAll, uint32_t* __restrict__ visSlotAll,
+         uint8_t* __restrict__ visClassAll, StatPartials st, DeviceCounters* ctr) {
     constexpr int NW = S == 1 ? 1 : 4;      // mask words of one half tile
     constexpr int PASS = NT / 2;            // list entries per pass through LDS
     constexpr int UNITS = NT * 16;          // units expanded into LDS per round
