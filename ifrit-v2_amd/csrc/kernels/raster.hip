@@ -384,9 +384,14 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
     unsigned long long frags = 0, covs = 0;
 
     // thread 0 claims the next item before the output stage of the current
-    // one, so that the claim's latency overlaps the output loads
+    // one, so that the claim's latency overlaps the output stage.  The
+    // counter's address is moved to vector registers: a uniform address would
+    // turn the claim into a wave-aggregated atomic whose result is awaited at
+    // once.
+    __attribute__((address_space(1))) unsigned* head = (__attribute__((address_space(1))) unsigned*)&ctr->itemHead;
+    asm volatile("" : "+v"(head));
     unsigned claimed = 0;
-    if (t == 0) claimed = atomicAdd(&ctr->itemHead, 1u);
+    if (t == 0) claimed = __hip_atomic_fetch_add(head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (;;) {
         if (t == 0) sItem = claimed;
         __syncthreads();
@@ -537,7 +542,7 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
             __syncthreads();  // sRec, sTa and sWave are reused by the next pass
         }
 
-        if (t == 0) claimed = atomicAdd(&ctr->itemHead, 1u);
+        if (t == 0) claimed = __hip_atomic_fetch_add(head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (pslot != 0xFFFFFFFFu) {
             // one item of a longer list: its keys, sample-major like sVis
             unsigned long long* g = partials + (size_t)pslot * 256 * S;
