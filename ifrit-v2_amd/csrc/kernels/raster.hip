@@ -85,9 +85,10 @@ __device__ __forceinline__ unsigned block_square_mask(const Edges& E, float x0, 
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         const int i = q & 3, j = q >> 2;
-        const bool c = (fmaf(E.y[0], cy[0][j], ax[0][i]) < nz[0]) & (fmaf(E.y[1], cy[1][j], ax[1][i]) < nz[1]) &
-                       (fmaf(E.y[2], cy[2][j], ax[2][i]) < nz[2]);
-        m |= (unsigned)c << q;
+        const unsigned c = (unsigned)(fmaf(E.y[0], cy[0][j], ax[0][i]) < nz[0]) &
+                           (unsigned)(fmaf(E.y[1], cy[1][j], ax[1][i]) < nz[1]) &
+                           (unsigned)(fmaf(E.y[2], cy[2][j], ax[2][i]) < nz[2]);
+        m |= c << q;
     }
     return m;
 }

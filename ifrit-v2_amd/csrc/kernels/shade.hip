@@ -64,37 +64,62 @@ __device__ __forceinline__ void vertex_weights(const ShadeTri& T, float x, float
     w[2] = 1.0f - w[0] - w[1];
 }
 
-__device__ __forceinline__ float4 fetch_texel(const uint8_t* pool, const TexDesc& D, int level, int x, int y) {
-    const int w = max(1, D.width >> level), h = max(1, D.height >> level);
-    x &= (w - 1);  // power-of-two sizes, repeat addressing
-    y &= (h - 1);
-    const uchar4 c = reinterpret_cast<const uchar4*>(pool + D.base)[D.levelOffset[level] + (unsigned)y * w + x];
-    return make_float4(c.x * (1.0f / 255.0f), c.y * (1.0f / 255.0f), c.z * (1.0f / 255.0f), c.w * (1.0f / 255.0f));
-}
-
 __device__ __forceinline__ float4 lerp4(float4 a, float4 b, float t) {
     return make_float4(fmaf(b.x - a.x, t, a.x), fmaf(b.y - a.y, t, a.y), fmaf(b.z - a.z, t, a.z), fmaf(b.w - a.w, t, a.w));
 }
 
-__device__ float4 bilinear(const uint8_t* pool, const TexDesc& D, int level, float u, float v) {
-    const float w = (float)max(1, D.width >> level), h = (float)max(1, D.height >> level);
-    const float s = fmaf(u, w, -0.5f), r = fmaf(v, h, -0.5f);
-    const float fs = floorf(s), fr = floorf(r);
-    const float ax = s - fs, ay = r - fr;
-    const int x0 = (int)fs, y0 = (int)fr;
-    const float4 c00 = fetch_texel(pool, D, level, x0, y0), c10 = fetch_texel(pool, D, level, x0 + 1, y0);
-    const float4 c01 = fetch_texel(pool, D, level, x0, y0 + 1), c11 = fetch_texel(pool, D, level, x0 + 1, y0 + 1);
-    return lerp4(lerp4(c00, c10, ax), lerp4(c01, c11, ax), ay);
-}
+// The two mip levels a trilinear lookup at lod blends: level l0 = floor of
+// the clamped lod and l1 = l0 + 1 (or l0 at the last level), as the address
+// of level l0's first texel, the texel distance to level l1, the levels'
+// log2 sizes (textures are powers of two) and the blend fraction.
+struct TriLevels {
+    const uchar4* t0;
+    unsigned d1;    // level l1 starts d1 texels after level l0
+    unsigned dims;  // log2 width, log2 height of l0, then of l1, 5 bits each
+    float f;
+};
 
-__device__ float4 trilinear(const uint8_t* pool, const TexDesc& D, float u, float v, float lod) {
+__device__ __forceinline__ TriLevels tri_levels(const uint8_t* pool, const TexDesc& D, float lod) {
     lod = fminf(fmaxf(lod, 0.0f), (float)(D.levels - 1));
     const int l0 = (int)floorf(lod);
     const int l1 = min(l0 + 1, D.levels - 1);
-    const float f = lod - (float)l0;
-    const float4 a = bilinear(pool, D, l0, u, v);
-    if (f == 0.0f) return a;
-    return lerp4(a, bilinear(pool, D, l1, u, v), f);
+    const int lw = __builtin_ctz((unsigned)D.width), lh = __builtin_ctz((unsigned)D.height);
+    TriLevels L;
+    L.t0 = reinterpret_cast<const uchar4*>(pool + D.base) + D.levelOffset[l0];
+    L.d1 = D.levelOffset[l1] - D.levelOffset[l0];
+    L.dims = (unsigned)max(lw - l0, 0) | ((unsigned)max(lh - l0, 0) << 5) | ((unsigned)max(lw - l1, 0) << 10) |
+             ((unsigned)max(lh - l1, 0) << 15);
+    L.f = lod - (float)l0;
+    return L;
+}
+
+// Bilinear lookup in one level of 2^lw x 2^lh texels with repeat
+// addressing, texel centres at (i + 0.5) / size; the result is on the 0..255
+// scale of the stored bytes.
+__device__ __forceinline__ float4 bilinear_level(const uchar4* __restrict__ T, int lw, int lh, float u, float v) {
+    const int w = 1 << lw, h = 1 << lh;
+    const float s = fmaf(u, (float)w, -0.5f), r = fmaf(v, (float)h, -0.5f);
+    const float fs = floorf(s), fr = floorf(r);
+    const float ax = s - fs, ay = r - fr;
+    const int x0 = (int)fs & (w - 1), y0 = (int)fr & (h - 1);
+    const int x1 = (x0 + 1) & (w - 1), y1 = (y0 + 1) & (h - 1);
+    const uchar4 a = T[(y0 << lw) + x0], b = T[(y0 << lw) + x1], c = T[(y1 << lw) + x0], d = T[(y1 << lw) + x1];
+    const float4 c00 = make_float4(a.x, a.y, a.z, a.w), c10 = make_float4(b.x, b.y, b.z, b.w);
+    const float4 c01 = make_float4(c.x, c.y, c.z, c.w), c11 = make_float4(d.x, d.y, d.z, d.w);
+    return lerp4(lerp4(c00, c10, ax), lerp4(c01, c11, ax), ay);
+}
+
+// Trilinear lookup with prepared levels, scaled to 0..1 once at the end.
+__device__ __forceinline__ float4 trilinear_levels(const TriLevels& L, float u, float v) {
+    constexpr float k = 1.0f / 255.0f;
+    float4 a = bilinear_level(L.t0, (int)(L.dims & 31u), (int)((L.dims >> 5) & 31u), u, v);
+    if (L.f != 0.0f)
+        a = lerp4(a, bilinear_level(L.t0 + L.d1, (int)((L.dims >> 10) & 31u), (int)((L.dims >> 15) & 31u), u, v), L.f);
+    return make_float4(a.x * k, a.y * k, a.z * k, a.w * k);
+}
+
+__device__ float4 trilinear(const uint8_t* pool, const TexDesc& D, float u, float v, float lod) {
+    return trilinear_levels(tri_levels(pool, D, lod), u, v);
 }
 
 // Footprint of a textured fragment: number of taps along the major axis,
@@ -425,6 +450,7 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
     __syncthreads();
 
     if (P.program == 5 && S == 1 && inFb && nf == 0) P.neural.featOut[4 * (pixBase + pix) + 1] = make_float4(0, 0, 0, 0);
+    const bool layered = P.program == 3 || P.program == 5;  // three textures per tap
     const float o = S == 1 ? 0.0f : 0.5f;
     float4 sum = make_float4(0, 0, 0, 0);
     for (int c0 = 0; c0 < F; c0 += 64) {
@@ -446,6 +472,14 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
                 cLod[lane] = R.fp.lod;
                 cN[lane] = n;
                 cMat[lane] = R.mat;
+                if (!NEURAL && !layered) {
+                    // the material's two levels, once per fragment, over tCol2 (only the layered and
+                    // neural programs use it)
+                    const TriLevels L = tri_levels(pool, tex[R.mat], R.fp.lod);
+                    const unsigned long long a = (unsigned long long)(uintptr_t)L.t0;
+                    reinterpret_cast<uint4*>(tCol2)[lane] = make_uint4((unsigned)a, (unsigned)(a >> 32), L.d1, L.dims);
+                    cLod[lane] = L.f;
+                }
             }
         }
         const int tincl = wave_incl_scan(n);
@@ -475,11 +509,19 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
                     tCol2[j] = make_float4(z[4], z[5], z[6], z[7]);
                     continue;
                 }
-                tCol[j] = trilinear(pool, tex[cMat[fl]], tu, tv, cLod[fl]);
-                if (P.program == 3 || P.program == 5) {
+                if (layered) {
+                    tCol[j] = trilinear(pool, tex[cMat[fl]], tu, tv, cLod[fl]);
                     const float4 nm = trilinear(pool, tex[1], tu, tv, cLod[fl]);
                     const float4 rg = trilinear(pool, tex[2], tu, tv, cLod[fl]);
                     tCol2[j] = make_float4(nm.x, nm.y, nm.z, rg.x);
+                } else {
+                    const uint4 q = reinterpret_cast<const uint4*>(tCol2)[fl];
+                    TriLevels L;
+                    L.t0 = reinterpret_cast<const uchar4*>((uintptr_t)(((unsigned long long)q.y << 32) | q.x));
+                    L.d1 = q.z;
+                    L.dims = q.w;
+                    L.f = cLod[fl];
+                    tCol[j] = trilinear_levels(L, tu, tv);
                 }
             }
             __syncthreads();
@@ -490,7 +532,7 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
                 acc.y += c.y;
                 acc.z += c.z;
                 acc.w += c.w;
-                if (P.program >= 3) {
+                if (NEURAL || layered) {
                     const float4 c2 = tCol2[j];
                     acc2.x += c2.x;
                     acc2.y += c2.y;
@@ -761,12 +803,13 @@ __global__ void __launch_bounds__(256) k_blend_ordered(ShadeParams P, BlendCoefs
     }
 }
 
+// Four waves per SIMD (at most 128 VGPRs), as the neural program.
 template <int S, int N>
-__global__ void __launch_bounds__(64) k_shade(ShadeParams P, Views V, TriRecs recs, const float4* __restrict__ bary,
-                                              const uint32_t* __restrict__ visSlot, const float* __restrict__ vtx,
-                                              const int* __restrict__ idx, const TexDesc* __restrict__ tex,
-                                              const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8,
-                                              float4* __restrict__ rgba32f, int tiles, int perXcd) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_shade(ShadeParams P, Views V, TriRecs recs, const float4* __restrict__ bary, const uint32_t* __restrict__ visSlot,
+        const float* __restrict__ vtx, const int* __restrict__ idx, const TexDesc* __restrict__ tex,
+        const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8, float4* __restrict__ rgba32f, int tiles,
+        int perXcd) {
     shade_tiles<S, false, false, N>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles, perXcd);
 }
 
