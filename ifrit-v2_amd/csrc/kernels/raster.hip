@@ -130,25 +130,18 @@ __device__ __forceinline__ unsigned long long sum_nt(unsigned long long v) {
     }
 }
 
-// Candidate mask of one half of a tile (block rows 2h and 2h + 1) for one
-// list entry; two lanes share an entry.  S > 1: 128 pixel bits in four words
-// (word w holds tile rows 4h + 2w' ... as bit (y & 1) * 16 + x of tile words
-// 4h..4h+3); S == 1: 16 group bits ((block - 8h) * 2 + row pair).  Also
-// returns the trivially accepted blocks of the half (bit block - 8h).
-template <int S>
-__device__ __forceinline__ void entry_mask_half(const Edges& E, bool full, int tx, int ty, int h,
-                                                const unsigned* fbMask, unsigned* m, unsigned& ta) {
-    constexpr int NW = S == 1 ? 1 : 4;
-    const float tileMinX = (float)(tx * 16) * 1.0f, tileMinY = (float)(ty * 16) * 1.0f;
-#pragma unroll
-    for (int w = 0; w < NW; w++) m[w] = 0u;
+// block_test of the eight 4x4 blocks of tile half h (blocks 8h .. 8h + 7,
+// block rows 2h and 2h + 1): bit j of tr when block 8h + j is not rejected,
+// of ta when it is trivially accepted; a tile-accepted entry (full) takes
+// every block.
+__device__ __forceinline__ void half_blocks(const Edges& E, bool full, int tx, int ty, int h, unsigned& tr,
+                                            unsigned& ta) {
     if (full) {
-        ta = 0xFFu;
-#pragma unroll
-        for (int w = 0; w < NW; w++) m[w] = fbMask[w];
+        tr = ta = 0xFFu;
         return;
     }
-    unsigned tr = 0;
+    const float tileMinX = (float)(tx * 16) * 1.0f, tileMinY = (float)(ty * 16) * 1.0f;
+    tr = 0;
     ta = 0;
 #pragma unroll
     for (int j = 0; j < 8; j++) {
@@ -158,40 +151,41 @@ __device__ __forceinline__ void entry_mask_half(const Edges& E, bool full, int t
         tr |= (unsigned)(trn == 3) << j;
         ta |= (unsigned)(tan == 3) << j;
     }
+}
+
+// Pixels of a 4x4 block inside the frame (bit 4 * row + column), given the
+// frame's columns and rows left from the block's corner.
+__device__ __forceinline__ unsigned block_fb_mask(int ncols, int nrows) {
+    const unsigned c = ncols >= 4 ? 0xFu : (ncols <= 0 ? 0u : (1u << ncols) - 1u);
+    const unsigned r = nrows >= 4 ? 0xFFFFu : (nrows <= 0 ? 0u : (1u << (4 * nrows)) - 1u);
+    return (c * 0x1111u) & r;
+}
+
+// Candidate 4x2 groups of one half of a tile at 1 sample per pixel for one
+// list entry (two lanes share an entry): 16 bits (block - 8h) * 2 + row
+// pair, masked by fbMask.  Also returns the trivially accepted blocks of the
+// half (bit block - 8h).
+__device__ __forceinline__ void group_mask_half(const Edges& E, bool full, int tx, int ty, int h, unsigned fbMask,
+                                                unsigned& m, unsigned& ta) {
+    unsigned tr;
+    half_blocks(E, full, tx, ty, h, tr, ta);
+    if (full) {
+        m = fbMask;
+        return;
+    }
+    m = 0;
     while (tr) {
         const int j = __ffs(tr) - 1;
         tr &= tr - 1u;
         const int bi = 8 * h + j;
-        const bool acc = (ta >> j) & 1u;
         const float x0 = (float)(tx * 16 + (bi & 3) * 4), y0 = (float)(ty * 16 + (bi >> 2) * 4);
-        if constexpr (S == 1) {
-            unsigned g = 3u;
-            if (!acc)
-                g = (unsigned)rect_may_cover(E, x0, y0, 3.0f, 1.0f) |
-                    ((unsigned)rect_may_cover(E, x0, y0 + 2.0f, 3.0f, 1.0f) << 1);
-            m[0] |= g << (j * 2);
-        } else {
-            // block rows 0-1 go to local word (j >> 2) * 2, rows 2-3 to the next,
-            // at bit offset (row & 1) * 16 + (bi & 3) * 4
-            unsigned lo = 0xF000Fu, hi = 0xF000Fu;
-            if (!acc) {
-                // rows 0-1 of the block to lo, rows 2-3 to hi, odd rows at bit 16
-                const unsigned b = block_square_mask(E, x0, y0);
-                lo = (b & 0xFu) | ((b & 0xF0u) << 12);
-                hi = ((b >> 8) & 0xFu) | ((b & 0xF000u) << 4);
-            }
-            const int sh = (bi & 3) * 4;
-            if (j < 4) {
-                m[0] |= lo << sh;
-                m[1] |= hi << sh;
-            } else {
-                m[2] |= lo << sh;
-                m[3] |= hi << sh;
-            }
-        }
+        unsigned g = 3u;
+        if (!((ta >> j) & 1u))
+            g = (unsigned)rect_may_cover(E, x0, y0, 3.0f, 1.0f) |
+                ((unsigned)rect_may_cover(E, x0, y0 + 2.0f, 3.0f, 1.0f) << 1);
+        m |= g << (j * 2);
     }
-#pragma unroll
-    for (int w = 0; w < NW; w++) m[w] &= fbMask[w];
+    m &= fbMask;
 }
 
 // Coverage class and depth of the winning triangle at a 1-sample pixel,
@@ -367,9 +361,9 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
          const unsigned* __restrict__ binList, const uint4* __restrict__ items, unsigned itemCap,
          unsigned long long* __restrict__ partials, float* __restrict__ visDepthAll, uint32_t* __restrict__ visSlotAll,
          uint8_t* __restrict__ visClassAll, StatPartials st, DeviceCounters* ctr) {
-    constexpr int NW = S == 1 ? 1 : 4;      // mask words of one half tile
     constexpr int PASS = NT / 2;            // list entries per pass through LDS
-    constexpr int UNITS = NT * 16;          // units expanded into LDS per round
+    constexpr int UNITS = S == 1 ? NT * 16 : NT * 8;  // units expanded into LDS per round
+    constexpr int PAIRS = S == 1 ? 1 : 16 * PASS;     // (entry, block) pairs of a pass (S > 1)
     constexpr int PPT = 256 / NT;           // tile pixels per thread
     static_assert(PASS <= kPass, "units encode the entry in 7 bits");
     __shared__ unsigned long long sVis[S * 256];  // key of sample s of tile pixel p at s * 256 + p
@@ -377,6 +371,7 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
     __shared__ TriRec sRec[PASS];
     __shared__ unsigned char sTa[2 * PASS];  // accepted blocks of each half
     __shared__ unsigned short sUnit[UNITS];
+    __shared__ unsigned short sPair[PAIRS];
     __shared__ unsigned sWave[4];
     __shared__ unsigned sItem;
 
@@ -423,122 +418,164 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
 
         // two lanes per list entry: lane h = t & 1 takes block rows 2h, 2h + 1
         const int e0 = t >> 1, h = t & 1;
-        // framebuffer mask of this half tile in the candidate layout
+        // frame mask of this half tile: 4x2 groups with a pixel inside (S == 1;
+        // pixels are checked again per unit), blocks with a pixel inside (S > 1)
         const int cols = min(16, G.width - tx0), rows = min(16, G.height - ty0);
-        unsigned fbMask[NW];
+        unsigned fbMask = 0, blkFb = 0;
         if constexpr (S == 1) {
-            // groups with at least one pixel inside; pixels are checked again per unit
-            unsigned f = 0;
 #pragma unroll
             for (int j = 0; j < 16; j++) {
                 const int g = 16 * h + j;
-                f |= (unsigned)(((g >> 1) & 3) * 4 < cols && ((g >> 3) * 4 + (g & 1) * 2) < rows) << j;
+                fbMask |= (unsigned)(((g >> 1) & 3) * 4 < cols && ((g >> 3) * 4 + (g & 1) * 2) < rows) << j;
             }
-            fbMask[0] = f;
         } else {
-            const unsigned row = cols >= 16 ? 0xFFFFu : ((1u << cols) - 1u);
 #pragma unroll
-            for (int w = 0; w < 4; w++) {
-                const int r0 = 2 * (4 * h + w);
-                fbMask[w] = (r0 < rows ? row : 0u) | (r0 + 1 < rows ? row << 16 : 0u);
-            }
+            for (int j = 0; j < 8; j++) blkFb |= (unsigned)((j & 3) * 4 < cols && (2 * h + (j >> 2)) * 4 < rows) << j;
         }
+
+        // merges units sUnit[0, nu) (entry << 8 | tile pixel or group) into sVis
+        auto merge_units = [&](int nu) {
+            for (int i = t; i < nu; i += NT) {
+                const unsigned u = sUnit[i];
+                const int e = (int)(u >> 8), c = (int)(u & 255u);
+                const TriRec R = sRec[e];
+                const Edges E = load_edges(R);
+                const unsigned tie = __float_as_uint(R.q2.y);
+                if constexpr (S == 1) {
+                    const int bi = c >> 1;
+                    const unsigned ta = sTa[2 * e + (bi >> 3)];
+                    const int px0 = tx0 + (bi & 3) * 4, gy = ty0 + (bi >> 2) * 4 + (c & 1) * 2;
+                    // pixels of the 4x2 group: (0,0) (1,0) (0,1) (1,1) (2,0) (3,0) (2,1) (3,1)
+                    unsigned acc = 0xFFu;
+                    if (!((ta >> (bi & 7)) & 1u)) {
+                        acc = 0;
+#pragma unroll
+                        for (int l = 0; l < 8; l++) {
+                            const int ox = (l & 1) + ((l >> 2) << 1), oy = (l >> 1) & 1;
+                            acc |= (unsigned)inside(E, (float)(px0 + ox), (float)(gy + oy)) << l;
+                        }
+                    }
+#pragma unroll
+                    for (int l = 0; l < 8; l++) {
+                        if (!((acc >> l) & 1u)) continue;
+                        const int px = px0 + (l & 1) + ((l >> 2) << 1), py = gy + ((l >> 1) & 1);
+                        if (px >= G.width || py >= G.height) continue;
+                        const unsigned quad = (acc >> (l & 4)) & 0xFu;
+                        const unsigned cls = acc == 0xFFu ? 1u : (quad == 0xFu ? 2u : 3u);
+                        frags++;
+                        covs++;
+                        merge(&sVis[(py - ty0) * 16 + (px - tx0)],
+                              class_depth(cls, R.f1, R.f2, R.f3, (float)px, (float)py), tie);
+                    }
+                } else {
+                    const int lx = c & 15, ly = c >> 4;
+                    const int bi = (ly >> 2) * 4 + (lx >> 2);
+                    const bool accept = (sTa[2 * e + (bi >> 3)] >> (bi & 7)) & 1u;
+                    const float fx = (float)(tx0 + lx), fy = (float)(ty0 + ly);
+                    bool any = false;
+#pragma unroll
+                    for (int s = 0; s < S; s++) {
+                        const float sx = fx + P.sx[s], sy = fy + P.sy[s];
+                        if (!accept && !inside(E, sx, sy)) continue;
+                        any = true;
+                        covs++;
+                        merge(&sVis[s * 256 + c], depth256(R.f1, R.f2, R.f3, sx, sy), tie);
+                    }
+                    frags += any;
+                }
+            }
+        };
 
         // passes of PASS entries through LDS, all merging into sVis
         for (int pb = 0; pb < cnt; pb += PASS) {
             const int pc = min(PASS, cnt - pb);
-            unsigned m[NW];
-            unsigned n = 0;
-            if (e0 < pc) {
-                const unsigned ent = binList[first + pb + e0];
-                const TriRec R = load_rec(recs, ent & 0x7fffffffu);
-                if (h == 0) sRec[e0] = R;
-                unsigned ta;
-                entry_mask_half<S>(load_edges(R), (ent >> 31) != 0, ttx, tty, h, fbMask, m, ta);
-                sTa[t] = (unsigned char)ta;
-#pragma unroll
-                for (int w = 0; w < NW; w++) n += __popc(m[w]);
-            } else {
-#pragma unroll
-                for (int w = 0; w < NW; w++) m[w] = 0u;
-            }
-            unsigned T;
-            const unsigned off = scan_nt<NT>(n, sWave, T);
+            if constexpr (S == 1) {
+                unsigned m = 0;
+                if (e0 < pc) {
+                    const unsigned ent = binList[first + pb + e0];
+                    const TriRec R = load_rec(recs, ent & 0x7fffffffu);
+                    if (h == 0) sRec[e0] = R;
+                    unsigned ta;
+                    group_mask_half(load_edges(R), (ent >> 31) != 0, ttx, tty, h, fbMask, m, ta);
+                    sTa[t] = (unsigned char)ta;
+                }
+                const unsigned n = (unsigned)__popc(m);
+                unsigned T;
+                const unsigned off = scan_nt<NT>(n, sWave, T);
 
-            for (unsigned w0 = 0; w0 < T; w0 += UNITS) {
-                // write this lane's units that fall in [w0, w0 + UNITS)
-                int pos = (int)off - (int)w0;
-                if (n && pos < UNITS && pos + (int)n > 0) {
-#pragma unroll
-                    for (int w = 0; w < NW; w++) {
-                        unsigned bits = m[w];
-                        const int c = __popc(bits);
-                        if (pos + c <= 0 || pos >= UNITS) {
-                            pos += c;
-                            continue;
-                        }
+                for (unsigned w0 = 0; w0 < T; w0 += UNITS) {
+                    // write this lane's units that fall in [w0, w0 + UNITS): entry << 8 | group
+                    int pos = (int)off - (int)w0;
+                    if (n && pos < UNITS && pos + (int)n > 0) {
+                        unsigned bits = m;
                         while (bits) {
                             const int b = __ffs(bits) - 1;
                             bits &= bits - 1u;
-                            // unit: entry << 8 | tile pixel (S > 1) or group (S == 1)
-                            const int cu = S == 1 ? 16 * h + b : (4 * h + w) * 32 + b;
-                            if (pos >= 0 && pos < UNITS) sUnit[pos] = (unsigned short)((e0 << 8) | cu);
+                            if (pos >= 0 && pos < UNITS) sUnit[pos] = (unsigned short)((e0 << 8) | (16 * h + b));
                             pos++;
                         }
                     }
+                    __syncthreads();
+                    merge_units((int)min((unsigned)UNITS, T - w0));
+                    __syncthreads();
+                }
+            } else {
+                // entry lanes find the blocks their half may cover; every
+                // (entry, block) pair then takes one lane for its pixel mask
+                unsigned tr = 0;
+                if (e0 < pc) {
+                    const unsigned ent = binList[first + pb + e0];
+                    const TriRec R = load_rec(recs, ent & 0x7fffffffu);
+                    if (h == 0) sRec[e0] = R;
+                    unsigned ta;
+                    half_blocks(load_edges(R), (ent >> 31) != 0, ttx, tty, h, tr, ta);
+                    tr &= blkFb;
+                    sTa[t] = (unsigned char)ta;
+                }
+                unsigned TP;
+                int pp = (int)scan_nt<NT>((unsigned)__popc(tr), sWave, TP);
+                while (tr) {
+                    const int j = __ffs(tr) - 1;
+                    tr &= tr - 1u;
+                    sPair[pp++] = (unsigned short)((e0 << 4) | (8 * h + j));
                 }
                 __syncthreads();
-                const int nu = (int)min((unsigned)UNITS, T - w0);
-                for (int i = t; i < nu; i += NT) {
-                    const unsigned u = sUnit[i];
-                    const int e = (int)(u >> 8), c = (int)(u & 255u);
-                    const TriRec R = sRec[e];
-                    const Edges E = load_edges(R);
-                    const unsigned tie = __float_as_uint(R.q2.y);
-                    if constexpr (S == 1) {
-                        const int bi = c >> 1;
-                        const unsigned ta = sTa[2 * e + (bi >> 3)];
-                        const int px0 = tx0 + (bi & 3) * 4, gy = ty0 + (bi >> 2) * 4 + (c & 1) * 2;
-                        // pixels of the 4x2 group: (0,0) (1,0) (0,1) (1,1) (2,0) (3,0) (2,1) (3,1)
-                        unsigned acc = 0xFFu;
-                        if (!((ta >> (bi & 7)) & 1u)) {
-                            acc = 0;
-#pragma unroll
-                            for (int l = 0; l < 8; l++) {
-                                const int ox = (l & 1) + ((l >> 2) << 1), oy = (l >> 1) & 1;
-                                acc |= (unsigned)inside(E, (float)(px0 + ox), (float)(gy + oy)) << l;
+                for (unsigned p0 = 0; p0 < TP; p0 += NT) {
+                    unsigned pm = 0;
+                    int pe = 0, bi = 0;
+                    if (p0 + t < TP) {
+                        const unsigned pr = sPair[p0 + t];
+                        pe = (int)(pr >> 4);
+                        bi = (int)(pr & 15u);
+                        pm = block_fb_mask(cols - (bi & 3) * 4, rows - (bi >> 2) * 4);
+                        if (!((sTa[2 * pe + (bi >> 3)] >> (bi & 7)) & 1u)) {
+                            const Edges E = load_edges(sRec[pe]);
+                            pm &= block_square_mask(E, (float)(tx0 + (bi & 3) * 4), (float)(ty0 + (bi >> 2) * 4));
+                        }
+                    }
+                    const int n = __popc(pm);
+                    unsigned T;
+                    const unsigned off = scan_nt<NT>((unsigned)n, sWave, T);
+                    for (unsigned w0 = 0; w0 < T; w0 += UNITS) {
+                        // write this lane's units that fall in [w0, w0 + UNITS): tile pixel
+                        // (4 (bi >> 2) + (b >> 2)) * 16 + 4 (bi & 3) + (b & 3) of block bit b
+                        int pos = (int)off - (int)w0;
+                        if (n && pos < UNITS && pos + n > 0) {
+                            unsigned bits = pm;
+                            while (bits) {
+                                const int b = __ffs(bits) - 1;
+                                bits &= bits - 1u;
+                                const int c = ((bi >> 2) * 4 + (b >> 2)) * 16 + (bi & 3) * 4 + (b & 3);
+                                if (pos >= 0 && pos < UNITS) sUnit[pos] = (unsigned short)((pe << 8) | c);
+                                pos++;
                             }
                         }
-#pragma unroll
-                        for (int l = 0; l < 8; l++) {
-                            if (!((acc >> l) & 1u)) continue;
-                            const int px = px0 + (l & 1) + ((l >> 2) << 1), py = gy + ((l >> 1) & 1);
-                            if (px >= G.width || py >= G.height) continue;
-                            const unsigned quad = (acc >> (l & 4)) & 0xFu;
-                            const unsigned cls = acc == 0xFFu ? 1u : (quad == 0xFu ? 2u : 3u);
-                            frags++;
-                            covs++;
-                            merge(&sVis[(py - ty0) * 16 + (px - tx0)],
-                                  class_depth(cls, R.f1, R.f2, R.f3, (float)px, (float)py), tie);
-                        }
-                    } else {
-                        const int lx = c & 15, ly = c >> 4;
-                        const int bi = (ly >> 2) * 4 + (lx >> 2);
-                        const bool accept = (sTa[2 * e + (bi >> 3)] >> (bi & 7)) & 1u;
-                        const float fx = (float)(tx0 + lx), fy = (float)(ty0 + ly);
-                        bool any = false;
-#pragma unroll
-                        for (int s = 0; s < S; s++) {
-                            const float sx = fx + P.sx[s], sy = fy + P.sy[s];
-                            if (!accept && !inside(E, sx, sy)) continue;
-                            any = true;
-                            covs++;
-                            merge(&sVis[s * 256 + c], depth256(R.f1, R.f2, R.f3, sx, sy), tie);
-                        }
-                        frags += any;
+                        __syncthreads();
+                        merge_units((int)min((unsigned)UNITS, T - w0));
+                        __syncthreads();
                     }
+                    if (T == 0) __syncthreads();  // sWave is read by the next round's scan
                 }
-                __syncthreads();
             }
             __syncthreads();  // sRec, sTa and sWave are reused by the next pass
         }
