@@ -285,13 +285,13 @@ struct TileOut {
     unsigned slot[256 * S];
 };
 
-// Outputs of one pixel of a cleared draw into the staging buffer; the 1-spp
-// class byte goes straight out.
+// Outputs of one pixel of a cleared draw from its S final keys: depth and
+// record slot per sample (the clear values where no fragment won) and, at 1
+// sample per pixel, the coverage class.
 template <int S>
-__device__ __forceinline__ void stage_pixel(const RasterParams& P, const TriRecs& recs, const unsigned* slotOf,
-                                            const PixelCtx& C, int t, const unsigned long long* keys, TileOut<S>& O,
-                                            uint8_t* visClass) {
-    if (!C.inFb) return;
+__device__ __forceinline__ unsigned pixel_outputs(const RasterParams& P, const TriRecs& recs, const unsigned* slotOf,
+                                                  const PixelCtx& C, const unsigned long long* keys, bool wantClass,
+                                                  float* d, unsigned* slot) {
     unsigned cls = 0;
     const unsigned long long clearKey = frag_key(kClearDepth, 0u);
     unsigned slots[S];
@@ -299,23 +299,61 @@ __device__ __forceinline__ void stage_pixel(const RasterParams& P, const TriRecs
 #pragma unroll
     for (int s = 0; s < S; s++) {
         const unsigned long long k = keys[s];
-        float d = kClearDepth;
-        unsigned slot = 0xffffffffu;
+        d[s] = kClearDepth;
+        slot[s] = 0xffffffffu;
         if (k != clearKey) {
-            slot = slots[s];
-            d = order_key_inv((unsigned)(k >> 32));
-            if (S == 1 && (visClass || (unsigned)(k >> 32) == 0x80000000u)) {
+            slot[s] = slots[s];
+            d[s] = order_key_inv((unsigned)(k >> 32));
+            if (S == 1 && (wantClass || (unsigned)(k >> 32) == 0x80000000u)) {
                 // the class output, or +0 and -0 sharing a key: recompute from the winner's record
-                cls = winner_class(load_rec(recs, slot), C.tx, C.ty, C.fpx, C.fpy, C.bMinX, C.bMinY, d);
+                cls = winner_class(load_rec(recs, slot[s]), C.tx, C.ty, C.fpx, C.fpy, C.bMinX, C.bMinY, d[s]);
             } else if (S > 1 && (unsigned)(k >> 32) == 0x80000000u) {
                 // +0 and -0 share a key; recompute to return the winner's own bits
-                d = depth256(recs.f1[slot], recs.f2[slot], recs.f3[slot], C.fpx + P.sx[s], C.fpy + P.sy[s]);
+                d[s] = depth256(recs.f1[slot[s]], recs.f2[slot[s]], recs.f3[slot[s]], C.fpx + P.sx[s],
+                                C.fpy + P.sy[s]);
             }
         }
-        O.d[t * S + s] = d;
-        O.slot[t * S + s] = slot;
+    }
+    return cls;
+}
+
+// Outputs of one pixel of a cleared draw into the staging buffer; the 1-spp
+// class byte goes straight out.
+template <int S>
+__device__ __forceinline__ void stage_pixel(const RasterParams& P, const TriRecs& recs, const unsigned* slotOf,
+                                            const PixelCtx& C, int t, const unsigned long long* keys, TileOut<S>& O,
+                                            uint8_t* visClass) {
+    if (!C.inFb) return;
+    float d[S];
+    unsigned slot[S];
+    const unsigned cls = pixel_outputs<S>(P, recs, slotOf, C, keys, visClass != nullptr, d, slot);
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        O.d[t * S + s] = d[s];
+        O.slot[t * S + s] = slot[s];
     }
     if (S == 1 && visClass) visClass[C.pix] = (uint8_t)cls;
+}
+
+// Outputs of one pixel of a cleared draw straight to memory with 16-byte
+// stores (S a multiple of 4: a pixel's samples fill whole 16-byte words, and
+// a row of pixels whole lines).
+template <int S>
+__device__ __forceinline__ void emit_pixel(const RasterParams& P, const TriRecs& recs, const unsigned* slotOf,
+                                           const PixelCtx& C, const unsigned long long* keys, float* visDepth,
+                                           uint32_t* visSlot) {
+    static_assert(S % 4 == 0, "whole 16-byte words per pixel");
+    if (!C.inFb) return;
+    float d[S];
+    unsigned slot[S];
+    pixel_outputs<S>(P, recs, slotOf, C, keys, false, d, slot);
+    float4* const dd = reinterpret_cast<float4*>(visDepth + C.pix * S);
+    uint4* const ds = reinterpret_cast<uint4*>(visSlot + C.pix * S);
+#pragma unroll
+    for (int v = 0; v < S / 4; v++) {
+        dd[v] = make_float4(d[4 * v], d[4 * v + 1], d[4 * v + 2], d[4 * v + 3]);
+        ds[v] = make_uint4(slot[4 * v], slot[4 * v + 1], slot[4 * v + 2], slot[4 * v + 3]);
+    }
 }
 
 // Stores a staged tile with 16-byte stores; partial columns at the right
@@ -595,16 +633,25 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
 #pragma unroll
                 for (int s = 0; s < S; s++) keys[k][s] = sVis[s * 256 + t + k * NT];
             if (P.clear) {
-                // every key is in registers: stage the outputs over sVis
-                TileOut<S>* O = reinterpret_cast<TileOut<S>*>(sVis);
-                __syncthreads();
+                if constexpr (S > 1) {
+                    // a pixel's samples are whole 16-byte words: straight out
 #pragma unroll
-                for (int k = 0; k < PPT; k++) {
-                    const int p = t + k * NT;
-                    stage_pixel<S>(P, recs, slotOf, pixel_ctx(G, p & 15, p >> 4), p, keys[k], *O, visClass);
+                    for (int k = 0; k < PPT; k++) {
+                        const int p = t + k * NT;
+                        emit_pixel<S>(P, recs, slotOf, pixel_ctx(G, p & 15, p >> 4), keys[k], visDepth, visSlot);
+                    }
+                } else {
+                    // every key is in registers: stage the outputs over sVis
+                    TileOut<S>* O = reinterpret_cast<TileOut<S>*>(sVis);
+                    __syncthreads();
+#pragma unroll
+                    for (int k = 0; k < PPT; k++) {
+                        const int p = t + k * NT;
+                        stage_pixel<S>(P, recs, slotOf, pixel_ctx(G, p & 15, p >> 4), p, keys[k], *O, visClass);
+                    }
+                    __syncthreads();
+                    store_tile<S, NT>(G, ttx, tty, O, visDepth, visSlot);
                 }
-                __syncthreads();
-                store_tile<S, NT>(G, ttx, tty, O, visDepth, visSlot);
             } else {
 #pragma unroll
                 for (int k = 0; k < PPT; k++) {
@@ -637,7 +684,6 @@ __global__ void __launch_bounds__(256) k_raster_merge(RasterParams P, Views V, T
                                                       float* __restrict__ visDepthAll, uint32_t* __restrict__ visSlotAll,
                                                       uint8_t* __restrict__ visClassAll, int tiles, int partShift,
                                                       const DeviceCounters* ctr) {
-    __shared__ TileOut<S> O;
     const unsigned numMulti = ctr->numMulti;
     const unsigned numEmpty = P.clear ? ctr->numEmpty : 0u;
     const int t = threadIdx.x;
@@ -706,10 +752,15 @@ __global__ void __launch_bounds__(256) k_raster_merge(RasterParams P, Views V, T
             for (int s = 0; s < S; s++) keys[s] = min(keys[s], g[s * 256]);
         }
         if (P.clear) {
-            __syncthreads();  // the previous tile's staging has been stored
-            stage_pixel<S>(P, recs, slotOf, C, t, keys, O, visClass);
-            __syncthreads();
-            store_tile<S>(G, C.tx, C.ty, &O, visDepth, visSlot);
+            if constexpr (S > 1) {
+                emit_pixel<S>(P, recs, slotOf, C, keys, visDepth, visSlot);
+            } else {
+                __shared__ TileOut<S> O;  // output staging at 1 sample per pixel
+                __syncthreads();          // the previous tile's staging has been stored
+                stage_pixel<S>(P, recs, slotOf, C, t, keys, O, visClass);
+                __syncthreads();
+                store_tile<S>(G, C.tx, C.ty, &O, visDepth, visSlot);
+            }
         } else {
             write_pixel<S>(P, recs, slotOf, C, keys, visDepth, visSlot, visClass);
         }
