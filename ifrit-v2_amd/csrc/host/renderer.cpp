@@ -471,6 +471,11 @@ bool TileRasterRenderer::layoutViews(int ntri) {
         setError("too many triangles times views: record slots must stay below 2^31");
         return false;
     }
+    if (samples_ == 1 && ntri >= (1 << 28)) {
+        // 1-spp fragment keys carry the coverage class below the primitive key
+        setError("too many triangles: at 1 sample per pixel a draw takes fewer than 2^28");
+        return false;
+    }
     std::vector<ViewDesc> d(hv.size());
     std::vector<int> layout;
     layout.reserve(3 * hv.size() + 1);

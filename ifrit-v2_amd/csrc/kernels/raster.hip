@@ -221,6 +221,16 @@ __device__ __forceinline__ unsigned winner_class(const TriRec& R, int tx, int ty
     return c;
 }
 
+// Fragment keys: the order key of the depth in the high word; in the low
+// word the primitive key prim * 4 + fan (S > 1), or that key times 4 plus
+// the fragment's coverage class at 1 sample per pixel (a pixel never holds
+// two fragments of one (prim, fan), so the class bits never decide a
+// minimum; primitives stay below 2^28 there).
+template <int S>
+__device__ __forceinline__ unsigned key_tie(unsigned long long k) {
+    return S == 1 ? (unsigned)k >> 2 : (unsigned)k;
+}
+
 // Record slots of a pixel's S final keys (slotOf[prim] + fan), all S lookups
 // issued before the first is used; a key that holds no fragment looks up
 // primitive 0 and its slot is not used.
@@ -229,7 +239,7 @@ __device__ __forceinline__ void key_slots(const unsigned* slotOf, const unsigned
                                           unsigned long long none, unsigned* slots) {
 #pragma unroll
     for (int s = 0; s < S; s++) {
-        const unsigned tie = (unsigned)keys[s];
+        const unsigned tie = key_tie<S>(keys[s]);
         slots[s] = slotOf[keys[s] != none ? tie >> 2 : 0u] + (tie & 3u);
     }
 }
@@ -262,9 +272,11 @@ __device__ __forceinline__ void write_pixel(const RasterParams& P, const TriRecs
         won = true;
         const unsigned slot = slots[s];
         float d = order_key_inv((unsigned)(k >> 32));
-        if (S == 1 && (visClass || (unsigned)(k >> 32) == 0x80000000u)) {
-            // the class output, or +0 and -0 sharing a key: recompute from the winner's record
+        if (S == 1 && (unsigned)(k >> 32) == 0x80000000u) {
+            // +0 and -0 share a key: recompute the winner's own depth bits from its record
             cls = winner_class(load_rec(recs, slot), C.tx, C.ty, C.fpx, C.fpy, C.bMinX, C.bMinY, d);
+        } else if (S == 1) {
+            cls = (unsigned)k & 3u;
         } else if (S > 1 && (unsigned)(k >> 32) == 0x80000000u) {
             // +0 and -0 share a key; recompute to return the winner's own bits
             d = depth256(recs.f1[slot], recs.f2[slot], recs.f3[slot], C.fpx + P.sx[s], C.fpy + P.sy[s]);
@@ -290,8 +302,8 @@ struct TileOut {
 // sample per pixel, the coverage class.
 template <int S>
 __device__ __forceinline__ unsigned pixel_outputs(const RasterParams& P, const TriRecs& recs, const unsigned* slotOf,
-                                                  const PixelCtx& C, const unsigned long long* keys, bool wantClass,
-                                                  float* d, unsigned* slot) {
+                                                  const PixelCtx& C, const unsigned long long* keys, float* d,
+                                                  unsigned* slot) {
     unsigned cls = 0;
     const unsigned long long clearKey = frag_key(kClearDepth, 0u);
     unsigned slots[S];
@@ -304,9 +316,11 @@ __device__ __forceinline__ unsigned pixel_outputs(const RasterParams& P, const T
         if (k != clearKey) {
             slot[s] = slots[s];
             d[s] = order_key_inv((unsigned)(k >> 32));
-            if (S == 1 && (wantClass || (unsigned)(k >> 32) == 0x80000000u)) {
-                // the class output, or +0 and -0 sharing a key: recompute from the winner's record
+            if (S == 1 && (unsigned)(k >> 32) == 0x80000000u) {
+                // +0 and -0 share a key: recompute the winner's own depth bits from its record
                 cls = winner_class(load_rec(recs, slot[s]), C.tx, C.ty, C.fpx, C.fpy, C.bMinX, C.bMinY, d[s]);
+            } else if (S == 1) {
+                cls = (unsigned)k & 3u;
             } else if (S > 1 && (unsigned)(k >> 32) == 0x80000000u) {
                 // +0 and -0 share a key; recompute to return the winner's own bits
                 d[s] = depth256(recs.f1[slot[s]], recs.f2[slot[s]], recs.f3[slot[s]], C.fpx + P.sx[s],
@@ -326,7 +340,7 @@ __device__ __forceinline__ void stage_pixel(const RasterParams& P, const TriRecs
     if (!C.inFb) return;
     float d[S];
     unsigned slot[S];
-    const unsigned cls = pixel_outputs<S>(P, recs, slotOf, C, keys, visClass != nullptr, d, slot);
+    const unsigned cls = pixel_outputs<S>(P, recs, slotOf, C, keys, d, slot);
 #pragma unroll
     for (int s = 0; s < S; s++) {
         O.d[t * S + s] = d[s];
@@ -346,7 +360,7 @@ __device__ __forceinline__ void emit_pixel(const RasterParams& P, const TriRecs&
     if (!C.inFb) return;
     float d[S];
     unsigned slot[S];
-    pixel_outputs<S>(P, recs, slotOf, C, keys, false, d, slot);
+    pixel_outputs<S>(P, recs, slotOf, C, keys, d, slot);
     float4* const dd = reinterpret_cast<float4*>(visDepth + C.pix * S);
     uint4* const ds = reinterpret_cast<uint4*>(visSlot + C.pix * S);
 #pragma unroll
@@ -503,7 +517,7 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
                         frags++;
                         covs++;
                         merge(&sVis[(py - ty0) * 16 + (px - tx0)],
-                              class_depth(cls, R.f1, R.f2, R.f3, (float)px, (float)py), tie);
+                              class_depth(cls, R.f1, R.f2, R.f3, (float)px, (float)py), (tie << 2) | cls);
                     }
                 } else {
                     const int lx = c & 15, ly = c >> 4;
