@@ -422,7 +422,7 @@ bool TileRasterRenderer::prepare(int indexCount) {
                  clipList_.ensure((size_t)clipCap_ * 4 * nv) &&
                  statPart_.ensure(((size_t)(ntri + 255) / 256 * nv + 2 * (size_t)rasterGrid() + 2) * 8) &&
                  items_.ensure((size_t)itemCap_ * 16) && tilePartial_.ensure(tiles * 4) &&
-                 planSums_.ensure(((tiles + 2047) / 2048 + 1) * 6 * 4) &&
+                 planSums_.ensure(((tiles + kPlanTiles - 1) / kPlanTiles + 1) * 6 * 4) &&
                  multiList_.ensure(tiles * 8) && partials_.ensure((size_t)partialCap_ * 256 * samples_ * 8) &&
                  binCount_.ensure(tiles * 4) && binStart_.ensure(tiles * 4) && binCursor_.ensure(tiles * 4) &&
                  binList_.ensure((size_t)binCap_ * 4) && bigList_.ensure((size_t)bigCap_ * 4) &&

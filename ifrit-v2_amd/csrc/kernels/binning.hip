@@ -142,8 +142,8 @@ __global__ void __launch_bounds__(256) k_bin_big(BinParams P, Views V, TriRecs r
 // Lists need not be in tile order: every tile's range is contiguous and the
 // ranges are disjoint.
 // ---------------------------------------------------------------------------
-constexpr int kPlanPer = 8;                 // tiles per thread
-constexpr int kPlanBlock = 256 * kPlanPer;  // tiles per block
+constexpr int kPlanBlock = kPlanTiles;       // tiles per block
+constexpr int kPlanPer = kPlanBlock / 256;  // tiles per thread
 constexpr int kPlanQ = 6;                   // entries, long items, short items, partial blocks, multi, empty
 
 struct TilePlan {

@@ -181,6 +181,9 @@ struct DeviceCounters {
 // one tile (256 to 1024), taken in passes of kRasterPass entries (the LDS
 // capacity).
 constexpr int kRasterPass = 128;
+// Tiles per workgroup of the tile-plan kernels (one per thread): their block
+// sums take (tiles + kPlanTiles - 1) / kPlanTiles + 1 groups of six words.
+constexpr int kPlanTiles = 256;
 constexpr int kRasterPartShift = 9;
 
 // Raster work: items {tile, first list entry, entries, partial block or
