@@ -51,7 +51,7 @@ def algorithmic_bytes(scene, stage: str) -> float:
 # kernels of each timed stage (the raster stage's outputs are written by both
 # k_raster and, for tiles whose list was cut into items, k_raster_merge)
 STAGE_KERNELS = {"vertex": ["k_vertex"], "setup": ["k_setup", "k_clip"],
-                 "bin": ["k_bin_small", "k_bin_big", "k_plan_sums", "k_plan_scan", "k_plan_write"],
+                 "bin": ["k_bin_small", "k_bin_big_count", "k_plan_sums", "k_plan_scan", "k_plan_write"],
                  "raster": ["k_raster", "k_raster_merge"], "shade": ["k_shade"]}
 
 

@@ -27,6 +27,42 @@ __device__ __forceinline__ int4 unpack_box(const uint4& e) {
 }
 
 // Shared traversal of the small triangles over the bin queues of all views.
+// Triangles deferred by the small-triangle pass (more tiles than the
+// threshold): one workgroup per triangle from `first` in steps of `stride`,
+// its threads over the tile rectangle.
+template <bool SCATTER>
+__device__ __forceinline__ void bin_big(const BinParams& P, const Views& V, const TriRecs& recs,
+                                        const uint4* __restrict__ queueAll, unsigned* __restrict__ binCount,
+                                        const unsigned* __restrict__ binStart, unsigned* __restrict__ binCursor,
+                                        unsigned* __restrict__ binList, const unsigned* __restrict__ bigList,
+                                        const DeviceCounters* ctr, unsigned first, unsigned stride) {
+    const unsigned nbig = min(ctr->bigCount, P.bigCap);
+    for (unsigned i = first; i < nbig; i += stride) {
+        const unsigned qi = bigList[i];
+        const ViewDesc& D = V.desc[qi / V.slotsPerView];
+        const unsigned tileBase = D.tileBase;
+        const int tilesX = D.tilesX;
+        const uint4 e = queueAll[qi];
+        const unsigned s = e.x;
+        const int4 b = unpack_box(e);
+        const Edges E = load_edges(recs, s);
+        const int w = b.z - b.x + 1;
+        const int area = w * (b.w - b.y + 1);
+        for (int q = threadIdx.x; q < area; q += blockDim.x) {
+            const int tx = b.x + q % w, ty = b.y + q / w;
+            const int c = tile_class(E, tx, ty);
+            if (!c) continue;
+            const int t = (int)tileBase + ty * tilesX + tx;
+            if (!SCATTER) {
+                atomicAdd(&binCount[t], 1u);
+            } else {
+                const unsigned at = binStart[t] + atomicAdd(&binCursor[t], 1u);
+                if (at < P.binCap) binList[at] = s | (c == 2 ? 0x80000000u : 0u);
+            }
+        }
+    }
+}
+
 // The queue entries of view v are numbered in chunks of 256 that continue
 // from the previous view's, and chunk c goes to workgroup c mod gridDim.x, so
 // the work spreads evenly whatever the views' sizes.  SCATTER = false counts
@@ -94,39 +130,18 @@ __global__ void __launch_bounds__(256) k_bin_small(BinParams P, Views V, TriRecs
             }
         }
     }
+    // the scatter pass also takes the deferred triangles, whose list the count pass completed
+    if constexpr (SCATTER)
+        bin_big<true>(P, V, recs, queueAll, binCount, binStart, binCursor, binList, bigList, ctr, blockIdx.x, G);
 }
 
-template <bool SCATTER>
-__global__ void __launch_bounds__(256) k_bin_big(BinParams P, Views V, TriRecs recs,
-                                                 const uint4* __restrict__ queueAll, unsigned* __restrict__ binCount,
-                                                 const unsigned* __restrict__ binStart, unsigned* __restrict__ binCursor,
-                                                 unsigned* __restrict__ binList, const unsigned* __restrict__ bigList,
-                                                 const DeviceCounters* ctr) {
-    const unsigned nbig = min(ctr->bigCount, P.bigCap);
-    for (unsigned i = blockIdx.x; i < nbig; i += gridDim.x) {
-        const unsigned qi = bigList[i];
-        const ViewDesc& D = V.desc[qi / V.slotsPerView];
-        const unsigned tileBase = D.tileBase;
-        const int tilesX = D.tilesX;
-        const uint4 e = queueAll[qi];
-        const unsigned s = e.x;
-        const int4 b = unpack_box(e);
-        const Edges E = load_edges(recs, s);
-        const int w = b.z - b.x + 1;
-        const int area = w * (b.w - b.y + 1);
-        for (int q = threadIdx.x; q < area; q += blockDim.x) {
-            const int tx = b.x + q % w, ty = b.y + q / w;
-            const int c = tile_class(E, tx, ty);
-            if (!c) continue;
-            const int t = (int)tileBase + ty * tilesX + tx;
-            if (!SCATTER) {
-                atomicAdd(&binCount[t], 1u);
-            } else {
-                const unsigned at = binStart[t] + atomicAdd(&binCursor[t], 1u);
-                if (at < P.binCap) binList[at] = s | (c == 2 ? 0x80000000u : 0u);
-            }
-        }
-    }
+// The count pass's deferred triangles (the scatter pass takes its own at the
+// end of k_bin_small: the list is complete by then).
+__global__ void __launch_bounds__(256) k_bin_big_count(BinParams P, Views V, TriRecs recs,
+                                                       const uint4* __restrict__ queueAll,
+                                                       unsigned* __restrict__ binCount,
+                                                       const unsigned* __restrict__ bigList, const DeviceCounters* ctr) {
+    bin_big<false>(P, V, recs, queueAll, binCount, nullptr, nullptr, nullptr, bigList, ctr, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -222,26 +237,36 @@ __global__ void __launch_bounds__(1024) k_plan_scan(unsigned* __restrict__ block
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     unsigned carry[kPlanQ] = {0, 0, 0, 0, 0, 0};
     for (int base = 0; base < nblk; base += 1024) {
+        // the six quantities of block b scanned together: two barriers per 1024 blocks
         const int b = base + t;
+        unsigned v[kPlanQ], x[kPlanQ];
 #pragma unroll
         for (int k = 0; k < kPlanQ; k++) {
-            const unsigned v = b < nblk ? blockSums[b * kPlanQ + k] : 0u;
-            unsigned x = v;
+            v[k] = b < nblk ? blockSums[b * kPlanQ + k] : 0u;
+            x[k] = v[k];
+        }
 #pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const unsigned y = __shfl_up(x, off, 64);
-                if (lane >= off) x += y;
+        for (int off = 1; off < 64; off <<= 1) {
+#pragma unroll
+            for (int k = 0; k < kPlanQ; k++) {
+                const unsigned y = __shfl_up(x[k], off, 64);
+                if (lane >= off) x[k] += y;
             }
-            __syncthreads();  // sWave[k] is free
-            if (lane == 63) sWave[k][w] = x;
-            __syncthreads();
+        }
+        __syncthreads();  // sWave is free
+        if (lane == 63)
+#pragma unroll
+            for (int k = 0; k < kPlanQ; k++) sWave[k][w] = x[k];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kPlanQ; k++) {
             unsigned pre = 0, tot = 0;
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 pre += i < w ? sWave[k][i] : 0u;
                 tot += sWave[k][i];
             }
-            if (b < nblk) blockSums[b * kPlanQ + k] = carry[k] + pre + x - v;
+            if (b < nblk) blockSums[b * kPlanQ + k] = carry[k] + pre + x[k] - v[k];
             carry[k] += tot;
         }
     }
@@ -312,8 +337,8 @@ void launch_bin(const BinParams& p, const Views& views, TriRecs recs, const uint
     const dim3 g(p.grid);
     hipLaunchKernelGGL(k_bin_small<false>, g, dim3(256), 0, s, p, views, recs, binQueue, binCount, binStart, binCursor,
                        binList, bigList, ctr);
-    hipLaunchKernelGGL(k_bin_big<false>, dim3(256), dim3(256), 0, s, p, views, recs, binQueue, binCount, binStart,
-                       binCursor, binList, bigList, ctr);
+    hipLaunchKernelGGL(k_bin_big_count, dim3(256), dim3(256), 0, s, p, views, recs, binQueue, binCount, bigList,
+                       ctr);
     const int nblk = (p.tiles + kPlanBlock - 1) / kPlanBlock;
     hipLaunchKernelGGL(k_plan_sums, dim3(nblk), dim3(256), 0, s, binCount, p.tiles, q.partShift, q.planSums);
     hipLaunchKernelGGL(k_plan_scan, dim3(1), dim3(1024), 0, s, q.planSums, nblk, p.binCap, q, ctr);
@@ -321,8 +346,6 @@ void launch_bin(const BinParams& p, const Views& views, TriRecs recs, const uint
                        q.planSums, q);
     hipLaunchKernelGGL(k_bin_small<true>, g, dim3(256), 0, s, p, views, recs, binQueue, binCount, binStart, binCursor,
                        binList, bigList, ctr);
-    hipLaunchKernelGGL(k_bin_big<true>, dim3(256), dim3(256), 0, s, p, views, recs, binQueue, binCount, binStart,
-                       binCursor, binList, bigList, ctr);
 }
 
 }  // namespace ifrit_hip
