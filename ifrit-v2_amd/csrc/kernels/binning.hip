@@ -282,7 +282,7 @@ __global__ void __launch_bounds__(1024) k_plan_scan(unsigned* __restrict__ block
         ctr->numPartials = carry[3];
         ctr->numMulti = go ? carry[4] : 0u;
         ctr->numEmpty = go ? carry[5] : 0u;
-        ctr->itemHead = 0u;
+        for (int k = 0; k < kItemStreams; k++) ctr->itemHeads[k][0] = 0u;
     }
 }
 

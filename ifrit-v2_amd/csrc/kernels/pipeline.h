@@ -160,6 +160,12 @@ struct StatPartials {
 };
 
 // Counters written by the kernels (device memory, zeroed each draw by the vertex or reset kernel).
+// Raster work items are drained through this many counters, one per XCD:
+// stream k holds items k, k + kItemStreams, ... and is taken by the
+// workgroups with blockIdx.x % kItemStreams == k (one XCD's), so the claims
+// of the persistent raster spread over separate lines instead of one.
+constexpr int kItemStreams = 8;
+
 struct DeviceCounters {
     unsigned status;
     unsigned clipCount;  // clip fan slots of all views (reduced with the statistics)
@@ -169,12 +175,12 @@ struct DeviceCounters {
     unsigned long long coveredSamples;
     unsigned long long assembled;
     unsigned clipMax;    // largest per-view clipCount or clipInCount (sizes clipCap on overflow)
-    unsigned itemHead;   // next raster work item
     unsigned numItems;
     unsigned numMulti;     // tiles cut into several parts
     unsigned numPartials;  // partial blocks those need
     unsigned numEmpty;     // tiles with an empty list
     unsigned numLongItems; // items of more than kRasterPass entries (front of the item array)
+    alignas(128) unsigned itemHeads[kItemStreams][32];  // next item of each stream, one 128-byte line each
 };
 
 // Raster work: a work item covers at most 1 << partShift list entries of
@@ -187,7 +193,7 @@ constexpr int kPlanTiles = 256;
 constexpr int kRasterPartShift = 9;
 
 // Raster work: items {tile, first list entry, entries, partial block or
-// ~0u} drained through DeviceCounters::itemHead; per tile the first partial
+// ~0u} drained through DeviceCounters::itemHeads; per tile the first partial
 // block of a tile cut into several items (-1 otherwise); the list of such
 // tiles followed, from index `tiles`, by the empty tiles; partial key blocks
 // of 256 * samples keys each; planning scratch (6 words per 2048 tiles).

@@ -431,19 +431,21 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
     const unsigned numItems = ctr->numItems, numLong = ctr->numLongItems;
     unsigned long long frags = 0, covs = 0;
 
-    // thread 0 claims the next item before the output stage of the current
-    // one, so that the claim's latency overlaps the output stage.  The
-    // counter's address is moved to vector registers: a uniform address would
-    // turn the claim into a wave-aggregated atomic whose result is awaited at
-    // once.
-    __attribute__((address_space(1))) unsigned* head = (__attribute__((address_space(1))) unsigned*)&ctr->itemHead;
+    // thread 0 claims the next item of the workgroup's stream before the
+    // output stage of the current one, so that the claim's latency overlaps
+    // the output stage.  The counter's address is moved to vector registers:
+    // a uniform address would turn the claim into a wave-aggregated atomic
+    // whose result is awaited at once.
+    const unsigned stream = blockIdx.x % kItemStreams;
+    __attribute__((address_space(1))) unsigned* head =
+        (__attribute__((address_space(1))) unsigned*)&ctr->itemHeads[stream][0];
     asm volatile("" : "+v"(head));
     unsigned claimed = 0;
     if (t == 0) claimed = __hip_atomic_fetch_add(head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (;;) {
         if (t == 0) sItem = claimed;
         __syncthreads();
-        const unsigned item = sItem;
+        const unsigned item = sItem * kItemStreams + stream;
         if (item >= numItems) break;
         // long items from the front of the array, then short ones from its back
         const uint4 it = items[item < numLong ? item : itemCap - 1u - (item - numLong)];
