@@ -667,7 +667,7 @@ void TileRasterRenderer::enqueue(bool clear) {
     } else if (depthFunc_ == IF_COMPARE_OP_LESS) {
         // order-independent: the lexicographic minimum of (depth, primitive)
         launch_raster(rp, V, samples_, recPtrs(), slotOf_.as<unsigned>(), binStart_.as<unsigned>(),
-                      binCount_.as<unsigned>(), binList_.as<unsigned>(), q, visDepth_.as<float>(),
+                      binCount_.as<unsigned>(), binList_.as<unsigned>(), q, tiles, visDepth_.as<float>(),
                       visSlot_.as<uint32_t>(), cls, statPtrs(), ctr_.as<DeviceCounters>(), s);
         mark(4);
         launch_raster_merge(rp, V, samples_, recPtrs(), slotOf_.as<unsigned>(), binStart_.as<unsigned>(),

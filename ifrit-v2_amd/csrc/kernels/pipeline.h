@@ -223,8 +223,8 @@ void launch_bin(const BinParams& p, const Views& views, TriRecs recs, const uint
                 hipStream_t s);
 void launch_raster(const RasterParams& p, const Views& views, int samples, TriRecs recs, const unsigned* slotOf,
                    const unsigned* binStart, const unsigned* binCount, const unsigned* binList, const RasterQueue& q,
-                   float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st, DeviceCounters* ctr,
-                   hipStream_t s);
+                   int tiles, float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st,
+                   DeviceCounters* ctr, hipStream_t s);
 void launch_raster_merge(const RasterParams& p, const Views& views, int samples, TriRecs recs, const unsigned* slotOf,
                          const unsigned* binStart, const unsigned* binCount, const RasterQueue& q, float* visDepth,
                          uint32_t* visSlot, uint8_t* visClass, int tiles, DeviceCounters* ctr, hipStream_t s);

@@ -403,6 +403,43 @@ __device__ __forceinline__ void store_tile(const ViewGeom& G, int tx, int ty, co
     }
 }
 
+// Clear values of one tile whose list is empty (depth kClearDepth, slot
+// 0xffffffff and, at 1 sample per pixel, class 0) by NT threads, with 16-byte
+// stores where the tile's rows are whole.
+template <int S, int NT>
+__device__ __forceinline__ void clear_tile(const ViewGeom& G, float* visDepthAll, uint32_t* visSlotAll,
+                                           uint8_t* visClassAll) {
+    const int tx = G.tile % G.tilesX, ty = G.tile / G.tilesX;
+    const int cols = min(16, G.width - tx * 16), rows = min(16, G.height - ty * 16);
+    const bool vec = cols == 16 && (S > 1 || ((G.width & 3) == 0 && (G.pixBase & 3) == 0));
+    float* const visDepth = visDepthAll + G.pixBase * S;
+    uint32_t* const visSlot = visSlotAll + G.pixBase * S;
+    constexpr int perRow = 16 * S / 4;  // 16-byte words per tile row
+    for (int i = threadIdx.x; i < 64 * S; i += NT) {
+        const int row = i / perRow, q = i - row * perRow;
+        if (row >= rows) continue;
+        const size_t g = ((size_t)(ty * 16 + row) * G.width + tx * 16) * S + q * 4;
+        if (vec) {
+            *reinterpret_cast<float4*>(&visDepth[g]) = make_float4(kClearDepth, kClearDepth, kClearDepth, kClearDepth);
+            *reinterpret_cast<uint4*>(&visSlot[g]) = make_uint4(~0u, ~0u, ~0u, ~0u);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                if ((q * 4 + e) / S >= cols) continue;
+                visDepth[g + e] = kClearDepth;
+                visSlot[g + e] = 0xffffffffu;
+            }
+        }
+    }
+    if (S == 1 && visClassAll) {
+        uint8_t* const visClass = visClassAll + G.pixBase;
+        for (int i = threadIdx.x; i < 256; i += NT) {
+            const int row = i >> 4, col = i & 15;
+            if (row < rows && col < cols) visClass[(size_t)(ty * 16 + row) * G.width + tx * 16 + col] = 0;
+        }
+    }
+}
+
 // NT threads per tile: 256 (one pixel per thread) or, at 1 sample per pixel,
 // 64 (one wave per tile, four pixels per lane: more tiles in flight per
 // compute unit, for many small tiles).  Passes hold NT / 2 entries.  Held to
@@ -411,6 +448,7 @@ template <int S, int NT>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slotOfAll,
          const unsigned* __restrict__ binList, const uint4* __restrict__ items, unsigned itemCap,
+         const unsigned* __restrict__ emptyList,
          unsigned long long* __restrict__ partials, float* __restrict__ visDepthAll, uint32_t* __restrict__ visSlotAll,
          uint8_t* __restrict__ visClassAll, StatPartials st, DeviceCounters* ctr) {
     constexpr int PASS = NT / 2;            // list entries per pass through LDS
@@ -429,6 +467,11 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
 
     const int t = threadIdx.x;
     const unsigned numItems = ctr->numItems, numLong = ctr->numLongItems;
+    // tiles with an empty list follow the items: their clear values, KE tiles
+    // per claim at 1 sample per pixel (a small write each)
+    constexpr unsigned KE = S == 1 ? 8u : 1u;
+    const unsigned numEmpty = P.clear ? ctr->numEmpty : 0u;
+    const unsigned numEmptyItems = (numEmpty + KE - 1u) / KE;
     unsigned long long frags = 0, covs = 0;
 
     // thread 0 claims the next item of the workgroup's stream before the
@@ -446,7 +489,15 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
         if (t == 0) sItem = claimed;
         __syncthreads();
         const unsigned item = sItem * kItemStreams + stream;
-        if (item >= numItems) break;
+        if (item >= numItems) {
+            const unsigned e = item - numItems;
+            if (e >= numEmptyItems) break;
+            if (t == 0) claimed = __hip_atomic_fetch_add(head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (unsigned k = e * KE; k < min(e * KE + KE, numEmpty); k++)
+                clear_tile<S, NT>(view_of_tile(V, (int)emptyList[k]), visDepthAll, visSlotAll, visClassAll);
+            __syncthreads();  // every lane has read sItem
+            continue;
+        }
         // long items from the front of the array, then short ones from its back
         const uint4 it = items[item < numLong ? item : itemCap - 1u - (item - numLong)];
         const int tile = __builtin_amdgcn_readfirstlane((int)it.x);
@@ -687,8 +738,8 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
     }
 }
 
-// Minimum over the parts of every tile whose list was cut, then the outputs;
-// also the clear values of tiles with an empty list (when clearing).
+// Minimum over the parts of every tile whose list was cut, then the outputs
+// (the empty tiles' clear values come from k_raster).
 template <int S>
 __global__ void __launch_bounds__(256) k_raster_merge(RasterParams P, Views V, TriRecs recs,
                                                       const unsigned* __restrict__ slotOfAll,
@@ -701,51 +752,7 @@ __global__ void __launch_bounds__(256) k_raster_merge(RasterParams P, Views V, T
                                                       uint8_t* __restrict__ visClassAll, int tiles, int partShift,
                                                       const DeviceCounters* ctr) {
     const unsigned numMulti = ctr->numMulti;
-    const unsigned numEmpty = P.clear ? ctr->numEmpty : 0u;
     const int t = threadIdx.x;
-    // clear values of the empty tiles: a tile is 64 * S lines of 16 bytes per output, so at 1 spp
-    // four tiles are cleared per round by 64 lanes each
-    {
-        constexpr int kLines = 64 * S;
-        constexpr int kPerRound = kLines >= 256 ? 1 : 256 / kLines;
-        constexpr int kLanes = 256 / kPerRound;
-        const int sub = t / kLanes, lane = t % kLanes;
-        for (unsigned e0 = blockIdx.x * kPerRound; e0 < numEmpty; e0 += gridDim.x * kPerRound) {
-            const unsigned e = e0 + (unsigned)sub;
-            if (e >= numEmpty) continue;
-            const int tile = (int)multiList[tiles + e];
-            const ViewGeom G = view_of_tile(V, tile);
-            const int tx = G.tile % G.tilesX, ty = G.tile / G.tilesX;
-            const int cols = min(16, G.width - tx * 16), rows = min(16, G.height - ty * 16);
-            const bool vec = cols == 16 && (S > 1 || ((G.width & 3) == 0 && (G.pixBase & 3) == 0));
-            float* const visDepth = visDepthAll + G.pixBase * S;
-            uint32_t* const visSlot = visSlotAll + G.pixBase * S;
-            constexpr int perRow = 16 * S / 4;  // lines per tile row
-            for (int i = lane; i < kLines; i += kLanes) {
-                const int row = i / perRow, q = i - row * perRow;
-                if (row >= rows) continue;
-                const size_t g = ((size_t)(ty * 16 + row) * G.width + tx * 16) * S + q * 4;
-                if (vec) {
-                    *reinterpret_cast<float4*>(&visDepth[g]) = make_float4(kClearDepth, kClearDepth, kClearDepth, kClearDepth);
-                    *reinterpret_cast<uint4*>(&visSlot[g]) = make_uint4(~0u, ~0u, ~0u, ~0u);
-                } else {
-#pragma unroll
-                    for (int e2 = 0; e2 < 4; e2++) {
-                        if ((q * 4 + e2) / S >= cols) continue;
-                        visDepth[g + e2] = kClearDepth;
-                        visSlot[g + e2] = 0xffffffffu;
-                    }
-                }
-            }
-            if (S == 1 && visClassAll) {
-                uint8_t* const visClass = visClassAll + G.pixBase;
-                for (int i = lane; i < 256; i += kLanes) {
-                    const int row = i >> 4, col = i & 15;
-                    if (row < rows && col < cols) visClass[(size_t)(ty * 16 + row) * G.width + tx * 16 + col] = 0;
-                }
-            }
-        }
-    }
     // tiles whose list was cut into several items: the minimum over their partial blocks
     for (unsigned m = blockIdx.x; m < numMulti; m += gridDim.x) {
         const int tile = (int)multiList[m];
@@ -916,13 +923,13 @@ void launch_slot_to_prim(const uint32_t* visSlot, TriRecs recs, size_t n, uint32
 // The raster and merge launches are separate so that the host can time them.
 void launch_raster(const RasterParams& p, const Views& views, int samples, TriRecs recs, const unsigned* slotOf,
                    const unsigned* binStart, const unsigned* binCount, const unsigned* binList, const RasterQueue& q,
-                   float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st, DeviceCounters* ctr,
-                   hipStream_t s) {
+                   int tiles, float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st,
+                   DeviceCounters* ctr, hipStream_t s) {
     (void)binStart;
     (void)binCount;
 #define IFRIT_RASTER_LAUNCH(SN, NT)                                                                                 \
     hipLaunchKernelGGL((k_raster<SN, NT>), dim3(q.grid), dim3(NT), 0, s, p, views, recs, slotOf, binList, q.items, \
-                       q.itemCap, q.partials, visDepth, visSlot, visClass, st, ctr)
+                       q.itemCap, q.multiList + tiles, q.partials, visDepth, visSlot, visClass, st, ctr)
     switch (samples) {
         case 1:
             if (q.tileThreads == 64)
