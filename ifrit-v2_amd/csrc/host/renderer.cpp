@@ -58,6 +58,7 @@ TileRasterRenderer::~TileRasterRenderer() {
                       &bigList_, &ctr_, &clipList_, &statPart_, &items_, &tilePartial_, &multiList_, &partials_, &visId_, &visDepth_, &visSlot_, &visClass_, &rgba8_, &rgba32f_, &texDesc_, &nnParams_, &nnGrads_, &nnM_, &nnV_,
                       &nnPacked_, &nnQinfo_, &lat32_, &latGrad_, &latM_, &latV_, &lat16_, &nnData_, &nnFeat_, &nnCount_,
                       &nnLoss_, &nnEvalIn_, &nnEvalOut_, &nnGradPart_, &viewDev_, &tileView_, &shadeView_,
+                      &groupBounds_, &groupOut_,
                       &viewCtr_, &digest_, &planSums_})
         b->release();
     for (auto& t : textures_)
@@ -143,6 +144,7 @@ void TileRasterRenderer::bindVertexDataFloat(const float* data, int stride, int 
     nverts_ = n;
     stride_ = stride;
     vtxDirty_ = false;
+    boundsDirty_ = true;
 }
 
 void TileRasterRenderer::bindIndexData(const int32_t* data, int n) {
@@ -156,6 +158,7 @@ void TileRasterRenderer::bindIndexData(const int32_t* data, int n) {
     if (!ok(hipMemcpy(idx_.ptr, data, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice), "index upload")) return;
     nidx_ = n;
     idxDirty_ = false;
+    boundsDirty_ = true;
 }
 
 void TileRasterRenderer::bindVertexShaderFunc(IfritVertexShaderFunctionalPtr f, VaryingDescriptor* vd) {
@@ -351,6 +354,7 @@ bool TileRasterRenderer::prepare(int indexCount) {
     if (hostVs_ && hostVb_) {
         runHostVertexShader();
         vtxDirty_ = false;
+        boundsDirty_ = true;
     } else if (vtxDirty_ && hostVb_) {
         if (hostVb_->elementSize % 4) {
             setError("vertex records must be a multiple of 4 bytes");
@@ -363,6 +367,7 @@ bool TileRasterRenderer::prepare(int indexCount) {
         nverts_ = hostVb_->vertexCount;
         stride_ = hostVb_->elementSize / 4;
         vtxDirty_ = false;
+        boundsDirty_ = true;
     }
     if (idxDirty_ && hostIb_) {
         if (!idx_.ensure(hostIb_->data.size())) return setError("hipMalloc index buffer"), false;
@@ -371,6 +376,7 @@ bool TileRasterRenderer::prepare(int indexCount) {
             return false;
         nidx_ = (int)(hostIb_->data.size() / 4);
         idxDirty_ = false;
+        boundsDirty_ = true;
     }
     if (nverts_ <= 0 || !vtx_.ptr) {
         setError("no vertex buffer bound");
@@ -397,6 +403,17 @@ bool TileRasterRenderer::prepare(int indexCount) {
         setError("the clip pass-through vertex program needs 4 floats per vertex");
         return false;
     }
+    // object-space bounds of the triangle groups, for culling whole groups per view
+    if (!hostVs_ && vs_.desc.kind == IFHIP_VS_TRANSFORM) {
+        const int all = nidx_ / 3, groups = (all + kGroupTris - 1) / kGroupTris;
+        if (boundsDirty_ || boundsTris_ != all) {
+            if (!groupBounds_.ensure((size_t)std::max(groups, 1) * 32)) return setError("hipMalloc group bounds"), false;
+            launch_group_bounds(vtx_.as<float>(), stride_, nverts_, idx_.as<int>(), all, groupBounds_.as<float4>(),
+                                stream_);
+            boundsDirty_ = false;
+            boundsTris_ = all;
+        }
+    }
     const int ntri = indexCount / 3;
     curTris_ = ntri;
     clipCap_ = std::max<unsigned>(clipCap_, std::max(4096u, (unsigned)(ntri / 8)));
@@ -415,7 +432,9 @@ bool TileRasterRenderer::prepare(int indexCount) {
     const size_t slots = ((size_t)ntri + clipCap_) * nv;
     const bool wantFloat = batch_.empty() && ((fb_ && !fb_->color.empty() && fb_->color[0]) || floatOut_ ||
                                               blend_.blendEnable);
-    bool okAll = clip_.ensure((size_t)nverts_ * 16 * nv) && recs_[0].ensure(slots * 16) &&
+    bool okAll = clip_.ensure((size_t)nverts_ * 16 * nv) &&
+                 groupOut_.ensure(std::max<size_t>(1, (size_t)(boundsTris_ + kGroupTris - 1) / kGroupTris) * nv) &&
+                 recs_[0].ensure(slots * 16) &&
                  recs_[1].ensure(slots * 16) && recs_[2].ensure(slots * 16) && recs_[3].ensure(slots * 16) &&
                  recs_[4].ensure(slots * 16) && recs_[5].ensure(slots * 16) && bary_.ensure(slots * 32) &&
                  binQueue_.ensure(slots * 16) && slotOf_.ensure((size_t)std::max(1, ntri) * 4 * nv) &&
@@ -617,12 +636,16 @@ void TileRasterRenderer::enqueue(bool clear) {
         if (stageTiming_ && !capturing_) (void)hipEventRecord(ev_[i], s);
     };
     mark(0);
+    // triangle groups outside a view are marked by the vertex launch and skipped by setup
+    const bool cullGroups = !hostVs_ && vs_.desc.kind == IFHIP_VS_TRANSFORM && groupOut_.ptr;
+    const int groups = (boundsTris_ + kGroupTris - 1) / kGroupTris;
     if (hostVs_)
         launch_reset(binCount_.as<unsigned>(), binCursor_.as<unsigned>(), tiles, V, ctr_.as<DeviceCounters>(), s);
     else  // the vertex launch also clears the counters
         launch_vertex(vtx_.as<float>(), stride_, nverts_, V, vs_.desc.kind == IFHIP_VS_CLIP_PASSTHROUGH ? 1 : 0,
                       clip_.as<float4>(), binCount_.as<unsigned>(), binCursor_.as<unsigned>(), tiles,
-                      ctr_.as<DeviceCounters>(), s);
+                      ctr_.as<DeviceCounters>(), groupBounds_.as<float4>(), groups,
+                      cullGroups ? groupOut_.as<unsigned char>() : nullptr, s);
     mark(1);
     SetupParams sp{};
     sp.ntri = curTris_;
@@ -631,6 +654,8 @@ void TileRasterRenderer::enqueue(bool clear) {
     sp.msaa = samples_ > 1 ? 1 : 0;
     sp.clipCap = clipCap_;
     sp.invTileWidth = ifx::rcp_nr_ss(16.0f);
+    sp.groupOut = cullGroups ? groupOut_.as<unsigned char>() : nullptr;
+    sp.groups = groups;
     launch_setup(sp, V, clip_.as<float4>(), idx_.as<int>(), recPtrs(), bary_.as<float4>(), binQueue_.as<uint4>(),
                  slotOf_.as<unsigned>(), clipList_.as<unsigned>(), statPtrs(), ctr_.as<DeviceCounters>(), s);
     mark(2);

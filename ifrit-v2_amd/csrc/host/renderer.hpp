@@ -179,6 +179,8 @@ private:
     const VertexBuffer* hostVb_ = nullptr;
     const Buffer* hostIb_ = nullptr;
     bool vtxDirty_ = true, idxDirty_ = true;
+    bool boundsDirty_ = true;  // group bounds to recompute (geometry changed)
+    int boundsTris_ = 0;       // triangles the group bounds cover
     int nverts_ = 0, stride_ = 0, nidx_ = 0;
     IfritVertexShaderFunctionalPtr hostVs_ = nullptr;
     VaryingDescriptor* hostVsVary_ = nullptr;
@@ -216,6 +218,7 @@ private:
 
     // device buffers
     DevBuf vtx_, idx_, clip_, recs_[6], bary_, binQueue_, slotOf_, binCount_, binStart_, binCursor_, binList_, bigList_, ctr_;
+    DevBuf groupBounds_, groupOut_;  // per triangle group: bounds; per view and group: outside flag
     DevBuf clipList_, statPart_, items_, tilePartial_, multiList_, partials_, visId_, visDepth_, visSlot_, visClass_, rgba8_, rgba32f_, texDesc_;
     unsigned clipCap_ = 0, binCap_ = 0, bigCap_ = 0, itemCap_ = 0, partialCap_ = 0;
     int numCUs_ = 256;

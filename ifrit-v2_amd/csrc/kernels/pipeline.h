@@ -91,7 +91,14 @@ struct SetupParams {
     unsigned clipCap;   // capacity for clip fan triangles and for triangles to clip
     float invW, invH;   // 1 / width, 1 / height (TileRasterRenderer.cpp:59-60)
     float invTileWidth; // ApproxReciprocal(16.0f) (TileRasterWorker.cpp:499)
+    // per view, per group of kGroupTris triangles: 1 when the group's bounds lie
+    // outside a frustum plane with a margin (null: no group culling)
+    const unsigned char* groupOut;
+    int groups;
 };
+
+// Triangles per culling group (a multiple of the 64-lane wave).
+constexpr int kGroupTris = 256;
 
 struct BinParams {
     int tiles;          // tiles of all views
@@ -212,7 +219,10 @@ struct RasterQueue {
 
 // Launch wrappers (pipeline.hip).
 void launch_vertex(const float* vtx, int stride, int nverts, const Views& views, int passthrough, float4* clip,
-                   unsigned* binCount, unsigned* binCursor, int tiles, DeviceCounters* ctr, hipStream_t s);
+                   unsigned* binCount, unsigned* binCursor, int tiles, DeviceCounters* ctr,
+                   const float4* groupBounds, int groups, unsigned char* groupOut, hipStream_t s);
+void launch_group_bounds(const float* vtx, int stride, int nverts, const int* idx, int ntri, float4* bounds,
+                         hipStream_t s);
 void launch_reset(unsigned* binCount, unsigned* binCursor, int tiles, const Views& views, DeviceCounters* ctr,
                   hipStream_t s);
 void launch_setup(const SetupParams& p, const Views& views, const float4* clip, const int* idx, TriRecs recs, float4* bary, uint4* binQueue,
