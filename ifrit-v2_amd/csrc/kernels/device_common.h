@@ -162,22 +162,29 @@ __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
 
 // Wave-aggregated counter increment: every active lane adds one to
 // ctr[key] and receives a unique position; lanes sharing a key share one
-// atomic.  Must be called by the whole (converged) wave.
+// atomic.  The lanes are grouped by key with ballots first, so all groups'
+// atomics go out as one instruction and the wave waits for memory once.
+// Must be called by the whole (converged) wave.
 __device__ __forceinline__ unsigned wave_inc(unsigned* ctr, int key, bool active) {
     const int lane = lane_id();
     unsigned long long pending = __ballot(active);
-    unsigned result = 0;
+    int leader = lane;
+    unsigned rank = 0, size = 0;
     while (pending) {
-        const int leader = __ffsll((unsigned long long)pending) - 1;
-        const int lk = __shfl(key, leader, 64);
+        const int l = __ffsll((unsigned long long)pending) - 1;
+        const int lk = __shfl(key, l, 64);
         const unsigned long long grp = __ballot(active && key == lk) & pending;
-        unsigned base = 0;
-        if (lane == leader) base = atomicAdd(&ctr[lk], (unsigned)__popcll(grp));
-        base = __shfl(base, leader, 64);
-        if ((grp >> lane) & 1ull) result = base + (unsigned)__popcll(grp & ((1ull << lane) - 1ull));
+        if ((grp >> lane) & 1ull) {
+            leader = l;
+            rank = (unsigned)__popcll(grp & ((1ull << lane) - 1ull));
+        }
+        if (lane == l) size = (unsigned)__popcll(grp);
         pending &= ~grp;
     }
-    return result;
+    unsigned base = 0;
+    if (active && leader == lane) base = atomicAdd(&ctr[key], size);
+    base = __shfl(base, leader, 64);
+    return base + rank;
 }
 
 // Sum over a 256-thread workgroup (all threads must call it).
