@@ -69,7 +69,8 @@ __device__ __forceinline__ float4 lerp4(float4 a, float4 b, float t) {
 }
 
 // The two mip levels a trilinear lookup at lod blends: level l0 = floor of
-// the clamped lod and l1 = l0 + 1 (or l0 at the last level), as the address
+// the clamped lod and l1 = l0 + 1 (l0 at the last level or where the lod is
+// a whole number, i.e. the blend fraction is 0), as the address
 // of level l0's first texel, the texel distance to level l1, the levels'
 // log2 sizes (textures are powers of two) and the blend fraction.
 struct TriLevels {
@@ -82,7 +83,7 @@ struct TriLevels {
 __device__ __forceinline__ TriLevels tri_levels(const uint8_t* pool, const TexDesc& D, float lod) {
     lod = fminf(fmaxf(lod, 0.0f), (float)(D.levels - 1));
     const int l0 = (int)floorf(lod);
-    const int l1 = min(l0 + 1, D.levels - 1);
+    const int l1 = lod > (float)l0 ? min(l0 + 1, D.levels - 1) : l0;
     const int lw = __builtin_ctz((unsigned)D.width), lh = __builtin_ctz((unsigned)D.height);
     TriLevels L;
     L.t0 = reinterpret_cast<const uchar4*>(pool + D.base) + D.levelOffset[l0];
@@ -94,28 +95,52 @@ __device__ __forceinline__ TriLevels tri_levels(const uint8_t* pool, const TexDe
 }
 
 // Bilinear lookup in one level of 2^lw x 2^lh texels with repeat
-// addressing, texel centres at (i + 0.5) / size; the result is on the 0..255
-// scale of the stored bytes.
-__device__ __forceinline__ float4 bilinear_level(const uchar4* __restrict__ T, int lw, int lh, float u, float v) {
+// addressing, texel centres at (i + 0.5) / size, split into the fetch of the
+// four texels and their blend, so that a caller can put the loads of several
+// lookups in flight before it blends any of them.
+struct BilinearTexels {
+    unsigned t00, t10, t01, t11;  // RGBA8 texels
+    float ax, ay;
+};
+
+__device__ __forceinline__ BilinearTexels bilinear_fetch(const uchar4* __restrict__ T, int lw, int lh, float u,
+                                                         float v) {
     const int w = 1 << lw, h = 1 << lh;
     const float s = fmaf(u, (float)w, -0.5f), r = fmaf(v, (float)h, -0.5f);
     const float fs = floorf(s), fr = floorf(r);
-    const float ax = s - fs, ay = r - fr;
     const int x0 = (int)fs & (w - 1), y0 = (int)fr & (h - 1);
     const int x1 = (x0 + 1) & (w - 1), y1 = (y0 + 1) & (h - 1);
-    const uchar4 a = T[(y0 << lw) + x0], b = T[(y0 << lw) + x1], c = T[(y1 << lw) + x0], d = T[(y1 << lw) + x1];
-    const float4 c00 = make_float4(a.x, a.y, a.z, a.w), c10 = make_float4(b.x, b.y, b.z, b.w);
-    const float4 c01 = make_float4(c.x, c.y, c.z, c.w), c11 = make_float4(d.x, d.y, d.z, d.w);
-    return lerp4(lerp4(c00, c10, ax), lerp4(c01, c11, ax), ay);
+    const unsigned* const U = reinterpret_cast<const unsigned*>(T);
+    BilinearTexels B;
+    B.t00 = U[(y0 << lw) + x0];
+    B.t10 = U[(y0 << lw) + x1];
+    B.t01 = U[(y1 << lw) + x0];
+    B.t11 = U[(y1 << lw) + x1];
+    B.ax = s - fs;
+    B.ay = r - fr;
+    return B;
+}
+
+__device__ __forceinline__ float4 texel_f4(unsigned t) {
+    return make_float4((float)(t & 255u), (float)((t >> 8) & 255u), (float)((t >> 16) & 255u), (float)(t >> 24));
+}
+
+// The blend, on the 0..255 scale of the stored bytes.
+__device__ __forceinline__ float4 bilinear_blend(const BilinearTexels& B) {
+    return lerp4(lerp4(texel_f4(B.t00), texel_f4(B.t10), B.ax), lerp4(texel_f4(B.t01), texel_f4(B.t11), B.ax), B.ay);
 }
 
 // Trilinear lookup with prepared levels, scaled to 0..1 once at the end.
+// Both levels are always fetched before either is blended, so all eight
+// texel loads are in flight together (one memory round trip per tap).  With
+// a blend fraction of 0 the second level is the first, both blends give the
+// same value a, and a + (a - a)·0 is a exactly.
 __device__ __forceinline__ float4 trilinear_levels(const TriLevels& L, float u, float v) {
     constexpr float k = 1.0f / 255.0f;
-    float4 a = bilinear_level(L.t0, (int)(L.dims & 31u), (int)((L.dims >> 5) & 31u), u, v);
-    if (L.f != 0.0f)
-        a = lerp4(a, bilinear_level(L.t0 + L.d1, (int)((L.dims >> 10) & 31u), (int)((L.dims >> 15) & 31u), u, v), L.f);
-    return make_float4(a.x * k, a.y * k, a.z * k, a.w * k);
+    const BilinearTexels B0 = bilinear_fetch(L.t0, (int)(L.dims & 31u), (int)((L.dims >> 5) & 31u), u, v);
+    const BilinearTexels B1 = bilinear_fetch(L.t0 + L.d1, (int)((L.dims >> 10) & 31u), (int)((L.dims >> 15) & 31u), u, v);
+    const float4 c = lerp4(bilinear_blend(B0), bilinear_blend(B1), L.f);
+    return make_float4(c.x * k, c.y * k, c.z * k, c.w * k);
 }
 
 __device__ float4 trilinear(const uint8_t* pool, const TexDesc& D, float u, float v, float lod) {
@@ -476,7 +501,9 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
                     // the material's two levels, once per fragment, over tCol2 (only the layered and
                     // neural programs use it)
                     const TriLevels L = tri_levels(pool, tex[R.mat], R.fp.lod);
-                    const unsigned long long a = (unsigned long long)(uintptr_t)L.t0;
+                    // kept as a byte offset into the pool: a pointer rebuilt from the pool
+                    // argument stays in the global address space (global, not flat, loads)
+                    const unsigned long long a = (unsigned long long)(reinterpret_cast<const uint8_t*>(L.t0) - pool);
                     reinterpret_cast<uint4*>(tCol2)[lane] = make_uint4((unsigned)a, (unsigned)(a >> 32), L.d1, L.dims);
                     cLod[lane] = L.f;
                 }
@@ -517,7 +544,7 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
                 } else {
                     const uint4 q = reinterpret_cast<const uint4*>(tCol2)[fl];
                     TriLevels L;
-                    L.t0 = reinterpret_cast<const uchar4*>((uintptr_t)(((unsigned long long)q.y << 32) | q.x));
+                    L.t0 = reinterpret_cast<const uchar4*>(pool + (((unsigned long long)q.y << 32) | q.x));
                     L.d1 = q.z;
                     L.dims = q.w;
                     L.f = cLod[fl];
