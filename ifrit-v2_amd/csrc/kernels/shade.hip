@@ -80,16 +80,45 @@ struct TriLevels {
     float f;
 };
 
-__device__ __forceinline__ TriLevels tri_levels(const uint8_t* pool, const TexDesc& D, float lod) {
-    lod = fminf(fmaxf(lod, 0.0f), (float)(D.levels - 1));
+// What the sampler needs of a texture: level 0's byte offset in the pool,
+// log2 of its width and height, and the number of levels.
+struct TexHead {
+    unsigned long long base;
+    int lw, lh, levels;
+};
+
+__device__ __forceinline__ TexHead tex_head(const TexDesc& D) {
+    return TexHead{D.base, __builtin_ctz((unsigned)D.width), __builtin_ctz((unsigned)D.height), D.levels};
+}
+
+// Texel offset of level l in the packed chain of a 2^a x 2^b texture whose
+// level k holds max(1, 2^(a-k)) x max(1, 2^(b-k)) texels (the host's layout,
+// also kept in TexDesc::levelOffset).  Levels k <= min(a, b) hold 2^(a+b-2k)
+// texels, a geometric sum; then the long side halves alone down to 1, and
+// any further levels hold one texel.
+__device__ __forceinline__ unsigned level_offset(int a, int b, int l) {
+    const int m = min(a, b), M = max(a, b);
+    const unsigned long long S4 = 1ull << (a + b + 2);
+    unsigned long long off = (S4 - (S4 >> (2 * min(l, m + 1)))) / 3u;
+    if (l > m + 1) {
+        const int lc = min(l, M + 1);
+        off += (1ull << (M - m)) - (1ull << (M - lc + 1)) + (unsigned long long)(l - lc);
+    }
+    return (unsigned)off;
+}
+
+// The level addresses come from the header and level_offset, so preparing
+// them needs no memory access beyond the header itself.
+__device__ __forceinline__ TriLevels tri_levels(const uint8_t* pool, const TexHead& H, float lod) {
+    lod = fminf(fmaxf(lod, 0.0f), (float)(H.levels - 1));
     const int l0 = (int)floorf(lod);
-    const int l1 = lod > (float)l0 ? min(l0 + 1, D.levels - 1) : l0;
-    const int lw = __builtin_ctz((unsigned)D.width), lh = __builtin_ctz((unsigned)D.height);
+    const int l1 = lod > (float)l0 ? min(l0 + 1, H.levels - 1) : l0;
+    const int w0 = max(H.lw - l0, 0), h0 = max(H.lh - l0, 0);
     TriLevels L;
-    L.t0 = reinterpret_cast<const uchar4*>(pool + D.base) + D.levelOffset[l0];
-    L.d1 = D.levelOffset[l1] - D.levelOffset[l0];
-    L.dims = (unsigned)max(lw - l0, 0) | ((unsigned)max(lh - l0, 0) << 5) | ((unsigned)max(lw - l1, 0) << 10) |
-             ((unsigned)max(lh - l1, 0) << 15);
+    L.t0 = reinterpret_cast<const uchar4*>(pool + H.base) + level_offset(H.lw, H.lh, l0);
+    L.d1 = l1 > l0 ? 1u << (w0 + h0) : 0u;  // level l0's texel count
+    L.dims = (unsigned)w0 | ((unsigned)h0 << 5) | ((unsigned)max(H.lw - l1, 0) << 10) |
+             ((unsigned)max(H.lh - l1, 0) << 15);
     L.f = lod - (float)l0;
     return L;
 }
@@ -135,16 +164,31 @@ __device__ __forceinline__ float4 bilinear_blend(const BilinearTexels& B) {
 // texel loads are in flight together (one memory round trip per tap).  With
 // a blend fraction of 0 the second level is the first, both blends give the
 // same value a, and a + (a - a)·0 is a exactly.
-__device__ __forceinline__ float4 trilinear_levels(const TriLevels& L, float u, float v) {
+struct TrilinearTexels {
+    BilinearTexels b0, b1;
+    float f;
+};
+
+__device__ __forceinline__ TrilinearTexels trilinear_fetch(const TriLevels& L, float u, float v) {
+    TrilinearTexels T;
+    T.b0 = bilinear_fetch(L.t0, (int)(L.dims & 31u), (int)((L.dims >> 5) & 31u), u, v);
+    T.b1 = bilinear_fetch(L.t0 + L.d1, (int)((L.dims >> 10) & 31u), (int)((L.dims >> 15) & 31u), u, v);
+    T.f = L.f;
+    return T;
+}
+
+__device__ __forceinline__ float4 trilinear_blend(const TrilinearTexels& T) {
     constexpr float k = 1.0f / 255.0f;
-    const BilinearTexels B0 = bilinear_fetch(L.t0, (int)(L.dims & 31u), (int)((L.dims >> 5) & 31u), u, v);
-    const BilinearTexels B1 = bilinear_fetch(L.t0 + L.d1, (int)((L.dims >> 10) & 31u), (int)((L.dims >> 15) & 31u), u, v);
-    const float4 c = lerp4(bilinear_blend(B0), bilinear_blend(B1), L.f);
+    const float4 c = lerp4(bilinear_blend(T.b0), bilinear_blend(T.b1), T.f);
     return make_float4(c.x * k, c.y * k, c.z * k, c.w * k);
 }
 
+__device__ __forceinline__ float4 trilinear_levels(const TriLevels& L, float u, float v) {
+    return trilinear_blend(trilinear_fetch(L, u, v));
+}
+
 __device__ float4 trilinear(const uint8_t* pool, const TexDesc& D, float u, float v, float lod) {
-    return trilinear_levels(tri_levels(pool, D, lod), u, v);
+    return trilinear_levels(tri_levels(pool, tex_head(D), lod), u, v);
 }
 
 // Footprint of a textured fragment: number of taps along the major axis,
@@ -155,11 +199,11 @@ struct TexFootprint {
     int n;
 };
 
-__device__ __forceinline__ TexFootprint footprint(const TexDesc& D, int sampler, int maxAniso, float dudx, float dvdx,
+__device__ __forceinline__ TexFootprint footprint(const TexHead& D, int sampler, int maxAniso, float dudx, float dvdx,
                                                   float dudy, float dvdy) {
     TexFootprint F{0.0f, 0.0f, 0.0f, 1};
     if (sampler == 0) return F;  // bilinear on level 0
-    const float W = (float)D.width, H = (float)D.height;
+    const float W = (float)(1 << D.lw), H = (float)(1 << D.lh);
     const float px = fsqrt((dudx * W) * (dudx * W) + (dvdx * H) * (dvdx * H));
     const float py = fsqrt((dudy * W) * (dudy * W) + (dvdy * H) * (dvdy * H));
     const float pmax = fmaxf(px, py), pmin = fminf(px, py);
@@ -200,7 +244,8 @@ struct FragPrep {
     float alpha;  // translucent flat colour: alpha from the hash
     float u, v;
     TexFootprint fp;
-    int mat;  // texture index, -1 when untextured
+    int mat;     // texture index, -1 when untextured
+    TexHead th;  // its header
     // layered GGX: shading frame, view direction and position
     float3 N, Tg, Bg, V, Pw;
 };
@@ -231,6 +276,7 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, float3 cam, const ShadeT
                                   const int* idx, const TexDesc* tex, int px, int py, float o) {
     FragPrep R;
     R.mat = -1;
+    R.th = TexHead{0ull, 0, 0, 1};
     R.u = R.v = 0.0f;
     R.fp = TexFootprint{0.0f, 0.0f, 0.0f, 0};
     R.alpha = 1.0f;
@@ -248,6 +294,17 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, float3 cam, const ShadeT
     const float* v0 = vtx + (size_t)idx[3 * T.prim] * P.stride;
     const float* v1 = vtx + (size_t)idx[3 * T.prim + 1] * P.stride;
     const float* v2 = vtx + (size_t)idx[3 * T.prim + 2] * P.stride;
+    if (P.numTextures > 0) {
+        // the texture header depends on the primitive only: it is read in the same round
+        // trip as the vertex indices (issued after them, so that they are in flight together)
+        const int mat = P.trisPerMaterial > 0 ? T.prim / P.trisPerMaterial : 0;
+        R.mat = P.program >= 3 ? 0 : min(mat, P.numTextures - 1);  // the layers share one footprint
+        R.th = tex_head(tex[R.mat]);
+    }
+    // texture coordinates are read with the other vertex attributes (always: an untextured
+    // vertex reads its first two floats)
+    const int ou = P.numTextures > 0 ? P.offUv : 0;
+    const float t0u = v0[ou], t0v = v0[ou + 1], t1u = v1[ou], t1v = v1[ou + 1], t2u = v2[ou], t2v = v2[ou + 1];
     float w[3];
     vertex_weights(T, x, y, w);
     auto interp = [&](int off) { return fmaf(w[0], v0[off], fmaf(w[1], v1[off], w[2] * v2[off])); };
@@ -267,8 +324,8 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, float3 cam, const ShadeT
         R.A = R.B = make_float3(0, 0, 0);
     }
     if (P.numTextures > 0) {
-        R.u = interp(P.offUv);
-        R.v = interp(P.offUv + 1);
+        R.u = fmaf(w[0], t0u, fmaf(w[1], t1u, w[2] * t2u));
+        R.v = fmaf(w[0], t0v, fmaf(w[1], t1v, w[2] * t2v));
         // fine quad derivatives (SBC:36-76): same triangle at the quad's other shading positions
         const float qx = (float)(px & ~1) + o, qy = (float)(py & ~1) + o;
         float wa[3], wb[3], wc[3], wd[3];
@@ -276,12 +333,11 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, float3 cam, const ShadeT
         vertex_weights(T, qx + 1.0f, y, wb);
         vertex_weights(T, x, qy, wc);
         vertex_weights(T, x, qy + 1.0f, wd);
-        auto iuv = [&](const float* ww, int off) { return fmaf(ww[0], v0[off], fmaf(ww[1], v1[off], ww[2] * v2[off])); };
-        const float dudx = iuv(wb, P.offUv) - iuv(wa, P.offUv), dvdx = iuv(wb, P.offUv + 1) - iuv(wa, P.offUv + 1);
-        const float dudy = iuv(wd, P.offUv) - iuv(wc, P.offUv), dvdy = iuv(wd, P.offUv + 1) - iuv(wc, P.offUv + 1);
-        int mat = P.trisPerMaterial > 0 ? T.prim / P.trisPerMaterial : 0;
-        R.mat = P.program >= 3 ? 0 : min(mat, P.numTextures - 1);  // the layers share one footprint
-        R.fp = footprint(tex[R.mat], P.sampler, P.maxAniso, dudx, dvdx, dudy, dvdy);
+        auto iu = [&](const float* ww) { return fmaf(ww[0], t0u, fmaf(ww[1], t1u, ww[2] * t2u)); };
+        auto iv = [&](const float* ww) { return fmaf(ww[0], t0v, fmaf(ww[1], t1v, ww[2] * t2v)); };
+        const float dudx = iu(wb) - iu(wa), dvdx = iv(wb) - iv(wa);
+        const float dudy = iu(wd) - iu(wc), dvdy = iv(wd) - iv(wc);
+        R.fp = footprint(R.th, P.sampler, P.maxAniso, dudx, dvdx, dudy, dvdy);
     }
     if (P.program == 4 && P.neural.singleTap && R.mat >= 0) {
         // the neural material's latent: one trilinear tap at the isotropic footprint log2(pmax)
@@ -500,7 +556,7 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
                 if (!NEURAL && !layered) {
                     // the material's two levels, once per fragment, over tCol2 (only the layered and
                     // neural programs use it)
-                    const TriLevels L = tri_levels(pool, tex[R.mat], R.fp.lod);
+                    const TriLevels L = tri_levels(pool, R.th, R.fp.lod);
                     // kept as a byte offset into the pool: a pointer rebuilt from the pool
                     // argument stays in the global address space (global, not flat, loads)
                     const unsigned long long a = (unsigned long long)(reinterpret_cast<const uint8_t*>(L.t0) - pool);
@@ -521,34 +577,59 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
             }
             __syncthreads();
             const int nt = min(kTapWindow, TT - w0);
-            for (int j = lane; j < nt; j += 64) {
-                const unsigned e = tList[j];
-                const int fl = (int)(e >> 4), ti = (int)(e & 15u);
-                const float4 uv = cUV[fl];
-                const int tn = cN[fl];
-                const float invn = frcp((float)tn);
-                const float to = ((float)ti + 0.5f) * invn - 0.5f;
-                const float tu = fmaf(uv.z, to, uv.x), tv = fmaf(uv.w, to, uv.y);
-                if (NEURAL) {
-                    float z[kLatentCh];
-                    nn::latent_tap_h(P.neural.latent, tu, tv, cLod[fl] - (float)P.neural.lodShift, z);
-                    tCol[j] = make_float4(z[0], z[1], z[2], z[3]);
-                    tCol2[j] = make_float4(z[4], z[5], z[6], z[7]);
-                    continue;
-                }
-                if (layered) {
-                    tCol[j] = trilinear(pool, tex[cMat[fl]], tu, tv, cLod[fl]);
-                    const float4 nm = trilinear(pool, tex[1], tu, tv, cLod[fl]);
-                    const float4 rg = trilinear(pool, tex[2], tu, tv, cLod[fl]);
-                    tCol2[j] = make_float4(nm.x, nm.y, nm.z, rg.x);
-                } else {
+            if (!NEURAL && !layered) {
+                // one texture, levels prepared per fragment
+                auto tap_texels = [&](int j) {
+                    const unsigned e = tList[j];
+                    const int fl = (int)(e >> 4), ti = (int)(e & 15u);
+                    const float4 uv = cUV[fl];
+                    const float invn = frcp((float)cN[fl]);
+                    const float to = ((float)ti + 0.5f) * invn - 0.5f;
+                    const float tu = fmaf(uv.z, to, uv.x), tv = fmaf(uv.w, to, uv.y);
                     const uint4 q = reinterpret_cast<const uint4*>(tCol2)[fl];
                     TriLevels L;
                     L.t0 = reinterpret_cast<const uchar4*>(pool + (((unsigned long long)q.y << 32) | q.x));
                     L.d1 = q.z;
                     L.dims = q.w;
                     L.f = cLod[fl];
-                    tCol[j] = trilinear_levels(L, tu, tv);
+                    return trilinear_fetch(L, tu, tv);
+                };
+                if constexpr (S > 1) {
+                    // a lane's two taps of the window (j and j + 64) fetch their sixteen texels
+                    // before either is blended (at 1 sample per pixel the extra registers spill)
+                    static_assert(kTapWindow == 128, "two taps per lane per window");
+                    if (lane < nt) {
+                        const TrilinearTexels A = tap_texels(lane);
+                        const bool two = lane + 64 < nt;
+                        TrilinearTexels B;
+                        if (two) B = tap_texels(lane + 64);
+                        tCol[lane] = trilinear_blend(A);
+                        if (two) tCol[lane + 64] = trilinear_blend(B);
+                    }
+                } else {
+                    for (int j = lane; j < nt; j += 64) tCol[j] = trilinear_blend(tap_texels(j));
+                }
+            } else {
+                for (int j = lane; j < nt; j += 64) {
+                    const unsigned e = tList[j];
+                    const int fl = (int)(e >> 4), ti = (int)(e & 15u);
+                    const float4 uv = cUV[fl];
+                    const int tn = cN[fl];
+                    const float invn = frcp((float)tn);
+                    const float to = ((float)ti + 0.5f) * invn - 0.5f;
+                    const float tu = fmaf(uv.z, to, uv.x), tv = fmaf(uv.w, to, uv.y);
+                    if (NEURAL) {
+                        float z[kLatentCh];
+                        nn::latent_tap_h(P.neural.latent, tu, tv, cLod[fl] - (float)P.neural.lodShift, z);
+                        tCol[j] = make_float4(z[0], z[1], z[2], z[3]);
+                        tCol2[j] = make_float4(z[4], z[5], z[6], z[7]);
+                        continue;
+                    }
+                    // layered GGX: three textures per tap
+                    tCol[j] = trilinear(pool, tex[cMat[fl]], tu, tv, cLod[fl]);
+                    const float4 nm = trilinear(pool, tex[1], tu, tv, cLod[fl]);
+                    const float4 rg = trilinear(pool, tex[2], tu, tv, cLod[fl]);
+                    tCol2[j] = make_float4(nm.x, nm.y, nm.z, rg.x);
                 }
             }
             __syncthreads();
