@@ -633,21 +633,28 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
                 }
             }
             __syncthreads();
+            // the lane's own taps of this window, added in tap order; four LDS reads are
+            // issued before their adds
             const int b0 = max(toff - w0, 0), b1 = min(toff + n - w0, kTapWindow);
-            for (int j = b0; j < b1; j++) {
-                const float4 c = tCol[j];
-                acc.x += c.x;
-                acc.y += c.y;
-                acc.z += c.z;
-                acc.w += c.w;
-                if (NEURAL || layered) {
-                    const float4 c2 = tCol2[j];
-                    acc2.x += c2.x;
-                    acc2.y += c2.y;
-                    acc2.z += c2.z;
-                    acc2.w += c2.w;
+            auto add4 = [](float4& a, const float4 c) {
+                a.x += c.x;
+                a.y += c.y;
+                a.z += c.z;
+                a.w += c.w;
+            };
+            auto sum_taps = [&](const float4* buf, float4& a) {
+                int j = b0;
+                for (; j + 4 <= b1; j += 4) {
+                    const float4 c0 = buf[j], c1 = buf[j + 1], c2 = buf[j + 2], c3 = buf[j + 3];
+                    add4(a, c0);
+                    add4(a, c1);
+                    add4(a, c2);
+                    add4(a, c3);
                 }
-            }
+                for (; j < b1; j++) add4(a, buf[j]);
+            };
+            sum_taps(tCol, acc);
+            if (NEURAL || layered) sum_taps(tCol2, acc2);
         }
         if (act) {
             float4 alb = make_float4(1, 1, 1, 1);
