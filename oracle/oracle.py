@@ -263,5 +263,18 @@ def ref_blend(width, height, clip, indices, factors, depth_func=1):
     return out
 
 
+def sample_tex(texels, width, height, levels, sampler, max_aniso, u, v, dudx, dvdx, dudy, dvdy):
+    """One FP64 texture lookup of the oracle's sampler (RGBA8 texels with the
+    packed mip chain); returns (rgba in 0..1, too-close-to-call flag)."""
+    L = lib()
+    f = L.oracle_sample_tex
+    f.restype = C.c_int
+    t = np.ascontiguousarray(texels, np.uint8)
+    out = np.zeros(4, np.float64)
+    flag = f(_p(t), C.c_int(width), C.c_int(height), C.c_int(levels), C.c_int(sampler), C.c_int(max_aniso),
+             *[C.c_double(x) for x in (u, v, dudx, dvdx, dudy, dvdy)], _p(out))
+    return out, bool(flag)
+
+
 def flat_hash(prim: int) -> int:
     return lib().oracle_flat_hash(prim)

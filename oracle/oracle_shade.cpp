@@ -316,6 +316,24 @@ void oracle_ggx_light(const double* n, const double* v, const double* l, const d
     out[2] = c.z;
 }
 
+// One texture lookup (sample_tex above) of an RGBA8 texture with its packed
+// mip chain, for the closed-form sampler tests; returns 1 when a discrete
+// decision (tap count) was too close to call.
+int oracle_sample_tex(const uint8_t* texels, int width, int height, int levels, int sampler, int maxAniso, double u,
+                      double v, double dudx, double dvdx, double dudy, double dvdy, double* out) {
+    Tex t;
+    t.base = texels;
+    t.w = width;
+    t.h = height;
+    t.levels = levels;
+    size_t off = 0;
+    for (int l = 0; l < levels; l++) {
+        t.off.push_back(off);
+        off += (size_t)std::max(1, width >> l) * std::max(1, height >> l) * 4;
+    }
+    return sample_tex(t, sampler, maxAniso, u, v, dudx, dvdx, dudy, dvdy, out) ? 1 : 0;
+}
+
 uint32_t oracle_flat_hash(uint32_t prim) {
     uint32_t h = prim * 0x9E3779B1u;
     h ^= h >> 15;

@@ -64,6 +64,13 @@ int oracle_shade(const OracleFrame* f, const OracleShadeParams* p, int W, int H,
 // Box-filter mip generation, (a + b + c + d + 2) >> 2 per channel.
 void oracle_build_mips(uint8_t* texels, int width, int height, int levels);
 
+// One texture lookup of the sampler (0 bilinear on level 0, 1 trilinear, 2
+// anisotropic) at (u, v) with the screen derivatives of (u, v), on an RGBA8
+// texture with its packed mip chain; out = RGBA in 0..1.  Returns 1 when the
+// tap count was too close to call (a whole-number anisotropy ratio).
+int oracle_sample_tex(const uint8_t* texels, int width, int height, int levels, int sampler, int maxAniso, double u,
+                      double v, double dudx, double dvdx, double dudy, double dvdy, double* out);
+
 #ifdef __cplusplus
 }
 #endif
