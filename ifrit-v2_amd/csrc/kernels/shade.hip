@@ -272,8 +272,11 @@ __device__ __forceinline__ float3 ggx_light(float3 n, float3 V, float3 L, float4
     return make_float3((alb.x * diff + spec) * ndl, (alb.y * diff + spec) * ndl, (alb.z * diff + spec) * ndl);
 }
 
+// frame = false for callers that never run the programs from 3 on (layered GGX
+// and the neural material): their shading-frame paths then fold away.
 __device__ FragPrep prep_fragment(const ShadeParams& P, float3 cam, const ShadeTri& T, const float* vtx,
-                                  const int* idx, const TexDesc* tex, int px, int py, float o) {
+                                  const int* idx, const TexDesc* tex, int px, int py, float o, bool frame) {
+    const bool framed = frame && P.program >= 3;
     FragPrep R;
     R.mat = -1;
     R.th = TexHead{0ull, 0, 0, 1};
@@ -298,7 +301,7 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, float3 cam, const ShadeT
         // the texture header depends on the primitive only: it is read in the same round
         // trip as the vertex indices (issued after them, so that they are in flight together)
         const int mat = P.trisPerMaterial > 0 ? T.prim / P.trisPerMaterial : 0;
-        R.mat = P.program >= 3 ? 0 : min(mat, P.numTextures - 1);  // the layers share one footprint
+        R.mat = framed ? 0 : min(mat, P.numTextures - 1);  // the layers share one footprint
         R.th = tex_head(tex[R.mat]);
     }
     // texture coordinates are read with the other vertex attributes (always: an untextured
@@ -310,7 +313,7 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, float3 cam, const ShadeT
     auto interp = [&](int off) { return fmaf(w[0], v0[off], fmaf(w[1], v1[off], w[2] * v2[off])); };
     const float3 Pw = make_float3(interp(P.offWorld), interp(P.offWorld + 1), interp(P.offWorld + 2));
     float3 N = make_float3(interp(P.offNormal), interp(P.offNormal + 1), interp(P.offNormal + 2));
-    if (P.program >= 3) {
+    if (framed) {
         // shading frame: interpolated normal, tangent made orthogonal to it, bitangent N x T
         N = norm3(N);
         float3 Tg = make_float3(interp(P.offTangent), interp(P.offTangent + 1), interp(P.offTangent + 2));
@@ -339,13 +342,13 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, float3 cam, const ShadeT
         const float dudy = iu(wd) - iu(wc), dvdy = iv(wd) - iv(wc);
         R.fp = footprint(R.th, P.sampler, P.maxAniso, dudx, dvdx, dudy, dvdy);
     }
-    if (P.program == 4 && P.neural.singleTap && R.mat >= 0) {
+    if (frame && P.program == 4 && P.neural.singleTap && R.mat >= 0) {
         // the neural material's latent: one trilinear tap at the isotropic footprint log2(pmax)
         R.fp.lod += flog2((float)R.fp.n);
         R.fp.n = 1;
         R.fp.au = R.fp.av = 0.0f;
     }
-    if (P.program >= 3) return R;  // lighting needs the sampled layers (or the network)
+    if (framed) return R;  // lighting needs the sampled layers (or the network)
     N = norm3(N);
     const float3 V = norm3(make_float3(cam.x - Pw.x, cam.y - Pw.y, cam.z - Pw.z));
     float3 A = make_float3(P.ambient, P.ambient, P.ambient), B = make_float3(0, 0, 0);
@@ -450,7 +453,7 @@ constexpr int kTapWindow = 128;
 //      fragments in sample order, so results do not depend on scheduling.
 // (ox, oy) is the sub-tile's first pixel in its view; visSlot, rgba8 and
 // rgba32f point at the view's outputs and pixBase is the view's first pixel.
-template <int S, bool NEURAL, bool FP8>
+template <int S, bool NEURAL, bool FP8, bool LAYERED>
 __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, int width, int height,
                                               unsigned fanStart, int ox, int oy, size_t pixBase, const TriRecs& recs,
                                               const float4* __restrict__ bary, const uint32_t* __restrict__ visSlot,
@@ -530,8 +533,8 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
     }
     __syncthreads();
 
-    if (P.program == 5 && S == 1 && inFb && nf == 0) P.neural.featOut[4 * (pixBase + pix) + 1] = make_float4(0, 0, 0, 0);
-    const bool layered = P.program == 3 || P.program == 5;  // three textures per tap
+    if (LAYERED && P.program == 5 && S == 1 && inFb && nf == 0) P.neural.featOut[4 * (pixBase + pix) + 1] = make_float4(0, 0, 0, 0);
+    constexpr bool layered = LAYERED;  // programs 3 and 5: three textures per tap
     const float o = S == 1 ? 0.0f : 0.5f;
     float4 sum = make_float4(0, 0, 0, 0);
     for (int c0 = 0; c0 < F; c0 += 64) {
@@ -546,7 +549,7 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
             ShadeTri T;
             load_shade_tri(recs, bary, fanStart, slot, T);
             const int qx = ox + (fl & 7), qy = oy + (fl >> 3);
-            R = prep_fragment(P, cam, T, vtx, idx, tex, qx, qy, o);
+            R = prep_fragment(P, cam, T, vtx, idx, tex, qx, qy, o, NEURAL || LAYERED);
             if (R.mat >= 0) {
                 n = R.fp.n;
                 cUV[lane] = make_float4(R.u, R.v, R.fp.au, R.fp.av);
@@ -662,7 +665,7 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
                 const float invn = frcp((float)n);
                 alb = make_float4(acc.x * invn, acc.y * invn, acc.z * invn, acc.w * invn);
             }
-            if (P.program == 3 || P.program == 5) {
+            if (LAYERED) {
                 float4 nm = make_float4(0.5f, 0.5f, 1.0f, 0.0f);
                 if (R.mat >= 0) {
                     const float invn = frcp((float)n);
@@ -746,7 +749,7 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
 // 8N), its 8x8 sub-tiles in row order; consecutive tiles on one XCD share its
 // L2 (texture and vertex reuse).  Tiles of all shaded views are numbered one
 // after another.
-template <int S, bool NEURAL, bool FP8, int N>
+template <int S, bool NEURAL, bool FP8, int N, bool LAYERED>
 __device__ __forceinline__ void shade_tiles(const ShadeParams& P, const Views& V, const TriRecs& recs,
                                             const float4* __restrict__ bary, const uint32_t* __restrict__ visSlot,
                                             const float* __restrict__ vtx, const int* __restrict__ idx,
@@ -764,14 +767,14 @@ __device__ __forceinline__ void shade_tiles(const ShadeParams& P, const Views& V
     const float3 cam = make_float3(D.camera[0], D.camera[1], D.camera[2]);
     const unsigned fanStart = D.slotBase + (unsigned)P.ntri;
     if constexpr (N == 1) {
-        shade_subtile<S, NEURAL, FP8>(P, cam, width, height, fanStart, ox0, oy0, pb, recs, bary, visSlot + pb * S,
+        shade_subtile<S, NEURAL, FP8, LAYERED>(P, cam, width, height, fanStart, ox0, oy0, pb, recs, bary, visSlot + pb * S,
                                       vtx, idx, tex, pool, rgba8 + pb, rgba32f ? rgba32f + pb : nullptr);
     } else {
 #pragma unroll 1
         for (int k = 0; k < N * N; k++) {
             const int ox = ox0 + (k % N) * 8, oy = oy0 + (k / N) * 8;
             if (ox >= width || oy >= height) continue;
-            shade_subtile<S, NEURAL, FP8>(P, cam, width, height, fanStart, ox, oy, pb, recs, bary, visSlot + pb * S,
+            shade_subtile<S, NEURAL, FP8, LAYERED>(P, cam, width, height, fanStart, ox, oy, pb, recs, bary, visSlot + pb * S,
                                           vtx, idx, tex, pool, rgba8 + pb, rgba32f ? rgba32f + pb : nullptr);
         }
     }
@@ -782,7 +785,7 @@ __device__ __forceinline__ void shade_tiles(const ShadeParams& P, const Views& V
 // order), so an opaque fragment gets the same colour either way.
 __device__ float4 shade_fragment(const ShadeParams& P, float3 cam, const ShadeTri& T, const float* vtx, const int* idx,
                                  const TexDesc* tex, const uint8_t* pool, int px, int py) {
-    const FragPrep R = prep_fragment(P, cam, T, vtx, idx, tex, px, py, 0.0f);
+    const FragPrep R = prep_fragment(P, cam, T, vtx, idx, tex, px, py, 0.0f, true);
     if (P.program == 0 || P.program == 6) return make_float4(sat(R.B.x), sat(R.B.y), sat(R.B.z), R.alpha);
     float4 acc = make_float4(0, 0, 0, 0), acc2 = make_float4(0, 0, 0, 0);
     const int n = R.mat >= 0 ? R.fp.n : 0;
@@ -918,14 +921,16 @@ __global__ void __launch_bounds__(256) k_blend_ordered(ShadeParams P, BlendCoefs
     }
 }
 
-// Four waves per SIMD (at most 128 VGPRs), as the neural program.
-template <int S, int N>
+// Four waves per SIMD (at most 128 VGPRs), as the neural program.  LAYERED:
+// programs 3 and 5 (layered GGX); the others do not keep the shading frame.
+template <int S, int N, bool LAYERED>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_shade(ShadeParams P, Views V, TriRecs recs, const float4* __restrict__ bary, const uint32_t* __restrict__ visSlot,
         const float* __restrict__ vtx, const int* __restrict__ idx, const TexDesc* __restrict__ tex,
         const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8, float4* __restrict__ rgba32f, int tiles,
         int perXcd) {
-    shade_tiles<S, false, false, N>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles, perXcd);
+    shade_tiles<S, false, false, N, LAYERED>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles,
+                                             perXcd);
 }
 
 // The neural material, held to four waves per SIMD like the other programs so
@@ -936,7 +941,7 @@ k_shade_neural(ShadeParams P, Views V, TriRecs recs, const float4* __restrict__ 
                const uint32_t* __restrict__ visSlot, const float* __restrict__ vtx, const int* __restrict__ idx,
                const TexDesc* __restrict__ tex, const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8,
                float4* __restrict__ rgba32f, int tiles, int perXcd) {
-    shade_tiles<S, true, FP8, N>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles, perXcd);
+    shade_tiles<S, true, FP8, N, false>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles, perXcd);
 }
 
 }  // namespace
@@ -961,8 +966,10 @@ void launch_shade(const ShadeParams& p, const Views& views, int shadeTiles, TriR
 #define IFRIT_ARGS p, views, recs, bary, visSlot, vtx, idx, tex, texPool, rgba8, rgba32f, shadeTiles, perXcd
 #define IFRIT_SHADE_N(SN, NN)                                                                                      \
     do {                                                                                                            \
-        if (p.program != 4)                                                                                         \
-            hipLaunchKernelGGL((k_shade<SN, NN>), grid, dim3(64), 0, s, IFRIT_ARGS);                                \
+        if (p.program == 3 || p.program == 5)                                                                       \
+            hipLaunchKernelGGL((k_shade<SN, NN, true>), grid, dim3(64), 0, s, IFRIT_ARGS);                          \
+        else if (p.program != 4)                                                                                    \
+            hipLaunchKernelGGL((k_shade<SN, NN, false>), grid, dim3(64), 0, s, IFRIT_ARGS);                         \
         else if (p.neural.precision == 1)                                                                           \
             hipLaunchKernelGGL((k_shade_neural<SN, true, NN>), grid, dim3(64), 0, s, IFRIT_ARGS);                   \
         else                                                                                                        \
