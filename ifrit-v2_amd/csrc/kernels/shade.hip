@@ -466,15 +466,22 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
     __shared__ float4 cUV[64];      // u, v, au, av of the chunk's fragments
     __shared__ float cLod[64];
     __shared__ int cN[64];
-    __shared__ int cMat[64];
-    __shared__ float4 cCol[64];     // clamped colour of the chunk's fragments
+    // The layered and neural programs keep a second value per tap; a single
+    // texture needs only the fragment's prepared levels there, and its fragment
+    // colours reuse the tap colours' space (the taps are summed first).  Less
+    // LDS per wave lets more waves share a CU.
+    constexpr bool kTwo = NEURAL || LAYERED;
+    __shared__ int cMat[LAYERED ? 64 : 1];
+    __shared__ float4 cColOwn[kTwo ? 64 : 1];
     __shared__ float4 xdir[64];     // network inputs 10..13 (view z, light xyz)
     __shared__ unsigned short tList[kTapWindow];
     // per tap: colour or latent channels 0-3 (tCol); layered GGX normal xyz
-    // and roughness, or latent channels 4-7 (tCol2); network inputs after the taps
-    __shared__ float4 tBuf[2 * kTapWindow];
+    // and roughness, or latent channels 4-7 (tCol2), or per fragment its two
+    // mip levels (one texture); network inputs after the taps
+    __shared__ float4 tBuf[kTapWindow + (kTwo ? kTapWindow : 64)];
     float4* const tCol = tBuf;
     float4* const tCol2 = tBuf + kTapWindow;
+    float4* const cCol = kTwo ? cColOwn : tBuf;  // clamped colour of the chunk's fragments
 
     const int lane = threadIdx.x;
     const int px = ox + (lane & 7);
@@ -555,7 +562,7 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
                 cUV[lane] = make_float4(R.u, R.v, R.fp.au, R.fp.av);
                 cLod[lane] = R.fp.lod;
                 cN[lane] = n;
-                cMat[lane] = R.mat;
+                if constexpr (LAYERED) cMat[lane] = R.mat;
                 if (!NEURAL && !layered) {
                     // the material's two levels, once per fragment, over tCol2 (only the layered and
                     // neural programs use it)
@@ -921,16 +928,54 @@ __global__ void __launch_bounds__(256) k_blend_ordered(ShadeParams P, BlendCoefs
     }
 }
 
-// Four waves per SIMD (at most 128 VGPRs), as the neural program.  LAYERED:
-// programs 3 and 5 (layered GGX); the others do not keep the shading frame.
-template <int S, int N, bool LAYERED>
+// One texture (Blinn-Phong) or flat colour with MSAA: four waves per SIMD (at
+// most 128 VGPRs; held to 96 for a fifth wave, the 8-sample kernel spills and
+// is slower).
+template <int S, int N>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_shade(ShadeParams P, Views V, TriRecs recs, const float4* __restrict__ bary, const uint32_t* __restrict__ visSlot,
         const float* __restrict__ vtx, const int* __restrict__ idx, const TexDesc* __restrict__ tex,
         const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8, float4* __restrict__ rgba32f, int tiles,
         int perXcd) {
-    shade_tiles<S, false, false, N, LAYERED>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles,
-                                             perXcd);
+    shade_tiles<S, false, false, N, false>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles,
+                                           perXcd);
+}
+
+// The same at 1 sample per pixel: five waves per SIMD (at most 96 VGPRs, with
+// no spills, and under 8 KB of LDS per wave).
+template <int N>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8)))
+k_shade_1spp(ShadeParams P, Views V, TriRecs recs, const float4* __restrict__ bary,
+             const uint32_t* __restrict__ visSlot, const float* __restrict__ vtx, const int* __restrict__ idx,
+             const TexDesc* __restrict__ tex, const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8,
+             float4* __restrict__ rgba32f, int tiles, int perXcd) {
+    shade_tiles<1, false, false, N, false>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles,
+                                           perXcd);
+}
+
+template <int S, int N>
+void launch_shade_one_texture(dim3 grid, hipStream_t s, const ShadeParams& p, const Views& views, TriRecs recs,
+                              const float4* bary, const uint32_t* visSlot, const float* vtx, const int* idx,
+                              const TexDesc* tex, const uint8_t* texPool, uchar4* rgba8, float4* rgba32f,
+                              int shadeTiles, int perXcd) {
+    if constexpr (S == 1)
+        hipLaunchKernelGGL((k_shade_1spp<N>), grid, dim3(64), 0, s, p, views, recs, bary, visSlot, vtx, idx, tex,
+                           texPool, rgba8, rgba32f, shadeTiles, perXcd);
+    else
+        hipLaunchKernelGGL((k_shade<S, N>), grid, dim3(64), 0, s, p, views, recs, bary, visSlot, vtx, idx, tex,
+                           texPool, rgba8, rgba32f, shadeTiles, perXcd);
+}
+
+// Layered GGX (programs 3 and 5): four waves per SIMD (at most 128 VGPRs), as
+// the neural program.
+template <int S, int N>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_shade_layered(ShadeParams P, Views V, TriRecs recs, const float4* __restrict__ bary,
+                const uint32_t* __restrict__ visSlot, const float* __restrict__ vtx, const int* __restrict__ idx,
+                const TexDesc* __restrict__ tex, const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8,
+                float4* __restrict__ rgba32f, int tiles, int perXcd) {
+    shade_tiles<S, false, false, N, true>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles,
+                                          perXcd);
 }
 
 // The neural material, held to four waves per SIMD like the other programs so
@@ -967,9 +1012,9 @@ void launch_shade(const ShadeParams& p, const Views& views, int shadeTiles, TriR
 #define IFRIT_SHADE_N(SN, NN)                                                                                      \
     do {                                                                                                            \
         if (p.program == 3 || p.program == 5)                                                                       \
-            hipLaunchKernelGGL((k_shade<SN, NN, true>), grid, dim3(64), 0, s, IFRIT_ARGS);                          \
+            hipLaunchKernelGGL((k_shade_layered<SN, NN>), grid, dim3(64), 0, s, IFRIT_ARGS);                        \
         else if (p.program != 4)                                                                                    \
-            hipLaunchKernelGGL((k_shade<SN, NN, false>), grid, dim3(64), 0, s, IFRIT_ARGS);                         \
+            launch_shade_one_texture<SN, NN>(grid, s, IFRIT_ARGS);                                                  \
         else if (p.neural.precision == 1)                                                                           \
             hipLaunchKernelGGL((k_shade_neural<SN, true, NN>), grid, dim3(64), 0, s, IFRIT_ARGS);                   \
         else                                                                                                        \
