@@ -604,21 +604,7 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
                     L.f = cLod[fl];
                     return trilinear_fetch(L, tu, tv);
                 };
-                if constexpr (S > 1) {
-                    // a lane's two taps of the window (j and j + 64) fetch their sixteen texels
-                    // before either is blended (at 1 sample per pixel the extra registers spill)
-                    static_assert(kTapWindow == 128, "two taps per lane per window");
-                    if (lane < nt) {
-                        const TrilinearTexels A = tap_texels(lane);
-                        const bool two = lane + 64 < nt;
-                        TrilinearTexels B;
-                        if (two) B = tap_texels(lane + 64);
-                        tCol[lane] = trilinear_blend(A);
-                        if (two) tCol[lane + 64] = trilinear_blend(B);
-                    }
-                } else {
-                    for (int j = lane; j < nt; j += 64) tCol[j] = trilinear_blend(tap_texels(j));
-                }
+                for (int j = lane; j < nt; j += 64) tCol[j] = trilinear_blend(tap_texels(j));
             } else {
                 for (int j = lane; j < nt; j += 64) {
                     const unsigned e = tList[j];
@@ -928,42 +914,17 @@ __global__ void __launch_bounds__(256) k_blend_ordered(ShadeParams P, BlendCoefs
     }
 }
 
-// One texture (Blinn-Phong) or flat colour with MSAA: four waves per SIMD (at
-// most 128 VGPRs; held to 96 for a fifth wave, the 8-sample kernel spills and
-// is slower).
+// One texture (Blinn-Phong) or flat colour: five waves per SIMD (at most 96
+// VGPRs, with no spills, and under 8 KB of LDS per wave).  Fetching two taps
+// per lane at once needs more registers: at four waves it is slower.
 template <int S, int N>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8)))
 k_shade(ShadeParams P, Views V, TriRecs recs, const float4* __restrict__ bary, const uint32_t* __restrict__ visSlot,
         const float* __restrict__ vtx, const int* __restrict__ idx, const TexDesc* __restrict__ tex,
         const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8, float4* __restrict__ rgba32f, int tiles,
         int perXcd) {
     shade_tiles<S, false, false, N, false>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles,
                                            perXcd);
-}
-
-// The same at 1 sample per pixel: five waves per SIMD (at most 96 VGPRs, with
-// no spills, and under 8 KB of LDS per wave).
-template <int N>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8)))
-k_shade_1spp(ShadeParams P, Views V, TriRecs recs, const float4* __restrict__ bary,
-             const uint32_t* __restrict__ visSlot, const float* __restrict__ vtx, const int* __restrict__ idx,
-             const TexDesc* __restrict__ tex, const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8,
-             float4* __restrict__ rgba32f, int tiles, int perXcd) {
-    shade_tiles<1, false, false, N, false>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles,
-                                           perXcd);
-}
-
-template <int S, int N>
-void launch_shade_one_texture(dim3 grid, hipStream_t s, const ShadeParams& p, const Views& views, TriRecs recs,
-                              const float4* bary, const uint32_t* visSlot, const float* vtx, const int* idx,
-                              const TexDesc* tex, const uint8_t* texPool, uchar4* rgba8, float4* rgba32f,
-                              int shadeTiles, int perXcd) {
-    if constexpr (S == 1)
-        hipLaunchKernelGGL((k_shade_1spp<N>), grid, dim3(64), 0, s, p, views, recs, bary, visSlot, vtx, idx, tex,
-                           texPool, rgba8, rgba32f, shadeTiles, perXcd);
-    else
-        hipLaunchKernelGGL((k_shade<S, N>), grid, dim3(64), 0, s, p, views, recs, bary, visSlot, vtx, idx, tex,
-                           texPool, rgba8, rgba32f, shadeTiles, perXcd);
 }
 
 // Layered GGX (programs 3 and 5): four waves per SIMD (at most 128 VGPRs), as
@@ -1014,7 +975,7 @@ void launch_shade(const ShadeParams& p, const Views& views, int shadeTiles, TriR
         if (p.program == 3 || p.program == 5)                                                                       \
             hipLaunchKernelGGL((k_shade_layered<SN, NN>), grid, dim3(64), 0, s, IFRIT_ARGS);                        \
         else if (p.program != 4)                                                                                    \
-            launch_shade_one_texture<SN, NN>(grid, s, IFRIT_ARGS);                                                  \
+            hipLaunchKernelGGL((k_shade<SN, NN>), grid, dim3(64), 0, s, IFRIT_ARGS);                                \
         else if (p.neural.precision == 1)                                                                           \
             hipLaunchKernelGGL((k_shade_neural<SN, true, NN>), grid, dim3(64), 0, s, IFRIT_ARGS);                   \
         else                                                                                                        \
