@@ -460,7 +460,6 @@ bool TileRasterRenderer::prepare(int indexCount) {
             td.width = t.width;
             td.height = t.height;
             td.levels = t.levels;
-            for (int l = 0; l < t.levels; l++) td.levelOffset[l] = (unsigned)t.levelOffset[l];
             d[i] = td;
         }
         if (!texDesc_.ensure(d.size() * sizeof(TexDesc))) return setError("hipMalloc textures"), false;

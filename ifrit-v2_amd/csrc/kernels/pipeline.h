@@ -120,7 +120,8 @@ struct RasterParams {
 struct TexDesc {
     unsigned long long base;  // byte offset of level 0 in the texture pool
     int width, height, levels, pad;
-    unsigned levelOffset[16];  // texel offsets of each level relative to base
+    // levels follow level 0 packed, each max(1, width >> l) x max(1, height >> l);
+    // the sampler computes their offsets from width and height
 };
 
 struct ShadeParams {

@@ -92,8 +92,8 @@ __device__ __forceinline__ TexHead tex_head(const TexDesc& D) {
 }
 
 // Texel offset of level l in the packed chain of a 2^a x 2^b texture whose
-// level k holds max(1, 2^(a-k)) x max(1, 2^(b-k)) texels (the host's layout,
-// also kept in TexDesc::levelOffset).  Levels k <= min(a, b) hold 2^(a+b-2k)
+// level k holds max(1, 2^(a-k)) x max(1, 2^(b-k)) texels (the host's layout
+// in createTexture).  Levels k <= min(a, b) hold 2^(a+b-2k)
 // texels, a geometric sum; then the long side halves alone down to 1, and
 // any further levels hold one texel.
 __device__ __forceinline__ unsigned level_offset(int a, int b, int l) {
