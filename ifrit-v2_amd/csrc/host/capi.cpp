@@ -86,6 +86,7 @@ void iftrOptsetForceDeterministic(IfritRendererHandle* h, int opt) {
 void iftrOptSetDepthTestEnable(IfritRendererHandle* h, int opt) {
     guarded(h, [&](TileRasterRenderer& r) { r.optSetDepthTestEnable(opt != 0); });
 }
+void iftrOptsetDepthTestEnable(IfritRendererHandle* h, int opt) { iftrOptSetDepthTestEnable(h, opt); }
 void iftrDrawLegacy(IfritRendererHandle* h, int numVertices, int clearFramebuffer) {
     guarded(h, [&](TileRasterRenderer& r) { r.drawElements(numVertices, clearFramebuffer != 0); });
 }

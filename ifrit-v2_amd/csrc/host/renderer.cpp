@@ -246,6 +246,11 @@ int TileRasterRenderer::createTexture(int w, int h, int levels) {
         setError("textures must have power-of-two sizes and 1..16 levels");
         return -1;
     }
+    if ((long long)w * h >= (1ll << 28)) {
+        // the sampler addresses a texture's texels (all levels) with 32-bit offsets
+        setError("textures must have fewer than 2^28 texels in level 0");
+        return -1;
+    }
     TextureSlot t;
     t.width = w;
     t.height = h;
@@ -395,6 +400,12 @@ bool TileRasterRenderer::prepare(int indexCount) {
         return false;
     }
     if (fsBound_ && (fs_.desc.kind == IFHIP_FS_GGX_LAYERED || fs_.desc.kind == IFHIP_FS_NEURAL) &&
+        textures_.size() < 3) {
+        // the layered kernels read the descriptors of textures 0, 1 and 2
+        setError("the layered-GGX and neural programs need three textures (albedo, normal, roughness)");
+        return false;
+    }
+    if (fsBound_ && (fs_.desc.kind == IFHIP_FS_GGX_LAYERED || fs_.desc.kind == IFHIP_FS_NEURAL) &&
         (stride_ < 11 || fs_.desc.offTangent + 3 > stride_)) {
         setError("the layered-GGX and neural programs need a tangent in the vertex record");
         return false;
@@ -452,11 +463,16 @@ bool TileRasterRenderer::prepare(int indexCount) {
         return false;
     }
     if (texDescDirty_) {
+        // the pool pointer the kernels offset is the lowest texture allocation, so that every texture's
+        // base is a non-negative byte offset from a real device address
+        texPool_ = nullptr;
+        for (const TextureSlot& t : textures_)
+            if (!texPool_ || t.dev < texPool_) texPool_ = t.dev;
         std::vector<TexDesc> d(std::max<size_t>(1, textures_.size()));
         for (size_t i = 0; i < textures_.size(); i++) {
             const TextureSlot& t = textures_[i];
             TexDesc td{};
-            td.base = (unsigned long long)(uintptr_t)t.dev;
+            td.base = (unsigned long long)(t.dev - texPool_);
             td.width = t.width;
             td.height = t.height;
             td.levels = t.levels;
@@ -683,7 +699,7 @@ void TileRasterRenderer::enqueue(bool clear) {
     if (blend_.blendEnable) {
         // blending: fragments shaded and blended in primitive order (the shade stage is part of it)
         launch_blend_ordered(shadeParams(true), blendCoefs_, depthFunc_, clear ? 1 : 0, binCap_, V, recPtrs(),
-                             bary_.as<float4>(), vtx_.as<float>(), idx_.as<int>(), texDesc_.as<TexDesc>(), nullptr,
+                             bary_.as<float4>(), vtx_.as<float>(), idx_.as<int>(), texDesc_.as<TexDesc>(), texPool_,
                              binStart_.as<unsigned>(), binCount_.as<unsigned>(), binList_.as<unsigned>(), tiles,
                              rasterGrid(), visDepth_.as<float>(), visSlot_.as<uint32_t>(), cls, rgba8_.as<uchar4>(),
                              rgba32f_.as<float4>(), statPtrs(), s);
@@ -709,7 +725,7 @@ void TileRasterRenderer::enqueue(bool clear) {
         const bool wantFloat = batch_.empty() && ((fb_ && !fb_->color.empty() && fb_->color[0]) || floatOut_);
         const ShadeParams sh = shadeParams(wantFloat);
         launch_shade(sh, V, (int)shadeTiles_, recPtrs(), bary_.as<float4>(), visSlot_.as<uint32_t>(),
-                     vtx_.as<float>(), idx_.as<int>(), texDesc_.as<TexDesc>(), nullptr, rgba8_.as<uchar4>(),
+                     vtx_.as<float>(), idx_.as<int>(), texDesc_.as<TexDesc>(), texPool_, rgba8_.as<uchar4>(),
                      wantFloat ? rgba32f_.as<float4>() : nullptr, s);
     }
     mark(6);
@@ -799,12 +815,18 @@ bool TileRasterRenderer::finish(bool clear, int indexCount) {
             stats_.stageMs[i] = ms;
         }
     }
-    if (c.status & kStatusClipOverflow) clipCap_ = std::max(clipCap_ * 2, c.clipMax + 1024u);
-    if (c.status & kStatusBinOverflow) binCap_ = std::max<unsigned>(binCap_ * 2, c.binTotal + (1u << 20));
-    if (c.status & kStatusBigOverflow) bigCap_ = std::max(bigCap_ * 2, c.bigCount + 1024u);
+    // capacities grow in 64-bit arithmetic and stay below 2^32 - 16
+    auto grow = [](unsigned cap, unsigned need) {
+        const unsigned long long g = std::max<unsigned long long>(2ull * cap, need);
+        return (unsigned)std::min<unsigned long long>(g, 0xFFFFFFF0ull);
+    };
+    if (c.status & kStatusClipOverflow) clipCap_ = grow(clipCap_, c.clipMax + 1024u);
+    if (c.status & kStatusBinOverflow) binCap_ = grow(binCap_, (unsigned)std::min<unsigned long long>(
+                                                                   (unsigned long long)c.binTotal + (1u << 20), 0xFFFFFFF0ull));
+    if (c.status & kStatusBigOverflow) bigCap_ = grow(bigCap_, c.bigCount + 1024u);
     if (c.status & kStatusRasterOverflow) {
-        itemCap_ = std::max(itemCap_ * 2, c.numItems + 1024u);
-        partialCap_ = std::max(partialCap_ * 2, c.numPartials + 256u);
+        itemCap_ = grow(itemCap_, c.numItems + 1024u);
+        partialCap_ = grow(partialCap_, c.numPartials + 256u);
     }
     if (c.status & kStatusBadIndex) setError("index buffer references a vertex beyond the vertex buffer");
     return (c.status & (kStatusClipOverflow | kStatusBinOverflow | kStatusBigOverflow | kStatusRasterOverflow)) == 0;
@@ -812,7 +834,8 @@ bool TileRasterRenderer::finish(bool clear, int indexCount) {
 
 void TileRasterRenderer::drawElements(int indexCount, bool clear) {
     stats_.retries = 0;
-    for (int attempt = 0; attempt < 4; attempt++) {
+    constexpr int kAttempts = 4;
+    for (int attempt = 0; attempt < kAttempts; attempt++) {
         if (!prepare(indexCount)) return;
         submit(clear);
         if (!ok(hipGetLastError(), "kernel launch")) return;
@@ -822,6 +845,9 @@ void TileRasterRenderer::drawElements(int indexCount, bool clear) {
             setError("frame exceeded the binning buffers while drawing without clear");
             break;
         }
+        if (attempt == kAttempts - 1)
+            setError("frame still exceeded the binning buffers after growing them three times; "
+                     "the outputs are incomplete");
     }
     copyOutToHost();
 }
@@ -834,7 +860,10 @@ void TileRasterRenderer::drawAsync(int indexCount, bool clear) {
 
 void TileRasterRenderer::synchronize() {
     if (!stream_) return;
-    finish(true, curTris_ * 3);
+    // an asynchronous draw is not re-run: an overflow leaves an incomplete frame and is reported
+    // (the buffers have grown, so the next draw fits)
+    if (!finish(true, curTris_ * 3) && error_.empty())
+        setError("frame exceeded the binning buffers (they have grown for the next draw); the outputs are incomplete");
 }
 
 void TileRasterRenderer::copyOutToHost() {

@@ -228,6 +228,7 @@ private:
     int rasterGrid() const;
     std::vector<TextureSlot> textures_;
     bool texDescDirty_ = true;
+    uint8_t* texPool_ = nullptr;  // lowest texture allocation: TexDesc bases are byte offsets from it
 
     // neural material state
     DevBuf nnParams_, nnGrads_, nnM_, nnV_, nnPacked_, nnQinfo_, lat32_, latGrad_, latM_, latV_, lat16_;

@@ -94,7 +94,7 @@ class AutotuneReport(C.Structure):
 EXPORTED_SYMBOLS = [
     "iftrCreateInstance", "iftrDestroyInstance", "iftrBindFrameBuffer", "iftrBindVertexBuffer", "iftrBindIndexBuffer",
     "iftrBindVertexShaderFunc", "iftrBindFragmentShaderFunc", "iftrSetBlendFunc", "iftrSetDepthFunc",
-    "iftrOptsetForceDeterministic", "iftrOptSetDepthTestEnable", "iftrDrawLegacy", "iftrClear", "iftrInit", "iftrTest",
+    "iftrOptsetForceDeterministic", "iftrOptSetDepthTestEnable", "iftrOptsetDepthTestEnable", "iftrDrawLegacy", "iftrClear", "iftrInit", "iftrTest",
     "iftrBindVertexShader", "iftrBindFragmentShader", "iftrCreateFrameBuffer", "iftrDestroyFrameBuffer",
     "iftrSetFrameBufferColorAttachmentFP32", "iftrSetFrameBufferDepthAttachmentFP32", "iftrCreateVaryingDescriptor",
     "iftrDestroyVaryingDescriptor", "iftrWriteVaryingDescriptor", "iftrCreateVertexBuffer", "iftrDestroyVertexBuffer",
@@ -350,6 +350,9 @@ class Renderer:
     def synchronize(self):
         self.L.iftrSynchronize(self.h)
         self._check()
+        st = self.stats()
+        if st["status"]:
+            raise IfritError(f"frame incomplete, status bits {st['status']:#x}")
 
     def stage_timing(self, on: bool):
         self.L.iftrSetStageTiming(self.h, 1 if on else 0)

@@ -90,6 +90,10 @@ void iftrSetBlendFunc(IfritRendererHandle* h, IfritColorAttachmentBlendState* st
 void iftrSetDepthFunc(IfritRendererHandle* h, int32_t state);                       /* EXH:54-56 */
 void iftrOptsetForceDeterministic(IfritRendererHandle* h, int opt);                 /* EXH:57-58 */
 void iftrOptSetDepthTestEnable(IfritRendererHandle* h, int opt);                    /* EXH:59-60 */
+/* The same entry under the name the reference's C# binding imports
+   (bindings/csharp/Ifrit-v2-csharp/IfritLib/Native/NativeMethods.Engine.cs:103, EntryPoint
+   "iftrOptsetDepthTestEnable"), so that binding resolves against this library unchanged. */
+void iftrOptsetDepthTestEnable(IfritRendererHandle* h, int opt);
 void iftrDrawLegacy(IfritRendererHandle* h, int numVertices, int clearFramebuffer); /* EXH:61-63 */
 void iftrClear(IfritRendererHandle* h);                                             /* EXH:64-65 */
 void iftrInit(IfritRendererHandle* h);                                              /* EXH:66-67 */

@@ -20,6 +20,15 @@
 
 namespace {
 
+// Planted mistake (0 = none), set only by the closed-form tests through
+// oracle_set_plant to show that each of them fails on a plausible error:
+//   1 Phong reflection vector instead of the half vector,
+//   2 point-light attenuation 1 / (1 + k d) instead of 1 / (1 + k d^2),
+//   3 view vector flipped (P - camera),
+//   4 resolve divides by the covered samples instead of S,
+//   5 resolve ignores the sample counts (each distinct triangle once).
+int g_plant = 0;
+
 struct D3 {
     double x, y, z;
 };
@@ -262,7 +271,8 @@ struct Shader {
                               d[1] - c[1], alb);
         }
         N = normalize(N);
-        const D3 V = normalize(D3{p->camera[0], p->camera[1], p->camera[2]} - P);
+        const D3 cam{p->camera[0], p->camera[1], p->camera[2]};
+        const D3 V = normalize(g_plant == 3 ? P - cam : cam - P);
         D3 col = D3{alb[0], alb[1], alb[2]} * p->ambient;
         for (int l = 0; l < p->num_lights; l++) {
             const float* lp = p->light_pos[l];
@@ -275,13 +285,13 @@ struct Shader {
                 const D3 Lv = D3{lp[0], lp[1], lp[2]} - P;
                 const double d2 = dot(Lv, Lv);
                 L = Lv * (1.0 / std::sqrt(d2));
-                att = 1.0 / (1.0 + lc[3] * d2);
+                att = 1.0 / (1.0 + lc[3] * (g_plant == 2 ? std::sqrt(d2) : d2));
             }
             const double ndl = dot(N, L);
             if (std::fabs(ndl) < 1e-4) flag = true;
             if (ndl > 0) {
-                const D3 Hh = normalize(L + V);
-                const double ndh = std::max(dot(N, Hh), 0.0);
+                const D3 Hh = g_plant == 1 ? N * (2.0 * ndl) - L : normalize(L + V);
+                const double ndh = std::max(dot(g_plant == 1 ? V : N, Hh), 0.0);
                 const double spec = p->ks * std::pow(ndh, (double)p->shininess);
                 col.x += att * lc[0] * (p->kd * ndl * alb[0] + spec);
                 col.y += att * lc[1] * (p->kd * ndl * alb[1] + spec);
@@ -304,6 +314,8 @@ inline uint8_t to_u8(double c) {
 }  // namespace
 
 extern "C" {
+
+void oracle_set_plant(int plant) { g_plant = plant; }
 
 // One light of the layered-GGX material (ggx_light above), for the closed-form tests.
 void oracle_ggx_light(const double* n, const double* v, const double* l, const double* alb, double alpha, double kd,
@@ -391,6 +403,7 @@ int oracle_shade(const OracleFrame* f, const OracleShadeParams* p, int W, int H,
                 const uint32_t* ids = tri_ids + pix * S;
                 double sum[4] = {0, 0, 0, 0};
                 bool flag = false;
+                int covered = 0;
                 for (int s = 0; s < S; s++) {
                     if (ids[s] == 0xffffffffu) continue;
                     int first = s;
@@ -399,11 +412,14 @@ int oracle_shade(const OracleFrame* f, const OracleShadeParams* p, int W, int H,
                     if (first != s) continue;  // shade each distinct triangle once
                     int count = 0;
                     for (int q = s; q < S; q++) count += ids[q] == ids[s];
+                    covered += count;
+                    if (g_plant == 5) count = 1;
                     double c[4];
                     flag |= sh.shade(ids[s], x, row, o, c);
                     for (int k = 0; k < 4; k++) sum[k] += std::min(std::max(c[k], 0.0), 1.0) * count;
                 }
-                for (int k = 0; k < 4; k++) rgba8_out[4 * pix + k] = to_u8(sum[k] / S);
+                const int div = g_plant == 4 ? std::max(1, covered) : S;
+                for (int k = 0; k < 4; k++) rgba8_out[4 * pix + k] = to_u8(sum[k] / div);
                 if (flag_out) flag_out[pix] = flag;
                 if (flag) flagged++;
             }

@@ -61,6 +61,10 @@ int oracle_shade(const OracleFrame* f, const OracleShadeParams* p, int W, int H,
                  const int32_t* indices, const float* vertices, const OracleTexture* textures, int ntex,
                  uint8_t* rgba8_out, uint8_t* flag_out, int num_threads);
 
+// Test hook: plants a known mistake in the shading oracle (0 restores it);
+// see g_plant in oracle_shade.cpp.  Used only by the closed-form tests.
+void oracle_set_plant(int plant);
+
 // Box-filter mip generation, (a + b + c + d + 2) >> 2 per channel.
 void oracle_build_mips(uint8_t* texels, int width, int height, int levels);
 
