@@ -29,6 +29,7 @@ SOURCES = [
     "host/renderer.cpp",
     "host/neural.cpp",
     "host/capi.cpp",
+    "host/noalloc.hip",
 ]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch={ARCH}",

@@ -231,6 +231,18 @@ void iftrDestroyGpuProgram(void* p) {
     else if (tag == kFragmentProgramTag)
         delete static_cast<GpuFragmentProgram*>(p);
 }
+void iftrSetClearValues(IfritRendererHandle* h, const IfritVector4f* colors, size_t n, float depth) {
+    guarded(h, [&](TileRasterRenderer& r) { r.setClearValues(colors, n, depth); });
+}
+void iftrSetScissors(IfritRendererHandle* h, const IfritVector4f* rects, size_t n) {
+    guarded(h, [&](TileRasterRenderer& r) { r.SetScissors(rects, n); });
+}
+void iftrSetScissorTestEnable(IfritRendererHandle* h, int enable) {
+    guarded(h, [&](TileRasterRenderer& r) { r.setScissorTestEnable(enable != 0); });
+}
+void iftrSetCullMode(IfritRendererHandle* h, int mode) {
+    guarded(h, [&](TileRasterRenderer& r) { r.SetCullMode(mode); });
+}
 void iftrSetMsaaSamples(IfritRendererHandle* h, int s) {
     guarded(h, [&](TileRasterRenderer& r) { r.setMsaaSamples(s); });
 }
@@ -303,6 +315,10 @@ float iftrNeuralValidate(IfritRendererHandle* h, int precision, float minPsnrDb,
     guarded(h, [&](TileRasterRenderer& rr) { r = rr.validateNeural(precision, minPsnrDb, numIndices); });
     return r;
 }
+void iftrNeuralValidateEx(IfritRendererHandle* h, int precision, float minPsnrDb, int maxErrLsb, int numIndices,
+                          IfritNeuralValidation* out) {
+    guarded(h, [&](TileRasterRenderer& r) { r.validateNeuralEx(precision, minPsnrDb, maxErrLsb, numIndices, out); });
+}
 void iftrNeuralEval(IfritRendererHandle* h, const float* in, float* out, int n, int precision) {
     if (!in || !out) return;
     guarded(h, [&](TileRasterRenderer& r) { r.evalNeural(in, out, n, precision); });
@@ -334,6 +350,10 @@ void iftrGetTuning(IfritRendererHandle* h, IfritTuning* t) {
 }
 void iftrAutotune(IfritRendererHandle* h, int numIndices, int frames, IfritAutotuneReport* report) {
     guarded(h, [&](TileRasterRenderer& r) { r.autotune(numIndices, frames, report); });
+}
+void iftrDrawNoAlloc(IfritRendererHandle* h, const IfritNoAllocDraw* draw) {
+    if (!draw) return;
+    guarded(h, [&](TileRasterRenderer& r) { r.drawNoAlloc(*draw); });
 }
 uint64_t iftrGetDigest(IfritRendererHandle* h) {
     uint64_t d = 0;

@@ -319,18 +319,27 @@ void TileRasterRenderer::evalNeural(const float* in, float* out, int n, int prec
 }
 
 float TileRasterRenderer::validateNeural(int precision, float minPsnrDb, int indexCount) {
+    IfritNeuralValidation v{};
+    validateNeuralEx(precision, minPsnrDb, 255, indexCount, &v);
+    return v.psnrDb;
+}
+
+void TileRasterRenderer::validateNeuralEx(int precision, float minPsnrDb, int maxErrLsb, int indexCount,
+                                          IfritNeuralValidation* out) {
+    IfritNeuralValidation res{};
+    if (out) *out = res;
     if (!nnReady_) {
         setError("the neural material has not been trained or set");
-        return 0.0f;
+        return;
     }
     if (!fsBound_ || (fs_.desc.kind != IFHIP_FS_NEURAL && fs_.desc.kind != IFHIP_FS_GGX_LAYERED)) {
         setError("validation needs the layered-GGX or neural program bound");
-        return 0.0f;
+        return;
     }
     const int p = precision == 1 ? 1 : 0;
     if (!batch_.empty()) {
         setError("validation renders the single bound target: clear the view batch first");
-        return 0.0f;
+        return;
     }
     const IfritGpuFragmentProgramDesc saved = fs_.desc;
     FrameBuffer* fbSaved = fb_;
@@ -359,21 +368,45 @@ float TileRasterRenderer::validateNeural(int precision, float minPsnrDb, int ind
     nnFallback_[p] = fb;
     fs_.desc = saved;
     fb_ = fbSaved;
-    if (!error_.empty()) return 0.0f;
+    if (!error_.empty()) return;
+    // squared error over the RGB of covered pixels, and the distribution of each pixel's
+    // largest channel error (in RGBA8 steps)
     double se = 0.0;
     size_t n = 0;
+    unsigned long long hist[256] = {};
     for (size_t i = 0; i < px; i++) {
         if (slot0[i] == 0xffffffffu) continue;
+        int worst = 0;
         for (int c = 0; c < 3; c++) {
-            const double d = ((double)neural[4 * i + c] - (double)plain[4 * i + c]) / 255.0;
+            const int e = std::abs((int)neural[4 * i + c] - (int)plain[4 * i + c]);
+            worst = std::max(worst, e);
+            const double d = e / 255.0;
             se += d * d;
         }
+        hist[worst]++;
         n += 3;
     }
     const double mse = n ? se / (double)n : 0.0;
-    const float psnr = (float)(mse > 0 ? 10.0 * std::log10(1.0 / mse) : 120.0);
-    nnFallback_[p] = psnr < minPsnrDb;
-    return psnr;
+    res.psnrDb = (float)(mse > 0 ? 10.0 * std::log10(1.0 / mse) : 120.0);
+    res.pixels = (int64_t)(n / 3);
+    for (int e = 255; e >= 0; e--)
+        if (hist[e]) {
+            res.maxErrLsb = e;
+            break;
+        }
+    // the smallest error that 99.9 % of the covered pixels stay within
+    unsigned long long acc = 0;
+    const unsigned long long want = (unsigned long long)std::ceil(0.999 * (double)res.pixels);
+    for (int e = 0; e < 256; e++) {
+        acc += hist[e];
+        if (acc >= want) {
+            res.p999ErrLsb = e;
+            break;
+        }
+    }
+    res.fallback = (res.psnrDb < minPsnrDb || res.maxErrLsb > maxErrLsb) ? 1 : 0;
+    nnFallback_[p] = res.fallback != 0;
+    if (out) *out = res;
 }
 
 }  // namespace ifrit_hip

@@ -58,7 +58,7 @@ TileRasterRenderer::~TileRasterRenderer() {
                       &bigList_, &ctr_, &clipList_, &statPart_, &items_, &tilePartial_, &multiList_, &partials_, &visId_, &visDepth_, &visSlot_, &visClass_, &rgba8_, &rgba32f_, &texDesc_, &nnParams_, &nnGrads_, &nnM_, &nnV_,
                       &nnPacked_, &nnQinfo_, &lat32_, &latGrad_, &latM_, &latV_, &lat16_, &nnData_, &nnFeat_, &nnCount_,
                       &nnLoss_, &nnEvalIn_, &nnEvalOut_, &nnGradPart_, &viewDev_, &tileView_, &shadeView_,
-                      &groupBounds_, &groupOut_,
+                      &groupBounds_, &groupOut_, &swapped_, &scissorDev_,
                       &viewCtr_, &digest_, &planSums_})
         b->release();
     for (auto& t : textures_)
@@ -97,6 +97,8 @@ void TileRasterRenderer::Init() {
     for (auto& e : ev_)
         if (!ok(hipEventCreate(&e), "hipEventCreate")) return;
     if (!ctr_.ensure(sizeof(DeviceCounters))) setError("hipMalloc counters");
+    std::string err;
+    if (!noalloc_register_device(&err)) setError(err);
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
@@ -232,6 +234,46 @@ void TileRasterRenderer::optSetDepthTestEnable(bool opt) {
     depthFunc_ = opt ? depthFuncSaved_ : IF_COMPARE_OP_ALWAYS;
 }
 
+void TileRasterRenderer::setClearValues(const IfritVector4f* colors, size_t n, float depth) {
+    if (std::isnan(depth)) return setError("the clear depth must not be NaN");
+    // one colour attachment: the first clear colour applies, the others are kept by the caller
+    float c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (colors && n > 0) {
+        c[0] = colors[0].x;
+        c[1] = colors[0].y;
+        c[2] = colors[0].z;
+        c[3] = colors[0].w;
+    }
+    std::memcpy(clearColor_, c, sizeof(c));
+    clearDepth_ = depth;
+}
+
+void TileRasterRenderer::SetScissors(const IfritVector4f* rects, size_t n) {
+    if (n > (size_t)kMaxScissors) return setError("at most 8 scissor rectangles");
+    if (n && !rects) return setError("null scissor rectangles");
+    Init();
+    if (!initialized_) return;
+    // the kernels read the rectangles from device memory; a draw in flight may still read them
+    if (!scissorDev_.ensure(kMaxScissors * sizeof(float4))) return setError("hipMalloc scissors");
+    if (!ok(hipStreamSynchronize(stream_), "scissors") ||
+        (n && !ok(hipMemcpy(scissorDev_.ptr, rects, n * sizeof(float4), hipMemcpyHostToDevice), "scissors")))
+        return;
+    scissors_.assign(rects, rects + n);
+}
+
+Scissor TileRasterRenderer::activeScissor() const {
+    // the test off, or no rectangle, passes every pixel (TileRasterCoreInvocationCuda.cu:3688-3693)
+    Scissor sc{};
+    sc.count = scissorTest_ && scissorDev_.ptr ? (int)scissors_.size() : 0;
+    sc.rects = scissorDev_.as<float4>();
+    return sc;
+}
+
+void TileRasterRenderer::SetCullMode(int mode) {
+    if (mode < kCullNone || mode > kCullFrontAndBack) return setError("unknown cull mode");
+    cullMode_ = mode;
+}
+
 void TileRasterRenderer::setMsaaSamples(int s) {
     if (s != 1 && s != 4 && s != 8) {
         setError("MSAA sample count must be 1, 4 or 8");
@@ -350,7 +392,7 @@ bool TileRasterRenderer::prepare(int indexCount) {
     const int fk = fs_.desc.kind;
     if (blend_.blendEnable && (samples_ != 1 || !batch_.empty() || !fsBound_ ||
                                (fk != IFHIP_FS_FLAT && fk != IFHIP_FS_FLAT_ALPHA && fk != IFHIP_FS_BLINN_PHONG &&
-                                fk != IFHIP_FS_GGX_LAYERED))) {
+                                fk != IFHIP_FS_GGX_LAYERED && fk != IFHIP_FS_NORMAL_LAMBERT))) {
         setError("blending draws one target at 1 sample per pixel with the flat, translucent flat, Blinn-Phong or "
                  "layered-GGX program");
         return false;
@@ -449,6 +491,7 @@ bool TileRasterRenderer::prepare(int indexCount) {
                  recs_[1].ensure(slots * 16) && recs_[2].ensure(slots * 16) && recs_[3].ensure(slots * 16) &&
                  recs_[4].ensure(slots * 16) && recs_[5].ensure(slots * 16) && bary_.ensure(slots * 32) &&
                  binQueue_.ensure(slots * 16) && slotOf_.ensure((size_t)std::max(1, ntri) * 4 * nv) &&
+                 (!needSwapFlags() || swapped_.ensure(slots)) &&
                  clipList_.ensure((size_t)clipCap_ * 4 * nv) &&
                  statPart_.ensure(((size_t)(ntri + 255) / 256 * nv + 2 * (size_t)rasterGrid() + 2) * 8) &&
                  items_.ensure((size_t)itemCap_ * 16) && tilePartial_.ensure(tiles * 4) &&
@@ -628,7 +671,8 @@ ShadeParams TileRasterRenderer::shadeParams(bool writeFloat) const {
     sh.trisPerMaterial = d.trisPerMaterial;
     sh.sampler = d.sampler;
     sh.maxAniso = std::min(16, std::max(1, d.maxAniso));  // the tap list encodes up to 16 taps
-    sh.numTextures = (int)textures_.size();
+    // the normal-Lambert program samples no texture
+    sh.numTextures = d.kind == IFHIP_FS_NORMAL_LAMBERT ? 0 : (int)textures_.size();
     sh.ambient = d.ambient;
     sh.kd = d.kd;
     sh.ks = d.ks;
@@ -640,6 +684,10 @@ ShadeParams TileRasterRenderer::shadeParams(bool writeFloat) const {
     sh.ntri = curTris_;
     sh.shadeTile = shadeTile_;
     sh.neural = neuralShadeParams();
+    sh.swapped = needSwapFlags() ? swapped_.as<unsigned char>() : nullptr;
+    std::memcpy(sh.clearColor, clearColor_, sizeof(sh.clearColor));
+    sh.clearDepth = clearDepth_;
+    sh.scissor = activeScissor();
     return sh;
 }
 
@@ -665,7 +713,8 @@ void TileRasterRenderer::enqueue(bool clear) {
     SetupParams sp{};
     sp.ntri = curTris_;
     sp.nverts = nverts_;
-    sp.ccw = 0;
+    sp.cullMode = cullMode_;
+    sp.swapped = needSwapFlags() ? swapped_.as<unsigned char>() : nullptr;
     sp.msaa = samples_ > 1 ? 1 : 0;
     sp.clipCap = clipCap_;
     sp.invTileWidth = ifx::rcp_nr_ss(16.0f);
@@ -688,6 +737,8 @@ void TileRasterRenderer::enqueue(bool clear) {
     RasterParams rp{};
     rp.depthFunc = depthFunc_;
     rp.clear = clear ? 1 : 0;
+    rp.clearDepth = clearDepth_;
+    rp.scissor = activeScissor();
     rp.binCap = binCap_;
     const float* st = samples_ == 8 ? kS8 : (samples_ == 4 ? kS4 : kS1);
     for (int i = 0; i < samples_; i++) {
@@ -744,12 +795,16 @@ std::vector<uint8_t> TileRasterRenderer::graphKey(bool clear) const {
                         partShift_, rasterGroupsPerCU_, shadeTile_, bigThreshold_, nnFallback_[0] ? 1 : 0,
                         nnFallback_[1] ? 1 : 0, (int)textures_.size(), stageTiming_ ? 1 : 0,
                         fb_ && !fb_->color.empty() && fb_->color[0] ? 1 : 0, batch_.empty() ? 1 : 0,
-                        floatOut_ ? 1 : 0, blend_.blendEnable ? 1 : 0};
+                        floatOut_ ? 1 : 0, blend_.blendEnable ? 1 : 0, cullMode_};
     put(ints, sizeof(ints));
     const unsigned caps[] = {clipCap_, binCap_, bigCap_, itemCap_, partialCap_};
     put(caps, sizeof(caps));
     put(&totalPixels_, sizeof(totalPixels_));
     put(&blendCoefs_, sizeof(blendCoefs_));
+    put(clearColor_, sizeof(clearColor_));
+    put(&clearDepth_, sizeof(clearDepth_));
+    const Scissor sc = activeScissor();
+    put(&sc, sizeof(sc));
     put(&DevBuf::generation, sizeof(DevBuf::generation));
     IfritGpuFragmentProgramDesc fd = fs_.desc;
     std::memset(fd.camera, 0, sizeof(fd.camera));  // cameras live in the view table

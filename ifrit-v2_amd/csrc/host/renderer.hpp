@@ -87,6 +87,9 @@ struct DevBuf {
     }
 };
 
+// Looks up the no-allocation scratch array on the current device (noalloc.hip).
+bool noalloc_register_device(std::string* err);
+
 class TileRasterRenderer {
 public:
     TileRasterRenderer();
@@ -105,6 +108,10 @@ public:
     void SetDepthFunc(int func);
     void optsetForceDeterministic(bool opt) { deterministic_ = opt; }
     void optSetDepthTestEnable(bool opt);
+    void SetCullMode(int mode);  // TileRasterRendererCuda.h:109
+    void setClearValues(const IfritVector4f* colors, size_t n, float depth);  // TileRasterRendererCuda.h:111
+    void SetScissors(const IfritVector4f* rects, size_t n);                    // TileRasterRendererCuda.h:101
+    void setScissorTestEnable(bool on) { scissorTest_ = on; }                  // TileRasterRendererCuda.h:102
     void drawElements(int indexCount, bool clear);
     void clear();
 
@@ -142,12 +149,16 @@ public:
     void autotune(int indexCount, int frames, IfritAutotuneReport* out);
     uint64_t digest();
 
+    // the draw with every intermediate buffer in the static scratch array (noalloc.hip)
+    void drawNoAlloc(const IfritNoAllocDraw& d);
+
     // neural material (neural.cpp)
     void trainNeural(const IfritNeuralTrainDesc& d, IfritNeuralTrainReport* report);
     void setNeuralParams(const float* params);
     void getNeuralParams(float* params);
     void evalNeural(const float* in, float* out, int n, int precision);
     float validateNeural(int precision, float minPsnrDb, int indexCount);
+    void validateNeuralEx(int precision, float minPsnrDb, int maxErrLsb, int indexCount, IfritNeuralValidation* out);
 
 private:
     bool prepare(int indexCount);
@@ -168,6 +179,14 @@ private:
     int depthFunc_ = IF_COMPARE_OP_LESS;
     int depthFuncSaved_ = IF_COMPARE_OP_LESS;
     bool depthTest_ = true;
+    int cullMode_ = kCullBack;  // the CPU path's fixed back-face cull
+    float clearColor_[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // TileRasterWorker.cpp:113-118
+    float clearDepth_ = 3.3961514e38f;                  // 0x7f7f7f7f, TileRasterWorker.cpp:40
+    std::vector<IfritVector4f> scissors_;  // x, y, width, height in pixels
+    DevBuf scissorDev_;                    // the same on the device, read by the raster and shading kernels
+    bool scissorTest_ = false;
+    Scissor activeScissor() const;
+    bool needSwapFlags() const { return cullMode_ == kCullNone || cullMode_ == kCullFront; }
     IfritColorAttachmentBlendState blend_{};
     BlendCoefs blendCoefs_{};
     bool floatOut_ = false;  // shading also writes RGBA32F
@@ -219,6 +238,7 @@ private:
     // device buffers
     DevBuf vtx_, idx_, clip_, recs_[6], bary_, binQueue_, slotOf_, binCount_, binStart_, binCursor_, binList_, bigList_, ctr_;
     DevBuf groupBounds_, groupOut_;  // per triangle group: bounds; per view and group: outside flag
+    DevBuf swapped_;                 // per record slot: set up with vertices 0 and 2 exchanged
     DevBuf clipList_, statPart_, items_, tilePartial_, multiList_, partials_, visId_, visDepth_, visSlot_, visClass_, rgba8_, rgba32f_, texDesc_;
     unsigned clipCap_ = 0, binCap_ = 0, bigCap_ = 0, itemCap_ = 0, partialCap_ = 0;
     int numCUs_ = 256;

@@ -80,13 +80,31 @@ struct Views {
     unsigned slotsPerView;            // SV = T + clipCap
 };
 
+// Scissor rectangles (SetScissors / setScissorTestEnable of the CUDA renderer,
+// TileRasterRendererCuda.h:101-102): a pixel (x, y) of a view passes when
+// count is 0 or it lies in one rectangle, r.x <= x <= r.x + r.z and
+// r.y <= y <= r.y + r.w (TileRasterCoreInvocationCuda.cu:1699-1711).  A
+// draw leaves pixels that fail untouched: visibility, depth and colour
+// (its clear included, as the CUDA clear kernels are scissored,
+// TileRasterCoreInvocationCuda.cu:3336-3424).
+constexpr int kMaxScissors = 8;
+struct Scissor {
+    int count;            // 0: every pixel passes
+    const float4* rects;  // device memory: (x, y, width, height) per rectangle
+};
+
+// IfritCullMode (include/ifrit/softgraphics/engine/base/Constants.h:29-35).
+enum CullMode { kCullNone = 0, kCullFront = 1, kCullBack = 2, kCullFrontAndBack = 3 };
+
 // Per-view fields (width, height, tiles, invW, invH) are filled in by the
 // kernels from the view's ViewDesc.
 struct SetupParams {
     int width, height;
     int tilesX, tilesY;
     int ntri, nverts;
-    int ccw;
+    int cullMode;       // IfritCullMode: 0 none, 1 front, 2 back (the CPU path), 3 front and back
+    unsigned char* swapped;  // per record slot, cull modes none and front: 1 when the triangle was set
+                             // up with its first and third vertex exchanged (null for back)
     int msaa;           // 1 disables the small-triangle cull
     unsigned clipCap;   // capacity for clip fan triangles and for triangles to clip
     float invW, invH;   // 1 / width, 1 / height (TileRasterRenderer.cpp:59-60)
@@ -113,6 +131,8 @@ struct BinParams {
 struct RasterParams {
     int depthFunc;
     int clear;
+    float clearDepth;    // depth a cleared sample starts from (0x7f7f7f7f in the CPU path)
+    Scissor scissor;
     unsigned binCap;
     float sx[8], sy[8];  // sample offsets from the pixel corner
 };
@@ -140,6 +160,10 @@ struct ShadeParams {
     int shadeTile;           // pixels per side of a shading tile (8, 16 or 32): one wave shades
                              // its 8x8 sub-tiles in turn
     NeuralShadeParams neural;  // programs 4 (neural material) and 5 (training features)
+    const unsigned char* swapped;  // per record slot: vertices 0 and 2 exchanged by setup (null: none)
+    float clearColor[4];     // colour of samples no triangle covers (the resolve weighs them)
+    float clearDepth;        // depth a cleared pixel starts from (blending draws)
+    Scissor scissor;
 };
 
 // Blend factors, each encoded (c0, c1, c2, c3) as the reference does

@@ -251,12 +251,12 @@ template <int S>
 __device__ __forceinline__ void write_pixel(const RasterParams& P, const TriRecs& recs, const unsigned* slotOf,
                                             const PixelCtx& C, const unsigned long long* keys, float* visDepth,
                                             uint32_t* visSlot, uint8_t* visClass) {
-    if (!C.inFb) return;
+    if (!C.inFb || !scissor_pass(P.scissor, C.fpx, C.fpy)) return;
     unsigned long long prev[S];
 #pragma unroll
-    for (int s = 0; s < S; s++) prev[s] = frag_key(P.clear ? kClearDepth : visDepth[C.pix * S + s], 0u);
+    for (int s = 0; s < S; s++) prev[s] = frag_key(P.clear ? P.clearDepth : visDepth[C.pix * S + s], 0u);
     unsigned slots[S];
-    key_slots<S>(slotOf, keys, P.clear ? frag_key(kClearDepth, 0u) : ~0ull, slots);
+    key_slots<S>(slotOf, keys, P.clear ? frag_key(P.clearDepth, 0u) : ~0ull, slots);
     unsigned cls = 0;
     bool won = false;
 #pragma unroll
@@ -264,7 +264,7 @@ __device__ __forceinline__ void write_pixel(const RasterParams& P, const TriRecs
         const unsigned long long k = keys[s];
         if (k == prev[s]) {
             if (P.clear) {
-                visDepth[C.pix * S + s] = kClearDepth;
+                visDepth[C.pix * S + s] = P.clearDepth;
                 visSlot[C.pix * S + s] = 0xffffffffu;
             }
             continue;
@@ -305,13 +305,13 @@ __device__ __forceinline__ unsigned pixel_outputs(const RasterParams& P, const T
                                                   const PixelCtx& C, const unsigned long long* keys, float* d,
                                                   unsigned* slot) {
     unsigned cls = 0;
-    const unsigned long long clearKey = frag_key(kClearDepth, 0u);
+    const unsigned long long clearKey = frag_key(P.clearDepth, 0u);
     unsigned slots[S];
     key_slots<S>(slotOf, keys, clearKey, slots);
 #pragma unroll
     for (int s = 0; s < S; s++) {
         const unsigned long long k = keys[s];
-        d[s] = kClearDepth;
+        d[s] = P.clearDepth;
         slot[s] = 0xffffffffu;
         if (k != clearKey) {
             slot[s] = slots[s];
@@ -336,8 +336,17 @@ __device__ __forceinline__ unsigned pixel_outputs(const RasterParams& P, const T
 template <int S>
 __device__ __forceinline__ void stage_pixel(const RasterParams& P, const TriRecs& recs, const unsigned* slotOf,
                                             const PixelCtx& C, int t, const unsigned long long* keys, TileOut<S>& O,
-                                            uint8_t* visClass) {
+                                            uint8_t* visClass, const float* visDepth, const uint32_t* visSlot) {
     if (!C.inFb) return;
+    if (!scissor_pass(P.scissor, C.fpx, C.fpy)) {
+        // outside the scissor: the tile's line stores write the pixel's current contents back
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            O.d[t * S + s] = visDepth[C.pix * S + s];
+            O.slot[t * S + s] = visSlot[C.pix * S + s];
+        }
+        return;
+    }
     float d[S];
     unsigned slot[S];
     const unsigned cls = pixel_outputs<S>(P, recs, slotOf, C, keys, d, slot);
@@ -357,7 +366,7 @@ __device__ __forceinline__ void emit_pixel(const RasterParams& P, const TriRecs&
                                            const PixelCtx& C, const unsigned long long* keys, float* visDepth,
                                            uint32_t* visSlot) {
     static_assert(S % 4 == 0, "whole 16-byte words per pixel");
-    if (!C.inFb) return;
+    if (!C.inFb || !scissor_pass(P.scissor, C.fpx, C.fpy)) return;
     float d[S];
     unsigned slot[S];
     pixel_outputs<S>(P, recs, slotOf, C, keys, d, slot);
@@ -403,15 +412,16 @@ __device__ __forceinline__ void store_tile(const ViewGeom& G, int tx, int ty, co
     }
 }
 
-// Clear values of one tile whose list is empty (depth kClearDepth, slot
+// Clear values of one tile whose list is empty (depth clearDepth, slot
 // 0xffffffff and, at 1 sample per pixel, class 0) by NT threads, with 16-byte
 // stores where the tile's rows are whole.
 template <int S, int NT>
-__device__ __forceinline__ void clear_tile(const ViewGeom& G, float* visDepthAll, uint32_t* visSlotAll,
-                                           uint8_t* visClassAll) {
+__device__ __forceinline__ void clear_tile(const ViewGeom& G, float clearDepth, const Scissor& sc,
+                                           float* visDepthAll, uint32_t* visSlotAll, uint8_t* visClassAll) {
     const int tx = G.tile % G.tilesX, ty = G.tile / G.tilesX;
     const int cols = min(16, G.width - tx * 16), rows = min(16, G.height - ty * 16);
-    const bool vec = cols == 16 && (S > 1 || ((G.width & 3) == 0 && (G.pixBase & 3) == 0));
+    const bool whole = scissor_all(sc, tx * 16, ty * 16, tx * 16 + cols - 1, ty * 16 + rows - 1);
+    const bool vec = whole && cols == 16 && (S > 1 || ((G.width & 3) == 0 && (G.pixBase & 3) == 0));
     float* const visDepth = visDepthAll + G.pixBase * S;
     uint32_t* const visSlot = visSlotAll + G.pixBase * S;
     constexpr int perRow = 16 * S / 4;  // 16-byte words per tile row
@@ -420,13 +430,15 @@ __device__ __forceinline__ void clear_tile(const ViewGeom& G, float* visDepthAll
         if (row >= rows) continue;
         const size_t g = ((size_t)(ty * 16 + row) * G.width + tx * 16) * S + q * 4;
         if (vec) {
-            *reinterpret_cast<float4*>(&visDepth[g]) = make_float4(kClearDepth, kClearDepth, kClearDepth, kClearDepth);
+            *reinterpret_cast<float4*>(&visDepth[g]) = make_float4(clearDepth, clearDepth, clearDepth, clearDepth);
             *reinterpret_cast<uint4*>(&visSlot[g]) = make_uint4(~0u, ~0u, ~0u, ~0u);
         } else {
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                if ((q * 4 + e) / S >= cols) continue;
-                visDepth[g + e] = kClearDepth;
+                const int col = (q * 4 + e) / S;
+                if (col >= cols) continue;
+                if (!whole && !scissor_pass(sc, (float)(tx * 16 + col), (float)(ty * 16 + row))) continue;
+                visDepth[g + e] = clearDepth;
                 visSlot[g + e] = 0xffffffffu;
             }
         }
@@ -435,7 +447,9 @@ __device__ __forceinline__ void clear_tile(const ViewGeom& G, float* visDepthAll
         uint8_t* const visClass = visClassAll + G.pixBase;
         for (int i = threadIdx.x; i < 256; i += NT) {
             const int row = i >> 4, col = i & 15;
-            if (row < rows && col < cols) visClass[(size_t)(ty * 16 + row) * G.width + tx * 16 + col] = 0;
+            if (row < rows && col < cols &&
+                (whole || scissor_pass(sc, (float)(tx * 16 + col), (float)(ty * 16 + row))))
+                visClass[(size_t)(ty * 16 + row) * G.width + tx * 16 + col] = 0;
         }
     }
 }
@@ -494,7 +508,8 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
             if (e >= numEmptyItems) break;
             if (t == 0) claimed = __hip_atomic_fetch_add(head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             for (unsigned k = e * KE; k < min(e * KE + KE, numEmpty); k++)
-                clear_tile<S, NT>(view_of_tile(V, (int)emptyList[k]), visDepthAll, visSlotAll, visClassAll);
+                clear_tile<S, NT>(view_of_tile(V, (int)emptyList[k]), P.clearDepth, P.scissor, visDepthAll,
+                                  visSlotAll, visClassAll);
             __syncthreads();  // every lane has read sItem
             continue;
         }
@@ -518,7 +533,7 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
             const PixelCtx C = pixel_ctx(G, p & 15, p >> 4);
 #pragma unroll
             for (int s = 0; s < S; s++)
-                sVis[s * 256 + p] = frag_key(P.clear || !C.inFb ? kClearDepth : visDepth[C.pix * S + s], 0u);
+                sVis[s * 256 + p] = frag_key(P.clear || !C.inFb ? P.clearDepth : visDepth[C.pix * S + s], 0u);
         }
 
         // two lanes per list entry: lane h = t & 1 takes block rows 2h, 2h + 1
@@ -714,7 +729,8 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
 #pragma unroll
                     for (int k = 0; k < PPT; k++) {
                         const int p = t + k * NT;
-                        stage_pixel<S>(P, recs, slotOf, pixel_ctx(G, p & 15, p >> 4), p, keys[k], *O, visClass);
+                        stage_pixel<S>(P, recs, slotOf, pixel_ctx(G, p & 15, p >> 4), p, keys[k], *O, visClass,
+                                       visDepth, visSlot);
                     }
                     __syncthreads();
                     store_tile<S, NT>(G, ttx, tty, O, visDepth, visSlot);
@@ -780,7 +796,7 @@ __global__ void __launch_bounds__(256) k_raster_merge(RasterParams P, Views V, T
             } else {
                 __shared__ TileOut<S> O;  // output staging at 1 sample per pixel
                 __syncthreads();          // the previous tile's staging has been stored
-                stage_pixel<S>(P, recs, slotOf, C, t, keys, O, visClass);
+                stage_pixel<S>(P, recs, slotOf, C, t, keys, O, visClass, visDepth, visSlot);
                 __syncthreads();
                 store_tile<S>(G, C.tx, C.ty, &O, visDepth, visSlot);
             }
@@ -823,7 +839,7 @@ __global__ void __launch_bounds__(256) k_raster_ordered(RasterParams P, Views V,
         unsigned cls = 0;
 #pragma unroll
         for (int s = 0; s < S; s++) {
-            dep[s] = P.clear || !C.inFb ? kClearDepth : visDepth[C.pix * S + s];
+            dep[s] = P.clear || !C.inFb ? P.clearDepth : visDepth[C.pix * S + s];
             slot[s] = P.clear || !C.inFb ? 0xffffffffu : visSlot[C.pix * S + s];
         }
         if (!P.clear && S == 1 && visClass && C.inFb) cls = visClass[C.pix];
@@ -863,7 +879,7 @@ __global__ void __launch_bounds__(256) k_raster_ordered(RasterParams P, Views V,
                 frags += any;
             }
         });
-        if (C.inFb) {
+        if (C.inFb && scissor_pass(P.scissor, C.fpx, C.fpy)) {
 #pragma unroll
             for (int s = 0; s < S; s++) {
                 visDepth[C.pix * S + s] = dep[s];

@@ -36,6 +36,27 @@ __device__ __forceinline__ ViewGeom view_of_tile(const Views& V, int gtile) {
     return G;
 }
 
+// The rectangles are read from device memory at a uniform address (scalar
+// loads), so that the test holds no vector registers.
+__device__ __forceinline__ bool scissor_pass(const Scissor& sc, float x, float y) {
+    if (sc.count == 0) return true;
+    for (int i = 0; i < sc.count; i++) {
+        const float4 r = sc.rects[i];
+        if (x >= r.x && x <= r.x + r.z && y >= r.y && y <= r.y + r.w) return true;
+    }
+    return false;
+}
+
+// Every pixel of the rectangle [x0, x1] x [y0, y1] passes (one scissor holds it whole).
+__device__ __forceinline__ bool scissor_all(const Scissor& sc, int x0, int y0, int x1, int y1) {
+    if (sc.count == 0) return true;
+    for (int i = 0; i < sc.count; i++) {
+        const float4 r = sc.rects[i];
+        if ((float)x0 >= r.x && (float)x1 <= r.x + r.z && (float)y0 >= r.y && (float)y1 <= r.y + r.w) return true;
+    }
+    return false;
+}
+
 struct PixelCtx {
     int tx, ty, px, py;
     bool inFb;

@@ -32,15 +32,17 @@ struct ShadeTri {
     float4 f1, f2, f3;
     float bx[3], by[3];
     int prim;
+    int swapped;  // set up with its first and third vertex exchanged (cull modes none and front)
 };
 
 // Slots below fanStart (the view's first clip fan slot) are unclipped triangles.
 __device__ __forceinline__ void load_shade_tri(const TriRecs& recs, const float4* bary, unsigned fanStart,
-                                               unsigned slot, ShadeTri& T) {
+                                               unsigned slot, ShadeTri& T, const unsigned char* swapped) {
     T.f1 = recs.f1[slot];
     T.f2 = recs.f2[slot];
     T.f3 = recs.f3[slot];
     T.prim = (int)(__float_as_uint(recs.q2[slot].y) >> 2);
+    T.swapped = swapped ? swapped[slot] : 0;
     if (slot < fanStart) {
         // unclipped triangles keep bx = (0, 0, 1), by = (1, 0, 0) (TileRasterWorker.cpp:711-712)
         T.bx[0] = 0.0f; T.bx[1] = 0.0f; T.bx[2] = 1.0f;
@@ -294,9 +296,11 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, float3 cam, const ShadeT
         return R;
     }
     const float x = (float)px + o, y = (float)py + o;
-    const float* v0 = vtx + (size_t)idx[3 * T.prim] * P.stride;
+    // the planes weigh the vertices in the order setup took them: exchanged triangles
+    // fetch their first and third vertex the other way round
+    const float* v0 = vtx + (size_t)idx[3 * T.prim + (T.swapped ? 2 : 0)] * P.stride;
     const float* v1 = vtx + (size_t)idx[3 * T.prim + 1] * P.stride;
-    const float* v2 = vtx + (size_t)idx[3 * T.prim + 2] * P.stride;
+    const float* v2 = vtx + (size_t)idx[3 * T.prim + (T.swapped ? 0 : 2)] * P.stride;
     if (P.numTextures > 0) {
         // the texture header depends on the primitive only: it is read in the same round
         // trip as the vertex indices (issued after them, so that they are in flight together)
@@ -349,6 +353,19 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, float3 cam, const ShadeT
         R.fp.au = R.fp.av = 0.0f;
     }
     if (framed) return R;  // lighting needs the sampled layers (or the network)
+    if (P.program == 7) {
+        // Lambert plus a (n.l)^shininess highlight from the interpolated normal and one directional
+        // light given unnormalised, untextured: the reference CPU harness's shader
+        // (d = max(0, n.l / |n|), rgb = ambient + light.rgb * d + d^shininess)
+        const float inv = frsq(N.x * N.x + N.y * N.y + N.z * N.z + 1e-20f);
+        float d = (N.x * P.lightPos[0][0] + N.y * P.lightPos[0][1] + N.z * P.lightPos[0][2]) * inv;
+        d = d > 0.0f ? d : 0.0f;
+        const float sp = d > 0.0f ? fexp2(P.shininess * flog2(d)) : 0.0f;
+        R.A = make_float3(0, 0, 0);
+        R.B = make_float3(P.ambient + P.lightColor[0][0] * d + sp, P.ambient + P.lightColor[0][1] * d + sp,
+                          P.ambient + P.lightColor[0][2] * d + sp);
+        return R;
+    }
     N = norm3(N);
     const float3 V = norm3(make_float3(cam.x - Pw.x, cam.y - Pw.y, cam.z - Pw.z));
     float3 A = make_float3(P.ambient, P.ambient, P.ambient), B = make_float3(0, 0, 0);
@@ -543,7 +560,10 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
     if (LAYERED && P.program == 5 && S == 1 && inFb && nf == 0) P.neural.featOut[4 * (pixBase + pix) + 1] = make_float4(0, 0, 0, 0);
     constexpr bool layered = LAYERED;  // programs 3 and 5: three textures per tap
     const float o = S == 1 ? 0.0f : 0.5f;
-    float4 sum = make_float4(0, 0, 0, 0);
+    // every sample starts at the clear colour; a fragment replaces its samples' share
+    // (the clear colour 0 of the CPU path leaves this the plain sum of the fragments)
+    float4 sum = make_float4(P.clearColor[0] * (float)S, P.clearColor[1] * (float)S, P.clearColor[2] * (float)S,
+                             P.clearColor[3] * (float)S);
     for (int c0 = 0; c0 < F; c0 += 64) {
         // 2. prepare one fragment per lane
         const int fi = c0 + lane;
@@ -554,7 +574,7 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
             const unsigned slot = fSlot[fi];
             const int fl = fPix[fi];
             ShadeTri T;
-            load_shade_tri(recs, bary, fanStart, slot, T);
+            load_shade_tri(recs, bary, fanStart, slot, T, P.swapped);
             const int qx = ox + (fl & 7), qy = oy + (fl >> 3);
             R = prep_fragment(P, cam, T, vtx, idx, tex, qx, qy, o, NEURAL || LAYERED);
             if (R.mat >= 0) {
@@ -723,14 +743,14 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
         for (int k = k0; k < k1; k++) {
             const float4 c = cCol[k - c0];
             const float cnt = (float)fCnt[k];
-            sum.x += c.x * cnt;
-            sum.y += c.y * cnt;
-            sum.z += c.z * cnt;
-            sum.w += c.w * cnt;
+            sum.x += (c.x - P.clearColor[0]) * cnt;
+            sum.y += (c.y - P.clearColor[1]) * cnt;
+            sum.z += (c.z - P.clearColor[2]) * cnt;
+            sum.w += (c.w - P.clearColor[3]) * cnt;
         }
         __syncthreads();
     }
-    if (!inFb) return;
+    if (!inFb || !rc::scissor_pass(P.scissor, (float)px, (float)py)) return;
     const float inv = 1.0f / (float)S;
     const float4 r = make_float4(sum.x * inv, sum.y * inv, sum.z * inv, sum.w * inv);
     rgba8[pix] = make_uchar4((unsigned char)floorf(r.x * 255.0f + 0.5f), (unsigned char)floorf(r.y * 255.0f + 0.5f),
@@ -835,7 +855,6 @@ __device__ __forceinline__ float4 blend_pixel(const BlendCoefs& B, float4 src, f
 using dev::Edges;
 using dev::block_test;
 using dev::block_sum256;
-using dev::kClearDepth;
 using dev::load_edges;
 using dev::load_rec;
 
@@ -865,9 +884,11 @@ __global__ void __launch_bounds__(256) k_blend_ordered(ShadeParams P, BlendCoefs
         const size_t pix = G.pixBase + C.pix;
         const float3 cam = make_float3(D.camera[0], D.camera[1], D.camera[2]);
         const unsigned fanStart = D.slotBase + (unsigned)P.ntri;
-        float dep = kClearDepth;
+        const bool sciss = rc::scissor_pass(P.scissor, C.fpx, C.fpy);  // outside: the pixel is left as it is
+        float dep = P.clearDepth;
         unsigned slot = 0xffffffffu, cls = 0;
-        float4 col = make_float4(0, 0, 0, 0);  // the colour attachment is cleared to 0 (TileRasterWorker.cpp:113-118)
+        // the colour attachment is cleared to the clear colour (0 in the CPU path, TileRasterWorker.cpp:113-118)
+        float4 col = make_float4(P.clearColor[0], P.clearColor[1], P.clearColor[2], P.clearColor[3]);
         if (!clear && C.inFb) {
             dep = visDepthAll[pix];
             slot = visSlotAll[pix];
@@ -879,7 +900,7 @@ __global__ void __launch_bounds__(256) k_blend_ordered(ShadeParams P, BlendCoefs
         rc::for_each_ordered(recs, binList, start, cnt, sA, sB, [&](unsigned ent) {
             const unsigned sl = ent & 0x7fffffffu;
             const bool full = (ent >> 31) != 0;
-            if (!C.inFb) return;
+            if (!C.inFb || !sciss) return;
             const TriRec R = load_rec(recs, sl);
             const Edges E = load_edges(R);
             int trn, tan;
@@ -890,13 +911,13 @@ __global__ void __launch_bounds__(256) k_blend_ordered(ShadeParams P, BlendCoefs
             frags++;
             if (!rc::depth_pass(depthFunc, d, dep, c == 3u)) return;
             ShadeTri T;
-            load_shade_tri(recs, bary, fanStart, sl, T);
+            load_shade_tri(recs, bary, fanStart, sl, T, P.swapped);
             col = blend_pixel(B, shade_fragment(P, cam, T, vtx, idx, tex, pool, C.px, C.py), col);
             dep = d;
             slot = sl;
             cls = c;
         });
-        if (C.inFb) {
+        if (C.inFb && sciss) {
             visDepthAll[pix] = dep;
             visSlotAll[pix] = slot;
             if (visClassAll) visClassAll[pix] = (uint8_t)cls;

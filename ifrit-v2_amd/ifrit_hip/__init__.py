@@ -109,7 +109,25 @@ EXPORTED_SYMBOLS = [
     "iftrNeuralTrain", "iftrNeuralSetParams", "iftrNeuralGetParams", "iftrNeuralEval", "iftrNeuralValidate",
     "iftrSetViews", "iftrGetViewVisibility", "iftrGetViewColorRGBA8", "iftrSetGraphMode", "iftrSetTuning",
     "iftrGetTuning", "iftrAutotune", "iftrGetDigest", "iftrSetColorFloatOutput", "iftrGetColorRGBA32F",
+    "iftrDrawNoAlloc", "iftrNeuralValidateEx", "iftrSetCullMode", "iftrSetClearValues", "iftrSetScissors",
+    "iftrSetScissorTestEnable",
 ]
+
+
+class NeuralValidation(C.Structure):
+    _fields_ = [("psnrDb", C.c_float), ("maxErrLsb", C.c_int32), ("p999ErrLsb", C.c_int32), ("fallback", C.c_int32),
+                ("pixels", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+NOALLOC_SCRATCH_BYTES = 48 << 20  # IFHIP_NOALLOC_SCRATCH_BYTES
+
+
+class NoAllocDraw(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("strideFloats", C.c_int32), ("numVertices", C.c_int32),
+                ("indices", C.c_void_p), ("numIndices", C.c_int32), ("primIds", C.c_void_p), ("depth", C.c_void_p),
+                ("rgba8", C.c_void_p)]
 
 
 def lib():
@@ -161,6 +179,12 @@ def lib():
         "iftrAutotune": ([vp, C.c_int, C.c_int, C.POINTER(AutotuneReport)], None),
         "iftrGetDigest": ([vp], C.c_uint64),
         "iftrSetColorFloatOutput": ([vp, C.c_int], None), "iftrGetColorRGBA32F": ([vp, vp], None),
+        "iftrDrawNoAlloc": ([vp, C.POINTER(NoAllocDraw)], None),
+        "iftrSetCullMode": ([vp, C.c_int], None),
+        "iftrSetClearValues": ([vp, vp, C.c_size_t, C.c_float], None),
+        "iftrSetScissors": ([vp, vp, C.c_size_t], None), "iftrSetScissorTestEnable": ([vp, C.c_int], None),
+        "iftrNeuralValidateEx": ([vp, C.c_int, C.c_float, C.c_int, C.c_int, C.POINTER(NeuralValidation)], None),
+        "iftrOptsetDepthTestEnable": ([vp, C.c_int], None),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -212,6 +236,25 @@ class Renderer:
     def set_depth_func(self, func: int):
         """IF_COMPARE_OP_* (include/ifrit_hip.h); LESS (1) is the default."""
         self.L.iftrSetDepthFunc(self.h, func)
+        self._check()
+
+    def set_cull_mode(self, mode: int):
+        """IF_CULL_MODE_*: 0 none, 1 front, 2 back (the default), 3 front and back."""
+        self.L.iftrSetCullMode(self.h, mode)
+        self._check()
+
+    def set_clear_values(self, color=(0.0, 0.0, 0.0, 0.0), depth=None):
+        '''Colour and depth a draw with clear starts from (depth None: the CPU path's 0x7f7f7f7f).'''
+        d = float(np.frombuffer(np.uint32(0x7F7F7F7F).tobytes(), np.float32)[0]) if depth is None else depth
+        c = np.ascontiguousarray(color, np.float32).reshape(1, 4)
+        self.L.iftrSetClearValues(self.h, _p(c), 1, d)
+        self._check()
+
+    def set_scissors(self, rects, enable=True):
+        '''Scissor rectangles (x, y, width, height) in pixels and the test on/off.'''
+        r = np.ascontiguousarray(rects, np.float32).reshape(-1, 4)
+        self.L.iftrSetScissors(self.h, _p(r), r.shape[0])
+        self.L.iftrSetScissorTestEnable(self.h, 1 if enable else 0)
         self._check()
 
     def set_blend(self, factors=None):
@@ -288,6 +331,15 @@ class Renderer:
         v = self.L.iftrNeuralValidate(self.h, precision, min_psnr_db, self.num_indices)
         self._check()
         return float(v)
+
+    def validate_neural_ex(self, precision: int = 0, min_psnr_db: float = 40.0, max_err_lsb: int = 255) -> dict:
+        """PSNR, largest and 99.9th-percentile per-pixel error of the neural frame against the
+        layered-GGX frame; the neural program falls back below the PSNR bound or above the
+        per-pixel bound."""
+        v = NeuralValidation()
+        self.L.iftrNeuralValidateEx(self.h, precision, min_psnr_db, max_err_lsb, self.num_indices, C.byref(v))
+        self._check()
+        return v.as_dict()
 
     def set_neural_params(self, params: np.ndarray):
         p = np.ascontiguousarray(params, np.float32).reshape(-1)

@@ -26,6 +26,7 @@ PROGRAM_BLINN_PHONG = 1
 PROGRAM_DEPTH_ONLY = 2
 PROGRAM_GGX_LAYERED = 3
 PROGRAM_FLAT_ALPHA = 6
+PROGRAM_NORMAL_LAMBERT = 7
 SAMPLER_BILINEAR = 0
 SAMPLER_TRILINEAR = 1
 SAMPLER_ANISO = 2
@@ -469,3 +470,18 @@ def config3(width: int = 1920, height: int = 1080, samples: int = 4, gx: int = 2
     idx = np.concatenate([quads, np.asarray(bidx)]).astype(np.int32)
     return Scene("config3", width, height, samples, verts, idx, mvp64.astype(np.float32), program=PROGRAM_FLAT,
                  camera=tuple(eye))
+
+
+def normal_lambert(scene: Scene) -> Scene:
+    """`scene` shaded like the reference CPU path's timing workload: the
+    normal-Lambert program (no textures) with light direction (0.3, 0.8, -0.52)
+    used as given, ambient 0.1, diffuse weights (0.8, 0.6, 0.4) and the
+    highlight exponent 32."""
+    return dataclasses.replace(scene, program=PROGRAM_NORMAL_LAMBERT, ambient=0.1, shininess=32.0, ks=1.0, kd=1.0,
+                               lights=[((0.3, 0.8, -0.52, 0.0), (0.8, 0.6, 0.4, 0.0))], textures=[])
+
+
+def reference_workload(width: int = 1920, height: int = 1080, spheres: int = 256, level: int = 4) -> Scene:
+    """The config-2 geometry at one sample per pixel with the normal-Lambert
+    shading: the frame the reference CPU rasterizer is timed on."""
+    return normal_lambert(config2(width, height, 1, spheres=spheres, level=level))

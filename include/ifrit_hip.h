@@ -143,7 +143,11 @@ enum {
                                  2 roughness (red); needs a tangent in the vertex record */
     IFHIP_FS_NEURAL = 4,      /* the neural material trained by iftrNeuralTrain (same inputs as
                                  IFHIP_FS_GGX_LAYERED; light 0 only) */
-    IFHIP_FS_FLAT_ALPHA = 6   /* IFHIP_FS_FLAT with alpha = (64 + (hash byte 3 & 127)) / 255 */
+    IFHIP_FS_FLAT_ALPHA = 6,  /* IFHIP_FS_FLAT with alpha = (64 + (hash byte 3 & 127)) / 255 */
+    IFHIP_FS_NORMAL_LAMBERT = 7 /* untextured: d = max(0, n.l / |n|) with n the interpolated normal and
+                                   l = lightPos[0].xyz as given; rgb = ambient + lightColor[0].rgb * d
+                                   + d^shininess, alpha 1 (the shader of the reference CPU path's
+                                   timing harness) */
 };
 /* IfritBlendFactor, include/ifrit/softgraphics/engine/base/Constants.h:64-76.  Blending
    (iftrSetBlendFunc) supports ZERO, ONE, SRC_ALPHA, ONE_MINUS_SRC_ALPHA, DST_ALPHA and
@@ -182,6 +186,36 @@ typedef struct {
 void* iftrCreateGpuVertexProgram(const IfritGpuVertexProgramDesc* desc);
 void* iftrCreateGpuFragmentProgram(const IfritGpuFragmentProgramDesc* desc);
 void iftrDestroyGpuProgram(void* program);
+
+/* IfritCullMode, include/ifrit/softgraphics/engine/base/Constants.h:29-35 */
+enum {
+    IF_CULL_MODE_NONE = 0,
+    IF_CULL_MODE_FRONT = 1,
+    IF_CULL_MODE_BACK = 2,
+    IF_CULL_MODE_FRONT_AND_BACK = 3
+};
+/* SetCullMode of the CUDA renderer (TileRasterRendererCuda.h:109; behaviour of
+   TileRasterCoreInvocationCuda.cu:1433-1462).  BACK (the default) is the CPU path's back-face cull.
+   FRONT sets every triangle up with its first and third vertex exchanged, so only back faces
+   survive; NONE does so for the triangles (and clip fans) that BACK would cull; FRONT_AND_BACK
+   draws nothing.  Exchanged triangles shade with their own vertex attributes. */
+void iftrSetCullMode(IfritRendererHandle* h, int mode);
+
+/* setClearValues of the CUDA renderer (TileRasterRendererCuda.h:111): the colour and depth a draw
+   with clear starts from.  The first colour applies to the (one) colour attachment; samples no
+   triangle covers resolve with it.  The defaults are the CPU path's: colour 0 and depth
+   0x7f7f7f7f (TileRasterWorker.cpp:40, 113-118).  iftrClear keeps the CPU path's values. */
+void iftrSetClearValues(IfritRendererHandle* h, const IfritVector4f* colors, size_t numColors, float depth);
+
+/* SetScissors / setScissorTestEnable of the CUDA renderer (TileRasterRendererCuda.h:101-102):
+   up to 8 rectangles (x, y, width, height) in pixels of the render target (or of each view of a
+   batch); with the test on, a pixel (px, py) passes when it lies in one of them, x <= px <= x +
+   width and y <= py <= y + height (TileRasterCoreInvocationCuda.cu:1699-1711).  A draw leaves
+   the pixels that fail as they are: visibility, depth and colour, its clear included (the CUDA
+   clear kernels are scissored too, :3336-3424).  The test is off by default; on with no
+   rectangle, every pixel passes. */
+void iftrSetScissors(IfritRendererHandle* h, const IfritVector4f* rects, size_t numRects);
+void iftrSetScissorTestEnable(IfritRendererHandle* h, int enable);
 
 /* setMsaaSamples of the CUDA renderer (TileRasterRendererCuda.h:105): 1, 4 or 8. */
 void iftrSetMsaaSamples(IfritRendererHandle* h, int samples);
@@ -266,6 +300,19 @@ void iftrNeuralGetParams(IfritRendererHandle* h, float* params);
    against the material's.  Below minPsnrDb, IFHIP_FS_NEURAL at that precision falls back to the
    layered-GGX material until the next validation or training. */
 float iftrNeuralValidate(IfritRendererHandle* h, int precision, float minPsnrDb, int numIndices);
+/* iftrNeuralValidate with a per-pixel bound as well: the neural frame is compared with the
+   material's over the covered pixels; below minPsnrDb, or with any pixel whose largest RGB error
+   exceeds maxErrLsb RGBA8 steps (255: no per-pixel bound), IFHIP_FS_NEURAL at that precision falls
+   back to the layered-GGX material until the next validation or training. */
+typedef struct {
+    float psnrDb;                     /* RGB over covered pixels */
+    int32_t maxErrLsb;                /* largest per-pixel RGB error, RGBA8 steps */
+    int32_t p999ErrLsb;               /* error that 99.9 % of the covered pixels stay within */
+    int32_t fallback;                 /* 1: the neural program now draws the material */
+    int64_t pixels;                   /* covered pixels compared */
+} IfritNeuralValidation;
+void iftrNeuralValidateEx(IfritRendererHandle* h, int precision, float minPsnrDb, int maxErrLsb, int numIndices,
+                          IfritNeuralValidation* out);
 /* The MLP alone on n inputs ([n][16] floats, 14 used) -> [n][4] outputs (3 used). */
 void iftrNeuralEval(IfritRendererHandle* h, const float* in, float* out, int n, int precision);
 
@@ -316,6 +363,33 @@ typedef struct {
 void iftrAutotune(IfritRendererHandle* h, int numIndices, int frames, IfritAutotuneReport* report);
 /* Order-independent 64-bit digest of the last draw's depth, primitive IDs and colour. */
 uint64_t iftrGetDigest(IfritRendererHandle* h);
+
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+   hipGraph capture, tight serving loops, harnesses that forbid hipMalloc): the draw of
+   iftrDrawLegacy for the bound single target (iftrSetRenderTarget or iftrBindFrameBuffer size,
+   MSAA sample count, GPU vertex and fragment programs, textures, depth function), from the
+   caller's device-resident vertices and indices into the caller's device outputs.  It calls no
+   hipMalloc or hipFree: every intermediate buffer (clip positions, triangle records, bins, raster
+   queue, counters, the per-sample record slots) is carved out of one static device array of
+   IFHIP_NOALLOC_SCRATCH_BYTES per GPU, which the library holds from iftrInit until the process
+   ends.  A draw whose buffers do not fit is refused with an error (iftrGetLastError) before any
+   launch; 1920x1080 at 1 sample with 128K triangles, or 1024x1024 at 4 samples with 64K
+   triangles, fit.  Every call on a GPU uses that GPU's array and keeps the last call's contents:
+   calls that overlap on one GPU (two renderers, two threads) race.  The frame is always cleared;
+   it blocks until the frame is done.  No blending, host vertex functions or view batches.
+   Overflowing the bin or clip capacities (which a normal draw grows) is reported as an error. */
+#define IFHIP_NOALLOC_SCRATCH_BYTES (48u << 20)
+typedef struct {
+    const float* vertices;      /* device: numVertices records of strideFloats floats */
+    int32_t strideFloats;
+    int32_t numVertices;
+    const int32_t* indices;     /* device: numIndices int32 */
+    int32_t numIndices;
+    uint32_t* primIds;          /* device, W*H*S primitive IDs (0xffffffff: empty), or NULL */
+    float* depth;               /* device, W*H*S FP32 depth */
+    uint8_t* rgba8;             /* device, W*H*4 resolved colour, or NULL to skip shading */
+} IfritNoAllocDraw;
+void iftrDrawNoAlloc(IfritRendererHandle* h, const IfritNoAllocDraw* draw);
 
 #ifdef __cplusplus
 }

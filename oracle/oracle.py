@@ -22,9 +22,13 @@ REF_PATH = os.path.join(HERE, "_ref", "libifrit_ref.so")
 REFERENCE_ROOT = "/root/reference"
 
 
+CLEAR_DEPTH = float(np.frombuffer(np.uint32(0x7F7F7F7F).tobytes(), np.float32)[0])  # TileRasterWorker.cpp:40
+
+
 class RasterConfig(C.Structure):
     _fields_ = [("width", C.c_int), ("height", C.c_int), ("samples", C.c_int), ("ccw", C.c_int),
-                ("depth_func", C.c_int), ("clear", C.c_int), ("num_threads", C.c_int), ("live_rcp", C.c_int)]
+                ("depth_func", C.c_int), ("clear", C.c_int), ("num_threads", C.c_int), ("live_rcp", C.c_int),
+                ("cull_mode", C.c_int), ("clear_depth", C.c_float)]
 
 
 class Stats(C.Structure):
@@ -38,12 +42,12 @@ class Stats(C.Structure):
 class Tri(C.Structure):
     _fields_ = [("f1", C.c_float * 4), ("f2", C.c_float * 4), ("f3", C.c_float * 4), ("bx", C.c_float * 3),
                 ("by", C.c_float * 3), ("e1", C.c_float * 3), ("e2", C.c_float * 3), ("e3", C.c_float * 3),
-                ("bbox", C.c_float * 4), ("prim", C.c_int32), ("fan", C.c_int32)]
+                ("bbox", C.c_float * 4), ("prim", C.c_int32), ("fan", C.c_int32), ("swapped", C.c_int32)]
 
 
 TRI_DTYPE = np.dtype([("f1", "<f4", 4), ("f2", "<f4", 4), ("f3", "<f4", 4), ("bx", "<f4", 3), ("by", "<f4", 3),
                       ("e1", "<f4", 3), ("e2", "<f4", 3), ("e3", "<f4", 3), ("bbox", "<f4", 4), ("prim", "<i4"),
-                      ("fan", "<i4")])
+                      ("fan", "<i4"), ("swapped", "<i4")])
 
 
 class ShadeParams(C.Structure):
@@ -51,7 +55,7 @@ class ShadeParams(C.Structure):
                 ("off_uv", C.c_int), ("tris_per_material", C.c_int), ("sampler", C.c_int), ("max_aniso", C.c_int),
                 ("camera", C.c_float * 3), ("ambient", C.c_float), ("kd", C.c_float), ("ks", C.c_float),
                 ("shininess", C.c_float), ("num_lights", C.c_int), ("light_pos", (C.c_float * 4) * 4),
-                ("light_color", (C.c_float * 4) * 4), ("off_tangent", C.c_int)]
+                ("light_color", (C.c_float * 4) * 4), ("off_tangent", C.c_int), ("clear_color", C.c_float * 4)]
 
 
 class TextureDesc(C.Structure):
@@ -141,7 +145,7 @@ class Frame:
             self._h = None
 
     def render(self, width, height, samples, clip, indices, depth_func=1, ccw=0, threads=None, live_rcp=False,
-               clear=True, depth_in=None):
+               clear=True, depth_in=None, cull_mode=2, clear_depth=CLEAR_DEPTH):
         clip = np.ascontiguousarray(clip, np.float32)
         indices = np.ascontiguousarray(indices, np.int32)
         n = width * height
@@ -151,7 +155,7 @@ class Frame:
         cls = np.empty((n,), np.uint8)
         st = Stats()
         cfg = RasterConfig(width, height, samples, ccw, depth_func, 1 if clear else 0,
-                           threads or os.cpu_count() or 1, 1 if live_rcp else 0)
+                           threads or os.cpu_count() or 1, 1 if live_rcp else 0, cull_mode, clear_depth)
         din = None if depth_in is None else np.ascontiguousarray(depth_in, np.float32)
         rc = lib().oracle_render(self._h, C.byref(cfg), _p(clip), clip.shape[0], _p(indices), indices.size,
                                  None if din is None else _p(din), _p(ids), _p(depth), _p(tri), _p(cls), C.byref(st))
@@ -171,7 +175,8 @@ class Frame:
         ids = np.empty((n,), np.uint32)
         depth = np.empty((n,), np.float32)
         rgba = np.empty((n, 4), np.float32)
-        cfg = RasterConfig(width, height, 1, 0, depth_func, 1 if clear else 0, threads or os.cpu_count() or 1, 0)
+        cfg = RasterConfig(width, height, 1, 0, depth_func, 1 if clear else 0, threads or os.cpu_count() or 1, 0, 2,
+                           CLEAR_DEPTH)
         bl = (C.c_int * 4)(*factors)
         din = None if depth_in is None else np.ascontiguousarray(depth_in, np.float32)
         cin = None if rgba_in is None else np.ascontiguousarray(rgba_in, np.float32)
@@ -190,9 +195,11 @@ class Frame:
             lib().oracle_get_tris(self._h, _p(out))
         return out
 
-    def shade(self, scene, threads=None):
-        """RGBA8 colour and boundary flags from the FP64 shading oracle."""
+    def shade(self, scene, threads=None, clear_color=(0.0, 0.0, 0.0, 0.0)):
+        """RGBA8 colour and boundary flags from the FP64 shading oracle; samples no triangle covers
+        count as clear_color."""
         p = ShadeParams()
+        p.clear_color[:] = [float(x) for x in clear_color]
         p.program = scene.program
         p.stride_floats = scene.stride
         p.off_world, p.off_normal, p.off_uv = 0, 3, 6
@@ -223,10 +230,10 @@ class Frame:
         return rgba.reshape(self.height, self.width, 4), flags.reshape(self.height, self.width)
 
 
-def render_scene(scene, threads=None, live_rcp=False):
+def render_scene(scene, threads=None, live_rcp=False, cull_mode=2, clear_depth=CLEAR_DEPTH):
     clip = vertex_transform(scene.vertices, scene.mvp)
     f = Frame().render(scene.width, scene.height, scene.samples, clip, scene.indices, threads=threads,
-                       live_rcp=live_rcp)
+                       live_rcp=live_rcp, cull_mode=cull_mode, clear_depth=clear_depth)
     return f, clip
 
 

@@ -38,6 +38,9 @@ typedef struct {
     int clear;          // 1: depth starts at 0x7f7f7f7f; 0: depth starts from *_in buffers
     int num_threads;    // tile-parallel worker threads (>= 1)
     int live_rcp;       // 1: use this CPU's RCPSS/RCPPS instead of the captured table
+    int cull_mode;      // IfritCullMode: 2 back (the CPU path); 1 front and 0 none set triangles up with
+                        // vertices 0 and 2 exchanged (front: all; none: those back would cull); 3 all
+    float clear_depth;  // depth a clear starts from: 0x7f7f7f7f in the CPU path
 } OracleRasterConfig;
 
 typedef struct {
@@ -57,6 +60,7 @@ typedef struct {
     float bbox[4];      // tile-space bbox used by the binner (minx, miny, maxx, maxy)
     int32_t prim;       // originalPrimitive = index offset / 3
     int32_t fan;        // fan index inside a clipped primitive (0 for unclipped)
+    int32_t swapped;    // set up with vertices 0 and 2 exchanged (cull modes none and front)
 } OracleTri;
 
 // Sample positions (fractions of a pixel, relative to the integer pixel

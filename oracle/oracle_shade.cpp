@@ -167,15 +167,17 @@ struct Shader {
         w[1] = B[0] * t.by[0] + B[1] * t.by[1] + B[2] * t.by[2];
         w[2] = 1.0 - w[0] - w[1];
     }
-    const float* vert(int prim, int k) const {
-        return vtx + (size_t)idx[3 * prim + k] * p->stride_floats;
+    // vertex k of triangle t in the order setup took them (exchanged triangles: 2, 1, 0)
+    const float* vert(const OracleTri& t, int k) const {
+        const int kk = t.swapped ? 2 - k : k;
+        return vtx + (size_t)idx[3 * t.prim + kk] * p->stride_floats;
     }
     void uv_at(const OracleTri& t, double x, double y, double* uv) const {
         double w[3];
         weights(t, x, y, w);
         uv[0] = uv[1] = 0;
         for (int k = 0; k < 3; k++) {
-            const float* v = vert(t.prim, k);
+            const float* v = vert(t, k);
             uv[0] += w[k] * v[p->off_uv];
             uv[1] += w[k] * v[p->off_uv + 1];
         }
@@ -193,6 +195,25 @@ struct Shader {
             rgba[3] = 1.0;
             return false;
         }
+        if (p->program == ORP_NORMAL_LAMBERT) {
+            // the shader of the reference CPU timing harness (oracle/ref/ref_harness.cpp LambertFS)
+            double w[3];
+            weights(t, px + o, py + o, w);
+            D3 N{0, 0, 0};
+            for (int k = 0; k < 3; k++) {
+                const float* v = vert(t, k);
+                N = N + D3{v[p->off_normal], v[p->off_normal + 1], v[p->off_normal + 2]} * w[k];
+            }
+            const double l = std::sqrt(dot(N, N));
+            const double nl = dot(N, D3{p->light_pos[0][0], p->light_pos[0][1], p->light_pos[0][2]});
+            double d = l > 0 ? nl / l : 0.0;
+            const bool flag = std::fabs(d) < 1e-4;
+            d = std::max(d, 0.0);
+            const double sp = std::pow(d, (double)p->shininess);
+            for (int k = 0; k < 3; k++) rgba[k] = p->ambient + p->light_color[0][k] * d + sp;
+            rgba[3] = 1.0;
+            return flag;
+        }
         if (p->program != ORP_BLINN_PHONG && p->program != ORP_GGX_LAYERED) {
             rgba[0] = rgba[1] = rgba[2] = rgba[3] = 0;
             return false;
@@ -203,7 +224,7 @@ struct Shader {
         D3 P{0, 0, 0}, N{0, 0, 0};
         double uv[2] = {0, 0};
         for (int k = 0; k < 3; k++) {
-            const float* v = vert(t.prim, k);
+            const float* v = vert(t, k);
             P = P + D3{v[p->off_world], v[p->off_world + 1], v[p->off_world + 2]} * w[k];
             N = N + D3{v[p->off_normal], v[p->off_normal + 1], v[p->off_normal + 2]} * w[k];
             uv[0] += w[k] * v[p->off_uv];
@@ -220,7 +241,7 @@ struct Shader {
         if (p->program == ORP_GGX_LAYERED) {
             D3 T{0, 0, 0};
             for (int k = 0; k < 3; k++) {
-                const float* v = vert(t.prim, k);
+                const float* v = vert(t, k);
                 T = T + D3{v[p->off_tangent], v[p->off_tangent + 1], v[p->off_tangent + 2]} * w[k];
             }
             double alb[4] = {1, 1, 1, 1}, nm[4] = {0.5, 0.5, 1, 1}, rg[4] = {0, 0, 0, 0};
@@ -418,6 +439,8 @@ int oracle_shade(const OracleFrame* f, const OracleShadeParams* p, int W, int H,
                     flag |= sh.shade(ids[s], x, row, o, c);
                     for (int k = 0; k < 4; k++) sum[k] += std::min(std::max(c[k], 0.0), 1.0) * count;
                 }
+                if (g_plant != 4)
+                    for (int k = 0; k < 4; k++) sum[k] += (double)p->clear_color[k] * (S - covered);
                 const int div = g_plant == 4 ? std::max(1, covered) : S;
                 for (int k = 0; k < 4; k++) rgba8_out[4 * pix + k] = to_u8(sum[k] / div);
                 if (flag_out) flag_out[pix] = flag;

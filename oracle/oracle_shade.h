@@ -24,7 +24,8 @@ enum OracleProgram {
     ORP_FLAT = 0,         // colour = bytes of hash(prim), alpha 255
     ORP_BLINN_PHONG = 1,  // textured Blinn-Phong, directional and point lights
     ORP_DEPTH_ONLY = 2,   // no colour output
-    ORP_GGX_LAYERED = 3   // layered GGX: texture 0 albedo, 1 tangent-space normal, 2 roughness
+    ORP_GGX_LAYERED = 3,  // layered GGX: texture 0 albedo, 1 tangent-space normal, 2 roughness
+    ORP_NORMAL_LAMBERT = 7  // untextured: d = max(0, n.l / |n|), rgb = ambient + light rgb d + d^shininess
 };
 enum OracleSampler { ORS_BILINEAR = 0, ORS_TRILINEAR = 1, ORS_ANISO = 2 };
 
@@ -43,6 +44,7 @@ typedef struct {
     float light_pos[4][4];  // w = 0: direction towards the light; w = 1: point light position
     float light_color[4][4];  // rgb, attenuation coefficient k in 1 / (1 + k d^2)
     int off_tangent;        // float offset of the tangent (xyz), ORP_GGX_LAYERED
+    float clear_color[4];   // colour of samples no triangle covers (0 in the CPU path)
 } OracleShadeParams;
 
 typedef struct {
@@ -56,7 +58,7 @@ uint32_t oracle_flat_hash(uint32_t prim);
 // Shades every pixel from the per-sample winners tri_ids (indices into the
 // frame's assembled triangles, 0xffffffff = empty; each distinct triangle is
 // shaded once per pixel), clamps each sample's colour to [0, 1], averages the
-// samples (empty samples count as 0) and rounds to RGBA8.  Returns the number of flagged pixels.
+// samples (empty samples count as the clear colour) and rounds to RGBA8.  Returns the number of flagged pixels.
 int oracle_shade(const OracleFrame* f, const OracleShadeParams* p, int W, int H, int S, const uint32_t* tri_ids,
                  const int32_t* indices, const float* vertices, const OracleTexture* textures, int ntex,
                  uint8_t* rgba8_out, uint8_t* flag_out, int num_threads);

@@ -1,0 +1,46 @@
+"""The reference CPU path's own workload on the GPU.
+
+The CPU baseline times the reference rasterizer (built from its own sources)
+drawing the config-2 geometry at 1920x1080 with one sample per pixel and its
+timing harness's shading: Lambert plus a (n.l)^32 highlight from the
+interpolated normal (oracle/ref/ref_harness.cpp LambertFS), FP32 colour.  The
+GPU draws the same frame with the normal-Lambert program.  Visibility must be
+bit-exact against the oracle, and the colour within one RGBA8 step of both
+the FP64 oracle and the reference build's own FP32 colour output (when the
+reference build is present).
+"""
+import numpy as np
+import pytest
+
+from ifrit_hip import scenes
+from oracle import oracle as O
+from test_oracle_pins import _ref_frame_color
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("size", [(480, 270, 24), (1920, 1080, 256)])
+def test_reference_workload_matches_oracle_and_reference(renderer_factory, size):
+    W, H, spheres = size
+    s = scenes.reference_workload(W, H, spheres=spheres, level=4 if spheres == 256 else 3)
+    r = renderer_factory()
+    r.load_scene(s)
+    st = r.draw()
+    ids, depth = r.visibility()
+    rgba = r.color()
+    f, _ = O.render_scene(s)
+    bad_id = int((ids != f.ids).sum())
+    bad_d = int((depth.view(np.uint32) != f.depth.view(np.uint32)).sum())
+    print(f"[reference workload {W}x{H}] fragments {st['fragments']} id/depth mismatches {bad_id}/{bad_d}", flush=True)
+    assert bad_id == 0 and bad_d == 0
+    ref, flags = f.shade(s)
+    diff = np.abs(rgba.astype(int) - ref.astype(int)).max(-1)
+    print(f"[reference workload {W}x{H}] vs FP64 oracle: max diff {diff.max()}, >1 unflagged "
+          f"{int(((diff > 1) & (flags == 0)).sum())}", flush=True)
+    assert ((diff > 1) & (flags == 0)).sum() == 0
+    if O.ref_available():
+        want = np.floor(np.clip(_ref_frame_color(s), 0, 1) * 255 + 0.5).astype(int)
+        d2 = np.abs(rgba.astype(int) - want).max(-1)
+        print(f"[reference workload {W}x{H}] vs reference build: max diff {d2.max()}, >1 unflagged "
+              f"{int(((d2 > 1) & (flags == 0)).sum())}", flush=True)
+        assert ((d2 > 1) & (flags == 0)).sum() == 0

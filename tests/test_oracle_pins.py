@@ -216,3 +216,40 @@ def test_blend_factor_encoding_closed_forms():
     assert np.allclose(f.rgba[m][:, :3], src * (1 - a), rtol=1e-6) and (f.rgba[..., 3] == 0).all()
     f = O.Frame().render_blend(8, 8, clip, idx, (BF_DST_ALPHA, BF_ZERO, BF_ONE_MINUS_DST_ALPHA, BF_ZERO))
     assert np.allclose(f.rgba[m][:, :3], src, rtol=1e-6) and (f.rgba[..., 3] == 0).all()
+
+
+def _ref_frame_color(s):
+    """FP32 colour attachment of the reference build drawing scene `s` with its
+    timing harness's vertex transform and normal-Lambert fragment shader."""
+    import ctypes as C
+
+    v = np.ascontiguousarray(s.vertices, np.float32)
+    i = np.ascontiguousarray(s.indices, np.int32)
+    m = np.ascontiguousarray(s.mvp, np.float32)
+    out = np.zeros((s.height, s.width, 4), np.float32)
+    O.ref().ref_render_timed(s.width, s.height, v.ctypes.data_as(C.c_void_p), v.shape[1], v.shape[0],
+                             i.ctypes.data_as(C.c_void_p), i.size, m.ctypes.data_as(C.c_void_p), 1,
+                             out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("which", ["config1", "config2"])
+def test_oracle_normal_lambert_matches_reference_build(which):
+    """The oracle's normal-Lambert program against the reference CPU path's own colour output
+    on its own workload: RGBA8 within one step of the reference's FP32 colour, clamped and
+    rounded, outside pixels the oracle flags (n.l within 1e-4 of 0)."""
+    from ifrit_hip import scenes
+
+    base = scenes.config1(320, 240) if which == "config1" else scenes.config2(480, 270, 1, spheres=24, level=3,
+                                                                               tex_size=64, host_textures=False)
+    s = scenes.normal_lambert(base)
+    ref = _ref_frame_color(s)
+    f, _ = O.render_scene(s, threads=8)
+    got, flags = f.shade(s)
+    want = np.floor(np.clip(ref, 0, 1) * 255 + 0.5).astype(int)
+    diff = np.abs(got.astype(int) - want).max(-1)
+    covered = (f.ids != 0xFFFFFFFF).reshape(s.height, s.width)
+    assert covered.sum() > 2000
+    assert ((diff > 1) & (flags == 0)).sum() == 0, (diff.max(), np.argwhere((diff > 1) & (flags == 0))[:5])
+    assert np.all(got[~covered] == 0) and np.all(ref[~covered] == 0)
