@@ -257,6 +257,22 @@ int iftrCreateTexture(IfritRendererHandle* h, int w, int hh, int levels) {
     guarded(h, [&](TileRasterRenderer& r) { id = r.createTexture(w, hh, levels); });
     return id;
 }
+int iftrCreateTextureLayers(IfritRendererHandle* h, int w, int hh, int levels, int layers) {
+    int id = -1;
+    guarded(h, [&](TileRasterRenderer& r) { id = r.createTexture(w, hh, levels, layers); });
+    return id;
+}
+void iftrCopyHostBufferToImageLayer(IfritRendererHandle* h, int t, int layer, const void* p) {
+    if (p) guarded(h, [&](TileRasterRenderer& r) { r.copyHostBufferToImageLayer(t, layer, p); });
+}
+int iftrCreateSampler(IfritRendererHandle* h, const IfritSamplerDesc* d) {
+    int id = -1;
+    if (d) guarded(h, [&](TileRasterRenderer& r) { id = r.CreateSampler(*d); });
+    return id;
+}
+void iftrBlitImage(IfritRendererHandle* h, int src, int dst, const IfritImageBlit* region, int filter) {
+    if (region) guarded(h, [&](TileRasterRenderer& r) { r.blitImage(src, dst, *region, filter); });
+}
 void iftrCopyHostBufferToImage(IfritRendererHandle* h, int t, const void* p) {
     if (p) guarded(h, [&](TileRasterRenderer& r) { r.copyHostBufferToImage(t, p); });
 }

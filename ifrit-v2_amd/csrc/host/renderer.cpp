@@ -282,25 +282,30 @@ void TileRasterRenderer::setMsaaSamples(int s) {
     samples_ = s;
 }
 
-int TileRasterRenderer::createTexture(int w, int h, int levels) {
+int TileRasterRenderer::createTexture(int w, int h, int levels, int layers) {
     Init();
     if ((w & (w - 1)) || (h & (h - 1)) || w <= 0 || h <= 0 || levels <= 0 || levels > 16) {
         setError("textures must have power-of-two sizes and 1..16 levels");
         return -1;
     }
-    if ((long long)w * h >= (1ll << 28)) {
-        // the sampler addresses a texture's texels (all levels) with 32-bit offsets
-        setError("textures must have fewer than 2^28 texels in level 0");
+    if (layers < 1 || layers > 64) {
+        setError("textures have 1..64 layers (a cube map has 6)");
+        return -1;
+    }
+    if ((long long)w * h * layers >= (1ll << 28)) {
+        // the sampler addresses a texture's texels (all levels and layers) with 32-bit offsets
+        setError("textures must have fewer than 2^28 texels in level 0 over all layers");
         return -1;
     }
     TextureSlot t;
     t.width = w;
     t.height = h;
     t.levels = levels;
+    t.layers = layers;
     size_t off = 0;
     for (int l = 0; l < levels; l++) {
         t.levelOffset.push_back(off);
-        off += (size_t)std::max(1, w >> l) * std::max(1, h >> l);
+        off += t.levelTexels(l) * (size_t)layers;
     }
     t.bytes = off * 4;
     if (!ok(hipMalloc(&t.dev, t.bytes), "hipMalloc texture")) return -1;
@@ -309,10 +314,15 @@ int TileRasterRenderer::createTexture(int w, int h, int levels) {
     return (int)textures_.size() - 1;
 }
 
-void TileRasterRenderer::copyHostBufferToImage(int tex, const void* rgba8) {
+void TileRasterRenderer::copyHostBufferToImage(int tex, const void* rgba8) { copyHostBufferToImageLayer(tex, 0, rgba8); }
+
+// copyHostBufferToImage (TileRasterImageOpInvocationsCuda.cu:157-181): level 0 of one layer.
+void TileRasterRenderer::copyHostBufferToImageLayer(int tex, int layer, const void* rgba8) {
     if (tex < 0 || tex >= (int)textures_.size()) return setError("bad texture id");
     const TextureSlot& t = textures_[tex];
-    ok(hipMemcpy(t.dev, rgba8, (size_t)t.width * t.height * 4, hipMemcpyHostToDevice), "texture upload");
+    if (layer < 0 || layer >= t.layers) return setError("bad texture layer");
+    const size_t n = t.levelTexels(0) * 4;
+    ok(hipMemcpy(t.dev + n * (size_t)layer, rgba8, n, hipMemcpyHostToDevice), "texture upload");
 }
 
 void TileRasterRenderer::uploadTextureLevels(int tex, const void* rgba8) {
@@ -341,10 +351,70 @@ void TileRasterRenderer::generateMipmap(int tex) {
     for (int l = 1; l < t.levels; l++) {
         const int sw = std::max(1, t.width >> (l - 1)), sh = std::max(1, t.height >> (l - 1));
         const int dw = std::max(1, t.width >> l), dh = std::max(1, t.height >> l);
-        launch_mip_level(t.dev + t.levelOffset[l - 1] * 4, sw, sh, t.dev + t.levelOffset[l] * 4, dw, dh, stream_);
+        for (int f = 0; f < t.layers; f++)
+            launch_mip_level(t.dev + (t.levelOffset[l - 1] + t.levelTexels(l - 1) * f) * 4, sw, sh,
+                             t.dev + (t.levelOffset[l] + t.levelTexels(l) * f) * 4, dw, dh, stream_);
     }
     ok(hipGetLastError(), "mip launch");
     ok(hipStreamSynchronize(stream_), "mip sync");
+}
+
+// blitImage (TileRasterRendererCuda.h:95; TileRasterImageOpInvocationsCuda.cu:87-131) between levels and
+// layers of RGBA8 textures.  The source layer is the source subresource's (the CUDA code reads the
+// destination's baseArrayLayer for both, TileRasterImageOpInvocationsCuda.cu:117).
+void TileRasterRenderer::blitImage(int src, int dst, const IfritImageBlit& r, int filter) {
+    if (src < 0 || src >= (int)textures_.size() || dst < 0 || dst >= (int)textures_.size())
+        return setError("bad texture id");
+    if (filter != 0 && filter != 1) return setError("blit filter is IF_FILTER_NEAREST (0) or IF_FILTER_LINEAR (1)");
+    const TextureSlot& S = textures_[src];
+    const TextureSlot& D = textures_[dst];
+    const int sl = (int)r.srcSubresource.mipLevel, dl = (int)r.dstSubresource.mipLevel;
+    if (sl >= S.levels || dl >= D.levels || (int)r.srcSubresource.baseArrayLayer >= S.layers ||
+        (int)r.dstSubresource.baseArrayLayer >= D.layers)
+        return setError("blit level or layer out of range");
+    BlitArgs a{};
+    a.sw = std::max(1, S.width >> sl);
+    a.sh = std::max(1, S.height >> sl);
+    a.dw = std::max(1, D.width >> dl);
+    a.dh = std::max(1, D.height >> dl);
+    a.sx0 = r.srcExtentSt.width;
+    a.sy0 = r.srcExtentSt.height;
+    a.sx1 = r.srcExtentEd.width;
+    a.sy1 = r.srcExtentEd.height;
+    a.dx0 = r.dstExtentSt.width;
+    a.dy0 = r.dstExtentSt.height;
+    a.dx1 = r.dstExtentEd.width;
+    a.dy1 = r.dstExtentEd.height;
+    a.linear = filter;
+    if (a.sx1 > (unsigned)a.sw || a.sy1 > (unsigned)a.sh || a.dx1 > (unsigned)a.dw || a.dy1 > (unsigned)a.dh ||
+        a.sx0 >= a.sx1 || a.sy0 >= a.sy1 || a.dx0 >= a.dx1 || a.dy0 >= a.dy1)
+        return setError("blit regions must be non-empty and inside their levels");
+    if (src == dst && sl == dl && r.srcSubresource.baseArrayLayer == r.dstSubresource.baseArrayLayer)
+        return setError("blit source and destination must be different levels or layers");
+    const uint8_t* sp = S.dev + (S.levelOffset[sl] + S.levelTexels(sl) * r.srcSubresource.baseArrayLayer) * 4;
+    uint8_t* dp = D.dev + (D.levelOffset[dl] + D.levelTexels(dl) * r.dstSubresource.baseArrayLayer) * 4;
+    launch_blit(sp, dp, a, stream_);
+    ok(hipGetLastError(), "blit launch");
+    ok(hipStreamSynchronize(stream_), "blit");
+}
+
+// CreateSampler (TileRasterRendererCuda.h:92): the sampler objects a fragment program selects by id.
+int TileRasterRenderer::CreateSampler(const IfritSamplerDesc& d) {
+    auto mode_ok = [](int m) { return m >= kAddrRepeat && m <= kAddrMirrorClampToEdge; };
+    if ((d.filterMode != 0 && d.filterMode != 1) || !mode_ok(d.addressModeU) || !mode_ok(d.addressModeV) ||
+        !mode_ok(d.addressModeW) || (d.borderColor != 0 && d.borderColor != 1)) {
+        setError("sampler: filter NEAREST/LINEAR, address modes 0..4, border colour BLACK/WHITE");
+        return -1;
+    }
+    SamplerObject o;
+    o.mode.addrU = d.addressModeU;
+    o.mode.addrV = d.addressModeV;
+    o.mode.nearest = d.filterMode == 0 ? 1 : 0;
+    o.mode.border = d.borderColor == 1 ? 0xffffffffu : 0xff000000u;
+    o.anisotropy = d.anisotropyEnable != 0;
+    o.maxAnisotropy = d.maxAnisotropy;
+    samplers_.push_back(o);
+    return (int)samplers_.size();  // ids start at 1: a zero-initialised program description names none
 }
 
 void TileRasterRenderer::runHostVertexShader() {
@@ -441,6 +511,25 @@ bool TileRasterRenderer::prepare(int indexCount) {
         setError("the neural material has not been trained (iftrNeuralTrain) or its textures are not bound");
         return false;
     }
+    if (fsBound_ && fs_.desc.samplerId > (int)samplers_.size()) {
+        setError("the fragment program names a sampler that does not exist");
+        return false;
+    }
+    if (fsBound_ && fs_.desc.kind != IFHIP_FS_FLAT && fs_.desc.kind != IFHIP_FS_FLAT_ALPHA &&
+        fs_.desc.kind != IFHIP_FS_DEPTH_ONLY && fs_.desc.kind != IFHIP_FS_NORMAL_LAMBERT) {
+        // the cube-map program reads six-layer textures, every other textured program one-layer ones
+        const int want = fs_.desc.kind == IFHIP_FS_CUBE_MAP ? 6 : 1;
+        for (const TextureSlot& t : textures_)
+            if (t.layers != want) {
+                setError(want == 6 ? "the cube-map program needs textures of six layers"
+                                   : "textured programs other than the cube map need one-layer textures");
+                return false;
+            }
+        if (fs_.desc.kind == IFHIP_FS_CUBE_MAP && textures_.empty()) {
+            setError("the cube-map program needs a texture");
+            return false;
+        }
+    }
     if (fsBound_ && (fs_.desc.kind == IFHIP_FS_GGX_LAYERED || fs_.desc.kind == IFHIP_FS_NEURAL) &&
         textures_.size() < 3) {
         // the layered kernels read the descriptors of textures 0, 1 and 2
@@ -519,6 +608,7 @@ bool TileRasterRenderer::prepare(int indexCount) {
             td.width = t.width;
             td.height = t.height;
             td.levels = t.levels;
+            td.layers = t.layers;
             d[i] = td;
         }
         if (!texDesc_.ensure(d.size() * sizeof(TexDesc))) return setError("hipMalloc textures"), false;
@@ -671,6 +761,17 @@ ShadeParams TileRasterRenderer::shadeParams(bool writeFloat) const {
     sh.trisPerMaterial = d.trisPerMaterial;
     sh.sampler = d.sampler;
     sh.maxAniso = std::min(16, std::max(1, d.maxAniso));  // the tap list encodes up to 16 taps
+    sh.samp = SamplerMode{kAddrRepeat, kAddrRepeat, 0, 0xff000000u};
+    sh.lod = d.lod;
+    if (d.samplerId >= 1 && d.samplerId <= (int)samplers_.size()) {
+        // a sampler object: its addressing, filter and border; anisotropic filtering when it enables
+        // it (maxAnisotropy taps at most), else the trilinear level of detail (SBC:357-377)
+        const SamplerObject& o = samplers_[d.samplerId - 1];
+        sh.samp = o.mode;
+        sh.sampler = o.anisotropy ? 2 : 1;
+        sh.maxAniso = std::min(16, std::max(1, (int)std::ceil(o.maxAnisotropy)));
+    }
+    sh.sampGeneric = (sh.samp.addrU != kAddrRepeat || sh.samp.addrV != kAddrRepeat || sh.samp.nearest) ? 1 : 0;
     // the normal-Lambert program samples no texture
     sh.numTextures = d.kind == IFHIP_FS_NORMAL_LAMBERT ? 0 : (int)textures_.size();
     sh.ambient = d.ambient;

@@ -7,6 +7,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -67,10 +68,20 @@ struct GpuFragmentProgram {
 };
 
 struct TextureSlot {
-    int width = 0, height = 0, levels = 0;
+    int width = 0, height = 0, levels = 0, layers = 1;
     uint8_t* dev = nullptr;
-    std::vector<size_t> levelOffset;  // texel offsets
+    std::vector<size_t> levelOffset;  // texel offset of each level's first layer
     size_t bytes = 0;
+    size_t levelTexels(int l) const {  // texels of one layer of level l
+        return (size_t)std::max(1, width >> l) * (size_t)std::max(1, height >> l);
+    }
+};
+
+// CreateSampler's state (IfritSamplerT, Structures.h:52-61).
+struct SamplerObject {
+    SamplerMode mode;
+    bool anisotropy = false;
+    float maxAnisotropy = 1.0f;
 };
 
 // Grow-only device allocation.  Every (re)allocation bumps `generation`, so
@@ -120,7 +131,10 @@ public:
     void bindVertexDataFloat(const float* data, int strideFloats, int numVertices);
     void bindIndexData(const int32_t* data, int numIndices);
     void setRenderTarget(int width, int height);
-    int createTexture(int w, int h, int levels);
+    int createTexture(int w, int h, int levels, int layers = 1);
+    void copyHostBufferToImageLayer(int tex, int layer, const void* rgba8);
+    void blitImage(int src, int dst, const IfritImageBlit& region, int filter);
+    int CreateSampler(const IfritSamplerDesc& d);
     void copyHostBufferToImage(int tex, const void* rgba8);
     void uploadTextureLevels(int tex, const void* rgba8);
     void generateFbmTexture(int tex, uint32_t seed);
@@ -247,6 +261,7 @@ private:
     StatPartials statPtrs() const;
     int rasterGrid() const;
     std::vector<TextureSlot> textures_;
+    std::vector<SamplerObject> samplers_;
     bool texDescDirty_ = true;
     uint8_t* texPool_ = nullptr;  // lowest texture allocation: TexDesc bases are byte offsets from it
 

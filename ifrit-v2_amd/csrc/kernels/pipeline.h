@@ -139,9 +139,20 @@ struct RasterParams {
 
 struct TexDesc {
     unsigned long long base;  // byte offset of level 0 in the texture pool
-    int width, height, levels, pad;
-    // levels follow level 0 packed, each max(1, width >> l) x max(1, height >> l);
-    // the sampler computes their offsets from width and height
+    int width, height, levels, layers;
+    // levels follow level 0 packed, each max(1, width >> l) x max(1, height >> l) texels per
+    // layer, the layers of a level one after another (a cube map has six, +x -x +y -y +z -z);
+    // the sampler computes the offsets from width, height and layers
+};
+
+// Sampler state of a program (IfritSamplerT, include/ifrit/softgraphics/engine/base/Structures.h:52-61)
+// beyond the level-of-detail choice: addressing per axis, the filter within a level and the border.
+enum AddressMode { kAddrRepeat = 0, kAddrMirroredRepeat = 1, kAddrClampToEdge = 2, kAddrClampToBorder = 3,
+                   kAddrMirrorClampToEdge = 4 };
+struct SamplerMode {
+    int addrU, addrV;  // AddressMode
+    int nearest;       // 1: nearest texel within a level (IF_FILTER_NEAREST), 0: bilinear
+    unsigned border;   // RGBA8 border colour: 0xff000000 opaque black, 0xffffffff opaque white
 };
 
 struct ShadeParams {
@@ -164,6 +175,9 @@ struct ShadeParams {
     float clearColor[4];     // colour of samples no triangle covers (the resolve weighs them)
     float clearDepth;        // depth a cleared pixel starts from (blending draws)
     Scissor scissor;
+    SamplerMode samp;        // addressing and filter of the program's lookups
+    int sampGeneric;         // 1: samp differs from repeat + bilinear (the sampled-kernel variant runs)
+    float lod;               // cube-map program: the level of detail of its lookups
 };
 
 // Blend factors, each encoded (c0, c1, c2, c3) as the reference does
@@ -289,5 +303,17 @@ void launch_blend_ordered(const ShadeParams& p, const BlendCoefs& b, int depthFu
 void launch_digest(const uint32_t* data, size_t n, unsigned long long* out, hipStream_t s);
 void launch_fbm_texture(uint8_t* level0, int size, unsigned seed, hipStream_t s);
 void launch_mip_level(const uint8_t* src, int sw, int sh, uint8_t* dst, int dw, int dh, hipStream_t s);
+// blitImage (TileRasterImageOpInvocationsCuda.cu:36-131) between RGBA8 levels: destination texel
+// (dx0 + x, dy0 + y) for x < dx1 - dx0, y < dy1 - dy0 reads the source at
+// (x / (dx1 - dx0) (sx1 - sx0) + sx0, likewise y); linear: bilinear with the right and lower
+// neighbours clamped to the source level, on the 0..255 values, rounded half up; nearest:
+// the texel at the rounded coordinate, clamped.
+struct BlitArgs {
+    int sw, sh, dw, dh;               // level sizes
+    unsigned sx0, sy0, sx1, sy1;      // source region
+    unsigned dx0, dy0, dx1, dy1;      // destination region
+    int linear;
+};
+void launch_blit(const uint8_t* src, uint8_t* dst, const BlitArgs& a, hipStream_t s);
 
 }  // namespace ifrit_hip

@@ -454,6 +454,12 @@ __device__ __forceinline__ void clear_tile(const ViewGeom& G, float clearDepth, 
     }
 }
 
+// LDS layout of the tile's keys: sample-major, each sample's 256 pixels
+// padded by one key after every 32 (a 64-bit key covers two of the 64 LDS
+// banks, so without the padding pixels 32 apart share banks).
+constexpr int kVisStride = 264;
+__device__ __forceinline__ int vis_at(int s, int p) { return s * kVisStride + p + (p >> 5); }
+
 // NT threads per tile: 256 (one pixel per thread) or, at 1 sample per pixel,
 // 64 (one wave per tile, four pixels per lane: more tiles in flight per
 // compute unit, for many small tiles).  Passes hold NT / 2 entries.  Held to
@@ -470,7 +476,9 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
     constexpr int PAIRS = S == 1 ? 1 : 16 * PASS;     // (entry, block) pairs of a pass (S > 1)
     constexpr int PPT = 256 / NT;           // tile pixels per thread
     static_assert(PASS <= kPass, "units encode the entry in 7 bits");
-    __shared__ unsigned long long sVis[S * 256];  // key of sample s of tile pixel p at s * 256 + p
+    // key of sample s of tile pixel p at vis_at(s, p): one 8-byte slot of padding after every 32
+    // pixels, so that pixels two rows apart (32 keys, 256 bytes) fall in other LDS banks
+    __shared__ unsigned long long sVis[S * kVisStride];
     static_assert(sizeof(TileOut<S>) <= sizeof(sVis), "output staging reuses sVis");
     __shared__ TriRec sRec[PASS];
     __shared__ unsigned char sTa[2 * PASS];  // accepted blocks of each half
@@ -533,7 +541,7 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
             const PixelCtx C = pixel_ctx(G, p & 15, p >> 4);
 #pragma unroll
             for (int s = 0; s < S; s++)
-                sVis[s * 256 + p] = frag_key(P.clear || !C.inFb ? P.clearDepth : visDepth[C.pix * S + s], 0u);
+                sVis[vis_at(s, p)] = frag_key(P.clear || !C.inFb ? P.clearDepth : visDepth[C.pix * S + s], 0u);
         }
 
         // two lanes per list entry: lane h = t & 1 takes block rows 2h, 2h + 1
@@ -584,7 +592,7 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
                         const unsigned cls = acc == 0xFFu ? 1u : (quad == 0xFu ? 2u : 3u);
                         frags++;
                         covs++;
-                        merge(&sVis[(py - ty0) * 16 + (px - tx0)],
+                        merge(&sVis[vis_at(0, (py - ty0) * 16 + (px - tx0))],
                               class_depth(cls, R.f1, R.f2, R.f3, (float)px, (float)py), (tie << 2) | cls);
                     }
                 } else {
@@ -599,7 +607,7 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
                         if (!accept && !inside(E, sx, sy)) continue;
                         any = true;
                         covs++;
-                        merge(&sVis[s * 256 + c], depth256(R.f1, R.f2, R.f3, sx, sy), tie);
+                        merge(&sVis[vis_at(s, c)], depth256(R.f1, R.f2, R.f3, sx, sy), tie);
                     }
                     frags += any;
                 }
@@ -707,13 +715,13 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
 #pragma unroll
             for (int k = 0; k < PPT; k++)
 #pragma unroll
-                for (int s = 0; s < S; s++) g[s * 256 + t + k * NT] = sVis[s * 256 + t + k * NT];
+                for (int s = 0; s < S; s++) g[s * 256 + t + k * NT] = sVis[vis_at(s, t + k * NT)];
         } else {
             unsigned long long keys[PPT][S];
 #pragma unroll
             for (int k = 0; k < PPT; k++)
 #pragma unroll
-                for (int s = 0; s < S; s++) keys[k][s] = sVis[s * 256 + t + k * NT];
+                for (int s = 0; s < S; s++) keys[k][s] = sVis[vis_at(s, t + k * NT)];
             if (P.clear) {
                 if constexpr (S > 1) {
                     // a pixel's samples are whole 16-byte words: straight out

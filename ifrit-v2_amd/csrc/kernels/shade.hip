@@ -152,6 +152,53 @@ __device__ __forceinline__ BilinearTexels bilinear_fetch(const uchar4* __restric
     return B;
 }
 
+// Texel index i of a level n texels wide under an addressing mode (n a power
+// of two), or -1 where clamp-to-border gives the border colour.  Mirrored
+// repeat reflects every other period; mirror-clamp-to-edge reflects once
+// about 0 and then clamps (the Vulkan definitions; the CUDA spec's
+// mirrorclamp, ShaderOpsCuda.cuh:36-37, swaps the ends instead of mirroring).
+__device__ __forceinline__ int texel_index(int i, int n, int mode) {
+    switch (mode) {
+        case kAddrMirroredRepeat: {
+            const int m = i & (2 * n - 1);
+            return m >= n ? 2 * n - 1 - m : m;
+        }
+        case kAddrClampToEdge: return min(max(i, 0), n - 1);
+        case kAddrClampToBorder: return (i < 0 || i >= n) ? -1 : i;
+        case kAddrMirrorClampToEdge: return min(i < 0 ? -1 - i : i, n - 1);
+        default: return i & (n - 1);
+    }
+}
+
+// bilinear_fetch under a sampler's addressing and filter: nearest takes the
+// texel holding (u, v), floor(u w); border texels are the border colour.
+__device__ __forceinline__ BilinearTexels bilinear_fetch_mode(const uchar4* __restrict__ T, int lw, int lh, float u,
+                                                              float v, const SamplerMode& m) {
+    const int w = 1 << lw, h = 1 << lh;
+    const unsigned* const U = reinterpret_cast<const unsigned*>(T);
+    auto at = [&](int x, int y) {
+        const int ix = texel_index(x, w, m.addrU), iy = texel_index(y, h, m.addrV);
+        return (ix < 0 || iy < 0) ? m.border : U[(iy << lw) + ix];
+    };
+    BilinearTexels B;
+    if (m.nearest) {
+        B.t00 = at((int)floorf(u * (float)w), (int)floorf(v * (float)h));
+        B.t10 = B.t01 = B.t11 = B.t00;
+        B.ax = B.ay = 0.0f;
+        return B;
+    }
+    const float s = fmaf(u, (float)w, -0.5f), r = fmaf(v, (float)h, -0.5f);
+    const float fs = floorf(s), fr = floorf(r);
+    const int x0 = (int)fs, y0 = (int)fr;
+    B.t00 = at(x0, y0);
+    B.t10 = at(x0 + 1, y0);
+    B.t01 = at(x0, y0 + 1);
+    B.t11 = at(x0 + 1, y0 + 1);
+    B.ax = s - fs;
+    B.ay = r - fr;
+    return B;
+}
+
 __device__ __forceinline__ float4 texel_f4(unsigned t) {
     return make_float4((float)(t & 255u), (float)((t >> 8) & 255u), (float)((t >> 16) & 255u), (float)(t >> 24));
 }
@@ -179,6 +226,15 @@ __device__ __forceinline__ TrilinearTexels trilinear_fetch(const TriLevels& L, f
     return T;
 }
 
+__device__ __forceinline__ TrilinearTexels trilinear_fetch_mode(const TriLevels& L, float u, float v,
+                                                                const SamplerMode& m) {
+    TrilinearTexels T;
+    T.b0 = bilinear_fetch_mode(L.t0, (int)(L.dims & 31u), (int)((L.dims >> 5) & 31u), u, v, m);
+    T.b1 = bilinear_fetch_mode(L.t0 + L.d1, (int)((L.dims >> 10) & 31u), (int)((L.dims >> 15) & 31u), u, v, m);
+    T.f = L.f;
+    return T;
+}
+
 __device__ __forceinline__ float4 trilinear_blend(const TrilinearTexels& T) {
     constexpr float k = 1.0f / 255.0f;
     const float4 c = lerp4(bilinear_blend(T.b0), bilinear_blend(T.b1), T.f);
@@ -191,6 +247,48 @@ __device__ __forceinline__ float4 trilinear_levels(const TriLevels& L, float u, 
 
 __device__ float4 trilinear(const uint8_t* pool, const TexDesc& D, float u, float v, float lod) {
     return trilinear_levels(tri_levels(pool, tex_head(D), lod), u, v);
+}
+
+// Cube-map lookup (textureCubeLod, ShaderBuiltinCuda.cuh:184-231): the face
+// and its (u, v) from the direction's major axis (0 +x, 1 -x, 2 +y, 3 -y,
+// 4 +z, 5 -z, the Vulkan face table), then a trilinear lookup in that face at
+// the given level of detail under the sampler's addressing and filter.  The
+// texture holds six layers per level.
+__device__ float4 cube_lookup(const uint8_t* pool, const TexDesc& D, const SamplerMode& m, float3 d, float lod) {
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int face;
+    float ma, sc, tc;
+    if (az >= ax && az >= ay) {
+        face = d.z < 0.0f ? 5 : 4;
+        ma = az;
+        sc = d.z < 0.0f ? -d.x : d.x;
+        tc = -d.y;
+    } else if (ay >= ax) {
+        face = d.y < 0.0f ? 3 : 2;
+        ma = ay;
+        sc = d.x;
+        tc = d.y < 0.0f ? -d.z : d.z;
+    } else {
+        face = d.x < 0.0f ? 1 : 0;
+        ma = ax;
+        sc = d.x < 0.0f ? d.z : -d.z;
+        tc = -d.y;
+    }
+    const float norm = 0.5f / ma;
+    const float u = sc * norm + 0.5f, v = tc * norm + 0.5f;
+    const TexHead H = tex_head(D);
+    lod = fminf(fmaxf(lod, 0.0f), (float)(H.levels - 1));
+    const int l0 = (int)floorf(lod);
+    const int l1 = lod > (float)l0 ? min(l0 + 1, H.levels - 1) : l0;
+    const int w0 = max(H.lw - l0, 0), h0 = max(H.lh - l0, 0), w1 = max(H.lw - l1, 0), h1 = max(H.lh - l1, 0);
+    const uchar4* base = reinterpret_cast<const uchar4*>(pool + H.base);
+    const uchar4* t0 = base + (size_t)D.layers * level_offset(H.lw, H.lh, l0) + ((size_t)face << (w0 + h0));
+    const uchar4* t1 = base + (size_t)D.layers * level_offset(H.lw, H.lh, l1) + ((size_t)face << (w1 + h1));
+    TrilinearTexels T;
+    T.b0 = bilinear_fetch_mode(t0, w0, h0, u, v, m);
+    T.b1 = bilinear_fetch_mode(t1, w1, h1, u, v, m);
+    T.f = lod - (float)l0;
+    return trilinear_blend(T);
 }
 
 // Footprint of a textured fragment: number of taps along the major axis,
@@ -222,6 +320,8 @@ __device__ __forceinline__ TexFootprint footprint(const TexHead& D, int sampler,
     F.av = px >= py ? dvdx : dvdy;
     return F;
 }
+
+__device__ __forceinline__ float sat(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
 
 __device__ __forceinline__ uint32_t flat_hash(uint32_t prim) {
     uint32_t h = prim * 0x9E3779B1u;
@@ -277,7 +377,8 @@ __device__ __forceinline__ float3 ggx_light(float3 n, float3 V, float3 L, float4
 // frame = false for callers that never run the programs from 3 on (layered GGX
 // and the neural material): their shading-frame paths then fold away.
 __device__ FragPrep prep_fragment(const ShadeParams& P, float3 cam, const ShadeTri& T, const float* vtx,
-                                  const int* idx, const TexDesc* tex, int px, int py, float o, bool frame) {
+                                  const int* idx, const TexDesc* tex, const uint8_t* pool, int px, int py, float o,
+                                  bool frame, bool generic = false) {
     const bool framed = frame && P.program >= 3;
     FragPrep R;
     R.mat = -1;
@@ -353,6 +454,15 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, float3 cam, const ShadeT
         R.fp.au = R.fp.av = 0.0f;
     }
     if (framed) return R;  // lighting needs the sampled layers (or the network)
+    if (generic && P.program == 8 && R.mat >= 0) {
+        // cube-map lookup along the interpolated normal at the program's level of detail
+        const float4 c = cube_lookup(pool, tex[R.mat], P.samp, N, P.lod);
+        R.A = make_float3(0, 0, 0);
+        R.B = make_float3(c.x, c.y, c.z);
+        R.alpha = sat(c.w);
+        R.mat = -1;  // no anisotropic taps
+        return R;
+    }
     if (P.program == 7) {
         // Lambert plus a (n.l)^shininess highlight from the interpolated normal and one directional
         // light given unnormalised, untextured: the reference CPU harness's shader
@@ -402,7 +512,6 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, float3 cam, const ShadeT
     return R;
 }
 
-__device__ __forceinline__ float sat(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
 
 // Clamped colour of a layered-GGX fragment from its filtered layers: albedo,
 // tangent-space normal (xyz, encoded as 0.5 n + 0.5) and roughness (w).
@@ -470,7 +579,7 @@ constexpr int kTapWindow = 128;
 //      fragments in sample order, so results do not depend on scheduling.
 // (ox, oy) is the sub-tile's first pixel in its view; visSlot, rgba8 and
 // rgba32f point at the view's outputs and pixBase is the view's first pixel.
-template <int S, bool NEURAL, bool FP8, bool LAYERED>
+template <int S, bool NEURAL, bool FP8, bool LAYERED, bool GENERIC = false>
 __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, int width, int height,
                                               unsigned fanStart, int ox, int oy, size_t pixBase, const TriRecs& recs,
                                               const float4* __restrict__ bary, const uint32_t* __restrict__ visSlot,
@@ -576,7 +685,7 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
             ShadeTri T;
             load_shade_tri(recs, bary, fanStart, slot, T, P.swapped);
             const int qx = ox + (fl & 7), qy = oy + (fl >> 3);
-            R = prep_fragment(P, cam, T, vtx, idx, tex, qx, qy, o, NEURAL || LAYERED);
+            R = prep_fragment(P, cam, T, vtx, idx, tex, pool, qx, qy, o, NEURAL || LAYERED, GENERIC);
             if (R.mat >= 0) {
                 n = R.fp.n;
                 cUV[lane] = make_float4(R.u, R.v, R.fp.au, R.fp.av);
@@ -622,6 +731,7 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
                     L.d1 = q.z;
                     L.dims = q.w;
                     L.f = cLod[fl];
+                    if constexpr (GENERIC) return trilinear_fetch_mode(L, tu, tv, P.samp);
                     return trilinear_fetch(L, tu, tv);
                 };
                 for (int j = lane; j < nt; j += 64) tCol[j] = trilinear_blend(tap_texels(j));
@@ -712,7 +822,7 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
                 cLod[lane] = vt.y;
                 xdir[lane] = make_float4(vt.z, lt.x, lt.y, lt.z);
             } else {
-                cCol[lane] = P.program == 0 || P.program == 6
+                cCol[lane] = P.program == 0 || P.program == 6 || (GENERIC && P.program == 8)
                                  ? make_float4(sat(R.B.x), sat(R.B.y), sat(R.B.z), R.alpha)
                                  : make_float4(sat(fmaf(alb.x, R.A.x, R.B.x)),
                                                sat(fmaf(alb.y, R.A.y, R.B.y)), sat(fmaf(alb.z, R.A.z, R.B.z)),
@@ -762,7 +872,7 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
 // 8N), its 8x8 sub-tiles in row order; consecutive tiles on one XCD share its
 // L2 (texture and vertex reuse).  Tiles of all shaded views are numbered one
 // after another.
-template <int S, bool NEURAL, bool FP8, int N, bool LAYERED>
+template <int S, bool NEURAL, bool FP8, int N, bool LAYERED, bool GENERIC = false>
 __device__ __forceinline__ void shade_tiles(const ShadeParams& P, const Views& V, const TriRecs& recs,
                                             const float4* __restrict__ bary, const uint32_t* __restrict__ visSlot,
                                             const float* __restrict__ vtx, const int* __restrict__ idx,
@@ -780,14 +890,16 @@ __device__ __forceinline__ void shade_tiles(const ShadeParams& P, const Views& V
     const float3 cam = make_float3(D.camera[0], D.camera[1], D.camera[2]);
     const unsigned fanStart = D.slotBase + (unsigned)P.ntri;
     if constexpr (N == 1) {
-        shade_subtile<S, NEURAL, FP8, LAYERED>(P, cam, width, height, fanStart, ox0, oy0, pb, recs, bary, visSlot + pb * S,
+        shade_subtile<S, NEURAL, FP8, LAYERED, GENERIC>(P, cam, width, height, fanStart, ox0, oy0, pb, recs, bary,
+                                                        visSlot + pb * S,
                                       vtx, idx, tex, pool, rgba8 + pb, rgba32f ? rgba32f + pb : nullptr);
     } else {
 #pragma unroll 1
         for (int k = 0; k < N * N; k++) {
             const int ox = ox0 + (k % N) * 8, oy = oy0 + (k / N) * 8;
             if (ox >= width || oy >= height) continue;
-            shade_subtile<S, NEURAL, FP8, LAYERED>(P, cam, width, height, fanStart, ox, oy, pb, recs, bary, visSlot + pb * S,
+            shade_subtile<S, NEURAL, FP8, LAYERED, GENERIC>(P, cam, width, height, fanStart, ox, oy, pb, recs, bary,
+                                                            visSlot + pb * S,
                                           vtx, idx, tex, pool, rgba8 + pb, rgba32f ? rgba32f + pb : nullptr);
         }
     }
@@ -798,7 +910,7 @@ __device__ __forceinline__ void shade_tiles(const ShadeParams& P, const Views& V
 // order), so an opaque fragment gets the same colour either way.
 __device__ float4 shade_fragment(const ShadeParams& P, float3 cam, const ShadeTri& T, const float* vtx, const int* idx,
                                  const TexDesc* tex, const uint8_t* pool, int px, int py) {
-    const FragPrep R = prep_fragment(P, cam, T, vtx, idx, tex, px, py, 0.0f, true);
+    const FragPrep R = prep_fragment(P, cam, T, vtx, idx, tex, pool, px, py, 0.0f, true);
     if (P.program == 0 || P.program == 6) return make_float4(sat(R.B.x), sat(R.B.y), sat(R.B.z), R.alpha);
     float4 acc = make_float4(0, 0, 0, 0), acc2 = make_float4(0, 0, 0, 0);
     const int n = R.mat >= 0 ? R.fp.n : 0;
@@ -948,6 +1060,19 @@ k_shade(ShadeParams P, Views V, TriRecs recs, const float4* __restrict__ bary, c
                                            perXcd);
 }
 
+// The single-texture and cube-map programs under a sampler other than
+// repeat + bilinear (address modes, border colours, nearest texels): a
+// variant of k_shade, so that the default sampler's kernel carries none of it.
+template <int S, int N>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_shade_sampled(ShadeParams P, Views V, TriRecs recs, const float4* __restrict__ bary,
+                const uint32_t* __restrict__ visSlot, const float* __restrict__ vtx, const int* __restrict__ idx,
+                const TexDesc* __restrict__ tex, const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8,
+                float4* __restrict__ rgba32f, int tiles, int perXcd) {
+    shade_tiles<S, false, false, N, false, true>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles,
+                                                 perXcd);
+}
+
 // Layered GGX (programs 3 and 5): four waves per SIMD (at most 128 VGPRs), as
 // the neural program.
 template <int S, int N>
@@ -995,6 +1120,8 @@ void launch_shade(const ShadeParams& p, const Views& views, int shadeTiles, TriR
     do {                                                                                                            \
         if (p.program == 3 || p.program == 5)                                                                       \
             hipLaunchKernelGGL((k_shade_layered<SN, NN>), grid, dim3(64), 0, s, IFRIT_ARGS);                        \
+        else if (p.sampGeneric || p.program == 8)                                                                   \
+            hipLaunchKernelGGL((k_shade_sampled<SN, NN>), grid, dim3(64), 0, s, IFRIT_ARGS);                        \
         else if (p.program != 4)                                                                                    \
             hipLaunchKernelGGL((k_shade<SN, NN>), grid, dim3(64), 0, s, IFRIT_ARGS);                                \
         else if (p.neural.precision == 1)                                                                           \

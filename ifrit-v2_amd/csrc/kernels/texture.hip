@@ -73,7 +73,44 @@ __global__ void __launch_bounds__(256) k_mip(const uchar4* __restrict__ src, int
                                           (unsigned char)((a.w + b.w + c.w + d.w + 2) >> 2));
 }
 
+__global__ void __launch_bounds__(256) k_blit(const uchar4* __restrict__ src, uchar4* __restrict__ dst, BlitArgs a) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int rw = (int)(a.dx1 - a.dx0), rh = (int)(a.dy1 - a.dy0);
+    if (x >= rw || y >= rh) return;
+    const float px = (float)x / (float)rw, py = (float)y / (float)rh;
+    const float cx = px * (float)(a.sx1 - a.sx0) + (float)a.sx0;
+    const float cy = py * (float)(a.sy1 - a.sy0) + (float)a.sy0;
+    uchar4 out;
+    if (a.linear) {
+        const int ix = (int)cx, iy = (int)cy;
+        const float fx = cx - (float)ix, fy = cy - (float)iy;
+        const int ix1 = min(ix + 1, a.sw - 1), iy1 = min(iy + 1, a.sh - 1);
+        const uchar4 c00 = src[(size_t)iy * a.sw + ix], c01 = src[(size_t)iy * a.sw + ix1];
+        const uchar4 c10 = src[(size_t)iy1 * a.sw + ix], c11 = src[(size_t)iy1 * a.sw + ix1];
+        auto lerp = [](float p, float q, float t) { return p + (q - p) * t; };
+        auto ch = [&](float a00, float a01, float a10, float a11) {
+            const float v = lerp(lerp(a00, a01, fx), lerp(a10, a11, fx), fy);
+            return (unsigned char)fminf(fmaxf(floorf(v + 0.5f), 0.0f), 255.0f);
+        };
+        out = make_uchar4(ch(c00.x, c01.x, c10.x, c11.x), ch(c00.y, c01.y, c10.y, c11.y),
+                          ch(c00.z, c01.z, c10.z, c11.z), ch(c00.w, c01.w, c10.w, c11.w));
+    } else {
+        const int sx = min((int)roundf(cx), a.sw - 1), sy = min((int)roundf(cy), a.sh - 1);
+        out = src[(size_t)sy * a.sw + sx];
+    }
+    dst[(size_t)(a.dy0 + y) * a.dw + a.dx0 + x] = out;
+}
+
 }  // namespace
+
+void launch_blit(const uint8_t* src, uint8_t* dst, const BlitArgs& a, hipStream_t s) {
+    const int rw = (int)(a.dx1 - a.dx0), rh = (int)(a.dy1 - a.dy0);
+    if (rw <= 0 || rh <= 0) return;
+    const dim3 grid((rw + 15) / 16, (rh + 15) / 16);
+    hipLaunchKernelGGL(k_blit, grid, dim3(256), 0, s, reinterpret_cast<const uchar4*>(src), reinterpret_cast<uchar4*>(dst),
+                       a);
+}
 
 void launch_fbm_texture(uint8_t* level0, int size, unsigned seed, hipStream_t s) {
     int log2size = 0;

@@ -28,7 +28,20 @@ class FragmentProgramDesc(C.Structure):
                 ("trisPerMaterial", C.c_int32), ("sampler", C.c_int32), ("maxAniso", C.c_int32),
                 ("camera", C.c_float * 3), ("ambient", C.c_float), ("kd", C.c_float), ("ks", C.c_float),
                 ("shininess", C.c_float), ("numLights", C.c_int32), ("lightPos", (C.c_float * 4) * 4),
-                ("lightColor", (C.c_float * 4) * 4), ("offTangent", C.c_int32), ("precision", C.c_int32)]
+                ("lightColor", (C.c_float * 4) * 4), ("offTangent", C.c_int32), ("precision", C.c_int32),
+                ("samplerId", C.c_int32), ("lod", C.c_float)]
+
+
+class SamplerDesc(C.Structure):
+    _fields_ = [("filterMode", C.c_int32), ("addressModeU", C.c_int32), ("addressModeV", C.c_int32),
+                ("addressModeW", C.c_int32), ("borderColor", C.c_int32), ("anisotropyEnable", C.c_int32),
+                ("maxAnisotropy", C.c_float)]
+
+
+class ImageBlit(C.Structure):
+    _fields_ = [("srcMip", C.c_uint32), ("srcLayer", C.c_uint32), ("srcSt", C.c_uint32 * 3), ("srcEd", C.c_uint32 * 3),
+                ("dstMip", C.c_uint32), ("dstLayer", C.c_uint32), ("dstSt", C.c_uint32 * 3),
+                ("dstEd", C.c_uint32 * 3)]
 
 
 class NeuralTrainDesc(C.Structure):
@@ -110,7 +123,8 @@ EXPORTED_SYMBOLS = [
     "iftrSetViews", "iftrGetViewVisibility", "iftrGetViewColorRGBA8", "iftrSetGraphMode", "iftrSetTuning",
     "iftrGetTuning", "iftrAutotune", "iftrGetDigest", "iftrSetColorFloatOutput", "iftrGetColorRGBA32F",
     "iftrDrawNoAlloc", "iftrNeuralValidateEx", "iftrSetCullMode", "iftrSetClearValues", "iftrSetScissors",
-    "iftrSetScissorTestEnable",
+    "iftrSetScissorTestEnable", "iftrCreateTextureLayers", "iftrCopyHostBufferToImageLayer", "iftrCreateSampler",
+    "iftrBlitImage",
 ]
 
 
@@ -183,6 +197,10 @@ def lib():
         "iftrSetCullMode": ([vp, C.c_int], None),
         "iftrSetClearValues": ([vp, vp, C.c_size_t, C.c_float], None),
         "iftrSetScissors": ([vp, vp, C.c_size_t], None), "iftrSetScissorTestEnable": ([vp, C.c_int], None),
+        "iftrCreateTextureLayers": ([vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
+        "iftrCopyHostBufferToImageLayer": ([vp, C.c_int, C.c_int, vp], None),
+        "iftrCreateSampler": ([vp, C.POINTER(SamplerDesc)], C.c_int),
+        "iftrBlitImage": ([vp, C.c_int, C.c_int, C.POINTER(ImageBlit), C.c_int], None),
         "iftrNeuralValidateEx": ([vp, C.c_int, C.c_float, C.c_int, C.c_int, C.POINTER(NeuralValidation)], None),
         "iftrOptsetDepthTestEnable": ([vp, C.c_int], None),
     }
@@ -294,10 +312,28 @@ class Renderer:
         self.num_indices = int(i.size)
         self._check()
 
+    def create_sampler(self, filter=1, address_u=0, address_v=0, border=0, anisotropy=False, max_anisotropy=1.0):
+        '''CreateSampler: IF_FILTER_* within a level, IF_SAMPLER_ADDRESS_MODE_* per axis, IF_BORDER_COLOR_*,
+        anisotropic taps up to max_anisotropy.  Returns the sampler id.'''
+        d = SamplerDesc(filter, address_u, address_v, 0, border, 1 if anisotropy else 0, max_anisotropy)
+        sid = self.L.iftrCreateSampler(self.h, C.byref(d))
+        self._check()
+        return sid
+
+    def blit(self, src, dst, src_region, dst_region, src_level=0, dst_level=0, src_layer=0, dst_layer=0, linear=True):
+        '''blitImage: regions are (x0, y0, x1, y1) of the given levels and layers.'''
+        b = ImageBlit(src_level, src_layer, (C.c_uint32 * 3)(src_region[0], src_region[1], 0),
+                      (C.c_uint32 * 3)(src_region[2], src_region[3], 1), dst_level, dst_layer,
+                      (C.c_uint32 * 3)(dst_region[0], dst_region[1], 0), (C.c_uint32 * 3)(dst_region[2], dst_region[3], 1))
+        self.L.iftrBlitImage(self.h, src, dst, C.byref(b), 1 if linear else 0)
+        self._check()
+
     def bind_fragment_program(self, scene, program=None, precision=0):
         d = FragmentProgramDesc()
         d.kind = scene.program if program is None else program
         d.precision = precision
+        d.samplerId = getattr(scene, "sampler_id", 0)
+        d.lod = getattr(scene, "lod", 0.0)
         d.offWorld, d.offNormal, d.offUv = 0, 3, 6
         d.offTangent = 8 if scene.stride >= 11 else 0
         d.trisPerMaterial = scene.tris_per_material
@@ -362,7 +398,7 @@ class Renderer:
         return out[:, :3]
 
     def add_texture(self, tex) -> int:
-        tid = self.L.iftrCreateTexture(self.h, tex.width, tex.height, tex.levels)
+        tid = self.L.iftrCreateTextureLayers(self.h, tex.width, tex.height, tex.levels, getattr(tex, "layers", 1))
         self._check()
         if tex.texels is not None:
             arr = np.ascontiguousarray(tex.texels, np.uint8)
@@ -374,7 +410,7 @@ class Renderer:
         return tid
 
     def download_texture(self, tid: int, tex) -> np.ndarray:
-        n = sum(max(1, tex.width >> l) * max(1, tex.height >> l) for l in range(tex.levels)) * 4
+        n = sum(max(1, tex.width >> l) * max(1, tex.height >> l) for l in range(tex.levels)) * 4 * getattr(tex, "layers", 1)
         out = np.empty((n,), np.uint8)
         self.L.iftrDownloadTexture(self.h, tid, _p(out))
         self._check()
@@ -386,6 +422,8 @@ class Renderer:
         self.bind_geometry(scene.vertices, scene.indices, scene.mvp)
         for t in scene.textures:
             self.add_texture(t)
+        if scene.sampler_state is not None:
+            scene.sampler_id = self.create_sampler(**scene.sampler_state)
         self.bind_fragment_program(scene)
 
     def draw(self, clear: bool = True):

@@ -27,6 +27,7 @@ PROGRAM_DEPTH_ONLY = 2
 PROGRAM_GGX_LAYERED = 3
 PROGRAM_FLAT_ALPHA = 6
 PROGRAM_NORMAL_LAMBERT = 7
+PROGRAM_CUBE_MAP = 8
 SAMPLER_BILINEAR = 0
 SAMPLER_TRILINEAR = 1
 SAMPLER_ANISO = 2
@@ -78,8 +79,10 @@ class Texture:
     width: int
     height: int
     levels: int
-    texels: np.ndarray | None = None      # uint8, all levels packed, or None when generated on the device
+    texels: np.ndarray | None = None      # uint8, all levels packed (layers one after another per level),
+                                          # or None when generated on the device
     fbm_seed: int | None = None           # device-generated fBm texture seed
+    layers: int = 1                       # 6 for a cube map (+x -x +y -y +z -z)
 
 
 @dataclasses.dataclass
@@ -102,6 +105,9 @@ class Scene:
     shininess: float = 32.0
     lights: list = dataclasses.field(default_factory=list)   # [(pos4, color4)]
     textures: list = dataclasses.field(default_factory=list)
+    sampler_state: dict | None = None   # Renderer.create_sampler arguments (None: the program's own sampler)
+    sampler_id: int = 0                 # set when the scene is loaded (0: none)
+    lod: float = 0.0                    # cube-map program: level of detail
 
     @property
     def stride(self) -> int:
@@ -485,3 +491,36 @@ def reference_workload(width: int = 1920, height: int = 1080, spheres: int = 256
     """The config-2 geometry at one sample per pixel with the normal-Lambert
     shading: the frame the reference CPU rasterizer is timed on."""
     return normal_lambert(config2(width, height, 1, spheres=spheres, level=level))
+
+
+def layered_mips(faces) -> np.ndarray:
+    """The packed chain of a layered texture: each layer's box-filtered mips
+    (build_mips), level by level with the layers of a level one after another
+    (the library's layout; a cube map's layers are +x -x +y -y +z -z)."""
+    chains = [build_mips(f) for f in faces]
+    h, w = faces[0].shape[:2]
+    out, off = [], 0
+    for l in range(num_levels(w, h)):
+        n = max(1, w >> l) * max(1, h >> l) * 4
+        out.extend(c[off:off + n] for c in chains)
+        off += n
+    return np.concatenate(out)
+
+
+def cube_texture(size: int = 64, seed: int = 11) -> Texture:
+    """A seeded cube map: per face an fBm pattern tinted by a face-specific colour, with mips."""
+    tints = [(255, 80, 80), (80, 255, 80), (80, 80, 255), (255, 255, 80), (255, 80, 255), (80, 255, 255)]
+    faces = []
+    for k, tint in enumerate(tints):
+        base = fbm_texture(size, seed + 17 * k).astype(np.int32)
+        f = (base * np.array(tint + (255,), np.int32)) // 255
+        faces.append(f.astype(np.uint8))
+    return Texture(size, size, num_levels(size, size), layered_mips(faces), None, layers=6)
+
+
+def sphere_scene(width: int = 256, height: int = 192, samples: int = 1, seg: int = 48) -> Scene:
+    """One UV sphere in front of the camera (normals cover every direction), for the cube-map program."""
+    sv, si = uv_sphere(seg, seg // 2, 1.0)
+    eye = (0.0, 0.3, -3.0)
+    return Scene("sphere", width, height, samples, sv, si, _mvp(eye, (0, 0, 0), 50.0, width, height),
+                 program=PROGRAM_CUBE_MAP, camera=eye)

@@ -144,10 +144,13 @@ enum {
     IFHIP_FS_NEURAL = 4,      /* the neural material trained by iftrNeuralTrain (same inputs as
                                  IFHIP_FS_GGX_LAYERED; light 0 only) */
     IFHIP_FS_FLAT_ALPHA = 6,  /* IFHIP_FS_FLAT with alpha = (64 + (hash byte 3 & 127)) / 255 */
-    IFHIP_FS_NORMAL_LAMBERT = 7 /* untextured: d = max(0, n.l / |n|) with n the interpolated normal and
+    IFHIP_FS_NORMAL_LAMBERT = 7, /* untextured: d = max(0, n.l / |n|) with n the interpolated normal and
                                    l = lightPos[0].xyz as given; rgb = ambient + lightColor[0].rgb * d
                                    + d^shininess, alpha 1 (the shader of the reference CPU path's
                                    timing harness) */
+    IFHIP_FS_CUBE_MAP = 8       /* colour = the cube map (a six-layer texture, the one trisPerMaterial
+                                   selects) along the interpolated normal at level of detail `lod`
+                                   (textureCubeLod, ShaderBuiltinCuda.cuh:184-231, 379-392) */
 };
 /* IfritBlendFactor, include/ifrit/softgraphics/engine/base/Constants.h:64-76.  Blending
    (iftrSetBlendFunc) supports ZERO, ONE, SRC_ALPHA, ONE_MINUS_SRC_ALPHA, DST_ALPHA and
@@ -181,6 +184,10 @@ typedef struct {
     float lightColor[4][4];     /* rgb, attenuation k of 1 / (1 + k d^2) */
     int32_t offTangent;         /* float offset of the tangent (xyz), IFHIP_FS_GGX_LAYERED */
     int32_t precision;          /* IFHIP_FS_NEURAL: 0 BF16 MFMA, 1 FP8 (E4M3) MFMA */
+    int32_t samplerId;          /* 0: `sampler` / `maxAniso` above with repeat addressing and
+                                   bilinear texels; else an id from iftrCreateSampler (Blinn-Phong and
+                                   cube-map programs) */
+    float lod;                  /* IFHIP_FS_CUBE_MAP: level of detail */
 } IfritGpuFragmentProgramDesc;
 
 void* iftrCreateGpuVertexProgram(const IfritGpuVertexProgramDesc* desc);
@@ -225,8 +232,56 @@ void iftrBindVertexDataFloat(IfritRendererHandle* h, const float* data, int stri
 void iftrBindIndexData(IfritRendererHandle* h, const int32_t* data, int numIndices);
 
 /* Textures (CreateTexture / copyHostBufferToImage / generateMipmap of the CUDA renderer,
-   TileRasterRendererCuda.h:90-96), RGBA8 with a full mip chain, power-of-two sizes. */
+   TileRasterRendererCuda.h:90-96), RGBA8 with a full mip chain, power-of-two sizes.  A texture
+   may hold several layers (IfritImageCreateInfo::arrayLayers, Structures.h:62-69): level l holds
+   max(1, w >> l) x max(1, h >> l) texels per layer, its layers one after another; a cube map is
+   six layers, +x -x +y -y +z -z. */
 int iftrCreateTexture(IfritRendererHandle* h, int width, int height, int levels);
+int iftrCreateTextureLayers(IfritRendererHandle* h, int width, int height, int levels, int layers);
+void iftrCopyHostBufferToImageLayer(IfritRendererHandle* h, int texture, int layer, const void* rgba8Level0);
+
+/* IfritFilter, IfritSamplerAddressMode, IfritBorderColor (Constants.h:36-53) and IfritSamplerT
+   (Structures.h:52-61): CreateSampler of the CUDA renderer (TileRasterRendererCuda.h:92). */
+enum { IF_FILTER_NEAREST = 0, IF_FILTER_LINEAR = 1 };
+enum {
+    IF_SAMPLER_ADDRESS_MODE_REPEAT = 0,
+    IF_SAMPLER_ADDRESS_MODE_MIRRORED_REPEAT = 1,
+    IF_SAMPLER_ADDRESS_MODE_CLAMP_TO_EDGE = 2,
+    IF_SAMPLER_ADDRESS_MODE_CLAMP_TO_BORDER = 3,
+    IF_SAMPLER_ADDRESS_MODE_MIRROR_CLAMP_TO_EDGE = 4
+};
+enum { IF_BORDER_COLOR_BLACK = 0, IF_BORDER_COLOR_WHITE = 1 };
+typedef struct {
+    int32_t filterMode;                         /* within a level: NEAREST texel or LINEAR (bilinear) */
+    int32_t addressModeU, addressModeV, addressModeW;  /* W is kept, 2D and cube lookups use U and V */
+    int32_t borderColor;                        /* CLAMP_TO_BORDER: opaque black or white */
+    int32_t anisotropyEnable;                   /* 1: anisotropic taps (up to maxAnisotropy, at most 16) */
+    float maxAnisotropy;
+} IfritSamplerDesc;
+/* Returns the sampler id (1, 2, ...) for IfritGpuFragmentProgramDesc::samplerId (-1 on error).  Texel centres
+   are at (i + 0.5) / size; mirrored repeat reflects every other period and mirror-clamp-to-edge
+   reflects once about 0 and then clamps (the Vulkan definitions); clamp-to-border gives border
+   texels to the filter one by one. */
+int iftrCreateSampler(IfritRendererHandle* h, const IfritSamplerDesc* desc);
+
+/* blitImage (TileRasterRendererCuda.h:95; IfritImageBlit, Structures.h:70-83): a region of one level
+   and layer into a region of another, scaled; destination texel (dst start + (x, y)) reads the
+   source at (x / dst width * src width + src start, likewise y); IF_FILTER_LINEAR blends the four
+   neighbours (right and lower clamped to the source level) and rounds half up, IF_FILTER_NEAREST
+   takes the texel at the rounded coordinate. */
+typedef struct {
+    uint32_t mipLevel, baseArrayLayer;
+} IfritImageSubresourceLayers;
+typedef struct {
+    uint32_t width, height, depth;
+} IfritExtent3D;
+typedef struct {
+    IfritImageSubresourceLayers srcSubresource;
+    IfritExtent3D srcExtentSt, srcExtentEd;
+    IfritImageSubresourceLayers dstSubresource;
+    IfritExtent3D dstExtentSt, dstExtentEd;
+} IfritImageBlit;
+void iftrBlitImage(IfritRendererHandle* h, int srcTexture, int dstTexture, const IfritImageBlit* region, int filter);
 void iftrCopyHostBufferToImage(IfritRendererHandle* h, int texture, const void* rgba8Level0);
 void iftrUploadTextureLevels(IfritRendererHandle* h, int texture, const void* rgba8AllLevels);
 void iftrGenerateFbmTexture(IfritRendererHandle* h, int texture, uint32_t seed);

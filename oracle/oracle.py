@@ -55,11 +55,25 @@ class ShadeParams(C.Structure):
                 ("off_uv", C.c_int), ("tris_per_material", C.c_int), ("sampler", C.c_int), ("max_aniso", C.c_int),
                 ("camera", C.c_float * 3), ("ambient", C.c_float), ("kd", C.c_float), ("ks", C.c_float),
                 ("shininess", C.c_float), ("num_lights", C.c_int), ("light_pos", (C.c_float * 4) * 4),
-                ("light_color", (C.c_float * 4) * 4), ("off_tangent", C.c_int), ("clear_color", C.c_float * 4)]
+                ("light_color", (C.c_float * 4) * 4), ("off_tangent", C.c_int), ("clear_color", C.c_float * 4),
+                ("addr_u", C.c_int), ("addr_v", C.c_int), ("nearest", C.c_int), ("border", C.c_float * 4),
+                ("lod", C.c_float)]
 
 
 class TextureDesc(C.Structure):
-    _fields_ = [("texels", C.c_void_p), ("width", C.c_int), ("height", C.c_int), ("levels", C.c_int)]
+    _fields_ = [("texels", C.c_void_p), ("width", C.c_int), ("height", C.c_int), ("levels", C.c_int),
+                ("layers", C.c_int)]
+
+
+def sampler_params(state):
+    """(sampler kind, max taps, addr_u, addr_v, nearest, border rgba) of a Renderer.create_sampler
+    argument dict (the library's mapping: anisotropic taps when enabled, else trilinear)."""
+    import math
+
+    aniso = bool(state.get("anisotropy", False))
+    border = (1.0, 1.0, 1.0, 1.0) if state.get("border", 0) == 1 else (0.0, 0.0, 0.0, 1.0)
+    return (2 if aniso else 1, max(1, min(16, math.ceil(state.get("max_anisotropy", 1.0)))),
+            state.get("address_u", 0), state.get("address_v", 0), 1 if state.get("filter", 1) == 0 else 0, border)
 
 
 def build(force: bool = False) -> None:
@@ -206,6 +220,12 @@ class Frame:
         p.off_tangent = 8 if scene.stride >= 11 else 0
         p.tris_per_material = scene.tris_per_material
         p.sampler, p.max_aniso = scene.sampler, scene.max_aniso
+        p.border[:] = [0.0, 0.0, 0.0, 1.0]
+        if getattr(scene, "sampler_state", None) is not None:
+            kind, taps, au, av, nearest, border = sampler_params(scene.sampler_state)
+            p.sampler, p.max_aniso, p.addr_u, p.addr_v, p.nearest = kind, taps, au, av, nearest
+            p.border[:] = list(border)
+        p.lod = getattr(scene, "lod", 0.0)
         p.camera[:] = list(scene.camera)
         p.ambient, p.kd, p.ks, p.shininess = scene.ambient, scene.kd, scene.ks, scene.shininess
         p.num_lights = len(scene.lights)
@@ -219,7 +239,7 @@ class Frame:
                 raise ValueError("texture texels must be materialised for the shading oracle")
             arr = np.ascontiguousarray(t.texels, np.uint8)
             keep.append(arr)
-            descs[i] = TextureDesc(arr.ctypes.data, t.width, t.height, t.levels)
+            descs[i] = TextureDesc(arr.ctypes.data, t.width, t.height, t.levels, getattr(t, "layers", 1))
         n = self.width * self.height
         rgba = np.empty((n * 4,), np.uint8)
         flags = np.empty((n,), np.uint8)
@@ -283,5 +303,73 @@ def sample_tex(texels, width, height, levels, sampler, max_aniso, u, v, dudx, dv
     return out, bool(flag)
 
 
+def sample_tex_mode(texels, width, height, levels, sampler, max_aniso, addr_u, addr_v, nearest, border, u, v, dudx,
+                    dvdx, dudy, dvdy):
+    """One FP64 lookup under a sampler's addressing, filter and border colour."""
+    L = lib()
+    f = L.oracle_sample_tex_mode
+    f.restype = C.c_int
+    t = np.ascontiguousarray(texels, np.uint8)
+    b = np.ascontiguousarray(border, np.float32)
+    out = np.zeros(4, np.float64)
+    flag = f(_p(t), C.c_int(width), C.c_int(height), C.c_int(levels), C.c_int(sampler), C.c_int(max_aniso),
+             C.c_int(addr_u), C.c_int(addr_v), C.c_int(nearest), _p(b),
+             *[C.c_double(x) for x in (u, v, dudx, dvdx, dudy, dvdy)], _p(out))
+    return out, bool(flag)
+
+
+def cube_lookup(texels, size, levels, d, lod, addr_u=2, addr_v=2, nearest=0, border=(0, 0, 0, 1)):
+    """FP64 cube-map lookup of a six-layer texture along direction d; returns (rgba, too-close flag)."""
+    L = lib()
+    f = L.oracle_cube_lookup
+    f.restype = C.c_int
+    t = np.ascontiguousarray(texels, np.uint8)
+    b = np.ascontiguousarray(border, np.float32)
+    dd = np.ascontiguousarray(d, np.float64)
+    out = np.zeros(4, np.float64)
+    flag = f(_p(t), C.c_int(size), C.c_int(levels), C.c_int(addr_u), C.c_int(addr_v), C.c_int(nearest), _p(b), _p(dd),
+             C.c_double(lod), _p(out))
+    return out, bool(flag)
+
+
 def flat_hash(prim: int) -> int:
     return lib().oracle_flat_hash(prim)
+
+
+def blit(src, dst, src_region, dst_region, linear=True):
+    """blitImage (TileRasterImageOpInvocationsCuda.cu:36-82) restated in float32 on the 0..255 values:
+    src, dst are (h, w, 4) uint8 levels, regions (x0, y0, x1, y1).  Destination texel (dx0 + x, dy0 + y)
+    reads the source at x / (dx1 - dx0) * (sx1 - sx0) + sx0 (likewise y); linear blends the four
+    neighbours (right and lower clamped to the level) and rounds half up, nearest takes the texel at
+    the rounded (half away from zero) coordinate, clamped.  Returns the new destination."""
+    f32 = np.float32
+    sx0, sy0, sx1, sy1 = src_region
+    dx0, dy0, dx1, dy1 = dst_region
+    sh, sw = src.shape[:2]
+    out = dst.copy()
+    rw, rh = dx1 - dx0, dy1 - dy0
+    y, x = np.mgrid[0:rh, 0:rw]
+    px = x.astype(f32) / f32(rw)
+    py = y.astype(f32) / f32(rh)
+    cx = px * f32(sx1 - sx0) + f32(sx0)
+    cy = py * f32(sy1 - sy0) + f32(sy0)
+    S = src.astype(f32)
+    if linear:
+        ix, iy = cx.astype(np.int64), cy.astype(np.int64)  # truncation: the coordinates are >= 0
+        fx, fy = cx - ix.astype(f32), cy - iy.astype(f32)
+        ix1, iy1 = np.minimum(ix + 1, sw - 1), np.minimum(iy + 1, sh - 1)
+        fx, fy = fx[..., None], fy[..., None]
+
+        def lerp(p, q, t):
+            return p + (q - p) * t
+
+        v = lerp(lerp(S[iy, ix], S[iy, ix1], fx), lerp(S[iy1, ix], S[iy1, ix1], fx), fy)
+        res = np.clip(np.floor(v + f32(0.5)), 0, 255).astype(np.uint8)
+    else:
+        def rnd(c):
+            fl = np.floor(c)
+            return (fl + (c - fl >= f32(0.5))).astype(np.int64)
+
+        res = src[np.minimum(rnd(cy), sh - 1), np.minimum(rnd(cx), sw - 1)]
+    out[dy0:dy1, dx0:dx1] = res
+    return out

@@ -64,31 +64,64 @@ D3 ggx_light(D3 n, D3 V, D3 L, D3 alb, double alpha, double kd, double ks, bool*
     return D3{alb.x * diff + ks * spec, alb.y * diff + ks * spec, alb.z * diff + ks * spec} * ndl;
 }
 
+// Addressing, filter within a level and border colour of a lookup
+// (IfritSamplerT, Structures.h:52-61); the default is repeat + bilinear.
+struct Mode {
+    int u = 0, v = 0;  // IfritSamplerAddressMode
+    int nearest = 0;
+    double border[4] = {0, 0, 0, 1};
+};
+
 struct Tex {
     const uint8_t* base;
     int w, h, levels;
-    std::vector<size_t> off;
+    std::vector<size_t> off;  // byte offset of each level's first layer
+    int layers = 1;
 };
 
-void texel(const Tex& t, int l, long x, long y, double* c) {
+// Texel index i of a level n texels wide, or -1 for the border (the Vulkan
+// definitions: mirrored repeat reflects every other period, mirror clamp to
+// edge reflects once about 0 and then clamps).
+long address(long i, long n, int mode) {
+    switch (mode) {
+        case 1: {
+            const long m = ((i % (2 * n)) + 2 * n) % (2 * n);
+            return m >= n ? 2 * n - 1 - m : m;
+        }
+        case 2: return std::min(std::max(i, 0l), n - 1);
+        case 3: return (i < 0 || i >= n) ? -1 : i;
+        case 4: return std::min(i < 0 ? -1 - i : i, n - 1);
+        default: return ((i % n) + n) % n;
+    }
+}
+
+void texel(const Tex& t, int l, long x, long y, double* c, const Mode& m = Mode(), int layer = 0) {
     const long w = std::max(1, t.w >> l), h = std::max(1, t.h >> l);
-    x = ((x % w) + w) % w;
-    y = ((y % h) + h) % h;
-    const uint8_t* p = t.base + t.off[l] + 4 * ((size_t)y * w + x);
+    x = address(x, w, m.u);
+    y = address(y, h, m.v);
+    if (x < 0 || y < 0) {
+        for (int k = 0; k < 4; k++) c[k] = m.border[k];
+        return;
+    }
+    const uint8_t* p = t.base + t.off[l] + 4 * ((size_t)w * h * layer + (size_t)y * w + x);
     for (int k = 0; k < 4; k++) c[k] = p[k] / 255.0;
 }
 
-void bilinear(const Tex& t, int l, double u, double v, double* c) {
+void bilinear(const Tex& t, int l, double u, double v, double* c, const Mode& m = Mode(), int layer = 0) {
     const double w = std::max(1, t.w >> l), h = std::max(1, t.h >> l);
+    if (m.nearest) {
+        texel(t, l, (long)std::floor(u * w), (long)std::floor(v * h), c, m, layer);
+        return;
+    }
     const double s = u * w - 0.5, r = v * h - 0.5;
     const double fx0 = std::floor(s), fy0 = std::floor(r);
     const double fx = s - fx0, fy = r - fy0;
     const long x0 = (long)fx0, y0 = (long)fy0;
     double a[4], b[4], cc[4], d[4];
-    texel(t, l, x0, y0, a);
-    texel(t, l, x0 + 1, y0, b);
-    texel(t, l, x0, y0 + 1, cc);
-    texel(t, l, x0 + 1, y0 + 1, d);
+    texel(t, l, x0, y0, a, m, layer);
+    texel(t, l, x0 + 1, y0, b, m, layer);
+    texel(t, l, x0, y0 + 1, cc, m, layer);
+    texel(t, l, x0 + 1, y0 + 1, d, m, layer);
     for (int k = 0; k < 4; k++) {
         const double top = a[k] + (b[k] - a[k]) * fx;
         const double bot = cc[k] + (d[k] - cc[k]) * fx;
@@ -96,24 +129,53 @@ void bilinear(const Tex& t, int l, double u, double v, double* c) {
     }
 }
 
-void trilinear(const Tex& t, double u, double v, double lod, double* c) {
+void trilinear(const Tex& t, double u, double v, double lod, double* c, const Mode& m = Mode(), int layer = 0) {
     lod = std::min(std::max(lod, 0.0), (double)(t.levels - 1));
     const int l0 = (int)std::floor(lod);
     const int l1 = std::min(l0 + 1, t.levels - 1);
     const double f = lod - l0;
     double a[4], b[4];
-    bilinear(t, l0, u, v, a);
-    bilinear(t, l1, u, v, b);
+    bilinear(t, l0, u, v, a, m, layer);
+    bilinear(t, l1, u, v, b, m, layer);
     for (int k = 0; k < 4; k++) c[k] = a[k] + (b[k] - a[k]) * f;
+}
+
+// Cube-map face and (u, v) from a direction's major axis (the Vulkan face
+// table that textureCubeCoordConversion, ShaderBuiltinCuda.cuh:184-208,
+// follows); true when the choice is too close to call.
+bool cube_face(const double* d, int* face, double* u, double* v) {
+    const double ax = std::fabs(d[0]), ay = std::fabs(d[1]), az = std::fabs(d[2]);
+    double ma, sc, tc;
+    if (az >= ax && az >= ay) {
+        *face = d[2] < 0 ? 5 : 4;
+        ma = az;
+        sc = d[2] < 0 ? -d[0] : d[0];
+        tc = -d[1];
+    } else if (ay >= ax) {
+        *face = d[1] < 0 ? 3 : 2;
+        ma = ay;
+        sc = d[0];
+        tc = d[1] < 0 ? -d[2] : d[2];
+    } else {
+        *face = d[0] < 0 ? 1 : 0;
+        ma = ax;
+        sc = d[0] < 0 ? d[2] : -d[2];
+        tc = -d[1];
+    }
+    *u = sc * (0.5 / ma) + 0.5;
+    *v = tc * (0.5 / ma) + 0.5;
+    const double top = std::max(ax, std::max(ay, az));
+    const double second = ax + ay + az - top - std::min(ax, std::min(ay, az));
+    return top - second <= 1e-6 * top;
 }
 
 bool near_integer(double r) { return std::fabs(r - std::nearbyint(r)) < 1e-4 * std::max(1.0, std::fabs(r)); }
 
 // Returns true when a discontinuous decision is too close to call.
 bool sample_tex(const Tex& t, int sampler, int maxAniso, double u, double v, double dudx, double dvdx, double dudy,
-                double dvdy, double* c) {
+                double dvdy, double* c, const Mode& m = Mode()) {
     if (sampler == ORS_BILINEAR) {
-        bilinear(t, 0, u, v, c);
+        bilinear(t, 0, u, v, c, m);
         return false;
     }
     const double px = std::sqrt((dudx * t.w) * (dudx * t.w) + (dvdx * t.h) * (dvdx * t.h));
@@ -121,7 +183,7 @@ bool sample_tex(const Tex& t, int sampler, int maxAniso, double u, double v, dou
     const double pmax = std::max(px, py), pmin = std::min(px, py);
     if (sampler == ORS_TRILINEAR || pmax <= 0) {
         const double lod = pmax > 0 ? std::log2(pmax) : 0.0;
-        trilinear(t, u, v, lod, c);
+        trilinear(t, u, v, lod, c, m);
         return false;
     }
     bool flag = std::fabs(px - py) <= 1e-4 * pmax;
@@ -142,7 +204,7 @@ bool sample_tex(const Tex& t, int sampler, int maxAniso, double u, double v, dou
     for (int i = 0; i < n; i++) {
         const double o = (i + 0.5) / n - 0.5;
         double s[4];
-        trilinear(t, u + au * o, v + av * o, lod, s);
+        trilinear(t, u + au * o, v + av * o, lod, s, m);
         for (int k = 0; k < 4; k++) acc[k] += s[k];
     }
     for (int k = 0; k < 4; k++) c[k] = acc[k] / n;
@@ -151,13 +213,18 @@ bool sample_tex(const Tex& t, int sampler, int maxAniso, double u, double v, dou
 
 struct Shader {
     const OracleShadeParams* p;
+    Mode mode;
     const std::vector<OracleTri>* tris;
     const int32_t* idx;
     const float* vtx;
     const std::vector<Tex>* tex;
 
-    // Perspective-correct vertex weights of triangle t at (x, y).
-    void weights(const OracleTri& t, double x, double y, double* w) const {
+    // Perspective-correct vertex weights of triangle t at (x, y).  Returns a
+    // bound on how far the FP32 evaluation of the three planes (whose terms
+    // may cancel: a sliver seen far from its vertices) can move a weight:
+    // each plane value is off by up to 2^-23 of the sum of its terms'
+    // magnitudes.  Above 0.01 a pixel is too close to call.
+    double weights(const OracleTri& t, double x, double y, double* w) const {
         const double b1 = (double)t.f1[0] * x + (double)t.f1[1] * y + (double)t.f1[2];
         const double b2 = (double)t.f2[0] * x + (double)t.f2[1] * y + (double)t.f2[2];
         const double b3 = (double)t.f3[0] * x + (double)t.f3[1] * y + (double)t.f3[2];
@@ -166,6 +233,16 @@ struct Shader {
         w[0] = B[0] * t.bx[0] + B[1] * t.bx[1] + B[2] * t.bx[2];
         w[1] = B[0] * t.by[0] + B[1] * t.by[1] + B[2] * t.by[2];
         w[2] = 1.0 - w[0] - w[1];
+        const double u = std::ldexp(1.0, -23);
+        const double e[3] = {
+            u * (std::fabs(t.f1[0] * x) + std::fabs(t.f1[1] * y) + std::fabs((double)t.f1[2])),
+            u * (std::fabs(t.f2[0] * x) + std::fabs(t.f2[1] * y) + std::fabs((double)t.f2[2])),
+            u * (std::fabs(t.f3[0] * x) + std::fabs(t.f3[1] * y) + std::fabs((double)t.f3[2]))};
+        const double es = e[0] + e[1] + e[2], as = std::fabs(s);
+        const double bb[3] = {b1, b2, b3};
+        double worst = 0.0;
+        for (int k = 0; k < 3; k++) worst = std::max(worst, e[k] / as + std::fabs(bb[k]) * es / (as * as));
+        return worst;
     }
     // vertex k of triangle t in the order setup took them (exchanged triangles: 2, 1, 0)
     const float* vert(const OracleTri& t, int k) const {
@@ -198,7 +275,7 @@ struct Shader {
         if (p->program == ORP_NORMAL_LAMBERT) {
             // the shader of the reference CPU timing harness (oracle/ref/ref_harness.cpp LambertFS)
             double w[3];
-            weights(t, px + o, py + o, w);
+            const bool ill = weights(t, px + o, py + o, w) > 0.01;
             D3 N{0, 0, 0};
             for (int k = 0; k < 3; k++) {
                 const float* v = vert(t, k);
@@ -207,11 +284,36 @@ struct Shader {
             const double l = std::sqrt(dot(N, N));
             const double nl = dot(N, D3{p->light_pos[0][0], p->light_pos[0][1], p->light_pos[0][2]});
             double d = l > 0 ? nl / l : 0.0;
-            const bool flag = std::fabs(d) < 1e-4;
+            const bool flag = std::fabs(d) < 1e-4 || ill;
             d = std::max(d, 0.0);
             const double sp = std::pow(d, (double)p->shininess);
             for (int k = 0; k < 3; k++) rgba[k] = p->ambient + p->light_color[0][k] * d + sp;
             rgba[3] = 1.0;
+            return flag;
+        }
+        if (p->program == ORP_CUBE_MAP) {
+            double w[3];
+            const bool ill = weights(t, px + o, py + o, w) > 0.01;
+            double N[3] = {0, 0, 0};
+            for (int k = 0; k < 3; k++) {
+                const float* v = vert(t, k);
+                for (int c = 0; c < 3; c++) N[c] += w[k] * v[p->off_normal + c];
+            }
+            const int mat = p->tris_per_material > 0 ? t.prim / p->tris_per_material : 0;
+            if (tex->empty()) {
+                rgba[0] = rgba[1] = rgba[2] = rgba[3] = 0;
+                return false;
+            }
+            const Tex& tx = (*tex)[std::min<int>(mat, (int)tex->size() - 1)];
+            int face;
+            double u, v;
+            const bool flag = cube_face(N, &face, &u, &v) || ill;
+            double lod = std::min(std::max((double)p->lod, 0.0), (double)(tx.levels - 1));
+            const int l0 = (int)std::floor(lod), l1 = std::min(l0 + 1, tx.levels - 1);
+            double a[4], b[4];
+            bilinear(tx, l0, u, v, a, mode, face);
+            bilinear(tx, l1, u, v, b, mode, face);
+            for (int k = 0; k < 4; k++) rgba[k] = a[k] + (b[k] - a[k]) * (lod - l0);
             return flag;
         }
         if (p->program != ORP_BLINN_PHONG && p->program != ORP_GGX_LAYERED) {
@@ -220,7 +322,7 @@ struct Shader {
         }
         const double x = px + o, y = py + o;
         double w[3];
-        weights(t, x, y, w);
+        const bool ill = weights(t, x, y, w) > 0.01;
         D3 P{0, 0, 0}, N{0, 0, 0};
         double uv[2] = {0, 0};
         for (int k = 0; k < 3; k++) {
@@ -245,7 +347,7 @@ struct Shader {
                 T = T + D3{v[p->off_tangent], v[p->off_tangent + 1], v[p->off_tangent + 2]} * w[k];
             }
             double alb[4] = {1, 1, 1, 1}, nm[4] = {0.5, 0.5, 1, 1}, rg[4] = {0, 0, 0, 0};
-            bool flag = false;
+            bool flag = ill;
             if (tex->size() >= 3) {
                 const double du = b[0] - a[0], dv = b[1] - a[1], eu = d[0] - c[0], ev = d[1] - c[1];
                 flag |= sample_tex((*tex)[0], p->sampler, p->max_aniso, uv[0], uv[1], du, dv, eu, ev, alb);
@@ -285,11 +387,11 @@ struct Shader {
         }
         const int mat = p->tris_per_material > 0 ? t.prim / p->tris_per_material : 0;
         double alb[4] = {1, 1, 1, 1};
-        bool flag = false;
+        bool flag = ill;
         if (!tex->empty()) {
             const Tex& tx = (*tex)[std::min<int>(mat, (int)tex->size() - 1)];
-            flag = sample_tex(tx, p->sampler, p->max_aniso, uv[0], uv[1], b[0] - a[0], b[1] - a[1], d[0] - c[0],
-                              d[1] - c[1], alb);
+            flag |= sample_tex(tx, p->sampler, p->max_aniso, uv[0], uv[1], b[0] - a[0], b[1] - a[1], d[0] - c[0],
+                              d[1] - c[1], alb, mode);
         }
         N = normalize(N);
         const D3 cam{p->camera[0], p->camera[1], p->camera[2]};
@@ -367,6 +469,56 @@ int oracle_sample_tex(const uint8_t* texels, int width, int height, int levels, 
     return sample_tex(t, sampler, maxAniso, u, v, dudx, dvdx, dudy, dvdy, out) ? 1 : 0;
 }
 
+int oracle_sample_tex_mode(const uint8_t* texels, int width, int height, int levels, int sampler, int maxAniso,
+                           int addr_u, int addr_v, int nearest, const float* border, double u, double v, double dudx,
+                           double dvdx, double dudy, double dvdy, double* out) {
+    Tex t;
+    t.base = texels;
+    t.w = width;
+    t.h = height;
+    t.levels = levels;
+    size_t off = 0;
+    for (int l = 0; l < levels; l++) {
+        t.off.push_back(off);
+        off += (size_t)std::max(1, width >> l) * std::max(1, height >> l) * 4;
+    }
+    Mode m;
+    m.u = addr_u;
+    m.v = addr_v;
+    m.nearest = nearest;
+    for (int k = 0; k < 4; k++) m.border[k] = border[k];
+    return sample_tex(t, sampler, maxAniso, u, v, dudx, dvdx, dudy, dvdy, out, m) ? 1 : 0;
+}
+
+int oracle_cube_lookup(const uint8_t* texels, int size, int levels, int addr_u, int addr_v, int nearest,
+                       const float* border, const double* d, double lod, double* out) {
+    Tex t;
+    t.base = texels;
+    t.w = t.h = size;
+    t.levels = levels;
+    t.layers = 6;
+    size_t off = 0;
+    for (int l = 0; l < levels; l++) {
+        t.off.push_back(off);
+        off += (size_t)std::max(1, size >> l) * std::max(1, size >> l) * 4 * 6;
+    }
+    Mode m;
+    m.u = addr_u;
+    m.v = addr_v;
+    m.nearest = nearest;
+    for (int k = 0; k < 4; k++) m.border[k] = border[k];
+    int face;
+    double u, v;
+    const bool flag = cube_face(d, &face, &u, &v);
+    lod = std::min(std::max(lod, 0.0), (double)(levels - 1));
+    const int l0 = (int)std::floor(lod), l1 = std::min(l0 + 1, levels - 1);
+    double a[4], b[4];
+    bilinear(t, l0, u, v, a, m, face);
+    bilinear(t, l1, u, v, b, m, face);
+    for (int k = 0; k < 4; k++) out[k] = a[k] + (b[k] - a[k]) * (lod - l0);
+    return flag ? 1 : 0;
+}
+
 uint32_t oracle_flat_hash(uint32_t prim) {
     uint32_t h = prim * 0x9E3779B1u;
     h ^= h >> 15;
@@ -406,13 +558,20 @@ int oracle_shade(const OracleFrame* f, const OracleShadeParams* p, int W, int H,
         tex[i].w = textures[i].width;
         tex[i].h = textures[i].height;
         tex[i].levels = textures[i].levels;
+        tex[i].layers = std::max(1, textures[i].layers);
         size_t off = 0;
         for (int l = 0; l < textures[i].levels; l++) {
             tex[i].off.push_back(off);
-            off += (size_t)std::max(1, textures[i].width >> l) * std::max(1, textures[i].height >> l) * 4;
+            off += (size_t)std::max(1, textures[i].width >> l) * std::max(1, textures[i].height >> l) * 4 *
+                   tex[i].layers;
         }
     }
-    Shader sh{p, &tris, indices, vertices, &tex};
+    Mode mode;
+    mode.u = p->addr_u;
+    mode.v = p->addr_v;
+    mode.nearest = p->nearest;
+    for (int k = 0; k < 4; k++) mode.border[k] = p->border[k];
+    Shader sh{p, mode, &tris, indices, vertices, &tex};
     const double o = S == 1 ? 0.0 : 0.5;
     std::atomic<int> next{0}, flagged{0};
     auto worker = [&]() {

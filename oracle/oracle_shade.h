@@ -25,7 +25,8 @@ enum OracleProgram {
     ORP_BLINN_PHONG = 1,  // textured Blinn-Phong, directional and point lights
     ORP_DEPTH_ONLY = 2,   // no colour output
     ORP_GGX_LAYERED = 3,  // layered GGX: texture 0 albedo, 1 tangent-space normal, 2 roughness
-    ORP_NORMAL_LAMBERT = 7  // untextured: d = max(0, n.l / |n|), rgb = ambient + light rgb d + d^shininess
+    ORP_NORMAL_LAMBERT = 7, // untextured: d = max(0, n.l / |n|), rgb = ambient + light rgb d + d^shininess
+    ORP_CUBE_MAP = 8        // the six-layer texture along the interpolated normal at level `lod`
 };
 enum OracleSampler { ORS_BILINEAR = 0, ORS_TRILINEAR = 1, ORS_ANISO = 2 };
 
@@ -45,11 +46,17 @@ typedef struct {
     float light_color[4][4];  // rgb, attenuation coefficient k in 1 / (1 + k d^2)
     int off_tangent;        // float offset of the tangent (xyz), ORP_GGX_LAYERED
     float clear_color[4];   // colour of samples no triangle covers (0 in the CPU path)
+    int addr_u, addr_v;     // IfritSamplerAddressMode of the lookups (0 repeat ... 4 mirror clamp to edge)
+    int nearest;            // 1: nearest texel within a level
+    float border[4];        // clamp-to-border colour, 0..1
+    float lod;              // ORP_CUBE_MAP: level of detail
 } OracleShadeParams;
 
 typedef struct {
-    const uint8_t* texels;  // RGBA8, levels stored consecutively, level l is (w >> l) x (h >> l)
+    const uint8_t* texels;  // RGBA8, levels stored consecutively, level l is (w >> l) x (h >> l) per layer,
+                            // its layers one after another
     int width, height, levels;
+    int layers;             // 1, or 6 for a cube map (+x -x +y -y +z -z)
 } OracleTexture;
 
 // 32-bit integer hash used for flat colours (shared definition with the product).
@@ -76,6 +83,15 @@ void oracle_build_mips(uint8_t* texels, int width, int height, int levels);
 // tap count was too close to call (a whole-number anisotropy ratio).
 int oracle_sample_tex(const uint8_t* texels, int width, int height, int levels, int sampler, int maxAniso, double u,
                       double v, double dudx, double dvdx, double dudy, double dvdy, double* out);
+// The same lookup under a sampler's addressing (addr_u, addr_v), filter within a level (nearest) and
+// border colour (0..1).
+int oracle_sample_tex_mode(const uint8_t* texels, int width, int height, int levels, int sampler, int maxAniso,
+                           int addr_u, int addr_v, int nearest, const float* border, double u, double v, double dudx,
+                           double dvdx, double dudy, double dvdy, double* out);
+// Cube-map lookup (six-layer texture) along direction d at a level of detail; returns 1 when the face
+// choice was too close to call (two direction components within a relative 1e-6).
+int oracle_cube_lookup(const uint8_t* texels, int size, int levels, int addr_u, int addr_v, int nearest,
+                       const float* border, const double* d, double lod, double* out);
 
 #ifdef __cplusplus
 }
