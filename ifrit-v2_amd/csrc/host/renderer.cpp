@@ -839,6 +839,9 @@ void TileRasterRenderer::enqueue(bool clear) {
     rp.depthFunc = depthFunc_;
     rp.clear = clear ? 1 : 0;
     rp.clearDepth = clearDepth_;
+    // optsetForceDeterministic(false) selects the tagging mode's depth expressions (TRW:1581-1675);
+    // blending keeps the deterministic path, as the reference's does (TRW:410)
+    rp.tagging = !deterministic_ && samples_ == 1 ? 1 : 0;
     rp.scissor = activeScissor();
     rp.binCap = binCap_;
     const float* st = samples_ == 8 ? kS8 : (samples_ == 4 ? kS4 : kS1);
@@ -896,7 +899,7 @@ std::vector<uint8_t> TileRasterRenderer::graphKey(bool clear) const {
                         partShift_, rasterGroupsPerCU_, shadeTile_, bigThreshold_, nnFallback_[0] ? 1 : 0,
                         nnFallback_[1] ? 1 : 0, (int)textures_.size(), stageTiming_ ? 1 : 0,
                         fb_ && !fb_->color.empty() && fb_->color[0] ? 1 : 0, batch_.empty() ? 1 : 0,
-                        floatOut_ ? 1 : 0, blend_.blendEnable ? 1 : 0, cullMode_};
+                        floatOut_ ? 1 : 0, blend_.blendEnable ? 1 : 0, cullMode_, deterministic_ ? 1 : 0};
     put(ints, sizeof(ints));
     const unsigned caps[] = {clipCap_, binCap_, bigCap_, itemCap_, partialCap_};
     put(caps, sizeof(caps));

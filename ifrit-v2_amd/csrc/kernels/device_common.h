@@ -142,6 +142,15 @@ __device__ __forceinline__ float depth_scalar(const float4& F1, const float4& F2
     return (F1.w * b0 + F2.w * b1) + (F3.w * b2 + 0.0f);
 }
 
+// Tagging mode, tile- and block-accepted fragments (pixelShadingSIMD256Grouped,
+// TileRasterWorker.cpp:2672-2684): the x term enters first, then y.
+__device__ __forceinline__ float depth_grouped(const float4& F1, const float4& F2, const float4& F3, float x, float y) {
+    const float b0 = fmaf(F1.y, y, fmaf(F1.x, x, F1.z));
+    const float b1 = fmaf(F2.y, y, fmaf(F2.x, x, F2.z));
+    const float b2 = fmaf(F3.y, y, fmaf(F3.x, x, F3.z));
+    return fmaf(b0, F1.w, fmaf(b1, F2.w, b2 * F3.w));
+}
+
 // Order-preserving map of FP32 depth to uint32 (-0 and +0 compare equal, as
 // they do under the CPU path's float comparison).
 __device__ __forceinline__ unsigned order_key(float d) {

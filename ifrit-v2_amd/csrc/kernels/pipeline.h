@@ -132,6 +132,7 @@ struct RasterParams {
     int depthFunc;
     int clear;
     float clearDepth;    // depth a cleared sample starts from (0x7f7f7f7f in the CPU path)
+    int tagging;         // 1 spp: the tagging mode's depth expressions (class_depth)
     Scissor scissor;
     unsigned binCap;
     float sx[8], sy[8];  // sample offsets from the pixel corner

@@ -191,33 +191,16 @@ __device__ __forceinline__ void group_mask_half(const Edges& E, bool full, int t
 // Coverage class and depth of the winning triangle at a 1-sample pixel,
 // recomputed with the same decisions as the raster phases.
 __device__ __forceinline__ unsigned winner_class(const TriRec& R, int tx, int ty, float fpx, float fpy, float bMinX,
-                                                 float bMinY, float& d) {
+                                                 float bMinY, float& d, bool tagging) {
     const Edges E = load_edges(R);
-    unsigned c;
-    if (tile_class(E, tx, ty) == 2) {
-        c = 1;
-    } else {
+    bool accepted = tile_class(E, tx, ty) == 2;
+    if (!accepted) {
         int trn, tan;
         block_test(E, bMinX, bMinY, trn, tan);
-        if (tan == 3) {
-            c = 1;
-        } else {
-            const float gx = bMinX, gy = floorf(fpy * 0.5f) * 2.0f;
-            bool g = true;
-#pragma unroll
-            for (int l = 0; l < 8; l++) g = g && inside(E, gx + (float)(l & 3), gy + (float)(l >> 2));
-            if (g) {
-                c = 1;
-            } else {
-                const float qx = floorf(fpx * 0.5f) * 2.0f;
-                bool q = true;
-#pragma unroll
-                for (int l = 0; l < 4; l++) q = q && inside(E, qx + (float)(l & 1), gy + (float)(l >> 1));
-                c = q ? 2u : 3u;
-            }
-        }
+        accepted = tan == 3;
     }
-    d = class_depth(c, R.f1, R.f2, R.f3, fpx, fpy);
+    const unsigned c = pixel_class(E, accepted, fpx, fpy, bMinX, tagging);
+    d = class_depth(c, R.f1, R.f2, R.f3, fpx, fpy, tagging);
     return c;
 }
 
@@ -274,7 +257,7 @@ __device__ __forceinline__ void write_pixel(const RasterParams& P, const TriRecs
         float d = order_key_inv((unsigned)(k >> 32));
         if (S == 1 && (unsigned)(k >> 32) == 0x80000000u) {
             // +0 and -0 share a key: recompute the winner's own depth bits from its record
-            cls = winner_class(load_rec(recs, slot), C.tx, C.ty, C.fpx, C.fpy, C.bMinX, C.bMinY, d);
+            cls = winner_class(load_rec(recs, slot), C.tx, C.ty, C.fpx, C.fpy, C.bMinX, C.bMinY, d, P.tagging);
         } else if (S == 1) {
             cls = (unsigned)k & 3u;
         } else if (S > 1 && (unsigned)(k >> 32) == 0x80000000u) {
@@ -284,7 +267,7 @@ __device__ __forceinline__ void write_pixel(const RasterParams& P, const TriRecs
         visDepth[C.pix * S + s] = d;
         visSlot[C.pix * S + s] = slot;
     }
-    if (S == 1 && visClass && (won || P.clear)) visClass[C.pix] = (uint8_t)cls;
+    if (S == 1 && visClass && (won || P.clear)) visClass[C.pix] = class_out(cls, P.tagging);
 }
 
 // One tile's outputs staged in LDS in global order (pixel-major, sample-
@@ -318,7 +301,8 @@ __device__ __forceinline__ unsigned pixel_outputs(const RasterParams& P, const T
             d[s] = order_key_inv((unsigned)(k >> 32));
             if (S == 1 && (unsigned)(k >> 32) == 0x80000000u) {
                 // +0 and -0 share a key: recompute the winner's own depth bits from its record
-                cls = winner_class(load_rec(recs, slot[s]), C.tx, C.ty, C.fpx, C.fpy, C.bMinX, C.bMinY, d[s]);
+                cls = winner_class(load_rec(recs, slot[s]), C.tx, C.ty, C.fpx, C.fpy, C.bMinX, C.bMinY, d[s],
+                                   P.tagging);
             } else if (S == 1) {
                 cls = (unsigned)k & 3u;
             } else if (S > 1 && (unsigned)(k >> 32) == 0x80000000u) {
@@ -355,7 +339,7 @@ __device__ __forceinline__ void stage_pixel(const RasterParams& P, const TriRecs
         O.d[t * S + s] = d[s];
         O.slot[t * S + s] = slot[s];
     }
-    if (S == 1 && visClass) visClass[C.pix] = (uint8_t)cls;
+    if (S == 1 && visClass) visClass[C.pix] = class_out(cls, P.tagging);
 }
 
 // Outputs of one pixel of a cleared draw straight to memory with 16-byte
@@ -574,8 +558,9 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
                     const unsigned ta = sTa[2 * e + (bi >> 3)];
                     const int px0 = tx0 + (bi & 3) * 4, gy = ty0 + (bi >> 2) * 4 + (c & 1) * 2;
                     // pixels of the 4x2 group: (0,0) (1,0) (0,1) (1,1) (2,0) (3,0) (2,1) (3,1)
+                    const bool accepted = (ta >> (bi & 7)) & 1u;
                     unsigned acc = 0xFFu;
-                    if (!((ta >> (bi & 7)) & 1u)) {
+                    if (!accepted) {
                         acc = 0;
 #pragma unroll
                         for (int l = 0; l < 8; l++) {
@@ -589,11 +574,11 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
                         const int px = px0 + (l & 1) + ((l >> 2) << 1), py = gy + ((l >> 1) & 1);
                         if (px >= G.width || py >= G.height) continue;
                         const unsigned quad = (acc >> (l & 4)) & 0xFu;
-                        const unsigned cls = acc == 0xFFu ? 1u : (quad == 0xFu ? 2u : 3u);
+                        const unsigned cls = expr_code(accepted, acc == 0xFFu, quad == 0xFu, P.tagging);
                         frags++;
                         covs++;
                         merge(&sVis[vis_at(0, (py - ty0) * 16 + (px - tx0))],
-                              class_depth(cls, R.f1, R.f2, R.f3, (float)px, (float)py), (tie << 2) | cls);
+                              class_depth(cls, R.f1, R.f2, R.f3, (float)px, (float)py, P.tagging), (tie << 2) | cls);
                     }
                 } else {
                     const int lx = c & 15, ly = c >> 4;
@@ -863,14 +848,15 @@ __global__ void __launch_bounds__(256) k_raster_ordered(RasterParams P, Views V,
             block_test(E, C.bMinX, C.bMinY, trn, tan);
             if constexpr (S == 1) {
                 if (!covers(E, full, tan, trn, C.fpx, C.fpy)) return;
-                const unsigned c = pixel_class(E, full || tan == 3, C.fpx, C.fpy, C.bMinX);
-                const float d = class_depth(c, R.f1, R.f2, R.f3, C.fpx, C.fpy);
+                const unsigned c = pixel_class(E, full || tan == 3, C.fpx, C.fpy, C.bMinX, P.tagging);
+                const float d = class_depth(c, R.f1, R.f2, R.f3, C.fpx, C.fpy, P.tagging);
                 frags++;
                 covs++;
-                if (!depth_pass(P.depthFunc, d, dep[0], c == 3u)) return;
+                // the scalar path's GREATER quirk belongs to the deterministic mode's single pixels
+                if (!depth_pass(P.depthFunc, d, dep[0], c == 3u && !P.tagging)) return;
                 dep[0] = d;
                 slot[0] = sl;
-                cls = c;
+                cls = class_out(c, P.tagging);
             } else {
                 bool any = false;
 #pragma unroll

@@ -9,10 +9,33 @@ namespace rc {
 
 using namespace dev;
 
-// Depth at a 1-sample pixel for a coverage class (1: SIMD256, 2: SIMD128, 3: scalar).
+// Depth expression codes of a 1-sample fragment, kept in the low bits of its
+// key: 1 SIMD256, 2 SIMD128, 3 scalar DPPS, or in the tagging mode
+// (optsetForceDeterministic(false), TileRasterWorker.cpp:1581-1675) 3 the
+// grouped form.  The deterministic mode draws tile- and block-accepted and
+// 4x2-group fragments with SIMD256, 2x2 quads with SIMD128 and single pixels
+// with the scalar form; the tagging mode draws accepted fragments grouped,
+// 4x2 groups with SIMD256, and quads and single pixels with SIMD128
+// (pixelShadingSingleQuad, TileRasterWorker.cpp:2123-2165).
 __device__ __forceinline__ float class_depth(unsigned c, const float4& F1, const float4& F2, const float4& F3, float x,
-                                             float y) {
-    return c == 1 ? depth256(F1, F2, F3, x, y) : (c == 2 ? depth128(F1, F2, F3, x, y) : depth_scalar(F1, F2, F3, x, y));
+                                             float y, bool tagging = false) {
+    if (c == 1) return depth256(F1, F2, F3, x, y);
+    if (c == 2) return depth128(F1, F2, F3, x, y);
+    return tagging ? depth_grouped(F1, F2, F3, x, y) : depth_scalar(F1, F2, F3, x, y);
+}
+
+// Expression code of a fragment from how it was covered: accepted (tile or
+// 4x4 block), its 4x2 group all inside, its 2x2 quad all inside, or alone.
+__device__ __forceinline__ unsigned expr_code(bool accepted, bool group, bool quad, bool tagging) {
+    if (accepted) return tagging ? 3u : 1u;
+    if (group) return 1u;
+    return quad || tagging ? 2u : 3u;
+}
+
+// The class byte a pixel reports: the expression code, with the tagging
+// mode's grouped form as 4.
+__device__ __forceinline__ uint8_t class_out(unsigned c, bool tagging) {
+    return (uint8_t)(tagging && c == 3u ? 4u : c);
 }
 
 // The view a tile belongs to: its size and where its outputs and slotOf
@@ -107,10 +130,11 @@ __device__ __forceinline__ bool covers(const Edges& E, bool full, int tan, int t
     return full || tan == 3 || (trn == 3 && inside(E, x, y));
 }
 
-// Class of a covered 1-sample pixel (1 SIMD256, 2 SIMD128, 3 scalar): its
-// 4x2 group all inside, else its 2x2 quad all inside, else a single pixel.
-__device__ __forceinline__ unsigned pixel_class(const Edges& E, bool accepted, float fpx, float fpy, float bMinX) {
-    if (accepted) return 1u;
+// Expression code of a covered 1-sample pixel (class_depth): accepted, its
+// 4x2 group all inside, its 2x2 quad all inside, or a single pixel.
+__device__ __forceinline__ unsigned pixel_class(const Edges& E, bool accepted, float fpx, float fpy, float bMinX,
+                                                bool tagging) {
+    if (accepted) return expr_code(true, false, false, tagging);
     const float gy = floorf(fpy * 0.5f) * 2.0f;
     bool g = true;
 #pragma unroll
@@ -120,7 +144,7 @@ __device__ __forceinline__ unsigned pixel_class(const Edges& E, bool accepted, f
     bool q = true;
 #pragma unroll
     for (int l = 0; l < 4; l++) q = q && inside(E, qx + (float)(l & 1), gy + (float)(l >> 1));
-    return q ? 2u : 3u;
+    return expr_code(false, false, q, tagging);
 }
 
 // Ascending bitonic sort of n 64-bit keys in LDS (n a power of two).

@@ -1018,7 +1018,7 @@ __global__ void __launch_bounds__(256) k_blend_ordered(ShadeParams P, BlendCoefs
             int trn, tan;
             block_test(E, C.bMinX, C.bMinY, trn, tan);
             if (!rc::covers(E, full, tan, trn, C.fpx, C.fpy)) return;
-            const unsigned c = rc::pixel_class(E, full || tan == 3, C.fpx, C.fpy, C.bMinX);
+            const unsigned c = rc::pixel_class(E, full || tan == 3, C.fpx, C.fpy, C.bMinX, false);
             const float d = rc::class_depth(c, R.f1, R.f2, R.f3, C.fpx, C.fpy);
             frags++;
             if (!rc::depth_pass(depthFunc, d, dep, c == 3u)) return;

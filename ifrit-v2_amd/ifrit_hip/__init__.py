@@ -256,6 +256,12 @@ class Renderer:
         self.L.iftrSetDepthFunc(self.h, func)
         self._check()
 
+    def set_deterministic(self, on: bool):
+        '''optsetForceDeterministic: False selects the tagging mode's depth expressions at 1 sample per
+        pixel (equal-depth ties still resolve to the lowest primitive).'''
+        self.L.iftrOptsetForceDeterministic(self.h, 1 if on else 0)
+        self._check()
+
     def set_cull_mode(self, mode: int):
         """IF_CULL_MODE_*: 0 none, 1 front, 2 back (the default), 3 front and back."""
         self.L.iftrSetCullMode(self.h, mode)

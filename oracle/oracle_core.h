@@ -26,7 +26,8 @@ enum OracleCoverClass {
     ORC_NONE = 0,
     ORC_SIMD256 = 1,  // TILE, BLOCK or PIXEL_PACK4X2 proposals (TRW:2758-2769)
     ORC_SIMD128 = 2,  // PIXEL_PACK2X2 proposals (TRW:2423-2434)
-    ORC_SCALAR = 3    // PIXEL proposals (TRW:2261-2270)
+    ORC_SCALAR = 3,   // PIXEL proposals (TRW:2261-2270)
+    ORC_GROUPED = 4   // tagging mode: TILE and BLOCK proposals (pixelShadingSIMD256Grouped, TRW:2614-2725)
 };
 
 typedef struct {
@@ -41,6 +42,10 @@ typedef struct {
     int cull_mode;      // IfritCullMode: 2 back (the CPU path); 1 front and 0 none set triangles up with
                         // vertices 0 and 2 exchanged (front: all; none: those back would cull); 3 all
     float clear_depth;  // depth a clear starts from: 0x7f7f7f7f in the CPU path
+    int tagging;        // 1: the tagging (non-deterministic, optsetForceDeterministic(false)) mode's depth
+                        // expressions at 1 sample per pixel: TILE and BLOCK proposals grouped
+                        // (TRW:2672-2684), PIXEL proposals as a quad (pixelShadingSingleQuad, TRW:2123-2165);
+                        // ignored with blending, which the reference always draws deterministically (TRW:410)
 } OracleRasterConfig;
 
 typedef struct {

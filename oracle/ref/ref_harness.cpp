@@ -131,11 +131,13 @@ struct Target {
 
 }  // namespace
 
-extern "C" int ref_render_visibility(int W, int H, int ccw, int depthFunc, const float* clip, int nverts,
-                                     const int32_t* idx, int nidx, uint32_t* id_out, float* depth_out) {
+// tagging: draws in the non-deterministic tagging mode (optsetForceDeterministic(false),
+// TRW:1581-1675), whose depth write-back uses the tile's pixel origin, so depth is read
+// from the depth attachment rather than from the fragment shader's output.
+static int render_visibility(int W, int H, int depthFunc, bool tagging, const float* clip, const int32_t* idx,
+                             int nidx, uint32_t* id_out, float* depth_out) {
     const int ntri = nidx / 3;
     if (ntri <= 0) return -1;
-    (void)nverts;
     Target tgt(W, H);
     VertexBuffer vb;
     vb.setLayout({TypeDescriptors.FLOAT4, TypeDescriptors.FLOAT4});
@@ -167,9 +169,10 @@ extern "C" int ref_render_visibility(int W, int H, int ccw, int depthFunc, const
     r->bindVertexShaderLegacy(vs, vd);
     r->bindFragmentShader(fs);
     r->SetDepthFunc((IfritCompareOp)depthFunc);
-    (void)ccw;
+    r->optsetForceDeterministic(!tagging);
     r->drawElements(3 * ntri, true);
     const float* c = tgt.color->getData();
+    const float* z = tgt.depth->getData();
     int bad = 0;
     for (size_t i = 0; i < (size_t)W * H; i++) {
         if (c[4 * i + 3] == 1.0f) {
@@ -177,13 +180,25 @@ extern "C" int ref_render_visibility(int W, int H, int ccw, int depthFunc, const
             const float rv = std::nearbyint(v);
             if (std::fabs(v - rv) > 0.01f) bad++;
             id_out[i] = (uint32_t)rv;
-            depth_out[i] = c[4 * i + 1];
+            depth_out[i] = tagging ? z[i] : c[4 * i + 1];
         } else {
             id_out[i] = 0xffffffffu;
             depth_out[i] = 0.0f;
         }
     }
     return bad;
+}
+
+extern "C" int ref_render_visibility(int W, int H, int ccw, int depthFunc, const float* clip, int nverts,
+                                     const int32_t* idx, int nidx, uint32_t* id_out, float* depth_out) {
+    (void)ccw;
+    (void)nverts;
+    return render_visibility(W, H, depthFunc, false, clip, idx, nidx, id_out, depth_out);
+}
+
+extern "C" int ref_render_visibility_tagging(int W, int H, int depthFunc, const float* clip, const int32_t* idx,
+                                             int nidx, uint32_t* id_out, float* depth_out) {
+    return render_visibility(W, H, depthFunc, true, clip, idx, nidx, id_out, depth_out);
 }
 
 extern "C" int ref_render_blend(int W, int H, int depthFunc, int srcColor, int dstColor, int srcAlpha, int dstAlpha,

@@ -37,6 +37,22 @@ def _load_golden():
 CLEAR_DEPTH_BITS = 0x7F7F7F7F  # TileRasterWorker.cpp:40
 
 
+def _is_intel():
+    if platform.machine() not in ("x86_64", "AMD64"):
+        return False
+    try:
+        with open("/proc/cpuinfo") as fh:
+            return "GenuineIntel" in fh.read()
+    except OSError:
+        return False
+
+
+# Comparisons with the reference build running on this host use the host's own RCPPS in the
+# oracle: the captured table is Intel's, and another vendor's instruction gives other setup values
+# (the table itself is pinned to the instruction on Intel hosts below).
+HOST_RCP = not _is_intel()
+
+
 def _assert_same(label, ids, depth, ref_ids, ref_depth):
     """The reference harness reads depth back from the fragment shader's
     output (the reference's own depth write-back uses the tile index as the
@@ -85,7 +101,7 @@ def test_oracle_matches_reference_build_random():
         W, H, clip, idx = _random_clip_scene(5000 + seed, seed % 4)
         ids, depth, bad = O.ref_visibility(W, H, clip, idx)
         assert bad == 0
-        f = O.Frame().render(W, H, 1, clip, idx, threads=4)
+        f = O.Frame().render(W, H, 1, clip, idx, threads=4, live_rcp=HOST_RCP)
         _assert_same(f"random seed {seed}", f.ids, f.depth, ids, depth)
 
 
@@ -94,21 +110,11 @@ def test_oracle_matches_reference_build_config1_full_size():
     from ifrit_hip import scenes
 
     s = scenes.config1()
-    f, clip = O.render_scene(s, threads=8)
+    f, clip = O.render_scene(s, threads=8, live_rcp=HOST_RCP)
     ids, depth, bad = O.ref_visibility(s.width, s.height, clip, s.indices)
     assert bad == 0
     _assert_same("config1", f.ids, f.depth, ids, depth)
     assert (ids != 0xFFFFFFFF).sum() > 0.2 * ids.size
-
-
-def _is_intel():
-    if platform.machine() not in ("x86_64", "AMD64"):
-        return False
-    try:
-        with open("/proc/cpuinfo") as fh:
-            return "GenuineIntel" in fh.read()
-    except OSError:
-        return False
 
 
 @pytest.mark.skipif(not _is_intel(), reason="the captured RCPSS table is Intel's")
@@ -199,7 +205,7 @@ def test_oracle_depth_functions_match_reference_build(func):
         W, H, clip, idx = _random_clip_scene(7000 + seed, seed % 4)
         ids, depth, bad = O.ref_visibility(W, H, clip, idx, depth_func=func)
         assert bad == 0
-        f = O.Frame().render(W, H, 1, clip, idx, depth_func=func, threads=4)
+        f = O.Frame().render(W, H, 1, clip, idx, depth_func=func, threads=4, live_rcp=HOST_RCP)
         _assert_same(f"{DEPTH_FUNCS[func]} seed {seed}", f.ids, f.depth, ids, depth)
 
 
@@ -220,7 +226,7 @@ def test_oracle_blending_matches_reference_build(blend, func):
     for seed in range(8):
         W, H, clip, idx = _random_clip_scene(9000 + seed, seed % 4)
         want = O.ref_blend(W, H, clip, idx, BLENDS[blend], depth_func=func)
-        f = O.Frame().render_blend(W, H, clip, idx, BLENDS[blend], depth_func=func, threads=4)
+        f = O.Frame().render_blend(W, H, clip, idx, BLENDS[blend], depth_func=func, threads=4, live_rcp=HOST_RCP)
         bad = np.flatnonzero((f.rgba.view(np.uint32) != want.view(np.uint32)).any(-1))
         assert bad.size == 0, f"seed {seed}: {bad.size} pixels differ, first {bad[:4]}"
         assert (want != 0).any()
@@ -275,7 +281,7 @@ def test_oracle_normal_lambert_matches_reference_build(which):
                                                                                tex_size=64, host_textures=False)
     s = scenes.normal_lambert(base)
     ref = _ref_frame_color(s)
-    f, _ = O.render_scene(s, threads=8)
+    f, _ = O.render_scene(s, threads=8, live_rcp=HOST_RCP)
     got, flags = f.shade(s)
     want = np.floor(np.clip(ref, 0, 1) * 255 + 0.5).astype(int)
     diff = np.abs(got.astype(int) - want).max(-1)
