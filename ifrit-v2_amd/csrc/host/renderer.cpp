@@ -788,6 +788,9 @@ ShadeParams TileRasterRenderer::shadeParams(bool writeFloat) const {
     sh.swapped = needSwapFlags() ? swapped_.as<unsigned char>() : nullptr;
     std::memcpy(sh.clearColor, clearColor_, sizeof(sh.clearColor));
     sh.clearDepth = clearDepth_;
+    sh.depthWrite = fsWritesDepth() ? 1 : 0;  // blending is always drawn deterministically
+    sh.depthScale = d.depthScale;
+    sh.depthBias = d.depthBias;
     sh.scissor = activeScissor();
     return sh;
 }
@@ -842,6 +845,10 @@ void TileRasterRenderer::enqueue(bool clear) {
     // optsetForceDeterministic(false) selects the tagging mode's depth expressions (TRW:1581-1675);
     // blending keeps the deterministic path, as the reference's does (TRW:410)
     rp.tagging = !deterministic_ && samples_ == 1 ? 1 : 0;
+    // a depth-writing program stores its depth after the test; the tagging mode discards it
+    rp.depthWrite = fsWritesDepth() && !rp.tagging ? 1 : 0;
+    rp.depthScale = fs_.desc.depthScale;
+    rp.depthBias = fs_.desc.depthBias;
     rp.scissor = activeScissor();
     rp.binCap = binCap_;
     const float* st = samples_ == 8 ? kS8 : (samples_ == 4 ? kS4 : kS1);
@@ -859,7 +866,7 @@ void TileRasterRenderer::enqueue(bool clear) {
                              rasterGrid(), visDepth_.as<float>(), visSlot_.as<uint32_t>(), cls, rgba8_.as<uchar4>(),
                              rgba32f_.as<float4>(), statPtrs(), s);
         mark(4);
-    } else if (depthFunc_ == IF_COMPARE_OP_LESS) {
+    } else if (depthFunc_ == IF_COMPARE_OP_LESS && !rp.depthWrite) {
         // order-independent: the lexicographic minimum of (depth, primitive)
         launch_raster(rp, V, samples_, recPtrs(), slotOf_.as<unsigned>(), binStart_.as<unsigned>(),
                       binCount_.as<unsigned>(), binList_.as<unsigned>(), q, tiles, visDepth_.as<float>(),

@@ -118,6 +118,7 @@ public:
     void setBlendFunc(const IfritColorAttachmentBlendState& s);
     void SetDepthFunc(int func);
     void optsetForceDeterministic(bool opt) { deterministic_ = opt; }
+    bool fsWritesDepth() const { return fsBound_ && fs_.desc.depthWrite != 0; }
     void optSetDepthTestEnable(bool opt);
     void SetCullMode(int mode);  // TileRasterRendererCuda.h:109
     void setClearValues(const IfritVector4f* colors, size_t n, float depth);  // TileRasterRendererCuda.h:111

@@ -133,6 +133,8 @@ struct RasterParams {
     int clear;
     float clearDepth;    // depth a cleared sample starts from (0x7f7f7f7f in the CPU path)
     int tagging;         // 1 spp: the tagging mode's depth expressions (class_depth)
+    int depthWrite;      // the fragment program stores depth * depthScale + depthBias after the test
+    float depthScale, depthBias;
     Scissor scissor;
     unsigned binCap;
     float sx[8], sy[8];  // sample offsets from the pixel corner
@@ -175,6 +177,8 @@ struct ShadeParams {
     const unsigned char* swapped;  // per record slot: vertices 0 and 2 exchanged by setup (null: none)
     float clearColor[4];     // colour of samples no triangle covers (the resolve weighs them)
     float clearDepth;        // depth a cleared pixel starts from (blending draws)
+    int depthWrite;          // blending draws: the program stores depth * depthScale + depthBias
+    float depthScale, depthBias;
     Scissor scissor;
     SamplerMode samp;        // addressing and filter of the program's lookups
     int sampGeneric;         // 1: samp differs from repeat + bilinear (the sampled-kernel variant runs)

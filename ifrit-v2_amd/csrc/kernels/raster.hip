@@ -854,7 +854,7 @@ __global__ void __launch_bounds__(256) k_raster_ordered(RasterParams P, Views V,
                 covs++;
                 // the scalar path's GREATER quirk belongs to the deterministic mode's single pixels
                 if (!depth_pass(P.depthFunc, d, dep[0], c == 3u && !P.tagging)) return;
-                dep[0] = d;
+                dep[0] = P.depthWrite ? d * P.depthScale + P.depthBias : d;
                 slot[0] = sl;
                 cls = class_out(c, P.tagging);
             } else {
@@ -867,7 +867,7 @@ __global__ void __launch_bounds__(256) k_raster_ordered(RasterParams P, Views V,
                     covs++;
                     const float d = depth256(R.f1, R.f2, R.f3, sx, sy);
                     if (!depth_pass(P.depthFunc, d, dep[s], false)) continue;
-                    dep[s] = d;
+                    dep[s] = P.depthWrite ? d * P.depthScale + P.depthBias : d;
                     slot[s] = sl;
                 }
                 frags += any;

@@ -1025,7 +1025,7 @@ __global__ void __launch_bounds__(256) k_blend_ordered(ShadeParams P, BlendCoefs
             ShadeTri T;
             load_shade_tri(recs, bary, fanStart, sl, T, P.swapped);
             col = blend_pixel(B, shade_fragment(P, cam, T, vtx, idx, tex, pool, C.px, C.py), col);
-            dep = d;
+            dep = P.depthWrite ? d * P.depthScale + P.depthBias : d;
             slot = sl;
             cls = c;
         });

@@ -29,7 +29,8 @@ class FragmentProgramDesc(C.Structure):
                 ("camera", C.c_float * 3), ("ambient", C.c_float), ("kd", C.c_float), ("ks", C.c_float),
                 ("shininess", C.c_float), ("numLights", C.c_int32), ("lightPos", (C.c_float * 4) * 4),
                 ("lightColor", (C.c_float * 4) * 4), ("offTangent", C.c_int32), ("precision", C.c_int32),
-                ("samplerId", C.c_int32), ("lod", C.c_float)]
+                ("samplerId", C.c_int32), ("lod", C.c_float), ("depthWrite", C.c_int32), ("depthScale", C.c_float),
+                ("depthBias", C.c_float)]
 
 
 class SamplerDesc(C.Structure):
@@ -340,6 +341,9 @@ class Renderer:
         d.precision = precision
         d.samplerId = getattr(scene, "sampler_id", 0)
         d.lod = getattr(scene, "lod", 0.0)
+        dw = getattr(scene, "depth_write", None)  # (scale, bias): the program writes depth * scale + bias
+        if dw is not None:
+            d.depthWrite, d.depthScale, d.depthBias = 1, float(dw[0]), float(dw[1])
         d.offWorld, d.offNormal, d.offUv = 0, 3, 6
         d.offTangent = 8 if scene.stride >= 11 else 0
         d.trisPerMaterial = scene.tris_per_material

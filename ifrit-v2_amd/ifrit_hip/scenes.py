@@ -108,6 +108,7 @@ class Scene:
     sampler_state: dict | None = None   # Renderer.create_sampler arguments (None: the program's own sampler)
     sampler_id: int = 0                 # set when the scene is loaded (0: none)
     lod: float = 0.0                    # cube-map program: level of detail
+    depth_write: tuple | None = None    # (scale, bias): the program writes depth * scale + bias
 
     @property
     def stride(self) -> int:

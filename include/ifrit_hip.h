@@ -188,6 +188,14 @@ typedef struct {
                                    bilinear texels; else an id from iftrCreateSampler (Blinn-Phong and
                                    cube-map programs) */
     float lod;                  /* IFHIP_FS_CUBE_MAP: level of detail */
+    int32_t depthWrite;         /* 1: the program writes depth' = depth * depthScale + depthBias through
+                                   the fragment shader's depth out-parameter (Shaders.h:64).  As in the
+                                   CPU path (TileRasterWorker.cpp:2272-2384) the depth test uses the
+                                   interpolated depth and the sample then stores depth'; the draw folds
+                                   fragments in primitive order.  The tagging mode's shading pass
+                                   discards written depth (TileRasterWorker.cpp:2015-2068), so it is
+                                   ignored there. */
+    float depthScale, depthBias;
 } IfritGpuFragmentProgramDesc;
 
 void* iftrCreateGpuVertexProgram(const IfritGpuVertexProgramDesc* desc);

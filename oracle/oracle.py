@@ -28,7 +28,8 @@ CLEAR_DEPTH = float(np.frombuffer(np.uint32(0x7F7F7F7F).tobytes(), np.float32)[0
 class RasterConfig(C.Structure):
     _fields_ = [("width", C.c_int), ("height", C.c_int), ("samples", C.c_int), ("ccw", C.c_int),
                 ("depth_func", C.c_int), ("clear", C.c_int), ("num_threads", C.c_int), ("live_rcp", C.c_int),
-                ("cull_mode", C.c_int), ("clear_depth", C.c_float), ("tagging", C.c_int)]
+                ("cull_mode", C.c_int), ("clear_depth", C.c_float), ("tagging", C.c_int), ("depth_write", C.c_int),
+                ("depth_scale", C.c_float), ("depth_bias", C.c_float)]
 
 
 class Stats(C.Structure):
@@ -141,6 +142,9 @@ def ref():
                                           C.c_void_p, C.c_int, C.c_void_p]
         _ref.ref_render_timed.restype = C.c_double
         _ref.ref_render_blend.argtypes = [C.c_int] * 7 + [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+        if hasattr(_ref, "ref_render_visibility_depthwrite"):
+            _ref.ref_render_visibility_depthwrite.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                                              C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         if hasattr(_ref, "ref_render_visibility_tagging"):
             _ref.ref_render_visibility_tagging.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
                                                            C.c_void_p, C.c_void_p]
@@ -177,7 +181,7 @@ class Frame:
             self._h = None
 
     def render(self, width, height, samples, clip, indices, depth_func=1, ccw=0, threads=None, live_rcp=False,
-               clear=True, depth_in=None, cull_mode=2, clear_depth=CLEAR_DEPTH, tagging=False):
+               clear=True, depth_in=None, cull_mode=2, clear_depth=CLEAR_DEPTH, tagging=False, depth_write=None):
         clip = np.ascontiguousarray(clip, np.float32)
         indices = np.ascontiguousarray(indices, np.int32)
         n = width * height
@@ -188,7 +192,7 @@ class Frame:
         st = Stats()
         cfg = RasterConfig(width, height, samples, ccw, depth_func, 1 if clear else 0,
                            threads or os.cpu_count() or 1, 1 if live_rcp else 0, cull_mode, clear_depth,
-                           1 if tagging else 0)
+                           1 if tagging else 0, *_depth_write(depth_write))
         din = None if depth_in is None else np.ascontiguousarray(depth_in, np.float32)
         rc = lib().oracle_render(self._h, C.byref(cfg), _p(clip), clip.shape[0], _p(indices), indices.size,
                                  None if din is None else _p(din), _p(ids), _p(depth), _p(tri), _p(cls), C.byref(st))
@@ -199,7 +203,7 @@ class Frame:
         return self
 
     def render_blend(self, width, height, clip, indices, factors, depth_func=1, threads=None, clear=True,
-                     depth_in=None, rgba_in=None, live_rcp=False):
+                     depth_in=None, rgba_in=None, live_rcp=False, depth_write=None):
         """Blending of the translucent flat colour in primitive order (1 sample per pixel);
         factors = (src colour, dst colour, src alpha, dst alpha) as IfritBlendFactor values."""
         clip = np.ascontiguousarray(clip, np.float32)
@@ -209,7 +213,7 @@ class Frame:
         depth = np.empty((n,), np.float32)
         rgba = np.empty((n, 4), np.float32)
         cfg = RasterConfig(width, height, 1, 0, depth_func, 1 if clear else 0, threads or os.cpu_count() or 1,
-                           1 if live_rcp else 0, 2, CLEAR_DEPTH, 0)
+                           1 if live_rcp else 0, 2, CLEAR_DEPTH, 0, *_depth_write(depth_write))
         bl = (C.c_int * 4)(*factors)
         din = None if depth_in is None else np.ascontiguousarray(depth_in, np.float32)
         cin = None if rgba_in is None else np.ascontiguousarray(rgba_in, np.float32)
@@ -269,10 +273,16 @@ class Frame:
         return rgba.reshape(self.height, self.width, 4), flags.reshape(self.height, self.width)
 
 
+def _depth_write(dw):
+    """(scale, bias) of a depth-writing fragment program, or None."""
+    return (0, 1.0, 0.0) if dw is None else (1, float(dw[0]), float(dw[1]))
+
+
 def render_scene(scene, threads=None, live_rcp=False, cull_mode=2, clear_depth=CLEAR_DEPTH, tagging=False):
     clip = vertex_transform(scene.vertices, scene.mvp)
     f = Frame().render(scene.width, scene.height, scene.samples, clip, scene.indices, threads=threads,
-                       live_rcp=live_rcp, cull_mode=cull_mode, clear_depth=clear_depth, tagging=tagging)
+                       live_rcp=live_rcp, cull_mode=cull_mode, clear_depth=clear_depth, tagging=tagging,
+                       depth_write=getattr(scene, "depth_write", None))
     return f, clip
 
 
@@ -295,6 +305,18 @@ def ref_visibility_tagging(width, height, clip, indices, depth_func=1):
     depth = np.empty((width * height,), np.float32)
     bad = ref().ref_render_visibility_tagging(width, height, depth_func, _p(clip), _p(indices), indices.size,
                                               _p(ids), _p(depth))
+    return ids, depth, bad
+
+
+def ref_visibility_depthwrite(width, height, clip, indices, scale, bias, depth_func=1):
+    """The reference build with a fragment shader that writes depth * scale + bias through its depth
+    out-parameter: primitive ID per pixel and the written depth (read back from the colour)."""
+    clip = np.ascontiguousarray(clip, np.float32)
+    indices = np.ascontiguousarray(indices, np.int32)
+    ids = np.empty((width * height,), np.uint32)
+    depth = np.empty((width * height,), np.float32)
+    bad = ref().ref_render_visibility_depthwrite(width, height, depth_func, scale, bias, _p(clip), _p(indices),
+                                                 indices.size, _p(ids), _p(depth))
     return ids, depth, bad
 
 

@@ -46,6 +46,10 @@ typedef struct {
                         // expressions at 1 sample per pixel: TILE and BLOCK proposals grouped
                         // (TRW:2672-2684), PIXEL proposals as a quad (pixelShadingSingleQuad, TRW:2123-2165);
                         // ignored with blending, which the reference always draws deterministically (TRW:410)
+    int depth_write;    // 1: a fragment that passes the depth test stores depth * depth_scale + depth_bias, as
+                        // a fragment shader writing its depth out-parameter does in the CPU path
+                        // (TRW:2272-2384); ignored in the tagging mode, whose shading pass discards it
+    float depth_scale, depth_bias;
 } OracleRasterConfig;
 
 typedef struct {

@@ -14,7 +14,10 @@
 //                          shading) timed with the reference's own 17 threads;
 //   ref_render_blend       the FP32 colour attachment of a draw with blending
 //                          and a given depth function, each triangle coloured
-//                          by a hash of its index (colour and alpha).
+//                          by a hash of its index (colour and alpha);
+//   ref_render_visibility_tagging / _depthwrite
+//                          ref_render_visibility in the tagging mode, or with a
+//                          fragment shader that writes depth * scale + bias.
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -118,6 +121,23 @@ public:
     }
 };
 
+// Writes depth * scale + bias through the depth out-parameter and records
+// (interpolated varying 0.x, written depth, 1, 1).
+class DepthAffineFS : public FragmentShader {
+public:
+    float scale = 1.0f, bias = 0.0f;
+    void execute(const void* vary, void* color, float* depth) override {
+        const Vector4f v = reinterpret_cast<const Vector4f*>(vary)[0];
+        const float d = *depth * scale + bias;
+        *depth = d;
+        Vector4f* c = reinterpret_cast<Vector4f*>(color);
+        c->x = v.x;
+        c->y = d;
+        c->z = 1.0f;
+        c->w = 1.0f;
+    }
+};
+
 struct Target {
     std::unique_ptr<ImageF32> color, depth;
     FrameBuffer fb;
@@ -135,7 +155,7 @@ struct Target {
 // TRW:1581-1675), whose depth write-back uses the tile's pixel origin, so depth is read
 // from the depth attachment rather than from the fragment shader's output.
 static int render_visibility(int W, int H, int depthFunc, bool tagging, const float* clip, const int32_t* idx,
-                             int nidx, uint32_t* id_out, float* depth_out) {
+                             int nidx, uint32_t* id_out, float* depth_out, const float* depthAffine = nullptr) {
     const int ntri = nidx / 3;
     if (ntri <= 0) return -1;
     Target tgt(W, H);
@@ -158,7 +178,13 @@ static int render_visibility(int W, int H, int depthFunc, bool tagging, const fl
     mgr->bufferData(ib, seq.data(), 0, seq.size() * sizeof(int));
 
     PassVS vs;
-    IdDepthFS fs;
+    IdDepthFS idfs;
+    DepthAffineFS affine;
+    if (depthAffine) {
+        affine.scale = depthAffine[0];
+        affine.bias = depthAffine[1];
+    }
+    FragmentShader& fs = depthAffine ? static_cast<FragmentShader&>(affine) : static_cast<FragmentShader&>(idfs);
     VaryingDescriptor vd;
     vd.setVaryingDescriptors({TypeDescriptors.FLOAT4});
     auto r = std::make_shared<TileRasterRenderer>();
@@ -194,6 +220,13 @@ extern "C" int ref_render_visibility(int W, int H, int ccw, int depthFunc, const
     (void)ccw;
     (void)nverts;
     return render_visibility(W, H, depthFunc, false, clip, idx, nidx, id_out, depth_out);
+}
+
+extern "C" int ref_render_visibility_depthwrite(int W, int H, int depthFunc, float scale, float bias,
+                                                const float* clip, const int32_t* idx, int nidx, uint32_t* id_out,
+                                                float* depth_out) {
+    const float affine[2] = {scale, bias};
+    return render_visibility(W, H, depthFunc, false, clip, idx, nidx, id_out, depth_out, affine);
 }
 
 extern "C" int ref_render_visibility_tagging(int W, int H, int depthFunc, const float* clip, const int32_t* idx,

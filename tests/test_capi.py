@@ -159,9 +159,12 @@ def test_batch_and_tuning_struct_layouts():
 #include <stdio.h>
 #include "ifrit_hip.h"
 int main(void) {
-    printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(IfritViewDesc), offsetof(IfritViewDesc, camera),
+    printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(IfritViewDesc), offsetof(IfritViewDesc, camera),
            offsetof(IfritViewDesc, shade), sizeof(IfritTuning), sizeof(IfritAutotuneReport),
-           offsetof(IfritAutotuneReport, tried), offsetof(IfritAutotuneReport, ms));
+           offsetof(IfritAutotuneReport, tried), offsetof(IfritAutotuneReport, ms),
+           sizeof(IfritGpuFragmentProgramDesc), offsetof(IfritGpuFragmentProgramDesc, samplerId),
+           offsetof(IfritGpuFragmentProgramDesc, depthWrite), offsetof(IfritGpuFragmentProgramDesc, depthBias),
+           sizeof(IfritSamplerDesc), sizeof(IfritImageBlit));
     return 0;
 }
 '''
@@ -175,7 +178,10 @@ int main(void) {
         got = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
     want = [C.sizeof(ifrit_hip.ViewDesc), ifrit_hip.ViewDesc.camera.offset, ifrit_hip.ViewDesc.shade.offset,
             C.sizeof(ifrit_hip.Tuning), C.sizeof(ifrit_hip.AutotuneReport), ifrit_hip.AutotuneReport.tried.offset,
-            ifrit_hip.AutotuneReport.ms.offset]
+            ifrit_hip.AutotuneReport.ms.offset, C.sizeof(ifrit_hip.FragmentProgramDesc),
+            ifrit_hip.FragmentProgramDesc.samplerId.offset, ifrit_hip.FragmentProgramDesc.depthWrite.offset,
+            ifrit_hip.FragmentProgramDesc.depthBias.offset, C.sizeof(ifrit_hip.SamplerDesc),
+            C.sizeof(ifrit_hip.ImageBlit)]
     assert got == want
 
 
