@@ -25,6 +25,7 @@ SOURCES = [
     "kernels/raster.hip",
     "kernels/shade.hip",
     "kernels/texture.hip",
+    "kernels/polygon.hip",
     "kernels/neural.hip",
     "host/renderer.cpp",
     "host/neural.cpp",

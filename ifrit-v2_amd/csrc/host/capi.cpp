@@ -243,6 +243,10 @@ void iftrSetScissorTestEnable(IfritRendererHandle* h, int enable) {
 void iftrSetCullMode(IfritRendererHandle* h, int mode) {
     guarded(h, [&](TileRasterRenderer& r) { r.SetCullMode(mode); });
 }
+
+void iftrSetRasterizerPolygonMode(IfritRendererHandle* h, int mode) {
+    guarded(h, [&](TileRasterRenderer& r) { r.setRasterizerPolygonMode(mode); });
+}
 void iftrSetMsaaSamples(IfritRendererHandle* h, int s) {
     guarded(h, [&](TileRasterRenderer& r) { r.setMsaaSamples(s); });
 }

@@ -274,6 +274,11 @@ void TileRasterRenderer::SetCullMode(int mode) {
     cullMode_ = mode;
 }
 
+void TileRasterRenderer::setRasterizerPolygonMode(int mode) {
+    if (mode < 0 || mode > 2) return setError("unknown polygon mode");
+    polygonMode_ = mode;
+}
+
 void TileRasterRenderer::setMsaaSamples(int s) {
     if (s != 1 && s != 4 && s != 8) {
         setError("MSAA sample count must be 1, 4 or 8");
@@ -459,6 +464,11 @@ bool TileRasterRenderer::prepare(int indexCount) {
         setError("unknown depth function");
         return false;
     }
+    if (polygonMode_ != 0 && (samples_ != 1 || !batch_.empty() || blend_.blendEnable || fsWritesDepth())) {
+        // the CUDA point and line pipelines draw one sample per pixel into one target (CUI:2931-3335)
+        setError("point and line modes draw a single target at 1 sample per pixel, without blending or depth writes");
+        return false;
+    }
     const int fk = fs_.desc.kind;
     if (blend_.blendEnable && (samples_ != 1 || !batch_.empty() || !fsBound_ ||
                                (fk != IFHIP_FS_FLAT && fk != IFHIP_FS_FLAT_ALPHA && fk != IFHIP_FS_BLINN_PHONG &&
@@ -558,6 +568,7 @@ bool TileRasterRenderer::prepare(int indexCount) {
     }
     const int ntri = indexCount / 3;
     curTris_ = ntri;
+    curIdx_ = indexCount;
     clipCap_ = std::max<unsigned>(clipCap_, std::max(4096u, (unsigned)(ntri / 8)));
     if (!layoutViews(ntri)) return false;
     const size_t nv = viewDesc_.size();
@@ -571,7 +582,8 @@ bool TileRasterRenderer::prepare(int indexCount) {
     bigCap_ = std::max<unsigned>(bigCap_, std::max(1024u, (unsigned)std::min<size_t>(0x7FFFFFFFull, ntri * nv / 16)));
     itemCap_ = std::max<unsigned>(itemCap_, (unsigned)tiles + binCap_ / part + 1u);
     partialCap_ = std::max<unsigned>(partialCap_, std::max(256u, binCap_ / (8u * part)));
-    const size_t slots = ((size_t)ntri + clipCap_) * nv;
+    // point and line records take one slot per index (3 per triangle)
+    const size_t slots = std::max(((size_t)ntri + clipCap_) * nv, polygonMode_ != 0 ? (size_t)indexCount : 0);
     const bool wantFloat = batch_.empty() && ((fb_ && !fb_->color.empty() && fb_->color[0]) || floatOut_ ||
                                               blend_.blendEnable);
     bool okAll = clip_.ensure((size_t)nverts_ * 16 * nv) &&
@@ -589,7 +601,8 @@ bool TileRasterRenderer::prepare(int indexCount) {
                  binCount_.ensure(tiles * 4) && binStart_.ensure(tiles * 4) && binCursor_.ensure(tiles * 4) &&
                  binList_.ensure((size_t)binCap_ * 4) && bigList_.ensure((size_t)bigCap_ * 4) &&
                  visId_.ensure(ps * 4) && visDepth_.ensure(ps * 4) && visSlot_.ensure(ps * 4) &&
-                 visClass_.ensure(px) && rgba8_.ensure(px * 4) && (!wantFloat || rgba32f_.ensure(px * 16));
+                 visClass_.ensure(px) && rgba8_.ensure(px * 4) && (!wantFloat || rgba32f_.ensure(px * 16)) &&
+                 (polygonMode_ == 0 || polyKeys_.ensure(px * 8));
     if (!okAll) {
         setError("device allocation failed");
         return false;
@@ -814,6 +827,10 @@ void TileRasterRenderer::enqueue(bool clear) {
                       ctr_.as<DeviceCounters>(), groupBounds_.as<float4>(), groups,
                       cullGroups ? groupOut_.as<unsigned char>() : nullptr, s);
     mark(1);
+    if (polygonMode_ != 0) {
+        enqueuePolygons(clear, V);
+        return;
+    }
     SetupParams sp{};
     sp.ntri = curTris_;
     sp.nverts = nverts_;
@@ -893,6 +910,44 @@ void TileRasterRenderer::enqueue(bool clear) {
     mark(6);
 }
 
+// Point and line modes: candidates per pixel, the resolve, then the deferred
+// shading of the winners' records (their planes weight the point's vertex or
+// the line's two ends).  Every workgroup of the candidate launch writes its
+// statistics partials, so the grid covers the setup and raster partials.
+void TileRasterRenderer::enqueuePolygons(bool clear, const Views& V) {
+    hipStream_t s = stream_;
+    auto mark = [&](int i) {
+        if (stageTiming_ && !capturing_) (void)hipEventRecord(ev_[i], s);
+    };
+    PolyParams pp{};
+    pp.mode = polygonMode_;
+    pp.width = width_;
+    pp.height = height_;
+    pp.nidx = curIdx_;
+    pp.nverts = nverts_;
+    pp.clear = clear ? 1 : 0;
+    pp.clearDepth = clearDepth_;
+    pp.keys = polyKeys_.as<unsigned long long>();
+    const StatPartials st = statPtrs();
+    const int grid = std::max(std::max(st.setupBlocks, st.rasterBlocks), 1);
+    mark(2);
+    mark(3);
+    launch_polygons(pp, clip_.as<float4>(), idx_.as<int>(), recPtrs(), bary_.as<float4>(), visDepth_.as<float>(),
+                    visSlot_.as<uint32_t>(), visClass_.as<uint8_t>(), st, ctr_.as<DeviceCounters>(), grid, s);
+    mark(4);
+    mark(5);
+    if (fsBound_ && fs_.desc.kind != IFHIP_FS_DEPTH_ONLY && shadeTiles_ > 0) {
+        const bool wantFloat = (fb_ && !fb_->color.empty() && fb_->color[0]) || floatOut_;
+        ShadeParams sh = shadeParams(wantFloat);
+        sh.ntri = 0;  // every record carries its own vertex weights (bx, by)
+        sh.swapped = nullptr;
+        launch_shade(sh, V, (int)shadeTiles_, recPtrs(), bary_.as<float4>(), visSlot_.as<uint32_t>(),
+                     vtx_.as<float>(), idx_.as<int>(), texDesc_.as<TexDesc>(), texPool_, rgba8_.as<uchar4>(),
+                     wantFloat ? rgba32f_.as<float4>() : nullptr, s);
+    }
+    mark(6);
+}
+
 // Everything a captured graph bakes into its launches.  View tables live in
 // device memory, so cameras and transforms change without a new capture.
 std::vector<uint8_t> TileRasterRenderer::graphKey(bool clear) const {
@@ -906,7 +961,8 @@ std::vector<uint8_t> TileRasterRenderer::graphKey(bool clear) const {
                         partShift_, rasterGroupsPerCU_, shadeTile_, bigThreshold_, nnFallback_[0] ? 1 : 0,
                         nnFallback_[1] ? 1 : 0, (int)textures_.size(), stageTiming_ ? 1 : 0,
                         fb_ && !fb_->color.empty() && fb_->color[0] ? 1 : 0, batch_.empty() ? 1 : 0,
-                        floatOut_ ? 1 : 0, blend_.blendEnable ? 1 : 0, cullMode_, deterministic_ ? 1 : 0};
+                        floatOut_ ? 1 : 0, blend_.blendEnable ? 1 : 0, cullMode_, deterministic_ ? 1 : 0,
+                        polygonMode_, curIdx_};
     put(ints, sizeof(ints));
     const unsigned caps[] = {clipCap_, binCap_, bigCap_, itemCap_, partialCap_};
     put(caps, sizeof(caps));
@@ -1071,7 +1127,8 @@ void TileRasterRenderer::getViewVisibility(int view, uint32_t* ids, float* depth
     const ViewDesc& D = viewDesc_[view];
     const size_t n = (size_t)D.width * D.height * samples_, off = (size_t)D.pixBase * samples_;
     if (ids) {
-        launch_slot_to_prim(visSlot_.as<uint32_t>() + off, recPtrs(), n, visId_.as<uint32_t>() + off, stream_);
+        launch_slot_to_prim(visSlot_.as<uint32_t>() + off, recPtrs(), n, visId_.as<uint32_t>() + off, stream_,
+                            polygonMode_ != 0);
         if (!ok(hipStreamSynchronize(stream_), "id expansion")) return;
         ok(hipMemcpy(ids, visId_.as<uint32_t>() + off, n * 4, hipMemcpyDeviceToHost), "id readback");
     }
@@ -1156,7 +1213,7 @@ uint64_t TileRasterRenderer::digest() {
     if (viewDesc_.empty() || !digest_.ensure(3 * 8)) return 0;
     const size_t ps = totalPixels_ * samples_;
     // primitive IDs (not record slots, whose numbering depends on capacities)
-    launch_slot_to_prim(visSlot_.as<uint32_t>(), recPtrs(), ps, visId_.as<uint32_t>(), stream_);
+    launch_slot_to_prim(visSlot_.as<uint32_t>(), recPtrs(), ps, visId_.as<uint32_t>(), stream_, polygonMode_ != 0);
     if (!ok(hipMemsetAsync(digest_.ptr, 0, 3 * 8, stream_), "digest")) return 0;
     // samples outside any view (64-pixel padding) hold stale values: digest views one by one
     for (const ViewDesc& D : viewDesc_) {

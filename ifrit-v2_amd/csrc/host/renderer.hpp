@@ -121,6 +121,7 @@ public:
     bool fsWritesDepth() const { return fsBound_ && fs_.desc.depthWrite != 0; }
     void optSetDepthTestEnable(bool opt);
     void SetCullMode(int mode);  // TileRasterRendererCuda.h:109
+    void setRasterizerPolygonMode(int mode);  // TileRasterRendererCuda.h:106: 0 fill, 1 line, 2 point
     void setClearValues(const IfritVector4f* colors, size_t n, float depth);  // TileRasterRendererCuda.h:111
     void SetScissors(const IfritVector4f* rects, size_t n);                    // TileRasterRendererCuda.h:101
     void setScissorTestEnable(bool on) { scissorTest_ = on; }                  // TileRasterRendererCuda.h:102
@@ -179,6 +180,7 @@ private:
     bool prepare(int indexCount);
     bool layoutViews(int ntri);
     void enqueue(bool clear);
+    void enqueuePolygons(bool clear, const Views& V);
     void submit(bool clear);
     std::vector<uint8_t> graphKey(bool clear) const;
     void dropGraph();
@@ -195,6 +197,7 @@ private:
     int depthFuncSaved_ = IF_COMPARE_OP_LESS;
     bool depthTest_ = true;
     int cullMode_ = kCullBack;  // the CPU path's fixed back-face cull
+    int polygonMode_ = 0;       // IfritPolygonMode: 0 fill, 1 line, 2 point
     float clearColor_[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // TileRasterWorker.cpp:113-118
     float clearDepth_ = 3.3961514e38f;                  // 0x7f7f7f7f, TileRasterWorker.cpp:40
     std::vector<IfritVector4f> scissors_;  // x, y, width, height in pixels
@@ -254,10 +257,12 @@ private:
     DevBuf vtx_, idx_, clip_, recs_[6], bary_, binQueue_, slotOf_, binCount_, binStart_, binCursor_, binList_, bigList_, ctr_;
     DevBuf groupBounds_, groupOut_;  // per triangle group: bounds; per view and group: outside flag
     DevBuf swapped_;                 // per record slot: set up with vertices 0 and 2 exchanged
+    DevBuf polyKeys_;                // point and line modes: per pixel the nearest candidate's key
     DevBuf clipList_, statPart_, items_, tilePartial_, multiList_, partials_, visId_, visDepth_, visSlot_, visClass_, rgba8_, rgba32f_, texDesc_;
     unsigned clipCap_ = 0, binCap_ = 0, bigCap_ = 0, itemCap_ = 0, partialCap_ = 0;
     int numCUs_ = 256;
     int curTris_ = 0;
+    int curIdx_ = 0;  // indices of the current draw (the point mode draws every index)
     TriRecs recPtrs() const;
     StatPartials statPtrs() const;
     int rasterGrid() const;

@@ -288,7 +288,10 @@ void launch_raster_ordered(const RasterParams& p, const Views& views, int sample
                            const unsigned* binStart, const unsigned* binCount, const unsigned* binList, int tiles,
                            int grid, float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st,
                            hipStream_t s);
-void launch_slot_to_prim(const uint32_t* visSlot, TriRecs recs, size_t n, uint32_t* visId, hipStream_t s);
+// IDs from record slots: the primitive key >> 2, or for point and line records (polygon) the key's
+// triangle * 3 + its point or edge.
+void launch_slot_to_prim(const uint32_t* visSlot, TriRecs recs, size_t n, uint32_t* visId, hipStream_t s,
+                         bool polygon = false);
 void launch_shade(const ShadeParams& p, const Views& views, int shadeTiles, TriRecs recs, const float4* bary, const uint32_t* visSlot,
                   const float* vtx, const int* idx, const TexDesc* tex, const uint8_t* texPool, uchar4* rgba8,
                   float4* rgba32f, hipStream_t s);
@@ -302,6 +305,23 @@ void launch_blend_ordered(const ShadeParams& p, const BlendCoefs& b, int depthFu
                           const unsigned* binCount, const unsigned* binList, int tiles, int grid, float* visDepth,
                           uint32_t* visSlot, uint8_t* visClass, uchar4* rgba8, float4* rgba32f, StatPartials st,
                           hipStream_t s);
+
+// Point (mode 2) and line (mode 1) polygon modes, one sample per pixel (polygon.hip):
+// candidates per pixel meet in keys (order_key(depth) << 32 | primitive id),
+// then the resolve writes depth and record slot where the nearest one is below
+// the stored depth.  Record slots: the index position of a point, 3 * triangle
+// + edge of a line.
+struct PolyParams {
+    int mode;
+    int width, height;
+    int nidx, nverts;
+    int clear;
+    float clearDepth;
+    unsigned long long* keys;  // width * height
+};
+void launch_polygons(const PolyParams& p, const float4* clip, const int* idx, TriRecs recs, float4* bary,
+                     float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st, DeviceCounters* ctr,
+                     int grid, hipStream_t s);
 
 // Texture helpers (texture.hip).
 // Order-independent 64-bit digest of n words (equal buffers give equal digests).

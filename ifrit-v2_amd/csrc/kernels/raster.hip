@@ -892,10 +892,11 @@ __global__ void __launch_bounds__(256) k_raster_ordered(RasterParams P, Views V,
 
 // Primitive ID per sample from the record slot (readback only).
 __global__ void __launch_bounds__(256) k_slot_to_prim(const uint32_t* __restrict__ visSlot, const float4* __restrict__ q2,
-                                                      size_t n, uint32_t* __restrict__ visId) {
+                                                      size_t n, uint32_t* __restrict__ visId, int polygon) {
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
         const unsigned slot = visSlot[i];
-        visId[i] = slot == 0xffffffffu ? 0xffffffffu : (__float_as_uint(q2[slot].y) >> 2);
+        const unsigned key = slot == 0xffffffffu ? 0u : __float_as_uint(q2[slot].y);
+        visId[i] = slot == 0xffffffffu ? 0xffffffffu : (polygon ? (key >> 2) * 3u + (key & 3u) : key >> 2);
     }
 }
 
@@ -926,8 +927,9 @@ void launch_digest(const uint32_t* data, size_t n, unsigned long long* out, hipS
                        out);
 }
 
-void launch_slot_to_prim(const uint32_t* visSlot, TriRecs recs, size_t n, uint32_t* visId, hipStream_t s) {
-    hipLaunchKernelGGL(k_slot_to_prim, dim3(2048), dim3(256), 0, s, visSlot, recs.q2, n, visId);
+void launch_slot_to_prim(const uint32_t* visSlot, TriRecs recs, size_t n, uint32_t* visId, hipStream_t s,
+                         bool polygon) {
+    hipLaunchKernelGGL(k_slot_to_prim, dim3(2048), dim3(256), 0, s, visSlot, recs.q2, n, visId, polygon ? 1 : 0);
 }
 
 // The raster and merge launches are separate so that the host can time them.

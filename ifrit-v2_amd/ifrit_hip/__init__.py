@@ -123,7 +123,7 @@ EXPORTED_SYMBOLS = [
     "iftrNeuralTrain", "iftrNeuralSetParams", "iftrNeuralGetParams", "iftrNeuralEval", "iftrNeuralValidate",
     "iftrSetViews", "iftrGetViewVisibility", "iftrGetViewColorRGBA8", "iftrSetGraphMode", "iftrSetTuning",
     "iftrGetTuning", "iftrAutotune", "iftrGetDigest", "iftrSetColorFloatOutput", "iftrGetColorRGBA32F",
-    "iftrDrawNoAlloc", "iftrNeuralValidateEx", "iftrSetCullMode", "iftrSetClearValues", "iftrSetScissors",
+    "iftrDrawNoAlloc", "iftrNeuralValidateEx", "iftrSetCullMode", "iftrSetRasterizerPolygonMode", "iftrSetClearValues", "iftrSetScissors",
     "iftrSetScissorTestEnable", "iftrCreateTextureLayers", "iftrCopyHostBufferToImageLayer", "iftrCreateSampler",
     "iftrBlitImage",
 ]
@@ -195,7 +195,7 @@ def lib():
         "iftrGetDigest": ([vp], C.c_uint64),
         "iftrSetColorFloatOutput": ([vp, C.c_int], None), "iftrGetColorRGBA32F": ([vp, vp], None),
         "iftrDrawNoAlloc": ([vp, C.POINTER(NoAllocDraw)], None),
-        "iftrSetCullMode": ([vp, C.c_int], None),
+        "iftrSetCullMode": ([vp, C.c_int], None), "iftrSetRasterizerPolygonMode": ([vp, C.c_int], None),
         "iftrSetClearValues": ([vp, vp, C.c_size_t, C.c_float], None),
         "iftrSetScissors": ([vp, vp, C.c_size_t], None), "iftrSetScissorTestEnable": ([vp, C.c_int], None),
         "iftrCreateTextureLayers": ([vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
@@ -261,6 +261,11 @@ class Renderer:
         '''optsetForceDeterministic: False selects the tagging mode's depth expressions at 1 sample per
         pixel (equal-depth ties still resolve to the lowest primitive).'''
         self.L.iftrOptsetForceDeterministic(self.h, 1 if on else 0)
+        self._check()
+
+    def set_polygon_mode(self, mode: int):
+        '''IF_POLYGON_MODE_*: 0 fill (the default), 1 line, 2 point.'''
+        self.L.iftrSetRasterizerPolygonMode(self.h, mode)
         self._check()
 
     def set_cull_mode(self, mode: int):

@@ -216,6 +216,23 @@ enum {
    draws nothing.  Exchanged triangles shade with their own vertex attributes. */
 void iftrSetCullMode(IfritRendererHandle* h, int mode);
 
+/* IfritPolygonMode, include/ifrit/softgraphics/engine/base/Constants.h:23-28 */
+enum { IF_POLYGON_MODE_FILL = 0, IF_POLYGON_MODE_LINE = 1, IF_POLYGON_MODE_POINT = 2 };
+/* setRasterizerPolygonMode of the CUDA renderer (TileRasterRendererCuda.h:106), after its point and
+   line pipelines (TileRasterCoreInvocationCuda.cu:2780-3335), at 1 sample per pixel on a single
+   target without blending.  POINT draws every index i of whole triangles as a point: clip position divided by w
+   (points with w < 1e-8 dropped), mapped to [0, 1] and culled outside it, at pixel
+   (int(x (W - 1)), int(y (H - 1))) with depth z / w; its primitive id is i.  LINE draws the three
+   edges (v0 v1, v1 v2, v2 v0) of every triangle: an end behind w = 0 is cut at w = 1, and a line is
+   culled when its first end leaves [0, 1]^2 or its second end has x < 0 or y < 0 (the spec's test);
+   its pixels are the spec's Bresenham walk, each with the depth interpolated along the line's major
+   axis; its primitive id is 3 * triangle + edge.  Per pixel the nearest candidate below 1e9 wins
+   (equal depths: the lowest id) when it is below the stored depth; the depth function is not used,
+   as in the spec.  The winner is shaded with the bound program from the point's vertex, or from the
+   line's two ends perspective-correctly; textures are sampled without screen-space derivatives.
+   Depth -0 is stored as +0, and pixels the spec would write outside the frame are dropped. */
+void iftrSetRasterizerPolygonMode(IfritRendererHandle* h, int mode);
+
 /* setClearValues of the CUDA renderer (TileRasterRendererCuda.h:111): the colour and depth a draw
    with clear starts from.  The first colour applies to the (one) colour attachment; samples no
    triangle covers resolve with it.  The defaults are the CPU path's: colour 0 and depth

@@ -103,6 +103,9 @@ def lib():
         _lib.oracle_render_blend.argtypes = [C.c_void_p, C.POINTER(RasterConfig), C.c_void_p, C.c_int, C.c_void_p,
                                              C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                              C.c_void_p]
+        _lib.oracle_render_polygons.argtypes = [C.c_void_p, C.POINTER(RasterConfig), C.c_int, C.c_void_p, C.c_int,
+                                                C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                C.POINTER(Stats)]
         _lib.oracle_vertex_transform.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
         _lib.oracle_shade.argtypes = [C.c_void_p, C.POINTER(ShadeParams), C.c_int, C.c_int, C.c_int, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]
@@ -223,6 +226,29 @@ class Frame:
         if rc != 0:
             raise RuntimeError(f"oracle_render_blend failed ({rc})")
         self.ids, self.depth, self.rgba = ids, depth, rgba.reshape(height, width, 4)
+        return self
+
+    def render_polygons(self, width, height, clip, indices, mode, threads=None, clear=True, depth_in=None,
+                        clear_depth=CLEAR_DEPTH):
+        """Polygon mode 1 (lines) or 2 (points), 1 sample per pixel."""
+        clip = np.ascontiguousarray(clip, np.float32)
+        indices = np.ascontiguousarray(indices, np.int32)
+        n = width * height
+        ids = np.empty((n,), np.uint32)
+        depth = np.empty((n,), np.float32)
+        tri = np.empty((n,), np.uint32)
+        st = Stats()
+        cfg = RasterConfig(width, height, 1, 0, 1, 1 if clear else 0, threads or os.cpu_count() or 1, 0, 2,
+                           clear_depth, 0, 0, 1.0, 0.0)
+        din = None if depth_in is None else np.ascontiguousarray(depth_in, np.float32)
+        rc = lib().oracle_render_polygons(self._h, C.byref(cfg), mode, _p(clip), clip.shape[0], _p(indices),
+                                          indices.size, None if din is None else _p(din), _p(ids), _p(depth), _p(tri),
+                                          C.byref(st))
+        if rc != 0:
+            raise RuntimeError(f"oracle_render_polygons failed ({rc})")
+        self.width, self.height, self.samples = width, height, 1
+        self.ids, self.depth, self.tri, self.stats = ids, depth, tri, st.as_dict()
+        self.cls = np.zeros((n,), np.uint8)
         return self
 
     def tris(self) -> np.ndarray:

@@ -90,6 +90,15 @@ void oracle_frame_destroy(OracleFrame* f);
 // (row-major pixels, samples innermost): prim id (0xffffffff = empty),
 // depth (FP32), assembled-triangle index (for shading) and, per pixel, the
 // coverage class of the winning fragment (1 spp only; MSAA writes SIMD256).
+// Point (mode 2) and line (mode 1) polygon modes at 1 sample per pixel
+// (TileRasterCoreInvocationCuda.cu:2780-3335): per pixel the primitive id (the
+// index position of a point, 3 * triangle + edge of a line) and depth of the
+// nearest candidate below the stored depth; tri_out names the shading record
+// (oracle_get_tris) of the winner.
+int oracle_render_polygons(OracleFrame* f, const OracleRasterConfig* cfg, int mode, const float* clip, int nverts,
+                           const int32_t* idx, int nidx, const float* depth_in, uint32_t* id_out, float* depth_out,
+                           uint32_t* tri_out, OracleStats* stats);
+
 int oracle_render(OracleFrame* f, const OracleRasterConfig* cfg, const float* clip_pos, int nverts,
                   const int32_t* indices, int nindices, const float* depth_in, uint32_t* id_out, float* depth_out,
                   uint32_t* tri_out, uint8_t* class_out, OracleStats* stats);
