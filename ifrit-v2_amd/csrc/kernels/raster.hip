@@ -438,11 +438,12 @@ __device__ __forceinline__ void clear_tile(const ViewGeom& G, float clearDepth, 
     }
 }
 
-// LDS layout of the tile's keys: sample-major, each sample's 256 pixels
-// padded by one key after every 32 (a 64-bit key covers two of the 64 LDS
-// banks, so without the padding pixels 32 apart share banks).
-constexpr int kVisStride = 264;
-__device__ __forceinline__ int vis_at(int s, int p) { return s * kVisStride + p + (p >> 5); }
+// LDS layout of the tile's keys: sample-major, 256 pixels per sample.  A
+// padded layout (one key after every 32) lowered the bank-conflict share of
+// LDS cycles from 40 % to 35 % and left the kernel time unchanged: conflicts
+// cost about 3 % of its cycles, and the kernel is bound by VALU issue.
+constexpr int kVisStride = 256;
+__device__ __forceinline__ int vis_at(int s, int p) { return s * kVisStride + p; }
 
 // NT threads per tile: 256 (one pixel per thread) or, at 1 sample per pixel,
 // 64 (one wave per tile, four pixels per lane: more tiles in flight per
@@ -460,8 +461,7 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
     constexpr int PAIRS = S == 1 ? 1 : 16 * PASS;     // (entry, block) pairs of a pass (S > 1)
     constexpr int PPT = 256 / NT;           // tile pixels per thread
     static_assert(PASS <= kPass, "units encode the entry in 7 bits");
-    // key of sample s of tile pixel p at vis_at(s, p): one 8-byte slot of padding after every 32
-    // pixels, so that pixels two rows apart (32 keys, 256 bytes) fall in other LDS banks
+    // key of sample s of tile pixel p at vis_at(s, p)
     __shared__ unsigned long long sVis[S * kVisStride];
     static_assert(sizeof(TileOut<S>) <= sizeof(sVis), "output staging reuses sVis");
     __shared__ TriRec sRec[PASS];

@@ -14,7 +14,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_ROOT = os.path.dirname(HERE)
-LIB_PATH = os.path.join(PKG_ROOT, "lib", "libifrit_hip.so")
+# IFRIT_HIP_LIB names another build of the library (an A/B build of the same sources)
+LIB_PATH = os.environ.get("IFRIT_HIP_LIB") or os.path.join(PKG_ROOT, "lib", "libifrit_hip.so")
 
 _lib = None
 
