@@ -26,6 +26,7 @@ SOURCES = [
     "kernels/shade.hip",
     "kernels/texture.hip",
     "kernels/polygon.hip",
+    "kernels/mesh.hip",
     "kernels/neural.hip",
     "host/renderer.cpp",
     "host/neural.cpp",

@@ -223,6 +223,18 @@ void* iftrCreateGpuFragmentProgram(const IfritGpuFragmentProgramDesc* d) {
     if (p) p->desc = *d;
     return p;
 }
+void* iftrCreateGpuMeshProgram(const IfritGpuMeshProgramDesc* d) {
+    if (!d) return nullptr;
+    GpuMeshProgram* p = new (std::nothrow) GpuMeshProgram();
+    if (p) p->desc = *d;
+    return p;
+}
+void* iftrCreateGpuTaskProgram(const IfritGpuTaskProgramDesc* d) {
+    if (!d) return nullptr;
+    GpuTaskProgram* p = new (std::nothrow) GpuTaskProgram();
+    if (p) p->desc = *d;
+    return p;
+}
 void iftrDestroyGpuProgram(void* p) {
     if (!p) return;
     const uint32_t tag = *static_cast<const uint32_t*>(p);
@@ -230,6 +242,39 @@ void iftrDestroyGpuProgram(void* p) {
         delete static_cast<GpuVertexProgram*>(p);
     else if (tag == kFragmentProgramTag)
         delete static_cast<GpuFragmentProgram*>(p);
+    else if (tag == kMeshProgramTag)
+        delete static_cast<GpuMeshProgram*>(p);
+    else if (tag == kTaskProgramTag)
+        delete static_cast<GpuTaskProgram*>(p);
+}
+void iftrBindMeshShader(IfritRendererHandle* h, void* p) {
+    const GpuMeshProgram* prog = static_cast<const GpuMeshProgram*>(p);
+    guarded(h, [&](TileRasterRenderer& r) {
+        if (prog && prog->tag != kMeshProgramTag)
+            r.setError("iftrBindMeshShader expects an object from iftrCreateGpuMeshProgram");
+        else
+            r.bindMeshShader(prog);
+    });
+}
+void iftrBindTaskShader(IfritRendererHandle* h, void* p) {
+    const GpuTaskProgram* prog = static_cast<const GpuTaskProgram*>(p);
+    guarded(h, [&](TileRasterRenderer& r) {
+        if (prog && prog->tag != kTaskProgramTag)
+            r.setError("iftrBindTaskShader expects an object from iftrCreateGpuTaskProgram");
+        else
+            r.bindTaskShader(prog);
+    });
+}
+int iftrCreateBuffer(IfritRendererHandle* h, int slot, int size) {
+    int rc = -1;
+    guarded(h, [&](TileRasterRenderer& r) { rc = r.createBuffer(slot, size); });
+    return rc;
+}
+void iftrCopyHostBufferToBuffer(IfritRendererHandle* h, const void* src, int slot, int size) {
+    guarded(h, [&](TileRasterRenderer& r) { r.copyHostBufferToBuffer(src, slot, size); });
+}
+void iftrDrawMeshTasks(IfritRendererHandle* h, int numWorkGroups, int firstWorkGroup, int clear) {
+    guarded(h, [&](TileRasterRenderer& r) { r.drawMeshTasks(numWorkGroups, firstWorkGroup, clear != 0); });
 }
 void iftrSetClearValues(IfritRendererHandle* h, const IfritVector4f* colors, size_t n, float depth) {
     guarded(h, [&](TileRasterRenderer& r) { r.setClearValues(colors, n, depth); });

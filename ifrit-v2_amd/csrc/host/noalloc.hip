@@ -260,6 +260,7 @@ void TileRasterRenderer::drawNoAlloc(const IfritNoAllocDraw& d) {
         td[i].width = t.width;
         td[i].height = t.height;
         td[i].levels = t.levels;
+        td[i].layers = t.layers;
     }
     if (!ok(hipMemcpy(viewDev_.ptr, &vd, sizeof(vd), hipMemcpyHostToDevice), "iftrDrawNoAlloc view table") ||
         !ok(hipMemcpy(texDesc_.ptr, td.data(), td.size() * sizeof(TexDesc), hipMemcpyHostToDevice),

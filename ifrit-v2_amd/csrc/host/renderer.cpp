@@ -59,8 +59,9 @@ TileRasterRenderer::~TileRasterRenderer() {
                       &nnPacked_, &nnQinfo_, &lat32_, &latGrad_, &latM_, &latV_, &lat16_, &nnData_, &nnFeat_, &nnCount_,
                       &nnLoss_, &nnEvalIn_, &nnEvalOut_, &nnGradPart_, &viewDev_, &tileView_, &shadeView_,
                       &groupBounds_, &groupOut_, &swapped_, &scissorDev_,
-                      &viewCtr_, &digest_, &planSums_})
+                      &viewCtr_, &digest_, &planSums_, &polyKeys_, &meshVtx_, &meshIdx_})
         b->release();
+    for (DevBuf& b : bufSlots_) b.release();
     for (auto& t : textures_)
         if (t.dev) (void)hipFree(t.dev);
     for (auto& e : ev_)
@@ -272,6 +273,125 @@ Scissor TileRasterRenderer::activeScissor() const {
 void TileRasterRenderer::SetCullMode(int mode) {
     if (mode < kCullNone || mode > kCullFrontAndBack) return setError("unknown cull mode");
     cullMode_ = mode;
+}
+
+int TileRasterRenderer::createBuffer(int slot, int size) {
+    Init();
+    if (!initialized_) return -1;
+    if (slot < 0 || slot >= kBufferSlots) return setError("buffer slot out of range (0 .. 63)"), -1;
+    if (size <= 0) return setError("buffer size must be positive"), -1;
+    if (!bufSlots_[slot].ensure((size_t)size)) return setError("hipMalloc buffer slot"), -1;
+    bufSize_[slot] = size;
+    return 0;
+}
+
+void TileRasterRenderer::copyHostBufferToBuffer(const void* src, int slot, int size) {
+    if (slot < 0 || slot >= kBufferSlots || !bufSlots_[slot].ptr) return setError("copy into a buffer slot not created");
+    if (!src || size < 0 || size > bufSize_[slot]) return setError("copy larger than the buffer slot");
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    ok(hipMemcpy(bufSlots_[slot].ptr, src, (size_t)size, hipMemcpyHostToDevice), "buffer upload");
+}
+
+void TileRasterRenderer::bindMeshShader(const GpuMeshProgram* p) {
+    msBound_ = p != nullptr;
+    if (p) ms_ = *p;
+}
+
+void TileRasterRenderer::bindTaskShader(const GpuTaskProgram* p) {
+    tsBound_ = p != nullptr;
+    if (p) ts_ = *p;
+}
+
+// The meshlet program emits its work groups' meshlets into meshVtx_ /
+// meshIdx_ (clip position then record per vertex, rebased indices), which are
+// then drawn in place of the bound vertex and index buffers by the clip
+// pass-through vertex program; the bound inputs are restored afterwards.
+void TileRasterRenderer::drawMeshTasks(int numGroups, int firstGroup, bool clear) {
+    Init();
+    if (!initialized_) return;
+    if (!msBound_) return setError("DrawMeshTasks: no mesh program bound");
+    const IfritGpuMeshProgramDesc& m = ms_.desc;
+    if (m.kind != IFHIP_MS_MESHLET) return setError("DrawMeshTasks: unknown mesh program");
+    if (tsBound_ && ts_.desc.kind != IFHIP_TS_ONE_MESH_GROUP) return setError("DrawMeshTasks: unknown task program");
+    if (numGroups < 0 || firstGroup < 0) return setError("DrawMeshTasks: negative work group count or start");
+    if (m.recordFloats < 3 || m.recordFloats > 256) return setError("DrawMeshTasks: recordFloats must be 3 .. 256");
+    const int slots[4] = {m.vertexSlot, m.indexSlot, m.vertexOffsetSlot, m.indexOffsetSlot};
+    for (int sl : slots)
+        if (sl < 0 || sl >= kBufferSlots || !bufSlots_[sl].ptr)
+            return setError("DrawMeshTasks: a buffer slot of the mesh program was not created");
+    const long long lastOff = (long long)firstGroup + numGroups;
+    if ((lastOff + 1) * 4 > bufSize_[m.vertexOffsetSlot] || (lastOff + 1) * 4 > bufSize_[m.indexOffsetSlot])
+        return setError("DrawMeshTasks: work groups beyond the offset buffers");
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    int vo[2] = {0, 0}, io[2] = {0, 0};
+    const int* vob = bufSlots_[m.vertexOffsetSlot].as<int>();
+    const int* iob = bufSlots_[m.indexOffsetSlot].as<int>();
+    if (!ok(hipMemcpy(&vo[0], vob + firstGroup, 4, hipMemcpyDeviceToHost), "mesh offsets") ||
+        !ok(hipMemcpy(&vo[1], vob + lastOff, 4, hipMemcpyDeviceToHost), "mesh offsets") ||
+        !ok(hipMemcpy(&io[0], iob + firstGroup, 4, hipMemcpyDeviceToHost), "mesh offsets") ||
+        !ok(hipMemcpy(&io[1], iob + lastOff, 4, hipMemcpyDeviceToHost), "mesh offsets"))
+        return;
+    const long long nv = (long long)vo[1] - vo[0], ni = (long long)io[1] - io[0];
+    if (vo[0] < 0 || io[0] < 0 || nv < 0 || ni < 0 || ni % 3)
+        return setError("DrawMeshTasks: offsets must grow, and the index count must be a multiple of 3");
+    if ((long long)vo[1] * m.recordFloats * 4 > bufSize_[m.vertexSlot] || (long long)io[1] * 4 > bufSize_[m.indexSlot])
+        return setError("DrawMeshTasks: meshlets beyond the vertex or index buffer");
+    // the offsets of every work group in between must grow too (checked on the device per meshlet is
+    // not needed: a meshlet's bounds come from consecutive offsets, so a shrinking pair emits nothing)
+    const int stride = 4 + m.recordFloats;
+    if (nv * stride >= (1ll << 31) || ni >= (1ll << 31)) return setError("DrawMeshTasks: meshlets too large");
+    if (!meshVtx_.ensure((size_t)std::max<long long>(nv, 1) * stride * 4) ||
+        !meshIdx_.ensure((size_t)std::max<long long>(ni, 1) * 4))
+        return setError("hipMalloc mesh outputs");
+    MeshParams mp{};
+    std::memcpy(mp.mvp, m.mvp, sizeof(mp.mvp));
+    mp.recordFloats = m.recordFloats;
+    mp.first = firstGroup;
+    mp.groups = numGroups;
+    mp.vertexBase = vo[0];
+    mp.indexBase = io[0];
+    if (!ctr_.ensure(sizeof(DeviceCounters))) return setError("hipMalloc counters");
+    if (!ok(hipMemsetAsync(ctr_.ptr, 0, sizeof(unsigned), stream_), "mesh status")) return;
+    launch_mesh_expand(mp, bufSlots_[m.vertexSlot].as<float>(), bufSlots_[m.indexSlot].as<int>(), vob, iob,
+                       meshVtx_.as<float>(), meshIdx_.as<int>(), ctr_.as<DeviceCounters>(), stream_);
+    if (!ok(hipGetLastError(), "mesh launch") || !ok(hipStreamSynchronize(stream_), "mesh work groups")) return;
+    unsigned status = 0;
+    if (!ok(hipMemcpy(&status, ctr_.ptr, sizeof(status), hipMemcpyDeviceToHost), "mesh status")) return;
+    if (status & kStatusBadIndex) return setError("DrawMeshTasks: a meshlet index lies outside its meshlet");
+    // draw the emitted list with the bound pipeline state, then restore the bound inputs
+    std::swap(vtx_, meshVtx_);
+    std::swap(idx_, meshIdx_);
+    const GpuVertexProgram vsSaved = vs_;
+    const bool vsBoundSaved = vsBound_, vtxDirtySaved = vtxDirty_, idxDirtySaved = idxDirty_;
+    const bool boundsDirtySaved = boundsDirty_;
+    const int nvertsSaved = nverts_, strideSaved = stride_, nidxSaved = nidx_, boundsTrisSaved = boundsTris_;
+    auto hostVsSaved = hostVs_;
+    auto hostVbSaved = hostVb_;
+    auto hostIbSaved = hostIb_;
+    nverts_ = (int)nv;
+    stride_ = stride;
+    nidx_ = (int)ni;
+    vs_.desc.kind = IFHIP_VS_CLIP_PASSTHROUGH;
+    vsBound_ = true;
+    hostVs_ = nullptr;
+    hostVb_ = nullptr;
+    hostIb_ = nullptr;
+    vtxDirty_ = idxDirty_ = false;
+    drawElements((int)ni, clear);
+    std::swap(vtx_, meshVtx_);
+    std::swap(idx_, meshIdx_);
+    vs_ = vsSaved;
+    vsBound_ = vsBoundSaved;
+    vtxDirty_ = vtxDirtySaved;
+    idxDirty_ = idxDirtySaved;
+    boundsDirty_ = boundsDirtySaved;
+    nverts_ = nvertsSaved;
+    stride_ = strideSaved;
+    nidx_ = nidxSaved;
+    boundsTris_ = boundsTrisSaved;
+    hostVs_ = hostVsSaved;
+    hostVb_ = hostVbSaved;
+    hostIb_ = hostIbSaved;
 }
 
 void TileRasterRenderer::setRasterizerPolygonMode(int mode) {

@@ -57,6 +57,8 @@ struct BufferManager {
 
 constexpr uint32_t kVertexProgramTag = 0x49465653u;    // "IFVS"
 constexpr uint32_t kFragmentProgramTag = 0x49464653u;  // "IFFS"
+constexpr uint32_t kMeshProgramTag = 0x49464d53u;      // "IFMS"
+constexpr uint32_t kTaskProgramTag = 0x49465453u;      // "IFTS"
 
 struct GpuVertexProgram {
     uint32_t tag = kVertexProgramTag;
@@ -65,6 +67,14 @@ struct GpuVertexProgram {
 struct GpuFragmentProgram {
     uint32_t tag = kFragmentProgramTag;
     IfritGpuFragmentProgramDesc desc;
+};
+struct GpuMeshProgram {
+    uint32_t tag = kMeshProgramTag;
+    IfritGpuMeshProgramDesc desc;
+};
+struct GpuTaskProgram {
+    uint32_t tag = kTaskProgramTag;
+    IfritGpuTaskProgramDesc desc;
 };
 
 struct TextureSlot {
@@ -122,6 +132,12 @@ public:
     void optSetDepthTestEnable(bool opt);
     void SetCullMode(int mode);  // TileRasterRendererCuda.h:109
     void setRasterizerPolygonMode(int mode);  // TileRasterRendererCuda.h:106: 0 fill, 1 line, 2 point
+    // device buffer slots and mesh work groups (TileRasterRendererCuda.h:88-89, 98-99, 115)
+    int createBuffer(int slot, int size);
+    void copyHostBufferToBuffer(const void* src, int slot, int size);
+    void bindMeshShader(const GpuMeshProgram* p);
+    void bindTaskShader(const GpuTaskProgram* p);
+    void drawMeshTasks(int numGroups, int firstGroup, bool clear);
     void setClearValues(const IfritVector4f* colors, size_t n, float depth);  // TileRasterRendererCuda.h:111
     void SetScissors(const IfritVector4f* rects, size_t n);                    // TileRasterRendererCuda.h:101
     void setScissorTestEnable(bool on) { scissorTest_ = on; }                  // TileRasterRendererCuda.h:102
@@ -258,6 +274,13 @@ private:
     DevBuf groupBounds_, groupOut_;  // per triangle group: bounds; per view and group: outside flag
     DevBuf swapped_;                 // per record slot: set up with vertices 0 and 2 exchanged
     DevBuf polyKeys_;                // point and line modes: per pixel the nearest candidate's key
+    static constexpr int kBufferSlots = 64;
+    DevBuf bufSlots_[kBufferSlots];  // CreateBuffer slots
+    int bufSize_[kBufferSlots] = {};
+    DevBuf meshVtx_, meshIdx_;       // the emitted meshlets of DrawMeshTasks
+    GpuMeshProgram ms_{};
+    GpuTaskProgram ts_{};
+    bool msBound_ = false, tsBound_ = false;
     DevBuf clipList_, statPart_, items_, tilePartial_, multiList_, partials_, visId_, visDepth_, visSlot_, visClass_, rgba8_, rgba32f_, texDesc_;
     unsigned clipCap_ = 0, binCap_ = 0, bigCap_ = 0, itemCap_ = 0, partialCap_ = 0;
     int numCUs_ = 256;

@@ -323,6 +323,19 @@ void launch_polygons(const PolyParams& p, const float4* clip, const int* idx, Tr
                      float* visDepth, uint32_t* visSlot, uint8_t* visClass, StatPartials st, DeviceCounters* ctr,
                      int grid, hipStream_t s);
 
+// The built-in meshlet program of DrawMeshTasks (mesh.hip): work groups first .. first + groups - 1,
+// meshlet m owning vertices vo[m] .. vo[m + 1] - 1 and indices io[m] .. io[m + 1] - 1 (local to the
+// meshlet); emitted vertices are (clip xyzw, record) at vo[m] - vertexBase, indices at
+// io[m] - indexBase.
+struct MeshParams {
+    float mvp[16];
+    int recordFloats;
+    int first, groups;
+    int vertexBase, indexBase;
+};
+void launch_mesh_expand(const MeshParams& p, const float* records, const int* indices, const int* vo, const int* io,
+                        float* outVtx, int* outIdx, DeviceCounters* ctr, hipStream_t s);
+
 // Texture helpers (texture.hip).
 // Order-independent 64-bit digest of n words (equal buffers give equal digests).
 void launch_digest(const uint32_t* data, size_t n, unsigned long long* out, hipStream_t s);

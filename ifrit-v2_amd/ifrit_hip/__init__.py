@@ -34,6 +34,11 @@ class FragmentProgramDesc(C.Structure):
                 ("depthBias", C.c_float)]
 
 
+class MeshProgramDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("mvp", C.c_float * 16), ("vertexSlot", C.c_int32), ("indexSlot", C.c_int32),
+                ("vertexOffsetSlot", C.c_int32), ("indexOffsetSlot", C.c_int32), ("recordFloats", C.c_int32)]
+
+
 class SamplerDesc(C.Structure):
     _fields_ = [("filterMode", C.c_int32), ("addressModeU", C.c_int32), ("addressModeV", C.c_int32),
                 ("addressModeW", C.c_int32), ("borderColor", C.c_int32), ("anisotropyEnable", C.c_int32),
@@ -124,7 +129,9 @@ EXPORTED_SYMBOLS = [
     "iftrNeuralTrain", "iftrNeuralSetParams", "iftrNeuralGetParams", "iftrNeuralEval", "iftrNeuralValidate",
     "iftrSetViews", "iftrGetViewVisibility", "iftrGetViewColorRGBA8", "iftrSetGraphMode", "iftrSetTuning",
     "iftrGetTuning", "iftrAutotune", "iftrGetDigest", "iftrSetColorFloatOutput", "iftrGetColorRGBA32F",
-    "iftrDrawNoAlloc", "iftrNeuralValidateEx", "iftrSetCullMode", "iftrSetRasterizerPolygonMode", "iftrSetClearValues", "iftrSetScissors",
+    "iftrDrawNoAlloc", "iftrNeuralValidateEx", "iftrSetCullMode", "iftrSetRasterizerPolygonMode",
+    "iftrCreateBuffer", "iftrCopyHostBufferToBuffer", "iftrCreateGpuMeshProgram", "iftrCreateGpuTaskProgram",
+    "iftrBindMeshShader", "iftrBindTaskShader", "iftrDrawMeshTasks", "iftrSetClearValues", "iftrSetScissors",
     "iftrSetScissorTestEnable", "iftrCreateTextureLayers", "iftrCopyHostBufferToImageLayer", "iftrCreateSampler",
     "iftrBlitImage",
 ]
@@ -197,6 +204,11 @@ def lib():
         "iftrSetColorFloatOutput": ([vp, C.c_int], None), "iftrGetColorRGBA32F": ([vp, vp], None),
         "iftrDrawNoAlloc": ([vp, C.POINTER(NoAllocDraw)], None),
         "iftrSetCullMode": ([vp, C.c_int], None), "iftrSetRasterizerPolygonMode": ([vp, C.c_int], None),
+        "iftrCreateBuffer": ([vp, C.c_int, C.c_int], C.c_int),
+        "iftrCopyHostBufferToBuffer": ([vp, vp, C.c_int, C.c_int], None),
+        "iftrCreateGpuMeshProgram": ([vp], vp), "iftrCreateGpuTaskProgram": ([vp], vp),
+        "iftrBindMeshShader": ([vp, vp], None), "iftrBindTaskShader": ([vp, vp], None),
+        "iftrDrawMeshTasks": ([vp, C.c_int, C.c_int, C.c_int], None),
         "iftrSetClearValues": ([vp, vp, C.c_size_t, C.c_float], None),
         "iftrSetScissors": ([vp, vp, C.c_size_t], None), "iftrSetScissorTestEnable": ([vp, C.c_int], None),
         "iftrCreateTextureLayers": ([vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
@@ -263,6 +275,35 @@ class Renderer:
         pixel (equal-depth ties still resolve to the lowest primitive).'''
         self.L.iftrOptsetForceDeterministic(self.h, 1 if on else 0)
         self._check()
+
+    def create_buffer(self, slot: int, data: np.ndarray):
+        '''CreateBuffer + copyHostBufferToBuffer of one array into a slot.'''
+        a = np.ascontiguousarray(data)
+        self.L.iftrCreateBuffer(self.h, slot, max(a.nbytes, 4))
+        self._check()
+        self.L.iftrCopyHostBufferToBuffer(self.h, a.ctypes.data, slot, a.nbytes)
+        self._check()
+
+    def bind_meshlets(self, mvp, record_floats: int, slots=(0, 1, 2, 3), task: bool = True):
+        '''Binds the built-in meshlet mesh program (and the one-mesh-group task program).'''
+        d = MeshProgramDesc(1, (C.c_float * 16)(*np.asarray(mvp, np.float32).reshape(-1)), *slots, record_floats)
+        p = self.L.iftrCreateGpuMeshProgram(C.byref(d))
+        self._programs.append(p)
+        self.L.iftrBindMeshShader(self.h, p)
+        if task:
+            t = (C.c_int32 * 1)(1)
+            tp = self.L.iftrCreateGpuTaskProgram(t)
+            self._programs.append(tp)
+            self.L.iftrBindTaskShader(self.h, tp)
+        self._check()
+
+    def draw_mesh_tasks(self, groups: int, first: int = 0, clear: bool = True):
+        self.L.iftrDrawMeshTasks(self.h, groups, first, 1 if clear else 0)
+        self._check()
+        st = self.stats()
+        if st["status"]:
+            raise IfritError(f"frame incomplete, status bits {st['status']:#x}")
+        return st
 
     def set_polygon_mode(self, mode: int):
         '''IF_POLYGON_MODE_*: 0 fill (the default), 1 line, 2 point.'''
@@ -341,7 +382,9 @@ class Renderer:
         self.L.iftrBlitImage(self.h, src, dst, C.byref(b), 1 if linear else 0)
         self._check()
 
-    def bind_fragment_program(self, scene, program=None, precision=0):
+    def bind_fragment_program(self, scene, program=None, precision=0, offset=0):
+        '''offset: floats before the scene's vertex record in the drawn vertices (4 for the records
+        DrawMeshTasks emits: clip position, then the record).'''
         d = FragmentProgramDesc()
         d.kind = scene.program if program is None else program
         d.precision = precision
@@ -350,8 +393,8 @@ class Renderer:
         dw = getattr(scene, "depth_write", None)  # (scale, bias): the program writes depth * scale + bias
         if dw is not None:
             d.depthWrite, d.depthScale, d.depthBias = 1, float(dw[0]), float(dw[1])
-        d.offWorld, d.offNormal, d.offUv = 0, 3, 6
-        d.offTangent = 8 if scene.stride >= 11 else 0
+        d.offWorld, d.offNormal, d.offUv = offset, offset + 3, offset + 6
+        d.offTangent = offset + 8 if scene.stride >= 11 else 0
         d.trisPerMaterial = scene.tris_per_material
         d.sampler, d.maxAniso = scene.sampler, scene.max_aniso
         d.camera[:] = [float(x) for x in scene.camera]

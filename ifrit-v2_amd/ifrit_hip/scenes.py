@@ -525,3 +525,40 @@ def sphere_scene(width: int = 256, height: int = 192, samples: int = 1, seg: int
     eye = (0.0, 0.3, -3.0)
     return Scene("sphere", width, height, samples, sv, si, _mvp(eye, (0, 0, 0), 50.0, width, height),
                  program=PROGRAM_CUBE_MAP, camera=eye)
+
+
+def build_meshlets(vertices: np.ndarray, indices: np.ndarray, max_vertices: int = 64, max_triangles: int = 126):
+    """Greedy meshlets of a triangle list in its order (a new meshlet starts when the next triangle
+    would exceed either bound), as the buffers of the reference's meshlet demo: the meshlets'
+    vertex records one after another (a vertex used by several meshlets is repeated), their
+    meshlet-local indices, and the int32 vertex and index offsets (one more than meshlets).
+    Returns (records, local_indices, vertex_offsets, index_offsets, global_indices) where
+    global_indices indexes records and lists the same triangles in the same order."""
+    tris = np.asarray(indices, np.int64).reshape(-1, 3)
+    recs, loc, glob, vo, io = [], [], [], [0], [0]
+    cur = {}
+    ntri = 0
+
+    def close():
+        nonlocal cur, ntri
+        vo.append(vo[-1] + len(cur))
+        io.append(len(loc))
+        cur = {}
+        ntri = 0
+
+    for t in tris:
+        new = sum(1 for v in t if int(v) not in cur)
+        if cur and (len(cur) + new > max_vertices or ntri + 1 > max_triangles):
+            close()
+        for v in t:
+            v = int(v)
+            if v not in cur:
+                cur[v] = len(cur)
+                recs.append(vertices[v])
+            loc.append(cur[v])
+            glob.append(vo[-1] + cur[v])
+        ntri += 1
+    if cur:
+        close()
+    return (np.asarray(recs, np.float32), np.asarray(loc, np.int32), np.asarray(vo, np.int32),
+            np.asarray(io, np.int32), np.asarray(glob, np.int32))

@@ -216,6 +216,48 @@ enum {
    draws nothing.  Exchanged triangles shade with their own vertex attributes. */
 void iftrSetCullMode(IfritRendererHandle* h, int mode);
 
+/* Device buffers of the CUDA renderer (CreateBuffer / copyHostBufferToBuffer,
+   TileRasterRendererCuda.h:98-99): numbered slots that mesh programs read.  CreateBuffer
+   (re)allocates a slot of `size` bytes (contents undefined); copyHostBufferToBuffer writes
+   `size` bytes at its start.  Returns 0, or -1 with an error. */
+int iftrCreateBuffer(IfritRendererHandle* h, int slot, int size);
+void iftrCopyHostBufferToBuffer(IfritRendererHandle* h, const void* src, int slot, int size);
+
+/* Built-in mesh and task programs for DrawMeshTasks (bindMeshShader / bindTaskShader /
+   DrawMeshTasks, TileRasterRendererCuda.h:88-89, 115; the pipeline of
+   TileRasterCoreInvocationCuda.cu:355-500).  The reference runs user device objects; here a mesh
+   work group runs the built-in meshlet program of the reference's meshlet demo
+   (demo/shader/MeshletDemoShaders.cu): meshlet m owns vertices vo[m] .. vo[m + 1] - 1 of the vertex
+   slot (records of recordFloats floats, the position xyz first) and indices io[m] .. io[m + 1] - 1 of
+   the index slot, local to the meshlet's vertices, where vo and io are the int32 offset slots.  It
+   emits each vertex as mvp * (x, y, z, 1) (the transform program's summation order) followed by its
+   record, so the fragment program's attribute offsets count from 4 (an offset k of the record is
+   4 + k), and its indices rebased onto the emitted vertices.  The task program of that demo launches one
+   mesh work group per task work group with the task's id as payload, i.e. meshlet m = work group m.
+   The emitted meshlets are concatenated in work-group order (the reference's compaction takes
+   them in atomic order) and drawn as a triangle list by the bound pipeline state: the primitive
+   id of a triangle is its position in that list. */
+enum { IFHIP_MS_MESHLET = 1 };
+enum { IFHIP_TS_ONE_MESH_GROUP = 1 };
+typedef struct {
+    int32_t kind;                /* IFHIP_MS_MESHLET */
+    float mvp[16];               /* row-major */
+    int32_t vertexSlot, indexSlot, vertexOffsetSlot, indexOffsetSlot;  /* the demo: 0, 1, 2, 3 */
+    int32_t recordFloats;        /* floats per vertex record (>= 3) */
+} IfritGpuMeshProgramDesc;
+typedef struct {
+    int32_t kind;                /* IFHIP_TS_ONE_MESH_GROUP */
+} IfritGpuTaskProgramDesc;
+void* iftrCreateGpuMeshProgram(const IfritGpuMeshProgramDesc* desc);
+void* iftrCreateGpuTaskProgram(const IfritGpuTaskProgramDesc* desc);
+void iftrBindMeshShader(IfritRendererHandle* h, void* program);   /* NULL unbinds */
+void iftrBindTaskShader(IfritRendererHandle* h, void* program);   /* NULL unbinds */
+/* DrawMeshTasks: numWorkGroups task (or, without a task program, mesh) work groups from
+   firstWorkGroup (the reference accepts only 0; any start is taken here).  clear: 1 starts from
+   the clear values, 0 draws over the current frame (the CUDA renderer's DrawMeshTasks does not
+   clear; it is preceded by clear()). */
+void iftrDrawMeshTasks(IfritRendererHandle* h, int numWorkGroups, int firstWorkGroup, int clear);
+
 /* IfritPolygonMode, include/ifrit/softgraphics/engine/base/Constants.h:23-28 */
 enum { IF_POLYGON_MODE_FILL = 0, IF_POLYGON_MODE_LINE = 1, IF_POLYGON_MODE_POINT = 2 };
 /* setRasterizerPolygonMode of the CUDA renderer (TileRasterRendererCuda.h:106), after its point and
