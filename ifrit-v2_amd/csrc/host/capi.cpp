@@ -235,6 +235,12 @@ void* iftrCreateGpuTaskProgram(const IfritGpuTaskProgramDesc* d) {
     if (p) p->desc = *d;
     return p;
 }
+void* iftrCreateGpuGeometryProgram(const IfritGpuGeometryProgramDesc* d) {
+    if (!d) return nullptr;
+    GpuGeometryProgram* p = new (std::nothrow) GpuGeometryProgram();
+    if (p) p->desc = *d;
+    return p;
+}
 void iftrDestroyGpuProgram(void* p) {
     if (!p) return;
     const uint32_t tag = *static_cast<const uint32_t*>(p);
@@ -246,6 +252,17 @@ void iftrDestroyGpuProgram(void* p) {
         delete static_cast<GpuMeshProgram*>(p);
     else if (tag == kTaskProgramTag)
         delete static_cast<GpuTaskProgram*>(p);
+    else if (tag == kGeometryProgramTag)
+        delete static_cast<GpuGeometryProgram*>(p);
+}
+void iftrBindGeometryShader(IfritRendererHandle* h, void* p) {
+    const GpuGeometryProgram* prog = static_cast<const GpuGeometryProgram*>(p);
+    guarded(h, [&](TileRasterRenderer& r) {
+        if (prog && prog->tag != kGeometryProgramTag)
+            r.setError("iftrBindGeometryShader expects an object from iftrCreateGpuGeometryProgram");
+        else
+            r.bindGeometryShader(prog);
+    });
 }
 void iftrBindMeshShader(IfritRendererHandle* h, void* p) {
     const GpuMeshProgram* prog = static_cast<const GpuMeshProgram*>(p);

@@ -90,6 +90,7 @@ void TileRasterRenderer::drawNoAlloc(const IfritNoAllocDraw& d) {
     if (hostVs_ || !vsBound_) return setError("iftrDrawNoAlloc needs a GPU vertex program");
     if (blend_.blendEnable) return setError("iftrDrawNoAlloc does not blend");
     if (polygonMode_ != 0) return setError("iftrDrawNoAlloc draws filled triangles only");
+    if (gsBound_) return setError("iftrDrawNoAlloc does not run geometry programs");
     if (depthFunc_ < IF_COMPARE_OP_NEVER || depthFunc_ > IF_COMPARE_OP_ALWAYS)
         return setError("unknown depth function");
     if (d.numIndices < 0 || d.numVertices <= 0 || d.strideFloats <= 0)

@@ -358,7 +358,10 @@ void TileRasterRenderer::drawMeshTasks(int numGroups, int firstGroup, bool clear
     unsigned status = 0;
     if (!ok(hipMemcpy(&status, ctr_.ptr, sizeof(status), hipMemcpyDeviceToHost), "mesh status")) return;
     if (status & kStatusBadIndex) return setError("DrawMeshTasks: a meshlet index lies outside its meshlet");
-    // draw the emitted list with the bound pipeline state, then restore the bound inputs
+    drawEmitted((int)nv, stride, m.recordFloats, (int)ni, clear, false);
+}
+
+void TileRasterRenderer::drawEmitted(int nv, int stride, int clipAt, int ni, bool clear, bool async) {
     std::swap(vtx_, meshVtx_);
     std::swap(idx_, meshIdx_);
     const GpuVertexProgram vsSaved = vs_;
@@ -368,18 +371,23 @@ void TileRasterRenderer::drawMeshTasks(int numGroups, int firstGroup, bool clear
     auto hostVsSaved = hostVs_;
     auto hostVbSaved = hostVb_;
     auto hostIbSaved = hostIb_;
-    nverts_ = (int)nv;
+    nverts_ = nv;
     stride_ = stride;
-    nidx_ = (int)ni;
+    nidx_ = ni;
     vs_.desc.kind = IFHIP_VS_CLIP_PASSTHROUGH;
     vsBound_ = true;
+    clipAt_ = clipAt;
     hostVs_ = nullptr;
     hostVb_ = nullptr;
     hostIb_ = nullptr;
     vtxDirty_ = idxDirty_ = false;
-    drawElements((int)ni, clear);
+    if (async)
+        drawAsync(ni, clear);
+    else
+        drawElements(ni, clear);
     std::swap(vtx_, meshVtx_);
     std::swap(idx_, meshIdx_);
+    clipAt_ = -1;
     vs_ = vsSaved;
     vsBound_ = vsBoundSaved;
     vtxDirty_ = vtxDirtySaved;
@@ -392,6 +400,52 @@ void TileRasterRenderer::drawMeshTasks(int numGroups, int firstGroup, bool clear
     hostVs_ = hostVsSaved;
     hostVb_ = hostVbSaved;
     hostIb_ = hostIbSaved;
+}
+
+void TileRasterRenderer::bindGeometryShader(const GpuGeometryProgram* p) {
+    gsBound_ = p != nullptr;
+    if (p) gs_ = *p;
+}
+
+// The geometry stage (TileRasterCoreInvocationCuda.cu:500-560) with the
+// built-in clip-offset program: every triangle of the draw is emitted once,
+// as three unshared vertices in triangle order (the order the reference's
+// deterministic sort keys give, :539-542), then drawn by the ordinary
+// pipeline; the primitive id is the triangle's index.
+void TileRasterRenderer::drawGeometry(int indexCount, bool clear, bool async) {
+    Init();
+    if (!initialized_) return;
+    if (gs_.desc.kind != IFHIP_GS_CLIP_OFFSET) return setError("unknown geometry program");
+    if (hostVs_) return setError("geometry programs follow a GPU vertex program, not a host vertex function");
+    if (!batch_.empty()) return setError("geometry programs draw a single view");
+    if (!vsBound_) return setError("no vertex program bound");
+    if (!uploadInputs()) return;
+    if (nverts_ <= 0 || !vtx_.ptr) return setError("no vertex buffer bound");
+    if (indexCount < 0 || indexCount > nidx_) return setError("index count exceeds the bound index buffer");
+    const bool pass = vs_.desc.kind == IFHIP_VS_CLIP_PASSTHROUGH;
+    if (stride_ < (pass ? 4 : 3)) return setError("vertex records too short for the vertex program");
+    const long long ntri = indexCount / 3, stride = stride_ + 4;
+    if (3 * ntri * stride >= (1ll << 31)) return setError("geometry program output too large");
+    if (stream_) (void)hipStreamSynchronize(stream_);  // an asynchronous draw may still read the last list
+    if (!meshVtx_.ensure((size_t)std::max<long long>(3 * ntri, 1) * stride * 4) ||
+        !meshIdx_.ensure((size_t)std::max<long long>(3 * ntri, 1) * 4))
+        return setError("hipMalloc geometry outputs");
+    GeometryParams gp{};
+    std::memcpy(gp.mvp, vs_.desc.mvp, sizeof(gp.mvp));
+    std::memcpy(gp.offset, gs_.desc.offset, sizeof(gp.offset));
+    gp.passthrough = pass ? 1 : 0;
+    gp.stride = stride_;
+    gp.nverts = nverts_;
+    gp.ntri = (int)ntri;
+    if (!ctr_.ensure(sizeof(DeviceCounters))) return setError("hipMalloc counters");
+    if (!ok(hipMemsetAsync(ctr_.ptr, 0, sizeof(unsigned), stream_), "geometry status")) return;
+    launch_geometry_expand(gp, vtx_.as<float>(), idx_.as<int>(), meshVtx_.as<float>(), meshIdx_.as<int>(),
+                           ctr_.as<DeviceCounters>(), stream_);
+    if (!ok(hipGetLastError(), "geometry launch") || !ok(hipStreamSynchronize(stream_), "geometry program")) return;
+    unsigned status = 0;
+    if (!ok(hipMemcpy(&status, ctr_.ptr, sizeof(status), hipMemcpyDeviceToHost), "geometry status")) return;
+    if (status & kStatusBadIndex) return setError("an index lies outside the bound vertex buffer");
+    drawEmitted((int)(3 * ntri), (int)stride, stride_, (int)(3 * ntri), clear, async);
 }
 
 void TileRasterRenderer::setRasterizerPolygonMode(int mode) {
@@ -565,6 +619,39 @@ void TileRasterRenderer::runHostVertexShader() {
     stride_ = 4 * std::max(1, nvary);
 }
 
+// Uploads host vertex and index buffers that changed (running a host vertex
+// function over the vertices first, if one is bound).
+bool TileRasterRenderer::uploadInputs() {
+    if (hostVs_ && hostVb_) {
+        runHostVertexShader();
+        vtxDirty_ = false;
+        boundsDirty_ = true;
+    } else if (vtxDirty_ && hostVb_) {
+        if (hostVb_->elementSize % 4) {
+            setError("vertex records must be a multiple of 4 bytes");
+            return false;
+        }
+        if (!vtx_.ensure(hostVb_->buffer.size())) return setError("hipMalloc vertex buffer"), false;
+        if (!ok(hipMemcpy(vtx_.ptr, hostVb_->buffer.data(), hostVb_->buffer.size(), hipMemcpyHostToDevice),
+                "vertex upload"))
+            return false;
+        nverts_ = hostVb_->vertexCount;
+        stride_ = hostVb_->elementSize / 4;
+        vtxDirty_ = false;
+        boundsDirty_ = true;
+    }
+    if (idxDirty_ && hostIb_) {
+        if (!idx_.ensure(hostIb_->data.size())) return setError("hipMalloc index buffer"), false;
+        if (!ok(hipMemcpy(idx_.ptr, hostIb_->data.data(), hostIb_->data.size(), hipMemcpyHostToDevice),
+                "index upload"))
+            return false;
+        nidx_ = (int)(hostIb_->data.size() / 4);
+        idxDirty_ = false;
+        boundsDirty_ = true;
+    }
+    return true;
+}
+
 bool TileRasterRenderer::prepare(int indexCount) {
     Init();
     if (!initialized_) return false;
@@ -597,34 +684,7 @@ bool TileRasterRenderer::prepare(int indexCount) {
                  "layered-GGX program");
         return false;
     }
-    // inputs
-    if (hostVs_ && hostVb_) {
-        runHostVertexShader();
-        vtxDirty_ = false;
-        boundsDirty_ = true;
-    } else if (vtxDirty_ && hostVb_) {
-        if (hostVb_->elementSize % 4) {
-            setError("vertex records must be a multiple of 4 bytes");
-            return false;
-        }
-        if (!vtx_.ensure(hostVb_->buffer.size())) return setError("hipMalloc vertex buffer"), false;
-        if (!ok(hipMemcpy(vtx_.ptr, hostVb_->buffer.data(), hostVb_->buffer.size(), hipMemcpyHostToDevice),
-                "vertex upload"))
-            return false;
-        nverts_ = hostVb_->vertexCount;
-        stride_ = hostVb_->elementSize / 4;
-        vtxDirty_ = false;
-        boundsDirty_ = true;
-    }
-    if (idxDirty_ && hostIb_) {
-        if (!idx_.ensure(hostIb_->data.size())) return setError("hipMalloc index buffer"), false;
-        if (!ok(hipMemcpy(idx_.ptr, hostIb_->data.data(), hostIb_->data.size(), hipMemcpyHostToDevice),
-                "index upload"))
-            return false;
-        nidx_ = (int)(hostIb_->data.size() / 4);
-        idxDirty_ = false;
-        boundsDirty_ = true;
-    }
+    if (!uploadInputs()) return false;
     if (nverts_ <= 0 || !vtx_.ptr) {
         setError("no vertex buffer bound");
         return false;
@@ -942,7 +1002,8 @@ void TileRasterRenderer::enqueue(bool clear) {
     if (hostVs_)
         launch_reset(binCount_.as<unsigned>(), binCursor_.as<unsigned>(), tiles, V, ctr_.as<DeviceCounters>(), s);
     else  // the vertex launch also clears the counters
-        launch_vertex(vtx_.as<float>(), stride_, nverts_, V, vs_.desc.kind == IFHIP_VS_CLIP_PASSTHROUGH ? 1 : 0,
+        launch_vertex(vtx_.as<float>(), stride_, nverts_, V,
+                      clipAt_ >= 0 ? clipAt_ : vs_.desc.kind == IFHIP_VS_CLIP_PASSTHROUGH ? 0 : -1,
                       clip_.as<float4>(), binCount_.as<unsigned>(), binCursor_.as<unsigned>(), tiles,
                       ctr_.as<DeviceCounters>(), groupBounds_.as<float4>(), groups,
                       cullGroups ? groupOut_.as<unsigned char>() : nullptr, s);
@@ -1082,8 +1143,11 @@ std::vector<uint8_t> TileRasterRenderer::graphKey(bool clear) const {
                         nnFallback_[1] ? 1 : 0, (int)textures_.size(), stageTiming_ ? 1 : 0,
                         fb_ && !fb_->color.empty() && fb_->color[0] ? 1 : 0, batch_.empty() ? 1 : 0,
                         floatOut_ ? 1 : 0, blend_.blendEnable ? 1 : 0, cullMode_, deterministic_ ? 1 : 0,
-                        polygonMode_, curIdx_};
+                        polygonMode_, curIdx_, clipAt_};
     put(ints, sizeof(ints));
+    // emitted lists swap buffers without reallocating them
+    const void* inputs[] = {vtx_.ptr, idx_.ptr};
+    put(inputs, sizeof(inputs));
     const unsigned caps[] = {clipCap_, binCap_, bigCap_, itemCap_, partialCap_};
     put(caps, sizeof(caps));
     put(&totalPixels_, sizeof(totalPixels_));
@@ -1175,6 +1239,7 @@ bool TileRasterRenderer::finish(bool clear, int indexCount) {
 }
 
 void TileRasterRenderer::drawElements(int indexCount, bool clear) {
+    if (gsBound_ && clipAt_ < 0) return drawGeometry(indexCount, clear, false);
     stats_.retries = 0;
     constexpr int kAttempts = 4;
     for (int attempt = 0; attempt < kAttempts; attempt++) {
@@ -1195,6 +1260,7 @@ void TileRasterRenderer::drawElements(int indexCount, bool clear) {
 }
 
 void TileRasterRenderer::drawAsync(int indexCount, bool clear) {
+    if (gsBound_ && clipAt_ < 0) return drawGeometry(indexCount, clear, true);
     if (!prepare(indexCount)) return;
     submit(clear);
     ok(hipGetLastError(), "kernel launch");

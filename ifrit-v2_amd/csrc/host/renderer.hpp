@@ -59,6 +59,7 @@ constexpr uint32_t kVertexProgramTag = 0x49465653u;    // "IFVS"
 constexpr uint32_t kFragmentProgramTag = 0x49464653u;  // "IFFS"
 constexpr uint32_t kMeshProgramTag = 0x49464d53u;      // "IFMS"
 constexpr uint32_t kTaskProgramTag = 0x49465453u;      // "IFTS"
+constexpr uint32_t kGeometryProgramTag = 0x49464753u;  // "IFGS"
 
 struct GpuVertexProgram {
     uint32_t tag = kVertexProgramTag;
@@ -75,6 +76,10 @@ struct GpuMeshProgram {
 struct GpuTaskProgram {
     uint32_t tag = kTaskProgramTag;
     IfritGpuTaskProgramDesc desc;
+};
+struct GpuGeometryProgram {
+    uint32_t tag = kGeometryProgramTag;
+    IfritGpuGeometryProgramDesc desc;
 };
 
 struct TextureSlot {
@@ -138,6 +143,7 @@ public:
     void bindMeshShader(const GpuMeshProgram* p);
     void bindTaskShader(const GpuTaskProgram* p);
     void drawMeshTasks(int numGroups, int firstGroup, bool clear);
+    void bindGeometryShader(const GpuGeometryProgram* p);  // TileRasterRendererCuda.h:87
     void setClearValues(const IfritVector4f* colors, size_t n, float depth);  // TileRasterRendererCuda.h:111
     void SetScissors(const IfritVector4f* rects, size_t n);                    // TileRasterRendererCuda.h:101
     void setScissorTestEnable(bool on) { scissorTest_ = on; }                  // TileRasterRendererCuda.h:102
@@ -277,10 +283,17 @@ private:
     static constexpr int kBufferSlots = 64;
     DevBuf bufSlots_[kBufferSlots];  // CreateBuffer slots
     int bufSize_[kBufferSlots] = {};
-    DevBuf meshVtx_, meshIdx_;       // the emitted meshlets of DrawMeshTasks
+    DevBuf meshVtx_, meshIdx_;       // the emitted vertices of DrawMeshTasks and geometry programs
     GpuMeshProgram ms_{};
     GpuTaskProgram ts_{};
-    bool msBound_ = false, tsBound_ = false;
+    GpuGeometryProgram gs_{};
+    bool msBound_ = false, tsBound_ = false, gsBound_ = false;
+    int clipAt_ = -1;  // >= 0 while an emitted list is drawn: the clip position's float offset in a record
+    // Draws meshVtx_ / meshIdx_ (nv records of `stride` floats, clip position at clipAt, ni indices)
+    // with the bound pipeline state, then restores the bound vertex and index inputs.
+    void drawEmitted(int nv, int stride, int clipAt, int ni, bool clear, bool async);
+    void drawGeometry(int indexCount, bool clear, bool async);
+    bool uploadInputs();
     DevBuf clipList_, statPart_, items_, tilePartial_, multiList_, partials_, visId_, visDepth_, visSlot_, visClass_, rgba8_, rgba32f_, texDesc_;
     unsigned clipCap_ = 0, binCap_ = 0, bigCap_ = 0, itemCap_ = 0, partialCap_ = 0;
     int numCUs_ = 256;

@@ -261,8 +261,9 @@ struct RasterQueue {
     int tileThreads;  // threads per tile at 1 sample per pixel: 64 or 256
 };
 
-// Launch wrappers (pipeline.hip).
-void launch_vertex(const float* vtx, int stride, int nverts, const Views& views, int passthrough, float4* clip,
+// Launch wrappers.  clipAt: -1 transforms attribute 0 (xyz) by each view's
+// matrix; otherwise the record holds the clip position at float clipAt.
+void launch_vertex(const float* vtx, int stride, int nverts, const Views& views, int clipAt, float4* clip,
                    unsigned* binCount, unsigned* binCursor, int tiles, DeviceCounters* ctr,
                    const float4* groupBounds, int groups, unsigned char* groupOut, hipStream_t s);
 void launch_group_bounds(const float* vtx, int stride, int nverts, const int* idx, int ntri, float4* bounds,
@@ -335,6 +336,17 @@ struct MeshParams {
 };
 void launch_mesh_expand(const MeshParams& p, const float* records, const int* indices, const int* vo, const int* io,
                         float* outVtx, int* outIdx, DeviceCounters* ctr, hipStream_t s);
+// The built-in geometry program: triangle t's vertices, transformed (or passed through) and moved
+// by offset, emitted as records (input record, then clip xyzw) 3t .. 3t + 2, indices 3t .. 3t + 2.
+struct GeometryParams {
+    float mvp[16];
+    float offset[4];
+    int passthrough;
+    int stride;        // floats per input vertex record
+    int nverts, ntri;
+};
+void launch_geometry_expand(const GeometryParams& p, const float* vtx, const int* idx, float* outVtx, int* outIdx,
+                            DeviceCounters* ctr, hipStream_t s);
 
 // Texture helpers (texture.hip).
 // Order-independent 64-bit digest of n words (equal buffers give equal digests).

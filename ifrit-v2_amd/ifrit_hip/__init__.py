@@ -39,6 +39,10 @@ class MeshProgramDesc(C.Structure):
                 ("vertexOffsetSlot", C.c_int32), ("indexOffsetSlot", C.c_int32), ("recordFloats", C.c_int32)]
 
 
+class GeometryProgramDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("offset", C.c_float * 4)]
+
+
 class SamplerDesc(C.Structure):
     _fields_ = [("filterMode", C.c_int32), ("addressModeU", C.c_int32), ("addressModeV", C.c_int32),
                 ("addressModeW", C.c_int32), ("borderColor", C.c_int32), ("anisotropyEnable", C.c_int32),
@@ -131,7 +135,8 @@ EXPORTED_SYMBOLS = [
     "iftrGetTuning", "iftrAutotune", "iftrGetDigest", "iftrSetColorFloatOutput", "iftrGetColorRGBA32F",
     "iftrDrawNoAlloc", "iftrNeuralValidateEx", "iftrSetCullMode", "iftrSetRasterizerPolygonMode",
     "iftrCreateBuffer", "iftrCopyHostBufferToBuffer", "iftrCreateGpuMeshProgram", "iftrCreateGpuTaskProgram",
-    "iftrBindMeshShader", "iftrBindTaskShader", "iftrDrawMeshTasks", "iftrSetClearValues", "iftrSetScissors",
+    "iftrBindMeshShader", "iftrBindTaskShader", "iftrDrawMeshTasks", "iftrCreateGpuGeometryProgram",
+    "iftrBindGeometryShader", "iftrSetClearValues", "iftrSetScissors",
     "iftrSetScissorTestEnable", "iftrCreateTextureLayers", "iftrCopyHostBufferToImageLayer", "iftrCreateSampler",
     "iftrBlitImage",
 ]
@@ -209,6 +214,7 @@ def lib():
         "iftrCreateGpuMeshProgram": ([vp], vp), "iftrCreateGpuTaskProgram": ([vp], vp),
         "iftrBindMeshShader": ([vp, vp], None), "iftrBindTaskShader": ([vp, vp], None),
         "iftrDrawMeshTasks": ([vp, C.c_int, C.c_int, C.c_int], None),
+        "iftrCreateGpuGeometryProgram": ([vp], vp), "iftrBindGeometryShader": ([vp, vp], None),
         "iftrSetClearValues": ([vp, vp, C.c_size_t, C.c_float], None),
         "iftrSetScissors": ([vp, vp, C.c_size_t], None), "iftrSetScissorTestEnable": ([vp, C.c_int], None),
         "iftrCreateTextureLayers": ([vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
@@ -305,6 +311,17 @@ class Renderer:
             raise IfritError(f"frame incomplete, status bits {st['status']:#x}")
         return st
 
+    def bind_clip_offset_geometry(self, offset=(0.03, 0.0, 0.0, 0.0)):
+        '''Binds the built-in geometry program that moves every clip position by offset (None unbinds).'''
+        if offset is None:
+            self.L.iftrBindGeometryShader(self.h, None)
+        else:
+            d = GeometryProgramDesc(1, (C.c_float * 4)(*[float(x) for x in offset]))
+            p = self.L.iftrCreateGpuGeometryProgram(C.byref(d))
+            self._programs.append(p)
+            self.L.iftrBindGeometryShader(self.h, p)
+        self._check()
+
     def set_polygon_mode(self, mode: int):
         '''IF_POLYGON_MODE_*: 0 fill (the default), 1 line, 2 point.'''
         self.L.iftrSetRasterizerPolygonMode(self.h, mode)
@@ -382,9 +399,7 @@ class Renderer:
         self.L.iftrBlitImage(self.h, src, dst, C.byref(b), 1 if linear else 0)
         self._check()
 
-    def bind_fragment_program(self, scene, program=None, precision=0, offset=0):
-        '''offset: floats before the scene's vertex record in the drawn vertices (4 for the records
-        DrawMeshTasks emits: clip position, then the record).'''
+    def bind_fragment_program(self, scene, program=None, precision=0):
         d = FragmentProgramDesc()
         d.kind = scene.program if program is None else program
         d.precision = precision
@@ -393,8 +408,8 @@ class Renderer:
         dw = getattr(scene, "depth_write", None)  # (scale, bias): the program writes depth * scale + bias
         if dw is not None:
             d.depthWrite, d.depthScale, d.depthBias = 1, float(dw[0]), float(dw[1])
-        d.offWorld, d.offNormal, d.offUv = offset, offset + 3, offset + 6
-        d.offTangent = offset + 8 if scene.stride >= 11 else 0
+        d.offWorld, d.offNormal, d.offUv = 0, 3, 6
+        d.offTangent = 8 if scene.stride >= 11 else 0
         d.trisPerMaterial = scene.tris_per_material
         d.sampler, d.maxAniso = scene.sampler, scene.max_aniso
         d.camera[:] = [float(x) for x in scene.camera]

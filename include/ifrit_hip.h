@@ -230,9 +230,9 @@ void iftrCopyHostBufferToBuffer(IfritRendererHandle* h, const void* src, int slo
    (demo/shader/MeshletDemoShaders.cu): meshlet m owns vertices vo[m] .. vo[m + 1] - 1 of the vertex
    slot (records of recordFloats floats, the position xyz first) and indices io[m] .. io[m + 1] - 1 of
    the index slot, local to the meshlet's vertices, where vo and io are the int32 offset slots.  It
-   emits each vertex as mvp * (x, y, z, 1) (the transform program's summation order) followed by its
-   record, so the fragment program's attribute offsets count from 4 (an offset k of the record is
-   4 + k), and its indices rebased onto the emitted vertices.  The task program of that demo launches one
+   emits each vertex as its record followed by mvp * (x, y, z, 1) (the transform program's summation
+   order), so the fragment program reads the record at its usual offsets, and its indices rebased
+   onto the emitted vertices.  The task program of that demo launches one
    mesh work group per task work group with the task's id as payload, i.e. meshlet m = work group m.
    The emitted meshlets are concatenated in work-group order (the reference's compaction takes
    them in atomic order) and drawn as a triangle list by the bound pipeline state: the primitive
@@ -257,6 +257,23 @@ void iftrBindTaskShader(IfritRendererHandle* h, void* program);   /* NULL unbind
    the clear values, 0 draws over the current frame (the CUDA renderer's DrawMeshTasks does not
    clear; it is preceded by clear()). */
 void iftrDrawMeshTasks(IfritRendererHandle* h, int numWorkGroups, int firstWorkGroup, int clear);
+
+/* Built-in geometry program (bindGeometryShader, TileRasterRendererCuda.h:87; the geometry stage
+   of TileRasterCoreInvocationCuda.cu:500-560).  The reference runs a user device object per
+   triangle; here IFHIP_GS_CLIP_OFFSET is the reference demo's geometry shader
+   (demo/shader/DefaultDemoShaders.cu:82-99): each triangle is emitted once, its three clip positions
+   moved by `offset` (the demo: x + 0.03), its vertex records passed on.  With a geometry program
+   bound, a draw emits the triangles in order (the order of the reference's deterministic sort keys,
+   :539-542) as unshared vertices and draws them with the bound fragment program and state; the
+   primitive id is the triangle's index.  The bound vertex program runs first (a GPU program; host
+   vertex functions, view batches and iftrDrawNoAlloc are refused).  NULL unbinds. */
+enum { IFHIP_GS_CLIP_OFFSET = 1 };
+typedef struct {
+    int32_t kind;       /* IFHIP_GS_CLIP_OFFSET */
+    float offset[4];    /* added to every emitted clip position */
+} IfritGpuGeometryProgramDesc;
+void* iftrCreateGpuGeometryProgram(const IfritGpuGeometryProgramDesc* desc);
+void iftrBindGeometryShader(IfritRendererHandle* h, void* program);
 
 /* IfritPolygonMode, include/ifrit/softgraphics/engine/base/Constants.h:23-28 */
 enum { IF_POLYGON_MODE_FILL = 0, IF_POLYGON_MODE_LINE = 1, IF_POLYGON_MODE_POINT = 2 };

@@ -452,3 +452,17 @@ def blit(src, dst, src_region, dst_region, linear=True):
         res = src[np.minimum(rnd(cy), sh - 1), np.minimum(rnd(cx), sw - 1)]
     out[dy0:dy1, dx0:dx1] = res
     return out
+
+
+def geometry_clip_offset(vertices, indices, mvp, offset, passthrough=False):
+    """The geometry stage (TileRasterCoreInvocationCuda.cu:502-546) running the demo's geometry
+    shader (demo/shader/DefaultDemoShaders.cu:82-99): every triangle is emitted once, its three
+    vertices unshared, each clip position (mvp * (x, y, z, 1), or the record's first four floats
+    for clip pass-through) plus `offset`, each varying record passed on unchanged.  Triangles land
+    in input order, the order the deterministic sort keys (:539-542) give.  Returns (clip, records,
+    indices) of the emitted list (indices 0 .. 3n - 1)."""
+    v = np.ascontiguousarray(vertices, np.float32)
+    idx = np.ascontiguousarray(indices, np.int32)[: len(indices) // 3 * 3]
+    clip = v[:, :4].copy() if passthrough else vertex_transform(v, mvp)
+    out = clip[idx] + np.asarray(offset, np.float32).reshape(1, 4)
+    return out.astype(np.float32), v[idx].copy(), np.arange(idx.size, dtype=np.int32)

@@ -39,7 +39,6 @@ def test_meshlets_equal_the_ordinary_draw(renderer_factory):
     for slot, a in enumerate((recs, loc, vo, io)):
         r.create_buffer(slot, a)
     r.bind_meshlets(s.mvp, recs.shape[1])
-    r.bind_fragment_program(flat, offset=4)
     st = r.draw_mesh_tasks(len(vo) - 1)
     ids1, d1 = r.visibility()
     print(f"[meshlets] {len(vo) - 1} meshlets, {len(loc) // 3} triangles, covered {(ids1 != 0xffffffff).sum()}",
@@ -49,7 +48,6 @@ def test_meshlets_equal_the_ordinary_draw(renderer_factory):
     assert np.array_equal(r.color(), c0)
     assert st["inputTriangles"] == len(loc) // 3
     # the bound inputs come back after the mesh draw
-    r.bind_fragment_program(flat)
     r.draw()
     assert np.array_equal(r.visibility()[0], ids0)
 
@@ -62,7 +60,6 @@ def test_meshlets_from_a_later_work_group(renderer_factory):
     for slot, a in enumerate((recs, loc, vo, io)):
         r.create_buffer(slot, a)
     r.bind_meshlets(s.mvp, recs.shape[1])
-    r.bind_fragment_program(flat, offset=4)
     r.draw_mesh_tasks(len(vo) - 1 - k, first=k)
     ids, depth = r.visibility()
     # the remaining triangles, rebased onto the remaining vertices, numbered from 0
