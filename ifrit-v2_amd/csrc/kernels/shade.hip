@@ -35,23 +35,30 @@ struct ShadeTri {
     int swapped;  // set up with its first and third vertex exchanged (cull modes none and front)
 };
 
-// Slots below fanStart (the view's first clip fan slot) are unclipped triangles.
+// The ntri slots below fanStart (the view's first clip fan slot) are the
+// unclipped triangles, triangle p in slot fanStart - ntri + p: their
+// primitive comes from the slot itself, so the vertex indices can be read in
+// the same round trip as the record (a fan slot reads its key first).  The
+// record's planes are read last: a load still in flight when the two kinds
+// of slot rejoin would otherwise be waited for there.
 __device__ __forceinline__ void load_shade_tri(const TriRecs& recs, const float4* bary, unsigned fanStart,
-                                               unsigned slot, ShadeTri& T, const unsigned char* swapped) {
-    T.f1 = recs.f1[slot];
-    T.f2 = recs.f2[slot];
-    T.f3 = recs.f3[slot];
-    T.prim = (int)(__float_as_uint(recs.q2[slot].y) >> 2);
-    T.swapped = swapped ? swapped[slot] : 0;
+                                               unsigned ntri, unsigned slot, ShadeTri& T,
+                                               const unsigned char* swapped) {
     if (slot < fanStart) {
+        T.prim = (int)(slot - (fanStart - ntri));
         // unclipped triangles keep bx = (0, 0, 1), by = (1, 0, 0) (TileRasterWorker.cpp:711-712)
         T.bx[0] = 0.0f; T.bx[1] = 0.0f; T.bx[2] = 1.0f;
         T.by[0] = 1.0f; T.by[1] = 0.0f; T.by[2] = 0.0f;
     } else {
+        T.prim = (int)(__float_as_uint(recs.q2[slot].y) >> 2);
         const float4 a = bary[2 * slot], b = bary[2 * slot + 1];
         T.bx[0] = a.x; T.bx[1] = a.y; T.bx[2] = a.z;
         T.by[0] = a.w; T.by[1] = b.x; T.by[2] = b.y;
     }
+    T.swapped = swapped ? swapped[slot] : 0;
+    T.f1 = recs.f1[slot];
+    T.f2 = recs.f2[slot];
+    T.f3 = recs.f3[slot];
 }
 
 // Perspective-correct weights of the three original vertices at (x, y).
@@ -64,10 +71,6 @@ __device__ __forceinline__ void vertex_weights(const ShadeTri& T, float x, float
     w[0] = fmaf(B1, T.bx[0], fmaf(B2, T.bx[1], B3 * T.bx[2]));
     w[1] = fmaf(B1, T.by[0], fmaf(B2, T.by[1], B3 * T.by[2]));
     w[2] = 1.0f - w[0] - w[1];
-}
-
-__device__ __forceinline__ float4 lerp4(float4 a, float4 b, float t) {
-    return make_float4(fmaf(b.x - a.x, t, a.x), fmaf(b.y - a.y, t, a.y), fmaf(b.z - a.z, t, a.z), fmaf(b.w - a.w, t, a.w));
 }
 
 // The two mip levels a trilinear lookup at lod blends: level l0 = floor of
@@ -199,20 +202,9 @@ __device__ __forceinline__ BilinearTexels bilinear_fetch_mode(const uchar4* __re
     return B;
 }
 
-__device__ __forceinline__ float4 texel_f4(unsigned t) {
-    return make_float4((float)(t & 255u), (float)((t >> 8) & 255u), (float)((t >> 16) & 255u), (float)(t >> 24));
-}
-
-// The blend, on the 0..255 scale of the stored bytes.
-__device__ __forceinline__ float4 bilinear_blend(const BilinearTexels& B) {
-    return lerp4(lerp4(texel_f4(B.t00), texel_f4(B.t10), B.ax), lerp4(texel_f4(B.t01), texel_f4(B.t11), B.ax), B.ay);
-}
-
-// Trilinear lookup with prepared levels, scaled to 0..1 once at the end.
-// Both levels are always fetched before either is blended, so all eight
-// texel loads are in flight together (one memory round trip per tap).  With
-// a blend fraction of 0 the second level is the first, both blends give the
-// same value a, and a + (a - a)·0 is a exactly.
+// Trilinear lookup with prepared levels.  Both levels are always fetched
+// before either is blended, so all eight texel loads are in flight together
+// (one memory round trip per tap).
 struct TrilinearTexels {
     BilinearTexels b0, b1;
     float f;
@@ -235,10 +227,34 @@ __device__ __forceinline__ TrilinearTexels trilinear_fetch_mode(const TriLevels&
     return T;
 }
 
+// The blend as one weighted sum of the eight texels: a texel's weight is its
+// bilinear weight times its level's weight (1 - f or f), with the scale of
+// the stored bytes (1/255) folded into the level weights.  With f = 0 the
+// second level's weights are 0.
+__device__ __forceinline__ void bilinear_weights(const BilinearTexels& B, float g, float w[4]) {
+    const float gy1 = g * B.ay, gy0 = g - gy1;
+    w[1] = gy0 * B.ax;
+    w[0] = gy0 - w[1];
+    w[3] = gy1 * B.ax;
+    w[2] = gy1 - w[3];
+}
+
 __device__ __forceinline__ float4 trilinear_blend(const TrilinearTexels& T) {
     constexpr float k = 1.0f / 255.0f;
-    const float4 c = lerp4(bilinear_blend(T.b0), bilinear_blend(T.b1), T.f);
-    return make_float4(c.x * k, c.y * k, c.z * k, c.w * k);
+    const float g1 = T.f * k;
+    float w[8];
+    bilinear_weights(T.b0, k - g1, w);
+    bilinear_weights(T.b1, g1, w + 4);
+    const unsigned t[8] = {T.b0.t00, T.b0.t10, T.b0.t01, T.b0.t11, T.b1.t00, T.b1.t10, T.b1.t01, T.b1.t11};
+    float c[4];
+#pragma unroll
+    for (int ch = 0; ch < 4; ch++) {
+        float r = w[0] * (float)((t[0] >> (8 * ch)) & 255u);
+#pragma unroll
+        for (int i = 1; i < 8; i++) r = fmaf(w[i], (float)((t[i] >> (8 * ch)) & 255u), r);
+        c[ch] = r;
+    }
+    return make_float4(c[0], c[1], c[2], c[3]);
 }
 
 __device__ __forceinline__ float4 trilinear_levels(const TriLevels& L, float u, float v) {
@@ -331,8 +347,12 @@ __device__ __forceinline__ uint32_t flat_hash(uint32_t prim) {
     return h;
 }
 
+// Dot products and lengths with fused multiply-adds (the shading is held
+// to within one step of an FP64 computation, not to the CPU path's bits).
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z)); }
+
 __device__ __forceinline__ float3 norm3(float3 a) {
-    const float l2 = a.x * a.x + a.y * a.y + a.z * a.z;
+    const float l2 = dot3(a, a);
     const float inv = l2 > 0.0f ? frsq(l2) : 1.0f;
     return make_float3(a.x * inv, a.y * inv, a.z * inv);
 }
@@ -352,7 +372,6 @@ struct FragPrep {
     float3 N, Tg, Bg, V, Pw;
 };
 
-__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
 // Layered GGX of one light (Trowbridge-Reitz with alpha = roughness,
 // Smith-Schlick k = alpha / 2, Schlick Fresnel F0 = 0.04, Lambert diffuse
@@ -398,10 +417,12 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, float3 cam, const ShadeT
     }
     const float x = (float)px + o, y = (float)py + o;
     // the planes weigh the vertices in the order setup took them: exchanged triangles
-    // fetch their first and third vertex the other way round
-    const float* v0 = vtx + (size_t)idx[3 * T.prim + (T.swapped ? 2 : 0)] * P.stride;
-    const float* v1 = vtx + (size_t)idx[3 * T.prim + 1] * P.stride;
-    const float* v2 = vtx + (size_t)idx[3 * T.prim + (T.swapped ? 0 : 2)] * P.stride;
+    // take their first and third vertex the other way round (chosen after the loads, so
+    // that the index loads need not wait for the flag)
+    const int i0 = idx[3 * T.prim], i1 = idx[3 * T.prim + 1], i2 = idx[3 * T.prim + 2];
+    const float* v0 = vtx + (size_t)(T.swapped ? i2 : i0) * P.stride;
+    const float* v1 = vtx + (size_t)i1 * P.stride;
+    const float* v2 = vtx + (size_t)(T.swapped ? i0 : i2) * P.stride;
     if (P.numTextures > 0) {
         // the texture header depends on the primitive only: it is read in the same round
         // trip as the vertex indices (issued after them, so that they are in flight together)
@@ -434,17 +455,20 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, float3 cam, const ShadeT
     if (P.numTextures > 0) {
         R.u = fmaf(w[0], t0u, fmaf(w[1], t1u, w[2] * t2u));
         R.v = fmaf(w[0], t0v, fmaf(w[1], t1v, w[2] * t2v));
-        // fine quad derivatives (SBC:36-76): same triangle at the quad's other shading positions
+        // fine quad derivatives (SBC:36-76): u(qx + 1, y) - u(qx, y) and u(x, qy + 1) - u(x, qy),
+        // the same triangle at the quad's other shading positions.  The pixel is qx or qx + 1
+        // (exactly, in float) and qy or qy + 1, so its own (u, v) is one end of each difference
+        // and only the other column and the other row are evaluated.
         const float qx = (float)(px & ~1) + o, qy = (float)(py & ~1) + o;
-        float wa[3], wb[3], wc[3], wd[3];
-        vertex_weights(T, qx, y, wa);
-        vertex_weights(T, qx + 1.0f, y, wb);
-        vertex_weights(T, x, qy, wc);
-        vertex_weights(T, x, qy + 1.0f, wd);
+        const bool oddx = (px & 1) != 0, oddy = (py & 1) != 0;
+        float wx[3], wy[3];
+        vertex_weights(T, oddx ? qx : qx + 1.0f, y, wx);
+        vertex_weights(T, x, oddy ? qy : qy + 1.0f, wy);
         auto iu = [&](const float* ww) { return fmaf(ww[0], t0u, fmaf(ww[1], t1u, ww[2] * t2u)); };
         auto iv = [&](const float* ww) { return fmaf(ww[0], t0v, fmaf(ww[1], t1v, ww[2] * t2v)); };
-        const float dudx = iu(wb) - iu(wa), dvdx = iv(wb) - iv(wa);
-        const float dudy = iu(wd) - iu(wc), dvdy = iv(wd) - iv(wc);
+        const float ux = iu(wx), vx = iv(wx), uy = iu(wy), vy = iv(wy);
+        const float dudx = oddx ? R.u - ux : ux - R.u, dvdx = oddx ? R.v - vx : vx - R.v;
+        const float dudy = oddy ? R.u - uy : uy - R.u, dvdy = oddy ? R.v - vy : vy - R.v;
         R.fp = footprint(R.th, P.sampler, P.maxAniso, dudx, dvdx, dudy, dvdy);
     }
     if (frame && P.program == 4 && P.neural.singleTap && R.mat >= 0) {
@@ -480,21 +504,24 @@ __device__ FragPrep prep_fragment(const ShadeParams& P, float3 cam, const ShadeT
     const float3 V = norm3(make_float3(cam.x - Pw.x, cam.y - Pw.y, cam.z - Pw.z));
     float3 A = make_float3(P.ambient, P.ambient, P.ambient), B = make_float3(0, 0, 0);
     for (int l = 0; l < P.numLights; l++) {
-        float3 L;
-        float att = 1.0f;
+        // n.l and the unnormalised half vector l + v, with l = Lv / |Lv|
+        float att = 1.0f, ndl;
+        float3 Hr;
         if (P.lightPos[l][3] == 0.0f) {
-            L = norm3(make_float3(P.lightPos[l][0], P.lightPos[l][1], P.lightPos[l][2]));
+            const float3 L = norm3(make_float3(P.lightPos[l][0], P.lightPos[l][1], P.lightPos[l][2]));
+            ndl = dot3(N, L);
+            Hr = make_float3(L.x + V.x, L.y + V.y, L.z + V.z);
         } else {
             const float3 Lv = make_float3(P.lightPos[l][0] - Pw.x, P.lightPos[l][1] - Pw.y, P.lightPos[l][2] - Pw.z);
-            const float d2 = Lv.x * Lv.x + Lv.y * Lv.y + Lv.z * Lv.z;
+            const float d2 = dot3(Lv, Lv);
             const float inv = frsq(d2);
-            L = make_float3(Lv.x * inv, Lv.y * inv, Lv.z * inv);
-            att = frcp(1.0f + P.lightColor[l][3] * d2);
+            ndl = dot3(N, Lv) * inv;
+            Hr = make_float3(fmaf(Lv.x, inv, V.x), fmaf(Lv.y, inv, V.y), fmaf(Lv.z, inv, V.z));
+            att = frcp(fmaf(P.lightColor[l][3], d2, 1.0f));
         }
-        const float ndl = N.x * L.x + N.y * L.y + N.z * L.z;
         if (ndl > 0.0f) {
-            const float3 Hh = norm3(make_float3(L.x + V.x, L.y + V.y, L.z + V.z));
-            const float ndh = fmaxf(N.x * Hh.x + N.y * Hh.y + N.z * Hh.z, 0.0f);
+            const float h2 = dot3(Hr, Hr);
+            const float ndh = fmaxf(dot3(N, Hr) * (h2 > 0.0f ? frsq(h2) : 1.0f), 0.0f);
             // ndh^shininess (powf's value at ndh = 0: 1 for shininess 0, else 0)
             const float pw = ndh > 0.0f ? fexp2(P.shininess * flog2(ndh)) : (P.shininess == 0.0f ? 1.0f : 0.0f);
             const float spec = P.ks * pw;
@@ -557,15 +584,20 @@ __device__ __forceinline__ void frame_dirs(const ShadeParams& P, const FragPrep&
     lt = make_float3(dot3(L, R.Tg), dot3(L, R.Bg), dot3(L, R.N));
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave with DPP moves: row
+// shifts by 1, 2, 4 and 8 (a lane whose source lies outside its row of 16
+// adds 0), then the last lane of rows 0 and 2 added to rows 1 and 3
+// (row_bcast:15) and the last lane of row 1 to rows 2 and 3 (row_bcast:31).
 __device__ __forceinline__ int wave_incl_scan(int v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(v, off, 64);
-        if (lane >= off) v += y;
-    }
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
     return v;
 }
+
 
 constexpr int kTapWindow = 128;
 
@@ -635,14 +667,25 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
             for (int s = 0; s < S; s++) ids[s] = visSlot[pix * S + s];
         }
     }
-    int nf = 0;
+    // one comparison per pair of samples: sample s is a first occurrence when no
+    // earlier sample holds its triangle, and each equal pair adds one to the
+    // earlier sample's count (so a first occurrence counts all its samples)
+    unsigned firstMask = 0;
+    int cnt[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) cnt[s] = 1;
 #pragma unroll
     for (int s = 0; s < S; s++) {
         bool first = ids[s] != 0xffffffffu;
 #pragma unroll
-        for (int q = 0; q < s; q++) first = first && ids[q] != ids[s];
-        nf += first;
+        for (int q = 0; q < s; q++) {
+            const bool e = ids[q] == ids[s];
+            first = first && !e;
+            cnt[q] += e ? 1 : 0;
+        }
+        firstMask |= first ? 1u << s : 0u;
     }
+    const int nf = __builtin_popcount(firstMask);
     const int incl = wave_incl_scan(nf);
     const int poff = incl - nf;
     const int F = __shfl(incl, 63, 64);
@@ -650,15 +693,9 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
         int k = poff;
 #pragma unroll
         for (int s = 0; s < S; s++) {
-            bool first = ids[s] != 0xffffffffu;
-#pragma unroll
-            for (int q = 0; q < s; q++) first = first && ids[q] != ids[s];
-            if (first) {
-                int count = 0;
-#pragma unroll
-                for (int q = 0; q < S; q++) count += ids[q] == ids[s];
+            if (firstMask & (1u << s)) {
                 fSlot[k] = ids[s];
-                fCnt[k] = (unsigned char)count;
+                fCnt[k] = (unsigned char)cnt[s];
                 fPix[k] = (unsigned char)lane;
                 k++;
             }
@@ -683,7 +720,7 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
             const unsigned slot = fSlot[fi];
             const int fl = fPix[fi];
             ShadeTri T;
-            load_shade_tri(recs, bary, fanStart, slot, T, P.swapped);
+            load_shade_tri(recs, bary, fanStart, (unsigned)P.ntri, slot, T, P.swapped);
             const int qx = ox + (fl & 7), qy = oy + (fl >> 3);
             R = prep_fragment(P, cam, T, vtx, idx, tex, pool, qx, qy, o, NEURAL || LAYERED, GENERIC);
             if (R.mat >= 0) {
@@ -869,9 +906,14 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
 }
 
 // One wave per shading tile of 8N x 8N pixels (N = 1, 2 or 4: P.shadeTile is
-// 8N), its 8x8 sub-tiles in row order; consecutive tiles on one XCD share its
-// L2 (texture and vertex reuse).  Tiles of all shaded views are numbered one
-// after another.
+// 8N), its 8x8 sub-tiles in row order.  Workgroups go to the XCDs in turn;
+// an XCD takes runs of kShadeRun consecutive tiles (texture and vertex reuse
+// in its L2), the runs of the eight XCDs interleaved, so that every XCD
+// shades a share of every part of the frame (one band per XCD leaves the
+// XCDs of the busy bands last to finish).  Tiles of all shaded views are
+// numbered one after another.
+constexpr int kShadeRun = 64;
+
 template <int S, bool NEURAL, bool FP8, int N, bool LAYERED, bool GENERIC = false>
 __device__ __forceinline__ void shade_tiles(const ShadeParams& P, const Views& V, const TriRecs& recs,
                                             const float4* __restrict__ bary, const uint32_t* __restrict__ visSlot,
@@ -880,7 +922,8 @@ __device__ __forceinline__ void shade_tiles(const ShadeParams& P, const Views& V
                                             uchar4* __restrict__ rgba8, float4* __restrict__ rgba32f, int tiles,
                                             int perXcd) {
     constexpr int T = 8 * N;
-    const int tile = (int)(blockIdx.x & 7u) * perXcd + (int)(blockIdx.x >> 3);
+    const int g = (int)(blockIdx.x >> 3);
+    const int tile = (g / kShadeRun) * (8 * kShadeRun) + (int)(blockIdx.x & 7u) * kShadeRun + g % kShadeRun;
     if (tile >= tiles) return;
     const ViewDesc& D = V.desc[V.shadeView[tile]];
     const int width = D.width, height = D.height;
@@ -1023,7 +1066,7 @@ __global__ void __launch_bounds__(256) k_blend_ordered(ShadeParams P, BlendCoefs
             frags++;
             if (!rc::depth_pass(depthFunc, d, dep, c == 3u)) return;
             ShadeTri T;
-            load_shade_tri(recs, bary, fanStart, sl, T, P.swapped);
+            load_shade_tri(recs, bary, fanStart, (unsigned)P.ntri, sl, T, P.swapped);
             col = blend_pixel(B, shade_fragment(P, cam, T, vtx, idx, tex, pool, C.px, C.py), col);
             dep = P.depthWrite ? d * P.depthScale + P.depthBias : d;
             slot = sl;
@@ -1113,7 +1156,8 @@ void launch_shade(const ShadeParams& p, const Views& views, int shadeTiles, TriR
                   const uint32_t* visSlot, const float* vtx, const int* idx, const TexDesc* tex,
                   const uint8_t* texPool, uchar4* rgba8, float4* rgba32f, hipStream_t s) {
     if (shadeTiles <= 0) return;
-    const int perXcd = (shadeTiles + 7) / 8;
+    // runs of kShadeRun tiles per XCD: the grid covers whole rounds of eight runs
+    const int perXcd = ((shadeTiles + 7) / 8 + kShadeRun - 1) / kShadeRun * kShadeRun;
     const dim3 grid(8 * perXcd);
 #define IFRIT_ARGS p, views, recs, bary, visSlot, vtx, idx, tex, texPool, rgba8, rgba32f, shadeTiles, perXcd
 #define IFRIT_SHADE_N(SN, NN)                                                                                      \
