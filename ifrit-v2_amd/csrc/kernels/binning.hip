@@ -190,12 +190,7 @@ __device__ __forceinline__ void scan256x6(unsigned* v, unsigned (*sWave)[4], uns
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < kPlanQ; k++) {
-        unsigned x = v[k];
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const unsigned y = __shfl_up(x, off, 64);
-            if (lane >= off) x += y;
-        }
+        const unsigned x = dev::wave_incl_scan(v[k]);
         if (lane == 63) sWave[k][w] = x;
         v[k] = x - v[k];
     }
@@ -246,13 +241,7 @@ __global__ void __launch_bounds__(1024) k_plan_scan(unsigned* __restrict__ block
             x[k] = v[k];
         }
 #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-#pragma unroll
-            for (int k = 0; k < kPlanQ; k++) {
-                const unsigned y = __shfl_up(x[k], off, 64);
-                if (lane >= off) x[k] += y;
-            }
-        }
+        for (int k = 0; k < kPlanQ; k++) x[k] = dev::wave_incl_scan(x[k]);
         __syncthreads();  // sWave is free
         if (lane == 63)
 #pragma unroll

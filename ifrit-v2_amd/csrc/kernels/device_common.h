@@ -212,5 +212,21 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave with DPP moves: row
+// shifts by 1, 2, 4 and 8 (a lane whose source lies outside its row of 16
+// adds 0), then the last lane of rows 0 and 2 added to rows 1 and 3
+// (row_bcast:15) and the last lane of row 1 to rows 2 and 3 (row_bcast:31).
+template <class T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+    static_assert(sizeof(T) == 4, "32-bit values");
+    v += (T)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);
+    v += (T)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);
+    v += (T)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);
+    v += (T)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);
+    v += (T)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
+    v += (T)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);
+    return v;
+}
+
 }  // namespace dev
 }  // namespace ifrit_hip

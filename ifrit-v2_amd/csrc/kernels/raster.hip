@@ -97,12 +97,7 @@ __device__ __forceinline__ unsigned block_square_mask(const Edges& E, float x0, 
 template <int NT>
 __device__ __forceinline__ unsigned scan_nt(unsigned v, unsigned* sWave, unsigned& total) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    unsigned x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const unsigned y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
+    const unsigned x = wave_incl_scan(v);
     if constexpr (NT == 64) {
         total = __shfl(x, 63, 64);
         return x - v;

@@ -584,21 +584,6 @@ __device__ __forceinline__ void frame_dirs(const ShadeParams& P, const FragPrep&
     lt = make_float3(dot3(L, R.Tg), dot3(L, R.Bg), dot3(L, R.N));
 }
 
-// Inclusive prefix sum over the 64 lanes of a wave with DPP moves: row
-// shifts by 1, 2, 4 and 8 (a lane whose source lies outside its row of 16
-// adds 0), then the last lane of rows 0 and 2 added to rows 1 and 3
-// (row_bcast:15) and the last lane of row 1 to rows 2 and 3 (row_bcast:31).
-__device__ __forceinline__ int wave_incl_scan(int v) {
-    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
-    return v;
-}
-
-
 constexpr int kTapWindow = 128;
 
 // One wave per 8x8 pixel sub-tile.  Work is expanded twice so that lanes stay
@@ -686,7 +671,7 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
         firstMask |= first ? 1u << s : 0u;
     }
     const int nf = __builtin_popcount(firstMask);
-    const int incl = wave_incl_scan(nf);
+    const int incl = dev::wave_incl_scan(nf);
     const int poff = incl - nf;
     const int F = __shfl(incl, 63, 64);
     {
@@ -741,7 +726,7 @@ __device__ __forceinline__ void shade_subtile(const ShadeParams& P, float3 cam, 
                 }
             }
         }
-        const int tincl = wave_incl_scan(n);
+        const int tincl = dev::wave_incl_scan(n);
         const int toff = tincl - n;
         const int TT = __shfl(tincl, 63, 64);
         float4 acc = make_float4(0, 0, 0, 0), acc2 = make_float4(0, 0, 0, 0);
