@@ -314,6 +314,8 @@ void TileRasterRenderer::drawMeshTasks(int numGroups, int firstGroup, bool clear
     if (m.kind != IFHIP_MS_MESHLET) return setError("DrawMeshTasks: unknown mesh program");
     if (tsBound_ && ts_.desc.kind != IFHIP_TS_ONE_MESH_GROUP) return setError("DrawMeshTasks: unknown task program");
     if (numGroups < 0 || firstGroup < 0) return setError("DrawMeshTasks: negative work group count or start");
+    // the meshlet program transforms by its own matrix: there is no per-view transform to apply
+    if (!batch_.empty()) return setError("DrawMeshTasks draws a single view");
     if (m.recordFloats < 3 || m.recordFloats > 256) return setError("DrawMeshTasks: recordFloats must be 3 .. 256");
     const int slots[4] = {m.vertexSlot, m.indexSlot, m.vertexOffsetSlot, m.indexOffsetSlot};
     for (int sl : slots)

@@ -236,7 +236,7 @@ void iftrCopyHostBufferToBuffer(IfritRendererHandle* h, const void* src, int slo
    mesh work group per task work group with the task's id as payload, i.e. meshlet m = work group m.
    The emitted meshlets are concatenated in work-group order (the reference's compaction takes
    them in atomic order) and drawn as a triangle list by the bound pipeline state: the primitive
-   id of a triangle is its position in that list. */
+   id of a triangle is its position in that list.  View batches (iftrSetViews) are refused. */
 enum { IFHIP_MS_MESHLET = 1 };
 enum { IFHIP_TS_ONE_MESH_GROUP = 1 };
 typedef struct {

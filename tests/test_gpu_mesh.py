@@ -84,6 +84,10 @@ def test_mesh_draw_errors(renderer_factory):
         r.create_buffer(slot, a)
     with pytest.raises(IfritError, match="beyond the offset buffers"):
         r.draw_mesh_tasks(len(vo))
+    r.set_views([dict(width=64, height=64, mvp=s.mvp)])
+    with pytest.raises(IfritError, match="single view"):
+        r.draw_mesh_tasks(len(vo) - 1)
+    r.set_views([])
     bad = loc.copy()
     bad[5] = 200  # beyond its meshlet's 64 vertices
     r.create_buffer(1, bad)
