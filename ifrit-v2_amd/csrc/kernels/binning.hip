@@ -223,6 +223,24 @@ __global__ void __launch_bounds__(256) k_plan_sums(const unsigned* __restrict__ 
     if (t < kPlanQ) blockSums[blockIdx.x * kPlanQ + t] = total[t];
 }
 
+// The plan's totals: list entries, items, partial blocks, multi-part and
+// empty tiles; the overflow flags; the item streams' heads reset.
+__device__ __forceinline__ void plan_totals(const unsigned* carry, unsigned cap, const RasterQueue& q,
+                                            DeviceCounters* ctr) {
+    ctr->binTotal = carry[0];
+    const bool binFits = carry[0] <= cap;
+    if (!binFits) atomicOr(&ctr->status, kStatusBinOverflow);
+    const bool fits = carry[1] + carry[2] <= q.itemCap && carry[3] <= q.partialCap;
+    if (!fits) atomicOr(&ctr->status, kStatusRasterOverflow);
+    const bool go = binFits && fits;
+    ctr->numItems = go ? carry[1] + carry[2] : 0u;
+    ctr->numLongItems = carry[1];
+    ctr->numPartials = carry[3];
+    ctr->numMulti = go ? carry[4] : 0u;
+    ctr->numEmpty = go ? carry[5] : 0u;
+    for (int k = 0; k < kItemStreams; k++) ctr->itemHeads[k][0] = 0u;
+}
+
 // One 1024-thread workgroup: exclusive offsets of the block sums (in place)
 // and the totals.  On a bin or work-list overflow nothing is rasterised; the
 // host grows the buffers and draws again.
@@ -259,27 +277,31 @@ __global__ void __launch_bounds__(1024) k_plan_scan(unsigned* __restrict__ block
             carry[k] += tot;
         }
     }
-    if (t == 0) {
-        ctr->binTotal = carry[0];
-        const bool binFits = carry[0] <= cap;
-        if (!binFits) atomicOr(&ctr->status, kStatusBinOverflow);
-        const bool fits = carry[1] + carry[2] <= q.itemCap && carry[3] <= q.partialCap;
-        if (!fits) atomicOr(&ctr->status, kStatusRasterOverflow);
-        const bool go = binFits && fits;
-        ctr->numItems = go ? carry[1] + carry[2] : 0u;
-        ctr->numLongItems = carry[1];
-        ctr->numPartials = carry[3];
-        ctr->numMulti = go ? carry[4] : 0u;
-        ctr->numEmpty = go ? carry[5] : 0u;
-        for (int k = 0; k < kItemStreams; k++) ctr->itemHeads[k][0] = 0u;
-    }
+    if (t == 0) plan_totals(carry, cap, q, ctr);
 }
 
+// Writes the tiles' list starts and work items.  SCAN: blockOffs holds the
+// blocks' own sums (at most 256 blocks), which every block scans itself, and
+// block 0 also stores the totals, in place of k_plan_scan; otherwise
+// blockOffs holds the blocks' offsets from k_plan_scan.
+template <bool SCAN>
 __global__ void __launch_bounds__(256) k_plan_write(const unsigned* __restrict__ cnt, unsigned* __restrict__ start,
-                                                    int n, int sh, const unsigned* __restrict__ blockOffs,
-                                                    RasterQueue q) {
+                                                    int n, int sh, const unsigned* __restrict__ blockOffs, int nblk,
+                                                    unsigned cap, RasterQueue q, DeviceCounters* ctr) {
     __shared__ unsigned sWave[kPlanQ][4];
+    __shared__ unsigned sOff[kPlanQ];
     const int t = threadIdx.x;
+    if constexpr (SCAN) {
+        unsigned b[kPlanQ], tot[kPlanQ];
+#pragma unroll
+        for (int k = 0; k < kPlanQ; k++) b[k] = t < nblk ? blockOffs[t * kPlanQ + k] : 0u;
+        scan256x6(b, sWave, tot);
+        if (t == (int)blockIdx.x)
+#pragma unroll
+            for (int k = 0; k < kPlanQ; k++) sOff[k] = b[k];
+        if (blockIdx.x == 0 && t == 0) plan_totals(tot, cap, q, ctr);
+        __syncthreads();
+    }
     unsigned c[kPlanPer];
     unsigned v[kPlanQ] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -292,7 +314,7 @@ __global__ void __launch_bounds__(256) k_plan_write(const unsigned* __restrict__
     }
     scan256x6(v, sWave, nullptr);
 #pragma unroll
-    for (int k = 0; k < kPlanQ; k++) v[k] += blockOffs[blockIdx.x * kPlanQ + k];
+    for (int k = 0; k < kPlanQ; k++) v[k] += SCAN ? sOff[k] : blockOffs[blockIdx.x * kPlanQ + k];
     const unsigned part = 1u << sh;
     for (int i = 0; i < kPlanPer; i++) {
         const int tile = plan_tile(blockIdx.x, i, t);
@@ -330,9 +352,15 @@ void launch_bin(const BinParams& p, const Views& views, TriRecs recs, const uint
                        ctr);
     const int nblk = (p.tiles + kPlanBlock - 1) / kPlanBlock;
     hipLaunchKernelGGL(k_plan_sums, dim3(nblk), dim3(256), 0, s, binCount, p.tiles, q.partShift, q.planSums);
-    hipLaunchKernelGGL(k_plan_scan, dim3(1), dim3(1024), 0, s, q.planSums, nblk, p.binCap, q, ctr);
-    hipLaunchKernelGGL(k_plan_write, dim3(nblk), dim3(256), 0, s, binCount, binStart, p.tiles, q.partShift,
-                       q.planSums, q);
+    if (nblk <= 256) {
+        // few blocks: each block scans the block sums itself (no third launch)
+        hipLaunchKernelGGL(k_plan_write<true>, dim3(nblk), dim3(256), 0, s, binCount, binStart, p.tiles,
+                           q.partShift, q.planSums, nblk, p.binCap, q, ctr);
+    } else {
+        hipLaunchKernelGGL(k_plan_scan, dim3(1), dim3(1024), 0, s, q.planSums, nblk, p.binCap, q, ctr);
+        hipLaunchKernelGGL(k_plan_write<false>, dim3(nblk), dim3(256), 0, s, binCount, binStart, p.tiles,
+                           q.partShift, q.planSums, nblk, p.binCap, q, ctr);
+    }
     hipLaunchKernelGGL(k_bin_small<true>, g, dim3(256), 0, s, p, views, recs, binQueue, binCount, binStart, binCursor,
                        binList, bigList, ctr);
 }
