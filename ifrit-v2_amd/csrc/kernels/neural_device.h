@@ -64,6 +64,13 @@ __device__ __forceinline__ void relu_slice(const f32x16& z, int s, float* v) {
     for (int q = 0; q < 8; q++) v[q] = fmaxf(z[8 * (s & 1) + q], 0.0f);
 }
 
+// The FP8 network runs its 64 x 64 hidden layer (79 % of the FLOPs) on FP8
+// (E4M3) MFMA and its input and output layers on BF16: E4M3 inputs (latent
+// channels, directions) or E4M3 inputs to the output layer cost 1.5 to 4 dB
+// of PSNR against the material, and the frame then falls below 40 dB.
+constexpr bool kFp8InputLayer = false;
+constexpr bool kFp8OutputLayer = false;
+
 // Forward pass for the 64 fragments of one wave (all 64 lanes must call it),
 // one 32-fragment column tile at a time to bound the live accumulators.
 //   xs: LDS [64][16] inputs (fragment-major, inputs 14 and 15 zero).
@@ -84,7 +91,7 @@ __device__ __forceinline__ void mlp_forward(const NeuralPacked* __restrict__ W, 
             float xv[8];
 #pragma unroll
             for (int q = 0; q < 8; q++) xv[q] = xs[f * 16 + 8 * g + q];
-            if constexpr (FP8) {
+            if constexpr (FP8 && kFp8InputLayer) {
                 const float inv = 1.0f / (W->qscale[0] * W->qscale[3]);
                 const long bx = to_fp8x8(xv, W->qscale[3]);
 #pragma unroll
@@ -153,7 +160,7 @@ __device__ __forceinline__ void mlp_forward(const NeuralPacked* __restrict__ W, 
         }
         // layer 3
         f32x16 y;
-        if constexpr (FP8) {
+        if constexpr (FP8 && kFp8OutputLayer) {
             const float sh = W->qscale[5], inv = 1.0f / (W->qscale[2] * sh);
             f32x16 acc = {};
 #pragma unroll

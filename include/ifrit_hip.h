@@ -183,7 +183,8 @@ typedef struct {
     float lightPos[4][4];       /* w = 0: direction towards the light, w = 1: point light */
     float lightColor[4][4];     /* rgb, attenuation k of 1 / (1 + k d^2) */
     int32_t offTangent;         /* float offset of the tangent (xyz), IFHIP_FS_GGX_LAYERED */
-    int32_t precision;          /* IFHIP_FS_NEURAL: 0 BF16 MFMA, 1 FP8 (E4M3) MFMA */
+    int32_t precision;          /* IFHIP_FS_NEURAL: 0 BF16 MFMA, 1 FP8 (E4M3) MFMA for the hidden layer,
+                                   BF16 for the input and output layers */
     int32_t samplerId;          /* 0: `sampler` / `maxAniso` above with repeat addressing and
                                    bilinear texels; else an id from iftrCreateSampler (Blinn-Phong and
                                    cube-map programs) */
