@@ -904,8 +904,7 @@ __device__ __forceinline__ void shade_tiles(const ShadeParams& P, const Views& V
                                             const float4* __restrict__ bary, const uint32_t* __restrict__ visSlot,
                                             const float* __restrict__ vtx, const int* __restrict__ idx,
                                             const TexDesc* __restrict__ tex, const uint8_t* __restrict__ pool,
-                                            uchar4* __restrict__ rgba8, float4* __restrict__ rgba32f, int tiles,
-                                            int perXcd) {
+                                            uchar4* __restrict__ rgba8, float4* __restrict__ rgba32f, int tiles) {
     constexpr int T = 8 * N;
     const int g = (int)(blockIdx.x >> 3);
     const int tile = (g / kShadeRun) * (8 * kShadeRun) + (int)(blockIdx.x & 7u) * kShadeRun + g % kShadeRun;
@@ -1082,10 +1081,8 @@ template <int S, int N>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8)))
 k_shade(ShadeParams P, Views V, TriRecs recs, const float4* __restrict__ bary, const uint32_t* __restrict__ visSlot,
         const float* __restrict__ vtx, const int* __restrict__ idx, const TexDesc* __restrict__ tex,
-        const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8, float4* __restrict__ rgba32f, int tiles,
-        int perXcd) {
-    shade_tiles<S, false, false, N, false>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles,
-                                           perXcd);
+        const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8, float4* __restrict__ rgba32f, int tiles) {
+    shade_tiles<S, false, false, N, false>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles);
 }
 
 // The single-texture and cube-map programs under a sampler other than
@@ -1096,9 +1093,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_shade_sampled(ShadeParams P, Views V, TriRecs recs, const float4* __restrict__ bary,
                 const uint32_t* __restrict__ visSlot, const float* __restrict__ vtx, const int* __restrict__ idx,
                 const TexDesc* __restrict__ tex, const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8,
-                float4* __restrict__ rgba32f, int tiles, int perXcd) {
-    shade_tiles<S, false, false, N, false, true>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles,
-                                                 perXcd);
+                float4* __restrict__ rgba32f, int tiles) {
+    shade_tiles<S, false, false, N, false, true>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles);
 }
 
 // Layered GGX (programs 3 and 5): four waves per SIMD (at most 128 VGPRs), as
@@ -1108,9 +1104,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_shade_layered(ShadeParams P, Views V, TriRecs recs, const float4* __restrict__ bary,
                 const uint32_t* __restrict__ visSlot, const float* __restrict__ vtx, const int* __restrict__ idx,
                 const TexDesc* __restrict__ tex, const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8,
-                float4* __restrict__ rgba32f, int tiles, int perXcd) {
-    shade_tiles<S, false, false, N, true>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles,
-                                          perXcd);
+                float4* __restrict__ rgba32f, int tiles) {
+    shade_tiles<S, false, false, N, true>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles);
 }
 
 // The neural material, held to four waves per SIMD like the other programs so
@@ -1120,8 +1115,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_shade_neural(ShadeParams P, Views V, TriRecs recs, const float4* __restrict__ bary,
                const uint32_t* __restrict__ visSlot, const float* __restrict__ vtx, const int* __restrict__ idx,
                const TexDesc* __restrict__ tex, const uint8_t* __restrict__ pool, uchar4* __restrict__ rgba8,
-               float4* __restrict__ rgba32f, int tiles, int perXcd) {
-    shade_tiles<S, true, FP8, N, false>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles, perXcd);
+               float4* __restrict__ rgba32f, int tiles) {
+    shade_tiles<S, true, FP8, N, false>(P, V, recs, bary, visSlot, vtx, idx, tex, pool, rgba8, rgba32f, tiles);
 }
 
 }  // namespace
@@ -1144,7 +1139,7 @@ void launch_shade(const ShadeParams& p, const Views& views, int shadeTiles, TriR
     // runs of kShadeRun tiles per XCD: the grid covers whole rounds of eight runs
     const int perXcd = ((shadeTiles + 7) / 8 + kShadeRun - 1) / kShadeRun * kShadeRun;
     const dim3 grid(8 * perXcd);
-#define IFRIT_ARGS p, views, recs, bary, visSlot, vtx, idx, tex, texPool, rgba8, rgba32f, shadeTiles, perXcd
+#define IFRIT_ARGS p, views, recs, bary, visSlot, vtx, idx, tex, texPool, rgba8, rgba32f, shadeTiles
 #define IFRIT_SHADE_N(SN, NN)                                                                                      \
     do {                                                                                                            \
         if (p.program == 3 || p.program == 5)                                                                       \
