@@ -433,10 +433,12 @@ __device__ __forceinline__ void clear_tile(const ViewGeom& G, float clearDepth, 
     }
 }
 
-// LDS layout of the tile's keys: sample-major, 256 pixels per sample.  A
-// padded layout (one key after every 32) lowered the bank-conflict share of
-// LDS cycles from 40 % to 35 % and left the kernel time unchanged: conflicts
-// cost about 3 % of its cycles, and the kernel is bound by VALU issue.
+// LDS layout of the tile's keys: sample-major, 256 pixels per sample.
+// Padded layouts lowered the bank-conflict share of LDS-active cycles (one
+// key after every 32: 40 % to 35 %; four after every 32: 48 % to 31 %) but
+// not the kernel time (unchanged, then 3 % slower from the extra address
+// arithmetic): conflicts cost about 4 % of its cycles, and the kernel is
+// bound by VALU issue.
 constexpr int kVisStride = 256;
 __device__ __forceinline__ int vis_at(int s, int p) { return s * kVisStride + p; }
 
