@@ -209,16 +209,31 @@ __device__ __forceinline__ unsigned key_tie(unsigned long long k) {
     return S == 1 ? (unsigned)k >> 2 : (unsigned)k;
 }
 
+// The record slot of a key's triangle: ptr[prim] + fan from the slot map, or,
+// in a view where no triangle crossed w = 0 (direct != all ones), direct +
+// prim: the setup kernel puts triangle p in slot slotBase + p, and only the
+// clip kernel moves a triangle's slots.
+struct SlotMap {
+    const unsigned* ptr;
+    unsigned direct;
+};
+
 // Record slots of a pixel's S final keys (slotOf[prim] + fan), all S lookups
 // issued before the first is used; a key that holds no fragment looks up
 // primitive 0 and its slot is not used.
 template <int S>
-__device__ __forceinline__ void key_slots(const unsigned* slotOf, const unsigned long long* keys,
+__device__ __forceinline__ void key_slots(const SlotMap& slotOf, const unsigned long long* keys,
                                           unsigned long long none, unsigned* slots) {
+    if (slotOf.direct != 0xffffffffu) {
+        // no triangle of the view was clipped: triangle p sits in slot slotBase + p
+#pragma unroll
+        for (int s = 0; s < S; s++) slots[s] = slotOf.direct + (key_tie<S>(keys[s]) >> 2);
+        return;
+    }
 #pragma unroll
     for (int s = 0; s < S; s++) {
         const unsigned tie = key_tie<S>(keys[s]);
-        slots[s] = slotOf[keys[s] != none ? tie >> 2 : 0u] + (tie & 3u);
+        slots[s] = slotOf.ptr[keys[s] != none ? tie >> 2 : 0u] + (tie & 3u);
     }
 }
 
@@ -226,7 +241,7 @@ __device__ __forceinline__ void key_slots(const unsigned* slotOf, const unsigned
 // slot per sample (the primitive ID is the slot's key >> 2, expanded only on
 // readback).
 template <int S>
-__device__ __forceinline__ void write_pixel(const RasterParams& P, const TriRecs& recs, const unsigned* slotOf,
+__device__ __forceinline__ void write_pixel(const RasterParams& P, const TriRecs& recs, const SlotMap& slotOf,
                                             const PixelCtx& C, const unsigned long long* keys, float* visDepth,
                                             uint32_t* visSlot, uint8_t* visClass) {
     if (!C.inFb || !scissor_pass(P.scissor, C.fpx, C.fpy)) return;
@@ -279,7 +294,7 @@ struct TileOut {
 // record slot per sample (the clear values where no fragment won) and, at 1
 // sample per pixel, the coverage class.
 template <int S>
-__device__ __forceinline__ unsigned pixel_outputs(const RasterParams& P, const TriRecs& recs, const unsigned* slotOf,
+__device__ __forceinline__ unsigned pixel_outputs(const RasterParams& P, const TriRecs& recs, const SlotMap& slotOf,
                                                   const PixelCtx& C, const unsigned long long* keys, float* d,
                                                   unsigned* slot) {
     unsigned cls = 0;
@@ -313,7 +328,7 @@ __device__ __forceinline__ unsigned pixel_outputs(const RasterParams& P, const T
 // Outputs of one pixel of a cleared draw into the staging buffer; the 1-spp
 // class byte goes straight out.
 template <int S>
-__device__ __forceinline__ void stage_pixel(const RasterParams& P, const TriRecs& recs, const unsigned* slotOf,
+__device__ __forceinline__ void stage_pixel(const RasterParams& P, const TriRecs& recs, const SlotMap& slotOf,
                                             const PixelCtx& C, int t, const unsigned long long* keys, TileOut<S>& O,
                                             uint8_t* visClass, const float* visDepth, const uint32_t* visSlot) {
     if (!C.inFb) return;
@@ -341,7 +356,7 @@ __device__ __forceinline__ void stage_pixel(const RasterParams& P, const TriRecs
 // stores (S a multiple of 4: a pixel's samples fill whole 16-byte words, and
 // a row of pixels whole lines).
 template <int S>
-__device__ __forceinline__ void emit_pixel(const RasterParams& P, const TriRecs& recs, const unsigned* slotOf,
+__device__ __forceinline__ void emit_pixel(const RasterParams& P, const TriRecs& recs, const SlotMap& slotOf,
                                            const PixelCtx& C, const unsigned long long* keys, float* visDepth,
                                            uint32_t* visSlot) {
     static_assert(S % 4 == 0, "whole 16-byte words per pixel");
@@ -512,7 +527,7 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
         float* const visDepth = visDepthAll + G.pixBase * S;
         uint32_t* const visSlot = visSlotAll + G.pixBase * S;
         uint8_t* const visClass = visClassAll ? visClassAll + G.pixBase : nullptr;
-        const unsigned* const slotOf = slotOfAll + G.primBase;
+        const SlotMap slotOf{slotOfAll + G.primBase, V.ctr[G.view].clipInCount == 0u ? G.slotBase : 0xffffffffu};
         const int ttx = G.tile % G.tilesX, tty = G.tile / G.tilesX;
         const int tx0 = ttx * 16, ty0 = tty * 16;
 
@@ -766,7 +781,7 @@ __global__ void __launch_bounds__(256) k_raster_merge(RasterParams P, Views V, T
         float* const visDepth = visDepthAll + G.pixBase * S;
         uint32_t* const visSlot = visSlotAll + G.pixBase * S;
         uint8_t* const visClass = visClassAll ? visClassAll + G.pixBase : nullptr;
-        const unsigned* const slotOf = slotOfAll + G.primBase;
+        const SlotMap slotOf{slotOfAll + G.primBase, V.ctr[G.view].clipInCount == 0u ? G.slotBase : 0xffffffffu};
         const PixelCtx C = pixel_ctx(G, t & 15, t >> 4);
         const unsigned start = binStart[tile];
         const unsigned total = start >= P.binCap ? 0u : min(binCount[tile], P.binCap - start);

@@ -45,6 +45,8 @@ struct ViewGeom {
     int tile;  // view-local tile index
     size_t pixBase;
     unsigned primBase;
+    unsigned slotBase;  // record slot of the view's triangle 0
+    int view;
 };
 
 __device__ __forceinline__ ViewGeom view_of_tile(const Views& V, int gtile) {
@@ -56,6 +58,8 @@ __device__ __forceinline__ ViewGeom view_of_tile(const Views& V, int gtile) {
     G.tile = gtile - (int)D.tileBase;
     G.pixBase = (size_t)D.pixBase;
     G.primBase = D.primBase;
+    G.slotBase = D.slotBase;
+    G.view = V.tileView[gtile];
     return G;
 }
 
