@@ -306,6 +306,10 @@ void iftrSetCullMode(IfritRendererHandle* h, int mode) {
     guarded(h, [&](TileRasterRenderer& r) { r.SetCullMode(mode); });
 }
 
+void iftrSetSetupReciprocal(IfritRendererHandle* h, int model) {
+    guarded(h, [&](TileRasterRenderer& r) { r.setSetupReciprocal(model); });
+}
+
 void iftrSetRasterizerPolygonMode(IfritRendererHandle* h, int mode) {
     guarded(h, [&](TileRasterRenderer& r) { r.setRasterizerPolygonMode(mode); });
 }

@@ -8,6 +8,8 @@
 
 #include "../common/x86_fp.h"
 
+#include <cpuid.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -273,6 +275,23 @@ Scissor TileRasterRenderer::activeScissor() const {
 void TileRasterRenderer::SetCullMode(int mode) {
     if (mode < kCullNone || mode > kCullFrontAndBack) return setError("unknown cull mode");
     cullMode_ = mode;
+}
+
+void TileRasterRenderer::setSetupReciprocal(int model) {
+    if (model == IFHIP_RCP_HOST) {
+        unsigned a = 0, b = 0, c = 0, d = 0;
+        char vendor[13] = {0};
+        if (__get_cpuid(0, &a, &b, &c, &d)) {
+            std::memcpy(vendor, &b, 4);
+            std::memcpy(vendor + 4, &d, 4);
+            std::memcpy(vendor + 8, &c, 4);
+        }
+        if (std::strcmp(vendor, "GenuineIntel") == 0) model = IFHIP_RCP_INTEL;
+        else if (std::strcmp(vendor, "AuthenticAMD") == 0) model = IFHIP_RCP_AMD;
+        else return setError(std::string("no RCPPS table for this CPU vendor: ") + vendor);
+    }
+    if (model != IFHIP_RCP_INTEL && model != IFHIP_RCP_AMD) return setError("unknown reciprocal model");
+    rcpModel_ = model == IFHIP_RCP_AMD ? ifx::kRcpAmd : ifx::kRcpIntel;
 }
 
 int TileRasterRenderer::createBuffer(int slot, int size) {
@@ -1021,7 +1040,8 @@ void TileRasterRenderer::enqueue(bool clear) {
     sp.swapped = needSwapFlags() ? swapped_.as<unsigned char>() : nullptr;
     sp.msaa = samples_ > 1 ? 1 : 0;
     sp.clipCap = clipCap_;
-    sp.invTileWidth = ifx::rcp_nr_ss(16.0f);
+    sp.invTileWidth = ifx::rcp_nr_ss(16.0f, rcpModel_);
+    sp.rcpModel = rcpModel_;
     sp.groupOut = cullGroups ? groupOut_.as<unsigned char>() : nullptr;
     sp.groups = groups;
     launch_setup(sp, V, clip_.as<float4>(), idx_.as<int>(), recPtrs(), bary_.as<float4>(), binQueue_.as<uint4>(),
@@ -1145,7 +1165,7 @@ std::vector<uint8_t> TileRasterRenderer::graphKey(bool clear) const {
                         nnFallback_[1] ? 1 : 0, (int)textures_.size(), stageTiming_ ? 1 : 0,
                         fb_ && !fb_->color.empty() && fb_->color[0] ? 1 : 0, batch_.empty() ? 1 : 0,
                         floatOut_ ? 1 : 0, blend_.blendEnable ? 1 : 0, cullMode_, deterministic_ ? 1 : 0,
-                        polygonMode_, curIdx_, clipAt_};
+                        polygonMode_, curIdx_, clipAt_, rcpModel_};
     put(ints, sizeof(ints));
     // emitted lists swap buffers without reallocating them
     const void* inputs[] = {vtx_.ptr, idx_.ptr};

@@ -136,6 +136,7 @@ public:
     bool fsWritesDepth() const { return fsBound_ && fs_.desc.depthWrite != 0; }
     void optSetDepthTestEnable(bool opt);
     void SetCullMode(int mode);  // TileRasterRendererCuda.h:109
+    void setSetupReciprocal(int model);  // IFHIP_RCP_*: whose RCPPS the triangle setup reproduces
     void setRasterizerPolygonMode(int mode);  // TileRasterRendererCuda.h:106: 0 fill, 1 line, 2 point
     // device buffer slots and mesh work groups (TileRasterRendererCuda.h:88-89, 98-99, 115)
     int createBuffer(int slot, int size);
@@ -219,6 +220,7 @@ private:
     int depthFuncSaved_ = IF_COMPARE_OP_LESS;
     bool depthTest_ = true;
     int cullMode_ = kCullBack;  // the CPU path's fixed back-face cull
+    int rcpModel_ = 0;          // ifx::RcpModel of the setup's reciprocals
     int polygonMode_ = 0;       // IfritPolygonMode: 0 fill, 1 line, 2 point
     float clearColor_[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // TileRasterWorker.cpp:113-118
     float clearDepth_ = 3.3961514e38f;                  // 0x7f7f7f7f, TileRasterWorker.cpp:40

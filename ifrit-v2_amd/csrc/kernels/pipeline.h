@@ -109,6 +109,7 @@ struct SetupParams {
     unsigned clipCap;   // capacity for clip fan triangles and for triangles to clip
     float invW, invH;   // 1 / width, 1 / height (TileRasterRenderer.cpp:59-60)
     float invTileWidth; // ApproxReciprocal(16.0f) (TileRasterWorker.cpp:499)
+    int rcpModel;       // ifx::RcpModel: whose RCPPS the reciprocals of w and of the area reproduce
     // per view, per group of kGroupTris triangles: 1 when the group's bounds lie
     // outside a frustum plane with a margin (null: no group culling)
     const unsigned char* groupOut;

@@ -138,7 +138,7 @@ EXPORTED_SYMBOLS = [
     "iftrBindMeshShader", "iftrBindTaskShader", "iftrDrawMeshTasks", "iftrCreateGpuGeometryProgram",
     "iftrBindGeometryShader", "iftrSetClearValues", "iftrSetScissors",
     "iftrSetScissorTestEnable", "iftrCreateTextureLayers", "iftrCopyHostBufferToImageLayer", "iftrCreateSampler",
-    "iftrBlitImage",
+    "iftrBlitImage", "iftrSetSetupReciprocal",
 ]
 
 
@@ -209,6 +209,7 @@ def lib():
         "iftrSetColorFloatOutput": ([vp, C.c_int], None), "iftrGetColorRGBA32F": ([vp, vp], None),
         "iftrDrawNoAlloc": ([vp, C.POINTER(NoAllocDraw)], None),
         "iftrSetCullMode": ([vp, C.c_int], None), "iftrSetRasterizerPolygonMode": ([vp, C.c_int], None),
+        "iftrSetSetupReciprocal": ([vp, C.c_int], None),
         "iftrCreateBuffer": ([vp, C.c_int, C.c_int], C.c_int),
         "iftrCopyHostBufferToBuffer": ([vp, vp, C.c_int, C.c_int], None),
         "iftrCreateGpuMeshProgram": ([vp], vp), "iftrCreateGpuTaskProgram": ([vp], vp),
@@ -330,6 +331,15 @@ class Renderer:
     def set_cull_mode(self, mode: int):
         """IF_CULL_MODE_*: 0 none, 1 front, 2 back (the default), 3 front and back."""
         self.L.iftrSetCullMode(self.h, mode)
+        self._check()
+
+    RCP_MODELS = {"intel": 0, "amd": 1, "host": 2}
+
+    def set_setup_reciprocal(self, model="intel"):
+        """Whose RCPPS the triangle setup reproduces: "intel" (the default), "amd", or "host" (the
+        CPU this process runs on).  The instruction is implementation-defined, and the reference's
+        setup values follow the CPU it runs on."""
+        self.L.iftrSetSetupReciprocal(self.h, self.RCP_MODELS[model] if isinstance(model, str) else int(model))
         self._check()
 
     def set_clear_values(self, color=(0.0, 0.0, 0.0, 0.0), depth=None):

@@ -217,6 +217,16 @@ enum {
    draws nothing.  Exchanged triangles shade with their own vertex attributes. */
 void iftrSetCullMode(IfritRendererHandle* h, int mode);
 
+/* Whose RCPPS the triangle setup reproduces.  The CPU path divides by w and by the signed area with
+   _mm256_rcp_ps / _mm_rcp_ss and one Newton-Raphson step (TileRasterWorker.cpp:508-533,
+   FastUtilSimd.h:27-33, 87-97); the approximation is implementation-defined, so the setup values,
+   and with them edge-boundary pixels, depend on the CPU the reference runs on.  INTEL (the
+   default) is the table of an Intel Xeon (top 11 mantissa bits), AMD that of an AMD EPYC (top 12),
+   HOST picks the one of the CPU this process runs on (an error on other vendors).  No counterpart
+   in the reference's interface, whose CPU answers the question by itself. */
+enum { IFHIP_RCP_INTEL = 0, IFHIP_RCP_AMD = 1, IFHIP_RCP_HOST = 2 };
+void iftrSetSetupReciprocal(IfritRendererHandle* h, int model);
+
 /* Device buffers of the CUDA renderer (CreateBuffer / copyHostBufferToBuffer,
    TileRasterRendererCuda.h:98-99): numbered slots that mesh programs read.  CreateBuffer
    (re)allocates a slot of `size` bytes (contents undefined); copyHostBufferToBuffer writes

@@ -25,6 +25,13 @@ REFERENCE_ROOT = "/root/reference"
 CLEAR_DEPTH = float(np.frombuffer(np.uint32(0x7F7F7F7F).tobytes(), np.float32)[0])  # TileRasterWorker.cpp:40
 
 
+def _rcp_mode(live_rcp, rcp_table):
+    """RCPSS of the setup: this CPU's own instruction (live_rcp), else the captured Intel or AMD table."""
+    if live_rcp:
+        return 1
+    return {"intel": 0, "amd": 2}[rcp_table]
+
+
 class RasterConfig(C.Structure):
     _fields_ = [("width", C.c_int), ("height", C.c_int), ("samples", C.c_int), ("ccw", C.c_int),
                 ("depth_func", C.c_int), ("clear", C.c_int), ("num_threads", C.c_int), ("live_rcp", C.c_int),
@@ -184,7 +191,8 @@ class Frame:
             self._h = None
 
     def render(self, width, height, samples, clip, indices, depth_func=1, ccw=0, threads=None, live_rcp=False,
-               clear=True, depth_in=None, cull_mode=2, clear_depth=CLEAR_DEPTH, tagging=False, depth_write=None):
+               clear=True, depth_in=None, cull_mode=2, clear_depth=CLEAR_DEPTH, tagging=False, depth_write=None,
+               rcp_table="intel"):
         clip = np.ascontiguousarray(clip, np.float32)
         indices = np.ascontiguousarray(indices, np.int32)
         n = width * height
@@ -194,7 +202,7 @@ class Frame:
         cls = np.empty((n,), np.uint8)
         st = Stats()
         cfg = RasterConfig(width, height, samples, ccw, depth_func, 1 if clear else 0,
-                           threads or os.cpu_count() or 1, 1 if live_rcp else 0, cull_mode, clear_depth,
+                           threads or os.cpu_count() or 1, _rcp_mode(live_rcp, rcp_table), cull_mode, clear_depth,
                            1 if tagging else 0, *_depth_write(depth_write))
         din = None if depth_in is None else np.ascontiguousarray(depth_in, np.float32)
         rc = lib().oracle_render(self._h, C.byref(cfg), _p(clip), clip.shape[0], _p(indices), indices.size,
@@ -206,7 +214,7 @@ class Frame:
         return self
 
     def render_blend(self, width, height, clip, indices, factors, depth_func=1, threads=None, clear=True,
-                     depth_in=None, rgba_in=None, live_rcp=False, depth_write=None):
+                     depth_in=None, rgba_in=None, live_rcp=False, depth_write=None, rcp_table="intel"):
         """Blending of the translucent flat colour in primitive order (1 sample per pixel);
         factors = (src colour, dst colour, src alpha, dst alpha) as IfritBlendFactor values."""
         clip = np.ascontiguousarray(clip, np.float32)
@@ -216,7 +224,7 @@ class Frame:
         depth = np.empty((n,), np.float32)
         rgba = np.empty((n, 4), np.float32)
         cfg = RasterConfig(width, height, 1, 0, depth_func, 1 if clear else 0, threads or os.cpu_count() or 1,
-                           1 if live_rcp else 0, 2, CLEAR_DEPTH, 0, *_depth_write(depth_write))
+                           _rcp_mode(live_rcp, rcp_table), 2, CLEAR_DEPTH, 0, *_depth_write(depth_write))
         bl = (C.c_int * 4)(*factors)
         din = None if depth_in is None else np.ascontiguousarray(depth_in, np.float32)
         cin = None if rgba_in is None else np.ascontiguousarray(rgba_in, np.float32)
@@ -304,10 +312,12 @@ def _depth_write(dw):
     return (0, 1.0, 0.0) if dw is None else (1, float(dw[0]), float(dw[1]))
 
 
-def render_scene(scene, threads=None, live_rcp=False, cull_mode=2, clear_depth=CLEAR_DEPTH, tagging=False):
+def render_scene(scene, threads=None, live_rcp=False, cull_mode=2, clear_depth=CLEAR_DEPTH, tagging=False,
+                 rcp_table="intel"):
     clip = vertex_transform(scene.vertices, scene.mvp)
     f = Frame().render(scene.width, scene.height, scene.samples, clip, scene.indices, threads=threads,
                        live_rcp=live_rcp, cull_mode=cull_mode, clear_depth=clear_depth, tagging=tagging,
+                       rcp_table=rcp_table,
                        depth_write=getattr(scene, "depth_write", None))
     return f, clip
 

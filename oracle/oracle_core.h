@@ -38,7 +38,7 @@ typedef struct {
     int depth_func;     // IfritCompareOp (Constants.h); LESS = 1
     int clear;          // 1: depth starts at 0x7f7f7f7f; 0: depth starts from *_in buffers
     int num_threads;    // tile-parallel worker threads (>= 1)
-    int live_rcp;       // 1: use this CPU's RCPSS/RCPPS instead of the captured table
+    int live_rcp;       // RCPSS/RCPPS: 0 the captured Intel table, 1 this CPU's own instruction, 2 the AMD table
     int cull_mode;      // IfritCullMode: 2 back (the CPU path); 1 front and 0 none set triangles up with
                         // vertices 0 and 2 exchanged (front: all; none: those back would cull); 3 all
     float clear_depth;  // depth a clear starts from: 0x7f7f7f7f in the CPU path

@@ -1,5 +1,6 @@
 // Compares the RCPSS emulation used by the kernels (x86_fp.h) with this
-// CPU's own RCPSS instruction.  Exit status 0 when every tested input
+// CPU's own RCPSS instruction, using the table of the model named by the
+// first argument ("intel" or "amd").  Exit status 0 when every tested input
 // agrees bit for bit; prints the first few disagreements otherwise.
 //
 // Inputs: every mantissa for a set of exponents near the ends and the middle
@@ -21,10 +22,12 @@ static uint32_t live(uint32_t b) {
     return o;
 }
 
+static int model = ifx::kRcpIntel;
+
 static uint32_t emul(uint32_t b) {
     float x;
     memcpy(&x, &b, 4);
-    return ifx::f2u(ifx::rcp_ps(x));
+    return ifx::f2u(ifx::rcp_ps(x, model));
 }
 
 static long bad = 0, checked = 0;
@@ -38,7 +41,9 @@ static void check(uint32_t b) {
     }
 }
 
-int main() {
+int main(int argc, char** argv) {
+    if (argc > 1 && strcmp(argv[1], "amd") == 0) model = ifx::kRcpAmd;
+    printf("model %s\n", model == ifx::kRcpAmd ? "amd" : "intel");
     const uint32_t exps[] = {1, 2, 3, 64, 100, 125, 126, 127, 128, 129, 200, 250, 251, 252, 253, 254};
     for (uint32_t s = 0; s < 2; s++)
         for (uint32_t e : exps)

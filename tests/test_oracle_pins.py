@@ -37,14 +37,20 @@ def _load_golden():
 CLEAR_DEPTH_BITS = 0x7F7F7F7F  # TileRasterWorker.cpp:40
 
 
-def _is_intel():
+def _host_vendor():
+    """"intel" or "amd" for an x86 host of either vendor, else None."""
     if platform.machine() not in ("x86_64", "AMD64"):
-        return False
+        return None
     try:
         with open("/proc/cpuinfo") as fh:
-            return "GenuineIntel" in fh.read()
+            text = fh.read()
     except OSError:
-        return False
+        return None
+    return "intel" if "GenuineIntel" in text else "amd" if "AuthenticAMD" in text else None
+
+
+def _is_intel():
+    return _host_vendor() == "intel"
 
 
 # Comparisons with the reference build running on this host use the host's own RCPPS in the
@@ -117,25 +123,52 @@ def test_oracle_matches_reference_build_config1_full_size():
     assert (ids != 0xFFFFFFFF).sum() > 0.2 * ids.size
 
 
-@pytest.mark.skipif(not _is_intel(), reason="the captured RCPSS table is Intel's")
+@pytest.mark.skipif(_host_vendor() is None, reason="RCPSS tables exist for Intel and AMD x86 CPUs")
 def test_rcp_table_matches_cpu_instruction(tmp_path):
+    """The emulation with this host vendor's table equals the host's own instruction bit for bit."""
     exe = tmp_path / "rcp_check"
     subprocess.run(["g++", "-O2", "-std=c++17", "-msse4.2", "-I", os.path.join(ROOT, "ifrit-v2_amd", "csrc", "common"),
                     os.path.join(HERE, "native", "rcp_check.cpp"), "-o", str(exe)], check=True)
-    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([str(exe), _host_vendor()], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout
     assert "0 disagreements" in r.stdout
 
 
+def _rcp_table(name):
+    path = os.path.join(ROOT, "ifrit-v2_amd", "csrc", "common", name)
+    with open(path) as fh:
+        body = fh.read().split("*/", 1)[1]
+    return np.array([int(t.strip().rstrip("u"), 16) for t in body.replace("\n", " ").split(",") if t.strip()],
+                    np.uint32)
+
+
+@pytest.mark.parametrize("name,width", [("x86_rcp_table.inc", 11), ("x86_rcp_table_amd.inc", 12)])
+def test_rcp_tables_meet_the_instruction_error_bound(name, width):
+    """Both captured tables meet RCPSS's documented bound, |r x - 1| <= 1.5 * 2^-12, at both ends of
+    every input interval that shares a table entry; they are non-increasing; their outputs lie in
+    [0.5, 1); and the two vendors' tables differ (so a mix-up of them shows)."""
+    t = _rcp_table(name)
+    assert t.size == 1 << width
+    r = t.view(np.float32).astype(np.float64)
+    i = np.arange(t.size, dtype=np.float64)
+    lo = 1.0 + i / t.size
+    hi = 1.0 + (i + 1) / t.size - 2.0 ** -23  # the largest float of the interval
+    bound = 1.5 * 2.0 ** -12
+    assert np.abs(r * lo - 1).max() <= bound and np.abs(r * hi - 1).max() <= bound
+    assert (np.diff(r) <= 0).all() and (r >= 0.5).all() and (r < 1.0).all()
+    if width == 12:
+        intel = _rcp_table("x86_rcp_table.inc").view(np.float32)
+        assert (np.repeat(intel, 2) != t.view(np.float32)).sum() > 100
+
+
 def test_live_and_table_reciprocal_render_identically():
-    """On an Intel host the oracle's live-RCPSS mode and its captured table
-    produce the same frames (on other CPUs the live mode differs and is only
-    informative)."""
-    if not _is_intel():
-        pytest.skip("captured table is Intel's")
+    """The oracle's live-RCPSS mode and the captured table of the host's
+    vendor produce the same frames."""
+    if _host_vendor() is None:
+        pytest.skip("tables exist for Intel and AMD x86 CPUs")
     W, H, clip, idx = _random_clip_scene(77, 1)
-    a = O.Frame().render(W, H, 1, clip, idx, threads=2)
+    a = O.Frame().render(W, H, 1, clip, idx, threads=2, rcp_table=_host_vendor())
     b = O.Frame().render(W, H, 1, clip, idx, threads=2, live_rcp=True)
     _assert_same("live rcp", a.ids, a.depth, b.ids, b.depth)
 
@@ -146,8 +179,8 @@ def test_reference_build_equals_oracle_with_this_hosts_reciprocal(seed):
     """On any x86 host the reference build draws what the oracle draws with
     the host's own RCPPS (live mode): the only host dependence of the
     reference's visibility is that instruction.  On a non-Intel host the
-    captured (Intel) table then gives a different frame, whose differing
-    samples are counted and printed."""
+    Intel table then gives a different frame, whose differing samples are
+    counted and printed; the host vendor's table gives the same frame."""
     if platform.machine() not in ("x86_64", "AMD64"):
         pytest.skip("x86 only")
     if seed == "config1":
@@ -168,6 +201,9 @@ def test_reference_build_equals_oracle_with_this_hosts_reciprocal(seed):
     print(f"[{O.host_cpu()}] captured-table frame vs reference build: {diff} of {ids.size} samples differ")
     if _is_intel():
         assert diff == 0
+    if _host_vendor() is not None:
+        own = O.Frame().render(W, H, 1, clip, idx, threads=2, rcp_table=_host_vendor())
+        _assert_same("host vendor's table vs reference build", own.ids, own.depth, ids, depth)
 
 
 def test_msaa_sample_tables_known_answer():
