@@ -47,6 +47,15 @@ using namespace rc;
 
 constexpr int kPass = kRasterPass;  // the most list entries per pass through LDS
 
+// inside() with the three edge results combined by bitwise and: the same
+// arithmetic, and the compiler keeps the sample loop free of branches.
+__device__ __forceinline__ bool inside_and(const Edges& E, float x, float y) {
+    const bool a = fmaf(E.y[0], y, E.x[0] * x) < ifx::neg(E.z[0]);
+    const bool b = fmaf(E.y[1], y, E.x[1] * x) < ifx::neg(E.z[1]);
+    const bool c = fmaf(E.y[2], y, E.x[2] * x) < ifx::neg(E.z[2]);
+    return a & b & c;
+}
+
 __device__ __forceinline__ void merge(unsigned long long* cell, float d, unsigned tie) {
     if (d == d) atomicMin(cell, frag_key(d, tie));
 }
@@ -597,16 +606,20 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
                     const int bi = (ly >> 2) * 4 + (lx >> 2);
                     const bool accept = (sTa[2 * e + (bi >> 3)] >> (bi & 7)) & 1u;
                     const float fx = (float)(tx0 + lx), fy = (float)(ty0 + ly);
-                    bool any = false;
+                    // every sample's coverage and depth are computed (the lanes of a
+                    // wave cover different pixels, so branches would run both ways);
+                    // only the merge is conditional
+                    unsigned ncov = 0;
 #pragma unroll
                     for (int s = 0; s < S; s++) {
                         const float sx = fx + P.sx[s], sy = fy + P.sy[s];
-                        if (!accept && !inside(E, sx, sy)) continue;
-                        any = true;
-                        covs++;
-                        merge(&sVis[vis_at(s, c)], depth256(R.f1, R.f2, R.f3, sx, sy), tie);
+                        const bool cov = accept | inside_and(E, sx, sy);
+                        const float d = depth256(R.f1, R.f2, R.f3, sx, sy);
+                        ncov += cov;
+                        if (cov & (d == d)) atomicMin(&sVis[vis_at(s, c)], frag_key(d, tie));
                     }
-                    frags += any;
+                    covs += ncov;
+                    frags += ncov != 0u;
                 }
             }
         };
