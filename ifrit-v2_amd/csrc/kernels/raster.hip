@@ -589,18 +589,22 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
                             acc |= (unsigned)inside(E, (float)(px0 + ox), (float)(gy + oy)) << l;
                         }
                     }
+                    // as with MSAA: every pixel's class depth is computed, only the merge is
+                    // conditional
+                    unsigned non = 0;
 #pragma unroll
                     for (int l = 0; l < 8; l++) {
-                        if (!((acc >> l) & 1u)) continue;
                         const int px = px0 + (l & 1) + ((l >> 2) << 1), py = gy + ((l >> 1) & 1);
-                        if (px >= G.width || py >= G.height) continue;
+                        const bool on = ((acc >> l) & 1u) & (px < G.width) & (py < G.height);
                         const unsigned quad = (acc >> (l & 4)) & 0xFu;
                         const unsigned cls = expr_code(accepted, acc == 0xFFu, quad == 0xFu, P.tagging);
-                        frags++;
-                        covs++;
-                        merge(&sVis[vis_at(0, (py - ty0) * 16 + (px - tx0))],
-                              class_depth(cls, R.f1, R.f2, R.f3, (float)px, (float)py, P.tagging), (tie << 2) | cls);
+                        const float d = class_depth_all(cls, R.f1, R.f2, R.f3, (float)px, (float)py, P.tagging);
+                        non += on;
+                        if (on & (d == d))
+                            atomicMin(&sVis[vis_at(0, (py - ty0) * 16 + (px - tx0))], frag_key(d, (tie << 2) | cls));
                     }
+                    frags += non;
+                    covs += non;
                 } else {
                     const int lx = c & 15, ly = c >> 4;
                     const int bi = (ly >> 2) * 4 + (lx >> 2);

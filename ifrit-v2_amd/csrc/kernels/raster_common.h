@@ -24,6 +24,19 @@ __device__ __forceinline__ float class_depth(unsigned c, const float4& F1, const
     return tagging ? depth_grouped(F1, F2, F3, x, y) : depth_scalar(F1, F2, F3, x, y);
 }
 
+// class_depth with every expression evaluated and the class's selected:
+// SIMD256 and SIMD128 share their three plane values.
+__device__ __forceinline__ float class_depth_all(unsigned c, const float4& F1, const float4& F2, const float4& F3,
+                                                 float x, float y, bool tagging) {
+    const float b0 = fmaf(F1.x, x, fmaf(F1.y, y, F1.z));
+    const float b1 = fmaf(F2.x, x, fmaf(F2.y, y, F2.z));
+    const float b2 = fmaf(F3.x, x, fmaf(F3.y, y, F3.z));
+    const float d256 = fmaf(b0, F1.w, fmaf(b1, F2.w, b2 * F3.w));
+    const float d128 = fmaf(b2, F3.w, fmaf(b0, F1.w, b1 * F2.w));
+    const float d3 = tagging ? depth_grouped(F1, F2, F3, x, y) : depth_scalar(F1, F2, F3, x, y);
+    return c == 1 ? d256 : (c == 2 ? d128 : d3);
+}
+
 // Expression code of a fragment from how it was covered: accepted (tile or
 // 4x4 block), its 4x2 group all inside, its 2x2 quad all inside, or alone.
 __device__ __forceinline__ unsigned expr_code(bool accepted, bool group, bool quad, bool tagging) {
