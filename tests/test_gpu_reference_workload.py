@@ -55,3 +55,15 @@ def test_reference_workload_matches_oracle_and_reference(renderer_factory, size,
               f"{d2.max()}, >1 unflagged {int(((d2 > 1) & (flags == 0)).sum())}", flush=True)
         if table == vendor:
             assert ((d2 > 1) & (flags == 0)).sum() == 0
+
+
+def test_setup_reciprocal_model_errors(renderer_factory):
+    """An unknown reciprocal model is an error; the host model resolves on Intel and AMD hosts."""
+    from ifrit_hip import IfritError
+
+    r = renderer_factory()
+    with pytest.raises(IfritError, match="unknown reciprocal model"):
+        r.set_setup_reciprocal(7)
+    if _host_vendor() is not None:
+        r.set_setup_reciprocal("host")
+    r.set_setup_reciprocal("intel")
