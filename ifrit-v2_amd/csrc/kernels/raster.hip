@@ -650,11 +650,21 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
                     int pos = (int)off - (int)w0;
                     if (n && pos < UNITS && pos + (int)n > 0) {
                         unsigned bits = m;
-                        while (bits) {
-                            const int b = __ffs(bits) - 1;
-                            bits &= bits - 1u;
-                            if (pos >= 0 && pos < UNITS) sUnit[pos] = (unsigned short)((e0 << 8) | (16 * h + b));
-                            pos++;
+                        const int cb = (e0 << 8) | (16 * h);
+                        if (pos >= 0 && pos + (int)n <= UNITS) {
+                            // the lane's units all fall in this window
+                            while (bits) {
+                                const int b = __ffs(bits) - 1;
+                                bits &= bits - 1u;
+                                sUnit[pos++] = (unsigned short)(cb + b);
+                            }
+                        } else {
+                            while (bits) {
+                                const int b = __ffs(bits) - 1;
+                                bits &= bits - 1u;
+                                if (pos >= 0 && pos < UNITS) sUnit[pos] = (unsigned short)(cb + b);
+                                pos++;
+                            }
                         }
                     }
                     __syncthreads();
@@ -704,12 +714,22 @@ k_raster(RasterParams P, Views V, TriRecs recs, const unsigned* __restrict__ slo
                         int pos = (int)off - (int)w0;
                         if (n && pos < UNITS && pos + n > 0) {
                             unsigned bits = pm;
-                            while (bits) {
-                                const int b = __ffs(bits) - 1;
-                                bits &= bits - 1u;
-                                const int c = ((bi >> 2) * 4 + (b >> 2)) * 16 + (bi & 3) * 4 + (b & 3);
-                                if (pos >= 0 && pos < UNITS) sUnit[pos] = (unsigned short)((pe << 8) | c);
-                                pos++;
+                            const int cb = (pe << 8) | ((bi >> 2) * 64 + (bi & 3) * 4);
+                            if (pos >= 0 && pos + n <= UNITS) {
+                                // the lane's units all fall in this window
+                                while (bits) {
+                                    const int b = __ffs(bits) - 1;
+                                    bits &= bits - 1u;
+                                    sUnit[pos++] = (unsigned short)(cb + ((b >> 2) << 4) + (b & 3));
+                                }
+                            } else {
+                                while (bits) {
+                                    const int b = __ffs(bits) - 1;
+                                    bits &= bits - 1u;
+                                    if (pos >= 0 && pos < UNITS)
+                                        sUnit[pos] = (unsigned short)(cb + ((b >> 2) << 4) + (b & 3));
+                                    pos++;
+                                }
                             }
                         }
                         __syncthreads();
