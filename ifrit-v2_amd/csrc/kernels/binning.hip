@@ -121,9 +121,11 @@ __global__ void __launch_bounds__(256) k_bin_small(BinParams P, Views V, TriRecs
                 if (!SCATTER) {
                     (void)dev::wave_inc(binCount, t, hit);
                 } else {
+                    // the list start is loaded before the atomic, so that the two round trips overlap
+                    const unsigned st0 = hit ? binStart[t] : 0u;
                     const unsigned pos = dev::wave_inc(binCursor, t, hit);
                     if (hit) {
-                        const unsigned at = binStart[t] + pos;
+                        const unsigned at = st0 + pos;
                         if (at < P.binCap) binList[at] = s | (cls == 2 ? 0x80000000u : 0u);
                     }
                 }
